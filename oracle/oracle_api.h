@@ -1,0 +1,88 @@
+/* Test infrastructure only: the CPU oracle of the volume-rendering hot path.
+ * Nothing in the shipped library includes, links or calls this.
+ *
+ * C entry points of liboracle.so, called from tests/ through ctypes. Volumes are
+ * in the reference's layout: voxel (i, j, k) at i + dims[0] * (j + dims[1] * k). */
+#ifndef VVR_ORACLE_API_H
+#define VVR_ORACLE_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../include/vvr.h"
+#include "../include/vvr_uniforms.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The textures the shaders sample (binding numbers of shaders/common.glsl). */
+typedef struct oracle_scene {
+    uint32_t dims[3];
+    const uint16_t* density;  /* binding 1, R16_UNORM                              */
+    const uint16_t* lut;      /* binding 2, 4096 x RGBA16F bits                    */
+    const float* preint;      /* binding 12, 4096 x RGBA32F                        */
+    const uint8_t* labels;    /* binding 4, R8_UINT (NULL: 1-voxel placeholder 0)  */
+    const int8_t* normals;    /* binding 5, RGBA8_SNORM                            */
+    const uint16_t* mask;     /* binding 13, RG16_UNORM (NULL outside label modes) */
+    const float* majorant;    /* binding 22, R32F, ceil(dims / 8)                  */
+} oracle_scene;
+
+uint16_t oracle_hu_to_unorm16(int16_t hu);
+void oracle_tf_evaluate(const vvr_control_point* pts, uint32_t n, float hu, float out[4]);
+void oracle_tf_lut(const vvr_control_point* pts, uint32_t n, float* lut4096x4);
+void oracle_tf_lut_f16(const vvr_control_point* pts, uint32_t n, uint16_t* out);
+void oracle_tf_cumulative(const vvr_control_point* pts, uint32_t n, float* out);
+void oracle_bucket_max(const vvr_control_point* pts, uint32_t n, float* out64);
+void oracle_camera_framing(const float center[3], float diagonal, vvr_camera* out);
+void oracle_geometry(const vvr_volume* vol, float world_to_index[16], float index_to_world[16], float center[3],
+                     float* diagonal, float* min_spacing, float* max_spacing);
+void oracle_pack_uniforms(const vvr_volume* vol, const vvr_frame_params* p, const float* bucket_max64, int has_labels,
+                          uint32_t accumulated, uint32_t width, uint32_t height, uint64_t frame, vvr_uniforms* u);
+
+uint16_t oracle_f32_to_f16(float f);
+float oracle_f16_to_f32(uint16_t h);
+
+/* majorant.comp over all bricks. */
+void oracle_majorant(const oracle_scene* s, const vvr_uniforms* u, float* out);
+/* normals.comp: 4 snorm8 per voxel. */
+void oracle_normals(const oracle_scene* s, const vvr_uniforms* u, int8_t* out);
+/* mask.comp, all four passes: 2 unorm16 per voxel (the final image A).
+ * soft: NULL or grid of soft_dims nodes x 4 bytes; soft_push = pc.soft
+ * (xyz source axes, w flip bits | 8 present); margin = pc.params.x, radius = pc.params.z. */
+void oracle_mask(const oracle_scene* s, const vvr_uniforms* u, const uint8_t* soft, const uint32_t soft_dims[3],
+                 const int32_t soft_push[4], int32_t margin, int32_t radius, uint16_t* out);
+/* The binary pass (pass 0) alone, float before quantisation: 2 per voxel. */
+void oracle_mask_binary(const oracle_scene* s, const vvr_uniforms* u, const uint8_t* soft, const uint32_t soft_dims[3],
+                        const int32_t soft_push[4], int32_t margin, float* out);
+
+/* dvr.comp for `n` pixels (x, y pairs): rgba (radiance, transmittance) before the
+ * RGBA16F store. precision 32 or 64. */
+void oracle_dvr(const oracle_scene* s, const vvr_uniforms* u, const uint32_t* pixels, size_t n, int precision,
+                float* rgba);
+/* pt.comp for `n` pixels: the values the dispatch adds to the accumulation,
+ * albedo-guide and normal-guide images (4 floats each per pixel). */
+void oracle_pt(const oracle_scene* s, const vvr_uniforms* u, const uint32_t* pixels, size_t n, int precision,
+               float* accum, float* albedo, float* normal);
+
+/* Building blocks, exposed for closed-form checks. */
+float oracle_density_cubic(const oracle_scene* s, const vvr_uniforms* u, const float idx[3], int precision);
+void oracle_density_gradient_cubic(const oracle_scene* s, const vvr_uniforms* u, const float idx[3], int precision,
+                                   float out[3]);
+float oracle_density_linear(const oracle_scene* s, const vvr_uniforms* u, const float idx[3], int precision);
+uint32_t oracle_pcg_sequence(uint32_t seed, uint32_t n, uint32_t* out);
+void oracle_rand_sequence(uint32_t seed, uint32_t n, float* out);
+float oracle_phase_hg(float g, float cos_t);
+void oracle_sample_hg(float g, uint32_t seed, uint32_t n, float* cos_out);
+float oracle_ggx_distribution(float nh, float a2);
+void oracle_single_scattering_albedo(const float a[3], float out[3]);
+/* free_flight in a homogeneous scene: 1 if the ray collided, distance to the collision. */
+int oracle_free_flight(const oracle_scene* s, const vvr_uniforms* u, const float origin_idx[3], const float dir[3],
+                       uint32_t seed, float* t_out, float spectral[3]);
+void oracle_transmittance(const oracle_scene* s, const vvr_uniforms* u, const float origin_idx[3], const float dir[3],
+                          uint32_t seed, float out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,174 @@
+"""GPU parity of the ray marcher and the path tracer against the oracle.
+
+DVR: the RGBA16F frame within max(1e-3, half an fp16 ulp) + 1e-3 of the FP32
+oracle in linear RGB and transmittance, on all but at most 0.1 % of pixels
+(discrete decisions — brick skip, T > 0.01, the first-surface bisection — can
+flip a pixel on a one-ulp difference; flipped pixels are listed), and the same
+against the FP64 oracle.
+PT: each path draws the same PCG stream on the GPU and in the oracle (the
+reference's seed of a frame of one path), so per-pixel sums agree except where
+a float comparison flips a path; the share of such pixels is bounded and the
+image-level relative difference is < 1 %.
+"""
+import numpy as np
+import pytest
+
+from helpers import all_pixels, frame_params
+from volview_amd import _abi, phantom, workloads
+from volview_amd.renderer import Renderer, TransferFunction, make_volume
+
+pytestmark = pytest.mark.gpu
+
+
+def f16_tol(v):
+    return 1e-3 + np.abs(v) * 2.0 ** -11
+
+
+def oracle_for(oracle, r, hu, tf, params, w, h, has_labels=False, accumulated=0, labels=None):
+    u = oracle.pack_uniforms(make_volume(hu), params, oracle.bucket_max(tf), has_labels, accumulated, w, h, 1)
+    sc = oracle.Scene(oracle.hu_to_unorm16(hu), tf=tf, labels=labels)
+    sc.set("normals", r.read_volume(_abi.VOL_NORMALS))  # bit-exact (test_gpu_volumes)
+    if has_labels:
+        sc.set("mask", r.read_volume(_abi.VOL_MASK))
+    sc.set("majorant", r.read_majorant())
+    return u, sc
+
+
+def compare_dvr(got, want, max_flip=1e-3):
+    err = np.abs(got - want)
+    bad = np.any(err > f16_tol(want), axis=-1)
+    assert bad.mean() <= max_flip, f"{bad.sum()} of {bad.size} pixels off, worst {err.max()}"
+    return bad
+
+
+@pytest.mark.parametrize("camera", [0, 3])
+def test_dvr_small_full_image(ctx, oracle, camera):
+    hu = phantom.ct((64, 64, 64), 2)
+    tf = TransferFunction.preset(0)
+    r = Renderer(ctx, hu, tf)
+    p = workloads.dvr_params(r, camera)
+    w, h = 96, 80
+    r.resize(w, h)
+    r.record(1, p)
+    got = r.read_hdr().reshape(-1, 4)
+    u, sc = oracle_for(oracle, r, hu, tf, p, w, h)
+    px = all_pixels(w, h)
+    compare_dvr(got, sc.dvr(u, px, 32))
+    compare_dvr(got, sc.dvr(u, px, 64))
+    assert (got[:, 3] < 0.5).mean() > 0.01  # something opaque was drawn
+
+
+def test_dvr_config_full_size_sampled_pixels(ctx, oracle):
+    """DVR-1: 256^3, bone preset, 0.5-voxel step, skipping, cubic, 512^2."""
+    s = workloads.dvr_scene(ctx)
+    s.renderer.record(1, s.params)
+    got = s.renderer.read_hdr()
+    u, sc = oracle_for(oracle, s.renderer, s.hu, TransferFunction.preset(0), s.params, 512, 512)
+    rng = np.random.default_rng(11)
+    px = np.stack([rng.integers(0, 512, 3000), rng.integers(0, 512, 3000)], 1).astype(np.uint32)
+    g = got[px[:, 1], px[:, 0]]
+    compare_dvr(g, sc.dvr(u, px, 32))
+    compare_dvr(g, sc.dvr(u, px, 64))
+
+
+@pytest.mark.parametrize("mode", [_abi.LABELS_ISOLATE, _abi.LABELS_HYBRID, _abi.LABELS_TINT])
+def test_dvr_label_modes(ctx, oracle, mode):
+    dims = (72, 64, 60)
+    hu = phantom.ct(dims, 8)
+    labels, sg, _, n = phantom.organs(dims, 8, 40)
+    tf = TransferFunction.preset(2)
+    r = Renderer(ctx, hu, tf)
+    r.set_labels(labels, sg)
+    p = workloads.dvr_params(r, 1)
+    workloads.organ_label_params(p, n, 3, mode)
+    p.labels.ghost[2] = 1
+    if mode == _abi.LABELS_HYBRID:
+        for k in range(1, n + 1):
+            p.labels.density[k][:] = (0.2, 0.25, 0.26, 0.01)
+    w, h = 64, 64
+    r.resize(w, h)
+    r.record(1, p)
+    got = r.read_hdr().reshape(-1, 4)
+    u, sc = oracle_for(oracle, r, hu, tf, p, w, h, has_labels=True, labels=labels)
+    compare_dvr(got, sc.dvr(u, all_pixels(w, h), 32), max_flip=5e-3)
+
+
+def pt_compare(got, want, what):
+    """Per-pixel sums from identical random streams."""
+    err = np.abs(got - want)
+    off = np.any(err > 1e-3 * (1 + np.abs(want)), axis=-1)
+    assert off.mean() < 0.05, f"{what}: {off.sum()} of {off.size} pixels diverged"
+    rel = np.sqrt(np.mean((got - want) ** 2)) / max(np.sqrt(np.mean(want ** 2)), 1e-6)
+    assert rel < 0.01, f"{what}: relative RMSE {rel}"
+
+
+@pytest.mark.parametrize("labels_on", [True, False])
+def test_pt_matches_oracle_paths(ctx, oracle, labels_on):
+    dims = (64, 64, 48)
+    hu = phantom.ct(dims, 12)
+    tf = TransferFunction.preset(0 if not labels_on else 2)
+    r = Renderer(ctx, hu, tf)
+    labels = None
+    n = 0
+    if labels_on:
+        labels, sg, _, n = phantom.organs(dims, 12, 40)
+        r.set_labels(labels, sg)
+    w, h, spp = 32, 32, 3
+    p = workloads.pt_params(r, n, spp, bounces=4, g=0.8, camera_index=2)
+    r.resize(w, h)
+    r.record(1, p)
+    acc, alb, nrm = r.read_accum()
+    assert r.accumulated_samples() == spp
+    # The GPU traced sample indices 0..spp-1; the oracle traces each as a frame of one path.
+    want = [np.zeros((w * h, 4), np.float32) for _ in range(3)]
+    for k in range(spp):
+        q = workloads.pt_params(r, n, 1, bounces=4, g=0.8, camera_index=2)
+        u, sc = oracle_for(oracle, r, hu, tf, q, w, h, has_labels=labels_on, accumulated=k, labels=labels)
+        a, b, c = sc.pt(u, all_pixels(w, h), 32)
+        want[0] += a
+        want[1] += b
+        want[2] += c
+    pt_compare(acc.reshape(-1, 4), want[0], "radiance")
+    pt_compare(alb.reshape(-1, 4)[:, :3], want[1][:, :3], "albedo guide")
+    pt_compare(nrm.reshape(-1, 4), want[2], "normal guide")
+    hdr = r.read_hdr().reshape(-1, 4)
+    assert np.allclose(hdr, (acc.reshape(-1, 4) / spp).astype(np.float16).astype(np.float32), rtol=2e-3, atol=1e-4)
+
+
+def test_pt_accumulates_across_frames(ctx):
+    dims = (48, 48, 40)
+    hu = phantom.ct(dims, 13)
+    r = Renderer(ctx, hu, TransferFunction.preset(0))
+    w, h = 24, 24
+    p = workloads.pt_params(r, 0, 2, bounces=3)
+    r.resize(w, h)
+    r.record(1, p)
+    r.record(2, p)
+    a2, _, _ = r.read_accum()
+    assert r.accumulated_samples() == 4
+    # Same four sample indices in one frame.
+    r2 = Renderer(ctx, hu, TransferFunction.preset(0))
+    r2.resize(w, h)
+    r2.record(1, workloads.pt_params(r2, 0, 4, bounces=3))
+    a4, _, _ = r2.read_accum()
+    assert np.allclose(a2, a4, rtol=1e-5, atol=1e-6)
+    # Changing anything but the denoiser restarts accumulation.
+    p.lighting.intensity = 2.0
+    r.record(3, p)
+    assert r.accumulated_samples() == 2
+
+
+def test_record_before_resize_and_unsupported_lighting_fail(ctx):
+    hu = phantom.ct((32, 32, 32), 1)
+    r = Renderer(ctx, hu, TransferFunction.preset(0))
+    p = frame_params(hu)
+    with pytest.raises(_abi.VvrError) as e:
+        r.record(1, p)
+    assert e.value.code == _abi.VVR_ESTATE
+    r.resize(8, 8)
+    p.lighting.shadows = 1
+    with pytest.raises(_abi.VvrError) as e:
+        r.record(1, p)
+    assert e.value.code == _abi.VVR_EINVAL
+    with pytest.raises(_abi.VvrError):
+        r.set_labels(np.zeros(10, np.uint8))

@@ -1,0 +1,108 @@
+"""GPU parity of the volume passes against the oracle: density upload
+(transfer.rs:121-124, all 65,536 HU values), normals.comp, mask.comp and
+majorant.comp. Quantised outputs must match bit for bit; the soft-mask pass
+uses exp(), whose last-bit rounding may differ between the GPU and the host
+libm, so it may differ by one unorm16 step on a small share of voxels.
+"""
+import numpy as np
+import pytest
+
+from helpers import frame_params
+from volview_amd import _abi, phantom
+from volview_amd.renderer import Renderer, TransferFunction
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_uniforms(oracle, r, hu, params, tf, has_labels=False):
+    from volview_amd.renderer import make_volume
+    g = r.geometry()
+    return oracle.pack_uniforms(make_volume(hu), params, oracle.bucket_max(tf), has_labels, 0, 8, 8, 1)
+
+
+def test_density_upload_every_hu_value(ctx, oracle):
+    hu = np.arange(-32768, 32768, dtype=np.int32).astype(np.int16).reshape(32, 32, 64)
+    r = Renderer(ctx, hu, TransferFunction.preset(0))
+    got = r.read_volume(_abi.VOL_DENSITY)
+    assert np.array_equal(got, oracle.hu_to_unorm16(hu))
+
+
+@pytest.mark.parametrize("case", ["phantom", "anisotropic"])
+def test_normals_bit_exact(ctx, oracle, case):
+    if case == "phantom":
+        hu, spacing = phantom.ct((96, 80, 72), 5), (1.0, 1.0, 1.0)
+    else:
+        hu = np.random.default_rng(2).integers(-1000, 2000, (29, 41, 37)).astype(np.int16)
+        spacing = (0.7, 0.9, 1.3)
+    tf = TransferFunction.preset(0)
+    r = Renderer(ctx, hu, tf, spacing=spacing)
+    got = r.read_volume(_abi.VOL_NORMALS)
+    from volview_amd.renderer import make_volume
+    p = frame_params(hu, spacing=spacing)
+    u = oracle.pack_uniforms(make_volume(hu, spacing), p, oracle.bucket_max(tf), False, 0, 8, 8, 1)
+    want = oracle.Scene(oracle.hu_to_unorm16(hu)).normals(u)
+    mism = np.argwhere(np.any(got != want, axis=-1))
+    assert len(mism) == 0, f"{len(mism)} voxels differ, first {mism[:5].tolist()}"
+
+
+def render_once(r, params, w=8, h=8):
+    r.resize(w, h)
+    r.record(1, params)
+    r.synchronize()
+
+
+@pytest.mark.parametrize("dims", [(256, 256, 256), (77, 50, 43)])
+def test_majorant_mode_off_bit_exact(ctx, oracle, dims):
+    hu = phantom.ct(dims, 1)
+    tf = TransferFunction.preset(0)
+    r = Renderer(ctx, hu, tf)
+    p = frame_params(hu)
+    render_once(r, p)
+    got = r.read_majorant()
+    u = oracle_uniforms(oracle, r, hu, p, tf)
+    want = oracle.Scene(oracle.hu_to_unorm16(hu), tf=tf).majorant(u)
+    assert np.array_equal(got, want)
+    assert (got > 0).mean() < 0.9 and (got == 0).any()  # bricks are skipped
+
+
+def organ_scene(ctx, dims, seed, mode, soft=True, **label_kw):
+    hu = phantom.ct(dims, seed)
+    labels, sg, _, n = phantom.organs(dims, seed, 40)
+    tf = TransferFunction.preset(0)
+    r = Renderer(ctx, hu, tf)
+    r.set_labels(labels, sg if soft else None)
+    p = frame_params(hu)
+    p.labels.mode = mode
+    for k, v in label_kw.items():
+        setattr(p.labels, k, v)
+    return hu, labels, sg, tf, r, p
+
+
+@pytest.mark.parametrize("mode", [_abi.LABELS_TINT, _abi.LABELS_ISOLATE, _abi.LABELS_HYBRID])
+def test_mask_binary_path_bit_exact(ctx, oracle, mode):
+    hu, labels, sg, tf, r, p = organ_scene(ctx, (70, 64, 58), 4, mode, soft=False)
+    p.labels.colors[3][3] = 0.0  # a hidden label (cut in tint mode)
+    p.labels.ghost[5] = 1
+    render_once(r, p)
+    got = r.read_volume(_abi.VOL_MASK)
+    u = oracle_uniforms(oracle, r, hu, p, tf, has_labels=True)
+    sc = oracle.Scene(oracle.hu_to_unorm16(hu), tf=tf, labels=labels)
+    want = sc.mask(u, None, None, (0, 0, 0, 0), margin=2, radius=1)
+    assert np.array_equal(got, want)
+    sc.set("mask", got)
+    assert np.array_equal(r.read_majorant(), sc.majorant(u))
+
+
+def test_mask_soft_grid_and_majorant(ctx, oracle):
+    hu, labels, sg, tf, r, p = organ_scene(ctx, (80, 72, 64), 6, _abi.LABELS_ISOLATE)
+    render_once(r, p)
+    got = r.read_volume(_abi.VOL_MASK)
+    u = oracle_uniforms(oracle, r, hu, p, tf, has_labels=True)
+    sc = oracle.Scene(oracle.hu_to_unorm16(hu), tf=tf, labels=labels)
+    want = sc.mask(u, sg, hu.shape[::-1], (0, 1, 2, 8), margin=0, radius=1)
+    diff = np.abs(got.astype(np.int64) - want.astype(np.int64))
+    assert diff.max() <= 1, diff.max()
+    assert (diff > 0).mean() < 5e-3
+    # The majorant over the GPU's own mask is bit-exact.
+    sc.set("mask", got)
+    assert np.array_equal(r.read_majorant(), sc.majorant(u))

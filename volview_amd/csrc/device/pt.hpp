@@ -1,0 +1,367 @@
+// Transport of the path tracer: the HIP counterpart of shaders/pt.comp:41-371.
+// A path's random numbers come from one PCG stream (pt.comp:41-53) seeded per
+// (pixel, sample index) exactly as the reference seeds a frame of one path.
+#pragma once
+#include "shading.hpp"
+
+namespace vvr {
+
+constexpr float kMaxExtinction = 8.0f;     // pt.comp:34
+constexpr uint32_t kMaxVolumeEvents = 32;  // pt.comp:112
+
+struct Rng {
+    uint32_t state;
+    __device__ __forceinline__ uint32_t next() {  // pt.comp:44-49
+        const uint32_t s = state;
+        state = state * 747796405u + 2891336453u;
+        const uint32_t word = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+        return (word >> 22u) ^ word;
+    }
+    __device__ __forceinline__ float rand() { return float(next() >> 8) * (1.0f / 16777216.0f); }  // pt.comp:51-53
+};
+
+// pt.comp:378-379 for a frame whose first path index is `path_index`.
+__device__ __forceinline__ Rng seed_rng(uint32_t px, uint32_t py, uint32_t path_index) {
+    Rng r{(px * 1973u + py * 9277u + path_index * 26699u) | 1u};
+    r.next();
+    return r;
+}
+
+// pt.comp:56-82, for the dome and the studio (no light map).
+__device__ inline V3 environment(const vvr_uniforms* u, V3 d) {
+    const V3 amb(u->ambient[0], u->ambient[1], u->ambient[2]);
+    if (u->lighting[3] == 1u) {
+        const V3 f = normalize(V3(u->forward[0], u->forward[1], u->forward[2]));
+        const V3 r = normalize(V3(u->right[0], u->right[1], u->right[2]));
+        const V3 up = normalize(V3(u->up[0], u->up[1], u->up[2]));
+        const float key_side = dot(V3(u->light_dir[0], u->light_dir[1], u->light_dir[2]), r) >= 0.0f ? 1.0f : -1.0f;
+        const V3 fill = normalize(((-f) - (0.9f * key_side) * r) - 0.1f * up);
+        const V3 rim = normalize((f + 0.8f * up) - (0.4f * key_side) * r);
+        const float fill_lobe = 0.5f * powf(gmax(dot(d, fill), 0.0f), 6.0f);
+        const float rim_lobe = 2.5f * powf(gmax(dot(d, rim), 0.0f), 24.0f);
+        const V3 studio = V3(0.02f) + fill_lobe * V3(0.95f, 0.98f, 1.05f) + rim_lobe * V3(0.9f, 0.95f, 1.1f);
+        return amb * studio * (1.0f / 0.106f);
+    }
+    const float up = 0.5f + 0.5f * d.z;
+    return amb * vmix(V3(1.04f, 1.0f, 0.94f) * 0.35f, V3(0.92f, 0.96f, 1.04f), up) / 0.675f;
+}
+
+__device__ __forceinline__ M3 basis(V3 n) {  // pt.comp:85-92
+    const float s = n.z >= 0.0f ? 1.0f : -1.0f;
+    const float a = -1.0f / (s + n.z);
+    const float b = n.x * n.y * a;
+    M3 m;
+    m.c0 = V3(1.0f + s * n.x * n.x * a, s * b, -s * n.x);
+    m.c1 = V3(b, s + n.y * n.y * a, -n.y);
+    m.c2 = n;
+    return m;
+}
+
+__device__ __forceinline__ V3 sample_light(const vvr_uniforms* u, const M3& frame, Rng& rng) {  // pt.comp:96-102
+    const float cos_max = cosf(u->material[3]);
+    const float cos_t = mixf(cos_max, 1.0f, rng.rand());
+    const float sin_t = sqrtf(gmax(0.0f, 1.0f - cos_t * cos_t));
+    const float phi = 2.0f * kPi * rng.rand();
+    return frame.mul(V3(sin_t * cosf(phi), sin_t * sinf(phi), cos_t));
+}
+
+__device__ __forceinline__ V3 single_scattering_albedo(V3 a) {  // pt.comp:119-123
+    a = vclamp(a, 0.0f, 0.999f);
+    const V3 sq = 9.59217f + 41.6808f * a + 17.7126f * a * a;
+    const V3 s = 4.09712f + 4.20863f * a - V3(sqrtf(sq.x), sqrtf(sq.y), sqrtf(sq.z));
+    return 1.0f - s * s;
+}
+
+__device__ __forceinline__ V3 chroma(const vvr_uniforms* u, const Sample& s) {  // pt.comp:132-139
+    if (s.ghost) return V3(1.0f);
+    return vmix(V3(1.0f), V3(0.62f, 0.77f, 1.0f), clampf(2.0f * s.subsurface, 0.0f, 1.0f) * u->env[3]);
+}
+
+__device__ __forceinline__ V3 path_sigma(const vvr_uniforms* u, const Sample& s, float extinction) {  // :143-145
+    return measured(u, s) ? tissue_sigma(u, s) : extinction * chroma(u, s);
+}
+
+__device__ __forceinline__ V3 event_albedo(const vvr_uniforms* u, const Sample& s) {  // pt.comp:150-153
+    const V3 reduced = single_scattering_albedo(s.albedo);
+    return measured(u, s) ? 1.0f - (1.0f - reduced) * (1.0f - clampf(u->env[2], 0.0f, 0.95f)) : reduced;
+}
+
+__device__ __forceinline__ float flight_majorant(const DevScene& sc, V3 p) {  // pt.comp:157-163
+    const vvr_uniforms* u = sc.u;
+    float m = brick_majorant(sc, p);
+    if (m > 0.0f && u->labels[0] >= 2.0f) m = gmax(m, u->tissue[0]);
+    return gmin(m, gmax(kMaxExtinction, gmax(u->labels[2], u->tissue[0])));
+}
+
+__device__ __forceinline__ float phase_hg(const vvr_uniforms* u, float cos_t) {  // pt.comp:167-171
+    const float g = u->env[2];
+    const float d = 1.0f + g * g - 2.0f * g * cos_t;
+    return (1.0f - g * g) / (4.0f * kPi * d * sqrtf(gmax(d, 1e-8f)));
+}
+
+__device__ __forceinline__ V3 sample_hg(const vvr_uniforms* u, V3 dir, Rng& rng) {  // pt.comp:173-185
+    const float g = u->env[2];
+    float cos_t;
+    if (fabsf(g) < 1e-3f) {
+        cos_t = 1.0f - 2.0f * rng.rand();
+    } else {
+        const float q = (1.0f - g * g) / (1.0f - g + 2.0f * g * rng.rand());
+        cos_t = clampf((1.0f + g * g - q * q) / (2.0f * g), -1.0f, 1.0f);
+    }
+    const float sin_t = sqrtf(gmax(0.0f, 1.0f - cos_t * cos_t));
+    const float phi = 2.0f * kPi * rng.rand();
+    return basis(dir).mul(V3(sin_t * cosf(phi), sin_t * sinf(phi), cos_t));
+}
+
+__device__ __forceinline__ float mis(float a, float b) { return a * a / gmax(a * a + b * b, 1e-20f); }
+__device__ __forceinline__ float coat_share(const Sample& s) { return 0.5f * s.coat; }
+__device__ __forceinline__ float specular_share(const Sample& s) { return mixf(0.2f, 1.0f, s.metalness); }
+
+__device__ __forceinline__ float ggx_pdf(V3 n, V3 v, V3 l, float a) {  // pt.comp:246-250
+    const V3 h = normalize(v + l);
+    const float nh = gmax(dot(n, h), 0.0f);
+    return ggx_distribution(nh, a * a) * nh / (4.0f * gmax(dot(v, h), 1e-4f));
+}
+
+__device__ __forceinline__ V3 sample_ggx(const M3& frame, V3 v, float a, Rng& rng) {  // pt.comp:252-260
+    const float a2 = a * a;
+    const float phi = 2.0f * kPi * rng.rand();
+    const float xi = rng.rand();
+    const float cos_t = sqrtf((1.0f - xi) / (1.0f + (a2 - 1.0f) * xi));
+    const float sin_t = sqrtf(gmax(0.0f, 1.0f - cos_t * cos_t));
+    const V3 h = frame.mul(V3(sin_t * cosf(phi), sin_t * sinf(phi), cos_t));
+    return reflect(-v, h);
+}
+
+__device__ __forceinline__ float bsdf_pdf(V3 n, V3 v, V3 l, const Sample& s) {  // pt.comp:263-273
+    const float nl = dot(n, l);
+    if (nl <= 0.0f) return 0.0f;
+    const float a = gmax(s.roughness * s.roughness, 2e-3f);
+    const float share = specular_share(s);
+    const float base = share * ggx_pdf(n, v, l, a) + (1.0f - share) * nl / kPi;
+    const float coat = coat_share(s);
+    return coat * ggx_pdf(n, v, l, coat_roughness(s) * coat_roughness(s)) + (1.0f - coat) * base;
+}
+
+// pt.comp:277-292
+__device__ inline float extinction_at(const DevScene& sc, V3 idx, V3 dir, bool smooth_density, Sample& s, int& cap_plane,
+                                      float& cut) {
+    cut = clip_depth(sc.u, idx, cap_plane);
+    if (cut > 0.0f) return 0.0f;
+    s = apply_labels(sc, tf_point(sc, smooth_density ? density_view(sc, idx) : density_at_index(sc, idx)), idx);
+    if (s.extinction <= 0.0f) return 0.0f;
+    if (s.ghost) {
+        V3 n;
+        float surfaceness;
+        return s.extinction * surface_frame(sc, idx, s, -dir, cap_plane, cut, 0.0f, n, surfaceness);
+    }
+    return s.extinction;
+}
+
+// Delta tracking against the brick bounds (pt.comp:300-334).
+__device__ inline bool free_flight(const DevScene& sc, V3 origin, V3 dir, bool smooth_density, int hero, V3& spectral,
+                                   V3& idx, Sample& s, int& cap_plane, float& cut, Rng& rng) {
+    const vvr_uniforms* u = sc.u;
+    const V3 d_idx = to_index(u).mul(dir);
+    float t0, t1;
+    intersect_volume(u, origin, d_idx, t0, t1);
+    float tt = t0;
+    for (int i = 0; i < 4096 && tt < t1; i++) {
+        const V3 p = origin + d_idx * tt;
+        const float majorant = flight_majorant(sc, p);
+        const float exit = tt + brick_exit(p, d_idx) + 1e-3f;
+        if (majorant <= 0.0f) {
+            tt = exit;
+            continue;
+        }
+        const float next = tt - logf(1.0f - rng.rand()) / majorant;
+        if (next >= exit) {
+            tt = exit;
+            continue;
+        }
+        tt = next;
+        idx = origin + d_idx * tt;
+        const float extinction = extinction_at(sc, idx, dir, smooth_density, s, cap_plane, cut);
+        if (extinction <= 0.0f) continue;
+        const V3 sigma = path_sigma(u, s, extinction);
+        const float sh = component(sigma, hero);
+        if (rng.rand() * majorant < sh) {
+            spectral = spectral * (sigma / sh);
+            return true;
+        }
+        spectral = spectral * (vmax(V3(majorant) - sigma, V3(0.0f)) / (majorant - sh));
+    }
+    return false;
+}
+
+// Ratio tracking with Russian roulette (pt.comp:339-371).
+__device__ inline V3 transmittance(const DevScene& sc, V3 origin, V3 dir, Rng& rng) {
+    const vvr_uniforms* u = sc.u;
+    const V3 d_idx = to_index(u).mul(dir);
+    float t0, t1;
+    intersect_volume(u, origin, d_idx, t0, t1);
+    float tt = t0;
+    V3 result(1.0f);
+    for (int i = 0; i < 4096 && tt < t1; i++) {
+        const float peak = gmax(result.x, gmax(result.y, result.z));
+        if (peak < 0.1f) {
+            if (rng.rand() * 0.1f >= peak) return V3(0.0f);
+            result = result * (0.1f / peak);
+        }
+        const V3 p = origin + d_idx * tt;
+        const float majorant = flight_majorant(sc, p);
+        const float exit = tt + brick_exit(p, d_idx) + 1e-3f;
+        if (majorant <= 0.0f) {
+            tt = exit;
+            continue;
+        }
+        const float next = tt - logf(1.0f - rng.rand()) / majorant;
+        if (next >= exit) {
+            tt = exit;
+            continue;
+        }
+        tt = next;
+        const Sample m = sample_point(sc, origin + d_idx * tt);
+        result = result * vmax(1.0f - path_sigma(u, m, m.extinction) / majorant, V3(0.0f));
+    }
+    return result;
+}
+
+// What one path adds to the accumulation and guide images (pt.comp:403-552).
+struct PathResult {
+    V3 radiance;     // after the firefly clamp
+    float background;
+    V3 albedo;       // first-event albedo (1 for background)
+    V3 normal;       // first-event normal facing the viewer
+    float depth;     // index-space distance to the first event (1e4 for background)
+};
+
+__device__ inline PathResult trace_path(const DevScene& sc, uint32_t px, uint32_t py, Rng& rng) {
+    const vvr_uniforms* u = sc.u;
+    const uint32_t max_bounces = u->path[1];
+    const uint32_t volume_limit = uint32_t(float(kMaxVolumeEvents) / (1.0f - clampf(u->env[2], 0.0f, 0.95f)));
+    const float step_mm = u->dims[3];
+    const V3 ldir(u->light_dir[0], u->light_dir[1], u->light_dir[2]);
+    const M3 light_frame = basis(ldir);
+    const V3 light(u->light_dir[3]);
+    const float cos_light = cosf(u->material[3]);
+    const bool area_light = u->material[3] > 1e-4f;
+    const float light_pdf = area_light ? 1.0f / (2.0f * kPi * (1.0f - cos_light)) : 0.0f;
+    const M3 ti = to_index(u);
+    const V3 eye_idx = world_to_index_point(u, V3(u->eye[0], u->eye[1], u->eye[2]));
+
+    PathResult res;
+    res.radiance = V3(0.0f);
+    res.background = 0.0f;
+    res.albedo = V3(0.0f);
+    res.normal = V3(0.0f);
+    res.depth = 0.0f;
+
+    const float jx = rng.rand();
+    const float jy = rng.rand();
+    V3 dir = camera_ray(u, float(px) + jx, float(py) + jy);
+    V3 origin = eye_idx;
+    if (u->lens[0] > 0.0f) {  // thin lens (pt.comp:406-416)
+        V3 eye(u->eye[0], u->eye[1], u->eye[2]);
+        const V3 fwd = normalize(V3(u->forward[0], u->forward[1], u->forward[2]));
+        const V3 focal = eye + dir * (u->lens[1] / dot(dir, fwd));
+        const float r = u->lens[0] * sqrtf(rng.rand());
+        const float phi = 2.0f * kPi * rng.rand();
+        const V3 rr = normalize(V3(u->right[0], u->right[1], u->right[2]));
+        const V3 uu = normalize(V3(u->up[0], u->up[1], u->up[2]));
+        eye = eye + r * (cosf(phi) * rr + sinf(phi) * uu);
+        dir = normalize(focal - eye);
+        origin = world_to_index_point(u, eye);
+    }
+    V3 throughput(1.0f);
+    V3 radiance(0.0f);
+    const int hero = min(int(rng.rand() * 3.0f), 2);
+    V3 spectral(1.0f);
+    float last_pdf = 0.0f;
+    for (uint32_t bounce = 0, surfaces = 0, volumes = 0; surfaces <= max_bounces && volumes <= volume_limit;
+         bounce++) {
+        V3 idx;
+        Sample s;
+        int cap_plane = -1;
+        float cut = 0.0f;
+        if (!free_flight(sc, origin, dir, bounce == 0u, hero, spectral, idx, s, cap_plane, cut, rng)) {
+            if (bounce == 0u) {
+                res.background = 1.0f;
+                res.albedo = V3(1.0f);
+                res.depth = 1e4f;
+            } else {
+                const V3 path_weight = throughput * spectral / dot(spectral, V3(1.0f / 3.0f));
+                radiance = radiance + path_weight * environment(u, dir);
+                if (area_light && dot(dir, ldir) >= cos_light)
+                    radiance = radiance + path_weight * light * light_pdf * mis(last_pdf, light_pdf);
+            }
+            break;
+        }
+        const V3 path_weight = throughput * spectral / dot(spectral, V3(1.0f / 3.0f));
+        const V3 v = -dir;
+        V3 n;
+        float surfaceness;
+        surface_frame(sc, idx, s, v, cap_plane, cut, 1.5f * step_mm, n, surfaceness);
+        apply_surface_texture(u, idx, s);
+        const float xi_surface = rng.rand();
+        const bool surface_event = xi_surface < surfaceness * (1.0f - s.subsurface) && dot(n, n) > 0.5f;
+        if (surface_event || bounce == 0u) n = bump_normal(u, idx, s, n);
+        if (bounce == 0u) {
+            res.albedo = s.albedo;
+            res.normal = dot(n, v) < 0.0f ? -n : n;
+            res.depth = length(idx - eye_idx);
+        }
+        const V3 light_dir = sample_light(u, light_frame, rng);
+        const V3 light_t = transmittance(sc, idx, light_dir, rng);
+        if (surface_event) {
+            n = dot(n, v) < 0.0f ? -n : n;
+            V3 diffuse;
+            V3 specular = brdf_coated(n, v, light_dir, s, diffuse);
+            const float light_weight = area_light ? mis(light_pdf, bsdf_pdf(n, v, light_dir, s)) : 1.0f;
+            radiance = radiance +
+                       path_weight * (diffuse + specular) * gmax(dot(n, light_dir), 0.0f) * light * light_t * light_weight;
+            const M3 frame = basis(n);
+            V3 l;
+            if (rng.rand() < coat_share(s)) {
+                l = sample_ggx(frame, v, coat_roughness(s) * coat_roughness(s), rng);
+            } else if (rng.rand() < specular_share(s)) {
+                l = sample_ggx(frame, v, gmax(s.roughness * s.roughness, 2e-3f), rng);
+            } else {
+                const float r = sqrtf(rng.rand());
+                const float phi = 2.0f * kPi * rng.rand();
+                l = frame.mul(V3(r * cosf(phi), r * sinf(phi), sqrtf(gmax(0.0f, 1.0f - r * r))));
+            }
+            const float nl = dot(n, l);
+            if (nl <= 0.0f) break;
+            const float pdf = bsdf_pdf(n, v, l, s);
+            specular = brdf_coated(n, v, l, s, diffuse);
+            throughput = throughput * (((diffuse + specular) * nl) / gmax(pdf, 1e-6f));
+            last_pdf = pdf;
+            surfaces++;
+            origin = idx + ti.mul(n * step_mm);
+            dir = l;
+        } else {
+            const V3 albedo = event_albedo(u, s);
+            const float light_phase = phase_hg(u, dot(dir, light_dir));
+            radiance = radiance + path_weight * albedo * light_phase * light * light_t *
+                                      (area_light ? mis(light_pdf, light_phase) : 1.0f);
+            throughput = throughput * albedo;
+            volumes++;
+            origin = idx;
+            const V3 next = sample_hg(u, dir, rng);
+            last_pdf = phase_hg(u, dot(dir, next));
+            dir = next;
+        }
+        if (bounce >= 2u) {
+            const V3 w = throughput * spectral / dot(spectral, V3(1.0f / 3.0f));
+            const float survive = clampf(gmax(w.x, gmax(w.y, w.z)), 0.05f, 1.0f);
+            if (rng.rand() > survive) break;
+            throughput = throughput / survive;
+        }
+    }
+    const float peak = gmax(radiance.x, gmax(radiance.y, radiance.z));
+    res.radiance = peak > 20.0f ? radiance * (20.0f / peak) : radiance;
+    return res;
+}
+
+}  // namespace vvr

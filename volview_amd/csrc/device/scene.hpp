@@ -1,0 +1,495 @@
+// Device-side volume access and the shared fetch library of the ray marcher and
+// path tracer: the HIP counterpart of shaders/common.glsl.
+//
+// Volumes live in HBM as 4^3-voxel micro-bricks (64 voxels, x fastest inside a
+// brick, bricks x fastest): a 2^3 trilinear footprint touches 1-2 bricks on
+// average and the 4^3 cubic B-spline footprint at most 8 instead of 16 rows of a
+// linear layout. Filtering is done in software with the exact texture-unit
+// model: weights frac(idx), texel indices clamped to the volume.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../../include/vvr_uniforms.h"
+#include "types.hpp"
+#include "vec.hpp"
+
+namespace vvr {
+
+__device__ __forceinline__ int iclamp(int v, int lo, int hi) { return min(max(v, lo), hi); }
+// Float to int for an index that may lie far outside the volume.
+__device__ __forceinline__ int fidx(float f) { return int(fminf(fmaxf(f, -1.0e8f), 1.0e8f)); }
+
+constexpr float kInv65535 = 1.0f / 65535.0f;
+
+// ---- linear fetches (common.glsl:76-78 and the mask / normal images) -----------
+
+__device__ __forceinline__ float density_at_index(const DevScene& s, V3 idx) {
+    const float fx = floorf(idx.x), fy = floorf(idx.y), fz = floorf(idx.z);
+    const float ax = idx.x - fx, ay = idx.y - fy, az = idx.z - fz;
+    const int x0 = fidx(fx), y0 = fidx(fy), z0 = fidx(fz);
+    const BrickGrid& g = s.g;
+    const uint32_t X0 = g.xoff(iclamp(x0, 0, g.nx - 1)), X1 = g.xoff(iclamp(x0 + 1, 0, g.nx - 1));
+    const uint32_t Y0 = g.yoff(iclamp(y0, 0, g.ny - 1)), Y1 = g.yoff(iclamp(y0 + 1, 0, g.ny - 1));
+    const uint32_t Z0 = g.zoff(iclamp(z0, 0, g.nz - 1)), Z1 = g.zoff(iclamp(z0 + 1, 0, g.nz - 1));
+    const uint16_t* d = s.density;
+    const float c000 = d[X0 + Y0 + Z0], c100 = d[X1 + Y0 + Z0], c010 = d[X0 + Y1 + Z0], c110 = d[X1 + Y1 + Z0];
+    const float c001 = d[X0 + Y0 + Z1], c101 = d[X1 + Y0 + Z1], c011 = d[X0 + Y1 + Z1], c111 = d[X1 + Y1 + Z1];
+    const float c00 = mixf(c000, c100, ax), c10 = mixf(c010, c110, ax);
+    const float c01 = mixf(c001, c101, ax), c11 = mixf(c011, c111, ax);
+    return mixf(mixf(c00, c10, ay), mixf(c01, c11, ay), az) * kInv65535;
+}
+
+struct F2 {
+    float r, g;
+};
+
+__device__ __forceinline__ F2 mask_at(const DevScene& s, V3 idx) {
+    const float fx = floorf(idx.x), fy = floorf(idx.y), fz = floorf(idx.z);
+    const float ax = idx.x - fx, ay = idx.y - fy, az = idx.z - fz;
+    const int x0 = fidx(fx), y0 = fidx(fy), z0 = fidx(fz);
+    const BrickGrid& g = s.g;
+    const uint32_t X0 = g.xoff(iclamp(x0, 0, g.nx - 1)), X1 = g.xoff(iclamp(x0 + 1, 0, g.nx - 1));
+    const uint32_t Y0 = g.yoff(iclamp(y0, 0, g.ny - 1)), Y1 = g.yoff(iclamp(y0 + 1, 0, g.ny - 1));
+    const uint32_t Z0 = g.zoff(iclamp(z0, 0, g.nz - 1)), Z1 = g.zoff(iclamp(z0 + 1, 0, g.nz - 1));
+    const uint32_t* m = s.mask;
+    const uint32_t t[8] = {m[X0 + Y0 + Z0], m[X1 + Y0 + Z0], m[X0 + Y1 + Z0], m[X1 + Y1 + Z0],
+                           m[X0 + Y0 + Z1], m[X1 + Y0 + Z1], m[X0 + Y1 + Z1], m[X1 + Y1 + Z1]};
+    F2 out;
+#pragma unroll
+    for (int ch = 0; ch < 2; ch++) {
+        const int sh = 16 * ch;
+        const float c00 = mixf(float((t[0] >> sh) & 0xffffu), float((t[1] >> sh) & 0xffffu), ax);
+        const float c10 = mixf(float((t[2] >> sh) & 0xffffu), float((t[3] >> sh) & 0xffffu), ax);
+        const float c01 = mixf(float((t[4] >> sh) & 0xffffu), float((t[5] >> sh) & 0xffffu), ax);
+        const float c11 = mixf(float((t[6] >> sh) & 0xffffu), float((t[7] >> sh) & 0xffffu), ax);
+        const float v = mixf(mixf(c00, c10, ay), mixf(c01, c11, ay), az) * kInv65535;
+        if (ch == 0) out.r = v;
+        else out.g = v;
+    }
+    return out;
+}
+
+__device__ __forceinline__ float mask_channel(const DevScene& s, V3 idx, int ch) {
+    const F2 m = mask_at(s, idx);
+    return ch == 0 ? m.r : m.g;
+}
+
+struct F4 {
+    float x, y, z, w;
+};
+
+__device__ __forceinline__ float snorm8(uint32_t word, int ch) {
+    return gmax(float(int8_t((word >> (8 * ch)) & 0xffu)) * (1.0f / 127.0f), -1.0f);
+}
+
+__device__ __forceinline__ F4 normal_at(const DevScene& s, V3 idx) {
+    const float fx = floorf(idx.x), fy = floorf(idx.y), fz = floorf(idx.z);
+    const float ax = idx.x - fx, ay = idx.y - fy, az = idx.z - fz;
+    const int x0 = fidx(fx), y0 = fidx(fy), z0 = fidx(fz);
+    const BrickGrid& g = s.g;
+    const uint32_t X0 = g.xoff(iclamp(x0, 0, g.nx - 1)), X1 = g.xoff(iclamp(x0 + 1, 0, g.nx - 1));
+    const uint32_t Y0 = g.yoff(iclamp(y0, 0, g.ny - 1)), Y1 = g.yoff(iclamp(y0 + 1, 0, g.ny - 1));
+    const uint32_t Z0 = g.zoff(iclamp(z0, 0, g.nz - 1)), Z1 = g.zoff(iclamp(z0 + 1, 0, g.nz - 1));
+    const uint32_t* n = s.normals;
+    const uint32_t t[8] = {n[X0 + Y0 + Z0], n[X1 + Y0 + Z0], n[X0 + Y1 + Z0], n[X1 + Y1 + Z0],
+                           n[X0 + Y0 + Z1], n[X1 + Y0 + Z1], n[X0 + Y1 + Z1], n[X1 + Y1 + Z1]};
+    float r[4];
+#pragma unroll
+    for (int ch = 0; ch < 4; ch++) {
+        const float c00 = mixf(snorm8(t[0], ch), snorm8(t[1], ch), ax);
+        const float c10 = mixf(snorm8(t[2], ch), snorm8(t[3], ch), ax);
+        const float c01 = mixf(snorm8(t[4], ch), snorm8(t[5], ch), ax);
+        const float c11 = mixf(snorm8(t[6], ch), snorm8(t[7], ch), ax);
+        r[ch] = mixf(mixf(c00, c10, ay), mixf(c01, c11, ay), az);
+    }
+    return F4{r[0], r[1], r[2], r[3]};
+}
+
+// ---- cubic B-spline, 8 trilinear taps (common.glsl:84-136) --------------------------
+//
+// The 8 (or 24 for the gradient) trilinear fetches of Sigg and Hadwiger read only
+// the 4^3 voxels floor(idx) - 1 .. floor(idx) + 2; they are loaded once and the
+// taps are evaluated from registers in the reference's order of operations.
+
+struct Taps1 {
+    float h0, h1, g0, g1, dh0, dh1, dg0, dg1;
+};
+
+__device__ __forceinline__ Taps1 cubic_taps1(float idx, float i) {
+    const float f = idx - i;
+    const float f2 = f * f;
+    const float f3 = f2 * f;
+    const float w0 = ((1.0f - 3.0f * f) + 3.0f * f2 - f3) / 6.0f;
+    const float w1 = ((4.0f - 6.0f * f2) + 3.0f * f3) / 6.0f;
+    const float w2 = (((1.0f + 3.0f * f) + 3.0f * f2) - 3.0f * f3) / 6.0f;
+    const float w3 = f3 / 6.0f;
+    const float d0 = (-0.5f * (1.0f - f)) * (1.0f - f);
+    const float d1 = (0.5f * f) * (3.0f * f - 4.0f);
+    const float d2 = (0.5f * (1.0f - f)) * (3.0f * f + 1.0f);
+    const float d3 = 0.5f * f2;
+    Taps1 t;
+    t.g0 = w0 + w1;
+    t.g1 = w2 + w3;
+    t.h0 = (i - 1.0f) + w1 / t.g0;
+    t.h1 = (i + 1.0f) + w3 / t.g1;
+    t.dg0 = d0 + d1;
+    t.dg1 = d2 + d3;
+    t.dh0 = (i - 1.0f) + d1 / gmin(t.dg0, -1e-7f);
+    t.dh1 = (i + 1.0f) + d3 / gmax(t.dg1, 1e-7f);
+    return t;
+}
+
+// A linear tap inside the 4-voxel window starting at i - 1: lower voxel offset and weight.
+struct Tap {
+    int o;    // 0..3, window offset of the lower texel
+    float a;  // weight of the upper texel
+};
+__device__ __forceinline__ Tap tap_of(float h, float i) {
+    const float fl = floorf(h);
+    Tap t;
+    t.o = int(fl - (i - 1.0f));  // 0, 1 (value/derivative low taps) or 2, 3 (high taps)
+    t.a = h - fl;
+    return t;
+}
+
+// A 4x4x4 window of raw density values around floor(idx).
+struct Window {
+    float v[4][4][4];  // [z][y][x]
+};
+
+__device__ __forceinline__ void load_window(const DevScene& s, int ix, int iy, int iz, Window& w) {
+    const BrickGrid& g = s.g;
+    uint32_t X[4], Y[4], Z[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        X[k] = g.xoff(iclamp(ix - 1 + k, 0, g.nx - 1));
+        Y[k] = g.yoff(iclamp(iy - 1 + k, 0, g.ny - 1));
+        Z[k] = g.zoff(iclamp(iz - 1 + k, 0, g.nz - 1));
+    }
+#pragma unroll
+    for (int z = 0; z < 4; z++)
+#pragma unroll
+        for (int y = 0; y < 4; y++)
+#pragma unroll
+            for (int x = 0; x < 4; x++) w.v[z][y][x] = float(s.density[X[x] + Y[y] + Z[z]]);
+}
+
+// Linear interpolation of 4 window values at tap t (upper index clamped to the
+// window: it only leaves it with weight 0).
+__device__ __forceinline__ float lerp4(float v0, float v1, float v2, float v3, Tap t) {
+    const float lo = t.o <= 0 ? v0 : (t.o == 1 ? v1 : (t.o == 2 ? v2 : v3));
+    const float hi = t.o <= 0 ? v1 : (t.o == 1 ? v2 : v3);
+    return mixf(lo, hi, t.a);
+}
+
+// The 8 trilinear fetches of cubic_fetch (common.glsl:116-122) on the window, in
+// raw units, sharing work between fetches: every fetch lerps along x, then y,
+// then z, exactly as one texture fetch would, but rows lerped along x at the
+// same tap are computed once.
+struct XRows {
+    float r[4][4];  // [z][y]
+};
+__device__ __forceinline__ void x_pass(const Window& w, Tap t, XRows& o) {
+#pragma unroll
+    for (int z = 0; z < 4; z++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) o.r[z][y] = lerp4(w.v[z][y][0], w.v[z][y][1], w.v[z][y][2], w.v[z][y][3], t);
+}
+// y then z lerps of x-lerped rows: one trilinear fetch.
+__device__ __forceinline__ float yz_pass(const XRows& a, Tap ty, Tap tz) {
+    float zz[4];
+#pragma unroll
+    for (int z = 0; z < 4; z++) zz[z] = lerp4(a.r[z][0], a.r[z][1], a.r[z][2], a.r[z][3], ty);
+    return lerp4(zz[0], zz[1], zz[2], zz[3], tz);
+}
+__device__ __forceinline__ float cubic_fetch_rows(const XRows& a0, const XRows& a1, Tap y0, Tap y1, Tap z0, Tap z1,
+                                                  float gx0, float gx1, float gy0, float gy1, float gz0, float gz1) {
+    const float y0z0 = gx0 * yz_pass(a0, y0, z0) + gx1 * yz_pass(a1, y0, z0);
+    const float y1z0 = gx0 * yz_pass(a0, y1, z0) + gx1 * yz_pass(a1, y1, z0);
+    const float y0z1 = gx0 * yz_pass(a0, y0, z1) + gx1 * yz_pass(a1, y0, z1);
+    const float y1z1 = gx0 * yz_pass(a0, y1, z1) + gx1 * yz_pass(a1, y1, z1);
+    return gz0 * (gy0 * y0z0 + gy1 * y1z0) + gz1 * (gy0 * y0z1 + gy1 * y1z1);
+}
+
+// density_cubic (common.glsl:124-127).
+__device__ __forceinline__ float density_cubic(const DevScene& s, V3 idx) {
+    const float ix = floorf(idx.x), iy = floorf(idx.y), iz = floorf(idx.z);
+    const Taps1 tx = cubic_taps1(idx.x, ix), ty = cubic_taps1(idx.y, iy), tz = cubic_taps1(idx.z, iz);
+    Window w;
+    load_window(s, fidx(ix), fidx(iy), fidx(iz), w);
+    XRows a0, a1;
+    x_pass(w, tap_of(tx.h0, ix), a0);
+    x_pass(w, tap_of(tx.h1, ix), a1);
+    const float r = cubic_fetch_rows(a0, a1, tap_of(ty.h0, iy), tap_of(ty.h1, iy), tap_of(tz.h0, iz),
+                                     tap_of(tz.h1, iz), tx.g0, tx.g1, ty.g0, ty.g1, tz.g0, tz.g1);
+    return r * kInv65535;
+}
+
+// density_gradient_cubic (common.glsl:130-136), per index unit.
+__device__ __forceinline__ V3 density_gradient_cubic(const DevScene& s, V3 idx) {
+    const float ix = floorf(idx.x), iy = floorf(idx.y), iz = floorf(idx.z);
+    const Taps1 tx = cubic_taps1(idx.x, ix), ty = cubic_taps1(idx.y, iy), tz = cubic_taps1(idx.z, iz);
+    Window w;
+    load_window(s, fidx(ix), fidx(iy), fidx(iz), w);
+    const Tap hx0 = tap_of(tx.h0, ix), hx1 = tap_of(tx.h1, ix), dx0 = tap_of(tx.dh0, ix), dx1 = tap_of(tx.dh1, ix);
+    const Tap hy0 = tap_of(ty.h0, iy), hy1 = tap_of(ty.h1, iy), dy0 = tap_of(ty.dh0, iy), dy1 = tap_of(ty.dh1, iy);
+    const Tap hz0 = tap_of(tz.h0, iz), hz1 = tap_of(tz.h1, iz), dz0 = tap_of(tz.dh0, iz), dz1 = tap_of(tz.dh1, iz);
+    XRows d0, d1, a0, a1;
+    x_pass(w, dx0, d0);
+    x_pass(w, dx1, d1);
+    x_pass(w, hx0, a0);
+    x_pass(w, hx1, a1);
+    const float gx = cubic_fetch_rows(d0, d1, hy0, hy1, hz0, hz1, tx.dg0, tx.dg1, ty.g0, ty.g1, tz.g0, tz.g1);
+    const float gy = cubic_fetch_rows(a0, a1, dy0, dy1, hz0, hz1, tx.g0, tx.g1, ty.dg0, ty.dg1, tz.g0, tz.g1);
+    const float gz = cubic_fetch_rows(a0, a1, hy0, hy1, dz0, dz1, tx.g0, tx.g1, ty.g0, ty.g1, tz.dg0, tz.dg1);
+    return V3(gx * kInv65535, gy * kInv65535, gz * kInv65535);
+}
+
+// density_view (common.glsl:160-162).
+__device__ __forceinline__ float density_view(const DevScene& s, V3 idx) {
+    return s.u->env[1] > 0.5f ? density_cubic(s, idx) : density_at_index(s, idx);
+}
+
+// ---- transfer function (common.glsl:170-230) ------------------------------------------
+
+struct Sample {
+    V3 albedo;
+    float extinction, roughness, metalness;
+    uint32_t label;
+    bool ghost;
+    float density, subsurface, coat;
+};
+
+__device__ __forceinline__ float4 transfer(const DevScene& s, float density) {
+    const float t = density * 4095.0f;
+    const float fl = floorf(t);
+    const float a = t - fl;
+    const int i0 = iclamp(fidx(fl), 0, 4095), i1 = iclamp(fidx(fl) + 1, 0, 4095);
+    const float4 p = s.lut[i0], q = s.lut[i1];
+    return make_float4(mixf(p.x, q.x, a), mixf(p.y, q.y, a), mixf(p.z, q.z, a), mixf(p.w, q.w, a));
+}
+
+__device__ __forceinline__ Sample tf_point(const DevScene& s, float density) {
+    const float4 tf = transfer(s, density);
+    Sample r;
+    r.albedo = V3(tf.x, tf.y, tf.z);
+    r.extinction = tf.w;
+    r.roughness = s.u->material[0];
+    r.metalness = s.u->material[1];
+    r.label = 0u;
+    r.ghost = false;
+    r.density = density;
+    r.subsurface = s.u->subsurface[0];
+    r.coat = 0.0f;
+    return r;
+}
+
+__device__ __forceinline__ float4 preint_at(const DevScene& s, float density) {
+    const float x = clampf(density, 0.0f, 1.0f) * 4095.0f;
+    const int i = min(int(x), 4094);
+    const float a = x - float(i);
+    const float4 p = s.preint[i], q = s.preint[i + 1];
+    return make_float4(mixf(p.x, q.x, a), mixf(p.y, q.y, a), mixf(p.z, q.z, a), mixf(p.w, q.w, a));
+}
+
+__device__ inline Sample tf_segment(const DevScene& s, float front, float back) {
+    if (fabsf(back - front) < 1e-4f) return tf_point(s, 0.5f * (front + back));
+    const float4 pb = preint_at(s, back), pf = preint_at(s, front);
+    const float den = back - front;
+    const float4 mean = make_float4((pb.x - pf.x) / den, (pb.y - pf.y) / den, (pb.z - pf.z) / den, (pb.w - pf.w) / den);
+    Sample r = tf_point(s, back);  // albedo of transfer(back) when the mean extinction vanishes
+    if (mean.w > 1e-6f) r.albedo = V3(mean.x, mean.y, mean.z) / mean.w;
+    r.extinction = gmax(mean.w, 0.0f);
+    r.density = back;
+    return r;
+}
+
+__device__ inline float preint_half_depth(const DevScene& s, float front, float back) {
+    const float target = 0.5f * (preint_at(s, front).w + preint_at(s, back).w);
+    float lo = 0.0f, hi = 1.0f;
+    for (int i = 0; i < 6; i++) {
+        const float mid = 0.5f * (lo + hi);
+        const bool before = (preint_at(s, mixf(front, back, mid)).w < target) == (back > front);
+        lo = before ? mid : lo;
+        hi = before ? hi : mid;
+    }
+    return 0.5f * (lo + hi);
+}
+
+// ---- labels (common.glsl:232-351) --------------------------------------------------------
+
+__device__ __forceinline__ bool label_shown(const vvr_uniforms* u, uint32_t l) { return u->label_colors[l][3] > 0.5f; }
+__device__ __forceinline__ bool label_ghosted(const vvr_uniforms* u, uint32_t l) {
+    return l != 0u && u->label_colors[l][3] > 1.5f;
+}
+__device__ __forceinline__ bool label_solid(const vvr_uniforms* u, uint32_t l, uint32_t mode) {
+    if (label_ghosted(u, l)) return false;
+    return mode == 1u ? (l == 0u || label_shown(u, l)) : (l != 0u && label_shown(u, l));
+}
+
+__device__ inline Sample with_label(const DevScene& sc, Sample s, uint32_t label, uint32_t mode, bool ghost) {
+    const vvr_uniforms* u = sc.u;
+    s.label = label;
+    s.ghost = ghost;
+    if (label == 0u) return s;
+    const float4 color = *reinterpret_cast<const float4*>(u->label_colors[label]);
+    const float4 mat = *reinterpret_cast<const float4*>(u->label_materials[label]);
+    if (mode == 1u) {
+        s.albedo = vmix(s.albedo, V3(color.x, color.y, color.z), u->labels[1]);
+    } else {
+        s.albedo = V3(color.x, color.y, color.z);
+        s.extinction = u->labels[2];
+        const float4 d = *reinterpret_cast<const float4*>(u->label_density[label]);
+        if (mode == 3u && !ghost && d.w > 0.0f) {
+            s.extinction *= smoothstepf(d.x, d.y, s.density);
+            const float t = clampf((s.density - d.z) / d.w, -1.5f, 1.5f);
+            s.albedo = vclamp(s.albedo * (1.0f + 0.2f * t), 0.0f, 1.0f);
+            const float kBoneDensity = (300.0f + 1024.0f) / 4095.0f;
+            if (d.z > kBoneDensity) {
+                const float thin = 1.0f - smoothstepf(-1.5f, -0.5f, t);
+                s.albedo = s.albedo * vmix(V3(1.0f), V3(0.92f, 0.6f, 0.5f), thin);
+            }
+        }
+    }
+    s.roughness = mat.x;
+    s.metalness = mat.y;
+    s.subsurface = mat.w;
+    s.coat = mat.z;
+    return s;
+}
+
+__device__ __forceinline__ uint32_t label_texel(const DevScene& s, int x, int y, int z) {
+    return s.labels[s.g.at(x, y, z)];
+}
+
+// drawn_label_near (common.glsl:302-324): the nearest voxel, the 8 cell corners,
+// then the 12 axis neighbours at +-1 and +-2.
+__device__ inline bool drawn_label_near(const DevScene& s, V3 idx, uint32_t mode, bool ghost, uint32_t& label) {
+    const int hx = s.g.nx - 1, hy = s.g.ny - 1, hz = s.g.nz - 1;
+    const int nx = iclamp(fidx(floorf(idx.x + 0.5f)), 0, hx);
+    const int ny = iclamp(fidx(floorf(idx.y + 0.5f)), 0, hy);
+    const int nz = iclamp(fidx(floorf(idx.z + 0.5f)), 0, hz);
+    const int bx = fidx(floorf(idx.x)), by = fidx(floorf(idx.y)), bz = fidx(floorf(idx.z));
+    for (int i = 0; i < 21; i++) {
+        int cx, cy, cz;
+        if (i == 0) {
+            cx = nx, cy = ny, cz = nz;
+        } else if (i < 9) {
+            cx = bx + ((i - 1) & 1), cy = by + (((i - 1) >> 1) & 1), cz = bz + ((i - 1) >> 2);
+        } else {
+            const int k = i - 9;
+            const int d = ((k / 3) % 2 == 0 ? 1 : -1) * (k < 6 ? 1 : 2);
+            const int a = k % 3;
+            cx = nx + (a == 0 ? d : 0), cy = ny + (a == 1 ? d : 0), cz = nz + (a == 2 ? d : 0);
+        }
+        label = label_texel(s, iclamp(cx, 0, hx), iclamp(cy, 0, hy), iclamp(cz, 0, hz));
+        if (ghost ? label_ghosted(s.u, label) : label_solid(s.u, label, mode)) return true;
+    }
+    return false;
+}
+
+__device__ inline Sample apply_labels(const DevScene& s, Sample smp, V3 idx) {
+    const uint32_t mode = uint32_t(s.u->labels[0]);
+    if (mode == 0u) return smp;
+    const F2 m = mask_at(s, idx);
+    uint32_t label = 0u;
+    if (m.r < (mode == 3u ? 0.25f : 0.5f)) {
+        if (m.g >= 0.5f && drawn_label_near(s, idx, mode, true, label)) return with_label(s, smp, label, mode, true);
+        smp.extinction = 0.0f;
+        return smp;
+    }
+    if (drawn_label_near(s, idx, mode, false, label)) return with_label(s, smp, label, mode, false);
+    if (mode >= 2u) smp.extinction = 0.0f;
+    return smp;
+}
+
+// ---- clipping and geometry (common.glsl:357-442) ----------------------------------------
+
+__device__ __forceinline__ float clip_depth(const vvr_uniforms* u, V3 idx, int& plane) {
+    plane = -1;
+    const uint32_t count = u->clip[0];
+    if (count == 0u) return -1e30f;
+    const bool all = u->clip[1] != 0u;
+    float depth = all ? 1e30f : -1e30f;
+    for (uint32_t i = 0; i < count; i++) {
+        const float d = dot(V3(u->clip_planes[i][0], u->clip_planes[i][1], u->clip_planes[i][2]), idx) - u->clip_planes[i][3];
+        if (all ? d < depth : d > depth) {
+            depth = d;
+            plane = int(i);
+        }
+    }
+    return depth;
+}
+
+__device__ __forceinline__ bool is_clipped(const vvr_uniforms* u, V3 idx) {
+    int plane;
+    return clip_depth(u, idx, plane) > 0.0f;
+}
+
+__device__ __forceinline__ Sample sample_point(const DevScene& s, V3 idx) {
+    Sample smp = apply_labels(s, tf_point(s, density_at_index(s, idx)), idx);
+    if (smp.ghost || is_clipped(s.u, idx)) smp.extinction = 0.0f;
+    return smp;
+}
+
+__device__ __forceinline__ M3 to_index(const vvr_uniforms* u) {
+    const float* m = u->world_to_index;
+    M3 r;
+    r.c0 = V3(m[0], m[1], m[2]);
+    r.c1 = V3(m[4], m[5], m[6]);
+    r.c2 = V3(m[8], m[9], m[10]);
+    return r;
+}
+
+__device__ __forceinline__ V3 world_to_index_point(const vvr_uniforms* u, V3 p) {
+    const float* m = u->world_to_index;
+    return V3(m[0] * p.x + m[4] * p.y + m[8] * p.z + m[12], m[1] * p.x + m[5] * p.y + m[9] * p.z + m[13],
+              m[2] * p.x + m[6] * p.y + m[10] * p.z + m[14]);
+}
+
+__device__ inline V3 mask_gradient(const DevScene& s, V3 idx, int ch) {
+    const float gx = mask_channel(s, idx + V3(1.f, 0.f, 0.f), ch) - mask_channel(s, idx - V3(1.f, 0.f, 0.f), ch);
+    const float gy = mask_channel(s, idx + V3(0.f, 1.f, 0.f), ch) - mask_channel(s, idx - V3(0.f, 1.f, 0.f), ch);
+    const float gz = mask_channel(s, idx + V3(0.f, 0.f, 1.f), ch) - mask_channel(s, idx - V3(0.f, 0.f, 1.f), ch);
+    return to_index(s.u).mul_t(0.5f * V3(gx, gy, gz));
+}
+
+__device__ __forceinline__ void intersect_volume(const vvr_uniforms* u, V3 origin, V3 dir, float& tn, float& tf) {
+    const V3 lo(-0.5f);
+    const V3 hi(u->dims[0] - 0.5f, u->dims[1] - 0.5f, u->dims[2] - 0.5f);
+    const V3 inv = V3(1.0f) / dir;
+    const V3 t0 = (lo - origin) * inv;
+    const V3 t1 = (hi - origin) * inv;
+    const V3 tmin = vmin(t0, t1);
+    const V3 tmax = vmax(t0, t1);
+    tn = gmax(gmax(tmin.x, tmin.y), gmax(tmin.z, 0.0f));
+    tf = gmin(gmin(tmax.x, tmax.y), tmax.z);
+}
+
+__device__ __forceinline__ float brick_majorant(const DevScene& s, V3 idx) {
+    if (s.u->path[3] == 0u) return 1e30f;
+    const V3 b = vfloor((idx + 0.5f) / 8.0f);
+    const int bx = iclamp(fidx(b.x), 0, s.mx - 1), by = iclamp(fidx(b.y), 0, s.my - 1),
+              bz = iclamp(fidx(b.z), 0, s.mz - 1);
+    return s.majorant[uint32_t(bx) + uint32_t(s.mx) * (uint32_t(by) + uint32_t(s.my) * uint32_t(bz))];
+}
+
+__device__ __forceinline__ float brick_exit(V3 idx, V3 d_idx) {
+    const V3 lo = vfloor((idx + 0.5f) / 8.0f) * 8.0f - 0.5f;
+    const V3 hi = lo + 8.0f;
+    const V3 d(d_idx.x == 0.0f ? 1e-9f : d_idx.x, d_idx.y == 0.0f ? 1e-9f : d_idx.y, d_idx.z == 0.0f ? 1e-9f : d_idx.z);
+    const V3 t_far = vmax((lo - idx) / d, (hi - idx) / d);
+    return gmax(gmin(t_far.x, gmin(t_far.y, t_far.z)), 0.0f);
+}
+
+__device__ __forceinline__ V3 camera_ray(const vvr_uniforms* u, float px, float py) {
+    const float nx = px / float(u->flags[2]) * 2.0f - 1.0f;
+    const float ny = py / float(u->flags[3]) * 2.0f - 1.0f;
+    const V3 f(u->forward[0], u->forward[1], u->forward[2]);
+    const V3 r(u->right[0], u->right[1], u->right[2]);
+    const V3 up(u->up[0], u->up[1], u->up[2]);
+    return normalize((f + nx * r) - ny * up);
+}
+
+}  // namespace vvr

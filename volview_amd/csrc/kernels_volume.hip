@@ -1,0 +1,379 @@
+// Volume passes: density upload, normals (normals.comp), label masks
+// (mask.comp) and the brick majorant (majorant.comp). They run once per volume
+// or per opacity change and are HBM scans; their quantised outputs are
+// reproduced bit for bit, so floating-point contraction is off in this file.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include "launch.hpp"
+
+namespace vvr {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ void brick_coords(const BrickGrid& g, uint32_t o, int& x, int& y, int& z) {
+    const uint32_t brick = o >> 6, w = o & 63u;
+    const uint32_t bxy = uint32_t(g.bx) * uint32_t(g.by);
+    const uint32_t bz = brick / bxy, rem = brick - bz * bxy;
+    const uint32_t by = rem / uint32_t(g.bx), bx = rem - by * uint32_t(g.bx);
+    x = int(bx * 4u + (w & 3u));
+    y = int(by * 4u + ((w >> 2) & 3u));
+    z = int(bz * 4u + (w >> 4));
+}
+
+// transfer.rs:121-124, in f32 with a correctly rounded division.
+__device__ __forceinline__ uint16_t hu_to_unorm16(int16_t hu) {
+    float t = (float(hu) - (-1024.0f)) / (3071.0f - (-1024.0f));
+    t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+    return uint16_t(roundf(t * 65535.0f));
+}
+
+__global__ void k_upload_density(const int16_t* __restrict__ hu, uint16_t* __restrict__ out, BrickGrid g) {
+    const uint32_t o = blockIdx.x * kBlock + threadIdx.x;
+    if (o >= g.padded()) return;
+    int x, y, z;
+    brick_coords(g, o, x, y, z);
+    uint16_t v = 0;
+    if (x < g.nx && y < g.ny && z < g.nz) v = hu_to_unorm16(hu[size_t(x) + size_t(g.nx) * (size_t(y) + size_t(g.ny) * z)]);
+    out[o] = v;
+}
+
+__global__ void k_upload_labels(const uint8_t* __restrict__ in, uint8_t* __restrict__ out, BrickGrid g) {
+    const uint32_t o = blockIdx.x * kBlock + threadIdx.x;
+    if (o >= g.padded()) return;
+    int x, y, z;
+    brick_coords(g, o, x, y, z);
+    uint8_t v = 0;
+    if (x < g.nx && y < g.ny && z < g.nz) v = in[size_t(x) + size_t(g.nx) * (size_t(y) + size_t(g.ny) * z)];
+    out[o] = v;
+}
+
+template <class T>
+__global__ void k_debrick(const T* __restrict__ in, T* __restrict__ out, BrickGrid g) {
+    const size_t n = size_t(g.nx) * g.ny * g.nz;
+    const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int x = int(i % g.nx), y = int((i / g.nx) % g.ny), z = int(i / (size_t(g.nx) * g.ny));
+    out[i] = in[g.at(x, y, z)];
+}
+
+__device__ __forceinline__ float dtexel(const uint16_t* __restrict__ d, const BrickGrid& g, int x, int y, int z) {
+    x = min(max(x, 0), g.nx - 1);
+    y = min(max(y, 0), g.ny - 1);
+    z = min(max(z, 0), g.nz - 1);
+    return float(d[g.at(x, y, z)]) / 65535.0f;
+}
+
+__device__ __forceinline__ uint32_t snorm8_bits(float f) {
+    f = f < -1.0f ? -1.0f : (f > 1.0f ? 1.0f : f);
+    return uint32_t(uint8_t(int8_t(roundf(f * 127.0f))));
+}
+
+// normals.comp:14-37: 3x3x3 Sobel, world gradient, outward normal and surfaceness.
+__global__ void k_normals(const uint16_t* __restrict__ d, uint32_t* __restrict__ out, BrickGrid g,
+                          const vvr_uniforms* __restrict__ u) {
+    const uint32_t o = blockIdx.x * kBlock + threadIdx.x;
+    if (o >= g.padded()) return;
+    int x, y, z;
+    brick_coords(g, o, x, y, z);
+    if (x >= g.nx || y >= g.ny || z >= g.nz) {
+        out[o] = 0u;
+        return;
+    }
+    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+    for (int a = -1; a <= 1; a++) {
+        for (int b = -1; b <= 1; b++) {
+            const float w = float((2 - abs(a)) * (2 - abs(b)));
+            gx += w * (dtexel(d, g, x + 1, y + a, z + b) - dtexel(d, g, x - 1, y + a, z + b));
+            gy += w * (dtexel(d, g, x + a, y + 1, z + b) - dtexel(d, g, x + a, y - 1, z + b));
+            gz += w * (dtexel(d, g, x + a, y + b, z + 1) - dtexel(d, g, x + a, y + b, z - 1));
+        }
+    }
+    gx /= 32.0f;
+    gy /= 32.0f;
+    gz /= 32.0f;
+    const float* m = u->world_to_index;
+    // transpose(mat3(world_to_index)) * g
+    const float wx = m[0] * gx + m[1] * gy + m[2] * gz;
+    const float wy = m[4] * gx + m[5] * gy + m[6] * gz;
+    const float wz = m[8] * gx + m[9] * gy + m[10] * gz;
+    const float len = sqrtf(wx * wx + wy * wy + wz * wz);
+    float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+    if (len > 1e-7f) {
+        nx = -wx / len;
+        ny = -wy / len;
+        nz = -wz / len;
+    }
+    float sness = len * 400.0f;
+    sness = sness < 0.0f ? 0.0f : (sness > 1.0f ? 1.0f : sness);
+    out[o] = snorm8_bits(nx) | (snorm8_bits(ny) << 8) | (snorm8_bits(nz) << 16) | (snorm8_bits(sness) << 24);
+}
+
+__device__ __forceinline__ uint32_t unorm16_bits(float f) {
+    f = f < 0.0f ? 0.0f : (f > 1.0f ? 1.0f : f);
+    return uint32_t(roundf(f * 65535.0f));
+}
+
+__device__ __forceinline__ bool shown(const vvr_uniforms* u, uint32_t l) { return u->label_colors[l][3] > 0.5f; }
+__device__ __forceinline__ bool ghosted(const vvr_uniforms* u, uint32_t l) {
+    return l != 0u && u->label_colors[l][3] > 1.5f;
+}
+__device__ __forceinline__ bool solid(const vvr_uniforms* u, uint32_t l, uint32_t mode) {
+    if (ghosted(u, l)) return false;
+    return mode == 1u ? (l == 0u || shown(u, l)) : (l != 0u && shown(u, l));
+}
+
+// mask.comp:32-37
+__device__ __forceinline__ float soft_score(bool best, bool second, float margin) {
+    if (best == second) return best ? 4.0f : -4.0f;
+    return best ? fminf(margin, 4.0f) : -fminf(margin, 4.0f);
+}
+
+// Trilinear density at continuous index v (float, as density_at_index).
+__device__ __forceinline__ float density_linear(const uint16_t* __restrict__ d, const BrickGrid& g, float vx, float vy,
+                                                float vz) {
+    const float fx = floorf(vx), fy = floorf(vy), fz = floorf(vz);
+    const float ax = vx - fx, ay = vy - fy, az = vz - fz;
+    const int x0 = int(fx), y0 = int(fy), z0 = int(fz);
+    auto t = [&](int x, int y, int z) { return dtexel(d, g, x, y, z); };
+    const float c00 = t(x0, y0, z0) * (1.0f - ax) + t(x0 + 1, y0, z0) * ax;
+    const float c10 = t(x0, y0 + 1, z0) * (1.0f - ax) + t(x0 + 1, y0 + 1, z0) * ax;
+    const float c01 = t(x0, y0, z0 + 1) * (1.0f - ax) + t(x0 + 1, y0, z0 + 1) * ax;
+    const float c11 = t(x0, y0 + 1, z0 + 1) * (1.0f - ax) + t(x0 + 1, y0 + 1, z0 + 1) * ax;
+    const float c0 = c00 * (1.0f - ay) + c10 * ay;
+    const float c1 = c01 * (1.0f - ay) + c11 * ay;
+    return c0 * (1.0f - az) + c1 * az;
+}
+
+// mask.comp:107-138 (binary_masks), with soft_coord / soft_masks (mask.comp:40-100).
+__global__ void k_mask_binary(const uint16_t* __restrict__ d, const uint8_t* __restrict__ labels, SoftGrid soft,
+                              int margin, uint32_t* __restrict__ out, BrickGrid g, const vvr_uniforms* __restrict__ u) {
+    const uint32_t o = blockIdx.x * kBlock + threadIdx.x;
+    if (o >= g.padded()) return;
+    int x, y, z;
+    brick_coords(g, o, x, y, z);
+    if (x >= g.nx || y >= g.ny || z >= g.nz) {
+        out[o] = 0u;
+        return;
+    }
+    const uint32_t mode = uint32_t(u->labels[0]);
+    const uint32_t label = labels[o];
+    float m0 = solid(u, label, mode) ? 1.0f : 0.0f;
+    float m1 = ghosted(u, label) ? 1.0f : 0.0f;
+    if ((soft.push[3] & 8) != 0 && label < 250u) {
+        const int p[3] = {x, y, z};
+        const int axis[3] = {soft.push[0], soft.push[1], soft.push[2]};
+        const int dims[3] = {g.nx, g.ny, g.nz};
+        float can[3], xs[3], to_grid[3];
+        for (int a = 0; a < 3; a++) {
+            can[a] = float(dims[axis[a]]);
+            const float v = float(p[axis[a]]);
+            xs[a] = ((soft.push[3] >> a) & 1) != 0 ? can[a] - 1.0f - v : v;
+            const float den = fmaxf(can[a] - 1.0f, 1.0f);
+            to_grid[a] = (float(soft.dims[a]) - 1.0f) / den;
+            xs[a] = xs[a] * (float(soft.dims[a]) - 1.0f) / den;
+        }
+        auto node = [&](int a, int b, int c) {
+            return soft.nodes[size_t(a) + size_t(soft.dims[0]) * (size_t(b) + size_t(soft.dims[1]) * c)];
+        };
+        int cx = int(roundf(xs[0])), cy = int(roundf(xs[1])), cz = int(roundf(xs[2]));
+        const bool overridden =
+            (node(min(max(cx, 0), soft.dims[0] - 1), min(max(cy, 0), soft.dims[1] - 1), min(max(cz, 0), soft.dims[2] - 1)) >>
+             24) != 0u;
+        if (!overridden) {
+            const float here = dtexel(d, g, x, y, z);
+            const float kSigmaCt = 40.0f / 4095.0f;
+            float s0 = 0.0f, s1 = 0.0f, weights = 0.0f;
+            for (int dz = -1; dz <= 1; dz++)
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dx = -1; dx <= 1; dx++) {
+                        const int n[3] = {min(max(cx + dx, 0), soft.dims[0] - 1), min(max(cy + dy, 0), soft.dims[1] - 1),
+                                          min(max(cz + dz, 0), soft.dims[2] - 1)};
+                        float dd[3], v[3];
+                        for (int a = 0; a < 3; a++) dd[a] = float(n[a]) - xs[a];
+                        const float w_space = expf(-(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]) / (2.0f * 0.6f * 0.6f));
+                        for (int a = 0; a < 3; a++) {
+                            const float c = float(n[a]) / to_grid[a];
+                            v[axis[a]] = ((soft.push[3] >> a) & 1) != 0 ? can[a] - 1.0f - c : c;
+                        }
+                        const float there = density_linear(d, g, v[0], v[1], v[2]);
+                        const float w_ct = expf(-(here - there) * (here - there) / (2.0f * kSigmaCt * kSigmaCt));
+                        const uint32_t t = node(n[0], n[1], n[2]);
+                        const uint32_t best = t & 0xffu, second = (t >> 8) & 0xffu;
+                        const float mg = float((t >> 16) & 0xffu) / 16.0f;
+                        const float sc0 = soft_score(solid(u, best, mode), solid(u, second, mode), mg);
+                        const float sc1 = soft_score(ghosted(u, best), ghosted(u, second), mg);
+                        const float w = w_space * w_ct + 1e-6f * w_space;
+                        s0 += w * sc0;
+                        s1 += w * sc1;
+                        weights += w;
+                    }
+            m0 = 0.5f + (s0 / weights) / (2.0f * 4.0f);
+            m1 = 0.5f + (s1 / weights) / (2.0f * 4.0f);
+            m0 = m0 < 0.0f ? 0.0f : (m0 > 1.0f ? 1.0f : m0);
+            m1 = m1 < 0.0f ? 0.0f : (m1 > 1.0f ? 1.0f : m1);
+        }
+    }
+    if (mode == 1u && margin > 0 && label == 0u) {
+        for (int a = -margin; a <= margin && m0 > 0.0f; a++)
+            for (int b = -margin; b <= margin && m0 > 0.0f; b++)
+                for (int c = -margin; c <= margin; c++) {
+                    const int qx = min(max(x + a, 0), g.nx - 1), qy = min(max(y + b, 0), g.ny - 1),
+                              qz = min(max(z + c, 0), g.nz - 1);
+                    const uint32_t l = labels[g.at(qx, qy, qz)];
+                    if (l != 0u && (!shown(u, l) || ghosted(u, l))) {
+                        m0 = 0.0f;
+                        break;
+                    }
+                }
+    }
+    out[o] = unorm16_bits(m0) | (unorm16_bits(m1) << 16);
+}
+
+// mask.comp:152-170: binomial weights C(2r, r + k) / 4^r along one axis.
+__global__ void k_mask_blur(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, int axis, int r, BrickGrid g) {
+    const uint32_t o = blockIdx.x * kBlock + threadIdx.x;
+    if (o >= g.padded()) return;
+    int x, y, z;
+    brick_coords(g, o, x, y, z);
+    if (x >= g.nx || y >= g.ny || z >= g.nz) {
+        dst[o] = 0u;
+        return;
+    }
+    float s0 = 0.0f, s1 = 0.0f, c = 1.0f;
+    for (int j = 0; j <= 2 * r; j++) {
+        int q[3] = {x, y, z};
+        const int n = axis == 0 ? g.nx : (axis == 1 ? g.ny : g.nz);
+        q[axis] = min(max(q[axis] + j - r, 0), n - 1);
+        const uint32_t v = src[g.at(q[0], q[1], q[2])];
+        s0 += c * (float(v & 0xffffu) / 65535.0f);
+        s1 += c * (float(v >> 16) / 65535.0f);
+        c = c * float(2 * r - j) / float(j + 1);
+    }
+    const float e = exp2f(float(2 * r));
+    dst[o] = unorm16_bits(s0 / e) | (unorm16_bits(s1 / e) << 16);
+}
+
+// majorant.comp:26-71. A workgroup covers 4 x 4 x 4 bricks. Every brick's window
+// [8b - 2, 8b + 9] (clamped to the volume) is a union of 6^3 aligned 2^3-voxel
+// cells, so the group first reduces the 18^3 cells of its bricks and their
+// halo into LDS, then each brick reduces its 216 cells. Densities stay integer:
+// int(d * 64) of d = c / 65535 equals floor(64 c / 65535) for every c (no
+// quotient lies within f32 rounding of a bucket edge), and mask max > 0 iff the
+// largest raw mask value is > 0.
+constexpr int kTile = 4;                 // bricks per axis per workgroup
+constexpr int kCells = 4 * kTile + 2;  // 18 cells per axis: 4 per brick and 1 of halo each side
+
+__global__ void __launch_bounds__(kBlock) k_majorant(const uint16_t* __restrict__ d, const uint32_t* __restrict__ mask,
+                                                     float* __restrict__ out, BrickGrid g, int mode, int mx, int my,
+                                                     int mz, const vvr_uniforms* __restrict__ u) {
+    __shared__ uint32_t s_minmax[kCells * kCells * kCells];  // min | max << 16
+    __shared__ uint16_t s_mask[kCells * kCells * kCells];
+    const int tbx = blockIdx.x * kTile, tby = blockIdx.y * kTile, tbz = blockIdx.z * kTile;
+    // Cell c covers voxels 2c, 2c + 1; the tile's first cell is 4 * tb - 1.
+    const int c0x = 4 * tbx - 1, c0y = 4 * tby - 1, c0z = 4 * tbz - 1;
+    for (int c = threadIdx.x; c < kCells * kCells * kCells; c += kBlock) {
+        const int cx = c % kCells, cy = (c / kCells) % kCells, cz = c / (kCells * kCells);
+        const int vx = 2 * (c0x + cx), vy = 2 * (c0y + cy), vz = 2 * (c0z + cz);
+        uint32_t lo = 0xffffu, hi = 0u, mm = 0u;
+        for (int k = 0; k < 2; k++) {
+            const int z = vz + k;
+            if (z < 0 || z >= g.nz) continue;
+            for (int j = 0; j < 2; j++) {
+                const int y = vy + j;
+                if (y < 0 || y >= g.ny) continue;
+                for (int i = 0; i < 2; i++) {
+                    const int x = vx + i;
+                    if (x < 0 || x >= g.nx) continue;
+                    const uint32_t a = g.at(x, y, z);
+                    const uint32_t v = d[a];
+                    lo = min(lo, v);
+                    hi = max(hi, v);
+                    if (mode != 0) {
+                        const uint32_t m = mask[a];
+                        mm = max(mm, max(m & 0xffffu, m >> 16));
+                    }
+                }
+            }
+        }
+        s_minmax[c] = lo | (hi << 16);
+        s_mask[c] = uint16_t(mm);
+    }
+    __syncthreads();
+    if (threadIdx.x >= kTile * kTile * kTile) return;
+    const int lx = threadIdx.x % kTile, ly = (threadIdx.x / kTile) % kTile, lz = threadIdx.x / (kTile * kTile);
+    const int bx = tbx + lx, by = tby + ly, bz = tbz + lz;
+    if (bx >= mx || by >= my || bz >= mz) return;
+    uint32_t lo = 0xffffu, hi = 0u, mm = 0u;
+    for (int k = 0; k < 6; k++)
+        for (int j = 0; j < 6; j++)
+            for (int i = 0; i < 6; i++) {
+                const int c = (4 * lx + i) + kCells * ((4 * ly + j) + kCells * (4 * lz + k));
+                const uint32_t v = s_minmax[c];
+                lo = min(lo, v & 0xffffu);
+                hi = max(hi, v >> 16);
+                mm = max(mm, uint32_t(s_mask[c]));
+            }
+    float bound;
+    auto tf_max = [&](uint32_t clo, uint32_t chi) {
+        const int first = int((64u * clo) / 65535u), last = int((64u * chi) / 65535u);
+        float m = 0.0f;
+        for (int b = min(first, 63); b <= min(last, 63); b++) m = fmaxf(m, u->tf_bucket_max[b / 4][b % 4]);
+        return m;
+    };
+    if (mode == 0) bound = tf_max(lo, hi);
+    else if (mm == 0u) bound = 0.0f;
+    else if (mode == 1) bound = tf_max(lo, hi);
+    else bound = u->labels[2];
+    out[size_t(bx) + size_t(mx) * (size_t(by) + size_t(my) * bz)] = bound;
+}
+
+inline unsigned blocks_for(size_t n) { return unsigned((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+void launch_upload_density(const int16_t* hu, uint16_t* out, const BrickGrid& g, hipStream_t st) {
+    hipLaunchKernelGGL(k_upload_density, dim3(blocks_for(g.padded())), dim3(kBlock), 0, st, hu, out, g);
+}
+
+void launch_upload_labels(const uint8_t* labels, uint8_t* out, const BrickGrid& g, hipStream_t st) {
+    hipLaunchKernelGGL(k_upload_labels, dim3(blocks_for(g.padded())), dim3(kBlock), 0, st, labels, out, g);
+}
+
+void launch_debrick(const void* in, void* out, const BrickGrid& g, int elem_bytes, hipStream_t st) {
+    const size_t n = size_t(g.nx) * g.ny * g.nz;
+    const dim3 grid(blocks_for(n));
+    if (elem_bytes == 1)
+        hipLaunchKernelGGL(k_debrick<uint8_t>, grid, dim3(kBlock), 0, st, (const uint8_t*)in, (uint8_t*)out, g);
+    else if (elem_bytes == 2)
+        hipLaunchKernelGGL(k_debrick<uint16_t>, grid, dim3(kBlock), 0, st, (const uint16_t*)in, (uint16_t*)out, g);
+    else
+        hipLaunchKernelGGL(k_debrick<uint32_t>, grid, dim3(kBlock), 0, st, (const uint32_t*)in, (uint32_t*)out, g);
+}
+
+void launch_normals(const uint16_t* density, uint32_t* normals, const BrickGrid& g, const vvr_uniforms* u,
+                    hipStream_t st) {
+    hipLaunchKernelGGL(k_normals, dim3(blocks_for(g.padded())), dim3(kBlock), 0, st, density, normals, g, u);
+}
+
+void launch_mask_binary(const uint16_t* density, const uint8_t* labels, SoftGrid soft, int margin, uint32_t* dst,
+                        const BrickGrid& g, const vvr_uniforms* u, hipStream_t st) {
+    hipLaunchKernelGGL(k_mask_binary, dim3(blocks_for(g.padded())), dim3(kBlock), 0, st, density, labels, soft, margin,
+                       dst, g, u);
+}
+
+void launch_mask_blur(const uint32_t* src, uint32_t* dst, int axis, int radius, const BrickGrid& g, hipStream_t st) {
+    hipLaunchKernelGGL(k_mask_blur, dim3(blocks_for(g.padded())), dim3(kBlock), 0, st, src, dst, axis, radius, g);
+}
+
+void launch_majorant(const uint16_t* density, const uint32_t* mask, float* out, const BrickGrid& g, int mode,
+                     const vvr_uniforms* u, hipStream_t st) {
+    const int mx = (g.nx + 7) / 8, my = (g.ny + 7) / 8, mz = (g.nz + 7) / 8;
+    const dim3 grid((mx + kTile - 1) / kTile, (my + kTile - 1) / kTile, (mz + kTile - 1) / kTile);
+    hipLaunchKernelGGL(k_majorant, grid, dim3(kBlock), 0, st, density, mask, out, g, mode, mx, my, mz, u);
+}
+
+}  // namespace vvr

@@ -1,0 +1,43 @@
+// Host-side launchers of the HIP kernels (kernels_volume.hip, kernels_render.hip).
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include "device/types.hpp"
+
+namespace vvr {
+
+// HU (linear, int16) -> unorm16 micro-bricks (transfer.rs:121-124, renderer.rs:577).
+void launch_upload_density(const int16_t* hu, uint16_t* out, const BrickGrid& g, hipStream_t st);
+// uint8 labels (linear) -> micro-bricks.
+void launch_upload_labels(const uint8_t* labels, uint8_t* out, const BrickGrid& g, hipStream_t st);
+// Micro-bricks -> linear, element size 1, 2 or 4 bytes.
+void launch_debrick(const void* in, void* out, const BrickGrid& g, int elem_bytes, hipStream_t st);
+
+// normals.comp
+void launch_normals(const uint16_t* density, uint32_t* normals, const BrickGrid& g, const vvr_uniforms* u,
+                    hipStream_t st);
+
+// mask.comp, pass 0 (binary / soft masks) into `dst`.
+struct SoftGrid {
+    const uint32_t* nodes;  // RGBA8 per node (best, runner-up, margin x 16, overridden), grid axis 0 fastest
+    int dims[3];
+    int push[4];            // source axes, flip bits | 8 when present
+};
+void launch_mask_binary(const uint16_t* density, const uint8_t* labels, SoftGrid soft, int margin, uint32_t* dst,
+                        const BrickGrid& g, const vvr_uniforms* u, hipStream_t st);
+// mask.comp, passes 1-3: binomial blur of radius r along `axis`.
+void launch_mask_blur(const uint32_t* src, uint32_t* dst, int axis, int radius, const BrickGrid& g, hipStream_t st);
+
+// majorant.comp: `mask` may be null when mode is 0.
+void launch_majorant(const uint16_t* density, const uint32_t* mask, float* out, const BrickGrid& g, int mode,
+                     const vvr_uniforms* u, hipStream_t st);
+
+// dvr.comp: hdr = RGBA16F (width x height x 4 halves).
+void launch_dvr(const DevScene& s, uint16_t* hdr, uint32_t width, uint32_t height, hipStream_t st);
+
+// pt.comp for `samples` paths per pixel at path indices first_path ..: adds to
+// accum/albedo/normal (reset when first_path == 0) and writes hdr.
+void launch_pt(const DevScene& s, float4* accum, float4* albedo, float4* normal, uint16_t* hdr, uint32_t width,
+               uint32_t height, uint32_t first_path, uint32_t samples, hipStream_t st);
+
+}  // namespace vvr
