@@ -148,7 +148,9 @@ __device__ inline float extinction_at(const DevScene& sc, V3 idx, V3 dir, bool s
                                       float& cut) {
     cut = clip_depth(sc.u, idx, cap_plane);
     if (cut > 0.0f) return 0.0f;
-    s = apply_labels(sc, tf_point(sc, smooth_density ? density_view(sc, idx) : density_at_index(sc, idx)), idx);
+    const float d =
+        density_unused(sc.u) ? 0.0f : (smooth_density ? density_view(sc, idx) : density_at_index(sc, idx));
+    s = apply_labels(sc, tf_point(sc, d), idx);
     if (s.extinction <= 0.0f) return 0.0f;
     if (s.ghost) {
         V3 n;
@@ -227,141 +229,199 @@ __device__ inline V3 transmittance(const DevScene& sc, V3 origin, V3 dir, Rng& r
     return result;
 }
 
-// What one path adds to the accumulation and guide images (pt.comp:403-552).
-struct PathResult {
-    V3 radiance;     // after the firefly clamp
-    float background;
-    V3 albedo;       // first-event albedo (1 for background)
-    V3 normal;       // first-event normal facing the viewer
-    float depth;     // index-space distance to the first event (1e4 for background)
+// Per-frame constants of the path tracer (pt.comp:381-397).
+struct FrameConsts {
+    uint32_t max_bounces, volume_limit;
+    float step_mm;
+    V3 ldir;
+    M3 light_frame;
+    V3 light;
+    float cos_light, light_pdf;
+    bool area_light;
+    M3 ti;
+    V3 eye_idx;
 };
 
-__device__ inline PathResult trace_path(const DevScene& sc, uint32_t px, uint32_t py, Rng& rng) {
+__device__ __forceinline__ FrameConsts frame_consts(const vvr_uniforms* u) {
+    FrameConsts c;
+    c.max_bounces = u->path[1];
+    c.volume_limit = uint32_t(float(kMaxVolumeEvents) / (1.0f - clampf(u->env[2], 0.0f, 0.95f)));
+    c.step_mm = u->dims[3];
+    c.ldir = V3(u->light_dir[0], u->light_dir[1], u->light_dir[2]);
+    c.light_frame = basis(c.ldir);
+    c.light = V3(u->light_dir[3]);
+    c.cos_light = cosf(u->material[3]);
+    c.area_light = u->material[3] > 1e-4f;
+    c.light_pdf = c.area_light ? 1.0f / (2.0f * kPi * (1.0f - c.cos_light)) : 0.0f;
+    c.ti = to_index(u);
+    c.eye_idx = world_to_index_point(u, V3(u->eye[0], u->eye[1], u->eye[2]));
+    return c;
+}
+
+// One path in flight: the locals of pt.comp's sample loop (pt.comp:404-545),
+// resumable between scattering events so a persistent lane can interleave them.
+struct PathState {
+    V3 origin, dir, throughput, spectral, radiance;
+    float last_pdf;
+    uint32_t bounce, surfaces, volumes;
+    int hero;
+    Rng rng;
+    // What the path adds to the guide images (first event or background).
+    float background;
+    V3 albedo, normal;
+    float depth;
+};
+
+// pt.comp:404-423: camera ray (thin lens), hero channel.
+__device__ inline void path_begin(const DevScene& sc, const FrameConsts& fc, uint32_t px, uint32_t py,
+                                  uint32_t path_index, PathState& st) {
     const vvr_uniforms* u = sc.u;
-    const uint32_t max_bounces = u->path[1];
-    const uint32_t volume_limit = uint32_t(float(kMaxVolumeEvents) / (1.0f - clampf(u->env[2], 0.0f, 0.95f)));
-    const float step_mm = u->dims[3];
-    const V3 ldir(u->light_dir[0], u->light_dir[1], u->light_dir[2]);
-    const M3 light_frame = basis(ldir);
-    const V3 light(u->light_dir[3]);
-    const float cos_light = cosf(u->material[3]);
-    const bool area_light = u->material[3] > 1e-4f;
-    const float light_pdf = area_light ? 1.0f / (2.0f * kPi * (1.0f - cos_light)) : 0.0f;
-    const M3 ti = to_index(u);
-    const V3 eye_idx = world_to_index_point(u, V3(u->eye[0], u->eye[1], u->eye[2]));
-
-    PathResult res;
-    res.radiance = V3(0.0f);
-    res.background = 0.0f;
-    res.albedo = V3(0.0f);
-    res.normal = V3(0.0f);
-    res.depth = 0.0f;
-
-    const float jx = rng.rand();
-    const float jy = rng.rand();
+    st.rng = seed_rng(px, py, path_index);
+    const float jx = st.rng.rand();
+    const float jy = st.rng.rand();
     V3 dir = camera_ray(u, float(px) + jx, float(py) + jy);
-    V3 origin = eye_idx;
+    V3 origin = fc.eye_idx;
     if (u->lens[0] > 0.0f) {  // thin lens (pt.comp:406-416)
         V3 eye(u->eye[0], u->eye[1], u->eye[2]);
         const V3 fwd = normalize(V3(u->forward[0], u->forward[1], u->forward[2]));
         const V3 focal = eye + dir * (u->lens[1] / dot(dir, fwd));
-        const float r = u->lens[0] * sqrtf(rng.rand());
-        const float phi = 2.0f * kPi * rng.rand();
+        const float r = u->lens[0] * sqrtf(st.rng.rand());
+        const float phi = 2.0f * kPi * st.rng.rand();
         const V3 rr = normalize(V3(u->right[0], u->right[1], u->right[2]));
         const V3 uu = normalize(V3(u->up[0], u->up[1], u->up[2]));
         eye = eye + r * (cosf(phi) * rr + sinf(phi) * uu);
         dir = normalize(focal - eye);
         origin = world_to_index_point(u, eye);
     }
-    V3 throughput(1.0f);
-    V3 radiance(0.0f);
-    const int hero = min(int(rng.rand() * 3.0f), 2);
-    V3 spectral(1.0f);
-    float last_pdf = 0.0f;
-    for (uint32_t bounce = 0, surfaces = 0, volumes = 0; surfaces <= max_bounces && volumes <= volume_limit;
-         bounce++) {
-        V3 idx;
-        Sample s;
-        int cap_plane = -1;
-        float cut = 0.0f;
-        if (!free_flight(sc, origin, dir, bounce == 0u, hero, spectral, idx, s, cap_plane, cut, rng)) {
-            if (bounce == 0u) {
-                res.background = 1.0f;
-                res.albedo = V3(1.0f);
-                res.depth = 1e4f;
-            } else {
-                const V3 path_weight = throughput * spectral / dot(spectral, V3(1.0f / 3.0f));
-                radiance = radiance + path_weight * environment(u, dir);
-                if (area_light && dot(dir, ldir) >= cos_light)
-                    radiance = radiance + path_weight * light * light_pdf * mis(last_pdf, light_pdf);
-            }
-            break;
-        }
-        const V3 path_weight = throughput * spectral / dot(spectral, V3(1.0f / 3.0f));
-        const V3 v = -dir;
-        V3 n;
-        float surfaceness;
-        surface_frame(sc, idx, s, v, cap_plane, cut, 1.5f * step_mm, n, surfaceness);
-        apply_surface_texture(u, idx, s);
-        const float xi_surface = rng.rand();
-        const bool surface_event = xi_surface < surfaceness * (1.0f - s.subsurface) && dot(n, n) > 0.5f;
-        if (surface_event || bounce == 0u) n = bump_normal(u, idx, s, n);
-        if (bounce == 0u) {
-            res.albedo = s.albedo;
-            res.normal = dot(n, v) < 0.0f ? -n : n;
-            res.depth = length(idx - eye_idx);
-        }
-        const V3 light_dir = sample_light(u, light_frame, rng);
-        const V3 light_t = transmittance(sc, idx, light_dir, rng);
-        if (surface_event) {
-            n = dot(n, v) < 0.0f ? -n : n;
-            V3 diffuse;
-            V3 specular = brdf_coated(n, v, light_dir, s, diffuse);
-            const float light_weight = area_light ? mis(light_pdf, bsdf_pdf(n, v, light_dir, s)) : 1.0f;
-            radiance = radiance +
-                       path_weight * (diffuse + specular) * gmax(dot(n, light_dir), 0.0f) * light * light_t * light_weight;
-            const M3 frame = basis(n);
-            V3 l;
-            if (rng.rand() < coat_share(s)) {
-                l = sample_ggx(frame, v, coat_roughness(s) * coat_roughness(s), rng);
-            } else if (rng.rand() < specular_share(s)) {
-                l = sample_ggx(frame, v, gmax(s.roughness * s.roughness, 2e-3f), rng);
-            } else {
-                const float r = sqrtf(rng.rand());
-                const float phi = 2.0f * kPi * rng.rand();
-                l = frame.mul(V3(r * cosf(phi), r * sinf(phi), sqrtf(gmax(0.0f, 1.0f - r * r))));
-            }
-            const float nl = dot(n, l);
-            if (nl <= 0.0f) break;
-            const float pdf = bsdf_pdf(n, v, l, s);
-            specular = brdf_coated(n, v, l, s, diffuse);
-            throughput = throughput * (((diffuse + specular) * nl) / gmax(pdf, 1e-6f));
-            last_pdf = pdf;
-            surfaces++;
-            origin = idx + ti.mul(n * step_mm);
-            dir = l;
-        } else {
-            const V3 albedo = event_albedo(u, s);
-            const float light_phase = phase_hg(u, dot(dir, light_dir));
-            radiance = radiance + path_weight * albedo * light_phase * light * light_t *
-                                      (area_light ? mis(light_pdf, light_phase) : 1.0f);
-            throughput = throughput * albedo;
-            volumes++;
-            origin = idx;
-            const V3 next = sample_hg(u, dir, rng);
-            last_pdf = phase_hg(u, dot(dir, next));
-            dir = next;
-        }
-        if (bounce >= 2u) {
-            const V3 w = throughput * spectral / dot(spectral, V3(1.0f / 3.0f));
-            const float survive = clampf(gmax(w.x, gmax(w.y, w.z)), 0.05f, 1.0f);
-            if (rng.rand() > survive) break;
-            throughput = throughput / survive;
-        }
-    }
-    const float peak = gmax(radiance.x, gmax(radiance.y, radiance.z));
-    res.radiance = peak > 20.0f ? radiance * (20.0f / peak) : radiance;
-    return res;
+    st.origin = origin;
+    st.dir = dir;
+    st.throughput = V3(1.0f);
+    st.radiance = V3(0.0f);
+    st.hero = min(int(st.rng.rand() * 3.0f), 2);
+    st.spectral = V3(1.0f);
+    st.last_pdf = 0.0f;
+    st.bounce = st.surfaces = st.volumes = 0u;
+    st.background = 0.0f;
+    st.albedo = V3(0.0f);
+    st.normal = V3(0.0f);
+    st.depth = 0.0f;
 }
+
+// One iteration of the bounce loop (pt.comp:427-545). Returns false once the
+// path has ended (then the firefly clamp of path_end applies).
+__device__ __forceinline__ bool path_bounce(const DevScene& sc, const FrameConsts& fc, PathState& st) {
+    const vvr_uniforms* u = sc.u;
+    if (!(st.surfaces <= fc.max_bounces && st.volumes <= fc.volume_limit)) return false;
+    V3 idx;
+    Sample s;
+    int cap_plane = -1;
+    float cut = 0.0f;
+    if (!free_flight(sc, st.origin, st.dir, st.bounce == 0u, st.hero, st.spectral, idx, s, cap_plane, cut, st.rng)) {
+        if (st.bounce == 0u) {
+            st.background = 1.0f;
+            st.albedo = V3(1.0f);
+            st.depth = 1e4f;
+        } else {
+            const V3 path_weight = st.throughput * st.spectral / dot(st.spectral, V3(1.0f / 3.0f));
+            st.radiance = st.radiance + path_weight * environment(u, st.dir);
+            if (fc.area_light && dot(st.dir, fc.ldir) >= fc.cos_light)
+                st.radiance = st.radiance + path_weight * fc.light * fc.light_pdf * mis(st.last_pdf, fc.light_pdf);
+        }
+        return false;
+    }
+    const V3 path_weight = st.throughput * st.spectral / dot(st.spectral, V3(1.0f / 3.0f));
+    const V3 v = -st.dir;
+    V3 n;
+    float surfaceness;
+    surface_frame(sc, idx, s, v, cap_plane, cut, 1.5f * fc.step_mm, n, surfaceness);
+    apply_surface_texture(u, idx, s);
+    const float xi_surface = st.rng.rand();
+    const bool surface_event = xi_surface < surfaceness * (1.0f - s.subsurface) && dot(n, n) > 0.5f;
+    if (surface_event || st.bounce == 0u) n = bump_normal(u, idx, s, n);
+    if (st.bounce == 0u) {
+        st.albedo = s.albedo;
+        st.normal = dot(n, v) < 0.0f ? -n : n;
+        st.depth = length(idx - fc.eye_idx);
+    }
+    const V3 light_dir = sample_light(u, fc.light_frame, st.rng);
+    const V3 light_t = transmittance(sc, idx, light_dir, st.rng);
+    if (surface_event) {
+        n = dot(n, v) < 0.0f ? -n : n;
+        V3 diffuse;
+        V3 specular = brdf_coated(n, v, light_dir, s, diffuse);
+        const float light_weight = fc.area_light ? mis(fc.light_pdf, bsdf_pdf(n, v, light_dir, s)) : 1.0f;
+        st.radiance = st.radiance + path_weight * (diffuse + specular) * gmax(dot(n, light_dir), 0.0f) * fc.light *
+                                        light_t * light_weight;
+        const M3 frame = basis(n);
+        V3 l;
+        if (st.rng.rand() < coat_share(s)) {
+            l = sample_ggx(frame, v, coat_roughness(s) * coat_roughness(s), st.rng);
+        } else if (st.rng.rand() < specular_share(s)) {
+            l = sample_ggx(frame, v, gmax(s.roughness * s.roughness, 2e-3f), st.rng);
+        } else {
+            const float r = sqrtf(st.rng.rand());
+            const float phi = 2.0f * kPi * st.rng.rand();
+            l = frame.mul(V3(r * cosf(phi), r * sinf(phi), sqrtf(gmax(0.0f, 1.0f - r * r))));
+        }
+        const float nl = dot(n, l);
+        if (nl <= 0.0f) return false;
+        const float pdf = bsdf_pdf(n, v, l, s);
+        specular = brdf_coated(n, v, l, s, diffuse);
+        st.throughput = st.throughput * (((diffuse + specular) * nl) / gmax(pdf, 1e-6f));
+        st.last_pdf = pdf;
+        st.surfaces++;
+        st.origin = idx + fc.ti.mul(n * fc.step_mm);
+        st.dir = l;
+    } else {
+        const V3 albedo = event_albedo(u, s);
+        const float light_phase = phase_hg(u, dot(st.dir, light_dir));
+        st.radiance = st.radiance + path_weight * albedo * light_phase * fc.light * light_t *
+                                        (fc.area_light ? mis(fc.light_pdf, light_phase) : 1.0f);
+        st.throughput = st.throughput * albedo;
+        st.volumes++;
+        st.origin = idx;
+        const V3 next = sample_hg(u, st.dir, st.rng);
+        st.last_pdf = phase_hg(u, dot(st.dir, next));
+        st.dir = next;
+    }
+    if (st.bounce >= 2u) {  // Russian roulette (pt.comp:536-544)
+        const V3 w = st.throughput * st.spectral / dot(st.spectral, V3(1.0f / 3.0f));
+        const float survive = clampf(gmax(w.x, gmax(w.y, w.z)), 0.05f, 1.0f);
+        if (st.rng.rand() > survive) return false;
+        st.throughput = st.throughput / survive;
+    }
+    st.bounce++;
+    return true;
+}
+
+// Firefly clamp (pt.comp:546-548): the radiance the path adds.
+__device__ __forceinline__ V3 path_end(const PathState& st) {
+    const float peak = gmax(st.radiance.x, gmax(st.radiance.y, st.radiance.z));
+    return peak > 20.0f ? st.radiance * (20.0f / peak) : st.radiance;
+}
+
+// What the paths of one pixel add to the accumulation and guide images (pt.comp:399-552).
+struct PixelSums {
+    V3 radiance;
+    float background;
+    V3 albedo;
+    float lum2;  // sum of squared luminances (albedo guide alpha)
+    V3 normal;
+    float depth;
+    __device__ __forceinline__ void clear() {
+        radiance = albedo = normal = V3(0.0f);
+        background = lum2 = depth = 0.0f;
+    }
+    __device__ __forceinline__ void add(const PathState& st) {
+        const V3 r = path_end(st);
+        radiance = radiance + r;
+        background += st.background;
+        albedo = albedo + st.albedo;
+        normal = normal + st.normal;
+        depth += st.depth;
+        const float lum = dot(r, V3(0.2126f, 0.7152f, 0.0722f));
+        lum2 += lum * lum;
+    }
+};
 
 }  // namespace vvr

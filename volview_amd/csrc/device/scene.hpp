@@ -153,96 +153,115 @@ __device__ __forceinline__ Tap tap_of(float h, float i) {
     return t;
 }
 
-// A 4x4x4 window of raw density values around floor(idx).
-struct Window {
-    float v[4][4][4];  // [z][y][x]
-};
-
-__device__ __forceinline__ void load_window(const DevScene& s, int ix, int iy, int iz, Window& w) {
-    const BrickGrid& g = s.g;
-    uint32_t X[4], Y[4], Z[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        X[k] = g.xoff(iclamp(ix - 1 + k, 0, g.nx - 1));
-        Y[k] = g.yoff(iclamp(iy - 1 + k, 0, g.ny - 1));
-        Z[k] = g.zoff(iclamp(iz - 1 + k, 0, g.nz - 1));
-    }
-#pragma unroll
-    for (int z = 0; z < 4; z++)
-#pragma unroll
-        for (int y = 0; y < 4; y++)
-#pragma unroll
-            for (int x = 0; x < 4; x++) w.v[z][y][x] = float(s.density[X[x] + Y[y] + Z[z]]);
-}
-
-// Linear interpolation of 4 window values at tap t (upper index clamped to the
-// window: it only leaves it with weight 0).
+// Linear interpolation of 4 window values at tap t (the upper index only leaves
+// the window with weight 0, so it is clamped to it).
 __device__ __forceinline__ float lerp4(float v0, float v1, float v2, float v3, Tap t) {
     const float lo = t.o <= 0 ? v0 : (t.o == 1 ? v1 : (t.o == 2 ? v2 : v3));
     const float hi = t.o <= 0 ? v1 : (t.o == 1 ? v2 : v3);
     return mixf(lo, hi, t.a);
 }
 
-// The 8 trilinear fetches of cubic_fetch (common.glsl:116-122) on the window, in
-// raw units, sharing work between fetches: every fetch lerps along x, then y,
-// then z, exactly as one texture fetch would, but rows lerped along x at the
-// same tap are computed once.
-struct XRows {
-    float r[4][4];  // [z][y]
+// Addresses of the 4 clamped voxel rows of the window floor(idx) - 1 .. + 2.
+struct WindowAddr {
+    uint32_t x[4], y[4], z[4];
 };
-__device__ __forceinline__ void x_pass(const Window& w, Tap t, XRows& o) {
+__device__ __forceinline__ WindowAddr window_addr(const BrickGrid& g, int ix, int iy, int iz) {
+    WindowAddr w;
 #pragma unroll
-    for (int z = 0; z < 4; z++)
-#pragma unroll
-        for (int y = 0; y < 4; y++) o.r[z][y] = lerp4(w.v[z][y][0], w.v[z][y][1], w.v[z][y][2], w.v[z][y][3], t);
-}
-// y then z lerps of x-lerped rows: one trilinear fetch.
-__device__ __forceinline__ float yz_pass(const XRows& a, Tap ty, Tap tz) {
-    float zz[4];
-#pragma unroll
-    for (int z = 0; z < 4; z++) zz[z] = lerp4(a.r[z][0], a.r[z][1], a.r[z][2], a.r[z][3], ty);
-    return lerp4(zz[0], zz[1], zz[2], zz[3], tz);
-}
-__device__ __forceinline__ float cubic_fetch_rows(const XRows& a0, const XRows& a1, Tap y0, Tap y1, Tap z0, Tap z1,
-                                                  float gx0, float gx1, float gy0, float gy1, float gz0, float gz1) {
-    const float y0z0 = gx0 * yz_pass(a0, y0, z0) + gx1 * yz_pass(a1, y0, z0);
-    const float y1z0 = gx0 * yz_pass(a0, y1, z0) + gx1 * yz_pass(a1, y1, z0);
-    const float y0z1 = gx0 * yz_pass(a0, y0, z1) + gx1 * yz_pass(a1, y0, z1);
-    const float y1z1 = gx0 * yz_pass(a0, y1, z1) + gx1 * yz_pass(a1, y1, z1);
-    return gz0 * (gy0 * y0z0 + gy1 * y1z0) + gz1 * (gy0 * y0z1 + gy1 * y1z1);
+    for (int k = 0; k < 4; k++) {
+        w.x[k] = g.xoff(iclamp(ix - 1 + k, 0, g.nx - 1));
+        w.y[k] = g.yoff(iclamp(iy - 1 + k, 0, g.ny - 1));
+        w.z[k] = g.zoff(iclamp(iz - 1 + k, 0, g.nz - 1));
+    }
+    return w;
 }
 
-// density_cubic (common.glsl:124-127).
-__device__ __forceinline__ float density_cubic(const DevScene& s, V3 idx) {
-    const float ix = floorf(idx.x), iy = floorf(idx.y), iz = floorf(idx.z);
-    const Taps1 tx = cubic_taps1(idx.x, ix), ty = cubic_taps1(idx.y, iy), tz = cubic_taps1(idx.z, iz);
-    Window w;
-    load_window(s, fidx(ix), fidx(iy), fidx(iz), w);
-    XRows a0, a1;
-    x_pass(w, tap_of(tx.h0, ix), a0);
-    x_pass(w, tap_of(tx.h1, ix), a1);
-    const float r = cubic_fetch_rows(a0, a1, tap_of(ty.h0, iy), tap_of(ty.h1, iy), tap_of(tz.h0, iz),
-                                     tap_of(tz.h1, iz), tx.g0, tx.g1, ty.g0, ty.g1, tz.g0, tz.g1);
-    return r * kInv65535;
+// One z slice (4 x 4 raw values) of the window.
+__device__ __forceinline__ void load_slice(const uint16_t* __restrict__ d, const WindowAddr& w, int z, float v[4][4]) {
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+        const uint32_t base = w.y[y] + w.z[z];
+#pragma unroll
+        for (int x = 0; x < 4; x++) v[y][x] = float(d[w.x[x] + base]);
+    }
+}
+// The slice's 4 rows lerped along x at tap t.
+__device__ __forceinline__ void slice_x(const float v[4][4], Tap t, float out[4]) {
+#pragma unroll
+    for (int y = 0; y < 4; y++) out[y] = lerp4(v[y][0], v[y][1], v[y][2], v[y][3], t);
 }
 
-// density_gradient_cubic (common.glsl:130-136), per index unit.
-__device__ __forceinline__ V3 density_gradient_cubic(const DevScene& s, V3 idx) {
+// density_cubic (common.glsl:124-127). The 8 trilinear fetches of cubic_fetch
+// (common.glsl:116-122) read only the 4^3 voxels around floor(idx); they are
+// evaluated from one pass over the window, a z slice at a time, with each
+// fetch's lerps along x, then y, then z, as a texture unit does them.
+__device__ inline float density_cubic(const DevScene& s, V3 idx) {
     const float ix = floorf(idx.x), iy = floorf(idx.y), iz = floorf(idx.z);
     const Taps1 tx = cubic_taps1(idx.x, ix), ty = cubic_taps1(idx.y, iy), tz = cubic_taps1(idx.z, iz);
-    Window w;
-    load_window(s, fidx(ix), fidx(iy), fidx(iz), w);
-    const Tap hx0 = tap_of(tx.h0, ix), hx1 = tap_of(tx.h1, ix), dx0 = tap_of(tx.dh0, ix), dx1 = tap_of(tx.dh1, ix);
-    const Tap hy0 = tap_of(ty.h0, iy), hy1 = tap_of(ty.h1, iy), dy0 = tap_of(ty.dh0, iy), dy1 = tap_of(ty.dh1, iy);
-    const Tap hz0 = tap_of(tz.h0, iz), hz1 = tap_of(tz.h1, iz), dz0 = tap_of(tz.dh0, iz), dz1 = tap_of(tz.dh1, iz);
-    XRows d0, d1, a0, a1;
-    x_pass(w, dx0, d0);
-    x_pass(w, dx1, d1);
-    x_pass(w, hx0, a0);
-    x_pass(w, hx1, a1);
-    const float gx = cubic_fetch_rows(d0, d1, hy0, hy1, hz0, hz1, tx.dg0, tx.dg1, ty.g0, ty.g1, tz.g0, tz.g1);
-    const float gy = cubic_fetch_rows(a0, a1, dy0, dy1, hz0, hz1, tx.g0, tx.g1, ty.dg0, ty.dg1, tz.g0, tz.g1);
-    const float gz = cubic_fetch_rows(a0, a1, hy0, hy1, dz0, dz1, tx.g0, tx.g1, ty.g0, ty.g1, tz.dg0, tz.dg1);
+    const Tap x0 = tap_of(tx.h0, ix), x1 = tap_of(tx.h1, ix);
+    const Tap y0 = tap_of(ty.h0, iy), y1 = tap_of(ty.h1, iy);
+    const Tap z0 = tap_of(tz.h0, iz), z1 = tap_of(tz.h1, iz);
+    const WindowAddr w = window_addr(s.g, fidx(ix), fidx(iy), fidx(iz));
+    float c[2][2][4];  // [x tap][y tap][z]
+#pragma unroll
+    for (int z = 0; z < 4; z++) {
+        float v[4][4], r0[4], r1[4];
+        load_slice(s.density, w, z, v);
+        slice_x(v, x0, r0);
+        slice_x(v, x1, r1);
+        c[0][0][z] = lerp4(r0[0], r0[1], r0[2], r0[3], y0);
+        c[0][1][z] = lerp4(r0[0], r0[1], r0[2], r0[3], y1);
+        c[1][0][z] = lerp4(r1[0], r1[1], r1[2], r1[3], y0);
+        c[1][1][z] = lerp4(r1[0], r1[1], r1[2], r1[3], y1);
+    }
+    auto T = [&](int a, int b, Tap t) { return lerp4(c[a][b][0], c[a][b][1], c[a][b][2], c[a][b][3], t); };
+    const float y0z0 = tx.g0 * T(0, 0, z0) + tx.g1 * T(1, 0, z0);
+    const float y1z0 = tx.g0 * T(0, 1, z0) + tx.g1 * T(1, 1, z0);
+    const float y0z1 = tx.g0 * T(0, 0, z1) + tx.g1 * T(1, 0, z1);
+    const float y1z1 = tx.g0 * T(0, 1, z1) + tx.g1 * T(1, 1, z1);
+    return (tz.g0 * (ty.g0 * y0z0 + ty.g1 * y1z0) + tz.g1 * (ty.g0 * y0z1 + ty.g1 * y1z1)) * kInv65535;
+}
+
+// density_gradient_cubic (common.glsl:130-136), per index unit: the 24 trilinear
+// fetches from one pass over the window.
+__device__ inline V3 density_gradient_cubic(const DevScene& s, V3 idx) {
+    const float ix = floorf(idx.x), iy = floorf(idx.y), iz = floorf(idx.z);
+    const Taps1 tx = cubic_taps1(idx.x, ix), ty = cubic_taps1(idx.y, iy), tz = cubic_taps1(idx.z, iz);
+    const Tap hx[2] = {tap_of(tx.h0, ix), tap_of(tx.h1, ix)}, dx[2] = {tap_of(tx.dh0, ix), tap_of(tx.dh1, ix)};
+    const Tap hy[2] = {tap_of(ty.h0, iy), tap_of(ty.h1, iy)}, dy[2] = {tap_of(ty.dh0, iy), tap_of(ty.dh1, iy)};
+    const Tap hz[2] = {tap_of(tz.h0, iz), tap_of(tz.h1, iz)}, dz[2] = {tap_of(tz.dh0, iz), tap_of(tz.dh1, iz)};
+    const WindowAddr w = window_addr(s.g, fidx(ix), fidx(iy), fidx(iz));
+    float gxz[2][2][4], gyz[2][2][4], hhz[2][2][4];  // [x tap][y tap][z] for d/dx, d/dy, and value taps
+#pragma unroll
+    for (int z = 0; z < 4; z++) {
+        float v[4][4], rh[2][4], rd[2][4];
+        load_slice(s.density, w, z, v);
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+            slice_x(v, hx[a], rh[a]);
+            slice_x(v, dx[a], rd[a]);
+        }
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                gxz[a][b][z] = lerp4(rd[a][0], rd[a][1], rd[a][2], rd[a][3], hy[b]);
+                gyz[a][b][z] = lerp4(rh[a][0], rh[a][1], rh[a][2], rh[a][3], dy[b]);
+                hhz[a][b][z] = lerp4(rh[a][0], rh[a][1], rh[a][2], rh[a][3], hy[b]);
+            }
+    }
+    auto fold = [](const float c[2][2][4], const Tap zt[2], float gx0, float gx1, float gy0, float gy1, float gz0,
+                   float gz1) {
+        auto T = [&](int a, int b, Tap t) { return lerp4(c[a][b][0], c[a][b][1], c[a][b][2], c[a][b][3], t); };
+        const float y0z0 = gx0 * T(0, 0, zt[0]) + gx1 * T(1, 0, zt[0]);
+        const float y1z0 = gx0 * T(0, 1, zt[0]) + gx1 * T(1, 1, zt[0]);
+        const float y0z1 = gx0 * T(0, 0, zt[1]) + gx1 * T(1, 0, zt[1]);
+        const float y1z1 = gx0 * T(0, 1, zt[1]) + gx1 * T(1, 1, zt[1]);
+        return gz0 * (gy0 * y0z0 + gy1 * y1z0) + gz1 * (gy0 * y0z1 + gy1 * y1z1);
+    };
+    const float gx = fold(gxz, hz, tx.dg0, tx.dg1, ty.g0, ty.g1, tz.g0, tz.g1);
+    const float gy = fold(gyz, hz, tx.g0, tx.g1, ty.dg0, ty.dg1, tz.g0, tz.g1);
+    const float gz = fold(hhz, dz, tx.g0, tx.g1, ty.g0, ty.g1, tz.dg0, tz.dg1);
     return V3(gx * kInv65535, gy * kInv65535, gz * kInv65535);
 }
 
@@ -389,7 +408,7 @@ __device__ inline bool drawn_label_near(const DevScene& s, V3 idx, uint32_t mode
     return false;
 }
 
-__device__ inline Sample apply_labels(const DevScene& s, Sample smp, V3 idx) {
+__device__ __forceinline__ Sample apply_labels(const DevScene& s, Sample smp, V3 idx) {
     const uint32_t mode = uint32_t(s.u->labels[0]);
     if (mode == 0u) return smp;
     const F2 m = mask_at(s, idx);
@@ -427,8 +446,14 @@ __device__ __forceinline__ bool is_clipped(const vvr_uniforms* u, V3 idx) {
     return clip_depth(u, idx, plane) > 0.0f;
 }
 
+// In isolate mode a drawn sample takes the label's albedo and extinction and an
+// undrawn one has none (common.glsl:253-351): the density is never read, so
+// its fetch is skipped. Every other mode reads it.
+__device__ __forceinline__ bool density_unused(const vvr_uniforms* u) { return uint32_t(u->labels[0]) == 2u; }
+
 __device__ __forceinline__ Sample sample_point(const DevScene& s, V3 idx) {
-    Sample smp = apply_labels(s, tf_point(s, density_at_index(s, idx)), idx);
+    const float d = density_unused(s.u) ? 0.0f : density_at_index(s, idx);
+    Sample smp = apply_labels(s, tf_point(s, d), idx);
     if (smp.ghost || is_clipped(s.u, idx)) smp.extinction = 0.0f;
     return smp;
 }

@@ -190,7 +190,7 @@ __device__ inline float lobule_border(V3 q, I3& lobule) {
     return border;
 }
 
-__device__ inline float surface_height(V3 p, float4 params) {
+__device__ __noinline__ float surface_height(V3 p, float4 params) {
     const V3 q = p / params.w;
     if (params.x > 1.5f) {
         I3 lobule;

@@ -121,36 +121,14 @@ __global__ void __launch_bounds__(64) k_dvr(DevScene sc, uint16_t* __restrict__ 
     store_half4(hdr, size_t(py) * width + px, radiance.x, radiance.y, radiance.z, transmittance);
 }
 
-// pt.comp:373-561, one lane per pixel: `samples` paths at path indices
-// first_path + k, each from its own PCG stream (the reference's seed of a frame
-// of one path), summed in path order.
-__global__ void __launch_bounds__(64) k_pt(DevScene sc, float4* __restrict__ accum, float4* __restrict__ albedo,
-                                           float4* __restrict__ normal, uint16_t* __restrict__ hdr, uint32_t width,
-                                           uint32_t height, uint32_t first_path, uint32_t samples) {
-    uint32_t px, py;
-    tile_pixel(px, py);
-    if (px >= width || py >= height) return;
-    V3 sum(0.0f);
-    float background = 0.0f;
-    V3 alb(0.0f);
-    float lum2 = 0.0f;
-    V3 nrm(0.0f);
-    float depth = 0.0f;
-    for (uint32_t k = 0; k < samples; k++) {
-        Rng rng = seed_rng(px, py, first_path + k);
-        const PathResult r = trace_path(sc, px, py, rng);
-        sum = sum + r.radiance;
-        background += r.background;
-        alb = alb + r.albedo;
-        nrm = nrm + r.normal;
-        depth += r.depth;
-        const float lum = dot(r.radiance, V3(0.2126f, 0.7152f, 0.0722f));
-        lum2 += lum * lum;
-    }
-    const size_t i = size_t(py) * width + px;
-    float4 a = make_float4(sum.x, sum.y, sum.z, background);
-    float4 b = make_float4(alb.x, alb.y, alb.z, lum2);
-    float4 c = make_float4(nrm.x, nrm.y, nrm.z, depth);
+// Adds a pixel's path sums to the accumulation and guide images (restarting
+// them for the first frame) and writes the running mean (pt.comp:554-560).
+__device__ __forceinline__ void store_pixel(const PixelSums& ps, size_t i, float4* __restrict__ accum,
+                                            float4* __restrict__ albedo, float4* __restrict__ normal,
+                                            uint16_t* __restrict__ hdr, uint32_t first_path, uint32_t samples) {
+    float4 a = make_float4(ps.radiance.x, ps.radiance.y, ps.radiance.z, ps.background);
+    float4 b = make_float4(ps.albedo.x, ps.albedo.y, ps.albedo.z, ps.lum2);
+    float4 c = make_float4(ps.normal.x, ps.normal.y, ps.normal.z, ps.depth);
     if (first_path != 0u) {
         const float4 pa = accum[i], pb = albedo[i], pc = normal[i];
         a = make_float4(pa.x + a.x, pa.y + a.y, pa.z + a.z, pa.w + a.w);
@@ -160,8 +138,108 @@ __global__ void __launch_bounds__(64) k_pt(DevScene sc, float4* __restrict__ acc
     accum[i] = a;
     albedo[i] = b;
     normal[i] = c;
-    const float inv = float(first_path + samples);
-    store_half4(hdr, i, a.x / inv, a.y / inv, a.z / inv, a.w / inv);
+    const float n = float(first_path + samples);
+    store_half4(hdr, i, a.x / n, a.y / n, a.z / n, a.w / n);
+}
+
+// pt.comp:373-561, one lane per pixel: `samples` paths at path indices
+// first_path + k, each from its own PCG stream (the reference's seed of a frame
+// of one path), summed in path order.
+__global__ void __launch_bounds__(64) k_pt(DevScene sc, float4* __restrict__ accum, float4* __restrict__ albedo,
+                                           float4* __restrict__ normal, uint16_t* __restrict__ hdr, uint32_t width,
+                                           uint32_t height, uint32_t first_path, uint32_t samples) {
+    uint32_t px, py;
+    tile_pixel(px, py);
+    if (px >= width || py >= height) return;
+    const FrameConsts fc = frame_consts(sc.u);
+    PixelSums ps;
+    ps.clear();
+    PathState st;
+    for (uint32_t k = 0; k < samples; k++) {
+        path_begin(sc, fc, px, py, first_path + k, st);
+        while (path_bounce(sc, fc, st)) {
+        }
+        ps.add(st);
+    }
+    store_pixel(ps, size_t(py) * width + px, accum, albedo, normal, hdr, first_path, samples);
+}
+
+// Pixel of work item `item`: 8 x 8 tiles in row-major order, so the first paths
+// of a wave are coherent.
+__device__ __forceinline__ void item_pixel(uint32_t item, uint32_t tiles_x, uint32_t& px, uint32_t& py) {
+    const uint32_t tile = item >> 6, w = item & 63u;
+    px = (tile % tiles_x) * 8u + (w & 7u);
+    py = (tile / tiles_x) * 8u + (w >> 3);
+}
+
+// The same paths as k_pt, traced by persistent waves with path regeneration:
+// a lane whose path ends starts its pixel's next path at once, and a lane whose
+// pixel is done takes the next pixel from a global queue. Lanes needing a pixel
+// are counted with a wave ballot; one atomic per wave reserves their items and
+// each lane takes its slot by its rank (mbcnt) among them. A lane traces its
+// pixel's paths in index order and writes the pixel alone, so the images equal
+// k_pt's.
+__global__ void __launch_bounds__(64) k_pt_persistent(DevScene sc, float4* __restrict__ accum,
+                                                      float4* __restrict__ albedo, float4* __restrict__ normal,
+                                                      uint16_t* __restrict__ hdr, uint32_t width, uint32_t height,
+                                                      uint32_t first_path, uint32_t samples,
+                                                      uint32_t* __restrict__ queue) {
+    const uint32_t tiles_x = (width + 7u) / 8u;
+    const uint32_t n_items = tiles_x * ((height + 7u) / 8u) * 64u;
+    const FrameConsts fc = frame_consts(sc.u);
+    const uint32_t lane = threadIdx.x;
+    constexpr uint32_t kNone = 0xffffffffu;
+    uint32_t item = kNone, k = 0, px = 0, py = 0;
+    bool active = false, done = false;
+    PixelSums ps;
+    ps.clear();
+    PathState st;
+    for (;;) {
+        // A lane whose pixel has all its paths writes it out.
+        if (!active && item != kNone && k == samples) {
+            if (px < width && py < height)
+                store_pixel(ps, size_t(py) * width + px, accum, albedo, normal, hdr, first_path, samples);
+            item = kNone;
+        }
+        // Lanes without a pixel take the next ones from the queue.
+        const bool need = item == kNone && !done;
+        const uint64_t mask = __ballot(need);
+        if (mask) {
+            const int leader = __ffsll((unsigned long long)mask) - 1;
+            uint32_t base = 0;
+            if (int(lane) == leader) base = atomicAdd(queue, uint32_t(__popcll(mask)));
+            base = __shfl(base, leader);
+            if (need) {
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+                const uint32_t it = base + rank;
+                if (it < n_items) {
+                    item = it;
+                    k = 0;
+                    ps.clear();
+                    item_pixel(it, tiles_x, px, py);
+                } else {
+                    done = true;
+                }
+            }
+        }
+        // Lanes with a pixel and no path in flight start its next path (off-image
+        // pixels of partial tiles are skipped).
+        if (!active && item != kNone && k < samples) {
+            if (px < width && py < height) {
+                path_begin(sc, fc, px, py, first_path + k, st);
+                active = true;
+            } else {
+                k = samples;
+            }
+        }
+        if (__ballot(active || (item != kNone)) == 0) break;
+        if (active && !path_bounce(sc, fc, st)) {
+            ps.add(st);
+            k++;
+            active = false;
+        }
+    }
 }
 
 }  // namespace
@@ -175,6 +253,19 @@ void launch_pt(const DevScene& s, float4* accum, float4* albedo, float4* normal,
                uint32_t height, uint32_t first_path, uint32_t samples, hipStream_t st) {
     const dim3 grid((width + 7) / 8, (height + 7) / 8);
     hipLaunchKernelGGL(k_pt, grid, dim3(64), 0, st, s, accum, albedo, normal, hdr, width, height, first_path, samples);
+}
+
+int pt_persistent_waves_per_cu() {
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_pt_persistent, 64, 0) != hipSuccess) return 8;
+    return blocks > 0 ? blocks : 1;
+}
+
+void launch_pt_persistent(const DevScene& s, float4* accum, float4* albedo, float4* normal, uint16_t* hdr,
+                          uint32_t width, uint32_t height, uint32_t first_path, uint32_t samples, uint32_t* queue,
+                          uint32_t waves, hipStream_t st) {
+    hipLaunchKernelGGL(k_pt_persistent, dim3(waves), dim3(64), 0, st, s, accum, albedo, normal, hdr, width, height,
+                       first_path, samples, queue);
 }
 
 }  // namespace vvr

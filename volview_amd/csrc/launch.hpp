@@ -40,4 +40,11 @@ void launch_dvr(const DevScene& s, uint16_t* hdr, uint32_t width, uint32_t heigh
 void launch_pt(const DevScene& s, float4* accum, float4* albedo, float4* normal, uint16_t* hdr, uint32_t width,
                uint32_t height, uint32_t first_path, uint32_t samples, hipStream_t st);
 
+// The same frame by persistent waves pulling pixels from `queue` (one uint32,
+// zero at launch): `waves` single-wave workgroups.
+int pt_persistent_waves_per_cu();
+void launch_pt_persistent(const DevScene& s, float4* accum, float4* albedo, float4* normal, uint16_t* hdr,
+                          uint32_t width, uint32_t height, uint32_t first_path, uint32_t samples, uint32_t* queue,
+                          uint32_t waves, hipStream_t st);
+
 }  // namespace vvr

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <optional>
@@ -82,6 +83,9 @@ struct vvr_renderer {
     float4* accum = nullptr;
     float4* albedo = nullptr;
     float4* normal = nullptr;
+    uint32_t* queue = nullptr;   // work queue of the persistent path tracer
+    uint32_t pt_waves = 0;       // its grid: waves resident on the device
+    bool pt_simple = false;      // VVR_PT_KERNEL=simple: one lane per pixel instead
     uint32_t accumulated = 0;
     bool acc_valid = false;
     vvr_frame_params acc_params{};
@@ -178,7 +182,7 @@ struct vvr_renderer {
         }
         free_frame();
         dfree(density), dfree(normals), dfree(majorant), dfree(labels), dfree(mask), dfree(mask_b), dfree(soft);
-        dfree(lut), dfree(preint), dfree(d_uniforms[0]), dfree(d_uniforms[1]);
+        dfree(lut), dfree(preint), dfree(d_uniforms[0]), dfree(d_uniforms[1]), dfree(queue);
         if (h_uniforms) (void)hipHostFree(h_uniforms);
         for (auto& e : slot_done)
             if (e) (void)hipEventDestroy(e);
@@ -234,6 +238,14 @@ void new_renderer(vvr_ctx* ctx, const vvr_volume* vol, const vvr_tf* tf, vvr_ren
     check(hipHostMalloc((void**)&r->h_uniforms, 2 * sizeof(vvr_uniforms), hipHostMallocDefault), "hipHostMalloc");
     for (auto& d : r->d_uniforms) d = dalloc<vvr_uniforms>(1, "uniforms");
     r->lut = dalloc<float4>(kLutSize, "transfer function");
+    r->queue = dalloc<uint32_t>(1, "work queue");
+    {
+        int cus = 0;
+        check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, r->device), "device attribute");
+        r->pt_waves = uint32_t(std::max(cus, 1) * pt_persistent_waves_per_cu());
+        const char* k = std::getenv("VVR_PT_KERNEL");
+        r->pt_simple = k && std::strcmp(k, "simple") == 0;
+    }
     r->preint = dalloc<float4>(kLutSize, "pre-integration table");
     r->upload_tf(*tf);
 
@@ -360,9 +372,17 @@ void record(vvr_renderer* r, uint64_t frame, const vvr_frame_params* p) {
     if (pt) {
         const uint32_t spf = std::max(p->path_tracing.samples_per_frame, 1u);
         const uint32_t first = r->accumulated;
-        r->timed(VVR_KERNEL_PT, [&] {
-            launch_pt(sc, r->accum, r->albedo, r->normal, r->hdr, r->width, r->height, first, spf, r->stream);
-        });
+        if (r->pt_simple) {
+            r->timed(VVR_KERNEL_PT, [&] {
+                launch_pt(sc, r->accum, r->albedo, r->normal, r->hdr, r->width, r->height, first, spf, r->stream);
+            });
+        } else {
+            check(hipMemsetAsync(r->queue, 0, sizeof(uint32_t), r->stream), "queue reset");
+            r->timed(VVR_KERNEL_PT, [&] {
+                launch_pt_persistent(sc, r->accum, r->albedo, r->normal, r->hdr, r->width, r->height, first, spf,
+                                     r->queue, r->pt_waves, r->stream);
+            });
+        }
         r->accumulated += spf;
     } else {
         r->timed(VVR_KERNEL_DVR, [&] { launch_dvr(sc, r->hdr, r->width, r->height, r->stream); });
