@@ -47,13 +47,16 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"build failed: {' '.join(cmd)}\n{proc.stdout}\n{proc.stderr}")
 
 
-def build(verbose: bool = False) -> Path:
+def build(verbose: bool = False, defines: tuple = (), out: Path | None = None) -> Path:
+    """Compiles every source and links the library (to `out`, default libvvr.so);
+    `defines` are extra -D flags for the HIP sources."""
     hipcc = _hipcc()
     BUILD.mkdir(exist_ok=True)
     jobs = []
     for src in HIP_SOURCES:
         obj = BUILD / (src + ".o")
-        jobs.append([hipcc, "-x", "hip", *COMMON, *DEVICE, "-c", str(CSRC / src), "-o", str(obj)])
+        jobs.append([hipcc, "-x", "hip", *COMMON, *DEVICE, *[f"-D{d}" for d in defines], "-c", str(CSRC / src),
+                     "-o", str(obj)])
     for src in HOST_SOURCES:
         obj = BUILD / (src + ".o")
         jobs.append([CXX, *COMMON, "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-c", str(CSRC / src), "-o", str(obj)])
@@ -64,11 +67,12 @@ def build(verbose: bool = False) -> Path:
                 print(" ".join(cmd), flush=True)
         list(ex.map(_run, jobs))
     objs = [str(BUILD / (s + ".o")) for s in HIP_SOURCES + HOST_SOURCES]
-    tmp = LIB.with_suffix(".so.tmp")
+    target = Path(out) if out is not None else LIB
+    tmp = target.with_suffix(".so.tmp")
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs, "-lpthread",
           f"-Wl,-rpath,{ROCM}/lib"])
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, target)
+    return target
 
 
 if __name__ == "__main__":

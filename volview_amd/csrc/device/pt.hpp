@@ -333,7 +333,7 @@ __device__ __forceinline__ bool path_bounce(const DevScene& sc, const FrameConst
     const V3 v = -st.dir;
     V3 n;
     float surfaceness;
-    surface_frame(sc, idx, s, v, cap_plane, cut, 1.5f * fc.step_mm, n, surfaceness);
+    surface_frame(sc, idx, s, v, cap_plane, cut, 1.5f * fc.step_mm, n, surfaceness, st.bounce > 0u);
     apply_surface_texture(u, idx, s);
     const float xi_surface = st.rng.rand();
     const bool surface_event = xi_surface < surfaceness * (1.0f - s.subsurface) && dot(n, n) > 0.5f;

@@ -190,40 +190,58 @@ __device__ __forceinline__ void slice_x(const float v[4][4], Tap t, float out[4]
 #pragma unroll
     for (int y = 0; y < 4; y++) out[y] = lerp4(v[y][0], v[y][1], v[y][2], v[y][3], t);
 }
+// Weight of window slice z in the lerp at tap t: 1 - a at o, a at o + 1.
+__device__ __forceinline__ float tap_weight(Tap t, int z) {
+    return z == t.o ? 1.0f - t.a : (z == t.o + 1 ? t.a : 0.0f);
+}
+// acc + c w, fused like the second product of mixf, so that summing the slices
+// in order reproduces mixf of the two weighted ones exactly.
+__device__ __forceinline__ float accum_tap(float acc, float c, float w) { return fmaf(c, w, acc); }
+
+// cubic_fetch's final combination (common.glsl:121) of the 8 trilinear values.
+__device__ __forceinline__ float fold8(const float T[2][2][2], float gx0, float gx1, float gy0, float gy1, float gz0,
+                                       float gz1) {
+    const float y0z0 = gx0 * T[0][0][0] + gx1 * T[1][0][0];
+    const float y1z0 = gx0 * T[0][1][0] + gx1 * T[1][1][0];
+    const float y0z1 = gx0 * T[0][0][1] + gx1 * T[1][0][1];
+    const float y1z1 = gx0 * T[0][1][1] + gx1 * T[1][1][1];
+    return gz0 * (gy0 * y0z0 + gy1 * y1z0) + gz1 * (gy0 * y0z1 + gy1 * y1z1);
+}
 
 // density_cubic (common.glsl:124-127). The 8 trilinear fetches of cubic_fetch
-// (common.glsl:116-122) read only the 4^3 voxels around floor(idx); they are
-// evaluated from one pass over the window, a z slice at a time, with each
-// fetch's lerps along x, then y, then z, as a texture unit does them.
+// (common.glsl:116-122) read only the 4^3 voxels around floor(idx). They are
+// evaluated in one pass over the window, a z slice at a time: each fetch lerps
+// along x, then y, then z as a texture unit does, the z lerp accumulated over
+// the slices.
 __device__ inline float density_cubic(const DevScene& s, V3 idx) {
     const float ix = floorf(idx.x), iy = floorf(idx.y), iz = floorf(idx.z);
     const Taps1 tx = cubic_taps1(idx.x, ix), ty = cubic_taps1(idx.y, iy), tz = cubic_taps1(idx.z, iz);
-    const Tap x0 = tap_of(tx.h0, ix), x1 = tap_of(tx.h1, ix);
-    const Tap y0 = tap_of(ty.h0, iy), y1 = tap_of(ty.h1, iy);
-    const Tap z0 = tap_of(tz.h0, iz), z1 = tap_of(tz.h1, iz);
+    const Tap xt[2] = {tap_of(tx.h0, ix), tap_of(tx.h1, ix)};
+    const Tap yt[2] = {tap_of(ty.h0, iy), tap_of(ty.h1, iy)};
+    const Tap zt[2] = {tap_of(tz.h0, iz), tap_of(tz.h1, iz)};
     const WindowAddr w = window_addr(s.g, fidx(ix), fidx(iy), fidx(iz));
-    float c[2][2][4];  // [x tap][y tap][z]
+    float T[2][2][2] = {{{0.f, 0.f}, {0.f, 0.f}}, {{0.f, 0.f}, {0.f, 0.f}}};  // [x tap][y tap][z tap]
 #pragma unroll
     for (int z = 0; z < 4; z++) {
-        float v[4][4], r0[4], r1[4];
+        float v[4][4], r[2][4];
         load_slice(s.density, w, z, v);
-        slice_x(v, x0, r0);
-        slice_x(v, x1, r1);
-        c[0][0][z] = lerp4(r0[0], r0[1], r0[2], r0[3], y0);
-        c[0][1][z] = lerp4(r0[0], r0[1], r0[2], r0[3], y1);
-        c[1][0][z] = lerp4(r1[0], r1[1], r1[2], r1[3], y0);
-        c[1][1][z] = lerp4(r1[0], r1[1], r1[2], r1[3], y1);
+        slice_x(v, xt[0], r[0]);
+        slice_x(v, xt[1], r[1]);
+        const float wz[2] = {tap_weight(zt[0], z), tap_weight(zt[1], z)};
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                const float c = lerp4(r[a][0], r[a][1], r[a][2], r[a][3], yt[b]);
+                T[a][b][0] = accum_tap(T[a][b][0], c, wz[0]);
+                T[a][b][1] = accum_tap(T[a][b][1], c, wz[1]);
+            }
     }
-    auto T = [&](int a, int b, Tap t) { return lerp4(c[a][b][0], c[a][b][1], c[a][b][2], c[a][b][3], t); };
-    const float y0z0 = tx.g0 * T(0, 0, z0) + tx.g1 * T(1, 0, z0);
-    const float y1z0 = tx.g0 * T(0, 1, z0) + tx.g1 * T(1, 1, z0);
-    const float y0z1 = tx.g0 * T(0, 0, z1) + tx.g1 * T(1, 0, z1);
-    const float y1z1 = tx.g0 * T(0, 1, z1) + tx.g1 * T(1, 1, z1);
-    return (tz.g0 * (ty.g0 * y0z0 + ty.g1 * y1z0) + tz.g1 * (ty.g0 * y0z1 + ty.g1 * y1z1)) * kInv65535;
+    return fold8(T, tx.g0, tx.g1, ty.g0, ty.g1, tz.g0, tz.g1) * kInv65535;
 }
 
 // density_gradient_cubic (common.glsl:130-136), per index unit: the 24 trilinear
-// fetches from one pass over the window.
+// fetches in one pass over the window.
 __device__ inline V3 density_gradient_cubic(const DevScene& s, V3 idx) {
     const float ix = floorf(idx.x), iy = floorf(idx.y), iz = floorf(idx.z);
     const Taps1 tx = cubic_taps1(idx.x, ix), ty = cubic_taps1(idx.y, iy), tz = cubic_taps1(idx.z, iz);
@@ -231,7 +249,8 @@ __device__ inline V3 density_gradient_cubic(const DevScene& s, V3 idx) {
     const Tap hy[2] = {tap_of(ty.h0, iy), tap_of(ty.h1, iy)}, dy[2] = {tap_of(ty.dh0, iy), tap_of(ty.dh1, iy)};
     const Tap hz[2] = {tap_of(tz.h0, iz), tap_of(tz.h1, iz)}, dz[2] = {tap_of(tz.dh0, iz), tap_of(tz.dh1, iz)};
     const WindowAddr w = window_addr(s.g, fidx(ix), fidx(iy), fidx(iz));
-    float gxz[2][2][4], gyz[2][2][4], hhz[2][2][4];  // [x tap][y tap][z] for d/dx, d/dy, and value taps
+    // [x tap][y tap][z tap] of the fetches of d/dx, d/dy and d/dz.
+    float GX[2][2][2] = {}, GY[2][2][2] = {}, GZ[2][2][2] = {};
 #pragma unroll
     for (int z = 0; z < 4; z++) {
         float v[4][4], rh[2][4], rd[2][4];
@@ -241,27 +260,26 @@ __device__ inline V3 density_gradient_cubic(const DevScene& s, V3 idx) {
             slice_x(v, hx[a], rh[a]);
             slice_x(v, dx[a], rd[a]);
         }
+        const float wh[2] = {tap_weight(hz[0], z), tap_weight(hz[1], z)};
+        const float wd[2] = {tap_weight(dz[0], z), tap_weight(dz[1], z)};
 #pragma unroll
         for (int a = 0; a < 2; a++)
 #pragma unroll
             for (int b = 0; b < 2; b++) {
-                gxz[a][b][z] = lerp4(rd[a][0], rd[a][1], rd[a][2], rd[a][3], hy[b]);
-                gyz[a][b][z] = lerp4(rh[a][0], rh[a][1], rh[a][2], rh[a][3], dy[b]);
-                hhz[a][b][z] = lerp4(rh[a][0], rh[a][1], rh[a][2], rh[a][3], hy[b]);
+                const float cx = lerp4(rd[a][0], rd[a][1], rd[a][2], rd[a][3], hy[b]);
+                const float cy = lerp4(rh[a][0], rh[a][1], rh[a][2], rh[a][3], dy[b]);
+                const float cz = lerp4(rh[a][0], rh[a][1], rh[a][2], rh[a][3], hy[b]);
+#pragma unroll
+                for (int e = 0; e < 2; e++) {
+                    GX[a][b][e] = accum_tap(GX[a][b][e], cx, wh[e]);
+                    GY[a][b][e] = accum_tap(GY[a][b][e], cy, wh[e]);
+                    GZ[a][b][e] = accum_tap(GZ[a][b][e], cz, wd[e]);
+                }
             }
     }
-    auto fold = [](const float c[2][2][4], const Tap zt[2], float gx0, float gx1, float gy0, float gy1, float gz0,
-                   float gz1) {
-        auto T = [&](int a, int b, Tap t) { return lerp4(c[a][b][0], c[a][b][1], c[a][b][2], c[a][b][3], t); };
-        const float y0z0 = gx0 * T(0, 0, zt[0]) + gx1 * T(1, 0, zt[0]);
-        const float y1z0 = gx0 * T(0, 1, zt[0]) + gx1 * T(1, 1, zt[0]);
-        const float y0z1 = gx0 * T(0, 0, zt[1]) + gx1 * T(1, 0, zt[1]);
-        const float y1z1 = gx0 * T(0, 1, zt[1]) + gx1 * T(1, 1, zt[1]);
-        return gz0 * (gy0 * y0z0 + gy1 * y1z0) + gz1 * (gy0 * y0z1 + gy1 * y1z1);
-    };
-    const float gx = fold(gxz, hz, tx.dg0, tx.dg1, ty.g0, ty.g1, tz.g0, tz.g1);
-    const float gy = fold(gyz, hz, tx.g0, tx.g1, ty.dg0, ty.dg1, tz.g0, tz.g1);
-    const float gz = fold(hhz, dz, tx.g0, tx.g1, ty.g0, ty.g1, tz.dg0, tz.dg1);
+    const float gx = fold8(GX, tx.dg0, tx.dg1, ty.g0, ty.g1, tz.g0, tz.g1);
+    const float gy = fold8(GY, tx.g0, tx.g1, ty.dg0, ty.dg1, tz.g0, tz.g1);
+    const float gz = fold8(GZ, tx.g0, tx.g1, ty.g0, ty.g1, tz.dg0, tz.dg1);
     return V3(gx * kInv65535, gy * kInv65535, gz * kInv65535);
 }
 
@@ -473,11 +491,81 @@ __device__ __forceinline__ V3 world_to_index_point(const vvr_uniforms* u, V3 p) 
               m[2] * p.x + m[6] * p.y + m[10] * p.z + m[14]);
 }
 
+// mask_gradient (common.glsl:395-404): central differences of the trilinear
+// mask at idx +- one voxel per axis. When every shifted position has the same
+// fraction as idx (it does unless idx +- 1 rounds in f32), the six 2^3
+// footprints cover only 32 distinct voxels, each loaded once; the lerps are the
+// same as six mask_at calls.
 __device__ inline V3 mask_gradient(const DevScene& s, V3 idx, int ch) {
-    const float gx = mask_channel(s, idx + V3(1.f, 0.f, 0.f), ch) - mask_channel(s, idx - V3(1.f, 0.f, 0.f), ch);
-    const float gy = mask_channel(s, idx + V3(0.f, 1.f, 0.f), ch) - mask_channel(s, idx - V3(0.f, 1.f, 0.f), ch);
-    const float gz = mask_channel(s, idx + V3(0.f, 0.f, 1.f), ch) - mask_channel(s, idx - V3(0.f, 0.f, 1.f), ch);
-    return to_index(s.u).mul_t(0.5f * V3(gx, gy, gz));
+    const V3 e[3] = {V3(1.f, 0.f, 0.f), V3(0.f, 1.f, 0.f), V3(0.f, 0.f, 1.f)};
+    const float f0x = floorf(idx.x), f0y = floorf(idx.y), f0z = floorf(idx.z);
+    const float ax = idx.x - f0x, ay = idx.y - f0y, az = idx.z - f0z;
+    bool shared = true;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const V3 p = idx + e[a], m = idx - e[a];
+        const float fp = floorf(component(p, a)), fm = floorf(component(m, a));
+        const float f0 = component(V3(f0x, f0y, f0z), a), a0 = component(V3(ax, ay, az), a);
+        shared = shared && fp == f0 + 1.0f && fm == f0 - 1.0f && component(p, a) - fp == a0 &&
+                 component(m, a) - fm == a0;
+    }
+    if (!shared) {
+        const float gx = mask_channel(s, idx + e[0], ch) - mask_channel(s, idx - e[0], ch);
+        const float gy = mask_channel(s, idx + e[1], ch) - mask_channel(s, idx - e[1], ch);
+        const float gz = mask_channel(s, idx + e[2], ch) - mask_channel(s, idx - e[2], ch);
+        return to_index(s.u).mul_t(0.5f * V3(gx, gy, gz));
+    }
+    const BrickGrid& g = s.g;
+    const int x0 = fidx(f0x), y0 = fidx(f0y), z0 = fidx(f0z);
+    uint32_t X[4], Y[4], Z[4];  // voxel rows x0 - 1 .. x0 + 2 (clamped), likewise y and z
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        X[k] = g.xoff(iclamp(x0 - 1 + k, 0, g.nx - 1));
+        Y[k] = g.yoff(iclamp(y0 - 1 + k, 0, g.ny - 1));
+        Z[k] = g.zoff(iclamp(z0 - 1 + k, 0, g.nz - 1));
+    }
+    const int sh = 16 * ch;
+    auto t = [&](int i, int j, int k) { return float((s.mask[X[i] + Y[j] + Z[k]] >> sh) & 0xffffu); };
+    // Trilinear value with lower corner (i, j, k) in window indices: lerp x, y, z.
+    auto tri = [&](float c000, float c100, float c010, float c110, float c001, float c101, float c011, float c111) {
+        const float c00 = mixf(c000, c100, ax), c10 = mixf(c010, c110, ax);
+        const float c01 = mixf(c001, c101, ax), c11 = mixf(c011, c111, ax);
+        return mixf(mixf(c00, c10, ay), mixf(c01, c11, ay), az) * kInv65535;
+    };
+    // x differences: rows y, z in {1, 2}, x in 0..3.
+    float vx[2][2][4];
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) vx[k][j][i] = t(i, j + 1, k + 1);
+    const float xp = tri(vx[0][0][2], vx[0][0][3], vx[0][1][2], vx[0][1][3], vx[1][0][2], vx[1][0][3], vx[1][1][2],
+                         vx[1][1][3]);
+    const float xm = tri(vx[0][0][0], vx[0][0][1], vx[0][1][0], vx[0][1][1], vx[1][0][0], vx[1][0][1], vx[1][1][0],
+                         vx[1][1][1]);
+    // The central 2^3 (x, y, z in {1, 2}) is vx[..][..][1..2]; y differences add y rows 0 and 3.
+    float y0r[2][2], y3r[2][2];  // [z][x]
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            y0r[k][i] = t(i + 1, 0, k + 1);
+            y3r[k][i] = t(i + 1, 3, k + 1);
+        }
+    const float yp = tri(vx[0][1][1], vx[0][1][2], y3r[0][0], y3r[0][1], vx[1][1][1], vx[1][1][2], y3r[1][0], y3r[1][1]);
+    const float ym = tri(y0r[0][0], y0r[0][1], vx[0][0][1], vx[0][0][2], y0r[1][0], y0r[1][1], vx[1][0][1], vx[1][0][2]);
+    float z0r[2][2], z3r[2][2];  // [y][x]
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            z0r[j][i] = t(i + 1, j + 1, 0);
+            z3r[j][i] = t(i + 1, j + 1, 3);
+        }
+    const float zp = tri(vx[1][0][1], vx[1][0][2], vx[1][1][1], vx[1][1][2], z3r[0][0], z3r[0][1], z3r[1][0], z3r[1][1]);
+    const float zm = tri(z0r[0][0], z0r[0][1], z0r[1][0], z0r[1][1], vx[0][0][1], vx[0][0][2], vx[0][1][1], vx[0][1][2]);
+    return to_index(s.u).mul_t(0.5f * V3(xp - xm, yp - ym, zp - zm));
 }
 
 __device__ __forceinline__ void intersect_volume(const vvr_uniforms* u, V3 origin, V3 dir, float& tn, float& tf) {

@@ -53,11 +53,42 @@ __device__ __forceinline__ float ign(float px, float py) {  // shading.glsl:61-6
     return fractf(52.9829189f * fractf(px * 0.06711056f + py * 0.00583715f));
 }
 
-// shading.glsl:73-144
+// shading.glsl:73-144.
+//
+// `lazy`: the caller uses n only where surfaceness > 0 (a path tracer event
+// after the first, pt.comp:454-458). Solid labelled tissue in isolate mode is
+// shaded by its mask: n is the mask normal wherever the mask gradient is
+// nonzero, and where it vanishes the surfaceness is 0, so the normal volume and
+// the B-spline gradient are read only when they can matter.
 __device__ inline float surface_frame(const DevScene& sc, V3 idx, const Sample& s, V3 v, int cap_plane, float cut,
-                                      float cap_depth, V3& n, float& surfaceness) {
+                                      float cap_depth, V3& n, float& surfaceness, bool lazy = false) {
     const vvr_uniforms* u = sc.u;
     const uint32_t label_mode = uint32_t(u->labels[0]);
+    if (lazy && label_mode == 2u && !s.ghost && s.label != 255u) {
+        const V3 g = mask_gradient(sc, idx, 0);
+        const float g_len = length(g);
+        surfaceness = clampf(g_len * 4.0f, 0.0f, 1.0f);
+        if (g_len > 1e-5f) {
+            n = (-g) / g_len;
+        } else if (surfaceness > 0.0f) {
+            const F4 nw = normal_at(sc, idx);
+            const V3 nxyz(nw.x, nw.y, nw.z);
+            const float n_len = length(nxyz);
+            n = n_len > 1e-3f ? nxyz / n_len : V3(0.0f);
+            if (u->env[1] > 0.5f && n_len > 1e-3f) {
+                const V3 gw = to_index(u).mul_t(density_gradient_cubic(sc, idx));
+                const float len = length(gw);
+                n = len > 1e-7f ? (-gw) / len : n;
+            }
+        } else {
+            n = V3(0.0f);  // surfaceness 0: never read
+        }
+        if (cap_plane >= 0 && -cut < cap_depth) {
+            n = V3(u->clip_normals[cap_plane][0], u->clip_normals[cap_plane][1], u->clip_normals[cap_plane][2]);
+            surfaceness = 1.0f;
+        }
+        return 1.0f;
+    }
     const F4 nw = normal_at(sc, idx);
     const V3 nxyz(nw.x, nw.y, nw.z);
     const float n_len = length(nxyz);

@@ -27,7 +27,10 @@ __device__ __forceinline__ V3 operator-(V3 a) { return V3(-a.x, -a.y, -a.z); }
 __device__ __forceinline__ float gmin(float a, float b) { return b < a ? b : a; }
 __device__ __forceinline__ float gmax(float a, float b) { return a < b ? b : a; }
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
-__device__ __forceinline__ float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+// GLSL mix, a (1 - t) + b t, with its one contraction pinned: b t is fused into
+// the sum. (Accumulating c0 w0 + c1 w1 + ... in this form, with w = 1 - t and t
+// at two taps and 0 elsewhere, gives exactly mix of the two taps.)
+__device__ __forceinline__ float mixf(float a, float b, float t) { return fmaf(b, t, a * (1.0f - t)); }
 __device__ __forceinline__ float fractf(float x) { return x - floorf(x); }
 __device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
     const float t = clampf((x - e0) / (e1 - e0), 0.0f, 1.0f);

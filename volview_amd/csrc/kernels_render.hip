@@ -179,7 +179,13 @@ __device__ __forceinline__ void item_pixel(uint32_t item, uint32_t tiles_x, uint
 // each lane takes its slot by its rank (mbcnt) among them. A lane traces its
 // pixel's paths in index order and writes the pixel alone, so the images equal
 // k_pt's.
-__global__ void __launch_bounds__(64) k_pt_persistent(DevScene sc, float4* __restrict__ accum,
+// Waves per SIMD the register allocation targets: the kernel's peak register
+// demand exceeds the 256 VGPRs of two waves, so this trades spills for occupancy.
+#ifndef VVR_PT_WAVES_PER_EU
+#define VVR_PT_WAVES_PER_EU 1
+#endif
+
+__global__ void __launch_bounds__(64, VVR_PT_WAVES_PER_EU) k_pt_persistent(DevScene sc, float4* __restrict__ accum,
                                                       float4* __restrict__ albedo, float4* __restrict__ normal,
                                                       uint16_t* __restrict__ hdr, uint32_t width, uint32_t height,
                                                       uint32_t first_path, uint32_t samples,
