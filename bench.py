@@ -8,8 +8,10 @@ pixel. One step is one request of the plain loop (crates/render/src/offscreen.rs
 record one frame of 64 paths per pixel, then wait for it.
 
 Prints one JSON line: PT Msamples/s (1 sample = 1 pixel x 1 path), per-request
-latency, the roofline of the path-tracing kernel (HIP events on its stream) and
-the CPU oracle's rate on a bounded sample of the same frame.
+latency, the roofline of the path-tracing pass (HIP events on the renderer's
+stream around the wavefront kernels of each frame: regen, extend, shade, shadow,
+scatter iterations and the resolve) and the CPU oracle's rate on a bounded
+sample of the same frame.
 """
 from __future__ import annotations
 
@@ -142,7 +144,7 @@ def main() -> int:
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-                     "kernel": "k_pt", "kernel_ms": round(avg_kernel_s * 1e3, 3), "algorithmic_bytes": algo},
+                     "kernel": "PT pass (k_wf_* iterations + k_wf_resolve)", "kernel_ms": round(avg_kernel_s * 1e3, 3), "algorithmic_bytes": algo},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, scene.params, args.cpu_seconds)

@@ -307,51 +307,84 @@ __device__ inline void path_begin(const DevScene& sc, const FrameConsts& fc, uin
     st.depth = 0.0f;
 }
 
-// One iteration of the bounce loop (pt.comp:427-545). Returns false once the
-// path has ended (then the firefly clamp of path_end applies).
-__device__ __forceinline__ bool path_bounce(const DevScene& sc, const FrameConsts& fc, PathState& st) {
+// A scattering event between the stages of one bounce (pt.comp:429-534).
+struct EventState {
+    V3 idx;          // event position (index space)
+    Sample s;        // the material there (after the surface texture)
+    int cap_plane;
+    float cut;
+    V3 n;            // shading normal (valid where used)
+    bool surface_event;
+    V3 light_dir;    // key-light sample
+    V3 light_t;      // its transmittance
+};
+
+// pt.comp:427-428: whether the bounce loop runs another iteration.
+__device__ __forceinline__ bool path_continues(const FrameConsts& fc, const PathState& st) {
+    return st.surfaces <= fc.max_bounces && st.volumes <= fc.volume_limit;
+}
+
+// pt.comp:433-447: free flight; on escape the environment and the key light
+// (bounce > 0) or the background guides (bounce 0). False once the path ended.
+__device__ __forceinline__ bool stage_extend(const DevScene& sc, const FrameConsts& fc, PathState& st, EventState& ev) {
     const vvr_uniforms* u = sc.u;
-    if (!(st.surfaces <= fc.max_bounces && st.volumes <= fc.volume_limit)) return false;
-    V3 idx;
-    Sample s;
-    int cap_plane = -1;
-    float cut = 0.0f;
-    if (!free_flight(sc, st.origin, st.dir, st.bounce == 0u, st.hero, st.spectral, idx, s, cap_plane, cut, st.rng)) {
-        if (st.bounce == 0u) {
-            st.background = 1.0f;
-            st.albedo = V3(1.0f);
-            st.depth = 1e4f;
-        } else {
-            const V3 path_weight = st.throughput * st.spectral / dot(st.spectral, V3(1.0f / 3.0f));
-            st.radiance = st.radiance + path_weight * environment(u, st.dir);
-            if (fc.area_light && dot(st.dir, fc.ldir) >= fc.cos_light)
-                st.radiance = st.radiance + path_weight * fc.light * fc.light_pdf * mis(st.last_pdf, fc.light_pdf);
-        }
-        return false;
+    ev.cap_plane = -1;
+    ev.cut = 0.0f;
+    if (free_flight(sc, st.origin, st.dir, st.bounce == 0u, st.hero, st.spectral, ev.idx, ev.s, ev.cap_plane, ev.cut,
+                    st.rng))
+        return true;
+    if (st.bounce == 0u) {
+        st.background = 1.0f;
+        st.albedo = V3(1.0f);
+        st.depth = 1e4f;
+    } else {
+        const V3 path_weight = st.throughput * st.spectral / dot(st.spectral, V3(1.0f / 3.0f));
+        st.radiance = st.radiance + path_weight * environment(u, st.dir);
+        if (fc.area_light && dot(st.dir, fc.ldir) >= fc.cos_light)
+            st.radiance = st.radiance + path_weight * fc.light * fc.light_pdf * mis(st.last_pdf, fc.light_pdf);
     }
+    return false;
+}
+
+// pt.comp:448-466 up to the shadow ray: shading frame, surface texture, the
+// surface-or-volume decision, first-event guides and the key-light sample.
+__device__ __forceinline__ void stage_shade(const DevScene& sc, const FrameConsts& fc, PathState& st, EventState& ev) {
+    const vvr_uniforms* u = sc.u;
+    const V3 v = -st.dir;
+    float surfaceness;
+    surface_frame(sc, ev.idx, ev.s, v, ev.cap_plane, ev.cut, 1.5f * fc.step_mm, ev.n, surfaceness, st.bounce > 0u);
+    apply_surface_texture(u, ev.idx, ev.s);
+    const float xi_surface = st.rng.rand();
+    ev.surface_event = xi_surface < surfaceness * (1.0f - ev.s.subsurface) && dot(ev.n, ev.n) > 0.5f;
+    if (ev.surface_event || st.bounce == 0u) ev.n = bump_normal(u, ev.idx, ev.s, ev.n);
+    if (st.bounce == 0u) {
+        st.albedo = ev.s.albedo;
+        st.normal = dot(ev.n, v) < 0.0f ? -ev.n : ev.n;
+        st.depth = length(ev.idx - fc.eye_idx);
+    }
+    ev.light_dir = sample_light(u, fc.light_frame, st.rng);
+}
+
+// pt.comp:466: the shadow ray.
+__device__ __forceinline__ void stage_shadow(const DevScene& sc, PathState& st, EventState& ev) {
+    ev.light_t = transmittance(sc, ev.idx, ev.light_dir, st.rng);
+}
+
+// pt.comp:478-544: next-event estimate, the next direction, Russian roulette.
+// False once the path ended.
+__device__ __forceinline__ bool stage_scatter(const DevScene& sc, const FrameConsts& fc, PathState& st,
+                                              EventState& ev) {
+    const vvr_uniforms* u = sc.u;
     const V3 path_weight = st.throughput * st.spectral / dot(st.spectral, V3(1.0f / 3.0f));
     const V3 v = -st.dir;
-    V3 n;
-    float surfaceness;
-    surface_frame(sc, idx, s, v, cap_plane, cut, 1.5f * fc.step_mm, n, surfaceness, st.bounce > 0u);
-    apply_surface_texture(u, idx, s);
-    const float xi_surface = st.rng.rand();
-    const bool surface_event = xi_surface < surfaceness * (1.0f - s.subsurface) && dot(n, n) > 0.5f;
-    if (surface_event || st.bounce == 0u) n = bump_normal(u, idx, s, n);
-    if (st.bounce == 0u) {
-        st.albedo = s.albedo;
-        st.normal = dot(n, v) < 0.0f ? -n : n;
-        st.depth = length(idx - fc.eye_idx);
-    }
-    const V3 light_dir = sample_light(u, fc.light_frame, st.rng);
-    const V3 light_t = transmittance(sc, idx, light_dir, st.rng);
-    if (surface_event) {
-        n = dot(n, v) < 0.0f ? -n : n;
+    const Sample& s = ev.s;
+    if (ev.surface_event) {
+        const V3 n = dot(ev.n, v) < 0.0f ? -ev.n : ev.n;
         V3 diffuse;
-        V3 specular = brdf_coated(n, v, light_dir, s, diffuse);
-        const float light_weight = fc.area_light ? mis(fc.light_pdf, bsdf_pdf(n, v, light_dir, s)) : 1.0f;
-        st.radiance = st.radiance + path_weight * (diffuse + specular) * gmax(dot(n, light_dir), 0.0f) * fc.light *
-                                        light_t * light_weight;
+        V3 specular = brdf_coated(n, v, ev.light_dir, s, diffuse);
+        const float light_weight = fc.area_light ? mis(fc.light_pdf, bsdf_pdf(n, v, ev.light_dir, s)) : 1.0f;
+        st.radiance = st.radiance + path_weight * (diffuse + specular) * gmax(dot(n, ev.light_dir), 0.0f) * fc.light *
+                                        ev.light_t * light_weight;
         const M3 frame = basis(n);
         V3 l;
         if (st.rng.rand() < coat_share(s)) {
@@ -370,16 +403,16 @@ __device__ __forceinline__ bool path_bounce(const DevScene& sc, const FrameConst
         st.throughput = st.throughput * (((diffuse + specular) * nl) / gmax(pdf, 1e-6f));
         st.last_pdf = pdf;
         st.surfaces++;
-        st.origin = idx + fc.ti.mul(n * fc.step_mm);
+        st.origin = ev.idx + fc.ti.mul(n * fc.step_mm);
         st.dir = l;
     } else {
         const V3 albedo = event_albedo(u, s);
-        const float light_phase = phase_hg(u, dot(st.dir, light_dir));
-        st.radiance = st.radiance + path_weight * albedo * light_phase * fc.light * light_t *
+        const float light_phase = phase_hg(u, dot(st.dir, ev.light_dir));
+        st.radiance = st.radiance + path_weight * albedo * light_phase * fc.light * ev.light_t *
                                         (fc.area_light ? mis(fc.light_pdf, light_phase) : 1.0f);
         st.throughput = st.throughput * albedo;
         st.volumes++;
-        st.origin = idx;
+        st.origin = ev.idx;
         const V3 next = sample_hg(u, st.dir, st.rng);
         st.last_pdf = phase_hg(u, dot(st.dir, next));
         st.dir = next;
@@ -392,6 +425,17 @@ __device__ __forceinline__ bool path_bounce(const DevScene& sc, const FrameConst
     }
     st.bounce++;
     return true;
+}
+
+// One iteration of the bounce loop (pt.comp:427-545). Returns false once the
+// path has ended (then the firefly clamp of path_end applies).
+__device__ __forceinline__ bool path_bounce(const DevScene& sc, const FrameConsts& fc, PathState& st) {
+    if (!path_continues(fc, st)) return false;
+    EventState ev;
+    if (!stage_extend(sc, fc, st, ev)) return false;
+    stage_shade(sc, fc, st, ev);
+    stage_shadow(sc, st, ev);
+    return stage_scatter(sc, fc, st, ev);
 }
 
 // Firefly clamp (pt.comp:546-548): the radiance the path adds.

@@ -52,4 +52,50 @@ struct DevScene {
     const vvr_uniforms* __restrict__ u;
 };
 
+// One path-tracing request of the wavefront engine: a frame of `samples` paths
+// per pixel at path indices first_path .. on one scene, cut into work items of
+// `chunk` consecutive paths of one pixel. Items are numbered chunk-major over
+// 8 x 8 pixel tiles: item = c * n_px + tile * 64 + pixel in tile.
+struct PtRequest {
+    DevScene scene;
+    float4* accum;
+    float4* albedo;
+    float4* normal;
+    uint16_t* hdr;
+    float4* partial;  // [n_chunks][3][n_px] chunk sums
+    uint32_t width, height, tiles_x;
+    uint32_t first_path, samples, chunk, n_chunks, n_px;
+    uint32_t item_base, n_items;  // global item range of this request
+};
+
+// Path slots of the wavefront engine, structure of float4 arrays (one float4
+// per slot per field; the w lanes carry packed integers).
+struct WfSlots {
+    float4* p_origin;  // origin (index space), last_pdf
+    float4* p_dir;     // direction (world), rng state
+    float4* p_thr;     // throughput, hero | surfaces << 2 | volumes << 8 | bounce << 18
+    float4* p_spec;    // spectral ratios, global item index
+    float4* p_rad;     // radiance, k | request << 16
+    float4* e_idx;     // event position, cut
+    float4* e_s0;      // sample albedo, extinction
+    float4* e_s1;      // roughness, metalness, subsurface, coat
+    float4* e_s2;      // density, label | ghost << 8 | (cap_plane + 1) << 9 | surface event << 12
+    float4* e_n;       // shading normal
+    float4* e_light;   // key-light direction
+    float4* e_lt;      // its transmittance
+    float4* c0;        // chunk sums: radiance, background
+    float4* c1;        //             albedo, luminance^2
+    float4* c2;        //             normal, depth
+};
+
+// Queue indices in WfQueues::counts.
+enum { kWfFF0 = 0, kWfFF1 = 1, kWfEV = 2, kWfEND0 = 3, kWfEND1 = 4, kWfHEAD = 5, kWfLIVE = 6, kWfCounts = 8 };
+
+struct WfQueues {
+    uint32_t* ff[2];   // slots to extend
+    uint32_t* ev;      // slots at an event (shade, shadow, scatter)
+    uint32_t* end[2];  // slots whose path ended (or that have no item)
+    uint32_t* counts;  // kWfCounts counters
+};
+
 }  // namespace vvr

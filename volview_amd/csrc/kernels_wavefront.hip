@@ -1,0 +1,457 @@
+// Wavefront (stream) path tracer: the bounce loop of pt.comp:427-545 cut into
+// stages, each its own kernel over a queue of the paths that are at that stage:
+//
+//   regen    paths that ended: add them to their pixel chunk, start the next
+//            path of the chunk or take a new chunk from the global item queue
+//   extend   free flight (delta tracking)               -> event or end
+//   shade    shading frame, surface decision, light sample
+//   shadow   ratio-tracked shadow ray
+//   scatter  next-event estimate, next direction, roulette -> extend or end
+//
+// Each kernel keeps few registers live and runs coherent waves; queues are
+// compacted with a wave ballot, one atomic per wave and the lanes' mbcnt ranks.
+// A path draws its random numbers in the same order as the megakernel, so it
+// traces the same path. Work items are chunks of consecutive paths of one
+// pixel; a chunk's sums go to its own slot of a partial buffer and are added in
+// chunk order, so frames are deterministic.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "device/pt.hpp"
+#include "launch.hpp"
+
+namespace vvr {
+
+namespace {
+
+constexpr uint32_t kNone = 0xffffffffu;
+
+__device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
+__device__ __forceinline__ float bitsf(uint32_t u) { return __uint_as_float(u); }
+__device__ __forceinline__ V3 xyz(float4 v) { return V3(v.x, v.y, v.z); }
+__device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+
+__device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+}
+
+// Appends `value` for the lanes with `push` set: one atomic per wave.
+__device__ __forceinline__ void wave_push(bool push, uint32_t value, uint32_t* __restrict__ queue,
+                                          uint32_t* __restrict__ count) {
+    const uint64_t mask = __ballot(push);
+    if (!mask) return;
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (int(__lane_id()) == leader) base = atomicAdd(count, uint32_t(__popcll(mask)));
+    base = __shfl(base, leader);
+    if (push) queue[base + lane_rank(mask)] = value;
+}
+
+// Reserves one value of *counter for each lane with `take` set; returns it.
+__device__ __forceinline__ uint32_t wave_take(bool take, uint32_t* __restrict__ counter) {
+    const uint64_t mask = __ballot(take);
+    if (!mask) return kNone;
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (int(__lane_id()) == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
+    base = __shfl(base, leader);
+    return take ? base + lane_rank(mask) : kNone;
+}
+
+template <bool kBatched>
+__device__ __forceinline__ const PtRequest& request(const PtRequest* __restrict__ reqs, uint32_t r) {
+    return kBatched ? reqs[r] : reqs[0];
+}
+
+// Request of global item `item` (requests are sorted by item_base).
+template <bool kBatched>
+__device__ __forceinline__ uint32_t request_of_item(const PtRequest* __restrict__ reqs, uint32_t n_reqs,
+                                                    uint32_t item) {
+    if (!kBatched) return 0u;
+    uint32_t lo = 0, hi = n_reqs - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (reqs[mid].item_base <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t pack_counters(const PathState& st) {
+    return uint32_t(st.hero) | (st.surfaces << 2) | (st.volumes << 8) | (st.bounce << 18);
+}
+
+struct SlotRef {
+    uint32_t item, k, req;
+};
+
+__device__ __forceinline__ void load_path(const WfSlots& S, uint32_t i, PathState& st, SlotRef& ref) {
+    const float4 a = S.p_origin[i], b = S.p_dir[i], c = S.p_thr[i], d = S.p_spec[i], e = S.p_rad[i];
+    st.origin = xyz(a);
+    st.last_pdf = a.w;
+    st.dir = xyz(b);
+    st.rng.state = fbits(b.w);
+    st.throughput = xyz(c);
+    const uint32_t cnt = fbits(c.w);
+    st.hero = int(cnt & 3u);
+    st.surfaces = (cnt >> 2) & 63u;
+    st.volumes = (cnt >> 8) & 1023u;
+    st.bounce = cnt >> 18;
+    st.spectral = xyz(d);
+    ref.item = fbits(d.w);
+    st.radiance = xyz(e);
+    ref.k = fbits(e.w) & 0xffffu;
+    ref.req = fbits(e.w) >> 16;
+}
+
+__device__ __forceinline__ void store_path(const WfSlots& S, uint32_t i, const PathState& st, const SlotRef& ref) {
+    S.p_origin[i] = f4(st.origin, st.last_pdf);
+    S.p_dir[i] = f4(st.dir, bitsf(st.rng.state));
+    S.p_thr[i] = f4(st.throughput, bitsf(pack_counters(st)));
+    S.p_spec[i] = f4(st.spectral, bitsf(ref.item));
+    S.p_rad[i] = f4(st.radiance, bitsf(ref.k | (ref.req << 16)));
+}
+
+__device__ __forceinline__ void load_event(const WfSlots& S, uint32_t i, EventState& ev) {
+    const float4 a = S.e_idx[i], b = S.e_s0[i], c = S.e_s1[i], d = S.e_s2[i];
+    ev.idx = xyz(a);
+    ev.cut = a.w;
+    ev.s.albedo = xyz(b);
+    ev.s.extinction = b.w;
+    ev.s.roughness = c.x;
+    ev.s.metalness = c.y;
+    ev.s.subsurface = c.z;
+    ev.s.coat = c.w;
+    ev.s.density = d.x;
+    const uint32_t bits = fbits(d.y);
+    ev.s.label = bits & 0xffu;
+    ev.s.ghost = ((bits >> 8) & 1u) != 0u;
+    ev.cap_plane = int((bits >> 9) & 3u) - 1;
+    ev.surface_event = ((bits >> 12) & 1u) != 0u;
+}
+
+__device__ __forceinline__ void store_event(const WfSlots& S, uint32_t i, const EventState& ev) {
+    S.e_idx[i] = f4(ev.idx, ev.cut);
+    S.e_s0[i] = f4(ev.s.albedo, ev.s.extinction);
+    S.e_s1[i] = make_float4(ev.s.roughness, ev.s.metalness, ev.s.subsurface, ev.s.coat);
+    const uint32_t bits = ev.s.label | (ev.s.ghost ? 1u << 8 : 0u) | (uint32_t(ev.cap_plane + 1) << 9) |
+                          (ev.surface_event ? 1u << 12 : 0u);
+    S.e_s2[i] = make_float4(ev.s.density, bitsf(bits), 0.0f, 0.0f);
+}
+
+__device__ __forceinline__ void add4(float4* p, float4 v) {
+    const float4 a = *p;
+    *p = make_float4(a.x + v.x, a.y + v.y, a.z + v.z, a.w + v.w);
+}
+
+// Pixel (possibly outside the image) and path index of sample k of item `item`.
+__device__ __forceinline__ void item_coords(const PtRequest& R, uint32_t item, uint32_t k, uint32_t& px, uint32_t& py,
+                                            uint32_t& chunk_index, uint32_t& p_item, uint32_t& path_index,
+                                            uint32_t& n_paths) {
+    const uint32_t within = item - R.item_base;
+    chunk_index = within / R.n_px;
+    p_item = within - chunk_index * R.n_px;
+    const uint32_t tile = p_item >> 6, w = p_item & 63u;
+    px = (tile % R.tiles_x) * 8u + (w & 7u);
+    py = (tile / R.tiles_x) * 8u + (w >> 3);
+    const uint32_t first = chunk_index * R.chunk;
+    n_paths = min(R.chunk, R.samples - first);
+    path_index = R.first_path + first + k;
+}
+
+__device__ __forceinline__ void store_partial(const PtRequest& R, uint32_t chunk_index, uint32_t p_item, float4 c0,
+                                              float4 c1, float4 c2) {
+    float4* base = R.partial + size_t(chunk_index) * 3u * R.n_px;
+    base[p_item] = c0;
+    base[R.n_px + p_item] = c1;
+    base[2u * R.n_px + p_item] = c2;
+}
+
+// Every slot empty and on the end queue; the item queue at 0.
+__global__ void k_wf_init(WfSlots S, WfQueues Q, uint32_t slots) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kWfCounts && i != kWfEND0 && i != kWfLIVE) Q.counts[i] = 0u;
+    if (i == kWfEND0 || i == kWfLIVE) Q.counts[i] = slots;
+    if (i >= slots) return;
+    Q.end[0][i] = i;
+    S.p_spec[i] = make_float4(0.0f, 0.0f, 0.0f, bitsf(kNone));
+    S.p_rad[i] = make_float4(0.0f, 0.0f, 0.0f, bitsf(0u));
+}
+
+// Stage 0: finish ended paths, start new ones.
+template <bool kBatched>
+__global__ void __launch_bounds__(64) k_wf_regen(const PtRequest* __restrict__ reqs, uint32_t n_reqs,
+                                                 uint32_t total_items, WfSlots S, WfQueues Q, uint32_t cur) {
+    const uint32_t nxt = cur ^ 1u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // queues this iteration's later stages fill
+        Q.counts[kWfFF0 + nxt] = 0u;
+        Q.counts[kWfEV] = 0u;
+        Q.counts[kWfEND0 + nxt] = 0u;
+    }
+    const uint32_t n = Q.counts[kWfEND0 + cur];
+    const uint32_t lane = __lane_id();
+    for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
+        const uint32_t q = base + lane;
+        const bool active = q < n;
+        const uint32_t slot = active ? Q.end[cur][q] : 0u;
+        bool push = false, need = false;
+        PathState st;
+        SlotRef ref{kNone, 0u, 0u};
+        float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f), c1 = c0, c2 = c0;
+        if (active) {
+            load_path(S, slot, st, ref);
+            need = true;
+            if (ref.item != kNone) {
+                const PtRequest& R = request<kBatched>(reqs, ref.req);
+                c0 = S.c0[slot], c1 = S.c1[slot], c2 = S.c2[slot];
+                // pt.comp:546-551: firefly clamp, sums of radiance and squared luminance.
+                const V3 r = path_end(st);
+                const float lum = dot(r, V3(0.2126f, 0.7152f, 0.0722f));
+                c0 = make_float4(c0.x + r.x, c0.y + r.y, c0.z + r.z, c0.w);
+                c1.w += lum * lum;
+                ref.k++;
+                uint32_t px, py, ci, pi, path_index, n_paths;
+                item_coords(R, ref.item, ref.k, px, py, ci, pi, path_index, n_paths);
+                if (ref.k < n_paths) {
+                    const FrameConsts fc = frame_consts(R.scene.u);
+                    path_begin(R.scene, fc, px, py, path_index, st);
+                    push = true;
+                    need = false;
+                } else {
+                    store_partial(R, ci, pi, c0, c1, c2);
+                }
+            }
+        }
+        // Lanes without a chunk take the next ones; chunks of pixels outside the
+        // image are finished at once.
+        while (__ballot(need)) {
+            const uint32_t it = wave_take(need, &Q.counts[kWfHEAD]);
+            if (need) {
+                if (it >= total_items) {
+                    ref.item = kNone;
+                    need = false;
+                } else {
+                    ref.req = request_of_item<kBatched>(reqs, n_reqs, it);
+                    const PtRequest& R = request<kBatched>(reqs, ref.req);
+                    ref.item = it;
+                    ref.k = 0u;
+                    uint32_t px, py, ci, pi, path_index, n_paths;
+                    item_coords(R, it, 0u, px, py, ci, pi, path_index, n_paths);
+                    if (px < R.width && py < R.height) {
+                        c0 = c1 = c2 = make_float4(0.f, 0.f, 0.f, 0.f);
+                        const FrameConsts fc = frame_consts(R.scene.u);
+                        path_begin(R.scene, fc, px, py, path_index, st);
+                        push = true;
+                        need = false;
+                    }
+                }
+            }
+        }
+        if (active) {
+            if (push) {
+                store_path(S, slot, st, ref);
+                S.c0[slot] = c0, S.c1[slot] = c1, S.c2[slot] = c2;
+            } else {
+                S.p_spec[slot] = make_float4(0.f, 0.f, 0.f, bitsf(kNone));
+            }
+        }
+        const bool idle = active && !push;
+        const uint64_t gone = __ballot(idle);
+        if (gone && int(lane) == __ffsll((unsigned long long)gone) - 1)
+            atomicSub(&Q.counts[kWfLIVE], uint32_t(__popcll(gone)));
+        wave_push(push, slot, Q.ff[cur], &Q.counts[kWfFF0 + cur]);
+    }
+}
+
+// Stage 1: free flight.
+template <bool kBatched>
+__global__ void __launch_bounds__(64) k_wf_extend(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
+                                                  uint32_t cur) {
+    const uint32_t nxt = cur ^ 1u;
+    const uint32_t n = Q.counts[kWfFF0 + cur];
+    for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
+        const uint32_t q = base + __lane_id();
+        const bool active = q < n;
+        const uint32_t slot = active ? Q.ff[cur][q] : 0u;
+        bool hit = false;
+        if (active) {
+            PathState st;
+            SlotRef ref;
+            load_path(S, slot, st, ref);
+            const PtRequest& R = request<kBatched>(reqs, ref.req);
+            const FrameConsts fc = frame_consts(R.scene.u);
+            EventState ev;
+            hit = stage_extend(R.scene, fc, st, ev);
+            if (hit) {
+                store_event(S, slot, ev);
+            } else {
+                S.p_rad[slot] = f4(st.radiance, bitsf(ref.k | (ref.req << 16)));
+                if (st.bounce == 0u) {  // background: pt.comp:434-438
+                    add4(&S.c0[slot], make_float4(0.f, 0.f, 0.f, 1.0f));
+                    add4(&S.c1[slot], make_float4(1.0f, 1.0f, 1.0f, 0.f));
+                    add4(&S.c2[slot], make_float4(0.f, 0.f, 0.f, 1e4f));
+                }
+            }
+            S.p_dir[slot].w = bitsf(st.rng.state);
+            S.p_spec[slot] = f4(st.spectral, bitsf(ref.item));
+        }
+        wave_push(active && hit, slot, Q.ev, &Q.counts[kWfEV]);
+        wave_push(active && !hit, slot, Q.end[nxt], &Q.counts[kWfEND0 + nxt]);
+    }
+}
+
+// Stage 2: shading frame and light sample.
+template <bool kBatched>
+__global__ void __launch_bounds__(64) k_wf_shade(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q) {
+    const uint32_t n = Q.counts[kWfEV];
+    for (uint32_t q = blockIdx.x * 64u + __lane_id(); q < n; q += gridDim.x * 64u) {
+        const uint32_t slot = Q.ev[q];
+        PathState st;  // the stage reads the direction, bounce and random stream
+        const float4 d = S.p_dir[slot];
+        st.dir = xyz(d);
+        st.rng.state = fbits(d.w);
+        st.bounce = fbits(S.p_thr[slot].w) >> 18;
+        const uint32_t req = fbits(S.p_rad[slot].w) >> 16;
+        const PtRequest& R = request<kBatched>(reqs, req);
+        const FrameConsts fc = frame_consts(R.scene.u);
+        EventState ev;
+        load_event(S, slot, ev);
+        stage_shade(R.scene, fc, st, ev);
+        store_event(S, slot, ev);
+        S.e_n[slot] = f4(ev.n, 0.0f);
+        S.e_light[slot] = f4(ev.light_dir, 0.0f);
+        S.p_dir[slot].w = bitsf(st.rng.state);
+        if (st.bounce == 0u) {  // guides of the first event (pt.comp:459-462)
+            add4(&S.c1[slot], f4(st.albedo, 0.0f));
+            add4(&S.c2[slot], f4(st.normal, st.depth));
+        }
+    }
+}
+
+// Stage 3: shadow ray.
+template <bool kBatched>
+__global__ void __launch_bounds__(64) k_wf_shadow(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q) {
+    const uint32_t n = Q.counts[kWfEV];
+    for (uint32_t q = blockIdx.x * 64u + __lane_id(); q < n; q += gridDim.x * 64u) {
+        const uint32_t slot = Q.ev[q];
+        const uint32_t req = fbits(S.p_rad[slot].w) >> 16;
+        const PtRequest& R = request<kBatched>(reqs, req);
+        PathState st;
+        st.rng.state = fbits(S.p_dir[slot].w);
+        EventState ev;
+        ev.idx = xyz(S.e_idx[slot]);
+        ev.light_dir = xyz(S.e_light[slot]);
+        stage_shadow(R.scene, st, ev);
+        S.e_lt[slot] = f4(ev.light_t, 0.0f);
+        S.p_dir[slot].w = bitsf(st.rng.state);
+    }
+}
+
+// Stage 4: next-event estimate, next direction, roulette.
+template <bool kBatched>
+__global__ void __launch_bounds__(64) k_wf_scatter(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
+                                                   uint32_t cur) {
+    const uint32_t nxt = cur ^ 1u;
+    const uint32_t n = Q.counts[kWfEV];
+    for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
+        const uint32_t q = base + __lane_id();
+        const bool active = q < n;
+        const uint32_t slot = active ? Q.ev[q] : 0u;
+        bool more = false;
+        if (active) {
+            PathState st;
+            SlotRef ref;
+            load_path(S, slot, st, ref);
+            const PtRequest& R = request<kBatched>(reqs, ref.req);
+            const FrameConsts fc = frame_consts(R.scene.u);
+            EventState ev;
+            load_event(S, slot, ev);
+            ev.n = xyz(S.e_n[slot]);
+            ev.light_dir = xyz(S.e_light[slot]);
+            ev.light_t = xyz(S.e_lt[slot]);
+            more = stage_scatter(R.scene, fc, st, ev) && path_continues(fc, st);
+            store_path(S, slot, st, ref);
+        }
+        wave_push(active && more, slot, Q.ff[nxt], &Q.counts[kWfFF0 + nxt]);
+        wave_push(active && !more, slot, Q.end[nxt], &Q.counts[kWfEND0 + nxt]);
+    }
+}
+
+// The chunk sums of every pixel of a request, added in chunk order to the
+// accumulation and guide images; the running mean to the HDR image (pt.comp:554-560).
+__global__ void k_wf_resolve(PtRequest R) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= R.n_px) return;
+    const uint32_t tile = p >> 6, w = p & 63u;
+    const uint32_t px = (tile % R.tiles_x) * 8u + (w & 7u), py = (tile / R.tiles_x) * 8u + (w >> 3);
+    if (px >= R.width || py >= R.height) return;
+    float4 s[3];
+    for (int j = 0; j < 3; j++) s[j] = R.partial[size_t(j) * R.n_px + p];
+    for (uint32_t c = 1; c < R.n_chunks; c++)
+        for (int j = 0; j < 3; j++) {
+            const float4 v = R.partial[(size_t(c) * 3u + j) * R.n_px + p];
+            s[j] = make_float4(s[j].x + v.x, s[j].y + v.y, s[j].z + v.z, s[j].w + v.w);
+        }
+    const size_t i = size_t(py) * R.width + px;
+    float4* out[3] = {R.accum, R.albedo, R.normal};
+    for (int j = 0; j < 3; j++) {
+        if (R.first_path != 0u) {
+            const float4 a = out[j][i];
+            s[j] = make_float4(a.x + s[j].x, a.y + s[j].y, a.z + s[j].z, a.w + s[j].w);
+        }
+        out[j][i] = s[j];
+    }
+    const float n = float(R.first_path + R.samples);
+    const __half h[4] = {__float2half_rn(s[0].x / n), __float2half_rn(s[0].y / n), __float2half_rn(s[0].z / n),
+                         __float2half_rn(s[0].w / n)};
+    uint2 word;
+    word.x = uint32_t(__half_as_ushort(h[0])) | (uint32_t(__half_as_ushort(h[1])) << 16);
+    word.y = uint32_t(__half_as_ushort(h[2])) | (uint32_t(__half_as_ushort(h[3])) << 16);
+    reinterpret_cast<uint2*>(R.hdr)[i] = word;
+}
+
+template <class K> int waves_for(K kernel, int cus, int per_cu_cap) {
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, 64, 0) != hipSuccess || blocks < 1) blocks = 4;
+    return cus * std::min(blocks, per_cu_cap);
+}
+
+}  // namespace
+
+void wf_init(const WfSlots& S, const WfQueues& Q, uint32_t slots, hipStream_t st) {
+    hipLaunchKernelGGL(k_wf_init, dim3((std::max(slots, uint32_t(kWfCounts)) + 255) / 256), dim3(256), 0, st, S, Q,
+                       slots);
+}
+
+WfGrid wf_grid(int cus) {
+    WfGrid g;
+    g.regen = waves_for(k_wf_regen<true>, cus, 32);
+    g.extend = waves_for(k_wf_extend<true>, cus, 32);
+    g.shade = waves_for(k_wf_shade<true>, cus, 32);
+    g.shadow = waves_for(k_wf_shadow<true>, cus, 32);
+    g.scatter = waves_for(k_wf_scatter<true>, cus, 32);
+    return g;
+}
+
+void wf_iteration(const PtRequest* reqs, uint32_t n_reqs, uint32_t total_items, const WfSlots& S, const WfQueues& Q,
+                  uint32_t cur, const WfGrid& g, hipStream_t st) {
+    if (n_reqs > 1) {
+        hipLaunchKernelGGL(k_wf_regen<true>, dim3(g.regen), dim3(64), 0, st, reqs, n_reqs, total_items, S, Q, cur);
+        hipLaunchKernelGGL(k_wf_extend<true>, dim3(g.extend), dim3(64), 0, st, reqs, S, Q, cur);
+        hipLaunchKernelGGL(k_wf_shade<true>, dim3(g.shade), dim3(64), 0, st, reqs, S, Q);
+        hipLaunchKernelGGL(k_wf_shadow<true>, dim3(g.shadow), dim3(64), 0, st, reqs, S, Q);
+        hipLaunchKernelGGL(k_wf_scatter<true>, dim3(g.scatter), dim3(64), 0, st, reqs, S, Q, cur);
+    } else {
+        hipLaunchKernelGGL(k_wf_regen<false>, dim3(g.regen), dim3(64), 0, st, reqs, n_reqs, total_items, S, Q, cur);
+        hipLaunchKernelGGL(k_wf_extend<false>, dim3(g.extend), dim3(64), 0, st, reqs, S, Q, cur);
+        hipLaunchKernelGGL(k_wf_shade<false>, dim3(g.shade), dim3(64), 0, st, reqs, S, Q);
+        hipLaunchKernelGGL(k_wf_shadow<false>, dim3(g.shadow), dim3(64), 0, st, reqs, S, Q);
+        hipLaunchKernelGGL(k_wf_scatter<false>, dim3(g.scatter), dim3(64), 0, st, reqs, S, Q, cur);
+    }
+}
+
+void wf_resolve(const PtRequest& r, hipStream_t st) {
+    hipLaunchKernelGGL(k_wf_resolve, dim3((r.n_px + 255) / 256), dim3(256), 0, st, r);
+}
+
+}  // namespace vvr

@@ -47,4 +47,18 @@ void launch_pt_persistent(const DevScene& s, float4* accum, float4* albedo, floa
                           uint32_t width, uint32_t height, uint32_t first_path, uint32_t samples, uint32_t* queue,
                           uint32_t waves, hipStream_t st);
 
+// Wavefront path tracer (kernels_wavefront.hip). One iteration runs the five
+// stages over the queues; `reqs` is a device array of n_reqs requests with
+// item ranges [item_base, item_base + n_items) covering [0, total_items).
+struct WfGrid {
+    int regen, extend, shade, shadow, scatter;  // single-wave workgroups per stage
+};
+WfGrid wf_grid(int cus);
+// All `slots` slots empty on the end queue of iteration 0, item queue at 0.
+void wf_init(const WfSlots& S, const WfQueues& Q, uint32_t slots, hipStream_t st);
+void wf_iteration(const PtRequest* reqs, uint32_t n_reqs, uint32_t total_items, const WfSlots& S, const WfQueues& Q,
+                  uint32_t cur, const WfGrid& g, hipStream_t st);
+// Chunk sums -> accumulation, guides and HDR images of one request.
+void wf_resolve(const PtRequest& r, hipStream_t st);
+
 }  // namespace vvr

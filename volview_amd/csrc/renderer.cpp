@@ -41,6 +41,145 @@ void dfree(void* p) {
     if (p) (void)hipFree(p);
 }
 
+uint32_t env_u32(const char* name, uint32_t fallback) {
+    const char* v = std::getenv(name);
+    if (!v || !*v) return fallback;
+    const unsigned long x = std::strtoul(v, nullptr, 10);
+    return x ? uint32_t(std::min<unsigned long>(x, 1ul << 30)) : fallback;
+}
+
+// Device state of the wavefront path tracer: the path slots, queues, the chunk
+// sums of the requests in flight and the request table.
+struct WfEngine {
+    int device = 0;
+    uint32_t slots = 0;
+    uint8_t* slot_mem = nullptr;
+    WfSlots S{};
+    WfQueues Q{};
+    float4* partial = nullptr;
+    size_t partial_cap = 0;  // float4s
+    PtRequest* d_reqs = nullptr;
+    PtRequest* h_reqs = nullptr;  // pinned staging
+    uint32_t req_cap = 0;
+    uint32_t* h_live = nullptr;   // pinned, two read-back slots
+    hipEvent_t live_ev[2] = {nullptr, nullptr};
+    WfGrid grid{};
+    uint32_t max_slots = 1u << 22;
+
+    explicit WfEngine(int dev) : device(dev) {
+        int cus = 0;
+        check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "device attribute");
+        grid = wf_grid(std::max(cus, 1));
+        // 4M slots (about 1 GiB): enough paths in flight that the long random
+        // walks of a frame's last iterations still fill the device.
+        max_slots = env_u32("VVR_PT_SLOTS", 1u << 22);
+        check(hipHostMalloc((void**)&h_live, 2 * sizeof(uint32_t), hipHostMallocDefault), "hipHostMalloc");
+        for (auto& e : live_ev) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    }
+    ~WfEngine() {
+        dfree(slot_mem), dfree(partial), dfree(d_reqs);
+        if (h_reqs) (void)hipHostFree(h_reqs);
+        if (h_live) (void)hipHostFree(h_live);
+        for (auto& e : live_ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+
+    void reserve_slots(uint32_t n) {
+        if (n <= slots) return;
+        dfree(slot_mem);
+        slot_mem = nullptr, slots = 0;
+        constexpr int kF4 = 15, kQueues = 5;
+        const size_t f4_bytes = size_t(n) * sizeof(float4), q_bytes = (size_t(n) * 4 + 255) / 256 * 256;
+        slot_mem = dalloc<uint8_t>(kF4 * f4_bytes + kQueues * q_bytes + 256, "path slots");
+        uint8_t* p = slot_mem;
+        float4** f4s[kF4] = {&S.p_origin, &S.p_dir, &S.p_thr, &S.p_spec, &S.p_rad, &S.e_idx, &S.e_s0, &S.e_s1,
+                             &S.e_s2,     &S.e_n,   &S.e_light, &S.e_lt, &S.c0,   &S.c1,   &S.c2};
+        for (auto f : f4s) *f = reinterpret_cast<float4*>(p), p += f4_bytes;
+        uint32_t** qs[kQueues] = {&Q.ff[0], &Q.ff[1], &Q.ev, &Q.end[0], &Q.end[1]};
+        for (auto q : qs) *q = reinterpret_cast<uint32_t*>(p), p += q_bytes;
+        Q.counts = reinterpret_cast<uint32_t*>(p);
+        slots = n;
+    }
+
+    void reserve(uint32_t n_reqs, size_t partial_f4) {
+        if (partial_f4 > partial_cap) {
+            dfree(partial);
+            partial = nullptr, partial_cap = 0;
+            partial = dalloc<float4>(partial_f4, "path-tracing chunk sums");
+            partial_cap = partial_f4;
+        }
+        if (n_reqs > req_cap) {
+            dfree(d_reqs);
+            if (h_reqs) (void)hipHostFree(h_reqs);
+            d_reqs = nullptr, h_reqs = nullptr, req_cap = 0;
+            d_reqs = dalloc<PtRequest>(n_reqs, "request table");
+            check(hipHostMalloc((void**)&h_reqs, n_reqs * sizeof(PtRequest), hipHostMallocDefault), "hipHostMalloc");
+            req_cap = n_reqs;
+        }
+    }
+
+    // Items of `samples` paths per pixel at `chunk` paths per item.
+    static void shape(PtRequest& q, uint32_t width, uint32_t height, uint32_t first, uint32_t samples,
+                      uint32_t chunk) {
+        q.width = width, q.height = height;
+        q.tiles_x = (width + 7) / 8;
+        q.n_px = q.tiles_x * ((height + 7) / 8) * 64u;
+        q.first_path = first, q.samples = samples;
+        q.chunk = std::min(std::max(chunk, 1u), samples);
+        q.n_chunks = (samples + q.chunk - 1) / q.chunk;
+        q.n_items = q.n_chunks * q.n_px;
+    }
+
+    // Renders the requests (scene, images and shape set) on `st`. Returns once
+    // every path is traced or queued to be; the resolves are queued last.
+    void run(const PtRequest* reqs, uint32_t n, hipStream_t st) {
+        if (n == 0) return;
+        size_t partial_f4 = 0;
+        uint64_t items = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            partial_f4 += size_t(reqs[i].n_chunks) * 3u * reqs[i].n_px;
+            items += reqs[i].n_items;
+        }
+        if (items >= (1ull << 31)) throw Error{VVR_ELIMIT, "too many paths in one frame (2^31 work items)"};
+        reserve(n, partial_f4);
+        const uint32_t want = uint32_t(std::min<uint64_t>(max_slots, (items + 63) / 64 * 64));
+        reserve_slots(want);
+        size_t off = 0;
+        uint32_t base = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            h_reqs[i] = reqs[i];
+            h_reqs[i].partial = partial + off;
+            h_reqs[i].item_base = base;
+            off += size_t(reqs[i].n_chunks) * 3u * reqs[i].n_px;
+            base += reqs[i].n_items;
+        }
+        check(hipMemcpyAsync(d_reqs, h_reqs, n * sizeof(PtRequest), hipMemcpyHostToDevice, st), "request upload");
+        wf_init(S, Q, want, st);
+        check(hipGetLastError(), "wavefront init");
+        // Iterations in batches; the live-slot count after each batch is read
+        // back while the next two batches run, so the device never waits on the host.
+        constexpr uint32_t kBatch = 4;
+        uint32_t iter = 0;
+        auto launch_batch = [&](int k) {
+            for (uint32_t j = 0; j < kBatch; j++, iter++) wf_iteration(d_reqs, n, base, S, Q, iter & 1u, grid, st);
+            check(hipGetLastError(), "wavefront iteration");
+            check(hipMemcpyAsync(&h_live[k], Q.counts + kWfLIVE, 4, hipMemcpyDeviceToHost, st), "live count");
+            check(hipEventRecord(live_ev[k], st), "hipEventRecord");
+        };
+        launch_batch(0);
+        launch_batch(1);
+        for (int k = 0;; k ^= 1) {
+            check(hipEventSynchronize(live_ev[k]), "wavefront progress");
+            if (h_live[k] == 0) break;
+            launch_batch(k);
+        }
+        for (uint32_t i = 0; i < n; i++) wf_resolve(h_reqs[i], st);
+        check(hipGetLastError(), "wavefront resolve");
+        // The staging above is read by copies that have completed (the loop
+        // waited on events recorded after them).
+    }
+};
+
 }  // namespace
 }  // namespace vvr
 
@@ -83,9 +222,13 @@ struct vvr_renderer {
     float4* accum = nullptr;
     float4* albedo = nullptr;
     float4* normal = nullptr;
+    // Path-tracing engine: the wavefront engine (default), or with
+    // VVR_PT_KERNEL=persistent / simple the persistent megakernel or one lane per pixel.
+    enum { kWavefront, kPersistent, kSimple } pt_kernel = kWavefront;
+    std::unique_ptr<WfEngine> wf;
+    uint32_t pt_chunk = 1;       // paths per wavefront work item (VVR_PT_CHUNK)
     uint32_t* queue = nullptr;   // work queue of the persistent path tracer
     uint32_t pt_waves = 0;       // its grid: waves resident on the device
-    bool pt_simple = false;      // VVR_PT_KERNEL=simple: one lane per pixel instead
     uint32_t accumulated = 0;
     bool acc_valid = false;
     vvr_frame_params acc_params{};
@@ -181,6 +324,7 @@ struct vvr_renderer {
             (void)hipEventDestroy(p.b);
         }
         free_frame();
+        wf.reset();
         dfree(density), dfree(normals), dfree(majorant), dfree(labels), dfree(mask), dfree(mask_b), dfree(soft);
         dfree(lut), dfree(preint), dfree(d_uniforms[0]), dfree(d_uniforms[1]), dfree(queue);
         if (h_uniforms) (void)hipHostFree(h_uniforms);
@@ -244,7 +388,9 @@ void new_renderer(vvr_ctx* ctx, const vvr_volume* vol, const vvr_tf* tf, vvr_ren
         check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, r->device), "device attribute");
         r->pt_waves = uint32_t(std::max(cus, 1) * pt_persistent_waves_per_cu());
         const char* k = std::getenv("VVR_PT_KERNEL");
-        r->pt_simple = k && std::strcmp(k, "simple") == 0;
+        if (k && std::strcmp(k, "simple") == 0) r->pt_kernel = vvr_renderer::kSimple;
+        if (k && std::strcmp(k, "persistent") == 0) r->pt_kernel = vvr_renderer::kPersistent;
+        r->pt_chunk = env_u32("VVR_PT_CHUNK", 1);
     }
     r->preint = dalloc<float4>(kLutSize, "pre-integration table");
     r->upload_tf(*tf);
@@ -372,7 +518,14 @@ void record(vvr_renderer* r, uint64_t frame, const vvr_frame_params* p) {
     if (pt) {
         const uint32_t spf = std::max(p->path_tracing.samples_per_frame, 1u);
         const uint32_t first = r->accumulated;
-        if (r->pt_simple) {
+        if (r->pt_kernel == vvr_renderer::kWavefront) {
+            if (!r->wf) r->wf = std::make_unique<WfEngine>(r->device);
+            PtRequest q{};
+            q.scene = sc;
+            q.accum = r->accum, q.albedo = r->albedo, q.normal = r->normal, q.hdr = r->hdr;
+            WfEngine::shape(q, r->width, r->height, first, spf, r->pt_chunk);
+            r->timed(VVR_KERNEL_PT, [&] { r->wf->run(&q, 1, r->stream); });
+        } else if (r->pt_kernel == vvr_renderer::kSimple) {
             r->timed(VVR_KERNEL_PT, [&] {
                 launch_pt(sc, r->accum, r->albedo, r->normal, r->hdr, r->width, r->height, first, spf, r->stream);
             });
