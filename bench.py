@@ -74,6 +74,27 @@ def cpu_baseline(scene, params, seconds: float) -> dict:
             "sample": f"{done} paths (1 per random pixel) of the PT-1 frame, {t_total:.1f} s, FP32 C++ oracle"}
 
 
+def copy_probe(nbytes: int) -> None:
+    """A fill (hipMemset) and a plain device-to-device copy (hipMemcpy) of
+    `nbytes`: kernels of known HBM traffic whose FETCH_SIZE / WRITE_SIZE readings
+    scale the path tracer's under rocprofv3 --pmc."""
+    import ctypes
+    hip = ctypes.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    n = ctypes.c_size_t(nbytes)
+    steps = (("hipMalloc", lambda: hip.hipMalloc(ctypes.byref(src), n)),
+             ("hipMalloc", lambda: hip.hipMalloc(ctypes.byref(dst), n)),
+             ("hipMemset", lambda: hip.hipMemset(src, 1, n)),
+             ("hipMemcpy", lambda: hip.hipMemcpy(dst, src, n, 3)),  # hipMemcpyDeviceToDevice
+             ("hipDeviceSynchronize", lambda: hip.hipDeviceSynchronize()),
+             ("hipFree", lambda: hip.hipFree(src)),
+             ("hipFree", lambda: hip.hipFree(dst)))
+    for what, call in steps:
+        rc = call()
+        if rc != 0:
+            raise RuntimeError(f"copy probe: {what} failed ({rc})")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +105,9 @@ def main() -> int:
     ap.add_argument("--dims", type=int, nargs=3, default=[512, 512, 400])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first copy MIB mebibytes device to device (a plain copy kernel whose HBM counters "
+                         "scale the path tracer's under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU: the renderer has no multi-GPU path to measure "
@@ -95,6 +119,10 @@ def main() -> int:
     scene = workloads.pt_scene(ctx, dims=tuple(args.dims), size=(args.size, args.size), spp=args.spp)
     r = scene.renderer
     print(f"setup {time.perf_counter() - t_setup:.1f} s: {scene.n_labels} organs", file=sys.stderr, flush=True)
+
+    if args.copy_probe:
+        copy_probe(args.copy_probe << 20)
+        print(f"copy probe: {args.copy_probe} MiB", file=sys.stderr, flush=True)
 
     frame = 0
     for _ in range(args.warmup):

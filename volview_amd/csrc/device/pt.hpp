@@ -160,29 +160,59 @@ __device__ inline float extinction_at(const DevScene& sc, V3 idx, V3 dir, bool s
     return s.extinction;
 }
 
+// The loop state of delta and ratio tracking (pt.comp:300-371): the ray in
+// index space, the current distance and far end, and the iteration count (at
+// most 4096).
+struct Flight {
+    V3 origin, d_idx;
+    float tt, t1;
+    int i;
+};
+
+__device__ __forceinline__ Flight flight_begin(const vvr_uniforms* u, V3 origin, V3 dir) {
+    Flight f;
+    f.origin = origin;
+    f.d_idx = to_index(u).mul(dir);
+    float t0;
+    intersect_volume(u, origin, f.d_idx, t0, f.t1);
+    f.tt = t0;
+    f.i = 0;
+    return f;
+}
+
+// One iteration of the tracking loop up to the density sample (pt.comp:302-318,
+// 351-362): the majorant of the brick at f.tt, then either the brick exit (an
+// empty brick, or a tentative distance past the exit: false) or a tentative
+// collision at the new f.tt (true, `majorant` set). Each iteration is one such
+// step for every lane, so the lanes of a wave crossing empty bricks and those
+// sampling share iterations. (Loading the majorants of several bricks ahead
+// along an empty stretch measured slower: the brick exits are ALU-bound.)
+__device__ __forceinline__ bool flight_step(const DevScene& sc, Flight& f, Rng& rng, float& majorant) {
+    const V3 p = f.origin + f.d_idx * f.tt;
+    majorant = flight_majorant(sc, p);
+    const float exit = f.tt + brick_exit(p, f.d_idx) + 1e-3f;
+    if (majorant <= 0.0f) {
+        f.tt = exit;
+        return false;
+    }
+    const float next = f.tt - logf(1.0f - rng.rand()) / majorant;
+    if (next >= exit) {
+        f.tt = exit;
+        return false;
+    }
+    f.tt = next;
+    return true;
+}
+
 // Delta tracking against the brick bounds (pt.comp:300-334).
 __device__ inline bool free_flight(const DevScene& sc, V3 origin, V3 dir, bool smooth_density, int hero, V3& spectral,
                                    V3& idx, Sample& s, int& cap_plane, float& cut, Rng& rng) {
     const vvr_uniforms* u = sc.u;
-    const V3 d_idx = to_index(u).mul(dir);
-    float t0, t1;
-    intersect_volume(u, origin, d_idx, t0, t1);
-    float tt = t0;
-    for (int i = 0; i < 4096 && tt < t1; i++) {
-        const V3 p = origin + d_idx * tt;
-        const float majorant = flight_majorant(sc, p);
-        const float exit = tt + brick_exit(p, d_idx) + 1e-3f;
-        if (majorant <= 0.0f) {
-            tt = exit;
-            continue;
-        }
-        const float next = tt - logf(1.0f - rng.rand()) / majorant;
-        if (next >= exit) {
-            tt = exit;
-            continue;
-        }
-        tt = next;
-        idx = origin + d_idx * tt;
+    Flight f = flight_begin(u, origin, dir);
+    for (; f.i < 4096 && f.tt < f.t1; f.i++) {
+        float majorant;
+        if (!flight_step(sc, f, rng, majorant)) continue;
+        idx = f.origin + f.d_idx * f.tt;
         const float extinction = extinction_at(sc, idx, dir, smooth_density, s, cap_plane, cut);
         if (extinction <= 0.0f) continue;
         const V3 sigma = path_sigma(u, s, extinction);
@@ -199,31 +229,17 @@ __device__ inline bool free_flight(const DevScene& sc, V3 origin, V3 dir, bool s
 // Ratio tracking with Russian roulette (pt.comp:339-371).
 __device__ inline V3 transmittance(const DevScene& sc, V3 origin, V3 dir, Rng& rng) {
     const vvr_uniforms* u = sc.u;
-    const V3 d_idx = to_index(u).mul(dir);
-    float t0, t1;
-    intersect_volume(u, origin, d_idx, t0, t1);
-    float tt = t0;
+    Flight f = flight_begin(u, origin, dir);
     V3 result(1.0f);
-    for (int i = 0; i < 4096 && tt < t1; i++) {
+    for (; f.i < 4096 && f.tt < f.t1; f.i++) {
         const float peak = gmax(result.x, gmax(result.y, result.z));
         if (peak < 0.1f) {
             if (rng.rand() * 0.1f >= peak) return V3(0.0f);
             result = result * (0.1f / peak);
         }
-        const V3 p = origin + d_idx * tt;
-        const float majorant = flight_majorant(sc, p);
-        const float exit = tt + brick_exit(p, d_idx) + 1e-3f;
-        if (majorant <= 0.0f) {
-            tt = exit;
-            continue;
-        }
-        const float next = tt - logf(1.0f - rng.rand()) / majorant;
-        if (next >= exit) {
-            tt = exit;
-            continue;
-        }
-        tt = next;
-        const Sample m = sample_point(sc, origin + d_idx * tt);
+        float majorant;
+        if (!flight_step(sc, f, rng, majorant)) continue;
+        const Sample m = sample_point(sc, f.origin + f.d_idx * f.tt);
         result = result * vmax(1.0f - path_sigma(u, m, m.extinction) / majorant, V3(0.0f));
     }
     return result;
@@ -324,15 +340,9 @@ __device__ __forceinline__ bool path_continues(const FrameConsts& fc, const Path
     return st.surfaces <= fc.max_bounces && st.volumes <= fc.volume_limit;
 }
 
-// pt.comp:433-447: free flight; on escape the environment and the key light
-// (bounce > 0) or the background guides (bounce 0). False once the path ended.
-__device__ __forceinline__ bool stage_extend(const DevScene& sc, const FrameConsts& fc, PathState& st, EventState& ev) {
-    const vvr_uniforms* u = sc.u;
-    ev.cap_plane = -1;
-    ev.cut = 0.0f;
-    if (free_flight(sc, st.origin, st.dir, st.bounce == 0u, st.hero, st.spectral, ev.idx, ev.s, ev.cap_plane, ev.cut,
-                    st.rng))
-        return true;
+// pt.comp:434-447: what a path that left the volume adds: the background guides
+// on bounce 0, else the environment and the key light seen along the ray.
+__device__ __forceinline__ void path_escape(const vvr_uniforms* u, const FrameConsts& fc, PathState& st) {
     if (st.bounce == 0u) {
         st.background = 1.0f;
         st.albedo = V3(1.0f);
@@ -343,6 +353,17 @@ __device__ __forceinline__ bool stage_extend(const DevScene& sc, const FrameCons
         if (fc.area_light && dot(st.dir, fc.ldir) >= fc.cos_light)
             st.radiance = st.radiance + path_weight * fc.light * fc.light_pdf * mis(st.last_pdf, fc.light_pdf);
     }
+}
+
+// pt.comp:433-447: free flight; on escape the environment and the key light
+// (bounce > 0) or the background guides (bounce 0). False once the path ended.
+__device__ __forceinline__ bool stage_extend(const DevScene& sc, const FrameConsts& fc, PathState& st, EventState& ev) {
+    ev.cap_plane = -1;
+    ev.cut = 0.0f;
+    if (free_flight(sc, st.origin, st.dir, st.bounce == 0u, st.hero, st.spectral, ev.idx, ev.s, ev.cap_plane, ev.cut,
+                    st.rng))
+        return true;
+    path_escape(sc.u, fc, st);
     return false;
 }
 
