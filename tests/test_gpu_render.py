@@ -131,6 +131,7 @@ def test_pt_traces_the_oracles_paths(ctx, oracle, labels_on):
                                   ("normal guide", nrm, want[2], 0.03)):
         err = np.abs(got.reshape(ref.shape) - ref)
         off = np.any(err > 1e-3 * (1 + np.abs(ref)), axis=-1)
+        print(f"{what}: {off.mean():.4f} of pixels differ (limit {limit})")
         assert off.mean() < limit, f"{what}: {off.sum()} of {off.size} pixels differ"
     hdr = r.read_hdr().reshape(-1, 4)
     assert np.allclose(hdr, (acc.reshape(-1, 4) / spp).astype(np.float16).astype(np.float32), rtol=2e-3, atol=1e-4)

@@ -165,6 +165,7 @@ __device__ inline float extinction_at(const DevScene& sc, V3 idx, V3 dir, bool s
 // most 4096).
 struct Flight {
     V3 origin, d_idx;
+    V3 inv_d;  // brick_exit_inverse(d_idx)
     float tt, t1;
     int i;
 };
@@ -173,6 +174,7 @@ __device__ __forceinline__ Flight flight_begin(const vvr_uniforms* u, V3 origin,
     Flight f;
     f.origin = origin;
     f.d_idx = to_index(u).mul(dir);
+    f.inv_d = brick_exit_inverse(f.d_idx);
     float t0;
     intersect_volume(u, origin, f.d_idx, t0, f.t1);
     f.tt = t0;
@@ -183,14 +185,11 @@ __device__ __forceinline__ Flight flight_begin(const vvr_uniforms* u, V3 origin,
 // One iteration of the tracking loop up to the density sample (pt.comp:302-318,
 // 351-362): the majorant of the brick at f.tt, then either the brick exit (an
 // empty brick, or a tentative distance past the exit: false) or a tentative
-// collision at the new f.tt (true, `majorant` set). Each iteration is one such
-// step for every lane, so the lanes of a wave crossing empty bricks and those
-// sampling share iterations. (Loading the majorants of several bricks ahead
-// along an empty stretch measured slower: the brick exits are ALU-bound.)
+// collision at the new f.tt (true, `majorant` set).
 __device__ __forceinline__ bool flight_step(const DevScene& sc, Flight& f, Rng& rng, float& majorant) {
     const V3 p = f.origin + f.d_idx * f.tt;
     majorant = flight_majorant(sc, p);
-    const float exit = f.tt + brick_exit(p, f.d_idx) + 1e-3f;
+    const float exit = f.tt + brick_exit_fast(p, f.inv_d) + 1e-3f;
     if (majorant <= 0.0f) {
         f.tt = exit;
         return false;
