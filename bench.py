@@ -95,6 +95,21 @@ def copy_probe(nbytes: int) -> None:
             raise RuntimeError(f"copy probe: {what} failed ({rc})")
 
 
+def measured_traffic(args) -> int | None:
+    """HBM bytes of one path-tracing pass at the default PT-1 configuration, as
+    measured by rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a copy of
+    known size (profiles/pt1_wavefront_hbm_traffic.md); None for other sizes."""
+    if (args.spp, args.size, tuple(args.dims)) != (64, 1024, (512, 512, 400)):
+        return None
+    prof = ROOT / "profiles" / "pt1_wavefront_hbm_traffic.md"
+    if not prof.exists():
+        return None
+    for line in prof.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_pass:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,7 +186,7 @@ def main() -> int:
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic(args),
                      "kernel": "PT pass (k_wf_* iterations + k_wf_resolve)", "kernel_ms": round(avg_kernel_s * 1e3, 3), "algorithmic_bytes": algo},
     }
     if not args.no_cpu:

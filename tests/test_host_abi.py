@@ -122,6 +122,29 @@ def test_uniform_hand_values(oracle):
     assert list(u.flags) == [7, 1, 200, 100]
 
 
+def test_world_to_index_is_the_closed_form_inverse():
+    """volume.rs:23-35: index_to_world = [D diag(s) | o] with D orthonormal, so
+    world_to_index = [diag(1/s) D^T | -diag(1/s) D^T o]. The library inverts the
+    f64 matrix and rounds to f32: equal to the closed form within 1 f32 ulp."""
+    rng = np.random.default_rng(4)
+    hu = np.zeros((5, 6, 7), np.int16)
+    tf = TransferFunction.preset(0)
+    for trial in range(30):
+        q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+        d = q * np.sign(np.diag(r))  # a random rotation or reflection
+        s = rng.uniform(0.2, 3.0, 3)
+        o = rng.uniform(-300.0, 300.0, 3)
+        vol = make_volume(hu, tuple(s), tuple(o), tuple(d.T.reshape(-1)))  # columns = axis directions
+        u = frame_uniforms(vol, tf, frame_params(hu), False, 0, 8, 8, 1)
+        lin = np.diag(1.0 / s) @ d.T
+        want = np.eye(4)
+        want[:3, :3] = lin
+        want[:3, 3] = -lin @ o
+        got = np.array(u.world_to_index, np.float32).reshape(4, 4).T  # stored column-major
+        ulp = np.spacing(np.abs(want.astype(np.float32)))
+        assert np.all(np.abs(got - want) <= np.maximum(ulp, 1e-12)), trial
+
+
 @pytest.mark.parametrize("dims,seed", [((40, 36, 30), 1), ((33, 47, 29), 99)])
 def test_phantom_equals_numpy_twin(dims, seed):
     assert np.array_equal(phantom.ct(dims, seed), phantom.ct_twin(dims, seed))
