@@ -206,7 +206,8 @@ int vvr_frame_uniforms(const vvr_volume* volume, const vvr_tf* tf, const vvr_fra
 
 enum { VVR_ABI_VOLUME = 0, VVR_ABI_CONTROL_POINT = 1, VVR_ABI_TF = 2, VVR_ABI_SOFT_GRID = 3, VVR_ABI_CAMERA = 4,
        VVR_ABI_LABEL_PARAMS = 5, VVR_ABI_LIGHTING = 6, VVR_ABI_CLIPPING = 7, VVR_ABI_PATH_TRACING = 8,
-       VVR_ABI_FRAME_PARAMS = 9, VVR_ABI_UNIFORMS = 10 };
+       VVR_ABI_FRAME_PARAMS = 9, VVR_ABI_UNIFORMS = 10, VVR_ABI_SERVER_CONFIG = 11, VVR_ABI_REQUEST = 12,
+       VVR_ABI_COMPLETION = 13, VVR_ABI_SERVER_STATS = 14 };
 /* sizeof of an ABI struct, for binding checks (0 for an unknown id). */
 size_t vvr_abi_sizeof(int which);
 
@@ -282,6 +283,66 @@ enum { VVR_KERNEL_MAJORANT = 0, VVR_KERNEL_NORMALS = 1, VVR_KERNEL_MASK = 2, VVR
 int vvr_renderer_set_timing(vvr_renderer* r, int enabled);
 /* Total milliseconds and launches of kernel `which` since timing was enabled. */
 int vvr_renderer_kernel_time(vvr_renderer* r, int which, double* total_ms, uint64_t* launches);
+
+/* ---- serving: many requests on one device ------------------------------------
+ * No counterpart in the reference, which renders one frame per renderer at a
+ * time (the plain loop of offscreen.rs:32-36). A server keeps the volumes of
+ * its callers' renderers resident and renders requests of any of them side by
+ * side: ray-marched frames in batched launches, path-traced requests as work
+ * items of one shared wavefront engine that admits new requests while others
+ * are in flight (continuous batching) up to an HBM budget. Each request's result
+ * is the same as rendering it alone on its renderer with samples_per_frame =
+ * pt_increment, frame after frame. A renderer attached to a running request
+ * must not be recorded on, resized, relabelled or destroyed meanwhile. */
+typedef struct vvr_server vvr_server;
+
+typedef struct vvr_server_config {
+    uint64_t hbm_budget;   /* bytes of request images and chunk sums admitted at once (0: 64 GiB) */
+    uint32_t slots;        /* path slots of the wavefront engine (0: 4,194,304) */
+    uint32_t pt_increment; /* paths per pixel per delivered frame of a path-traced request (0: all at once) */
+    uint32_t max_requests; /* requests admitted at once, at most 256 (0: 256) */
+} vvr_server_config;
+
+typedef struct vvr_request {
+    vvr_renderer* volume;            /* its volume, labels and transfer function */
+    const vvr_frame_params* params;  /* path tracing when path_tracing_enabled: samples_per_frame paths per pixel */
+    uint32_t width, height;
+    uint64_t tag;                    /* the caller's, returned in completions */
+} vvr_request;
+
+typedef struct vvr_completion {
+    uint64_t id, tag;
+    int32_t status;        /* VVR_OK, or the error that ended the request */
+    uint32_t spp;          /* paths per pixel in this frame (path tracing), 0 for ray marching */
+    uint32_t final_frame;  /* 1 for the request's last frame */
+    uint32_t pad;
+    uint64_t submit_ns;    /* steady-clock times: submission, admission, frame on the host */
+    uint64_t admit_ns;
+    uint64_t frame_ns;
+} vvr_completion;
+
+typedef struct vvr_server_stats {
+    uint64_t submitted, completed, frames;
+    uint64_t pt_paths;       /* paths traced */
+    uint64_t iterations;     /* wavefront iterations */
+    uint64_t hbm_in_use;     /* bytes charged to admitted requests */
+    uint64_t hbm_peak;
+} vvr_server_stats;
+
+int vvr_server_create(vvr_ctx* ctx, const vvr_server_config* config, vvr_server** out);
+/* Queues a request; returns at once with its id. Thread-safe. */
+int vvr_server_submit(vvr_server* s, const vvr_request* request, uint64_t* id);
+/* Up to `cap` frame completions (in completion order), waiting up to wait_ms
+ * for the first. *n is the number returned. */
+int vvr_server_poll(vvr_server* s, vvr_completion* out, uint32_t cap, uint32_t* n, uint32_t wait_ms);
+/* The request's latest delivered frame as float RGBA (linear radiance +
+ * transmittance or background), width * height * 4 floats. */
+int vvr_server_read_hdr(vvr_server* s, uint64_t id, float* rgba);
+/* Frees a request after its final frame. */
+int vvr_server_release(vvr_server* s, uint64_t id);
+int vvr_server_get_stats(vvr_server* s, vvr_server_stats* out);
+/* Waits for admitted requests to finish, then stops. */
+void vvr_server_destroy(vvr_server* s);
 
 /* ---- synthetic phantoms (workload generator) -------------------------------- */
 

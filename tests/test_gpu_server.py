@@ -1,0 +1,129 @@
+"""The serving loop (vvr_server_*) on the GPU, through the C ABI.
+
+- Each request's frames in a mix of ray-marched and path-traced requests on
+  several volumes equal, bit for bit, the frames of the same request rendered
+  alone by its renderer (path-traced ones with samples_per_frame = the
+  server's increment, frame after frame): work items carry their request, each
+  path its own seed, and each pixel's paths are summed in path order.
+- Progressive requests deliver one frame per increment, in order.
+- A small HBM budget makes requests wait for admission; all still finish and
+  the budget is never exceeded once more than one request is admitted.
+- Requests the renderer would reject are rejected at submission.
+"""
+import numpy as np
+import pytest
+
+from volview_amd import _abi, phantom, workloads
+from volview_amd.renderer import Renderer, TransferFunction
+from volview_amd.server import Server
+
+pytestmark = pytest.mark.gpu
+
+
+def volumes(ctx):
+    hu_a = phantom.ct((48, 40, 36), 3)
+    a = Renderer(ctx, hu_a, TransferFunction.preset(0))
+    dims = (40, 48, 32)
+    hu_b = phantom.ct(dims, 5)
+    labels, soft, _, n = phantom.organs(dims, 5, 12)
+    b = Renderer(ctx, hu_b, TransferFunction.preset(2))
+    b.set_labels(labels, soft)
+    return a, b, n
+
+
+def mixed_requests(a, b, n):
+    dvr_a = workloads.dvr_params(a, 0)
+    pt_a = workloads.pt_params(a, 0, 8, bounces=4, camera_index=1)
+    pt_b = workloads.pt_params(b, n, 12, bounces=4, camera_index=2)
+    dvr_b = workloads.dvr_params(b, 1)
+    workloads.organ_label_params(dvr_b, n, 7)  # same labels as pt_b: one opacity key per volume
+    return [(a, dvr_a, 64, 48), (a, pt_a, 32, 24), (b, pt_b, 24, 32), (b, dvr_b, 40, 40),
+            (a, workloads.dvr_params(a, 3), 17, 29)]
+
+
+def render_alone(r, p, w, h, increment):
+    r.resize(w, h)
+    if not p.path_tracing_enabled:
+        r.record(1, p)
+        return [r.read_hdr()]
+    total = p.path_tracing.samples_per_frame
+    q = _abi.FrameParams.from_buffer_copy(p)
+    q.path_tracing.samples_per_frame = increment
+    frames = []
+    for f in range(total // increment):
+        r.record(f + 1, q)
+        frames.append(r.read_hdr())
+    return frames
+
+
+def drain(srv, ids, timeout_polls=600):
+    got = {i: [] for i in ids}
+    finals = set()
+    for _ in range(timeout_polls):
+        for c in srv.poll(wait_ms=100):
+            assert c.status == 0, c.status
+            got[c.id].append(c)
+            if c.final_frame:
+                finals.add(c.id)
+        if finals == set(ids):
+            return got
+    raise AssertionError(f"requests did not finish: {set(ids) - finals}")
+
+
+def test_served_frames_equal_alone(ctx):
+    a, b, n = volumes(ctx)
+    reqs = mixed_requests(a, b, n)
+    inc = 4
+    srv = Server(ctx, pt_increment=inc)
+    ids = [srv.submit(r, p, w, h, tag=k) for k, (r, p, w, h) in enumerate(reqs)]
+    got = drain(srv, ids)
+    served = {i: srv.read_hdr(i, reqs[k][2], reqs[k][3]) for k, i in enumerate(ids)}
+    for k, i in enumerate(ids):
+        r, p, w, h = reqs[k]
+        spp = [c.spp for c in got[i]]
+        if p.path_tracing_enabled:
+            assert spp == list(range(inc, p.path_tracing.samples_per_frame + 1, inc)), spp
+        else:
+            assert spp == [0]
+        assert all(c.tag == k for c in got[i])
+        srv.release(i)
+    stats = srv.stats()
+    srv.close()
+    assert stats["pt_paths"] == sum(w * h * p.path_tracing.samples_per_frame for _, p, w, h in reqs
+                                    if p.path_tracing_enabled)
+    for k, i in enumerate(ids):
+        r, p, w, h = reqs[k]
+        alone = render_alone(r, p, w, h, inc)[-1]
+        assert np.array_equal(served[i], alone), (k, np.abs(served[i] - alone).max())
+        assert np.isfinite(alone).all() and np.abs(alone).max() > 0
+
+
+def test_budget_queues_requests_and_all_finish(ctx):
+    a, b, n = volumes(ctx)
+    reqs = mixed_requests(a, b, n) * 3
+    one = 32 * 24 * (8 + 48) + 4 * 3 * 32 * 24 * 16  # the larger path-traced request with one increment of chunk sums
+    srv = Server(ctx, pt_increment=4, hbm_budget=2 * one)
+    ids = [srv.submit(r, p, w, h) for r, p, w, h in reqs]
+    drain(srv, ids)
+    stats = srv.stats()
+    assert stats["completed"] == len(ids) and stats["hbm_in_use"] == 0
+    assert stats["hbm_peak"] <= 2 * one
+    for i in ids:
+        srv.release(i)
+    srv.close()
+
+
+def test_rejected_requests(ctx):
+    a, _, _ = volumes(ctx)
+    srv = Server(ctx, hbm_budget=1 << 20)
+    p = workloads.pt_params(a, 0, 4)
+    p.path_tracing.denoiser = _abi.DENOISE_ATROUS
+    with pytest.raises(_abi.VvrError) as e:
+        srv.submit(a, p, 16, 16)
+    assert e.value.code == _abi.VVR_EINVAL
+    with pytest.raises(_abi.VvrError) as e:  # needs more than the whole budget
+        srv.submit(a, workloads.pt_params(a, 0, 4), 1024, 1024)
+    assert e.value.code == _abi.VVR_ELIMIT
+    with pytest.raises(_abi.VvrError):
+        srv.read_hdr(12345, 4, 4)
+    srv.close()

@@ -80,6 +80,16 @@ def random_params(rng, hu):
     return p
 
 
+def test_server_argument_checks():
+    lib = _abi.load()
+    out = ctypes.c_void_p()
+    assert lib.vvr_server_create(None, None, ctypes.byref(out)) == _abi.VVR_EINVAL
+    assert lib.vvr_server_submit(None, None, None) == _abi.VVR_EINVAL
+    assert lib.vvr_server_poll(None, None, 0, None, 0) == _abi.VVR_EINVAL
+    assert lib.vvr_server_release(None, 1) == _abi.VVR_EINVAL
+    lib.vvr_server_destroy(None)  # no-op
+
+
 def test_uniform_packing_equals_oracle(oracle):
     rng = np.random.default_rng(0)
     tf = TransferFunction.preset(0)

@@ -7,7 +7,7 @@ from __future__ import annotations
 
 import ctypes
 from ctypes import POINTER, Structure, c_char_p, c_double, c_float, c_int, c_int16, c_size_t, c_uint8, c_uint16
-from ctypes import c_uint32, c_uint64, c_void_p
+from ctypes import c_int32, c_uint32, c_uint64, c_void_p
 from pathlib import Path
 
 F4x256 = (c_float * 4) * 256
@@ -19,7 +19,7 @@ DENOISE_OFF, DENOISE_ATROUS, DENOISE_LEARNED = 0, 1, 2
 VOL_DENSITY, VOL_LABELS, VOL_NORMALS, VOL_MASK, VOL_MAJORANT = 0, 1, 2, 3, 4
 KERNEL_MAJORANT, KERNEL_NORMALS, KERNEL_MASK, KERNEL_DVR, KERNEL_PT = 0, 1, 2, 3, 4
 ABI_STRUCTS = ['Volume', 'ControlPoint', 'TF', 'SoftGrid', 'Camera', 'LabelParams', 'Lighting', 'Clipping',
-               'PathTracing', 'FrameParams', 'Uniforms']
+               'PathTracing', 'FrameParams', 'Uniforms', 'ServerConfig', 'Request', 'Completion', 'ServerStats']
 
 
 class Volume(Structure):
@@ -97,6 +97,26 @@ class Uniforms(Structure):
 assert ctypes.sizeof(Uniforms) == 21200
 
 # (name, restype, argtypes) of every exported entry point of include/vvr.h.
+class ServerConfig(Structure):
+    _fields_ = [("hbm_budget", c_uint64), ("slots", c_uint32), ("pt_increment", c_uint32), ("max_requests", c_uint32)]
+
+
+class Request(Structure):
+    _fields_ = [("volume", c_void_p), ("params", POINTER(FrameParams)), ("width", c_uint32), ("height", c_uint32),
+                ("tag", c_uint64)]
+
+
+class Completion(Structure):
+    _fields_ = [("id", c_uint64), ("tag", c_uint64), ("status", c_int32), ("spp", c_uint32),
+                ("final_frame", c_uint32), ("pad", c_uint32), ("submit_ns", c_uint64), ("admit_ns", c_uint64),
+                ("frame_ns", c_uint64)]
+
+
+class ServerStats(Structure):
+    _fields_ = [("submitted", c_uint64), ("completed", c_uint64), ("frames", c_uint64), ("pt_paths", c_uint64),
+                ("iterations", c_uint64), ("hbm_in_use", c_uint64), ("hbm_peak", c_uint64)]
+
+
 SIGNATURES = [
     ("vvr_label_params_default", None, [POINTER(LabelParams)]),
     ("vvr_lighting_default", None, [POINTER(Lighting)]),
@@ -137,6 +157,13 @@ SIGNATURES = [
     ("vvr_frame_uniforms", c_int, [POINTER(Volume), POINTER(TF), POINTER(FrameParams), c_int, c_uint32, c_uint32,
                                    c_uint32, c_uint64, c_void_p, c_size_t]),
     ("vvr_abi_sizeof", c_size_t, [c_int]),
+    ("vvr_server_create", c_int, [c_void_p, POINTER(ServerConfig), POINTER(c_void_p)]),
+    ("vvr_server_submit", c_int, [c_void_p, POINTER(Request), POINTER(c_uint64)]),
+    ("vvr_server_poll", c_int, [c_void_p, POINTER(Completion), c_uint32, POINTER(c_uint32), c_uint32]),
+    ("vvr_server_read_hdr", c_int, [c_void_p, c_uint64, POINTER(c_float)]),
+    ("vvr_server_release", c_int, [c_void_p, c_uint64]),
+    ("vvr_server_get_stats", c_int, [c_void_p, POINTER(ServerStats)]),
+    ("vvr_server_destroy", None, [c_void_p]),
     ("vvr_last_error", c_char_p, []),
     ("vvr_version", c_char_p, []),
 ]

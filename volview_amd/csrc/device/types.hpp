@@ -52,6 +52,14 @@ struct DevScene {
     const vvr_uniforms* __restrict__ u;
 };
 
+// One ray-marched frame of a batched launch: its tiles are workgroups
+// tile_base .. tile_base + tiles_x * ceil(height / 8) - 1.
+struct DvrJob {
+    DevScene scene;
+    uint16_t* hdr;
+    uint32_t width, height, tiles_x, tile_base;
+};
+
 // One path-tracing request of the wavefront engine: a frame of `samples` paths
 // per pixel at path indices first_path .. on one scene, cut into work items of
 // `chunk` consecutive paths of one pixel. Items are numbered chunk-major over
@@ -88,14 +96,45 @@ struct WfSlots {
     float4* c2;        //             normal, depth
 };
 
-// Queue indices in WfQueues::counts.
-enum { kWfFF0 = 0, kWfFF1 = 1, kWfEV = 2, kWfEND0 = 3, kWfEND1 = 4, kWfHEAD = 5, kWfLIVE = 6, kWfCounts = 8 };
+// Counters in WfQueues::counts: queue lengths, the item queue head and the live
+// slots of a single run, and the idle queues of the served engine.
+enum {
+    kWfFF0 = 0,
+    kWfFF1 = 1,
+    kWfEV = 2,
+    kWfEND0 = 3,
+    kWfEND1 = 4,
+    kWfHEAD = 5,
+    kWfLIVE = 6,
+    kWfIDLE0 = 7,
+    kWfIDLE1 = 8,
+    kWfCounts = 16
+};
 
 struct WfQueues {
-    uint32_t* ff[2];   // slots to extend
-    uint32_t* ev;      // slots at an event (shade, shadow, scatter)
-    uint32_t* end[2];  // slots whose path ended (or that have no item)
-    uint32_t* counts;  // kWfCounts counters
+    uint32_t* ff[2];    // slots to extend
+    uint32_t* ev;       // slots at an event (shade, shadow, scatter)
+    uint32_t* end[2];   // slots whose path ended (or that have no item)
+    uint32_t* idle[2];  // served engine: slots waiting for work
+    uint32_t* counts;   // kWfCounts counters
+};
+
+// Requests the served engine draws work items from this iteration: entries of
+// its request table, passed by value.
+constexpr int kWfMaxActive = 64;
+struct WfActive {
+    uint32_t n;
+    uint16_t r[kWfMaxActive];
+};
+
+// The served engine's per-request counters (indexed by request-table entry)
+// and which idle queue holds the idle slots; with `revive` set the iteration
+// offers them work.
+struct WfServed {
+    uint32_t* head;  // items taken
+    uint32_t* done;  // items finished
+    uint32_t idle_in;
+    uint32_t revive;
 };
 
 }  // namespace vvr

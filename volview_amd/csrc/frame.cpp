@@ -422,6 +422,10 @@ extern "C" size_t vvr_abi_sizeof(int which) {
         case VVR_ABI_PATH_TRACING: return sizeof(vvr_path_tracing);
         case VVR_ABI_FRAME_PARAMS: return sizeof(vvr_frame_params);
         case VVR_ABI_UNIFORMS: return sizeof(vvr_uniforms);
+        case VVR_ABI_SERVER_CONFIG: return sizeof(vvr_server_config);
+        case VVR_ABI_REQUEST: return sizeof(vvr_request);
+        case VVR_ABI_COMPLETION: return sizeof(vvr_completion);
+        case VVR_ABI_SERVER_STATS: return sizeof(vvr_server_stats);
         default: return 0;
     }
 }

@@ -35,10 +35,8 @@ __device__ __forceinline__ void tile_pixel(uint32_t& px, uint32_t& py) {
 
 // dvr.comp:23-147. Shadow and ambient-occlusion volumes are not built: the host
 // rejects frames that ask for them.
-__global__ void __launch_bounds__(64) k_dvr(DevScene sc, uint16_t* __restrict__ hdr, uint32_t width, uint32_t height) {
-    uint32_t px, py;
-    tile_pixel(px, py);
-    if (px >= width || py >= height) return;
+__device__ __forceinline__ void dvr_pixel(const DevScene& sc, uint32_t px, uint32_t py, uint16_t* __restrict__ hdr,
+                                          uint32_t width) {
     const vvr_uniforms* u = sc.u;
     const V3 dir = camera_ray(u, float(px) + 0.5f, float(py) + 0.5f);
     const V3 o_idx = world_to_index_point(u, V3(u->eye[0], u->eye[1], u->eye[2]));
@@ -119,6 +117,30 @@ __global__ void __launch_bounds__(64) k_dvr(DevScene sc, uint16_t* __restrict__ 
         }
     }
     store_half4(hdr, size_t(py) * width + px, radiance.x, radiance.y, radiance.z, transmittance);
+}
+
+__global__ void __launch_bounds__(64) k_dvr(DevScene sc, uint16_t* __restrict__ hdr, uint32_t width, uint32_t height) {
+    uint32_t px, py;
+    tile_pixel(px, py);
+    if (px >= width || py >= height) return;
+    dvr_pixel(sc, px, py, hdr, width);
+}
+
+// Several ray-marched frames in one launch: one workgroup per 8 x 8 tile of any
+// job, jobs sorted by their first tile.
+__global__ void __launch_bounds__(64) k_dvr_batched(const DvrJob* __restrict__ jobs, uint32_t n_jobs) {
+    uint32_t lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].tile_base <= blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const DvrJob& job = jobs[lo];
+    const uint32_t tile = blockIdx.x - job.tile_base;
+    const uint32_t px = (tile % job.tiles_x) * 8u + (threadIdx.x & 7u);
+    const uint32_t py = (tile / job.tiles_x) * 8u + (threadIdx.x >> 3);
+    if (px >= job.width || py >= job.height) return;
+    dvr_pixel(job.scene, px, py, job.hdr, job.width);
 }
 
 // Adds a pixel's path sums to the accumulation and guide images (restarting
@@ -253,6 +275,11 @@ __global__ void __launch_bounds__(64, VVR_PT_WAVES_PER_EU) k_pt_persistent(DevSc
 void launch_dvr(const DevScene& s, uint16_t* hdr, uint32_t width, uint32_t height, hipStream_t st) {
     const dim3 grid((width + 7) / 8, (height + 7) / 8);
     hipLaunchKernelGGL(k_dvr, grid, dim3(64), 0, st, s, hdr, width, height);
+}
+
+void launch_dvr_batched(const DvrJob* jobs, uint32_t n_jobs, uint32_t total_tiles, hipStream_t st) {
+    if (n_jobs == 0 || total_tiles == 0) return;
+    hipLaunchKernelGGL(k_dvr_batched, dim3(total_tiles), dim3(64), 0, st, jobs, n_jobs);
 }
 
 void launch_pt(const DevScene& s, float4* accum, float4* albedo, float4* normal, uint16_t* hdr, uint32_t width,

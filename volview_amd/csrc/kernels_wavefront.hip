@@ -263,6 +263,102 @@ __global__ void __launch_bounds__(64) k_wf_regen(const PtRequest* __restrict__ r
     }
 }
 
+// Served engine: every slot idle, nothing queued.
+__global__ void k_wf_init_served(WfSlots S, WfQueues Q, uint32_t slots) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kWfCounts) Q.counts[i] = i == kWfIDLE0 ? slots : 0u;
+    if (i >= slots) return;
+    Q.idle[0][i] = i;
+    S.p_spec[i] = make_float4(0.0f, 0.0f, 0.0f, bitsf(kNone));
+    S.p_rad[i] = make_float4(0.0f, 0.0f, 0.0f, bitsf(0u));
+}
+
+// Stage 0 of the served engine: as k_wf_regen, but work items come from the
+// active requests' own counters (a wave tries them in turn from a wave-dependent
+// start, so requests are drawn from side by side), a finished chunk counts
+// towards its request's `done`, and slots without work wait on an idle queue
+// (offered work again when the host revives them) instead of retiring.
+__global__ void __launch_bounds__(64) k_wf_regen_served(const PtRequest* __restrict__ reqs, WfActive A, WfServed V,
+                                                        WfSlots S, WfQueues Q, uint32_t cur) {
+    const uint32_t nxt = cur ^ 1u;
+    const uint32_t idle_out = V.revive ? (V.idle_in ^ 1u) : V.idle_in;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        Q.counts[kWfFF0 + nxt] = 0u;
+        Q.counts[kWfEV] = 0u;
+        Q.counts[kWfEND0 + nxt] = 0u;
+    }
+    const uint32_t n_end = Q.counts[kWfEND0 + cur];
+    const uint32_t n = n_end + (V.revive ? Q.counts[kWfIDLE0 + V.idle_in] : 0u);
+    const uint32_t lane = __lane_id();
+    for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
+        const uint32_t q = base + lane;
+        const bool active = q < n;
+        const uint32_t slot = active ? (q < n_end ? Q.end[cur][q] : Q.idle[V.idle_in][q - n_end]) : 0u;
+        bool push = false, need = false;
+        PathState st;
+        SlotRef ref{kNone, 0u, 0u};
+        float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f), c1 = c0, c2 = c0;
+        if (active) {
+            load_path(S, slot, st, ref);
+            need = true;
+            if (ref.item != kNone) {
+                const PtRequest& R = reqs[ref.req];
+                c0 = S.c0[slot], c1 = S.c1[slot], c2 = S.c2[slot];
+                const V3 r = path_end(st);  // pt.comp:546-551
+                const float lum = dot(r, V3(0.2126f, 0.7152f, 0.0722f));
+                c0 = make_float4(c0.x + r.x, c0.y + r.y, c0.z + r.z, c0.w);
+                c1.w += lum * lum;
+                ref.k++;
+                uint32_t px, py, ci, pi, path_index, n_paths;
+                item_coords(R, ref.item, ref.k, px, py, ci, pi, path_index, n_paths);
+                if (ref.k < n_paths) {
+                    path_begin(R.scene, frame_consts(R.scene.u), px, py, path_index, st);
+                    push = true;
+                    need = false;
+                } else {
+                    store_partial(R, ci, pi, c0, c1, c2);
+                    atomicAdd(&V.done[ref.req], 1u);
+                }
+            }
+        }
+        const uint32_t start = (blockIdx.x + base / (gridDim.x * 64u)) % max(A.n, 1u);
+        uint32_t a = 0;
+        while (a < A.n && __ballot(need)) {
+            const uint32_t r = A.r[(start + a) % A.n];
+            const PtRequest& R = reqs[r];
+            const uint32_t it = wave_take(need, &V.head[r]);
+            const bool got = need && it < R.n_items;
+            const bool dry = need && !got;
+            if (got) {
+                ref.req = r;
+                ref.item = it;
+                ref.k = 0u;
+                uint32_t px, py, ci, pi, path_index, n_paths;
+                item_coords(R, it, 0u, px, py, ci, pi, path_index, n_paths);
+                if (px < R.width && py < R.height) {
+                    c0 = c1 = c2 = make_float4(0.f, 0.f, 0.f, 0.f);
+                    path_begin(R.scene, frame_consts(R.scene.u), px, py, path_index, st);
+                    push = true;
+                    need = false;
+                } else {
+                    atomicAdd(&V.done[r], 1u);  // a padding pixel: nothing to trace
+                }
+            }
+            if (__ballot(dry)) a++;
+        }
+        if (active) {
+            if (push) {
+                store_path(S, slot, st, ref);
+                S.c0[slot] = c0, S.c1[slot] = c1, S.c2[slot] = c2;
+            } else {
+                S.p_spec[slot] = make_float4(0.f, 0.f, 0.f, bitsf(kNone));
+            }
+        }
+        wave_push(push, slot, Q.ff[cur], &Q.counts[kWfFF0 + cur]);
+        wave_push(active && !push, slot, Q.idle[idle_out], &Q.counts[kWfIDLE0 + idle_out]);
+    }
+}
+
 // Stage 1: free flight.
 template <bool kBatched>
 __global__ void __launch_bounds__(64) k_wf_extend(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
@@ -448,6 +544,20 @@ void wf_iteration(const PtRequest* reqs, uint32_t n_reqs, uint32_t total_items, 
         hipLaunchKernelGGL(k_wf_shadow<false>, dim3(g.shadow), dim3(64), 0, st, reqs, S, Q);
         hipLaunchKernelGGL(k_wf_scatter<false>, dim3(g.scatter), dim3(64), 0, st, reqs, S, Q, cur);
     }
+}
+
+void wf_init_served(const WfSlots& S, const WfQueues& Q, uint32_t slots, hipStream_t st) {
+    hipLaunchKernelGGL(k_wf_init_served, dim3((std::max(slots, uint32_t(kWfCounts)) + 255) / 256), dim3(256), 0, st, S,
+                       Q, slots);
+}
+
+void wf_iteration_served(const PtRequest* reqs, const WfActive& A, const WfServed& V, const WfSlots& S,
+                         const WfQueues& Q, uint32_t cur, const WfGrid& g, hipStream_t st) {
+    hipLaunchKernelGGL(k_wf_regen_served, dim3(g.regen), dim3(64), 0, st, reqs, A, V, S, Q, cur);
+    hipLaunchKernelGGL(k_wf_extend<true>, dim3(g.extend), dim3(64), 0, st, reqs, S, Q, cur);
+    hipLaunchKernelGGL(k_wf_shade<true>, dim3(g.shade), dim3(64), 0, st, reqs, S, Q);
+    hipLaunchKernelGGL(k_wf_shadow<true>, dim3(g.shadow), dim3(64), 0, st, reqs, S, Q);
+    hipLaunchKernelGGL(k_wf_scatter<true>, dim3(g.scatter), dim3(64), 0, st, reqs, S, Q, cur);
 }
 
 void wf_resolve(const PtRequest& r, hipStream_t st) {

@@ -35,6 +35,10 @@ void launch_majorant(const uint16_t* density, const uint32_t* mask, float* out, 
 // dvr.comp: hdr = RGBA16F (width x height x 4 halves).
 void launch_dvr(const DevScene& s, uint16_t* hdr, uint32_t width, uint32_t height, hipStream_t st);
 
+// dvr.comp for n_jobs frames in one launch (`jobs` in device memory, sorted by
+// tile_base; total_tiles = the last job's tile_base + its tile count).
+void launch_dvr_batched(const DvrJob* jobs, uint32_t n_jobs, uint32_t total_tiles, hipStream_t st);
+
 // pt.comp for `samples` paths per pixel at path indices first_path ..: adds to
 // accum/albedo/normal (reset when first_path == 0) and writes hdr.
 void launch_pt(const DevScene& s, float4* accum, float4* albedo, float4* normal, uint16_t* hdr, uint32_t width,
@@ -58,6 +62,11 @@ WfGrid wf_grid(int cus);
 void wf_init(const WfSlots& S, const WfQueues& Q, uint32_t slots, hipStream_t st);
 void wf_iteration(const PtRequest* reqs, uint32_t n_reqs, uint32_t total_items, const WfSlots& S, const WfQueues& Q,
                   uint32_t cur, const WfGrid& g, hipStream_t st);
+// Served engine: all slots idle; one iteration drawing work from the active
+// requests of the table `reqs` (see k_wf_regen_served).
+void wf_init_served(const WfSlots& S, const WfQueues& Q, uint32_t slots, hipStream_t st);
+void wf_iteration_served(const PtRequest* reqs, const WfActive& A, const WfServed& V, const WfSlots& S,
+                         const WfQueues& Q, uint32_t cur, const WfGrid& g, hipStream_t st);
 // Chunk sums -> accumulation, guides and HDR images of one request.
 void wf_resolve(const PtRequest& r, hipStream_t st);
 

@@ -1,0 +1,69 @@
+"""The serving loop over libvvr (vvr_server_*): requests of many resident
+volumes rendered side by side on one device.
+
+    srv = Server(ctx, pt_increment=16)
+    rid = srv.submit(renderer, params, 1024, 1024, tag=7)
+    for c in srv.poll(wait_ms=100): ...            # frame completions
+    img = srv.read_hdr(rid, 1024, 1024)
+    srv.release(rid)
+
+A request's frames equal those of its renderer recording the same parameters
+with samples_per_frame = pt_increment, frame after frame (ray-marched frames
+equal one record()). The renderers stay owned by the caller; one attached to a
+request in flight must not be recorded on, resized or relabelled meanwhile.
+"""
+from __future__ import annotations
+
+from ctypes import POINTER, byref, c_float, c_uint32, c_uint64, c_void_p, pointer
+
+import numpy as np
+
+from ._abi import Completion, FrameParams, Request, ServerConfig, ServerStats, check, load
+from .renderer import Context, Renderer
+
+
+class Server:
+    def __init__(self, ctx: Context, hbm_budget: int = 0, slots: int = 0, pt_increment: int = 0,
+                 max_requests: int = 0):
+        self._lib = load()
+        self.ctx = ctx
+        cfg = ServerConfig(hbm_budget, slots, pt_increment, max_requests)
+        h = c_void_p()
+        check(self._lib.vvr_server_create(ctx.handle, byref(cfg), byref(h)))
+        self.handle = h
+
+    def submit(self, renderer: Renderer, params: FrameParams, width: int, height: int, tag: int = 0) -> int:
+        rq = Request(renderer.handle, pointer(params), width, height, tag)  # copied by the call
+        rid = c_uint64()
+        check(self._lib.vvr_server_submit(self.handle, byref(rq), byref(rid)))
+        return int(rid.value)
+
+    def poll(self, cap: int = 256, wait_ms: int = 0) -> list[Completion]:
+        out = (Completion * cap)()
+        n = c_uint32()
+        check(self._lib.vvr_server_poll(self.handle, out, cap, byref(n), wait_ms))
+        return [out[i] for i in range(n.value)]
+
+    def read_hdr(self, rid: int, width: int, height: int) -> np.ndarray:
+        img = np.zeros((height, width, 4), np.float32)
+        check(self._lib.vvr_server_read_hdr(self.handle, rid, img.ctypes.data_as(POINTER(c_float))))
+        return img
+
+    def release(self, rid: int):
+        check(self._lib.vvr_server_release(self.handle, rid))
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        check(self._lib.vvr_server_get_stats(self.handle, byref(s)))
+        return {k: int(getattr(s, k)) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.vvr_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
