@@ -110,6 +110,255 @@ def measured_traffic(args) -> int | None:
     return None
 
 
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def percentiles(xs) -> dict:
+    if not xs:
+        return {"p50": None, "p99": None, "n": 0}
+    a = np.asarray(xs) * 1e3
+    return {"p50": round(float(np.percentile(a, 50)), 2), "p99": round(float(np.percentile(a, 99)), 2), "n": len(xs)}
+
+
+def drain(srv, ids, limit_s: float) -> dict:
+    """Polls until every request in `ids` delivered its final frame; returns the completions per id."""
+    want, got, t_end = set(ids), {}, time.perf_counter() + limit_s
+    while want and time.perf_counter() < t_end:
+        for c in srv.poll(wait_ms=50):
+            got.setdefault(c.id, []).append((c.spp, c.final_frame, c.submit_ns, c.frame_ns, c.status))
+            if c.final_frame:
+                want.discard(c.id)
+    if want:
+        raise RuntimeError(f"{len(want)} requests did not finish within {limit_s} s")
+    return got
+
+
+def run_batched(ctx) -> dict:
+    """Batched-32 rendered one request at a time (record + synchronize each, as
+    the reference's plain loop) and by the server in one submission."""
+    from volview_amd import phantom
+    from volview_amd.renderer import Renderer, TransferFunction
+    from volview_amd.server import Server
+    t = time.perf_counter()
+    vols, reqs = workloads.batched32()
+    rs = [Renderer(ctx, phantom.ct(dims, seed), TransferFunction.preset(0)) for dims, seed in vols]
+    params = [workloads.request_params(rs[q.volume], q) for q in reqs]
+    for q in reqs:
+        rs[q.volume].resize(q.width, q.height)
+    pt_paths = sum(q.width * q.height * q.spp for q in reqs if q.path_traced)
+    log(f"batched-32 setup {time.perf_counter() - t:.1f} s: {sum(q.path_traced for q in reqs)} path-traced, "
+        f"{pt_paths / 1e6:.0f} M paths")
+
+    def plain() -> float:
+        t0 = time.perf_counter()
+        for k, (q, p) in enumerate(zip(reqs, params)):
+            rs[q.volume].record(k + 1, p)
+            rs[q.volume].synchronize()
+        return time.perf_counter() - t0
+
+    plain()  # warm-up: majorants, engines
+    t_plain = plain()
+    log(f"batched-32 plain {t_plain:.2f} s")
+    srv = Server(ctx)
+
+    def served() -> float:
+        t0 = time.perf_counter()
+        ids = [srv.submit(rs[q.volume], p, q.width, q.height) for q, p in zip(reqs, params)]
+        drain(srv, ids, 300.0)
+        dt = time.perf_counter() - t0
+        for i in ids:
+            srv.release(i)
+        return dt
+
+    served()
+    t_served = served()
+    srv.close()
+    log(f"batched-32 served {t_served:.2f} s")
+    for r in rs:
+        r.close()
+    side = lambda sec: {"seconds": round(sec, 3), "pt_msamples_s": round(pt_paths / sec / 1e6, 2),
+                        "requests_per_s": round(len(reqs) / sec, 2)}
+    return {"workload": "Batched-32: 32 seeded phantoms of edge U{128..512}, images U{256..1024}^2, "
+                        "16 DVR (bone preset) + 16 PT (U{1..64} spp, 8 bounces, g=0.8)",
+            "pt_paths": pt_paths, "plain": side(t_plain), "served": side(t_served),
+            "speedup": round(t_plain / t_served, 2)}
+
+
+def run_serving(ctx, seconds: float, grace: float = 5.0, increment: int = 16, slots: int = 1 << 20) -> dict:
+    """The serving trace in real time, twice: by a plain loop (one frame at a
+    time, record + synchronize: ray-marched requests first, then one increment
+    of the oldest progressive request) and by the server (all at once). A new
+    path-traced request on a volume replaces the one in progress there (the
+    viewer moved). Latency: arrival to frame complete (plain: on the device;
+    served: read back to the host). A frame never delivered counts with the
+    time it was waited for: until its request was replaced, or the end.
+    Useful samples: per progressive request, the paths per pixel of the newest
+    frame it delivered before it was replaced (or the end), x 1024^2."""
+    from volview_amd import phantom
+    from volview_amd.renderer import Renderer, TransferFunction
+    from volview_amd.server import Server
+    t = time.perf_counter()
+    vols, reqs = workloads.serving_trace(seconds)
+    pt_r, dvr_r, n_lab = [], [], []
+    for dims, seed in vols:
+        hu = phantom.ct(dims, seed)
+        labels, soft, _, n = phantom.organs(dims, seed, 40)
+        pair = []
+        for _ in range(2):
+            r = Renderer(ctx, hu, TransferFunction.preset(0))
+            r.set_labels(labels, soft)
+            pair.append(r)
+        pt_r.append(pair[0])
+        dvr_r.append(pair[1])
+        n_lab.append(n)
+    params, plain_params = [], []  # served: the whole path budget; plain: one increment per record
+    for q in reqs:
+        p = workloads.request_params(pt_r[q.volume], q, n_lab[q.volume])
+        params.append(p)
+        p = _abi.FrameParams.from_buffer_copy(p)
+        if q.path_traced:
+            p.path_tracing.samples_per_frame = increment
+        plain_params.append(p)
+    for v in range(len(vols)):
+        pt_r[v].resize(1024, 1024)
+        dvr_r[v].resize(1280, 720)
+    # warm-up: masks, majorants, engines
+    for v in range(len(vols)):
+        for r, q in ((pt_r[v], next(q for q in reqs if q.path_traced)), (dvr_r[v], next(q for q in reqs
+                                                                                           if not q.path_traced))):
+            r.record(1, workloads.request_params(r, q, n_lab[v]))
+            r.synchronize()
+    log(f"serving setup {time.perf_counter() - t:.1f} s: {len(reqs)} requests over {seconds} s, "
+        f"{sum(q.path_traced for q in reqs)} path-traced")
+
+    # ---- plain ----
+    from collections import deque
+    dvr_lat, pt_first, pt_paths, frame, useful = [], [], 0, 0, 0
+    dvr_q, pt_fifo, active = deque(), deque(), {}
+    i, t0 = 0, time.perf_counter()
+    while True:
+        now = time.perf_counter() - t0
+        if now >= seconds + grace:
+            break
+        while i < len(reqs) and reqs[i].arrival <= now:
+            q = reqs[i]
+            if q.path_traced:
+                if q.volume not in active:
+                    pt_fifo.append(q.volume)
+                else:
+                    old, old_done = active[q.volume]
+                    useful += old_done
+                    if old_done == 0:  # replaced before its first frame
+                        pt_first.append(q.arrival - reqs[old].arrival)
+                active[q.volume] = [i, 0]
+            else:
+                dvr_q.append(i)
+            i += 1
+        frame += 1
+        if dvr_q:
+            k = dvr_q.popleft()
+            r = dvr_r[reqs[k].volume]
+            r.record(frame, plain_params[k])
+            r.synchronize()
+            dvr_lat.append(time.perf_counter() - t0 - reqs[k].arrival)
+        elif pt_fifo:
+            v = pt_fifo.popleft()
+            k, done = active[v]
+            pt_r[v].record(frame, plain_params[k])
+            pt_r[v].synchronize()
+            done += increment
+            pt_paths += 1024 * 1024 * increment
+            if done == increment:
+                pt_first.append(time.perf_counter() - t0 - reqs[k].arrival)
+            if done < reqs[k].spp:
+                active[v][1] = done
+                pt_fifo.append(v)
+            else:
+                useful += done
+                del active[v]
+        elif i < len(reqs):
+            time.sleep(max(0.0, reqs[i].arrival - now))
+        else:
+            break
+    span = time.perf_counter() - t0
+    dvr_lat += [span - reqs[k].arrival for k in dvr_q]  # still waiting
+    pt_first += [span - reqs[k].arrival for k, done in active.values() if done == 0]
+    useful += sum(done for _, done in active.values())
+    window = seconds + grace
+    plain = {"dvr_latency_ms": percentiles(dvr_lat), "pt_first_frame_ms": percentiles(pt_first),
+             "pt_msamples_s": round(pt_paths / window / 1e6, 2), "pt_paths": pt_paths,
+             "pt_useful_msamples_s": round(useful * 1024 * 1024 / window / 1e6, 2), "busy_seconds": round(span, 2)}
+    log(f"serving plain: {plain}")
+
+    # ---- served ----
+    srv = Server(ctx, pt_increment=increment, slots=slots)
+    ids, kind, current, finals = {}, {}, {}, set()
+    first, dvr_done, pt_paths = {}, {}, 0
+    i, t0, t0_ns = 0, time.perf_counter(), time.monotonic_ns()  # the library stamps frames on the same clock
+    arrival, replaced, frames, history, replaced_at = {}, {}, {}, {}, {}
+
+    def take(cs):
+        nonlocal pt_paths
+        for c in cs:
+            if c.final_frame:
+                finals.add(c.id)
+            if c.status != 0:
+                continue
+            at = (c.frame_ns - t0_ns) / 1e9 - arrival[c.id]  # from the scheduled arrival, as for plain
+            if kind[c.id]:
+                if c.spp > frames.get(c.id, (0, 0))[1]:
+                    pt_paths += 1024 * 1024 * (c.spp - frames.get(c.id, (0, 0))[1])
+                    frames[c.id] = ((c.frame_ns - t0_ns) / 1e9, c.spp)
+                    history.setdefault(c.id, []).append(frames[c.id])
+                first.setdefault(c.id, at)
+            else:
+                dvr_done[c.id] = at
+
+    while True:
+        now = time.perf_counter() - t0
+        if now >= seconds + grace or (i == len(reqs) and len(finals) == len(kind)):
+            break
+        while i < len(reqs) and reqs[i].arrival <= now:
+            q = reqs[i]
+            rid = srv.submit(pt_r[q.volume], params[i], q.width, q.height, tag=i)
+            ids[i], kind[rid], arrival[rid] = rid, q.path_traced, q.arrival
+            if q.path_traced:
+                if q.volume in current:
+                    prev = current[q.volume]
+                    srv.cancel(prev)
+                    replaced_at[prev] = q.arrival
+                    if prev not in first:  # replaced before its first frame
+                        replaced[prev] = q.arrival - arrival[prev]
+                current[q.volume] = rid
+            i += 1
+        wait = reqs[i].arrival - (time.perf_counter() - t0) if i < len(reqs) else 0.02
+        take(srv.poll(wait_ms=max(0, min(int(wait * 1e3), 20))))
+    span = time.perf_counter() - t0
+    for rid in kind:
+        if rid not in finals:
+            srv.cancel(rid)
+    stats = srv.stats()
+    srv_slots = slots
+    srv.close()
+    dvr_lat = [dvr_done.get(ids[k], span - q.arrival) for k, q in enumerate(reqs) if not q.path_traced and k in ids]
+    pt_first = [first[r] if r in first else replaced.get(r, span - arrival[r]) for r in kind if kind[r]]
+    # The newest frame each request delivered before it was replaced (or the end).
+    useful = sum(max([spp for t, spp in history.get(r, []) if t <= replaced_at.get(r, window)], default=0)
+                 for r in kind if kind[r])
+    served = {"dvr_latency_ms": percentiles(dvr_lat), "pt_first_frame_ms": percentiles(pt_first),
+              "pt_msamples_s": round(pt_paths / window / 1e6, 2), "pt_paths": pt_paths,
+              "pt_useful_msamples_s": round(useful * 1024 * 1024 / window / 1e6, 2),
+              "busy_seconds": round(span, 2), "slots": srv_slots, "hbm_peak_bytes": stats["hbm_peak"]}
+    log(f"serving served: {served}")
+    for r in pt_r + dvr_r:
+        r.close()
+    return {"workload": f"Poisson {len(reqs) / seconds:.0f} req/s for {seconds:.0f} s (seed 7): 70% DVR 1280x720, "
+                        f"30% progressive PT 1024^2 in {increment}-spp frames up to 256 spp, a newer PT request "
+                        "on a volume replacing the older; 8 phantoms of edge U{128..512} with 40 organs",
+            "window_seconds": window, "plain": plain, "served": served}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +369,9 @@ def main() -> int:
     ap.add_argument("--dims", type=int, nargs=3, default=[512, 512, 400])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip Batched-32 (plain vs served)")
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="length of the Poisson serving trace (0: skip it)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB mebibytes device to device (a plain copy kernel whose HBM counters "
                          "scale the path tracer's under rocprofv3 --pmc)")
@@ -191,6 +443,11 @@ def main() -> int:
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, scene.params, args.cpu_seconds)
+    r.close()
+    if not args.no_batched:
+        out["batched"] = run_batched(ctx)
+    if args.serve_seconds > 0:
+        out["serving"] = run_serving(ctx, args.serve_seconds)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -32,6 +32,7 @@ extern "C" {
 #define VVR_ENOMEM -3  /* device allocation failed                            */
 #define VVR_EDEVICE -4 /* HIP runtime or kernel launch failure                */
 #define VVR_ESTATE -5  /* API misuse, e.g. record() before resize()           */
+#define VVR_ECANCELED -6 /* a served request cancelled before it rendered     */
 
 typedef struct vvr_ctx vvr_ctx;
 typedef struct vvr_renderer vvr_renderer;
@@ -338,6 +339,12 @@ int vvr_server_poll(vvr_server* s, vvr_completion* out, uint32_t cap, uint32_t* 
 /* The request's latest delivered frame as float RGBA (linear radiance +
  * transmittance or background), width * height * 4 floats. */
 int vvr_server_read_hdr(vvr_server* s, uint64_t id, float* rgba);
+/* Stops a request: one not admitted yet ends with status VVR_ECANCELED. A
+ * path-traced one in flight stops handing out paths at once, abandons the
+ * increment in flight and ends with a final completion whose frame is the last
+ * delivered one (status VVR_ECANCELED if it delivered none). A ray-marched one
+ * in flight finishes. */
+int vvr_server_cancel(vvr_server* s, uint64_t id);
 /* Frees a request after its final frame. */
 int vvr_server_release(vvr_server* s, uint64_t id);
 int vvr_server_get_stats(vvr_server* s, vvr_server_stats* out);

@@ -9,6 +9,8 @@
 - A small HBM budget makes requests wait for admission; all still finish and
   the budget is never exceeded once more than one request is admitted.
 - Requests the renderer would reject are rejected at submission.
+- Cancelling ends a queued request at once, and a path-traced one in flight
+  with its last delivered frame.
 """
 import numpy as np
 import pytest
@@ -56,12 +58,12 @@ def render_alone(r, p, w, h, increment):
     return frames
 
 
-def drain(srv, ids, timeout_polls=600):
+def drain(srv, ids, timeout_polls=600, cancelled=()):
     got = {i: [] for i in ids}
     finals = set()
     for _ in range(timeout_polls):
         for c in srv.poll(wait_ms=100):
-            assert c.status == 0, c.status
+            assert c.status == 0 or (c.id in cancelled and c.status == _abi.VVR_ECANCELED), c.status
             got[c.id].append(c)
             if c.final_frame:
                 finals.add(c.id)
@@ -126,4 +128,36 @@ def test_rejected_requests(ctx):
     assert e.value.code == _abi.VVR_ELIMIT
     with pytest.raises(_abi.VvrError):
         srv.read_hdr(12345, 4, 4)
+    srv.close()
+
+
+def test_cancel_queued_and_running(ctx):
+    a, _, _ = volumes(ctx)
+    srv = Server(ctx, pt_increment=4, max_requests=1)
+    long_pt = workloads.pt_params(a, 0, 400, bounces=4)
+    running = srv.submit(a, long_pt, 32, 32)
+    queued = srv.submit(a, workloads.dvr_params(a, 0), 16, 16)
+    first = None
+    for _ in range(300):
+        cs = srv.poll(wait_ms=100)
+        first = next((c for c in cs if c.id == running), None)
+        if first is not None:
+            break
+    assert first is not None and first.spp == 4 and not first.final_frame
+    srv.cancel(queued)  # max_requests = 1 keeps it queued behind the running request
+    srv.cancel(running)
+    got = drain(srv, [running, queued], cancelled=(queued,))
+    assert [c.status for c in got[queued]] == [_abi.VVR_ECANCELED]
+    last = got[running][-1]
+    delivered = [c.spp for c in got[running][:-1]]
+    # The final completion repeats the last delivered frame (the increment in
+    # flight is abandoned), unless the read-back of a frame was in flight.
+    assert last.final_frame and last.status == 0 and 4 <= last.spp < 400 and last.spp % 4 == 0
+    assert last.spp >= max(delivered, default=4)
+    with pytest.raises(_abi.VvrError):
+        srv.read_hdr(queued, 16, 16)
+    img = srv.read_hdr(running, 32, 32)
+    assert np.isfinite(img).all()
+    srv.release(running)
+    srv.release(queued)
     srv.close()

@@ -12,7 +12,7 @@ from pathlib import Path
 
 F4x256 = (c_float * 4) * 256
 
-VVR_OK, VVR_EINVAL, VVR_ELIMIT, VVR_ENOMEM, VVR_EDEVICE, VVR_ESTATE = 0, -1, -2, -3, -4, -5
+VVR_OK, VVR_EINVAL, VVR_ELIMIT, VVR_ENOMEM, VVR_EDEVICE, VVR_ESTATE, VVR_ECANCELED = 0, -1, -2, -3, -4, -5, -6
 LABELS_OFF, LABELS_TINT, LABELS_ISOLATE, LABELS_HYBRID = 0, 1, 2, 3
 ENV_DOME, ENV_STUDIO, ENV_MAP = 0, 1, 2
 DENOISE_OFF, DENOISE_ATROUS, DENOISE_LEARNED = 0, 1, 2
@@ -161,6 +161,7 @@ SIGNATURES = [
     ("vvr_server_submit", c_int, [c_void_p, POINTER(Request), POINTER(c_uint64)]),
     ("vvr_server_poll", c_int, [c_void_p, POINTER(Completion), c_uint32, POINTER(c_uint32), c_uint32]),
     ("vvr_server_read_hdr", c_int, [c_void_p, c_uint64, POINTER(c_float)]),
+    ("vvr_server_cancel", c_int, [c_void_p, c_uint64]),
     ("vvr_server_release", c_int, [c_void_p, c_uint64]),
     ("vvr_server_get_stats", c_int, [c_void_p, POINTER(ServerStats)]),
     ("vvr_server_destroy", None, [c_void_p]),

@@ -130,9 +130,10 @@ struct WfActive {
 // The served engine's per-request counters (indexed by request-table entry)
 // and which idle queue holds the idle slots; with `revive` set the iteration
 // offers them work.
+constexpr uint32_t kWfCounterStride = 32;  // uint32s between two requests' counters (one 128-byte line each)
 struct WfServed {
-    uint32_t* head;  // items taken
-    uint32_t* done;  // items finished
+    uint32_t* head;  // items taken, at r * kWfCounterStride
+    uint32_t* done;  // items finished, at r * kWfCounterStride
     uint32_t idle_in;
     uint32_t revive;
 };

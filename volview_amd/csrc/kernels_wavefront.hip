@@ -77,6 +77,26 @@ __device__ __forceinline__ uint32_t request_of_item(const PtRequest* __restrict_
     return lo;
 }
 
+// Runs f(r) for the lanes with `on` set, once per distinct request r among
+// them, with r wave-uniform: the request's scene is then read with scalar
+// loads, as in a single-request launch. A wave of one request takes one pass.
+template <bool kBatched, class F>
+__device__ __forceinline__ void for_each_request(bool on, uint32_t req, F&& f) {
+    if (!kBatched) {
+        if (on) f(0u);
+        return;
+    }
+    bool pending = on;
+    for (;;) {
+        const uint64_t m = __ballot(pending);
+        if (!m) break;
+        const uint32_t r = __builtin_amdgcn_readfirstlane(__shfl(req, __ffsll((unsigned long long)m) - 1));
+        const bool mine = pending && req == r;
+        if (mine) f(r);
+        pending = pending && !mine;
+    }
+}
+
 __device__ __forceinline__ uint32_t pack_counters(const PathState& st) {
     return uint32_t(st.hero) | (st.surfaces << 2) | (st.volumes << 8) | (st.bounce << 18);
 }
@@ -274,10 +294,9 @@ __global__ void k_wf_init_served(WfSlots S, WfQueues Q, uint32_t slots) {
 }
 
 // Stage 0 of the served engine: as k_wf_regen, but work items come from the
-// active requests' own counters (a wave tries them in turn from a wave-dependent
-// start, so requests are drawn from side by side), a finished chunk counts
-// towards its request's `done`, and slots without work wait on an idle queue
-// (offered work again when the host revives them) instead of retiring.
+// active requests' own counters, a finished chunk counts towards its request's
+// `done`, and slots without work wait on an idle queue (offered work again when
+// the host revives them) instead of retiring.
 __global__ void __launch_bounds__(64) k_wf_regen_served(const PtRequest* __restrict__ reqs, WfActive A, WfServed V,
                                                         WfSlots S, WfQueues Q, uint32_t cur) {
     const uint32_t nxt = cur ^ 1u;
@@ -298,35 +317,54 @@ __global__ void __launch_bounds__(64) k_wf_regen_served(const PtRequest* __restr
         PathState st;
         SlotRef ref{kNone, 0u, 0u};
         float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f), c1 = c0, c2 = c0;
+        bool finishing = false;
         if (active) {
-            load_path(S, slot, st, ref);
             need = true;
+            ref.item = fbits(S.p_spec[slot].w);  // an idle slot holds no path
             if (ref.item != kNone) {
-                const PtRequest& R = reqs[ref.req];
+                load_path(S, slot, st, ref);
+                finishing = true;
                 c0 = S.c0[slot], c1 = S.c1[slot], c2 = S.c2[slot];
                 const V3 r = path_end(st);  // pt.comp:546-551
                 const float lum = dot(r, V3(0.2126f, 0.7152f, 0.0722f));
                 c0 = make_float4(c0.x + r.x, c0.y + r.y, c0.z + r.z, c0.w);
                 c1.w += lum * lum;
                 ref.k++;
-                uint32_t px, py, ci, pi, path_index, n_paths;
-                item_coords(R, ref.item, ref.k, px, py, ci, pi, path_index, n_paths);
-                if (ref.k < n_paths) {
-                    path_begin(R.scene, frame_consts(R.scene.u), px, py, path_index, st);
-                    push = true;
-                    need = false;
-                } else {
-                    store_partial(R, ci, pi, c0, c1, c2);
-                    atomicAdd(&V.done[ref.req], 1u);
-                }
             }
         }
-        const uint32_t start = (blockIdx.x + base / (gridDim.x * 64u)) % max(A.n, 1u);
+        for_each_request<true>(finishing, ref.req, [&](uint32_t rq) {
+            const PtRequest& R = reqs[rq];
+            uint32_t px, py, ci, pi, path_index, n_paths;
+            item_coords(R, ref.item, ref.k, px, py, ci, pi, path_index, n_paths);
+            const bool chunk_done = ref.k >= n_paths;
+            if (!chunk_done) {
+                path_begin(R.scene, frame_consts(R.scene.u), px, py, path_index, st);
+                push = true;
+                need = false;
+            } else {
+                store_partial(R, ci, pi, c0, c1, c2);
+            }
+            // One atomic per wave and request: per-lane atomics on one counter
+            // serialise millions of updates an iteration.
+            const uint64_t m = __ballot(chunk_done);
+            if (m && int(__lane_id()) == __ffsll((unsigned long long)m) - 1)
+                atomicAdd(&V.done[rq * kWfCounterStride], uint32_t(__popcll(m)));
+        });
+        // New items from the active requests in list order (the host keeps the
+        // list in priority order): a request's work is handed out before the
+        // next one's, and the next one fills the slots its tail leaves idle.
         uint32_t a = 0;
         while (a < A.n && __ballot(need)) {
-            const uint32_t r = A.r[(start + a) % A.n];
+            const uint32_t r = A.r[a];
             const PtRequest& R = reqs[r];
-            const uint32_t it = wave_take(need, &V.head[r]);
+            // A request whose items are all handed out is passed over without an
+            // atomic (its head only grows; the atomic below stays authoritative).
+            uint32_t* head = &V.head[r * kWfCounterStride];
+            if (__hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= R.n_items) {
+                a++;
+                continue;
+            }
+            const uint32_t it = wave_take(need, head);
             const bool got = need && it < R.n_items;
             const bool dry = need && !got;
             if (got) {
@@ -340,10 +378,11 @@ __global__ void __launch_bounds__(64) k_wf_regen_served(const PtRequest* __restr
                     path_begin(R.scene, frame_consts(R.scene.u), px, py, path_index, st);
                     push = true;
                     need = false;
-                } else {
-                    atomicAdd(&V.done[r], 1u);  // a padding pixel: nothing to trace
                 }
             }
+            const uint64_t pad = __ballot(got && need);  // padding pixels: nothing to trace
+            if (pad && int(lane) == __ffsll((unsigned long long)pad) - 1)
+                atomicAdd(&V.done[r * kWfCounterStride], uint32_t(__popcll(pad)));
             if (__ballot(dry)) a++;
         }
         if (active) {
@@ -370,14 +409,15 @@ __global__ void __launch_bounds__(64) k_wf_extend(const PtRequest* __restrict__ 
         const bool active = q < n;
         const uint32_t slot = active ? Q.ff[cur][q] : 0u;
         bool hit = false;
+        PathState st;
+        SlotRef ref{0u, 0u, 0u};
+        EventState ev;
+        if (active) load_path(S, slot, st, ref);
+        for_each_request<kBatched>(active, ref.req, [&](uint32_t r) {
+            const PtRequest& R = reqs[r];
+            hit = stage_extend(R.scene, frame_consts(R.scene.u), st, ev);
+        });
         if (active) {
-            PathState st;
-            SlotRef ref;
-            load_path(S, slot, st, ref);
-            const PtRequest& R = request<kBatched>(reqs, ref.req);
-            const FrameConsts fc = frame_consts(R.scene.u);
-            EventState ev;
-            hit = stage_extend(R.scene, fc, st, ev);
             if (hit) {
                 store_event(S, slot, ev);
             } else {
@@ -408,11 +448,12 @@ __global__ void __launch_bounds__(64) k_wf_shade(const PtRequest* __restrict__ r
         st.rng.state = fbits(d.w);
         st.bounce = fbits(S.p_thr[slot].w) >> 18;
         const uint32_t req = fbits(S.p_rad[slot].w) >> 16;
-        const PtRequest& R = request<kBatched>(reqs, req);
-        const FrameConsts fc = frame_consts(R.scene.u);
         EventState ev;
         load_event(S, slot, ev);
-        stage_shade(R.scene, fc, st, ev);
+        for_each_request<kBatched>(true, req, [&](uint32_t r) {
+            const PtRequest& R = reqs[r];
+            stage_shade(R.scene, frame_consts(R.scene.u), st, ev);
+        });
         store_event(S, slot, ev);
         S.e_n[slot] = f4(ev.n, 0.0f);
         S.e_light[slot] = f4(ev.light_dir, 0.0f);
@@ -431,13 +472,12 @@ __global__ void __launch_bounds__(64) k_wf_shadow(const PtRequest* __restrict__ 
     for (uint32_t q = blockIdx.x * 64u + __lane_id(); q < n; q += gridDim.x * 64u) {
         const uint32_t slot = Q.ev[q];
         const uint32_t req = fbits(S.p_rad[slot].w) >> 16;
-        const PtRequest& R = request<kBatched>(reqs, req);
         PathState st;
         st.rng.state = fbits(S.p_dir[slot].w);
         EventState ev;
         ev.idx = xyz(S.e_idx[slot]);
         ev.light_dir = xyz(S.e_light[slot]);
-        stage_shadow(R.scene, st, ev);
+        for_each_request<kBatched>(true, req, [&](uint32_t r) { stage_shadow(reqs[r].scene, st, ev); });
         S.e_lt[slot] = f4(ev.light_t, 0.0f);
         S.p_dir[slot].w = bitsf(st.rng.state);
     }
@@ -454,20 +494,22 @@ __global__ void __launch_bounds__(64) k_wf_scatter(const PtRequest* __restrict__
         const bool active = q < n;
         const uint32_t slot = active ? Q.ev[q] : 0u;
         bool more = false;
+        PathState st;
+        SlotRef ref{0u, 0u, 0u};
+        EventState ev;
         if (active) {
-            PathState st;
-            SlotRef ref;
             load_path(S, slot, st, ref);
-            const PtRequest& R = request<kBatched>(reqs, ref.req);
-            const FrameConsts fc = frame_consts(R.scene.u);
-            EventState ev;
             load_event(S, slot, ev);
             ev.n = xyz(S.e_n[slot]);
             ev.light_dir = xyz(S.e_light[slot]);
             ev.light_t = xyz(S.e_lt[slot]);
-            more = stage_scatter(R.scene, fc, st, ev) && path_continues(fc, st);
-            store_path(S, slot, st, ref);
         }
+        for_each_request<kBatched>(active, ref.req, [&](uint32_t r) {
+            const PtRequest& R = reqs[r];
+            const FrameConsts fc = frame_consts(R.scene.u);
+            more = stage_scatter(R.scene, fc, st, ev) && path_continues(fc, st);
+        });
+        if (active) store_path(S, slot, st, ref);
         wave_push(active && more, slot, Q.ff[nxt], &Q.counts[kWfFF0 + nxt]);
         wave_push(active && !more, slot, Q.end[nxt], &Q.counts[kWfEND0 + nxt]);
     }

@@ -1,20 +1,25 @@
 // The serving loop: many requests of many resident volumes on one device.
 //
-// One thread owns the device work. Each pass of its loop:
+// One thread owns the device work, all of it on one stream. Each pass of its loop:
 //   1. admits queued requests while the HBM budget, the request table and the
 //      volumes allow (a volume whose label masks / majorant must be rebuilt for a
 //      request is rebuilt only while no admitted request reads it);
 //   2. launches the ray-marched requests admitted since the last pass as one
-//      batched kernel on the ray-marching stream, each followed by its read-back;
-//   3. runs a batch of wavefront iterations for the path-traced requests on the
-//      path-tracing stream (all of them draw work items side by side), and reads
-//      back the requests' finished-item counters two batches behind;
-//   4. resolves each path-traced increment whose items are all finished and
-//      reads its frame back; queues the request's next increment, if any;
+//      batched kernel, each followed by its read-back (ahead of the next
+//      path-tracing batch, so they wait for at most one short batch);
+//   3. runs a short batch of wavefront iterations for the path-traced requests
+//      and reads back every increment's item counters after it;
+//   4. resolves, in order, each request's increments whose items have all
+//      finished, reads the newest frame back, and starts a request's next
+//      increment as soon as the current one has handed out all its items (two
+//      increments of a request may be in flight: one in its tail, one filling
+//      the slots), so a progressive request always offers work;
 //   5. turns read-backs that have landed into completion records.
 // Requests that arrive meanwhile join at the next pass: they are admitted as
 // memory and table entries free up, and their paths share the iterations of
-// the requests already in flight (continuous batching).
+// the requests already in flight (continuous batching). Work items are handed
+// out in priority order: the increment with the lowest index first, then the
+// oldest request, so new requests reach their first frame quickly.
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -29,10 +34,11 @@
 namespace vvr {
 namespace {
 
-constexpr uint32_t kTable = 256;       // request-table entries (path-traced requests in flight)
-constexpr uint32_t kMaxJobs = 256;     // requests admitted at once
-constexpr uint32_t kBatch = 4;         // wavefront iterations per launch batch
-constexpr uint32_t kDvrRing = 8;       // batched ray-marching job tables in flight
+constexpr uint32_t kTable = 256;    // request-table entries (path-traced increments in flight)
+constexpr uint32_t kMaxJobs = 256;  // requests admitted at once
+constexpr uint32_t kBatch = 2;      // wavefront iterations per pass
+constexpr uint32_t kDvrRing = 8;    // batched ray-marching job tables in flight
+constexpr uint32_t kInFlight = 2;   // increments of one request in flight
 
 uint64_t now_ns() {
     return uint64_t(std::chrono::duration_cast<std::chrono::nanoseconds>(
@@ -59,23 +65,49 @@ struct Job {
     vvr_frame_params params{};
     uint32_t width = 0, height = 0;
     bool pt = false;
-    uint32_t spp_total = 0, increment = 0, spp_done = 0, spp_running = 0;
+    // path tracing: spp_total paths per pixel in n_inc increments of `increment`
+    uint32_t spp_total = 0, increment = 0, n_inc = 0;
+    uint32_t inc_started = 0;
+    std::deque<int> entries;  // table entries of the increments in flight, in increment order
+    uint32_t spp_resolved = 0;
+    std::atomic<uint32_t> spp_delivered{0};  // of the newest frame on the host
     uint64_t t_submit = 0, t_admit = 0;
-    enum State { kQueued, kRunning, kReading, kDone } state = kQueued;
-    bool final_read = false;    // the read-back in flight is the last frame
+    enum State { kQueued, kRunning, kDone };
+    std::atomic<State> state{kQueued};
+    bool read_pending = false;  // a frame read-back is in flight
+    uint32_t read_spp = 0;      // paths per pixel of that frame
     int status = VVR_OK;
     std::string error;
+    std::atomic<bool> cancelled{false};
+    bool draining = false;   // cancelled: no new items, waiting for those handed out
+    uint64_t drain_seq = 0;  // first batch that no longer hands out its items
     // device and pinned memory
-    size_t bytes = 0;            // charged to the HBM budget
-    int uslot = -1;              // uniform block
+    size_t bytes = 0;  // charged to the HBM budget
+    int uslot = -1;    // uniform block
     float4 *accum = nullptr, *albedo = nullptr, *normal = nullptr, *partial = nullptr;
     uint16_t* hdr = nullptr;
-    uint16_t* h_hdr = nullptr;   // pinned copy of the latest frame
+    uint16_t* h_hdr = nullptr;  // pinned copy of the newest frame
     hipEvent_t read_ev = nullptr;
-    int entry = -1;              // request-table entry while path tracing
-    uint32_t n_items = 0;        // of the running increment
-    uint64_t start_seq = 0;      // first wavefront batch that sees the increment's counters
 };
+
+// One increment of a path-traced request in the wavefront engine.
+struct Entry {
+    Job* job = nullptr;
+    uint32_t n_items = 0;
+    uint64_t start_seq = 0;    // first batch whose counter read-back sees its reset counters
+    bool handing_out = false;  // on the active list
+    bool complete = false;     // every item finished
+};
+
+size_t chunk_sum_bytes(const Job& j) {  // one increment's chunk sums
+    return size_t(j.increment) * 3u * ((j.width + 7) / 8) * ((j.height + 7) / 8) * 64u * sizeof(float4);
+}
+
+size_t request_bytes(const Job& j) {
+    const size_t px = size_t(j.width) * j.height;
+    if (!j.pt) return px * 8;
+    return px * (8 + 3 * 16) + std::min(j.n_inc, kInFlight) * chunk_sum_bytes(j);
+}
 
 }  // namespace
 }  // namespace vvr
@@ -83,23 +115,23 @@ struct Job {
 struct vvr_server {
     int device = 0;
     vvr_server_config cfg{};
-    hipStream_t s_pt = nullptr, s_dvr = nullptr;
+    hipStream_t stream = nullptr;  // every launch and copy of the server, in order
 
     // wavefront engine
-    WfEngine engine;  // slot arrays and queues
+    WfEngine engine;               // slot arrays and queues
     PtRequest* d_table = nullptr;
-    PtRequest* h_table = nullptr;  // pinned staging, one entry per table entry
-    uint32_t* d_ctr = nullptr;     // head[kTable], done[kTable]
-    uint32_t* h_done[2] = {nullptr, nullptr};
-    hipEvent_t done_ev[2] = {nullptr, nullptr};
-    bool done_pending[2] = {false, false};
+    PtRequest* h_table = nullptr;  // pinned staging, one per table entry
+    uint32_t* d_ctr = nullptr;     // per entry, kWfCounterStride apart: head block, then done block
+    uint32_t* h_ctr = nullptr;     // pinned read-back of both blocks
+    hipEvent_t ctr_ev = nullptr;
     uint64_t batch_seq = 0;        // wavefront batches launched
-    uint64_t snap_seq[2] = {0, 0}; // the batch each counter read-back follows
+    uint64_t snap_seq = 0;         // the batch the read-back follows
     uint32_t cur = 0, idle_in = 0;
     bool revive = false;
-    Job* table[kTable] = {};
+    Entry table[kTable];
     uint32_t active[kWfMaxActive] = {};
     uint32_t n_active = 0;
+    uint32_t entries_in_use = 0;
 
     // ray marching
     DvrJob* d_jobs[kDvrRing] = {};
@@ -113,15 +145,14 @@ struct vvr_server {
     vvr_uniforms* h_uniforms = nullptr;
     std::vector<int> free_uslots;
 
-    // volumes: admitted requests per renderer, and the event after its last rebuild
-    std::map<vvr_renderer*, int> readers;
+    std::map<vvr_renderer*, int> readers;  // admitted requests per volume
 
     // requests
     std::mutex mu;
     std::condition_variable cv_submit, cv_done;
-    std::deque<Job*> queue;                               // submitted, not admitted
+    std::deque<Job*> queue;  // submitted, not admitted
     std::unordered_map<uint64_t, std::unique_ptr<Job>> jobs;
-    std::vector<Job*> running;                            // admitted, not finished
+    std::vector<Job*> running;  // admitted, not finished
     std::deque<vvr_completion> completions;
     uint64_t next_id = 1;
     bool stopping = false;
@@ -133,28 +164,25 @@ struct vvr_server {
 
     void loop();
     void admit(std::vector<Job*>& fresh_dvr);
-    void start_increment(Job* j);
+    bool start_increment(Job* j);
     void launch_dvr(const std::vector<Job*>& batch);
     void pt_batch();
-    void check_increments(int k);
+    void check_entries();
+    void advance_jobs();
+    void stop_cancelled();
     void poll_reads();
-    void finish(Job* j, int status, const std::string& msg);
+    void read_back(Job* j, uint32_t spp);
+    void drop_active(uint32_t e);
+    void free_entry(int e);
+    void finish(Job* j, int status, const std::string& msg, const vvr_completion* last = nullptr);
     void release_memory(Job* j);
-    bool idle_locked() const { return queue.empty() && running.empty(); }
+    vvr_completion record(const Job* j, int status, uint32_t spp, bool final_frame) const {
+        vvr_completion c{};
+        c.id = j->id, c.tag = j->tag, c.status = status, c.spp = spp, c.final_frame = final_frame ? 1u : 0u;
+        c.submit_ns = j->t_submit, c.admit_ns = j->t_admit, c.frame_ns = now_ns();
+        return c;
+    }
 };
-
-namespace vvr {
-namespace {
-
-size_t frame_bytes(const Job& j, uint32_t chunk_count) {
-    const size_t px = size_t(j.width) * j.height;
-    const size_t tiles = size_t((j.width + 7) / 8) * ((j.height + 7) / 8) * 64u;
-    if (!j.pt) return px * 8;
-    return px * (8 + 3 * 16) + size_t(chunk_count) * 3u * tiles * 16u;
-}
-
-}  // namespace
-}  // namespace vvr
 
 void vvr_server::release_memory(Job* j) {
     dfree(j->accum), dfree(j->albedo), dfree(j->normal), dfree(j->partial), dfree(j->hdr);
@@ -170,19 +198,18 @@ void vvr_server::release_memory(Job* j) {
     if (j->vol) readers[j->vol]--;
 }
 
-void vvr_server::finish(Job* j, int status, const std::string& msg) {
+// Ends a request: frees its device memory, marks it done and publishes its
+// final completion (`last`, or an error record) in the same step, so a caller
+// that sees the completion finds the request finished.
+void vvr_server::finish(Job* j, int status, const std::string& msg, const vvr_completion* last) {
     release_memory(j);
     std::lock_guard<std::mutex> lk(mu);
     j->state = Job::kDone;
     j->status = status;
     j->error = msg;
     running.erase(std::remove(running.begin(), running.end(), j), running.end());
-    if (status != VVR_OK) {
-        vvr_completion c{};
-        c.id = j->id, c.tag = j->tag, c.status = status, c.final_frame = 1;
-        c.submit_ns = j->t_submit, c.admit_ns = j->t_admit, c.frame_ns = now_ns();
-        completions.push_back(c);
-    }
+    completions.push_back(last ? *last : record(j, status, 0, true));
+    if (last) stats.frames++;
     stats.completed++;
     cv_done.notify_all();
 }
@@ -196,19 +223,10 @@ void vvr_server::admit(std::vector<Job*>& fresh_dvr) {
         candidates.assign(queue.begin(), queue.end());
     }
     for (Job* j : candidates) {
-        const size_t need = frame_bytes(*j, j->pt ? (j->increment + 0) : 0);
+        const size_t need = request_bytes(*j);
         if (running.size() >= cfg.max_requests || free_uslots.empty()) break;
         if (stats.hbm_in_use + need > cfg.hbm_budget && !running.empty()) continue;
-        if (j->pt && n_active >= uint32_t(kWfMaxActive)) continue;
-        int entry = -1;
-        if (j->pt) {
-            for (uint32_t e = 0; e < kTable; e++)
-                if (!table[e]) {
-                    entry = int(e);
-                    break;
-                }
-            if (entry < 0) continue;
-        }
+        if (j->pt && (entries_in_use >= kTable || n_active >= uint32_t(kWfMaxActive))) continue;
         vvr_renderer* v = j->vol;
         const OpacityKey key = opacity_key(j->params.labels, v->generation, v->has_labels, v->geo.min_spacing);
         const bool rebuild = !v->mask_key || !v->majorant_key || *v->majorant_key != key ||
@@ -228,8 +246,8 @@ void vvr_server::admit(std::vector<Job*>& fresh_dvr) {
                 j->accum = dalloc<float4>(px, "request accumulation");
                 j->albedo = dalloc<float4>(px, "request albedo guide");
                 j->normal = dalloc<float4>(px, "request normal guide");
-                const size_t tiles = size_t((j->width + 7) / 8) * ((j->height + 7) / 8) * 64u;
-                j->partial = dalloc<float4>(size_t(j->increment) * 3u * tiles, "request chunk sums");
+                j->partial = reinterpret_cast<float4*>(
+                    dalloc<uint8_t>(std::min(j->n_inc, kInFlight) * chunk_sum_bytes(*j), "request chunk sums"));
             }
             j->h_hdr = halloc<uint16_t>(4 * px, "frame read-back");
             if (!j->read_ev) check(hipEventCreateWithFlags(&j->read_ev, hipEventDisableTiming), "hipEventCreate");
@@ -243,57 +261,68 @@ void vvr_server::admit(std::vector<Job*>& fresh_dvr) {
             vvr_uniforms& u = h_uniforms[j->uslot];
             pack_uniforms(v->geo, j->params, v->tf.bucket_max, v->has_labels, 0, j->width, j->height, j->id, &u);
             vvr_uniforms* du = d_uniforms + j->uslot;
-            check(hipMemcpyAsync(du, &u, sizeof(u), hipMemcpyHostToDevice, s_dvr), "uniform upload");
-            if (rebuild) {
-                v->prepare(j->params, du, s_dvr);
-                // Path tracing reads the rebuilt volumes too.
-                hipEvent_t ev;
-                check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
-                check(hipEventRecord(ev, s_dvr), "hipEventRecord");
-                check(hipStreamWaitEvent(s_pt, ev, 0), "hipStreamWaitEvent");
-                (void)hipEventDestroy(ev);
-            }
+            check(hipMemcpyAsync(du, &u, sizeof(u), hipMemcpyHostToDevice, stream), "uniform upload");
+            if (rebuild) v->prepare(j->params, du, stream);
             j->t_admit = now_ns();
             j->state = Job::kRunning;
-            if (j->pt) {
-                // The uniform block was written on the ray-marching stream.
-                hipEvent_t ev;
-                check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
-                check(hipEventRecord(ev, s_dvr), "hipEventRecord");
-                check(hipStreamWaitEvent(s_pt, ev, 0), "hipStreamWaitEvent");
-                (void)hipEventDestroy(ev);
-                j->entry = entry;
-                table[entry] = j;
-                start_increment(j);
-            } else {
-                fresh_dvr.push_back(j);
-            }
+            if (j->pt) start_increment(j);
+            else fresh_dvr.push_back(j);
         } catch (const Error& e) {
             finish(j, e.code, e.msg);
         }
     }
 }
 
-// The next increment of a path-traced request: a table entry with fresh counters.
-void vvr_server::start_increment(Job* j) {
-    const uint32_t first = j->spp_done;
+// Starts a request's next increment in a free table entry: fresh counters, its
+// half of the chunk-sum ring, on the active list. False when no entry is free.
+bool vvr_server::start_increment(Job* j) {
+    int e = -1;
+    for (uint32_t k = 0; k < kTable; k++)
+        if (!table[k].job) {
+            e = int(k);
+            break;
+        }
+    if (e < 0 || n_active >= uint32_t(kWfMaxActive)) return false;
+    const uint32_t k = j->inc_started;
+    const uint32_t first = k * j->increment;
     const uint32_t n = std::min(j->increment, j->spp_total - first);
     PtRequest q{};
     q.scene = j->vol->scene(d_uniforms + j->uslot);
     q.accum = j->accum, q.albedo = j->albedo, q.normal = j->normal, q.hdr = j->hdr;
     WfEngine::shape(q, j->width, j->height, first, n, 1);
-    q.partial = j->partial;
+    q.partial =
+        reinterpret_cast<float4*>(reinterpret_cast<uint8_t*>(j->partial) + (k % kInFlight) * chunk_sum_bytes(*j));
     q.item_base = 0;
-    const int e = j->entry;
     h_table[e] = q;
-    check(hipMemcpyAsync(d_table + e, h_table + e, sizeof(PtRequest), hipMemcpyHostToDevice, s_pt), "request upload");
-    check(hipMemsetAsync(d_ctr + e, 0, 4, s_pt), "counter reset");
-    check(hipMemsetAsync(d_ctr + kTable + e, 0, 4, s_pt), "counter reset");
-    j->n_items = q.n_items;
-    j->spp_running = n;
-    j->start_seq = batch_seq;  // counter snapshots from earlier batches predate the reset
+    check(hipMemcpyAsync(d_table + e, h_table + e, sizeof(PtRequest), hipMemcpyHostToDevice, stream), "request upload");
+    check(hipMemsetAsync(d_ctr + e * kWfCounterStride, 0, 4, stream), "counter reset");
+    check(hipMemsetAsync(d_ctr + (kTable + e) * kWfCounterStride, 0, 4, stream), "counter reset");
+    Entry& en = table[e];
+    en.job = j;
+    en.n_items = q.n_items;
+    en.start_seq = batch_seq;  // counter read-backs from earlier batches predate the reset
+    en.handing_out = true;
+    en.complete = false;
     active[n_active++] = uint32_t(e);
+    entries_in_use++;
+    j->entries.push_back(e);
+    j->inc_started++;
     revive = true;
+    return true;
+}
+
+void vvr_server::drop_active(uint32_t e) {
+    for (uint32_t i = 0; i < n_active; i++)
+        if (active[i] == e) {
+            active[i] = active[--n_active];
+            return;
+        }
+}
+
+void vvr_server::free_entry(int e) {
+    drop_active(uint32_t(e));
+    table[e] = Entry{};
+    entries_in_use--;
 }
 
 void vvr_server::launch_dvr(const std::vector<Job*>& batch) {
@@ -313,37 +342,46 @@ void vvr_server::launch_dvr(const std::vector<Job*>& batch) {
             d.tile_base = tiles;
             tiles += d.tiles_x * ((j->height + 7) / 8);
         }
-        check(hipMemcpyAsync(d_jobs[k], h_jobs[k], nb * sizeof(DvrJob), hipMemcpyHostToDevice, s_dvr), "job upload");
-        launch_dvr_batched(d_jobs[k], uint32_t(nb), tiles, s_dvr);
+        check(hipMemcpyAsync(d_jobs[k], h_jobs[k], nb * sizeof(DvrJob), hipMemcpyHostToDevice, stream), "job upload");
+        launch_dvr_batched(d_jobs[k], uint32_t(nb), tiles, stream);
         check(hipGetLastError(), "batched ray marching");
-        check(hipEventRecord(jobs_ev[k], s_dvr), "hipEventRecord");
+        check(hipEventRecord(jobs_ev[k], stream), "hipEventRecord");
         jobs_used[k] = true;
-        for (size_t i = 0; i < nb; i++) {
-            Job* j = batch[b0 + i];
-            const size_t bytes = size_t(j->width) * j->height * 8;
-            check(hipMemcpyAsync(j->h_hdr, j->hdr, bytes, hipMemcpyDeviceToHost, s_dvr), "frame read-back");
-            check(hipEventRecord(j->read_ev, s_dvr), "hipEventRecord");
-            j->final_read = true;
-            j->state = Job::kReading;
-        }
+        for (size_t i = 0; i < nb; i++) read_back(batch[b0 + i], 0);
     }
 }
 
-// A batch of wavefront iterations over the active requests, then the
-// finished-item counters into read-back slot `k`.
+// Copies a request's newest frame to its pinned buffer, after every earlier
+// launch on the stream (and after any earlier read-back into the same buffer).
+void vvr_server::read_back(Job* j, uint32_t spp) {
+    const size_t bytes = size_t(j->width) * j->height * 8;
+    check(hipMemcpyAsync(j->h_hdr, j->hdr, bytes, hipMemcpyDeviceToHost, stream), "frame read-back");
+    check(hipEventRecord(j->read_ev, stream), "hipEventRecord");
+    j->read_pending = true;
+    j->read_spp = spp;
+}
+
+// A short batch of wavefront iterations over the increments handing out work.
+// Priority: the lowest increment index first (a new request's first frame
+// before later frames of others), then the oldest request.
 void vvr_server::pt_batch() {
+    auto key = [&](uint32_t e) {
+        const Job* j = table[e].job;
+        return std::make_pair(j->inc_started, j->id);
+    };
+    std::sort(active, active + n_active, [&](uint32_t a, uint32_t b) { return key(a) < key(b); });
     WfActive A{};
     A.n = n_active;
     for (uint32_t i = 0; i < n_active; i++) A.r[i] = uint16_t(active[i]);
     WfServed V{};
     V.head = d_ctr;
-    V.done = d_ctr + kTable;
+    V.done = d_ctr + kTable * kWfCounterStride;
     for (uint32_t it = 0; it < kBatch; it++) {
         V.idle_in = idle_in;
         V.revive = revive ? 1u : 0u;
         if (revive)
-            check(hipMemsetAsync(engine.Q.counts + kWfIDLE0 + (idle_in ^ 1u), 0, 4, s_pt), "idle queue reset");
-        wf_iteration_served(d_table, A, V, engine.S, engine.Q, cur, engine.grid, s_pt);
+            check(hipMemsetAsync(engine.Q.counts + kWfIDLE0 + (idle_in ^ 1u), 0, 4, stream), "idle queue reset");
+        wf_iteration_served(d_table, A, V, engine.S, engine.Q, cur, engine.grid, stream);
         check(hipGetLastError(), "wavefront iteration");
         if (revive) idle_in ^= 1u;
         revive = false;
@@ -354,39 +392,84 @@ void vvr_server::pt_batch() {
     stats.iterations += kBatch;
 }
 
-// Read-back slot `k` (finished-item counters) has landed: resolve and read back
-// every increment whose items are all finished; queue the next increments.
-void vvr_server::check_increments(int k) {
-    const uint32_t* done = h_done[k];
-    std::vector<Job*> ended;
-    for (uint32_t i = 0; i < n_active; i++) {
-        Job* j = table[active[i]];
-        if (j && snap_seq[k] >= j->start_seq && done[active[i]] >= j->n_items) ended.push_back(j);
-    }
-    for (Job* j : ended) {
-        const uint32_t e = uint32_t(j->entry);
-        for (uint32_t i = 0; i < n_active; i++)
-            if (active[i] == e) {
-                active[i] = active[--n_active];
-                break;
-            }
-        wf_resolve(h_table[e], s_pt);
-        check(hipGetLastError(), "resolve");
-        j->spp_done += j->spp_running;
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            stats.pt_paths += uint64_t(j->width) * j->height * j->spp_running;
+// The counters have landed: increments that handed out all their items leave
+// the active list; increments whose items all finished are complete.
+void vvr_server::check_entries() {
+    const uint32_t* head = h_ctr;
+    const uint32_t* done = h_ctr + kTable * kWfCounterStride;
+    for (uint32_t e = 0; e < kTable; e++) {
+        Entry& en = table[e];
+        if (!en.job || snap_seq < en.start_seq) continue;
+        if (en.handing_out && head[e * kWfCounterStride] >= en.n_items) {
+            en.handing_out = false;
+            drop_active(e);
         }
-        const size_t bytes = size_t(j->width) * j->height * 8;
-        // A previous frame's read-back into the same pinned buffer has landed
-        // (increments are read back in order on this stream).
-        check(hipMemcpyAsync(j->h_hdr, j->hdr, bytes, hipMemcpyDeviceToHost, s_pt), "frame read-back");
-        check(hipEventRecord(j->read_ev, s_pt), "hipEventRecord");
-        j->final_read = j->spp_done >= j->spp_total;
-        j->state = Job::kReading;
-        if (j->final_read) {
-            table[e] = nullptr;
-            j->entry = -1;
+        if (!en.complete && done[e * kWfCounterStride] >= en.n_items) en.complete = true;
+    }
+}
+
+// Per path-traced request: resolve finished increments in order and read the
+// newest frame back; start the next increment once none in flight is still
+// handing out items; end cancelled requests once their paths have drained.
+void vvr_server::advance_jobs() {
+    const uint32_t* head = h_ctr;
+    const uint32_t* done = h_ctr + kTable * kWfCounterStride;
+    std::vector<Job*> snapshot;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        snapshot = running;
+    }
+    for (Job* j : snapshot) {
+        if (!j->pt) continue;
+        if (j->draining) {
+            bool drained = j->entries.empty() || snap_seq >= j->drain_seq;
+            for (int e : j->entries) {
+                const uint32_t handed = std::min(head[e * kWfCounterStride], table[e].n_items);
+                drained = drained && done[e * kWfCounterStride] >= handed;
+            }
+            if (drained && !j->read_pending) {
+                for (int e : j->entries) free_entry(e);
+                j->entries.clear();
+                const uint32_t spp = j->spp_delivered;
+                const vvr_completion c = record(j, spp > 0 ? VVR_OK : VVR_ECANCELED, spp, true);
+                finish(j, c.status, spp > 0 ? "" : "cancelled", &c);
+            }
+            continue;
+        }
+        bool resolved = false;
+        while (!j->entries.empty() && table[j->entries.front()].complete) {
+            const int e = j->entries.front();
+            j->entries.pop_front();
+            wf_resolve(h_table[e], stream);
+            check(hipGetLastError(), "resolve");
+            const uint32_t n = h_table[e].samples;
+            free_entry(e);
+            j->spp_resolved += n;
+            resolved = true;
+            std::lock_guard<std::mutex> lk(mu);
+            stats.pt_paths += uint64_t(j->width) * j->height * n;
+        }
+        if (resolved) read_back(j, j->spp_resolved);
+        const bool busy = std::any_of(j->entries.begin(), j->entries.end(), [&](int e) { return table[e].handing_out; });
+        if (!j->cancelled && !busy && j->inc_started < j->n_inc && j->entries.size() < kInFlight) start_increment(j);
+    }
+}
+
+// Cancelled path-traced requests stop handing out work at once; advance_jobs
+// ends them when the paths already handed out have finished.
+void vvr_server::stop_cancelled() {
+    std::vector<Job*> snapshot;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        snapshot = running;
+    }
+    for (Job* j : snapshot) {
+        if (!j->pt || !j->cancelled || j->draining) continue;
+        j->draining = true;
+        j->drain_seq = batch_seq;
+        for (int e : j->entries) {
+            table[e].handing_out = false;
+            drop_active(uint32_t(e));
         }
     }
 }
@@ -399,26 +482,21 @@ void vvr_server::poll_reads() {
         snapshot = running;
     }
     for (Job* j : snapshot) {
-        if (j->state != Job::kReading) continue;
+        if (!j->read_pending) continue;
         const hipError_t q = hipEventQuery(j->read_ev);
         if (q == hipErrorNotReady) continue;
         check(q, "frame read-back");
-        vvr_completion c{};
-        c.id = j->id, c.tag = j->tag, c.status = VVR_OK;
-        c.spp = j->pt ? j->spp_done : 0u;
-        c.final_frame = j->final_read ? 1u : 0u;
-        c.submit_ns = j->t_submit, c.admit_ns = j->t_admit, c.frame_ns = now_ns();
-        {
+        j->read_pending = false;
+        j->spp_delivered = j->read_spp;
+        const bool last = !j->pt || j->read_spp >= j->spp_total;
+        const vvr_completion c = record(j, VVR_OK, j->read_spp, last);
+        if (last) {
+            finish(j, VVR_OK, "", &c);
+        } else {
             std::lock_guard<std::mutex> lk(mu);
             completions.push_back(c);
             stats.frames++;
             cv_done.notify_all();
-        }
-        if (j->final_read) {
-            finish(j, VVR_OK, "");
-        } else {
-            j->state = Job::kRunning;
-            start_increment(j);
         }
     }
 }
@@ -426,7 +504,6 @@ void vvr_server::poll_reads() {
 void vvr_server::loop() {
     try {
         check(hipSetDevice(device), "hipSetDevice");
-        int slot_k = 0;  // read-back slot of the next batch
         for (;;) {
             {
                 std::unique_lock<std::mutex> lk(mu);
@@ -439,44 +516,33 @@ void vvr_server::loop() {
             std::vector<Job*> fresh_dvr;
             admit(fresh_dvr);
             if (!fresh_dvr.empty()) launch_dvr(fresh_dvr);
-            if (n_active > 0) {
+            stop_cancelled();
+            if (entries_in_use > 0) {
+                // One short batch per pass, waited for before the next pass: a
+                // ray-marched request admitted meanwhile waits for at most this
+                // batch (the host's turn-around between batches is ~0.1 ms).
                 pt_batch();
-                check(hipMemcpyAsync(h_done[slot_k], d_ctr + kTable, kTable * 4, hipMemcpyDeviceToHost, s_pt),
+                check(hipMemcpyAsync(h_ctr, d_ctr, 2 * kTable * kWfCounterStride * 4, hipMemcpyDeviceToHost, stream),
                       "counter read-back");
-                snap_seq[slot_k] = batch_seq - 1;
-                check(hipEventRecord(done_ev[slot_k], s_pt), "hipEventRecord");
-                done_pending[slot_k] = true;
-                slot_k ^= 1;
-                // The other slot's counters are one batch older: wait for them.
-                if (done_pending[slot_k]) {
-                    check(hipEventSynchronize(done_ev[slot_k]), "counter read-back");
-                    done_pending[slot_k] = false;
-                    check_increments(slot_k);
-                }
+                snap_seq = batch_seq - 1;
+                check(hipEventRecord(ctr_ev, stream), "hipEventRecord");
+                check(hipEventSynchronize(ctr_ev), "counter read-back");
+                check_entries();
             } else {
-                // Nothing to trace: drain counter read-backs still in flight.
-                for (int k = 0; k < 2; k++)
-                    if (done_pending[k]) {
-                        check(hipEventSynchronize(done_ev[k]), "counter read-back");
-                        done_pending[k] = false;
-                    }
                 bool reading = false;
                 {
                     std::lock_guard<std::mutex> lk(mu);
-                    for (Job* j : running) reading |= j->state == Job::kReading;
+                    for (Job* j : running) reading |= j->read_pending;
                 }
                 if (reading) std::this_thread::sleep_for(std::chrono::microseconds(50));
             }
+            advance_jobs();
             poll_reads();
         }
     } catch (const Error& e) {
         std::lock_guard<std::mutex> lk(mu);
         fatal = e.msg;
-        for (Job* j : running) {
-            vvr_completion c{};
-            c.id = j->id, c.tag = j->tag, c.status = e.code, c.final_frame = 1;
-            completions.push_back(c);
-        }
+        for (Job* j : running) completions.push_back(record(j, e.code, 0, true));
         cv_done.notify_all();
     }
 }
@@ -493,18 +559,15 @@ void create_server(vvr_ctx* ctx, const vvr_server_config* cfg, vvr_server** out)
     if (!c.max_requests || c.max_requests > kMaxJobs) c.max_requests = kMaxJobs;
     s->cfg = c;
     s->engine.max_slots = c.slots ? c.slots : s->engine.max_slots;
-    check(hipStreamCreateWithFlags(&s->s_pt, hipStreamNonBlocking), "hipStreamCreate");
-    check(hipStreamCreateWithFlags(&s->s_dvr, hipStreamNonBlocking), "hipStreamCreate");
+    check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
     s->engine.reserve_slots(s->engine.max_slots);
     s->engine.reserve(1, 1);
     s->d_table = dalloc<PtRequest>(kTable, "request table");
     s->h_table = halloc<PtRequest>(kTable, "request table staging");
-    s->d_ctr = dalloc<uint32_t>(2 * kTable, "request counters");
-    check(hipMemsetAsync(s->d_ctr, 0, 2 * kTable * 4, s->s_pt), "counter reset");
-    for (int k = 0; k < 2; k++) {
-        s->h_done[k] = halloc<uint32_t>(kTable, "counter read-back");
-        check(hipEventCreateWithFlags(&s->done_ev[k], hipEventDisableTiming), "hipEventCreate");
-    }
+    s->d_ctr = dalloc<uint32_t>(2 * kTable * kWfCounterStride, "request counters");
+    check(hipMemsetAsync(s->d_ctr, 0, 2 * kTable * kWfCounterStride * 4, s->stream), "counter reset");
+    s->h_ctr = halloc<uint32_t>(2 * kTable * kWfCounterStride, "counter read-back");
+    check(hipEventCreateWithFlags(&s->ctr_ev, hipEventDisableTiming), "hipEventCreate");
     for (uint32_t k = 0; k < kDvrRing; k++) {
         s->d_jobs[k] = dalloc<DvrJob>(64, "ray-marching jobs");
         s->h_jobs[k] = halloc<DvrJob>(64, "ray-marching job staging");
@@ -513,9 +576,9 @@ void create_server(vvr_ctx* ctx, const vvr_server_config* cfg, vvr_server** out)
     s->d_uniforms = dalloc<vvr_uniforms>(kMaxJobs, "request uniforms");
     s->h_uniforms = halloc<vvr_uniforms>(kMaxJobs, "request uniform staging");
     for (int i = int(kMaxJobs) - 1; i >= 0; i--) s->free_uslots.push_back(i);
-    wf_init_served(s->engine.S, s->engine.Q, s->engine.slots, s->s_pt);
+    wf_init_served(s->engine.S, s->engine.Q, s->engine.slots, s->stream);
     check(hipGetLastError(), "wavefront init");
-    check(hipStreamSynchronize(s->s_pt), "server set-up");
+    check(hipStreamSynchronize(s->stream), "server set-up");
     vvr_server* raw = s.release();
     raw->worker = std::thread([raw] { raw->loop(); });
     *out = raw;
@@ -544,8 +607,9 @@ int vvr_server_submit(vvr_server* s, const vvr_request* rq, uint64_t* id) {
         if (j->pt) {
             j->spp_total = std::max(rq->params->path_tracing.samples_per_frame, 1u);
             j->increment = s->cfg.pt_increment ? std::min(s->cfg.pt_increment, j->spp_total) : j->spp_total;
+            j->n_inc = (j->spp_total + j->increment - 1) / j->increment;
         }
-        if (size_t need = frame_bytes(*j, j->increment); need > s->cfg.hbm_budget)
+        if (size_t need = request_bytes(*j); need > s->cfg.hbm_budget)
             throw Error{VVR_ELIMIT, "request needs " + std::to_string(need) + " bytes, over the server's HBM budget"};
         j->t_submit = now_ns();
         std::lock_guard<std::mutex> lk(s->mu);
@@ -584,12 +648,36 @@ int vvr_server_read_hdr(vvr_server* s, uint64_t id, float* rgba) {
             auto it = s->jobs.find(id);
             if (it == s->jobs.end()) throw Error{VVR_EINVAL, "unknown request"};
             j = it->second.get();
-            if (!(j->state == Job::kDone || (j->pt && j->spp_done > 0 && j->state == Job::kRunning)))
+            const bool delivered = j->pt ? j->spp_delivered > 0 : (j->state == Job::kDone && j->status == VVR_OK);
+            if (!delivered) {
+                if (j->state == Job::kDone && j->status != VVR_OK)
+                    throw Error{j->status, "the request failed: " + j->error};
                 throw Error{VVR_ESTATE, "the request has no delivered frame"};
-            if (j->status != VVR_OK) throw Error{j->status, "the request failed: " + j->error};
+            }
         }
         const size_t n = size_t(j->width) * j->height * 4;
         for (size_t i = 0; i < n; i++) rgba[i] = f16_bits_to_f32(j->h_hdr[i]);
+    });
+}
+
+int vvr_server_cancel(vvr_server* s, uint64_t id) {
+    return guarded([&] {
+        if (!s) throw Error{VVR_EINVAL, "null argument"};
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->jobs.find(id);
+        if (it == s->jobs.end()) throw Error{VVR_EINVAL, "unknown request"};
+        Job* j = it->second.get();
+        j->cancelled = true;
+        auto q = std::find(s->queue.begin(), s->queue.end(), j);
+        if (q != s->queue.end()) {
+            s->queue.erase(q);
+            j->state = Job::kDone;
+            j->status = VVR_ECANCELED;
+            j->error = "cancelled";
+            s->completions.push_back(s->record(j, VVR_ECANCELED, 0, true));
+            s->stats.completed++;
+            s->cv_done.notify_all();
+        }
     });
 }
 
@@ -626,27 +714,22 @@ void vvr_server_destroy(vvr_server* s) {
     }
     if (s->worker.joinable()) s->worker.join();
     (void)hipSetDevice(s->device);
-    (void)hipStreamSynchronize(s->s_pt);
-    (void)hipStreamSynchronize(s->s_dvr);
+    (void)hipStreamSynchronize(s->stream);
     for (auto& kv : s->jobs) {
         Job* j = kv.second.get();
-        if (j->state != Job::kDone) s->release_memory(j);
+        if (j->state == Job::kRunning) s->release_memory(j);
         hfree(j->h_hdr);
         if (j->read_ev) (void)hipEventDestroy(j->read_ev);
     }
     dfree(s->d_table), dfree(s->d_ctr), dfree(s->d_uniforms);
-    hfree(s->h_table), hfree(s->h_uniforms);
-    for (int k = 0; k < 2; k++) {
-        hfree(s->h_done[k]);
-        if (s->done_ev[k]) (void)hipEventDestroy(s->done_ev[k]);
-    }
+    hfree(s->h_table), hfree(s->h_uniforms), hfree(s->h_ctr);
+    if (s->ctr_ev) (void)hipEventDestroy(s->ctr_ev);
     for (uint32_t k = 0; k < kDvrRing; k++) {
         dfree(s->d_jobs[k]);
         hfree(s->h_jobs[k]);
         if (s->jobs_ev[k]) (void)hipEventDestroy(s->jobs_ev[k]);
     }
-    if (s->s_pt) (void)hipStreamDestroy(s->s_pt);
-    if (s->s_dvr) (void)hipStreamDestroy(s->s_dvr);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
 

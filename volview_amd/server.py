@@ -49,6 +49,9 @@ class Server:
         check(self._lib.vvr_server_read_hdr(self.handle, rid, img.ctypes.data_as(POINTER(c_float))))
         return img
 
+    def cancel(self, rid: int):
+        check(self._lib.vvr_server_cancel(self.handle, rid))
+
     def release(self, rid: int):
         check(self._lib.vvr_server_release(self.handle, rid))
 

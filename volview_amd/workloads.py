@@ -107,3 +107,69 @@ def pt_scene(ctx: Context, dims=(512, 512, 400), seed: int = 1, size=(1024, 1024
     r.resize(*size)
     p = pt_params(r, n, spp, bounces=bounces, camera_index=camera_index)
     return Scene(r, p, size[0], size[1], n, hu=hu, labels=labels, soft=soft)
+
+
+# ---- Batched-32 and the serving trace ------------------------------------------------------
+
+
+@dataclass
+class RequestSpec:
+    volume: int          # index into the workload's volumes
+    path_traced: bool
+    width: int
+    height: int
+    spp: int = 0
+    camera_index: int = 0
+    arrival: float = 0.0  # seconds from the start of a trace
+
+
+def batched32(seed0: int = 100, n: int = 32):
+    """Batched-32: n requests with seeds seed0 .. seed0 + n - 1, each on its own
+    phantom: cubic volumes of edge U{128..512}, square images of edge
+    U{256..1024}, half ray-marched (bone preset, DVR-1 settings), half path-traced
+    (PT-1 settings without labels) with U{1..64} paths per pixel.
+    Returns (volume specs [(dims, seed)], request specs)."""
+    rng = np.random.default_rng(seed0)
+    kinds = rng.permutation(np.arange(n) % 2 == 1)
+    vols, reqs = [], []
+    for k in range(n):
+        edge = int(rng.integers(128, 513))
+        size = int(rng.integers(256, 1025))
+        spp = int(rng.integers(1, 65))
+        vols.append(((edge, edge, edge), seed0 + k))
+        reqs.append(RequestSpec(k, bool(kinds[k]), size, size, spp if kinds[k] else 0, camera_index=k % 5))
+    return vols, reqs
+
+
+def serving_trace(seconds: float, rate: float = 20.0, seed: int = 7, n_volumes: int = 8):
+    """The serving workload: Poisson arrivals at `rate` per second for `seconds`;
+    70 % ray-marched 1280 x 720 frames with a new camera each, 30 % progressive
+    path tracing at 1024^2 with a 256-path budget; 8 resident phantoms of edge
+    U{128..512} with 40 organ labels (isolate mode).
+    Returns (volume specs [(dims, seed)], request specs by arrival)."""
+    rng = np.random.default_rng(seed)
+    vols = [((e, e, e), 200 + k) for k, e in enumerate(rng.integers(128, 513, n_volumes))]
+    reqs, t = [], 0.0
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t >= seconds:
+            break
+        pt = bool(rng.random() < 0.3)
+        v = int(rng.integers(0, n_volumes))
+        cam = int(rng.integers(1, 10**6))
+        if pt:
+            reqs.append(RequestSpec(v, True, 1024, 1024, 256, cam, t))
+        else:
+            reqs.append(RequestSpec(v, False, 1280, 720, 0, cam, t))
+    return vols, reqs
+
+
+def request_params(renderer: Renderer, spec: RequestSpec, n_labels: int = 0) -> _abi.FrameParams:
+    """Frame parameters of a request: DVR-1 or PT-1 settings on its camera, with
+    the volume's organ labels (isolate mode) when it has them."""
+    if spec.path_traced:
+        return pt_params(renderer, n_labels, spec.spp, camera_index=spec.camera_index)
+    p = dvr_params(renderer, spec.camera_index)
+    if n_labels:
+        organ_label_params(p, n_labels, 7)
+    return p
