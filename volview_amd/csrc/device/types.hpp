@@ -94,6 +94,7 @@ struct WfSlots {
     float4* c0;        // chunk sums: radiance, background
     float4* c1;        //             albedo, luminance^2
     float4* c2;        //             normal, depth
+    uint16_t* req;     // request of the slot's path (engines over several requests)
 };
 
 // Counters in WfQueues::counts: queue lengths, the item queue head and the live
@@ -117,7 +118,14 @@ struct WfQueues {
     uint32_t* end[2];   // slots whose path ended (or that have no item)
     uint32_t* idle[2];  // served engine: slots waiting for work
     uint32_t* counts;   // kWfCounts counters
+    // Engines over several requests: ff or ev binned by request, so a wave's
+    // lanes mostly share a request, and the bins' counts and cursors
+    // (kWfBins each for the ff sort, then for the ev sort).
+    uint32_t* sorted;
+    uint32_t* bins;
 };
+
+constexpr uint32_t kWfBins = 256;  // request indices the queue sort bins
 
 // Requests the served engine draws work items from this iteration: entries of
 // its request table, passed by value.

@@ -187,6 +187,75 @@ __device__ __forceinline__ void store_partial(const PtRequest& R, uint32_t chunk
     base[2u * R.n_px + p_item] = c2;
 }
 
+// The count and cursor arrays of both queue sorts to zero (one wave of the
+// iteration's first kernel; the sorts run after it on the same stream).
+__device__ __forceinline__ void clear_bins(const WfQueues& Q) {
+    for (uint32_t i = threadIdx.x; i < 4u * kWfBins; i += blockDim.x) Q.bins[i] = 0u;
+}
+
+// Counting sort of a queue by the request of each slot's path, so that the
+// stages' waves mostly hold paths of one request (each distinct request in a
+// wave costs the stage one more pass). The order within a request is
+// arbitrary: no result depends on queue order. Pass 1: requests' counts.
+__global__ void __launch_bounds__(256) k_wf_bin_count(const uint32_t* __restrict__ in,
+                                                      const uint32_t* __restrict__ count,
+                                                      const uint16_t* __restrict__ req, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[kWfBins];
+    h[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t n = *count;
+    for (uint32_t q = blockIdx.x * 256u + threadIdx.x; q < n; q += gridDim.x * 256u) atomicAdd(&h[req[in[q]] & (kWfBins - 1u)], 1u);
+    __syncthreads();
+    if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+// Pass 2: each workgroup ranks a chunk of entries within their requests in
+// LDS, reserves each request's run with one atomic and writes the slots out.
+constexpr uint32_t kSortPer = 16;  // entries per lane and chunk
+__global__ void __launch_bounds__(256) k_wf_bin_scatter(const uint32_t* __restrict__ in,
+                                                        const uint32_t* __restrict__ count,
+                                                        const uint16_t* __restrict__ req,
+                                                        const uint32_t* __restrict__ hist, uint32_t* __restrict__ cursor,
+                                                        uint32_t* __restrict__ out) {
+    __shared__ uint32_t start[kWfBins], h[kWfBins], run[kWfBins];
+    const uint32_t t = threadIdx.x;
+    // Exclusive prefix of the counts: where each request's run starts.
+    const uint32_t mine = hist[t];
+    start[t] = mine;
+    __syncthreads();
+    for (uint32_t off = 1; off < kWfBins; off <<= 1) {
+        const uint32_t x = t >= off ? start[t - off] : 0u;
+        __syncthreads();
+        start[t] += x;
+        __syncthreads();
+    }
+    start[t] -= mine;
+    const uint32_t n = *count;
+    constexpr uint32_t kChunk = 256u * kSortPer;
+    for (uint32_t c = blockIdx.x * kChunk; c < n; c += gridDim.x * kChunk) {
+        h[t] = 0u;
+        __syncthreads();
+        uint32_t slot[kSortPer], rank[kSortPer];
+        uint16_t b[kSortPer];
+#pragma unroll
+        for (uint32_t j = 0; j < kSortPer; j++) {
+            const uint32_t q = c + j * 256u + t;
+            if (q < n) {
+                slot[j] = in[q];
+                b[j] = req[slot[j]] & (kWfBins - 1u);
+                rank[j] = atomicAdd(&h[b[j]], 1u);
+            }
+        }
+        __syncthreads();
+        if (h[t]) run[t] = start[t] + atomicAdd(&cursor[t], h[t]);
+        __syncthreads();
+#pragma unroll
+        for (uint32_t j = 0; j < kSortPer; j++)
+            if (c + j * 256u + t < n) out[run[b[j]] + rank[j]] = slot[j];
+        __syncthreads();
+    }
+}
+
 // Every slot empty and on the end queue; the item queue at 0.
 __global__ void k_wf_init(WfSlots S, WfQueues Q, uint32_t slots) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -208,6 +277,7 @@ __global__ void __launch_bounds__(64) k_wf_regen(const PtRequest* __restrict__ r
         Q.counts[kWfEV] = 0u;
         Q.counts[kWfEND0 + nxt] = 0u;
     }
+    if (kBatched && blockIdx.x == 0) clear_bins(Q);
     const uint32_t n = Q.counts[kWfEND0 + cur];
     const uint32_t lane = __lane_id();
     for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
@@ -271,6 +341,7 @@ __global__ void __launch_bounds__(64) k_wf_regen(const PtRequest* __restrict__ r
             if (push) {
                 store_path(S, slot, st, ref);
                 S.c0[slot] = c0, S.c1[slot] = c1, S.c2[slot] = c2;
+                if (kBatched) S.req[slot] = uint16_t(ref.req);
             } else {
                 S.p_spec[slot] = make_float4(0.f, 0.f, 0.f, bitsf(kNone));
             }
@@ -306,6 +377,7 @@ __global__ void __launch_bounds__(64) k_wf_regen_served(const PtRequest* __restr
         Q.counts[kWfEV] = 0u;
         Q.counts[kWfEND0 + nxt] = 0u;
     }
+    if (blockIdx.x == 0) clear_bins(Q);
     const uint32_t n_end = Q.counts[kWfEND0 + cur];
     const uint32_t n = n_end + (V.revive ? Q.counts[kWfIDLE0 + V.idle_in] : 0u);
     const uint32_t lane = __lane_id();
@@ -320,11 +392,13 @@ __global__ void __launch_bounds__(64) k_wf_regen_served(const PtRequest* __restr
         bool finishing = false;
         if (active) {
             need = true;
-            ref.item = fbits(S.p_spec[slot].w);  // an idle slot holds no path
+            // One round of loads: an idle slot holds no path (item kNone), and
+            // its other fields are ignored.
+            load_path(S, slot, st, ref);
+            const float4 s0 = S.c0[slot], s1 = S.c1[slot], s2 = S.c2[slot];
             if (ref.item != kNone) {
-                load_path(S, slot, st, ref);
                 finishing = true;
-                c0 = S.c0[slot], c1 = S.c1[slot], c2 = S.c2[slot];
+                c0 = s0, c1 = s1, c2 = s2;
                 const V3 r = path_end(st);  // pt.comp:546-551
                 const float lum = dot(r, V3(0.2126f, 0.7152f, 0.0722f));
                 c0 = make_float4(c0.x + r.x, c0.y + r.y, c0.z + r.z, c0.w);
@@ -353,18 +427,14 @@ __global__ void __launch_bounds__(64) k_wf_regen_served(const PtRequest* __restr
         // New items from the active requests in list order (the host keeps the
         // list in priority order): a request's work is handed out before the
         // next one's, and the next one fills the slots its tail leaves idle.
+        // A request whose items are all handed out costs one atomic per wave
+        // until the host drops it from the list after the batch (its head grows
+        // by at most the slot count per iteration, far from wrapping).
         uint32_t a = 0;
         while (a < A.n && __ballot(need)) {
             const uint32_t r = A.r[a];
             const PtRequest& R = reqs[r];
-            // A request whose items are all handed out is passed over without an
-            // atomic (its head only grows; the atomic below stays authoritative).
-            uint32_t* head = &V.head[r * kWfCounterStride];
-            if (__hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= R.n_items) {
-                a++;
-                continue;
-            }
-            const uint32_t it = wave_take(need, head);
+            const uint32_t it = wave_take(need, &V.head[r * kWfCounterStride]);
             const bool got = need && it < R.n_items;
             const bool dry = need && !got;
             if (got) {
@@ -389,6 +459,7 @@ __global__ void __launch_bounds__(64) k_wf_regen_served(const PtRequest* __restr
             if (push) {
                 store_path(S, slot, st, ref);
                 S.c0[slot] = c0, S.c1[slot] = c1, S.c2[slot] = c2;
+                S.req[slot] = uint16_t(ref.req);
             } else {
                 S.p_spec[slot] = make_float4(0.f, 0.f, 0.f, bitsf(kNone));
             }
@@ -401,13 +472,13 @@ __global__ void __launch_bounds__(64) k_wf_regen_served(const PtRequest* __restr
 // Stage 1: free flight.
 template <bool kBatched>
 __global__ void __launch_bounds__(64) k_wf_extend(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
-                                                  uint32_t cur) {
+                                                  const uint32_t* __restrict__ in, uint32_t cur) {
     const uint32_t nxt = cur ^ 1u;
     const uint32_t n = Q.counts[kWfFF0 + cur];
     for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
         const uint32_t q = base + __lane_id();
         const bool active = q < n;
-        const uint32_t slot = active ? Q.ff[cur][q] : 0u;
+        const uint32_t slot = active ? in[q] : 0u;
         bool hit = false;
         PathState st;
         SlotRef ref{0u, 0u, 0u};
@@ -438,10 +509,11 @@ __global__ void __launch_bounds__(64) k_wf_extend(const PtRequest* __restrict__ 
 
 // Stage 2: shading frame and light sample.
 template <bool kBatched>
-__global__ void __launch_bounds__(64) k_wf_shade(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q) {
+__global__ void __launch_bounds__(64) k_wf_shade(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
+                                                 const uint32_t* __restrict__ in) {
     const uint32_t n = Q.counts[kWfEV];
     for (uint32_t q = blockIdx.x * 64u + __lane_id(); q < n; q += gridDim.x * 64u) {
-        const uint32_t slot = Q.ev[q];
+        const uint32_t slot = in[q];
         PathState st;  // the stage reads the direction, bounce and random stream
         const float4 d = S.p_dir[slot];
         st.dir = xyz(d);
@@ -467,10 +539,11 @@ __global__ void __launch_bounds__(64) k_wf_shade(const PtRequest* __restrict__ r
 
 // Stage 3: shadow ray.
 template <bool kBatched>
-__global__ void __launch_bounds__(64) k_wf_shadow(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q) {
+__global__ void __launch_bounds__(64) k_wf_shadow(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
+                                                  const uint32_t* __restrict__ in) {
     const uint32_t n = Q.counts[kWfEV];
     for (uint32_t q = blockIdx.x * 64u + __lane_id(); q < n; q += gridDim.x * 64u) {
-        const uint32_t slot = Q.ev[q];
+        const uint32_t slot = in[q];
         const uint32_t req = fbits(S.p_rad[slot].w) >> 16;
         PathState st;
         st.rng.state = fbits(S.p_dir[slot].w);
@@ -486,13 +559,13 @@ __global__ void __launch_bounds__(64) k_wf_shadow(const PtRequest* __restrict__ 
 // Stage 4: next-event estimate, next direction, roulette.
 template <bool kBatched>
 __global__ void __launch_bounds__(64) k_wf_scatter(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
-                                                   uint32_t cur) {
+                                                   const uint32_t* __restrict__ in, uint32_t cur) {
     const uint32_t nxt = cur ^ 1u;
     const uint32_t n = Q.counts[kWfEV];
     for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
         const uint32_t q = base + __lane_id();
         const bool active = q < n;
-        const uint32_t slot = active ? Q.ev[q] : 0u;
+        const uint32_t slot = active ? in[q] : 0u;
         bool more = false;
         PathState st;
         SlotRef ref{0u, 0u, 0u};
@@ -568,23 +641,49 @@ WfGrid wf_grid(int cus) {
     g.shade = waves_for(k_wf_shade<true>, cus, 32);
     g.shadow = waves_for(k_wf_shadow<true>, cus, 32);
     g.scatter = waves_for(k_wf_scatter<true>, cus, 32);
+    g.sort = cus * 2;
     return g;
 }
+
+namespace {
+
+// Extend over ff[cur], then shade, shadow and scatter over ev; with `sort`,
+// each queue binned by request first.
+template <bool kBatched>
+void wf_stages(const PtRequest* reqs, const WfSlots& S, const WfQueues& Q, uint32_t cur, const WfGrid& g, bool sort,
+               hipStream_t st) {
+    const uint32_t* ff = Q.ff[cur];
+    const uint32_t* ev = Q.ev;
+    if (sort) {
+        uint32_t* hist = Q.bins;
+        hipLaunchKernelGGL(k_wf_bin_count, dim3(g.sort), dim3(256), 0, st, ff, Q.counts + kWfFF0 + cur, S.req, hist);
+        hipLaunchKernelGGL(k_wf_bin_scatter, dim3(g.sort), dim3(256), 0, st, ff, Q.counts + kWfFF0 + cur, S.req, hist,
+                           hist + kWfBins, Q.sorted);
+        ff = Q.sorted;
+    }
+    hipLaunchKernelGGL(k_wf_extend<kBatched>, dim3(g.extend), dim3(64), 0, st, reqs, S, Q, ff, cur);
+    if (sort) {
+        uint32_t* hist = Q.bins + 2 * kWfBins;
+        hipLaunchKernelGGL(k_wf_bin_count, dim3(g.sort), dim3(256), 0, st, ev, Q.counts + kWfEV, S.req, hist);
+        hipLaunchKernelGGL(k_wf_bin_scatter, dim3(g.sort), dim3(256), 0, st, ev, Q.counts + kWfEV, S.req, hist,
+                           hist + kWfBins, Q.sorted);
+        ev = Q.sorted;
+    }
+    hipLaunchKernelGGL(k_wf_shade<kBatched>, dim3(g.shade), dim3(64), 0, st, reqs, S, Q, ev);
+    hipLaunchKernelGGL(k_wf_shadow<kBatched>, dim3(g.shadow), dim3(64), 0, st, reqs, S, Q, ev);
+    hipLaunchKernelGGL(k_wf_scatter<kBatched>, dim3(g.scatter), dim3(64), 0, st, reqs, S, Q, ev, cur);
+}
+
+}  // namespace
 
 void wf_iteration(const PtRequest* reqs, uint32_t n_reqs, uint32_t total_items, const WfSlots& S, const WfQueues& Q,
                   uint32_t cur, const WfGrid& g, hipStream_t st) {
     if (n_reqs > 1) {
         hipLaunchKernelGGL(k_wf_regen<true>, dim3(g.regen), dim3(64), 0, st, reqs, n_reqs, total_items, S, Q, cur);
-        hipLaunchKernelGGL(k_wf_extend<true>, dim3(g.extend), dim3(64), 0, st, reqs, S, Q, cur);
-        hipLaunchKernelGGL(k_wf_shade<true>, dim3(g.shade), dim3(64), 0, st, reqs, S, Q);
-        hipLaunchKernelGGL(k_wf_shadow<true>, dim3(g.shadow), dim3(64), 0, st, reqs, S, Q);
-        hipLaunchKernelGGL(k_wf_scatter<true>, dim3(g.scatter), dim3(64), 0, st, reqs, S, Q, cur);
+        wf_stages<true>(reqs, S, Q, cur, g, n_reqs <= kWfBins, st);
     } else {
         hipLaunchKernelGGL(k_wf_regen<false>, dim3(g.regen), dim3(64), 0, st, reqs, n_reqs, total_items, S, Q, cur);
-        hipLaunchKernelGGL(k_wf_extend<false>, dim3(g.extend), dim3(64), 0, st, reqs, S, Q, cur);
-        hipLaunchKernelGGL(k_wf_shade<false>, dim3(g.shade), dim3(64), 0, st, reqs, S, Q);
-        hipLaunchKernelGGL(k_wf_shadow<false>, dim3(g.shadow), dim3(64), 0, st, reqs, S, Q);
-        hipLaunchKernelGGL(k_wf_scatter<false>, dim3(g.scatter), dim3(64), 0, st, reqs, S, Q, cur);
+        wf_stages<false>(reqs, S, Q, cur, g, false, st);
     }
 }
 
@@ -596,10 +695,7 @@ void wf_init_served(const WfSlots& S, const WfQueues& Q, uint32_t slots, hipStre
 void wf_iteration_served(const PtRequest* reqs, const WfActive& A, const WfServed& V, const WfSlots& S,
                          const WfQueues& Q, uint32_t cur, const WfGrid& g, hipStream_t st) {
     hipLaunchKernelGGL(k_wf_regen_served, dim3(g.regen), dim3(64), 0, st, reqs, A, V, S, Q, cur);
-    hipLaunchKernelGGL(k_wf_extend<true>, dim3(g.extend), dim3(64), 0, st, reqs, S, Q, cur);
-    hipLaunchKernelGGL(k_wf_shade<true>, dim3(g.shade), dim3(64), 0, st, reqs, S, Q);
-    hipLaunchKernelGGL(k_wf_shadow<true>, dim3(g.shadow), dim3(64), 0, st, reqs, S, Q);
-    hipLaunchKernelGGL(k_wf_scatter<true>, dim3(g.scatter), dim3(64), 0, st, reqs, S, Q, cur);
+    wf_stages<true>(reqs, S, Q, cur, g, true, st);
 }
 
 void wf_resolve(const PtRequest& r, hipStream_t st) {

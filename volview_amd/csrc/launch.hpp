@@ -56,6 +56,7 @@ void launch_pt_persistent(const DevScene& s, float4* accum, float4* albedo, floa
 // item ranges [item_base, item_base + n_items) covering [0, total_items).
 struct WfGrid {
     int regen, extend, shade, shadow, scatter;  // single-wave workgroups per stage
+    int sort;                                    // 256-lane workgroups of the queue sort
 };
 WfGrid wf_grid(int cus);
 // All `slots` slots empty on the end queue of iteration 0, item queue at 0.

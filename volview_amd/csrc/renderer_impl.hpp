@@ -87,16 +87,20 @@ struct WfEngine {
         if (n <= slots) return;
         dfree(slot_mem);
         slot_mem = nullptr, slots = 0;
-        constexpr int kF4 = 15, kQueues = 7;
+        constexpr int kF4 = 15, kQueues = 8;
         const size_t f4_bytes = size_t(n) * sizeof(float4), q_bytes = (size_t(n) * 4 + 255) / 256 * 256;
-        slot_mem = dalloc<uint8_t>(kF4 * f4_bytes + kQueues * q_bytes + 256, "path slots");
+        const size_t req_bytes = (size_t(n) * 2 + 255) / 256 * 256, bin_bytes = 4 * kWfBins * 4;
+        slot_mem = dalloc<uint8_t>(kF4 * f4_bytes + kQueues * q_bytes + req_bytes + 256 + bin_bytes, "path slots");
         uint8_t* p = slot_mem;
         float4** f4s[kF4] = {&S.p_origin, &S.p_dir, &S.p_thr, &S.p_spec, &S.p_rad, &S.e_idx, &S.e_s0, &S.e_s1,
                              &S.e_s2,     &S.e_n,   &S.e_light, &S.e_lt, &S.c0,   &S.c1,   &S.c2};
         for (auto f : f4s) *f = reinterpret_cast<float4*>(p), p += f4_bytes;
-        uint32_t** qs[kQueues] = {&Q.ff[0], &Q.ff[1], &Q.ev, &Q.end[0], &Q.end[1], &Q.idle[0], &Q.idle[1]};
+        uint32_t** qs[kQueues] = {&Q.ff[0],   &Q.ff[1],   &Q.ev,      &Q.end[0],
+                                  &Q.end[1], &Q.idle[0], &Q.idle[1], &Q.sorted};
         for (auto q : qs) *q = reinterpret_cast<uint32_t*>(p), p += q_bytes;
-        Q.counts = reinterpret_cast<uint32_t*>(p);
+        S.req = reinterpret_cast<uint16_t*>(p), p += req_bytes;
+        Q.counts = reinterpret_cast<uint32_t*>(p), p += 256;
+        Q.bins = reinterpret_cast<uint32_t*>(p);
         slots = n;
     }
 

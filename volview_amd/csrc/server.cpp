@@ -35,6 +35,7 @@ namespace vvr {
 namespace {
 
 constexpr uint32_t kTable = 256;    // request-table entries (path-traced increments in flight)
+static_assert(kTable <= kWfBins, "the queue sort bins paths by request-table entry");
 constexpr uint32_t kMaxJobs = 256;  // requests admitted at once
 constexpr uint32_t kBatch = 2;      // wavefront iterations per pass
 constexpr uint32_t kDvrRing = 8;    // batched ray-marching job tables in flight
@@ -608,6 +609,11 @@ int vvr_server_submit(vvr_server* s, const vvr_request* rq, uint64_t* id) {
             j->spp_total = std::max(rq->params->path_tracing.samples_per_frame, 1u);
             j->increment = s->cfg.pt_increment ? std::min(s->cfg.pt_increment, j->spp_total) : j->spp_total;
             j->n_inc = (j->spp_total + j->increment - 1) / j->increment;
+            // Work-item counters are 32-bit and may run past an increment's end
+            // by the slot count before the increment leaves the active list.
+            const uint64_t px = uint64_t((rq->width + 7) / 8) * ((rq->height + 7) / 8) * 64u;
+            if (px * j->increment >= (1ull << 31))
+                throw Error{VVR_ELIMIT, "one increment exceeds 2^31 paths: lower pt_increment"};
         }
         if (size_t need = request_bytes(*j); need > s->cfg.hbm_budget)
             throw Error{VVR_ELIMIT, "request needs " + std::to_string(need) + " bytes, over the server's HBM budget"};
