@@ -265,7 +265,9 @@ enum {
     VVR_VOL_LABELS = 1,   /* uint8, 1 per voxel                  */
     VVR_VOL_NORMALS = 2,  /* int8 snorm, 4 per voxel             */
     VVR_VOL_MASK = 3,     /* uint16 unorm, 2 per voxel (r, g)    */
-    VVR_VOL_MAJORANT = 4  /* float, 1 per 8^3 brick              */
+    VVR_VOL_MAJORANT = 4, /* float, 1 per 8^3 brick              */
+    VVR_VOL_SHADOW = 5,   /* RGBA16F bits, ceil(dims / 2) texels  */
+    VVR_VOL_AO = 6        /* R16F bits, ceil(dims / 4) texels     */
 };
 /* A derived volume in the reference's layout (x fastest), for parity checks.
  * `bytes` must equal the volume's size. */
@@ -279,7 +281,7 @@ int vvr_renderer_read_uniforms(vvr_renderer* r, void* out, size_t bytes);
 /* ---- measurement ------------------------------------------------------------ */
 
 enum { VVR_KERNEL_MAJORANT = 0, VVR_KERNEL_NORMALS = 1, VVR_KERNEL_MASK = 2, VVR_KERNEL_DVR = 3, VVR_KERNEL_PT = 4,
-       VVR_KERNEL_COUNT = 5 };
+       VVR_KERNEL_LIGHTING = 5, VVR_KERNEL_COUNT = 6 };
 /* Enables HIP-event timing of each kernel on the stream it is launched on. */
 int vvr_renderer_set_timing(vvr_renderer* r, int enabled);
 /* Total milliseconds and launches of kernel `which` since timing was enabled. */

@@ -27,7 +27,8 @@ from volview_amd import _abi  # noqa: E402
 class OracleScene(ctypes.Structure):
     _fields_ = [("dims", c_uint32 * 3), ("density", POINTER(c_uint16)), ("lut", POINTER(c_uint16)),
                 ("preint", POINTER(c_float)), ("labels", POINTER(c_uint8)), ("normals", POINTER(c_int8)),
-                ("mask", POINTER(c_uint16)), ("majorant", POINTER(c_float))]
+                ("mask", POINTER(c_uint16)), ("majorant", POINTER(c_float)), ("shadow", POINTER(c_uint16)),
+                ("ao", POINTER(c_uint16))]
 
 
 _lib = None
@@ -66,6 +67,10 @@ def load() -> ctypes.CDLL:
             "oracle_mask_binary": (None, [P(OracleScene), P(_abi.Uniforms), P(c_uint8), P(c_uint32), P(c_int32),
                                           c_int32, P(c_float)]),
             "oracle_dvr": (None, [P(OracleScene), P(_abi.Uniforms), P(c_uint32), c_size_t, c_int, P(c_float)]),
+            "oracle_shadow": (None, [P(OracleScene), P(_abi.Uniforms), c_int, P(c_uint16)]),
+            "oracle_ao": (None, [P(OracleScene), P(_abi.Uniforms), c_int, P(c_uint16)]),
+            "oracle_lighting_lookup": (None, [P(OracleScene), P(_abi.Uniforms), c_int, P(c_float), c_int,
+                                              P(c_float)]),
             "oracle_pt": (None, [P(OracleScene), P(_abi.Uniforms), P(c_uint32), c_size_t, c_int, P(c_float),
                                  P(c_float), P(c_float)]),
             "oracle_density_cubic": (c_float, [P(OracleScene), P(_abi.Uniforms), P(c_float), c_int]),
@@ -190,7 +195,8 @@ class Scene:
 
     def set(self, name, a):
         dtype, ctype = {"normals": (np.int8, c_int8), "mask": (np.uint16, c_uint16),
-                        "majorant": (np.float32, c_float), "labels": (np.uint8, c_uint8)}[name]
+                        "majorant": (np.float32, c_float), "labels": (np.uint8, c_uint8),
+                        "shadow": (np.uint16, c_uint16), "ao": (np.uint16, c_uint16)}[name]
         a = np.ascontiguousarray(a, dtype)
         self.arrays[name] = a
         setattr(self.c, name, _p(a, ctype))
@@ -224,6 +230,32 @@ class Scene:
         soft_a = np.ascontiguousarray(soft, np.uint8) if soft is not None else None
         load().oracle_mask_binary(byref(self.c), byref(u), _p(soft_a, c_uint8), sd, sp, margin, _p(out, c_float))
         return out
+
+    def shadow_dims(self):
+        return tuple((n + 1) // 2 for n in self.dims)
+
+    def ao_dims(self):
+        return tuple((n + 3) // 4 for n in self.dims)
+
+    def shadow(self, u: _abi.Uniforms, precision: int = 32) -> np.ndarray:
+        """shadow.comp swept over the volume: RGBA16F bits, (z, y, x, 4) at ceil(dims / 2)."""
+        sx, sy, sz = self.shadow_dims()
+        out = np.zeros((sz, sy, sx, 4), np.uint16)
+        load().oracle_shadow(byref(self.c), byref(u), precision, _p(out, c_uint16))
+        return out
+
+    def ao(self, u: _abi.Uniforms, precision: int = 32) -> np.ndarray:
+        """ao.comp: R16F bits, (z, y, x) at ceil(dims / 4)."""
+        ax, ay, az = self.ao_dims()
+        out = np.zeros((az, ay, ax), np.uint16)
+        load().oracle_ao(byref(self.c), byref(u), precision, _p(out, c_uint16))
+        return out
+
+    def lighting_lookup(self, u: _abi.Uniforms, which: int, tc, precision: int = 32) -> np.ndarray:
+        t = np.ascontiguousarray(tc, np.float32)
+        out = np.zeros(4, np.float32)
+        load().oracle_lighting_lookup(byref(self.c), byref(u), which, _p(t, c_float), precision, _p(out, c_float))
+        return out if which == 0 else out[:1]
 
     def dvr(self, u: _abi.Uniforms, pixels: np.ndarray, precision: int = 32) -> np.ndarray:
         px = np.ascontiguousarray(pixels, np.uint32)

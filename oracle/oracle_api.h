@@ -26,6 +26,8 @@ typedef struct oracle_scene {
     const int8_t* normals;    /* binding 5, RGBA8_SNORM                            */
     const uint16_t* mask;     /* binding 13, RG16_UNORM (NULL outside label modes) */
     const float* majorant;    /* binding 22, R32F, ceil(dims / 8)                  */
+    const uint16_t* shadow;   /* binding 6, RGBA16F bits, ceil(dims / 2) (NULL: off) */
+    const uint16_t* ao;       /* binding 7, R16F bits, ceil(dims / 4) (NULL: off)    */
 } oracle_scene;
 
 uint16_t oracle_hu_to_unorm16(int16_t hu);
@@ -60,6 +62,15 @@ void oracle_mask_binary(const oracle_scene* s, const vvr_uniforms* u, const uint
  * RGBA16F store. precision 32 or 64. */
 void oracle_dvr(const oracle_scene* s, const vvr_uniforms* u, const uint32_t* pixels, size_t n, int precision,
                 float* rgba);
+/* shadow.comp swept over every slice as Renderer::record_shadow_sweep does
+ * (renderer.rs:1907-1940): RGBA16F bits, ceil(dims / 2) texels x fastest. */
+void oracle_shadow(const oracle_scene* s, const vvr_uniforms* u, int precision, uint16_t* out);
+/* ao.comp: R16F bits, ceil(dims / 4) texels x fastest. */
+void oracle_ao(const oracle_scene* s, const vvr_uniforms* u, int precision, uint16_t* out);
+/* lighting_lookup (common.glsl:154-156) of the shadow (which = 0: rgba) or
+ * ambient-occlusion (which = 1: r) volume at texture coordinate tc. */
+void oracle_lighting_lookup(const oracle_scene* s, const vvr_uniforms* u, int which, const float tc[3], int precision,
+                            float out[4]);
 /* pt.comp for `n` pixels: the values the dispatch adds to the accumulation,
  * albedo-guide and normal-guide images (4 floats each per pixel). */
 void oracle_pt(const oracle_scene* s, const vvr_uniforms* u, const uint32_t* pixels, size_t n, int precision,

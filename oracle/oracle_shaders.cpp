@@ -3,7 +3,8 @@
 //
 // Line-by-line restatement of the reference's compute shaders:
 //   shaders/common.glsl, shaders/shading.glsl, shaders/majorant.comp,
-//   shaders/normals.comp, shaders/mask.comp, shaders/dvr.comp, shaders/pt.comp.
+//   shaders/normals.comp, shaders/mask.comp, shaders/dvr.comp, shaders/pt.comp,
+//   shaders/shadow.comp, shaders/ao.comp.
 // Each function names the lines it follows. Templated on the real type R: the
 // FP32 oracle (R = float, built without FMA contraction, so every operation is
 // one IEEE-rounded step in source order) and the FP64 oracle (R = double).
@@ -16,7 +17,9 @@
 // texels max(c / 127, -1), fp16 texels their exact value; the 1D LUT is
 // addressed at density * 4095 (common.glsl:170-173). Real texture units round the
 // sub-texel weight (Vulkan requires >= 4 bits, most have 8); that rounding is
-// implementation-defined and not modelled.
+// implementation-defined and not modelled. The lighting volumes are read through
+// the border sampler (renderer.rs:645-648, 1983-1984): the same filter, with
+// texels outside the image opaque white (1, 1, 1, 1).
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -39,11 +42,24 @@ template <class R> struct Ctx {
     int mdims[3];
     uint32_t rng;
 
+    int sdims[3];  // shadow volume: ceil(dims / 2) (SHADOW_DOWNSAMPLE, renderer.rs:35-36, 608-613)
+    int adims[3];  // ambient occlusion: ceil(dims / 4) (AO_DOWNSAMPLE, renderer.rs:39-40, 614-618)
+    const uint16_t* shadow_img;
+    // During a sweep: the slice being written. The shader's linear fetch of the
+    // previous slice may give the slice being written a weight of a rounding
+    // error (its position lands a hair off the texel centre); that texel is
+    // read from the previous slice instead, so the sweep never reads what the
+    // same pass writes.
+    int sweep_axis = -1, sweep_slice = 0, sweep_sign = 0;
+
     Ctx(const oracle_scene* sc, const vvr_uniforms* un) : s(sc), u(un), rng(1) {
         for (int a = 0; a < 3; a++) {
             dims[a] = (int)sc->dims[a];
             mdims[a] = (dims[a] + 7) / 8;
+            sdims[a] = (dims[a] + 1) / 2;
+            adims[a] = (dims[a] + 3) / 4;
         }
+        shadow_img = sc->shadow;
     }
 
     static int iclamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -146,6 +162,49 @@ template <class R> struct Ctx {
     }
     // common.glsl:160-162
     R density_view(V3<R> idx) const { return u->env[1] > 0.5f ? density_cubic(idx) : density_at_index(idx); }
+
+    // ---- lighting volumes (bindings 6 and 7) ------------------------------------------
+    R shadow_texel(int x, int y, int z, int ch) const {
+        int c[3] = {x, y, z};
+        if (sweep_axis >= 0 && c[sweep_axis] == sweep_slice) c[sweep_axis] += sweep_sign;
+        for (int a = 0; a < 3; a++)
+            if (c[a] < 0 || c[a] >= sdims[a]) return R(1);
+        size_t i = (size_t)c[0] + (size_t)sdims[0] * ((size_t)c[1] + (size_t)sdims[1] * c[2]);
+        return R(f16_to_f32(shadow_img[4 * i + ch]));
+    }
+    R ao_texel(int x, int y, int z) const {
+        if (x < 0 || y < 0 || z < 0 || x >= adims[0] || y >= adims[1] || z >= adims[2]) return R(1);
+        return R(f16_to_f32(s->ao[(size_t)x + (size_t)adims[0] * ((size_t)y + (size_t)adims[1] * z)]));
+    }
+    // Linear fetch of channel ch of the shadow (which = 0) or AO (which = 1) volume at texel position t.
+    R light_linear(int which, V3<R> t, int ch) const {
+        if (which == 0) return trilinear(t, [this, ch](int x, int y, int z) { return shadow_texel(x, y, z, ch); });
+        return trilinear(t, [this](int x, int y, int z) { return ao_texel(x, y, z); });
+    }
+    // texture_cubic (common.glsl:141-151): 8 linear fetches at the B-spline taps.
+    R light_cubic(int which, V3<R> t, int ch) const {
+        Taps k = cubic_taps(t);
+        auto f = [&](R x, R y, R z) { return light_linear(which, V3<R>(x, y, z), ch); };
+        R z0 = k.g0.y * (k.g0.x * f(k.h0.x, k.h0.y, k.h0.z) + k.g1.x * f(k.h1.x, k.h0.y, k.h0.z)) +
+               k.g1.y * (k.g0.x * f(k.h0.x, k.h1.y, k.h0.z) + k.g1.x * f(k.h1.x, k.h1.y, k.h0.z));
+        R z1 = k.g0.y * (k.g0.x * f(k.h0.x, k.h0.y, k.h1.z) + k.g1.x * f(k.h1.x, k.h0.y, k.h1.z)) +
+               k.g1.y * (k.g0.x * f(k.h0.x, k.h1.y, k.h1.z) + k.g1.x * f(k.h1.x, k.h1.y, k.h1.z));
+        return k.g0.z * z0 + k.g1.z * z1;
+    }
+    // lighting_lookup (common.glsl:154-156) at texture coordinate tc: cubic with
+    // smooth sampling, else one linear fetch.
+    R lighting_lookup(int which, V3<R> tc, int ch) const {
+        const int* size = which == 0 ? sdims : adims;
+        V3<R> t = V3<R>(tc.x * R(size[0]), tc.y * R(size[1]), tc.z * R(size[2])) - V3<R>(R(0.5));
+        return u->env[1] > 0.5f ? light_cubic(which, t, ch) : light_linear(which, t, ch);
+    }
+    V3<R> index_to_tex(V3<R> idx) const {  // common.glsl:68-70
+        return V3<R>((idx.x + R(0.5)) / R(u->dims[0]), (idx.y + R(0.5)) / R(u->dims[1]),
+                     (idx.z + R(0.5)) / R(u->dims[2]));
+    }
+    V3<R> tex_to_index(V3<R> t) const {  // common.glsl:72-74
+        return V3<R>(t.x * R(u->dims[0]) - R(0.5), t.y * R(u->dims[1]) - R(0.5), t.z * R(u->dims[2]) - R(0.5));
+    }
 
     // ---- transfer function (common.glsl:170-230) -------------------------------------
     V4<R> transfer(R density) const {
@@ -673,11 +732,18 @@ template <class R> struct Ctx {
                 R extinction_scale = surface_frame(idx, s_, v, cap_plane, cut, R(1.5) * step_mm, n, surfaceness);
                 apply_surface_texture(idx, s_);
                 if (s_.ghost) alpha = R(1) - std::exp(-s_.extinction * extinction_scale * step_mm);
-                // Shadow and ambient-occlusion volumes are not part of this oracle: the
-                // callers set lighting.x = lighting.y = 0 (checked by the caller).
-                R shadow = R(1);
+                // dvr.comp:107-119: lighting volumes looked up just outside the surface,
+                // the diffused light a little below it.
+                M3<R> ti = to_index();
+                V3<R> shadow_at = index_to_tex(idx + ti.mul(n * (R(1.5) * R(u->light_cells[0]) * surfaceness)));
+                V3<R> ao_at = index_to_tex(idx + ti.mul(n * (R(u->light_cells[1]) * surfaceness)));
+                R shadow = u->lighting[0] != 0u ? lighting_lookup(0, shadow_at, 3) : R(1);
+                V3<R> sss_at = index_to_tex(idx - ti.mul(n * (R(u->subsurface[3]) * surfaceness)));
                 V3<R> diffused(R(1));
-                R ao = R(1);
+                if (u->lighting[0] != 0u)
+                    diffused = V3<R>(lighting_lookup(0, sss_at, 0), lighting_lookup(0, sss_at, 1),
+                                     lighting_lookup(0, sss_at, 2));
+                R ao = u->lighting[1] != 0u ? lighting_lookup(1, ao_at, 0) : R(1);
                 n = bump_normal(idx, s_, n);
                 n = dot(n, v) < R(0) ? -n : n;
                 R nl = gmax(dot(n, l), R(0));
@@ -697,6 +763,61 @@ template <class R> struct Ctx {
         out[1] = (float)radiance.y;
         out[2] = (float)radiance.z;
         out[3] = (float)transmittance;
+    }
+
+    // ---- shadow.comp:18-68: texel p of slice p[axis] from the previous slice ----------
+    void shadow_texel_pass(const int p[3], int axis, R out[4]) const {
+        int u_axis = (axis + 1) % 3, v_axis = (axis + 2) % 3;
+        V3<R> sd = V3<R>(R(sdims[0]), R(sdims[1]), R(sdims[2]));
+        V3<R> ld(u->light_dir[0], u->light_dir[1], u->light_dir[2]);
+        V3<R> l_idx = to_index().mul(ld);
+        l_idx = V3<R>(l_idx.x * (sd.x / R(u->dims[0])), l_idx.y * (sd.y / R(u->dims[1])),
+                      l_idx.z * (sd.z / R(u->dims[2])));
+        R la[3] = {l_idx.x, l_idx.y, l_idx.z};
+        R step = R(1) / gmax(std::fabs(la[axis]), R(1e-6));
+        V3<R> here = V3<R>(R(p[0]), R(p[1]), R(p[2]));
+        V3<R> prev = here + l_idx * step;
+        V3<R> mid = (here + prev) * R(0.5) + V3<R>(R(0.5));
+        V3<R> mid_tex(mid.x / sd.x, mid.y / sd.y, mid.z / sd.z);
+        Sample m = sample_point(tex_to_index(mid_tex));
+        R direct = light_linear(0, prev, 3) * std::exp(-m.extinction * step);
+        R au[3] = {0, 0, 0}, av[3] = {0, 0, 0};
+        au[u_axis] = R(u->subsurface[2]);
+        av[v_axis] = R(u->subsurface[2]);
+        V3<R> axis_u(au[0], au[1], au[2]), axis_v(av[0], av[1], av[2]);
+        V3<R> gathered = V3<R>(light_linear(0, prev, 0), light_linear(0, prev, 1), light_linear(0, prev, 2)) * R(0.4);
+        for (int i = 0; i < 4; i++) {
+            V3<R> offset = (i & 1) == 0 ? axis_u : axis_v;
+            offset = offset * (i < 2 ? R(1) : R(-1));
+            V3<R> at = prev + offset;
+            gathered = gathered +
+                       V3<R>(light_linear(0, at, 0), light_linear(0, at, 1), light_linear(0, at, 2)) * R(0.15);
+        }
+        R mx = gmax(gmax(m.albedo.x, gmax(m.albedo.y, m.albedo.z)), R(1e-4));
+        V3<R> chroma = m.albedo / mx;
+        V3<R> ind_ext = (V3<R>(R(1.2)) - chroma) * (m.extinction * R(u->subsurface[1]));
+        V3<R> indirect(gathered.x * std::exp(-ind_ext.x * step), gathered.y * std::exp(-ind_ext.y * step),
+                       gathered.z * std::exp(-ind_ext.z * step));
+        out[0] = indirect.x, out[1] = indirect.y, out[2] = indirect.z, out[3] = direct;
+    }
+
+    // ---- ao.comp:20-43 ----------------------------------------------------------------------
+    R ao_texel_pass(const int p[3]) const {
+        V3<R> tc((R(p[0]) + R(0.5)) / R(adims[0]), (R(p[1]) + R(0.5)) / R(adims[1]), (R(p[2]) + R(0.5)) / R(adims[2]));
+        V3<R> center = tex_to_index(tc);
+        M3<R> ti = to_index();
+        R dt = R(15.0) / R(8);
+        R visible = 0;
+        for (int i = 0; i < 16; i++) {
+            R z = R(1) - (R(i) + R(0.5)) * (R(2) / R(16));
+            R r = std::sqrt(R(1) - z * z);
+            R phi = R(i) * R(2.39996323f);
+            V3<R> d = ti.mul(V3<R>(r * std::cos(phi), r * std::sin(phi), z));
+            R depth = 0;
+            for (int j = 0; j < 8; j++) depth += sample_point(center + d * ((R(j) + R(0.5)) * dt)).extinction * dt;
+            visible += std::exp(-depth);
+        }
+        return gmin(R(2) * visible / R(16), R(1));
     }
 
     // ---- pt.comp ----------------------------------------------------------------------------
@@ -1223,6 +1344,43 @@ struct MaskPass {
 
 // ============================== C entry points ===============================================
 
+namespace {
+
+// The sweep's axis and direction (renderer.rs:1914-1928): the light's dominant
+// axis in shadow-volume index units, slices from the lit side.
+template <class R> void shadow_sweep(const oracle_scene* s, const vvr_uniforms* u, uint16_t* out) {
+    Ctx<R> c(s, u);
+    c.shadow_img = out;
+    const int* sd = c.sdims;
+    float l[3];
+    for (int a = 0; a < 3; a++) {
+        const float* m = u->world_to_index;
+        float v = (m[a] * u->light_dir[0] + m[4 + a] * u->light_dir[1]) + m[8 + a] * u->light_dir[2];
+        l[a] = v * ((float)sd[a] / u->dims[a]);
+    }
+    float ax = std::fabs(l[0]), ay = std::fabs(l[1]), az = std::fabs(l[2]);
+    int axis = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+    int sign = l[axis] >= 0.0f ? 1 : -1;
+    int ua = (axis + 1) % 3, va = (axis + 2) % 3, n = sd[axis];
+    c.sweep_axis = axis, c.sweep_sign = sign;
+    for (int i = 0; i < n; i++) {
+        int slice = sign > 0 ? n - 1 - i : i;
+        c.sweep_slice = slice;
+#pragma omp parallel for schedule(static)
+        for (int b = 0; b < sd[va]; b++)
+            for (int a = 0; a < sd[ua]; a++) {
+                int p[3];
+                p[axis] = slice, p[ua] = a, p[va] = b;
+                R v[4];
+                c.shadow_texel_pass(p, axis, v);
+                size_t idx = (size_t)p[0] + (size_t)sd[0] * ((size_t)p[1] + (size_t)sd[1] * p[2]);
+                for (int ch = 0; ch < 4; ch++) out[4 * idx + ch] = f32_to_f16((float)v[ch]);
+            }
+    }
+}
+
+}  // namespace
+
 extern "C" {
 
 uint16_t oracle_f32_to_f16(float f) { return f32_to_f16(f); }
@@ -1296,6 +1454,37 @@ void oracle_dvr(const oracle_scene* s, const vvr_uniforms* u, const uint32_t* pi
         } else {
             Ctx<float> c(s, u);
             c.dvr_pixel(pixels[2 * i], pixels[2 * i + 1], rgba + 4 * i);
+        }
+    }
+}
+
+void oracle_shadow(const oracle_scene* s, const vvr_uniforms* u, int precision, uint16_t* out) {
+    if (precision == 64) shadow_sweep<double>(s, u, out);
+    else shadow_sweep<float>(s, u, out);
+}
+
+void oracle_ao(const oracle_scene* s, const vvr_uniforms* u, int precision, uint16_t* out) {
+    Ctx<float> c(s, u);
+    const int* ad = c.adims;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int z = 0; z < ad[2]; z++)
+        for (int y = 0; y < ad[1]; y++)
+            for (int x = 0; x < ad[0]; x++) {
+                int p[3] = {x, y, z};
+                float v = precision == 64 ? (float)Ctx<double>(s, u).ao_texel_pass(p) : Ctx<float>(s, u).ao_texel_pass(p);
+                out[(size_t)x + (size_t)ad[0] * ((size_t)y + (size_t)ad[1] * z)] = f32_to_f16(v);
+            }
+}
+
+void oracle_lighting_lookup(const oracle_scene* s, const vvr_uniforms* u, int which, const float tc[3], int precision,
+                            float out[4]) {
+    for (int ch = 0; ch < (which == 0 ? 4 : 1); ch++) {
+        if (precision == 64) {
+            Ctx<double> c(s, u);
+            out[ch] = (float)c.lighting_lookup(which, V3<double>(tc[0], tc[1], tc[2]), ch);
+        } else {
+            Ctx<float> c(s, u);
+            out[ch] = c.lighting_lookup(which, V3<float>(tc[0], tc[1], tc[2]), ch);
         }
     }
 }

@@ -195,7 +195,7 @@ def test_pt_accumulates_across_frames(ctx):
     assert r.accumulated_samples() == 2
 
 
-def test_record_before_resize_and_unsupported_lighting_fail(ctx):
+def test_record_before_resize_and_unsupported_denoiser_fail(ctx):
     hu = phantom.ct((32, 32, 32), 1)
     r = Renderer(ctx, hu, TransferFunction.preset(0))
     p = frame_params(hu)
@@ -203,7 +203,8 @@ def test_record_before_resize_and_unsupported_lighting_fail(ctx):
         r.record(1, p)
     assert e.value.code == _abi.VVR_ESTATE
     r.resize(8, 8)
-    p.lighting.shadows = 1
+    p = frame_params(hu, path_tracing=True)
+    p.path_tracing.denoiser = _abi.DENOISE_LEARNED
     with pytest.raises(_abi.VvrError) as e:
         r.record(1, p)
     assert e.value.code == _abi.VVR_EINVAL

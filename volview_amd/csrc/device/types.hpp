@@ -50,6 +50,12 @@ struct DevScene {
     const float4* __restrict__ lut;     // 4096 transfer-function entries, fp16-rounded
     const float4* __restrict__ preint;  // 4096 cumulative integrals
     const vvr_uniforms* __restrict__ u;
+    // Lighting volumes of the ray marcher (null when the frame does not use them):
+    // RGBA16F diffused light + direct transmittance at ceil(dims / 2), R16F
+    // ambient visibility at ceil(dims / 4), both x fastest.
+    const uint2* __restrict__ shadow;
+    const uint16_t* __restrict__ ao;
+    int sdims[3], adims[3];
 };
 
 // One ray-marched frame of a batched launch: its tiles are workgroups

@@ -5,6 +5,7 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include "device/lighting.hpp"
 #include "device/pt.hpp"
 #include "launch.hpp"
 
@@ -33,8 +34,8 @@ __device__ __forceinline__ void tile_pixel(uint32_t& px, uint32_t& py) {
     py = blockIdx.y * 8u + (threadIdx.x >> 3);
 }
 
-// dvr.comp:23-147. Shadow and ambient-occlusion volumes are not built: the host
-// rejects frames that ask for them.
+// dvr.comp:23-147. The lighting volumes (s.shadow, s.ao) are built before the
+// frame when its uniforms turn them on.
 __device__ __forceinline__ void dvr_pixel(const DevScene& sc, uint32_t px, uint32_t py, uint16_t* __restrict__ hdr,
                                           uint32_t width) {
     const vvr_uniforms* u = sc.u;
@@ -100,9 +101,17 @@ __device__ __forceinline__ void dvr_pixel(const DevScene& sc, uint32_t px, uint3
             const float extinction_scale = surface_frame(sc, idx, s, v, cap_plane, cut, 1.5f * step_mm, n, surfaceness);
             apply_surface_texture(u, idx, s);
             if (s.ghost) alpha = 1.0f - expf(-s.extinction * extinction_scale * step_mm);
-            const float shadow = 1.0f;
-            const V3 diffused(1.0f);
-            const float ao = 1.0f;
+            // dvr.comp:107-119: the coarse lighting volumes are read just outside the
+            // surface, the diffused light a little below it.
+            float shadow = 1.0f, ao = 1.0f;
+            V3 diffused(1.0f);
+            if (u->lighting[0] != 0u) {
+                shadow = shadow_lookup(sc, index_to_tex(u, idx + ti.mul(n * (1.5f * u->light_cells[0] * surfaceness)))).w;
+                const float4 d = shadow_lookup(sc, index_to_tex(u, idx - ti.mul(n * (u->subsurface[3] * surfaceness))));
+                diffused = V3(d.x, d.y, d.z);
+            }
+            if (u->lighting[1] != 0u)
+                ao = ao_lookup(sc, index_to_tex(u, idx + ti.mul(n * (u->light_cells[1] * surfaceness))));
             n = bump_normal(u, idx, s, n);
             n = dot(n, v) < 0.0f ? -n : n;
             const float nl = gmax(dot(n, l), 0.0f);

@@ -34,6 +34,11 @@ void launch_majorant(const uint16_t* density, const uint32_t* mask, float* out, 
 
 // dvr.comp: hdr = RGBA16F (width x height x 4 halves).
 void launch_dvr(const DevScene& s, uint16_t* hdr, uint32_t width, uint32_t height, hipStream_t st);
+// Lighting volumes of the ray marcher (shadow.comp, ao.comp) into s.sdims /
+// s.adims texel arrays; `hu` is the host copy of the uniforms s.u points to.
+void shadow_sweep_axis(const vvr_uniforms& hu, int& axis, int& sign);
+void launch_shadow_sweep(const DevScene& s, uint2* out, const vvr_uniforms& hu, hipStream_t st);
+void launch_ao(const DevScene& s, uint16_t* out, hipStream_t st);
 
 // dvr.comp for n_jobs frames in one launch (`jobs` in device memory, sorted by
 // tile_base; total_tiles = the last job's tile_base + its tile count).

@@ -143,7 +143,7 @@ void record(vvr_renderer* r, uint64_t frame, const vvr_frame_params* p) {
     vvr_uniforms u;
     pack_uniforms(r->geo, *p, r->tf.bucket_max, r->has_labels, r->accumulated, r->width, r->height, frame, &u);
     const vvr_uniforms* du = r->upload_uniforms(u, slot);
-    r->prepare(*p, du, r->stream);
+    r->prepare(*p, u, du, r->stream);
     const DevScene sc = r->scene(du);
     if (pt) {
         const uint32_t spf = std::max(p->path_tracing.samples_per_frame, 1u);
@@ -321,6 +321,19 @@ int vvr_renderer_read_volume(vvr_renderer* r, int which, void* out, size_t bytes
                 const size_t nb = size_t(r->mdims[0]) * r->mdims[1] * r->mdims[2];
                 if (bytes != nb * 4) throw Error{VVR_EINVAL, "majorant read-back size mismatch"};
                 check(hipMemcpyAsync(out, r->majorant, bytes, hipMemcpyDeviceToHost, r->stream), "read-back");
+                check(hipStreamSynchronize(r->stream), "read-back");
+                return;
+            }
+            case VVR_VOL_SHADOW:
+            case VVR_VOL_AO: {  // linear already
+                const bool sh = which == VVR_VOL_SHADOW;
+                const void* lv = sh ? static_cast<const void*>(r->shadow) : static_cast<const void*>(r->ao);
+                if (!lv) throw Error{VVR_ESTATE, "that lighting volume has not been built (record a ray-marched frame "
+                                                 "with it on first)"};
+                size_t n = 1;
+                for (int a = 0; a < 3; a++) n *= size_t(sh ? r->shadow_dims(a) : r->ao_dims(a));
+                if (bytes != n * (sh ? 8 : 2)) throw Error{VVR_EINVAL, "lighting read-back size mismatch"};
+                check(hipMemcpyAsync(out, lv, bytes, hipMemcpyDeviceToHost, r->stream), "read-back");
                 check(hipStreamSynchronize(r->stream), "read-back");
                 return;
             }

@@ -184,11 +184,36 @@ struct WfEngine {
 };
 
 // The frame parameters record() and the server accept.
+// What a lighting volume depends on beyond the opacity key (renderer.rs:1663-1667,
+// 1707-1749): the clip planes as uploaded and the label colours and densities
+// sample_point reads; the shadow volume also on the light direction and the
+// subsurface terms. Words of the uniform block, compared exactly.
+struct LightKey {
+    OpacityKey opacity;
+    std::vector<uint32_t> words;
+    bool operator==(const LightKey& o) const { return opacity == o.opacity && words == o.words; }
+    bool operator!=(const LightKey& o) const { return !(*this == o); }
+};
+
+inline LightKey light_key(const OpacityKey& opacity, const vvr_uniforms& u, bool shadow) {
+    LightKey k{opacity, {}};
+    auto add = [&](const void* p, size_t bytes) {
+        const uint32_t* w = static_cast<const uint32_t*>(p);
+        k.words.insert(k.words.end(), w, w + bytes / 4);
+    };
+    add(u.clip, sizeof(u.clip));
+    add(u.clip_planes, sizeof(u.clip_planes));
+    add(u.label_colors, sizeof(u.label_colors));
+    add(u.label_density, sizeof(u.label_density));
+    if (shadow) {
+        add(u.light_dir, 3 * sizeof(float));
+        add(&u.subsurface[1], 2 * sizeof(float));
+    }
+    return k;
+}
+
 inline void validate_frame(const vvr_frame_params& p) {
     const bool pt = p.path_tracing_enabled != 0;
-    if (!pt && (p.lighting.shadows || p.lighting.ambient_occlusion))
-        throw Error{VVR_EINVAL, "shadow and ambient-occlusion volumes are not available: set lighting.shadows and "
-                                "lighting.ambient_occlusion to 0 for ray marching"};
     if (pt && p.path_tracing.denoiser != VVR_DENOISE_OFF)
         throw Error{VVR_EINVAL, "no denoiser is available: set path_tracing.denoiser to VVR_DENOISE_OFF"};
     if (!(p.step_mm > 0.0f)) throw Error{VVR_EINVAL, "step_mm must be positive"};
@@ -231,6 +256,10 @@ struct vvr_renderer {
     bool has_labels = false;
     uint64_t generation = 0;
     std::optional<OpacityKey> mask_key, majorant_key;
+    // Lighting volumes of the ray marcher, allocated on first use.
+    uint2* shadow = nullptr;   // RGBA16F at ceil(dims / 2)
+    uint16_t* ao = nullptr;    // R16F at ceil(dims / 4)
+    std::optional<LightKey> shadow_key, ao_key;
 
     uint32_t width = 0, height = 0;
     uint16_t* hdr = nullptr;
@@ -254,15 +283,38 @@ struct vvr_renderer {
         hipEvent_t a, b;
     };
     std::vector<Pending> pending;
-    double total_ms[VVR_KERNEL_COUNT] = {0, 0, 0, 0, 0};
-    uint64_t launches[VVR_KERNEL_COUNT] = {0, 0, 0, 0, 0};
+    double total_ms[VVR_KERNEL_COUNT] = {};
+    uint64_t launches[VVR_KERNEL_COUNT] = {};
 
     void use_device() const { check(hipSetDevice(device), "hipSetDevice"); }
 
+    int shadow_dims(int a) const { return (int(geo.dims[a]) + 1) / 2; }  // SHADOW_DOWNSAMPLE (renderer.rs:35-36)
+    int ao_dims(int a) const { return (int(geo.dims[a]) + 3) / 4; }      // AO_DOWNSAMPLE (renderer.rs:39-40)
+
+    // Whether `p` needs the label masks or the majorant rebuilt.
+    bool needs_opacity_rebuild(const vvr_frame_params& p) const {
+        const OpacityKey key = opacity_key(p.labels, generation, has_labels, geo.min_spacing);
+        if (!majorant_key || *majorant_key != key) return true;
+        return key.mode != VVR_LABELS_OFF && (!mask_key || *mask_key != key);
+    }
+    // Whether `p`, a ray-marched frame with uniforms `hu` packed from it, needs a
+    // lighting volume rebuilt.
+    bool needs_lighting_rebuild(const vvr_frame_params& p, const vvr_uniforms& hu) const {
+        if (p.path_tracing_enabled) return false;
+        const OpacityKey key = opacity_key(p.labels, generation, has_labels, geo.min_spacing);
+        if (p.lighting.ambient_occlusion && (!ao_key || *ao_key != light_key(key, hu, false))) return true;
+        return p.lighting.shadows && (!shadow_key || *shadow_key != light_key(key, hu, true));
+    }
+    bool needs_prepare(const vvr_frame_params& p, const vvr_uniforms& hu) const {
+        return needs_opacity_rebuild(p) || needs_lighting_rebuild(p, hu);
+    }
+
     // The label masks and the majorant grid for the opacity key of `p`
-    // (renderer.rs:1468-1495, 1671-1706), rebuilt on `st` when the key changed.
-    // `du` is a uniform block packed from `p`. Timed when `st` is the renderer's stream.
-    OpacityKey prepare(const vvr_frame_params& p, const vvr_uniforms* du, hipStream_t st) {
+    // (renderer.rs:1468-1495, 1671-1706), then for a ray-marched frame the
+    // ambient-occlusion and shadow volumes (renderer.rs:1707-1749), each
+    // rebuilt on `st` when its key changed. `du` is the device copy of the
+    // uniforms `hu` packed from `p`. Timed when `st` is the renderer's stream.
+    OpacityKey prepare(const vvr_frame_params& p, const vvr_uniforms& hu, const vvr_uniforms* du, hipStream_t st) {
         const OpacityKey opacity = opacity_key(p.labels, generation, has_labels, geo.min_spacing);
         const BrickGrid& g = grid;
         auto run = [&](int which, auto f) {
@@ -288,6 +340,25 @@ struct vvr_renderer {
             });
             majorant_key = opacity;
         }
+        if (p.path_tracing_enabled) return opacity;  // the path tracer computes its own lighting
+        const size_t s_texels = size_t(shadow_dims(0)) * shadow_dims(1) * shadow_dims(2);
+        const size_t a_texels = size_t(ao_dims(0)) * ao_dims(1) * ao_dims(2);
+        if (p.lighting.ambient_occlusion) {
+            const LightKey key = light_key(opacity, hu, false);
+            if (!ao_key || *ao_key != key) {
+                if (!ao) ao = dalloc<uint16_t>(a_texels, "ambient-occlusion volume");
+                run(VVR_KERNEL_LIGHTING, [&] { launch_ao(scene(du), ao, st); });
+                ao_key = key;
+            }
+        }
+        if (p.lighting.shadows) {
+            const LightKey key = light_key(opacity, hu, true);
+            if (!shadow_key || *shadow_key != key) {
+                if (!shadow) shadow = dalloc<uint2>(s_texels, "shadow volume");
+                run(VVR_KERNEL_LIGHTING, [&] { launch_shadow_sweep(scene(du), shadow, hu, st); });
+                shadow_key = key;
+            }
+        }
         return opacity;
     }
 
@@ -303,6 +374,9 @@ struct vvr_renderer {
         s.lut = lut;
         s.preint = preint;
         s.u = u;
+        s.shadow = shadow;
+        s.ao = ao;
+        for (int a = 0; a < 3; a++) s.sdims[a] = shadow_dims(a), s.adims[a] = ao_dims(a);
         return s;
     }
 
@@ -373,6 +447,7 @@ struct vvr_renderer {
         free_frame();
         wf.reset();
         dfree(density), dfree(normals), dfree(majorant), dfree(labels), dfree(mask), dfree(mask_b), dfree(soft);
+        dfree(shadow), dfree(ao);
         dfree(lut), dfree(preint), dfree(d_uniforms[0]), dfree(d_uniforms[1]), dfree(queue);
         if (h_uniforms) (void)hipHostFree(h_uniforms);
         for (auto& e : slot_done)

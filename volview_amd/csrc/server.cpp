@@ -66,6 +66,7 @@ struct Job {
     vvr_frame_params params{};
     uint32_t width = 0, height = 0;
     bool pt = false;
+    bool lit = false;  // ray marched with shadow or ambient-occlusion volumes
     // path tracing: spp_total paths per pixel in n_inc increments of `increment`
     uint32_t spp_total = 0, increment = 0, n_inc = 0;
     uint32_t inc_started = 0;
@@ -146,7 +147,8 @@ struct vvr_server {
     vvr_uniforms* h_uniforms = nullptr;
     std::vector<int> free_uslots;
 
-    std::map<vvr_renderer*, int> readers;  // admitted requests per volume
+    std::map<vvr_renderer*, int> readers;        // admitted requests per volume
+    std::map<vvr_renderer*, int> light_readers;  // of them, ray-marched ones reading its lighting volumes
 
     // requests
     std::mutex mu;
@@ -197,6 +199,7 @@ void vvr_server::release_memory(Job* j) {
     }
     j->bytes = 0;
     if (j->vol) readers[j->vol]--;
+    if (j->vol && j->lit) light_readers[j->vol]--;
 }
 
 // Ends a request: frees its device memory, marks it done and publishes its
@@ -229,10 +232,14 @@ void vvr_server::admit(std::vector<Job*>& fresh_dvr) {
         if (stats.hbm_in_use + need > cfg.hbm_budget && !running.empty()) continue;
         if (j->pt && (entries_in_use >= kTable || n_active >= uint32_t(kWfMaxActive))) continue;
         vvr_renderer* v = j->vol;
-        const OpacityKey key = opacity_key(j->params.labels, v->generation, v->has_labels, v->geo.min_spacing);
-        const bool rebuild = !v->mask_key || !v->majorant_key || *v->majorant_key != key ||
-                             (key.mode != VVR_LABELS_OFF && *v->mask_key != key);
-        if (rebuild && readers[v] > 0) continue;  // wait until no admitted request reads this volume
+        vvr_uniforms packed;
+        pack_uniforms(v->geo, j->params, v->tf.bucket_max, v->has_labels, 0, j->width, j->height, j->id, &packed);
+        // A derived volume is rebuilt only while no admitted request reads it:
+        // masks and majorant wait for every reader of the volume, the lighting
+        // volumes for the ray-marched requests that read them.
+        if (v->needs_opacity_rebuild(j->params) && readers[v] > 0) continue;
+        if (!j->pt && v->needs_lighting_rebuild(j->params, packed) && light_readers[v] > 0) continue;
+        const bool rebuild = v->needs_prepare(j->params, packed);
         {
             std::lock_guard<std::mutex> lk(mu);
             queue.erase(std::find(queue.begin(), queue.end(), j));
@@ -241,6 +248,8 @@ void vvr_server::admit(std::vector<Job*>& fresh_dvr) {
             j->uslot = free_uslots.back();
             free_uslots.pop_back();
             readers[v]++;
+            j->lit = !j->pt && (j->params.lighting.shadows || j->params.lighting.ambient_occlusion);
+            if (j->lit) light_readers[v]++;
             const size_t px = size_t(j->width) * j->height;
             j->hdr = dalloc<uint16_t>(4 * px, "request image");
             if (j->pt) {
@@ -260,10 +269,10 @@ void vvr_server::admit(std::vector<Job*>& fresh_dvr) {
                 running.push_back(j);
             }
             vvr_uniforms& u = h_uniforms[j->uslot];
-            pack_uniforms(v->geo, j->params, v->tf.bucket_max, v->has_labels, 0, j->width, j->height, j->id, &u);
+            u = packed;
             vvr_uniforms* du = d_uniforms + j->uslot;
             check(hipMemcpyAsync(du, &u, sizeof(u), hipMemcpyHostToDevice, stream), "uniform upload");
-            if (rebuild) v->prepare(j->params, du, stream);
+            if (rebuild) v->prepare(j->params, u, du, stream);
             j->t_admit = now_ns();
             j->state = Job::kRunning;
             if (j->pt) start_increment(j);

@@ -220,6 +220,10 @@ class Renderer:
             out = np.zeros((nz, ny, nx, 4), np.int8)
         elif which == _abi.VOL_MASK:
             out = np.zeros((nz, ny, nx, 2), np.uint16)
+        elif which == _abi.VOL_SHADOW:  # RGBA16F bits
+            out = np.zeros(((nz + 1) // 2, (ny + 1) // 2, (nx + 1) // 2, 4), np.uint16)
+        elif which == _abi.VOL_AO:  # R16F bits
+            out = np.zeros(((nz + 3) // 4, (ny + 3) // 4, (nx + 3) // 4), np.uint16)
         else:
             raise ValueError(which)
         check(self._lib.vvr_renderer_read_volume(self.handle, which, out.ctypes.data_as(c_void_p), out.nbytes))

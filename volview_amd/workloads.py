@@ -66,11 +66,13 @@ def seeded_camera(renderer: Renderer, index: int):
     return cam
 
 
-def dvr_params(renderer: Renderer, camera_index: int = 0) -> _abi.FrameParams:
+def dvr_params(renderer: Renderer, camera_index: int = 0, lighting: bool = False) -> _abi.FrameParams:
+    """DVR-1 settings on a seeded camera; `lighting` turns the shadow and
+    ambient-occlusion volumes on (the reference's default Lighting)."""
     g = renderer.geometry()
     p = default_frame_params(seeded_camera(renderer, camera_index), g["min_spacing"])
-    p.lighting.shadows = 0
-    p.lighting.ambient_occlusion = 0
+    p.lighting.shadows = int(lighting)
+    p.lighting.ambient_occlusion = int(lighting)
     p.lighting.environment = _abi.ENV_STUDIO
     return p
 
@@ -91,11 +93,12 @@ def pt_params(renderer: Renderer, n_labels: int, spp: int, bounces: int = 8, g: 
     return p
 
 
-def dvr_scene(ctx: Context, dims=(256, 256, 256), seed: int = 1, size=(512, 512), camera_index: int = 0) -> Scene:
+def dvr_scene(ctx: Context, dims=(256, 256, 256), seed: int = 1, size=(512, 512), camera_index: int = 0,
+              lighting: bool = False) -> Scene:
     hu = phantom.ct(dims, seed)
     r = Renderer(ctx, hu, TransferFunction.preset(0))
     r.resize(*size)
-    return Scene(r, dvr_params(r, camera_index), size[0], size[1], hu=hu)
+    return Scene(r, dvr_params(r, camera_index, lighting), size[0], size[1], hu=hu)
 
 
 def pt_scene(ctx: Context, dims=(512, 512, 400), seed: int = 1, size=(1024, 1024), spp: int = 64, n_organs: int = 40,
