@@ -257,7 +257,9 @@ int vvr_renderer_read_hdr(vvr_renderer* r, float* rgba);
 /* Path-tracing images (bindings 17-19): accumulated radiance+background, albedo
  * guide+luminance^2, normal guide+depth; width*height*4 floats each. Any may be NULL. */
 int vvr_renderer_read_accum(vvr_renderer* r, float* accum, float* albedo, float* normal);
-/* The a-trous filtered image (denoise.comp output) when the denoiser ran. */
+/* The denoised image of the last frame (renderer.rs:1795-1797: with
+ * VVR_DENOISE_ATROUS the HDR image holds denoise.comp's output); -5 when the
+ * last frame was not denoised. */
 int vvr_renderer_read_denoised(vvr_renderer* r, float* rgba);
 
 enum {
@@ -281,7 +283,7 @@ int vvr_renderer_read_uniforms(vvr_renderer* r, void* out, size_t bytes);
 /* ---- measurement ------------------------------------------------------------ */
 
 enum { VVR_KERNEL_MAJORANT = 0, VVR_KERNEL_NORMALS = 1, VVR_KERNEL_MASK = 2, VVR_KERNEL_DVR = 3, VVR_KERNEL_PT = 4,
-       VVR_KERNEL_LIGHTING = 5, VVR_KERNEL_COUNT = 6 };
+       VVR_KERNEL_LIGHTING = 5, VVR_KERNEL_DENOISE = 6, VVR_KERNEL_COUNT = 7 };
 /* Enables HIP-event timing of each kernel on the stream it is launched on. */
 int vvr_renderer_set_timing(vvr_renderer* r, int enabled);
 /* Total milliseconds and launches of kernel `which` since timing was enabled. */

@@ -71,6 +71,8 @@ def load() -> ctypes.CDLL:
             "oracle_ao": (None, [P(OracleScene), P(_abi.Uniforms), c_int, P(c_uint16)]),
             "oracle_lighting_lookup": (None, [P(OracleScene), P(_abi.Uniforms), c_int, P(c_float), c_int,
                                               P(c_float)]),
+            "oracle_denoise": (None, [P(c_float), P(c_float), P(c_float), c_uint32, c_uint32, c_float, c_int, c_int,
+                                      P(c_float), P(c_float)]),
             "oracle_pt": (None, [P(OracleScene), P(_abi.Uniforms), P(c_uint32), c_size_t, c_int, P(c_float),
                                  P(c_float), P(c_float)]),
             "oracle_density_cubic": (c_float, [P(OracleScene), P(_abi.Uniforms), P(c_float), c_int]),
@@ -293,6 +295,18 @@ class Scene:
         load().oracle_transmittance(byref(self.c), byref(u), (c_float * 3)(*origin_idx), (c_float * 3)(*direction),
                                     seed, out)
         return np.array(out)
+
+
+def denoise(accum, albedo, normal, samples: float, levels: int = 5, precision: int = 32):
+    """denoise.comp over (h, w, 4) float images: (hdr before the RGBA16F store,
+    last filtered irradiance + variance)."""
+    a, b, c = (np.ascontiguousarray(x, np.float32) for x in (accum, albedo, normal))
+    h, w = a.shape[:2]
+    hdr = np.zeros((h, w, 4), np.float32)
+    last = np.zeros((h, w, 4), np.float32)
+    load().oracle_denoise(_p(a, c_float), _p(b, c_float), _p(c, c_float), w, h, samples, levels, precision,
+                          _p(hdr, c_float), _p(last, c_float))
+    return hdr, last
 
 
 def pcg_sequence(seed: int, n: int) -> np.ndarray:

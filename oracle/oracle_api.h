@@ -71,6 +71,13 @@ void oracle_ao(const oracle_scene* s, const vvr_uniforms* u, int precision, uint
  * ambient-occlusion (which = 1: r) volume at texture coordinate tc. */
 void oracle_lighting_lookup(const oracle_scene* s, const vvr_uniforms* u, int which, const float tc[3], int precision,
                             float out[4]);
+/* denoise.comp over a whole image, as Renderer::record_denoise runs it
+ * (renderer.rs:1868-1904): prepare, `levels` a-trous levels (the reference runs
+ * 5), finish. Images are width x height float4 row-major: the accumulation and
+ * guide images and their sample count. hdr: the finished image before its
+ * RGBA16F store; last (may be NULL): the last filtered irradiance + variance. */
+void oracle_denoise(const float* accum, const float* albedo, const float* normal, uint32_t width, uint32_t height,
+                    float samples, int levels, int precision, float* hdr, float* last);
 /* pt.comp for `n` pixels: the values the dispatch adds to the accumulation,
  * albedo-guide and normal-guide images (4 floats each per pixel). */
 void oracle_pt(const oracle_scene* s, const vvr_uniforms* u, const uint32_t* pixels, size_t n, int precision,

@@ -4,7 +4,7 @@
 // Line-by-line restatement of the reference's compute shaders:
 //   shaders/common.glsl, shaders/shading.glsl, shaders/majorant.comp,
 //   shaders/normals.comp, shaders/mask.comp, shaders/dvr.comp, shaders/pt.comp,
-//   shaders/shadow.comp, shaders/ao.comp.
+//   shaders/shadow.comp, shaders/ao.comp, shaders/denoise.comp.
 // Each function names the lines it follows. Templated on the real type R: the
 // FP32 oracle (R = float, built without FMA contraction, so every operation is
 // one IEEE-rounded step in source order) and the FP64 oracle (R = double).
@@ -1379,6 +1379,119 @@ template <class R> void shadow_sweep(const oracle_scene* s, const vvr_uniforms* 
     }
 }
 
+// denoise.comp: the edge-avoiding a-trous filter of the progressive path
+// tracer (pass 0 prepare, passes 1..levels filter at steps 1, 2, 4, ..., the
+// finish into the HDR image), as Renderer::record_denoise runs it
+// (renderer.rs:1868-1904). Images are width x height float4, row-major.
+template <class R> struct Denoise {
+    const float *accum, *alb, *nrm;
+    int w, h;
+    R n;
+    static R lum(V3<R> c) { return dot(c, V3<R>(R(0.2126f), R(0.7152f), R(0.0722f))); }
+    V3<R> rgb(const float* img, int x, int y) const {
+        const float* p = img + 4 * ((size_t)y * w + x);
+        return V3<R>(p[0], p[1], p[2]);
+    }
+    V3<R> mean_albedo(int x, int y) const {  // denoise.comp:37-39
+        return vmax(rgb(alb, x, y) / n, V3<R>(R(0.02f)));
+    }
+    void mean_normal_depth(int x, int y, V3<R>& nd, R& depth) const {  // denoise.comp:41-45
+        const float* p = nrm + 4 * ((size_t)y * w + x);
+        V3<R> g = V3<R>(p[0], p[1], p[2]) / n;
+        depth = R(p[3]) / n;
+        R len = length(g);
+        nd = len > R(1e-3f) ? g / len : V3<R>(R(0));
+    }
+    int cl(int v, int hi) const { return v < 0 ? 0 : (v > hi ? hi : v); }
+
+    // denoise.comp:63-80
+    void prepare(int x, int y, R out[4]) const {
+        V3<R> radiance = rgb(accum, x, y) / n;
+        V3<R> irr = radiance / mean_albedo(x, y);
+        R l = lum(radiance);
+        R temporal = gmax(R(alb[4 * ((size_t)y * w + x) + 3]) / n - l * l, R(0)) / n;
+        R s1 = 0, s2 = 0;
+        for (int dy = -2; dy <= 2; dy++)
+            for (int dx = -2; dx <= 2; dx++) {
+                R lq = lum(rgb(accum, cl(x + dx, w - 1), cl(y + dy, h - 1)) / n);
+                s1 += lq;
+                s2 += lq * lq;
+            }
+        R spatial = gmax(s2 / R(25) - (s1 / R(25)) * (s1 / R(25)), R(0));
+        R variance = gmax(temporal, spatial * gmin(R(1), R(4) / n));
+        R a = lum(mean_albedo(x, y));
+        out[0] = irr.x, out[1] = irr.y, out[2] = irr.z, out[3] = variance / (a * a);
+    }
+
+    // denoise.comp:91-139: one level at `step` from `in`.
+    void filter(const std::vector<R>& in, int x, int y, int step, R out[4]) const {
+        auto at = [&](int qx, int qy, int c) { return in[4 * ((size_t)qy * w + qx) + c]; };
+        V3<R> cg;
+        R cd;
+        mean_normal_depth(x, y, cg, cd);
+        V3<R> albedo = mean_albedo(x, y);
+        V3<R> center(at(x, y, 0), at(x, y, 1), at(x, y, 2));
+        R l_center = lum(center);
+        R bv = 0;
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++)
+                bv += at(cl(x + dx, w - 1), cl(y + dy, h - 1), 3) * R((2 - std::abs(dx)) * (2 - std::abs(dy))) / R(16);
+        R sigma_l = R(4) * std::sqrt(gmax(bv, R(0))) + R(1e-4f);
+        const R kernel[3] = {R(3.0 / 8.0), R(1.0 / 4.0), R(1.0 / 16.0)};
+        V3<R> sum(R(0));
+        R var = 0, weights = 0;
+        for (int dy = -2; dy <= 2; dy++)
+            for (int dx = -2; dx <= 2; dx++) {
+                int qx = x + dx * step, qy = y + dy * step;
+                if (qx < 0 || qy < 0 || qx >= w || qy >= h) continue;
+                V3<R> c(at(qx, qy, 0), at(qx, qy, 1), at(qx, qy, 2));
+                V3<R> g;
+                R gd;
+                mean_normal_depth(qx, qy, g, gd);
+                R w_normal = std::pow(gmax(dot(cg, g), R(0)), R(64));
+                if (dot(cg, cg) < R(0.5) || dot(g, g) < R(0.5))
+                    w_normal = R(dot(cg, cg) < R(0.5) && dot(g, g) < R(0.5) ? 1 : 0);
+                R w_depth = std::exp(-std::fabs(cd - gd) / (R(0.02f) * cd + R(1e-3f)));
+                V3<R> da = albedo - mean_albedo(qx, qy);
+                R w_albedo = std::exp(-dot(da, da) / R(0.01f));
+                R w_lum = std::exp(-std::fabs(l_center - lum(c)) / sigma_l);
+                R wt = kernel[std::abs(dx)] * kernel[std::abs(dy)] * w_normal * w_depth * w_albedo * w_lum;
+                sum = sum + c * wt;
+                var += wt * wt * at(qx, qy, 3);
+                weights += wt;
+            }
+        if (weights > R(0)) {
+            out[0] = sum.x / weights, out[1] = sum.y / weights, out[2] = sum.z / weights;
+            out[3] = var / (weights * weights);
+        } else {
+            for (int c = 0; c < 4; c++) out[c] = at(x, y, c);
+        }
+    }
+
+    void run(int levels, float* hdr, float* last) const {
+        const size_t np = (size_t)w * h;
+        std::vector<R> a(4 * np), b(4 * np);
+#pragma omp parallel for schedule(static)
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) prepare(x, y, &a[4 * ((size_t)y * w + x)]);
+        for (int level = 0; level < levels; level++) {
+#pragma omp parallel for schedule(static)
+            for (int y = 0; y < h; y++)
+                for (int x = 0; x < w; x++) filter(a, x, y, 1 << level, &b[4 * ((size_t)y * w + x)]);
+            std::swap(a, b);
+        }
+        for (size_t i = 0; i < np; i++) {  // denoise.comp:84-89
+            int x = int(i % w), y = int(i / w);
+            V3<R> irr(a[4 * i], a[4 * i + 1], a[4 * i + 2]);
+            V3<R> o = irr * mean_albedo(x, y);
+            hdr[4 * i] = (float)o.x, hdr[4 * i + 1] = (float)o.y, hdr[4 * i + 2] = (float)o.z;
+            hdr[4 * i + 3] = (float)(R(accum[4 * i + 3]) / n);
+            if (last)
+                for (int c = 0; c < 4; c++) last[4 * i + c] = (float)a[4 * i + c];
+        }
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -1486,6 +1599,17 @@ void oracle_lighting_lookup(const oracle_scene* s, const vvr_uniforms* u, int wh
             Ctx<float> c(s, u);
             out[ch] = c.lighting_lookup(which, V3<float>(tc[0], tc[1], tc[2]), ch);
         }
+    }
+}
+
+void oracle_denoise(const float* accum, const float* albedo, const float* normal, uint32_t width, uint32_t height,
+                    float samples, int levels, int precision, float* hdr, float* last) {
+    if (precision == 64) {
+        Denoise<double> d{accum, albedo, normal, (int)width, (int)height, (double)samples};
+        d.run(levels, hdr, last);
+    } else {
+        Denoise<float> d{accum, albedo, normal, (int)width, (int)height, samples};
+        d.run(levels, hdr, last);
     }
 }
 

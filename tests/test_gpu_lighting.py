@@ -2,7 +2,7 @@
 marcher reading them (dvr.comp:107-119), through the C ABI.
 
 - The shadow volume swept on the GPU equals the FP32 oracle's sweep but for
-  one-ulp binary16 differences on a few texels: both round every slice through
+  binary16 differences of a few spacings on a few texels: both round every slice through
   RGBA16F, and expf and fused multiply-adds differ in the last f32 bit. Cameras 0-4 put the light along
   different dominant axes and signs; one case renders with tinted labels,
   whose colours feed the diffused light.
@@ -30,12 +30,17 @@ def f16(bits):
 
 
 def compare_f16(got, want, name):
-    """At most 0.5 % of texels differ, by at most one binary16 ulp (measured on
-    MI355X: up to 0.04 % for the shadow volume, none for occlusion)."""
-    ulp = np.maximum(np.abs(want), 2.0 ** -14) * 2.0 ** -10
+    """At most 0.5 % of texels differ, by at most 8 binary16 spacings of the
+    larger value. A one-spacing difference in one slice of the sweep carries
+    into the slices after it (measured on MI355X: 0.04 % of shadow texels
+    differ, by up to 4 spacings at DVR-1; occlusion is bit-exact)."""
+    big = np.maximum(np.abs(got), np.abs(want)).astype(np.float16)
+    spacing = np.spacing(big).astype(np.float64)
     err = np.abs(got - want)
-    print(f"{name}: max |d| {err.max():.3g}, {np.mean(err > 0):.5f} of texels differ")
-    assert np.all(err <= ulp), (name, err.max())
+    n_sp = err / spacing
+    print(f"{name}: max |d| {err.max():.3g}, {np.mean(err > 0):.5f} of texels differ, "
+          f"max {n_sp.max():.0f} spacings, counts by spacings {np.bincount(np.minimum(n_sp, 9).astype(int).ravel())}")
+    assert np.all(n_sp <= 8), (name, n_sp.max())
     assert np.mean(err > 0) <= 5e-3, name
 
 

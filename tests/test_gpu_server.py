@@ -9,6 +9,8 @@
 - A small HBM budget makes requests wait for admission; all still finish and
   the budget is never exceeded once more than one request is admitted.
 - Requests the renderer would reject are rejected at submission.
+- A path-traced request with the a-trous denoiser delivers denoised frames,
+  equal to its renderer's.
 - Cancelling ends a queued request at once, and a path-traced one in flight
   with its last delivered frame.
 """
@@ -37,6 +39,7 @@ def mixed_requests(a, b, n):
     dvr_a = workloads.dvr_params(a, 0)
     pt_a = workloads.pt_params(a, 0, 8, bounces=4, camera_index=1)
     pt_b = workloads.pt_params(b, n, 12, bounces=4, camera_index=2)
+    pt_b.path_tracing.denoiser = _abi.DENOISE_ATROUS  # each delivered frame denoised
     dvr_b = workloads.dvr_params(b, 1)
     workloads.organ_label_params(dvr_b, n, 7)  # same labels as pt_b: one opacity key per volume
     return [(a, dvr_a, 64, 48), (a, pt_a, 32, 24), (b, pt_b, 24, 32), (b, dvr_b, 40, 40),
@@ -103,7 +106,10 @@ def test_served_frames_equal_alone(ctx):
 def test_budget_queues_requests_and_all_finish(ctx):
     a, b, n = volumes(ctx)
     reqs = mixed_requests(a, b, n) * 3
-    one = 32 * 24 * (8 + 48) + 4 * 3 * 32 * 24 * 16  # the larger path-traced request with one increment of chunk sums
+    # The largest request: the denoised path-traced one, its images, two
+    # increments' chunk sums in flight and the denoiser's four images.
+    px = 32 * 24
+    one = px * (8 + 48) + 2 * 4 * 3 * px * 16 + 4 * px * 16
     srv = Server(ctx, pt_increment=4, hbm_budget=2 * one)
     ids = [srv.submit(r, p, w, h) for r, p, w, h in reqs]
     drain(srv, ids)
@@ -119,7 +125,7 @@ def test_rejected_requests(ctx):
     a, _, _ = volumes(ctx)
     srv = Server(ctx, hbm_budget=1 << 20)
     p = workloads.pt_params(a, 0, 4)
-    p.path_tracing.denoiser = _abi.DENOISE_ATROUS
+    p.path_tracing.denoiser = _abi.DENOISE_LEARNED
     with pytest.raises(_abi.VvrError) as e:
         srv.submit(a, p, 16, 16)
     assert e.value.code == _abi.VVR_EINVAL

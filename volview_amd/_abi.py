@@ -17,7 +17,7 @@ LABELS_OFF, LABELS_TINT, LABELS_ISOLATE, LABELS_HYBRID = 0, 1, 2, 3
 ENV_DOME, ENV_STUDIO, ENV_MAP = 0, 1, 2
 DENOISE_OFF, DENOISE_ATROUS, DENOISE_LEARNED = 0, 1, 2
 VOL_DENSITY, VOL_LABELS, VOL_NORMALS, VOL_MASK, VOL_MAJORANT, VOL_SHADOW, VOL_AO = 0, 1, 2, 3, 4, 5, 6
-KERNEL_MAJORANT, KERNEL_NORMALS, KERNEL_MASK, KERNEL_DVR, KERNEL_PT, KERNEL_LIGHTING = 0, 1, 2, 3, 4, 5
+KERNEL_MAJORANT, KERNEL_NORMALS, KERNEL_MASK, KERNEL_DVR, KERNEL_PT, KERNEL_LIGHTING, KERNEL_DENOISE = 0, 1, 2, 3, 4, 5, 6
 ABI_STRUCTS = ['Volume', 'ControlPoint', 'TF', 'SoftGrid', 'Camera', 'LabelParams', 'Lighting', 'Clipping',
                'PathTracing', 'FrameParams', 'Uniforms', 'ServerConfig', 'Request', 'Completion', 'ServerStats']
 

@@ -39,6 +39,11 @@ void launch_dvr(const DevScene& s, uint16_t* hdr, uint32_t width, uint32_t heigh
 void shadow_sweep_axis(const vvr_uniforms& hu, int& axis, int& sign);
 void launch_shadow_sweep(const DevScene& s, uint2* out, const vvr_uniforms& hu, hipStream_t st);
 void launch_ao(const DevScene& s, uint16_t* out, hipStream_t st);
+// The a-trous denoiser over a path-traced frame of `samples` accumulated paths:
+// reads the accumulation and guide images, writes the HDR image. `scratch`
+// holds 4 float4 images of the frame.
+void launch_denoise(const float4* accum, const float4* albedo, const float4* normal, float4* scratch, uint16_t* hdr,
+                    uint32_t width, uint32_t height, float samples, hipStream_t st);
 
 // dvr.comp for n_jobs frames in one launch (`jobs` in device memory, sorted by
 // tile_base; total_tiles = the last job's tile_base + its tile count).

@@ -167,7 +167,18 @@ void record(vvr_renderer* r, uint64_t frame, const vvr_frame_params* p) {
             });
         }
         r->accumulated += spf;
+        // renderer.rs:1795-1797: the a-trous denoiser replaces the HDR image with
+        // its filtered version of everything accumulated so far.
+        r->denoised = p->path_tracing.denoiser == VVR_DENOISE_ATROUS;
+        if (r->denoised) {
+            if (!r->dn_scratch) r->dn_scratch = dalloc<float4>(4 * size_t(r->width) * r->height, "denoiser images");
+            r->timed(VVR_KERNEL_DENOISE, [&] {
+                launch_denoise(r->accum, r->albedo, r->normal, r->dn_scratch, r->hdr, r->width, r->height,
+                               float(r->accumulated), r->stream);
+            });
+        }
     } else {
+        r->denoised = false;
         r->timed(VVR_KERNEL_DVR, [&] { launch_dvr(sc, r->hdr, r->width, r->height, r->stream); });
     }
 }
@@ -298,11 +309,14 @@ int vvr_renderer_read_accum(vvr_renderer* r, float* accum, float* albedo, float*
 }
 
 int vvr_renderer_read_denoised(vvr_renderer* r, float* rgba) {
-    (void)rgba;
-    return guarded([&] {
-        if (!r) throw Error{VVR_EINVAL, "null argument"};
-        throw Error{VVR_ESTATE, "no denoiser is available"};
-    });
+    {
+        const int st = guarded([&] {
+            if (!r || !rgba) throw Error{VVR_EINVAL, "null argument"};
+            if (!r->denoised) throw Error{VVR_ESTATE, "the last frame was not denoised"};
+        });
+        if (st != VVR_OK) return st;
+    }
+    return vvr_renderer_read_hdr(r, rgba);
 }
 
 int vvr_renderer_read_volume(vvr_renderer* r, int which, void* out, size_t bytes) {

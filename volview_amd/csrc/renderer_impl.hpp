@@ -214,8 +214,8 @@ inline LightKey light_key(const OpacityKey& opacity, const vvr_uniforms& u, bool
 
 inline void validate_frame(const vvr_frame_params& p) {
     const bool pt = p.path_tracing_enabled != 0;
-    if (pt && p.path_tracing.denoiser != VVR_DENOISE_OFF)
-        throw Error{VVR_EINVAL, "no denoiser is available: set path_tracing.denoiser to VVR_DENOISE_OFF"};
+    if (pt && p.path_tracing.denoiser > VVR_DENOISE_ATROUS)
+        throw Error{VVR_EINVAL, "the learned denoiser is not available: use VVR_DENOISE_ATROUS or VVR_DENOISE_OFF"};
     if (!(p.step_mm > 0.0f)) throw Error{VVR_EINVAL, "step_mm must be positive"};
     if (p.labels.mode > VVR_LABELS_HYBRID) throw Error{VVR_EINVAL, "unknown label mode"};
 }
@@ -266,6 +266,8 @@ struct vvr_renderer {
     float4* accum = nullptr;
     float4* albedo = nullptr;
     float4* normal = nullptr;
+    float4* dn_scratch = nullptr;  // a-trous denoiser: 4 float4 images of the frame, allocated on first use
+    bool denoised = false;         // the HDR image holds the denoiser's output
     // Path-tracing engine: the wavefront engine (default), or with
     // VVR_PT_KERNEL=persistent / simple the persistent megakernel or one lane per pixel.
     enum { kWavefront, kPersistent, kSimple } pt_kernel = kWavefront;
@@ -433,8 +435,9 @@ struct vvr_renderer {
     }
 
     void free_frame() {
-        dfree(hdr), dfree(accum), dfree(albedo), dfree(normal);
-        hdr = nullptr, accum = albedo = normal = nullptr;
+        dfree(hdr), dfree(accum), dfree(albedo), dfree(normal), dfree(dn_scratch);
+        hdr = nullptr, accum = albedo = normal = dn_scratch = nullptr;
+        denoised = false;
     }
 
     ~vvr_renderer() {

@@ -87,6 +87,7 @@ struct Job {
     size_t bytes = 0;  // charged to the HBM budget
     int uslot = -1;    // uniform block
     float4 *accum = nullptr, *albedo = nullptr, *normal = nullptr, *partial = nullptr;
+    float4* dn_scratch = nullptr;  // a-trous denoiser images (4 per pixel), when the request asks for it
     uint16_t* hdr = nullptr;
     uint16_t* h_hdr = nullptr;  // pinned copy of the newest frame
     hipEvent_t read_ev = nullptr;
@@ -105,10 +106,12 @@ size_t chunk_sum_bytes(const Job& j) {  // one increment's chunk sums
     return size_t(j.increment) * 3u * ((j.width + 7) / 8) * ((j.height + 7) / 8) * 64u * sizeof(float4);
 }
 
+bool denoised(const Job& j) { return j.pt && j.params.path_tracing.denoiser == VVR_DENOISE_ATROUS; }
+
 size_t request_bytes(const Job& j) {
     const size_t px = size_t(j.width) * j.height;
     if (!j.pt) return px * 8;
-    return px * (8 + 3 * 16) + std::min(j.n_inc, kInFlight) * chunk_sum_bytes(j);
+    return px * (8 + 3 * 16) + std::min(j.n_inc, kInFlight) * chunk_sum_bytes(j) + (denoised(j) ? px * 4 * 16 : 0);
 }
 
 }  // namespace
@@ -188,8 +191,8 @@ struct vvr_server {
 };
 
 void vvr_server::release_memory(Job* j) {
-    dfree(j->accum), dfree(j->albedo), dfree(j->normal), dfree(j->partial), dfree(j->hdr);
-    j->accum = j->albedo = j->normal = j->partial = nullptr;
+    dfree(j->accum), dfree(j->albedo), dfree(j->normal), dfree(j->partial), dfree(j->hdr), dfree(j->dn_scratch);
+    j->accum = j->albedo = j->normal = j->partial = j->dn_scratch = nullptr;
     j->hdr = nullptr;
     if (j->uslot >= 0) free_uslots.push_back(j->uslot);
     j->uslot = -1;
@@ -258,6 +261,7 @@ void vvr_server::admit(std::vector<Job*>& fresh_dvr) {
                 j->normal = dalloc<float4>(px, "request normal guide");
                 j->partial = reinterpret_cast<float4*>(
                     dalloc<uint8_t>(std::min(j->n_inc, kInFlight) * chunk_sum_bytes(*j), "request chunk sums"));
+                if (denoised(*j)) j->dn_scratch = dalloc<float4>(4 * px, "request denoiser images");
             }
             j->h_hdr = halloc<uint16_t>(4 * px, "frame read-back");
             if (!j->read_ev) check(hipEventCreateWithFlags(&j->read_ev, hipEventDisableTiming), "hipEventCreate");
@@ -459,7 +463,16 @@ void vvr_server::advance_jobs() {
             std::lock_guard<std::mutex> lk(mu);
             stats.pt_paths += uint64_t(j->width) * j->height * n;
         }
-        if (resolved) read_back(j, j->spp_resolved);
+        if (resolved) {
+            // The denoised frame of everything resolved so far, as the renderer
+            // denoises after each frame (renderer.rs:1795-1797).
+            if (denoised(*j)) {
+                launch_denoise(j->accum, j->albedo, j->normal, j->dn_scratch, j->hdr, j->width, j->height,
+                               float(j->spp_resolved), stream);
+                check(hipGetLastError(), "denoiser");
+            }
+            read_back(j, j->spp_resolved);
+        }
         const bool busy = std::any_of(j->entries.begin(), j->entries.end(), [&](int e) { return table[e].handing_out; });
         if (!j->cancelled && !busy && j->inc_started < j->n_inc && j->entries.size() < kInFlight) start_increment(j);
     }

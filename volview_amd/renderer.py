@@ -202,6 +202,11 @@ class Renderer:
         check(self._lib.vvr_renderer_read_hdr(self.handle, _fptr(out)))
         return out
 
+    def read_denoised(self) -> np.ndarray:
+        out = np.zeros((self.height, self.width, 4), np.float32)
+        check(self._lib.vvr_renderer_read_denoised(self.handle, _fptr(out)))
+        return out
+
     def read_accum(self):
         shape = (self.height, self.width, 4)
         a, b, c = np.zeros(shape, np.float32), np.zeros(shape, np.float32), np.zeros(shape, np.float32)
