@@ -257,6 +257,16 @@ int vvr_renderer_read_hdr(vvr_renderer* r, float* rgba);
 /* Path-tracing images (bindings 17-19): accumulated radiance+background, albedo
  * guide+luminance^2, normal guide+depth; width*height*4 floats each. Any may be NULL. */
 int vvr_renderer_read_accum(vvr_renderer* r, float* accum, float* albedo, float* normal);
+/* The RGBA8 display image of the last record(): camera finishing and tone
+ * mapping of the HDR image (Renderer::output after record_post,
+ * renderer.rs:1801-1866; offscreen.rs:52-76 reads it back). 4 bytes a pixel. */
+int vvr_renderer_read_rgba(vvr_renderer* r, uint8_t* rgba);
+/* The adapted auto-exposure factor (bloom.comp pass 1; 0 = not yet metered). */
+int vvr_renderer_read_exposure(vvr_renderer* r, float* exposure);
+/* render_to_rgba (offscreen.rs:11-78): resize, record `frames` frames (path
+ * tracing accumulates them), then read the RGBA8 image back. */
+int vvr_render_to_rgba(vvr_renderer* r, const vvr_frame_params* params, uint32_t width, uint32_t height,
+                       uint32_t frames, uint8_t* rgba);
 /* The denoised image of the last frame (renderer.rs:1795-1797: with
  * VVR_DENOISE_ATROUS the HDR image holds denoise.comp's output); -5 when the
  * last frame was not denoised. */
@@ -283,7 +293,7 @@ int vvr_renderer_read_uniforms(vvr_renderer* r, void* out, size_t bytes);
 /* ---- measurement ------------------------------------------------------------ */
 
 enum { VVR_KERNEL_MAJORANT = 0, VVR_KERNEL_NORMALS = 1, VVR_KERNEL_MASK = 2, VVR_KERNEL_DVR = 3, VVR_KERNEL_PT = 4,
-       VVR_KERNEL_LIGHTING = 5, VVR_KERNEL_DENOISE = 6, VVR_KERNEL_COUNT = 7 };
+       VVR_KERNEL_LIGHTING = 5, VVR_KERNEL_DENOISE = 6, VVR_KERNEL_POST = 7, VVR_KERNEL_COUNT = 8 };
 /* Enables HIP-event timing of each kernel on the stream it is launched on. */
 int vvr_renderer_set_timing(vvr_renderer* r, int enabled);
 /* Total milliseconds and launches of kernel `which` since timing was enabled. */

@@ -73,6 +73,12 @@ def load() -> ctypes.CDLL:
                                               P(c_float)]),
             "oracle_denoise": (None, [P(c_float), P(c_float), P(c_float), c_uint32, c_uint32, c_float, c_int, c_int,
                                       P(c_float), P(c_float)]),
+            "oracle_post": (None, [P(OracleScene), P(_abi.Uniforms), P(c_float), c_uint32, c_uint32, P(c_float),
+                                   P(c_uint8), P(c_float), P(c_float)]),
+            "oracle_linear_to_srgb": (c_float, [c_float]),
+            "oracle_aces": (c_float, [c_float]),
+            "oracle_agx": (None, [P(c_float), P(c_float)]),
+            "oracle_hash01_bits": (c_uint32, [c_uint32]),
             "oracle_pt": (None, [P(OracleScene), P(_abi.Uniforms), P(c_uint32), c_size_t, c_int, P(c_float),
                                  P(c_float), P(c_float)]),
             "oracle_density_cubic": (c_float, [P(OracleScene), P(_abi.Uniforms), P(c_float), c_int]),
@@ -259,6 +265,20 @@ class Scene:
         load().oracle_lighting_lookup(byref(self.c), byref(u), which, _p(t, c_float), precision, _p(out, c_float))
         return out if which == 0 else out[:1]
 
+    def post(self, u: _abi.Uniforms, hdr: np.ndarray, exposure: float = 0.0):
+        """bloom.comp + tonemap.comp over an (h, w, 4) HDR image: (RGBA8 image,
+        adapted exposure, quarter- and eighth-resolution glow images)."""
+        img = np.ascontiguousarray(hdr, np.float32)
+        h, w = img.shape[:2]
+        qw, qh = (w + 3) // 4, (h + 3) // 4
+        out = np.zeros((h, w, 4), np.uint8)
+        a = np.zeros((qh, qw, 4), np.float32)
+        c = np.zeros(((qh + 1) // 2, (qw + 1) // 2, 4), np.float32)
+        ex = c_float(exposure)
+        load().oracle_post(byref(self.c), byref(u), _p(img, c_float), w, h, byref(ex), _p(out, c_uint8),
+                           _p(a, c_float), _p(c, c_float))
+        return out, ex.value, a, c
+
     def dvr(self, u: _abi.Uniforms, pixels: np.ndarray, precision: int = 32) -> np.ndarray:
         px = np.ascontiguousarray(pixels, np.uint32)
         out = np.zeros((len(px), 4), np.float32)
@@ -307,6 +327,25 @@ def denoise(accum, albedo, normal, samples: float, levels: int = 5, precision: i
     load().oracle_denoise(_p(a, c_float), _p(b, c_float), _p(c, c_float), w, h, samples, levels, precision,
                           _p(hdr, c_float), _p(last, c_float))
     return hdr, last
+
+
+def linear_to_srgb(c: float) -> float:
+    return float(load().oracle_linear_to_srgb(c))
+
+
+def aces(x: float) -> float:
+    return float(load().oracle_aces(x))
+
+
+def agx(rgb) -> np.ndarray:
+    i = np.ascontiguousarray(rgb, np.float32)
+    o = np.zeros(3, np.float32)
+    load().oracle_agx(_p(i, c_float), _p(o, c_float))
+    return o
+
+
+def hash01(v: int) -> float:
+    return float(np.uint32(load().oracle_hash01_bits(v)).view(np.float32))
 
 
 def pcg_sequence(seed: int, n: int) -> np.ndarray:

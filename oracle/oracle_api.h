@@ -78,6 +78,17 @@ void oracle_lighting_lookup(const oracle_scene* s, const vvr_uniforms* u, int wh
  * RGBA16F store; last (may be NULL): the last filtered irradiance + variance. */
 void oracle_denoise(const float* accum, const float* albedo, const float* normal, uint32_t width, uint32_t height,
                     float samples, int levels, int precision, float* hdr, float* last);
+/* Camera finishing: bloom.comp's 7 passes (renderer.rs:1815-1866) and
+ * tonemap.comp over an HDR image (width x height RGBA floats, the RGBA16F
+ * values). exposure: the adapted auto exposure in and out (0 = not yet
+ * metered). out: RGBA8 (UNORM stores round to nearest). bloom_a / bloom_c (may
+ * be NULL): the final quarter- and eighth-resolution glow images. */
+void oracle_post(const oracle_scene* s, const vvr_uniforms* u, const float* hdr, uint32_t width, uint32_t height,
+                 float* exposure, uint8_t* out, float* bloom_a, float* bloom_c);
+float oracle_linear_to_srgb(float c);
+float oracle_aces(float x);
+void oracle_agx(const float in[3], float out[3]);
+uint32_t oracle_hash01_bits(uint32_t v);
 /* pt.comp for `n` pixels: the values the dispatch adds to the accumulation,
  * albedo-guide and normal-guide images (4 floats each per pixel). */
 void oracle_pt(const oracle_scene* s, const vvr_uniforms* u, const uint32_t* pixels, size_t n, int precision,

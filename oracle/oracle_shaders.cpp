@@ -4,7 +4,8 @@
 // Line-by-line restatement of the reference's compute shaders:
 //   shaders/common.glsl, shaders/shading.glsl, shaders/majorant.comp,
 //   shaders/normals.comp, shaders/mask.comp, shaders/dvr.comp, shaders/pt.comp,
-//   shaders/shadow.comp, shaders/ao.comp, shaders/denoise.comp.
+//   shaders/shadow.comp, shaders/ao.comp, shaders/denoise.comp, shaders/bloom.comp,
+//   shaders/tonemap.comp.
 // Each function names the lines it follows. Templated on the real type R: the
 // FP32 oracle (R = float, built without FMA contraction, so every operation is
 // one IEEE-rounded step in source order) and the FP64 oracle (R = double).
@@ -1492,6 +1493,98 @@ template <class R> struct Denoise {
     }
 };
 
+// ---- bloom.comp and tonemap.comp: camera finishing of the HDR image ----------------
+// Images of the finishing passes are RGBA16F (renderer.rs:1373-1398): every store
+// rounds through binary16. Row-major, 4 floats a texel.
+struct Img {
+    int w = 0, h = 0;
+    std::vector<float> v;
+    Img(int w_, int h_) : w(w_), h(h_), v(4 * (size_t)w_ * h_, 0.0f) {}
+    float* at(int x, int y) { return &v[4 * ((size_t)y * w + x)]; }
+    const float* at(int x, int y) const { return &v[4 * ((size_t)y * w + x)]; }
+    void store(int x, int y, const float c[4]) {
+        for (int i = 0; i < 4; i++) at(x, y)[i] = f16_to_f32(f32_to_f16(c[i]));
+    }
+};
+
+inline float lum_f(float r, float g, float b) { return (r * 0.2126f + g * 0.7152f) + b * 0.0722f; }
+inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// 9-tap Gaussian along `ax` (bloom.comp:46-54).
+void blur(const Img& src, Img& dst, int ax) {
+    const float w[5] = {0.2270f, 0.1945f, 0.1216f, 0.0540f, 0.0162f};
+    for (int y = 0; y < src.h; y++)
+        for (int x = 0; x < src.w; x++) {
+            float sum[4];
+            for (int c = 0; c < 4; c++) sum[c] = src.at(x, y)[c] * w[0];
+            for (int i = 1; i < 5; i++) {
+                int px = ax == 0 ? clampi(x + i, 0, src.w - 1) : x, py = ax == 1 ? clampi(y + i, 0, src.h - 1) : y;
+                int mx = ax == 0 ? clampi(x - i, 0, src.w - 1) : x, my = ax == 1 ? clampi(y - i, 0, src.h - 1) : y;
+                for (int c = 0; c < 4; c++) sum[c] += w[i] * (src.at(px, py)[c] + src.at(mx, my)[c]);
+            }
+            dst.store(x, y, sum);
+        }
+}
+
+// Linear, clamp-to-edge lookup of an RGBA16F image at uv (the renderer's sampler).
+void bilinear(const Img& im, float u, float v, float out[4]) {
+    float tx = u * (float)im.w - 0.5f, ty = v * (float)im.h - 0.5f;
+    float fx = std::floor(tx), fy = std::floor(ty);
+    float ax = tx - fx, ay = ty - fy;
+    int x0 = (int)fx, y0 = (int)fy;
+    for (int c = 0; c < 4; c++) {
+        auto t = [&](int x, int y) { return im.at(clampi(x, 0, im.w - 1), clampi(y, 0, im.h - 1))[c]; };
+        float c0 = mixr(t(x0, y0), t(x0 + 1, y0), ax), c1 = mixr(t(x0, y0 + 1), t(x0 + 1, y0 + 1), ax);
+        out[c] = mixr(c0, c1, ay);
+    }
+}
+
+inline float hash01(uint32_t v) {  // tonemap.comp:13-17
+    uint32_t state = v * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (float)((word >> 22u) ^ word) * (1.0f / 4294967296.0f);
+}
+
+inline float aces1(float x) {  // tonemap.comp:20-22
+    float v = (x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f);
+    return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+}
+
+void agx3(float c[3]) {  // tonemap.comp:25-42
+    const float inset[3][3] = {{0.842479062253094f, 0.0423282422610123f, 0.0423756549057051f},
+                               {0.0784335999999992f, 0.878468636469772f, 0.0784336f},
+                               {0.0792237451477643f, 0.0791661274605434f, 0.879142973793104f}};
+    const float outset[3][3] = {{1.19687900512017f, -0.0528968517574562f, -0.0529716355144438f},
+                                {-0.0980208811401368f, 1.15190312990417f, -0.0980434501171241f},
+                                {-0.0990297440797205f, -0.0989611768448433f, 1.15107367264116f}};
+    const float min_ev = -12.47393f, max_ev = 4.026069f;
+    float t[3];
+    for (int i = 0; i < 3; i++) t[i] = (inset[0][i] * c[0] + inset[1][i] * c[1]) + inset[2][i] * c[2];
+    for (int i = 0; i < 3; i++) {
+        float l = std::log2(gmax(t[i], 1e-10f));
+        l = l < min_ev ? min_ev : (l > max_ev ? max_ev : l);
+        l = (l - min_ev) / (max_ev - min_ev);
+        float l2 = l * l, l4 = l2 * l2;
+        t[i] = (((((15.5f * l4 * l2 - 40.14f * l4 * l) + 31.96f * l4) - 6.868f * l2 * l) + 0.4298f * l2) + 0.1191f * l) -
+               0.00232f;
+    }
+    for (int i = 0; i < 3; i++) {
+        float o = (outset[0][i] * t[0] + outset[1][i] * t[1]) + outset[2][i] * t[2];
+        o = o < 0.0f ? 0.0f : (o > 1.0f ? 1.0f : o);
+        c[i] = std::pow(o, 2.2f);
+    }
+}
+
+inline float srgb1(float c) {  // common.glsl:444-447
+    c = c < 0.0f ? 0.0f : (c > 1.0f ? 1.0f : c);
+    return c < 0.0031308f ? 12.92f * c : 1.055f * std::pow(c, 1.0f / 2.4f) - 0.055f;
+}
+
+inline uint8_t unorm8(float c) {  // RGBA8_UNORM store: nearest
+    c = c < 0.0f ? 0.0f : (c > 1.0f ? 1.0f : c);
+    return (uint8_t)std::floor(c * 255.0f + 0.5f);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1611,6 +1704,128 @@ void oracle_denoise(const float* accum, const float* albedo, const float* normal
         Denoise<float> d{accum, albedo, normal, (int)width, (int)height, samples};
         d.run(levels, hdr, last);
     }
+}
+
+float oracle_linear_to_srgb(float c) { return srgb1(c); }
+float oracle_aces(float x) { return aces1(x); }
+void oracle_agx(const float in[3], float out[3]) {
+    for (int i = 0; i < 3; i++) out[i] = in[i];
+    agx3(out);
+}
+uint32_t oracle_hash01_bits(uint32_t v) {
+    float f = hash01(v);
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    return b;
+}
+
+void oracle_post(const oracle_scene* s, const vvr_uniforms* u, const float* hdr, uint32_t width, uint32_t height,
+                 float* exposure, uint8_t* out, float* bloom_a, float* bloom_c) {
+    const int w = (int)width, h = (int)height;
+    const int qw = (w + 3) / 4, qh = (h + 3) / 4, ew = (qw + 1) / 2, eh = (qh + 1) / 2;
+    Img A(qw, qh), B(qw, qh), C(ew, eh), D(ew, eh);
+    auto H = [&](int x, int y) { return hdr + 4 * ((size_t)y * w + x); };
+    // Pass 0 (bloom.comp:104-122): quarter-resolution Karis average and coverage.
+    for (int y = 0; y < qh; y++)
+        for (int x = 0; x < qw; x++) {
+            float sr = 0, sg = 0, sb = 0, weights = 0, coverage = 0;
+            for (int j = 0; j < 4; j++)
+                for (int i = 0; i < 4; i++) {
+                    const float* p = H(gmin(x * 4 + i, w - 1), gmin(y * 4 + j, h - 1));
+                    float wt = 1.0f / (1.0f + lum_f(p[0], p[1], p[2]));
+                    sr += p[0] * wt, sg += p[1] * wt, sb += p[2] * wt;
+                    weights += wt;
+                    coverage += 1.0f - p[3];
+                }
+            float c[4] = {sr / weights, sg / weights, sb / weights, coverage / 16.0f};
+            A.store(x, y, c);
+        }
+    // Pass 1 (bloom.comp:60-98): 64 lanes sum strided texels, then a tree.
+    {
+        float ls[64], ws[64];
+        for (int t = 0; t < 64; t++) {
+            float sum = 0, wsum = 0;
+            for (int i = t; i < qw * qh; i += 64) {
+                const float* c = A.at(i % qw, i / qw);
+                sum += c[3] * std::log(gmax(lum_f(c[0], c[1], c[2]), 1e-4f));
+                wsum += c[3];
+            }
+            ls[t] = sum, ws[t] = wsum;
+        }
+        for (int stride = 32; stride > 0; stride >>= 1)
+            for (int t = 0; t < stride; t++) ls[t] += ls[t + stride], ws[t] += ws[t + stride];
+        float target = 1.0f;
+        if (ws[0] > 1.0f) {
+            target = 0.24f / std::exp(ls[0] / ws[0]);
+            target = target < 0.25f ? 0.25f : (target > 4.0f ? 4.0f : target);
+        }
+        float prev = *exposure;
+        *exposure = prev <= 0.0f ? target : std::exp(mixr(std::log(prev), std::log(target), 0.15f));
+    }
+    blur(A, B, 0);  // pass 2
+    blur(B, A, 1);  // pass 3
+    for (int y = 0; y < eh; y++)  // pass 4: half-resolution copy of A
+        for (int x = 0; x < ew; x++) {
+            float sum[4] = {0, 0, 0, 0};
+            for (int i = 0; i < 4; i++) {
+                const float* c = A.at(gmin(x * 2 + (i & 1), qw - 1), gmin(y * 2 + (i >> 1), qh - 1));
+                for (int k = 0; k < 4; k++) sum[k] += c[k];
+            }
+            for (int k = 0; k < 4; k++) sum[k] *= 0.25f;
+            C.store(x, y, sum);
+        }
+    blur(C, D, 0);  // pass 5
+    blur(D, C, 1);  // pass 6
+    if (bloom_a) std::memcpy(bloom_a, A.v.data(), A.v.size() * 4);
+    if (bloom_c) std::memcpy(bloom_c, C.v.data(), C.v.size() * 4);
+    // tonemap.comp:48-85
+    Ctx<float> cx(s, u);
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            const float* hp = H(x, y);
+            float ex = u->ambient[3];
+            if (u->finish[3] > 0.5f) ex *= *exposure;
+            float c[3] = {hp[0], hp[1], hp[2]};
+            float uu = ((float)x + 0.5f) / (float)w, vv = ((float)y + 0.5f) / (float)h;
+            if (u->finish[0] > 0.0f) {
+                float a4[4], c4[4];
+                bilinear(A, uu, vv, a4);
+                bilinear(C, uu, vv, c4);
+                for (int k = 0; k < 3; k++) c[k] = mixr(c[k], 0.6f * a4[k] + 0.4f * c4[k], u->finish[0]);
+            }
+            float dx = uu * 2.0f - 1.0f, dy = vv * 2.0f - 1.0f;
+            float r = std::sqrt(dx * dx + dy * dy) * 0.7071f;
+            float vignette = 1.0f - u->finish[1] * std::pow(r, 2.5f);
+            for (int k = 0; k < 3; k++) c[k] *= ex * vignette;
+            uint32_t op = (uint32_t)u->material[2];
+            if (op == 1u)
+                for (int k = 0; k < 3; k++) c[k] = aces1(c[k]);
+            else if (op == 2u)
+                agx3(c);
+            else
+                for (int k = 0; k < 3; k++) c[k] = c[k] < 0.0f ? 0.0f : (c[k] > 1.0f ? 1.0f : c[k]);
+            if (u->lighting[3] < 1u) {  // background (common.glsl:449-455), display-referred
+                V3<float> d = cx.camera_ray((float)x + 0.5f, (float)y + 0.5f);
+                float t = 0.5f + 0.5f * -d.z;
+                const float lo[3] = {0.02f, 0.022f, 0.028f}, hi[3] = {0.08f, 0.085f, 0.1f};
+                for (int k = 0; k < 3; k++) c[k] += hp[3] * mixr(lo[k], hi[k], t);
+            }
+            if (u->flags[1] != 0u)
+                for (int k = 0; k < 3; k++) c[k] = srgb1(c[k]);
+            if (u->finish[2] > 0.0f) {
+                uint32_t px = (uint32_t)x, py = (uint32_t)y;
+                float n = hash01(px + 4099u * py) + hash01(px * 7919u + py + 1u) - 1.0f;
+                float yl = lum_f(c[0], c[1], c[2]);
+                yl = yl < 0.0f ? 0.0f : (yl > 1.0f ? 1.0f : yl);
+                for (int k = 0; k < 3; k++) {
+                    float v = c[k] + n * u->finish[2] * (0.3f + 2.8f * yl * (1.0f - yl));
+                    c[k] = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+                }
+            }
+            uint8_t* o = out + 4 * ((size_t)y * w + x);
+            o[0] = unorm8(c[0]), o[1] = unorm8(c[1]), o[2] = unorm8(c[2]), o[3] = 255;
+        }
 }
 
 void oracle_pt(const oracle_scene* s, const vvr_uniforms* u, const uint32_t* pixels, size_t n, int precision,

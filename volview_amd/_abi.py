@@ -16,8 +16,10 @@ VVR_OK, VVR_EINVAL, VVR_ELIMIT, VVR_ENOMEM, VVR_EDEVICE, VVR_ESTATE, VVR_ECANCEL
 LABELS_OFF, LABELS_TINT, LABELS_ISOLATE, LABELS_HYBRID = 0, 1, 2, 3
 ENV_DOME, ENV_STUDIO, ENV_MAP = 0, 1, 2
 DENOISE_OFF, DENOISE_ATROUS, DENOISE_LEARNED = 0, 1, 2
+TONE_NONE, TONE_ACES, TONE_AGX = 0, 1, 2
 VOL_DENSITY, VOL_LABELS, VOL_NORMALS, VOL_MASK, VOL_MAJORANT, VOL_SHADOW, VOL_AO = 0, 1, 2, 3, 4, 5, 6
-KERNEL_MAJORANT, KERNEL_NORMALS, KERNEL_MASK, KERNEL_DVR, KERNEL_PT, KERNEL_LIGHTING, KERNEL_DENOISE = 0, 1, 2, 3, 4, 5, 6
+KERNEL_MAJORANT, KERNEL_NORMALS, KERNEL_MASK, KERNEL_DVR, KERNEL_PT, KERNEL_LIGHTING, KERNEL_DENOISE, KERNEL_POST = (
+    0, 1, 2, 3, 4, 5, 6, 7)
 ABI_STRUCTS = ['Volume', 'ControlPoint', 'TF', 'SoftGrid', 'Camera', 'LabelParams', 'Lighting', 'Clipping',
                'PathTracing', 'FrameParams', 'Uniforms', 'ServerConfig', 'Request', 'Completion', 'ServerStats']
 
@@ -146,6 +148,9 @@ SIGNATURES = [
     ("vvr_renderer_read_hdr", c_int, [c_void_p, POINTER(c_float)]),
     ("vvr_renderer_read_accum", c_int, [c_void_p, POINTER(c_float), POINTER(c_float), POINTER(c_float)]),
     ("vvr_renderer_read_denoised", c_int, [c_void_p, POINTER(c_float)]),
+    ("vvr_renderer_read_rgba", c_int, [c_void_p, POINTER(c_uint8)]),
+    ("vvr_renderer_read_exposure", c_int, [c_void_p, POINTER(c_float)]),
+    ("vvr_render_to_rgba", c_int, [c_void_p, POINTER(FrameParams), c_uint32, c_uint32, c_uint32, POINTER(c_uint8)]),
     ("vvr_renderer_read_volume", c_int, [c_void_p, c_int, c_void_p, c_size_t]),
     ("vvr_renderer_read_majorant", c_int, [c_void_p, POINTER(c_float), c_size_t]),
     ("vvr_renderer_read_uniforms", c_int, [c_void_p, c_void_p, c_size_t]),

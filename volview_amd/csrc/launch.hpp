@@ -42,6 +42,12 @@ void launch_ao(const DevScene& s, uint16_t* out, hipStream_t st);
 // The a-trous denoiser over a path-traced frame of `samples` accumulated paths:
 // reads the accumulation and guide images, writes the HDR image. `scratch`
 // holds 4 float4 images of the frame.
+// Camera finishing (bloom.comp) and tone mapping (tonemap.comp) of the HDR
+// image into RGBA8 `out`; `bloom` holds bloom_bytes() of glow images and
+// `exposure` the adapted exposure (one float, 0 = not yet metered).
+void launch_post(const vvr_uniforms* du, const uint16_t* hdr, uint32_t width, uint32_t height, uint16_t* bloom,
+                 float* exposure, uint32_t* out, hipStream_t st);
+size_t bloom_bytes(uint32_t width, uint32_t height);
 void launch_denoise(const float4* accum, const float4* albedo, const float4* normal, float4* scratch, uint16_t* hdr,
                     uint32_t width, uint32_t height, float samples, hipStream_t st);
 

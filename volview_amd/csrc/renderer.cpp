@@ -39,6 +39,9 @@ void new_renderer(vvr_ctx* ctx, const vvr_volume* vol, const vvr_tf* tf, vvr_ren
     for (auto& d : r->d_uniforms) d = dalloc<vvr_uniforms>(1, "uniforms");
     r->lut = dalloc<float4>(kLutSize, "transfer function");
     r->queue = dalloc<uint32_t>(1, "work queue");
+    // Auto exposure, adapted over frames: 0 = not yet metered (renderer.rs:791-818).
+    r->exposure = dalloc<float>(1, "exposure");
+    check(hipMemsetAsync(r->exposure, 0, sizeof(float), r->stream), "hipMemset");
     {
         int cus = 0;
         check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, r->device), "device attribute");
@@ -181,6 +184,10 @@ void record(vvr_renderer* r, uint64_t frame, const vvr_frame_params* p) {
         r->denoised = false;
         r->timed(VVR_KERNEL_DVR, [&] { launch_dvr(sc, r->hdr, r->width, r->height, r->stream); });
     }
+    // Camera finishing and tone mapping into the RGBA8 output (renderer.rs:1801-1811).
+    r->timed(VVR_KERNEL_POST, [&] {
+        launch_post(du, r->hdr, r->width, r->height, r->bloom, r->exposure, r->output, r->stream);
+    });
 }
 
 }  // namespace
@@ -237,6 +244,8 @@ int vvr_renderer_resize(vvr_renderer* r, uint32_t width, uint32_t height) {
         r->accum = dalloc<float4>(n, "accumulation");
         r->albedo = dalloc<float4>(n, "albedo guide");
         r->normal = dalloc<float4>(n, "normal guide");
+        r->output = dalloc<uint32_t>(n, "output");
+        r->bloom = dalloc<uint16_t>(bloom_bytes(width, height) / 2, "bloom images");
         check(hipMemsetAsync(r->hdr, 0, 8 * n, r->stream), "hipMemset");
         r->width = width;
         r->height = height;
@@ -306,6 +315,33 @@ int vvr_renderer_read_accum(vvr_renderer* r, float* accum, float* albedo, float*
         if (normal) check(hipMemcpyAsync(normal, r->normal, bytes, hipMemcpyDeviceToHost, r->stream), "read-back");
         check(hipStreamSynchronize(r->stream), "read-back");
     });
+}
+
+int vvr_renderer_read_rgba(vvr_renderer* r, uint8_t* rgba) {
+    return guarded([&] {
+        if (!r || !rgba) throw Error{VVR_EINVAL, "null argument"};
+        if (!r->output) throw Error{VVR_ESTATE, "resize() must be called first"};
+        r->use_device();
+        check(hipMemcpyAsync(rgba, r->output, size_t(r->width) * r->height * 4, hipMemcpyDeviceToHost, r->stream),
+              "output read-back");
+        check(hipStreamSynchronize(r->stream), "output read-back");
+    });
+}
+
+int vvr_renderer_read_exposure(vvr_renderer* r, float* exposure) {
+    return guarded([&] {
+        if (!r || !exposure) throw Error{VVR_EINVAL, "null argument"};
+        r->use_device();
+        check(hipMemcpyAsync(exposure, r->exposure, sizeof(float), hipMemcpyDeviceToHost, r->stream), "read-back");
+        check(hipStreamSynchronize(r->stream), "read-back");
+    });
+}
+
+int vvr_render_to_rgba(vvr_renderer* r, const vvr_frame_params* params, uint32_t width, uint32_t height,
+                       uint32_t frames, uint8_t* rgba) {
+    int st = vvr_renderer_resize(r, width, height);
+    for (uint32_t f = 1; st == VVR_OK && f <= std::max(frames, 1u); f++) st = vvr_renderer_record(r, f, params);
+    return st == VVR_OK ? vvr_renderer_read_rgba(r, rgba) : st;
 }
 
 int vvr_renderer_read_denoised(vvr_renderer* r, float* rgba) {

@@ -267,6 +267,9 @@ struct vvr_renderer {
     float4* albedo = nullptr;
     float4* normal = nullptr;
     float4* dn_scratch = nullptr;  // a-trous denoiser: 4 float4 images of the frame, allocated on first use
+    uint32_t* output = nullptr;    // RGBA8 display image (tone-mapped)
+    uint16_t* bloom = nullptr;     // glow images A, B (quarter), C, D (eighth resolution), RGBA16F
+    float* exposure = nullptr;     // adapted auto exposure, persists across frames
     bool denoised = false;         // the HDR image holds the denoiser's output
     // Path-tracing engine: the wavefront engine (default), or with
     // VVR_PT_KERNEL=persistent / simple the persistent megakernel or one lane per pixel.
@@ -435,8 +438,9 @@ struct vvr_renderer {
     }
 
     void free_frame() {
-        dfree(hdr), dfree(accum), dfree(albedo), dfree(normal), dfree(dn_scratch);
+        dfree(hdr), dfree(accum), dfree(albedo), dfree(normal), dfree(dn_scratch), dfree(output), dfree(bloom);
         hdr = nullptr, accum = albedo = normal = dn_scratch = nullptr;
+        output = nullptr, bloom = nullptr;
         denoised = false;
     }
 
@@ -451,7 +455,7 @@ struct vvr_renderer {
         wf.reset();
         dfree(density), dfree(normals), dfree(majorant), dfree(labels), dfree(mask), dfree(mask_b), dfree(soft);
         dfree(shadow), dfree(ao);
-        dfree(lut), dfree(preint), dfree(d_uniforms[0]), dfree(d_uniforms[1]), dfree(queue);
+        dfree(lut), dfree(preint), dfree(d_uniforms[0]), dfree(d_uniforms[1]), dfree(queue), dfree(exposure);
         if (h_uniforms) (void)hipHostFree(h_uniforms);
         for (auto& e : slot_done)
             if (e) (void)hipEventDestroy(e);

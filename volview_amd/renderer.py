@@ -202,6 +202,17 @@ class Renderer:
         check(self._lib.vvr_renderer_read_hdr(self.handle, _fptr(out)))
         return out
 
+    def read_rgba(self) -> np.ndarray:
+        """The tone-mapped RGBA8 display image of the last record()."""
+        out = np.zeros((self.height, self.width, 4), np.uint8)
+        check(self._lib.vvr_renderer_read_rgba(self.handle, out.ctypes.data_as(POINTER(c_uint8))))
+        return out
+
+    def read_exposure(self) -> float:
+        v = c_float()
+        check(self._lib.vvr_renderer_read_exposure(self.handle, byref(v)))
+        return v.value
+
     def read_denoised(self) -> np.ndarray:
         out = np.zeros((self.height, self.width, 4), np.float32)
         check(self._lib.vvr_renderer_read_denoised(self.handle, _fptr(out)))
@@ -259,3 +270,13 @@ def render_hdr(renderer: Renderer, params: FrameParams, width: int, height: int,
         renderer.record(frame, params)
         renderer.synchronize()
     return renderer.read_hdr()
+
+
+def render_to_rgba(renderer: Renderer, params: FrameParams, width: int, height: int, frames: int = 1) -> np.ndarray:
+    """render_to_rgba (offscreen.rs:11-78): resize, `frames` frames (path tracing
+    accumulates them), then the tone-mapped RGBA8 image, (height, width, 4)."""
+    out = np.zeros((height, width, 4), np.uint8)
+    check(renderer._lib.vvr_render_to_rgba(renderer.handle, byref(params), width, height, frames,
+                                           out.ctypes.data_as(POINTER(c_uint8))))
+    renderer.width, renderer.height = width, height
+    return out
