@@ -290,6 +290,24 @@ int vvr_renderer_read_majorant(vvr_renderer* r, float* out, size_t n);
  * 1554-1660), for parity checks of the host-side packing. */
 int vvr_renderer_read_uniforms(vvr_renderer* r, void* out, size_t bytes);
 
+/* ---- light map ---------------------------------------------------------------- */
+
+/* environment::EnvironmentMap (environment.rs:12-17): linear-light RGB radiance,
+ * width x height texels, rows from the top (zenith) down. */
+typedef struct vvr_env_map {
+    uint32_t width;
+    uint32_t height;
+    const float* rgb;
+} vvr_env_map;
+/* Renderer::set_environment_map (renderer.rs:1283-1302): sets (or with NULL
+ * removes) the HDR light map of VVR_ENV_MAP, importance-sampled by the path
+ * tracer and convolved for the ray marcher; restarts accumulation. Copied at the
+ * call; waits for the renderer's stream. */
+int vvr_renderer_set_environment_map(vvr_renderer* r, const vvr_env_map* map);
+/* The set light map's brightest column (environment.rs:113-125) and, for parity
+ * checks, its (width + 1) x height x 2 sampling table (table may be NULL). */
+int vvr_renderer_read_environment(vvr_renderer* r, float* peak_u, float* table, size_t table_floats);
+
 /* ---- measurement ------------------------------------------------------------ */
 
 enum { VVR_KERNEL_MAJORANT = 0, VVR_KERNEL_NORMALS = 1, VVR_KERNEL_MASK = 2, VVR_KERNEL_DVR = 3, VVR_KERNEL_PT = 4,

@@ -28,7 +28,8 @@ class OracleScene(ctypes.Structure):
     _fields_ = [("dims", c_uint32 * 3), ("density", POINTER(c_uint16)), ("lut", POINTER(c_uint16)),
                 ("preint", POINTER(c_float)), ("labels", POINTER(c_uint8)), ("normals", POINTER(c_int8)),
                 ("mask", POINTER(c_uint16)), ("majorant", POINTER(c_float)), ("shadow", POINTER(c_uint16)),
-                ("ao", POINTER(c_uint16))]
+                ("ao", POINTER(c_uint16)), ("env_radiance", POINTER(c_uint16)), ("env_table", POINTER(c_float)),
+                ("env_diffuse", POINTER(c_uint16)), ("env_w", c_uint32), ("env_h", c_uint32)]
 
 
 _lib = None
@@ -73,6 +74,11 @@ def load() -> ctypes.CDLL:
                                               P(c_float)]),
             "oracle_denoise": (None, [P(c_float), P(c_float), P(c_float), c_uint32, c_uint32, c_float, c_int, c_int,
                                       P(c_float), P(c_float)]),
+            "oracle_pack_uniforms_env": (None, [P(_abi.Volume), P(_abi.FrameParams), P(c_float), c_int, c_uint32,
+                                                c_uint32, c_uint32, c_uint64, c_int, c_float, P(_abi.Uniforms)]),
+            "oracle_env_prepare": (c_float, [c_uint32, c_uint32, P(c_float), P(c_uint16), P(c_float), P(c_uint16)]),
+            "oracle_sample_env": (None, [P(OracleScene), P(_abi.Uniforms), c_uint32, c_uint32, P(c_float), P(c_float)]),
+            "oracle_env_pdf": (c_float, [P(OracleScene), P(_abi.Uniforms), P(c_float)]),
             "oracle_post": (None, [P(OracleScene), P(_abi.Uniforms), P(c_float), c_uint32, c_uint32, P(c_float),
                                    P(c_uint8), P(c_float), P(c_float)]),
             "oracle_linear_to_srgb": (c_float, [c_float]),
@@ -159,11 +165,26 @@ def tf_evaluate(tf, hu: float) -> np.ndarray:
 
 
 def pack_uniforms(volume: _abi.Volume, params: _abi.FrameParams, bmax: np.ndarray, has_labels: bool,
-                  accumulated: int, width: int, height: int, frame: int) -> _abi.Uniforms:
+                  accumulated: int, width: int, height: int, frame: int, env_peak_u=None) -> _abi.Uniforms:
+    """The uniform block; env_peak_u: the light map's brightest column, None without a map."""
     u = _abi.Uniforms()
-    load().oracle_pack_uniforms(byref(volume), byref(params), _p(np.ascontiguousarray(bmax, np.float32), c_float),
-                                1 if has_labels else 0, accumulated, width, height, frame, byref(u))
+    load().oracle_pack_uniforms_env(byref(volume), byref(params), _p(np.ascontiguousarray(bmax, np.float32), c_float),
+                                    1 if has_labels else 0, accumulated, width, height, frame,
+                                    0 if env_peak_u is None else 1, float(env_peak_u or 0.0), byref(u))
     return u
+
+
+def env_prepare(rgb: np.ndarray):
+    """EnvironmentMap::prepare + upload of an (h, w, 3) linear map: (radiance
+    RGBA16F bits (h, w, 4), sampling table (h, w + 1, 2), diffuse RGBA16F bits
+    (16, 32, 4), peak_u)."""
+    a = np.ascontiguousarray(rgb, np.float32)
+    h, w = a.shape[:2]
+    rad = np.zeros((h, w, 4), np.uint16)
+    table = np.zeros((h, w + 1, 2), np.float32)
+    dif = np.zeros((16, 32, 4), np.uint16)
+    peak = load().oracle_env_prepare(w, h, _p(a, c_float), _p(rad, c_uint16), _p(table, c_float), _p(dif, c_uint16))
+    return rad, table, dif, float(peak)
 
 
 class Scene:
@@ -238,6 +259,25 @@ class Scene:
         soft_a = np.ascontiguousarray(soft, np.uint8) if soft is not None else None
         load().oracle_mask_binary(byref(self.c), byref(u), _p(soft_a, c_uint8), sd, sp, margin, _p(out, c_float))
         return out
+
+    def set_env(self, radiance, table, diffuse):
+        """The light map's three images (env_prepare's output)."""
+        for name, a, dt, ct in (("env_radiance", radiance, np.uint16, c_uint16), ("env_table", table, np.float32, c_float),
+                                ("env_diffuse", diffuse, np.uint16, c_uint16)):
+            a = np.ascontiguousarray(a, dt)
+            self.arrays[name] = a
+            setattr(self.c, name, _p(a, ct))
+        self.c.env_h, self.c.env_w = radiance.shape[:2]
+
+    def sample_env(self, u: _abi.Uniforms, seed: int, n: int):
+        dirs = np.zeros((n, 3), np.float32)
+        pdfs = np.zeros(n, np.float32)
+        load().oracle_sample_env(byref(self.c), byref(u), seed, n, _p(dirs, c_float), _p(pdfs, c_float))
+        return dirs, pdfs
+
+    def env_pdf(self, u: _abi.Uniforms, d) -> float:
+        v = np.ascontiguousarray(d, np.float32)
+        return float(load().oracle_env_pdf(byref(self.c), byref(u), _p(v, c_float)))
 
     def shadow_dims(self):
         return tuple((n + 1) // 2 for n in self.dims)

@@ -28,6 +28,12 @@ typedef struct oracle_scene {
     const float* majorant;    /* binding 22, R32F, ceil(dims / 8)                  */
     const uint16_t* shadow;   /* binding 6, RGBA16F bits, ceil(dims / 2) (NULL: off) */
     const uint16_t* ao;       /* binding 7, R16F bits, ceil(dims / 4) (NULL: off)    */
+    /* Light map (bindings 25-27; NULL without one): RGBA16F radiance env_w x env_h,
+     * RG32F sampling table (env_w + 1) x env_h, RGBA16F diffuse 32 x 16. */
+    const uint16_t* env_radiance;
+    const float* env_table;
+    const uint16_t* env_diffuse;
+    uint32_t env_w, env_h;
 } oracle_scene;
 
 uint16_t oracle_hu_to_unorm16(int16_t hu);
@@ -41,6 +47,23 @@ void oracle_geometry(const vvr_volume* vol, float world_to_index[16], float inde
                      float* diagonal, float* min_spacing, float* max_spacing);
 void oracle_pack_uniforms(const vvr_volume* vol, const vvr_frame_params* p, const float* bucket_max64, int has_labels,
                           uint32_t accumulated, uint32_t width, uint32_t height, uint64_t frame, vvr_uniforms* u);
+/* As oracle_pack_uniforms with a light map whose brightest column is env_peak_u
+ * (has_map 0: none). */
+void oracle_pack_uniforms_env(const vvr_volume* vol, const vvr_frame_params* p, const float* bucket_max64,
+                              int has_labels, uint32_t accumulated, uint32_t width, uint32_t height, uint64_t frame,
+                              int has_map, float env_peak_u, vvr_uniforms* u);
+/* EnvironmentMap::prepare and upload_environment (environment.rs:56-180,
+ * renderer.rs:2224-2290) of a width x height linear RGB map, rows from the
+ * zenith: radiance RGBA16F bits (width x height x 4), the RG32F sampling table
+ * ((width + 1) x height x 2), the diffuse map RGBA16F bits (32 x 16 x 4); returns
+ * the brightest column peak_u. */
+float oracle_env_prepare(uint32_t width, uint32_t height, const float* rgb, uint16_t* radiance, float* table,
+                         uint16_t* diffuse);
+/* sample_env (pt.comp:203-226) with a PCG stream seeded `seed`: direction, pdf;
+ * and env_pdf (pt.comp:193-199) of a direction. */
+void oracle_sample_env(const oracle_scene* s, const vvr_uniforms* u, uint32_t seed, uint32_t n, float* dirs,
+                       float* pdfs);
+float oracle_env_pdf(const oracle_scene* s, const vvr_uniforms* u, const float d[3]);
 
 uint16_t oracle_f32_to_f16(float f);
 float oracle_f16_to_f32(uint16_t h);

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "../include/vvr.h"
 #include "../include/vvr_uniforms.h"
@@ -237,10 +238,11 @@ static V light_direction(const vvr_lighting& l, const vvr_camera& cam) {
 }
 
 // Renderer::record's uniform fill (renderer.rs:1516-1660) for a renderer with
-// volume `vol`, `accumulated` paths so far, labels uploaded or not, no light map.
-extern "C" void oracle_pack_uniforms(const vvr_volume* vol, const vvr_frame_params* p, const float* bucket_max64,
-                                     int has_labels, uint32_t accumulated, uint32_t width, uint32_t height,
-                                     uint64_t frame, vvr_uniforms* u) {
+// volume `vol`, `accumulated` paths so far, labels uploaded or not, and a light
+// map whose brightest column is env_peak_u when has_map.
+extern "C" void oracle_pack_uniforms_env(const vvr_volume* vol, const vvr_frame_params* p, const float* bucket_max64,
+                                         int has_labels, uint32_t accumulated, uint32_t width, uint32_t height,
+                                         uint64_t frame, int has_map, float env_peak_u, vvr_uniforms* u) {
     std::memset(u, 0, sizeof(*u));
     float w2i[16], i2w[16], center[3], diagonal, min_sp, max_sp;
     oracle_geometry(vol, w2i, i2w, center, &diagonal, &min_sp, &max_sp);
@@ -252,9 +254,15 @@ extern "C" void oracle_pack_uniforms(const vvr_volume* vol, const vvr_frame_para
     const vvr_lighting& light = p->lighting;
     V ld = light_direction(light, cam);
     const vvr_label_params& labels = p->labels;
-    // Map without a light map set falls back to the studio (renderer.rs:1521-1524).
-    uint32_t environment = light.environment == VVR_ENV_MAP ? (uint32_t)VVR_ENV_STUDIO : light.environment;
-    float env_rotation = 0.0f;  // no light map
+    // Map without a light map set falls back to the studio (renderer.rs:1524-1527).
+    uint32_t environment =
+        light.environment == VVR_ENV_MAP && !has_map ? (uint32_t)VVR_ENV_STUDIO : light.environment;
+    // The map's brightest column turned to the key light's azimuth (renderer.rs:1531-1534).
+    float env_rotation = 0.0f;
+    if (has_map) {
+        float key_phi = std::atan2(std::sin(light.azimuth), -std::cos(light.azimuth));
+        env_rotation = (env_peak_u - 0.5f) * 6.28318530717958647692f - key_phi + light.env_rotation;
+    }
 
     std::memcpy(u->world_to_index, w2i, sizeof(w2i));
     V f = cam_forward(cam);
@@ -361,4 +369,111 @@ extern "C" void oracle_pack_uniforms(const vvr_volume* vol, const vvr_frame_para
     float limit = 1.0f / (2.0f * max_sp);  // TISSUE_MIN_FREE_PATH_VOXELS
     u->tissue[0] = tissue_majorant(labels, light.anisotropy, limit);
     u->tissue[1] = limit;
+}
+
+extern "C" void oracle_pack_uniforms(const vvr_volume* vol, const vvr_frame_params* p, const float* bucket_max64,
+                                     int has_labels, uint32_t accumulated, uint32_t width, uint32_t height,
+                                     uint64_t frame, vvr_uniforms* u) {
+    oracle_pack_uniforms_env(vol, p, bucket_max64, has_labels, accumulated, width, height, frame, 0, 0.0f, u);
+}
+
+// ---- EnvironmentMap::prepare (environment.rs:56-180) and its upload (renderer.rs:2224-2290)
+namespace {
+const float kPiF = 3.14159265358979323846f;
+float env_lum(const float* c) { return (0.2126f * c[0] + 0.7152f * c[1]) + 0.0722f * c[2]; }
+void env_dir(float uu, float vv, float d[3]) {  // environment.rs:44-54
+    float phi = (uu - 0.5f) * 2.0f * kPiF;
+    float theta = vv * kPiF;
+    d[0] = std::sin(theta) * std::sin(phi);
+    d[1] = std::cos(theta);
+    d[2] = std::sin(theta) * std::cos(phi);
+}
+}  // namespace
+
+extern "C" float oracle_env_prepare(uint32_t width, uint32_t height, const float* rgb, uint16_t* radiance_out,
+                                    float* table, uint16_t* diffuse_out) {
+    const size_t w = width, h = height;
+    auto sin_theta = [&](size_t y) { return std::sin(kPiF * ((float)y + 0.5f) / (float)h); };
+    double sum[3] = {0, 0, 0}, weight = 0;
+    for (size_t y = 0; y < h; y++) {
+        double row[3] = {0, 0, 0};
+        for (size_t x = 0; x < w; x++)
+            for (int i = 0; i < 3; i++) row[i] += (double)rgb[3 * (y * w + x) + i];
+        float st = sin_theta(y);
+        for (int i = 0; i < 3; i++) sum[i] += row[i] * (double)st;
+        weight += (double)st * (double)w;
+    }
+    float mean_c[3] = {(float)(sum[0] / weight), (float)(sum[1] / weight), (float)(sum[2] / weight)};
+    float mean = std::max(env_lum(mean_c), 1e-6f);
+    std::vector<float> radiance(4 * w * h);
+    for (size_t i = 0; i < w * h; i++) {
+        for (int k = 0; k < 3; k++) radiance[4 * i + k] = rgb[3 * i + k] / mean;
+        radiance[4 * i + 3] = 1.0f;
+    }
+    std::vector<float> f(w * h);
+    for (size_t i = 0; i < w * h; i++) f[i] = env_lum(rgb + 3 * i) * sin_theta(i / w);
+    std::vector<double> row_sums(h, 0.0);
+    for (size_t y = 0; y < h; y++)
+        for (size_t x = 0; x < w; x++) row_sums[y] += (double)f[y * w + x];
+    double total = 0;
+    for (double r : row_sums) total += r;
+    total = std::max(total, 1e-12);
+    double marginal = 0;
+    for (size_t y = 0; y < h; y++) {
+        float* row = table + 2 * y * (w + 1);
+        double acc = 0;
+        for (size_t x = 0; x < w; x++) {
+            acc += (double)f[y * w + x];
+            row[2 * x] = row_sums[y] > 0.0 ? (float)(acc / row_sums[y]) : (float)(x + 1) / (float)w;
+            row[2 * x + 1] = (float)((double)f[y * w + x] / total);
+        }
+        row[2 * (w - 1)] = 1.0f;
+        marginal += row_sums[y];
+        row[2 * w] = (float)(marginal / total);
+        row[2 * w + 1] = (float)(row_sums[y] / total);
+    }
+    table[2 * ((h - 1) * (w + 1) + w)] = 1.0f;
+    double px = 0, pz = 0;
+    for (size_t y = 0; y < h; y++)
+        for (size_t x = 0; x < w; x++) {
+            double l = (double)f[y * w + x];
+            float d[3];
+            env_dir(((float)x + 0.5f) / (float)w, ((float)y + 0.5f) / (float)h, d);
+            px += l * l * (double)d[0];
+            pz += l * l * (double)d[2];
+        }
+    float peak_u = 0.5f + (float)(std::atan2(px, pz) / (2.0 * 3.14159265358979323846));
+    // Diffuse convolution (environment.rs:135-178).
+    const size_t sw = std::min<size_t>(128, w), sh = std::max<size_t>(sw / 2, 1);
+    std::vector<float> small_l(3 * sw * sh), small_d(3 * sw * sh);
+    for (size_t i = 0; i < sw * sh; i++) {
+        size_t sx = i % sw, sy = i / sw;
+        size_t x0 = sx * w / sw, x1 = std::max((sx + 1) * w / sw, x0 + 1);
+        size_t y0 = sy * h / sh, y1 = std::max((sy + 1) * h / sh, y0 + 1);
+        float c[3] = {0, 0, 0};
+        for (size_t y = y0; y < y1; y++)
+            for (size_t x = x0; x < x1; x++)
+                for (int k = 0; k < 3; k++) c[k] = c[k] + radiance[4 * (y * w + x) + k];
+        float n = (float)((y1 - y0) * (x1 - x0));
+        float uu = ((float)sx + 0.5f) / (float)sw, vv = ((float)sy + 0.5f) / (float)sh;
+        float solid = (2.0f * kPiF / (float)sw) * (kPiF / (float)sh) * std::sin(kPiF * vv);
+        for (int k = 0; k < 3; k++) small_l[3 * i + k] = c[k] / n * solid;
+        env_dir(uu, vv, &small_d[3 * i]);
+    }
+    const size_t dw = 32, dh = 16;
+    for (size_t i = 0; i < dw * dh; i++) {
+        float nrm[3];
+        env_dir(((float)(i % dw) + 0.5f) / (float)dw, ((float)(i / dw) + 0.5f) / (float)dh, nrm);
+        float c[3] = {0, 0, 0};
+        for (size_t j = 0; j < sw * sh; j++) {
+            const float* d = &small_d[3 * j];
+            float cs = (nrm[0] * d[0] + nrm[1] * d[1]) + nrm[2] * d[2];
+            if (cs > 0.0f)
+                for (int k = 0; k < 3; k++) c[k] = c[k] + small_l[3 * j + k] * cs;
+        }
+        for (int k = 0; k < 3; k++) diffuse_out[4 * i + k] = f32_to_f16(std::min(c[k] / kPiF, 60000.0f));
+        diffuse_out[4 * i + 3] = f32_to_f16(1.0f);
+    }
+    for (size_t i = 0; i < 4 * w * h; i++) radiance_out[i] = f32_to_f16(std::min(radiance[i], 60000.0f));
+    return peak_u;
 }

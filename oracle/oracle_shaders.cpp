@@ -754,7 +754,10 @@ template <class R> struct Ctx {
                                 diffuse * s_.subsurface * diffused;
                 V3<R> volume = s_.albedo / (R(4) * PI()) * vmix(V3<R>(shadow), diffused, s_.subsurface);
                 V3<R> direct = vmix(volume, surface, surfaceness) * light;
+                // dvr.comp:135-139: a light map's diffuse response where surfaces face it.
                 V3<R> ambient_light(u->ambient[0], u->ambient[1], u->ambient[2]);
+                if (u->lighting[3] == 2u)
+                    ambient_light = vmix(ambient_light, env_map_diffuse(n), surfaceness * (R(1) - s_.subsurface));
                 V3<R> ambient = s_.albedo * ambient_light * ao;
                 radiance = radiance + transmittance * alpha * (direct + ambient);
                 transmittance *= R(1) - alpha;
@@ -830,8 +833,87 @@ template <class R> struct Ctx {
     }
     R rand() { return R(pcg() >> 8) * R(1.0 / 16777216.0); }
 
-    V3<R> environment(V3<R> d) const {  // pt.comp:56-82 (no light map)
+    // ---- light map (common.glsl:459-485, pt.comp:193-226) --------------------------------
+    V3<R> basis_f() const { return normalize(V3<R>(u->forward[0], u->forward[1], u->forward[2])); }
+    V3<R> basis_r() const { return normalize(V3<R>(u->right[0], u->right[1], u->right[2])); }
+    V3<R> basis_u() const { return normalize(V3<R>(u->up[0], u->up[1], u->up[2])); }
+    void env_uv(V3<R> d, R& uu, R& vv) const {
+        V3<R> f = basis_f(), r = basis_r(), up = basis_u();
+        R phi = std::atan2(dot(d, r), dot(d, f));
+        R theta = std::acos(clampr(dot(d, up), R(-1), R(1)));
+        R x = R(0.5) + (phi + R(u->env[0])) / (R(2) * PI());
+        uu = x - std::floor(x);
+        vv = theta / PI();
+    }
+    V3<R> env_direction(R uu, R vv) const {
+        R phi = (uu - R(0.5)) * R(2) * PI() - R(u->env[0]);
+        R theta = vv * PI();
+        V3<R> f = basis_f(), r = basis_r(), up = basis_u();
+        return std::sin(theta) * (std::sin(phi) * r + std::cos(phi) * f) + std::cos(theta) * up;
+    }
+    // Linear clamp-to-edge lookup of an RGBA16F image (the renderer's sampler).
+    V3<R> image_rgb(const uint16_t* img, int w, int h, R uu, R vv) const {
+        R tx = uu * R(w) - R(0.5), ty = vv * R(h) - R(0.5);
+        R fx = std::floor(tx), fy = std::floor(ty);
+        R ax = tx - fx, ay = ty - fy;
+        int x0 = (int)fx, y0 = (int)fy;
+        R c[3];
+        for (int k = 0; k < 3; k++) {
+            auto t = [&](int x, int y) {
+                return R(f16_to_f32(img[4 * ((size_t)iclamp(y, 0, h - 1) * w + iclamp(x, 0, w - 1)) + k]));
+            };
+            c[k] = mixr(mixr(t(x0, y0), t(x0 + 1, y0), ax), mixr(t(x0, y0 + 1), t(x0 + 1, y0 + 1), ax), ay);
+        }
+        return V3<R>(c[0], c[1], c[2]);
+    }
+    V3<R> ambient_rgb() const { return V3<R>(u->ambient[0], u->ambient[1], u->ambient[2]); }
+    V3<R> env_map_radiance(V3<R> d) const {
+        R uu, vv;
+        env_uv(d, uu, vv);
+        return ambient_rgb() * image_rgb(s->env_radiance, (int)s->env_w, (int)s->env_h, uu, vv);
+    }
+    V3<R> env_map_diffuse(V3<R> n) const {
+        R uu, vv;
+        env_uv(n, uu, vv);
+        return ambient_rgb() * image_rgb(s->env_diffuse, 32, 16, uu, vv);
+    }
+    R table(int x, int y, int ch) const { return R(s->env_table[2 * ((size_t)y * (s->env_w + 1) + x) + ch]); }
+    R env_pdf(V3<R> d) const {
+        int w = (int)s->env_w, h = (int)s->env_h;
+        R uu, vv;
+        env_uv(d, uu, vv);
+        int tx = std::min((int)(uu * R(w)), w - 1), ty = std::min((int)(vv * R(h)), h - 1);
+        R p = table(tx, ty, 1);
+        return p * R(w * h) / (R(2) * PI() * PI() * gmax(std::sin(vv * PI()), R(1e-6)));
+    }
+    V3<R> sample_env(R& pdf) {
+        int w = (int)s->env_w, h = (int)s->env_h;
+        R xi = rand();
+        int lo = 0, hi = h - 1;
+        while (lo < hi) {
+            int mid = (lo + hi) / 2;
+            if (table(w, mid, 0) >= xi) hi = mid;
+            else lo = mid + 1;
+        }
+        int y = lo;
+        xi = rand();
+        lo = 0, hi = w - 1;
+        while (lo < hi) {
+            int mid = (lo + hi) / 2;
+            if (table(mid, y, 0) >= xi) hi = mid;
+            else lo = mid + 1;
+        }
+        R ju = rand();
+        R jv = rand();
+        R uu = (R(lo) + ju) / R(w), vv = (R(y) + jv) / R(h);
+        R p = table(lo, y, 1);
+        pdf = p * R(w * h) / (R(2) * PI() * PI() * gmax(std::sin(vv * PI()), R(1e-6)));
+        return env_direction(uu, vv);
+    }
+
+    V3<R> environment(V3<R> d) const {  // pt.comp:56-82
         V3<R> amb(u->ambient[0], u->ambient[1], u->ambient[2]);
+        if (u->lighting[3] == 2u) return env_map_radiance(d);
         if (u->lighting[3] == 1u) {
             V3<R> f = normalize(V3<R>(u->forward[0], u->forward[1], u->forward[2]));
             V3<R> r = normalize(V3<R>(u->right[0], u->right[1], u->right[2]));
@@ -1025,6 +1107,7 @@ template <class R> struct Ctx {
         R light_pdf = area_light ? R(1) / (R(2) * PI() * (R(1) - cos_light)) : R(0);
         M3<R> ti = to_index();
         V3<R> eye_idx = world_to_index_point(V3<R>(u->eye[0], u->eye[1], u->eye[2]));
+        bool env_map = u->lighting[3] == 2u;
 
         V3<R> sum(0);
         R background = 0;
@@ -1064,7 +1147,7 @@ template <class R> struct Ctx {
                         albedo_sum[0] += R(1), albedo_sum[1] += R(1), albedo_sum[2] += R(1);
                         normal_sum[3] += R(1e4);
                     } else {
-                        R weight = R(1);
+                        R weight = env_map ? mis(last_pdf, env_pdf(dir)) : R(1);
                         V3<R> path_weight = throughput * spectral / dot(spectral, V3<R>(R(1.0 / 3.0)));
                         radiance = radiance + path_weight * environment(dir) * weight;
                         if (area_light && dot(dir, ldir) >= cos_light)
@@ -1089,6 +1172,14 @@ template <class R> struct Ctx {
                 }
                 V3<R> light_dir = sample_light(light_frame);
                 V3<R> light_t = transmittance(idx, light_dir);
+                // pt.comp:467-476: a light-map sample and its transmittance.
+                R map_pdf = 0;
+                V3<R> map_dir(0), map_light(0);
+                if (env_map) {
+                    map_dir = sample_env(map_pdf);
+                    map_light = env_map_radiance(map_dir) / gmax(map_pdf, R(1e-8));
+                    if (map_pdf > R(0)) map_light = map_light * transmittance(idx, map_dir);
+                }
                 if (surface_event) {
                     n = dot(n, v) < R(0) ? -n : n;
                     V3<R> diffuse;
@@ -1096,6 +1187,11 @@ template <class R> struct Ctx {
                     R light_weight = area_light ? mis(light_pdf, bsdf_pdf(n, v, light_dir, s_)) : R(1);
                     radiance = radiance + path_weight * (diffuse + specular) * gmax(dot(n, light_dir), R(0)) * light *
                                               light_t * light_weight;
+                    if (map_pdf > R(0) && dot(n, map_dir) > R(0)) {  // pt.comp:485-489
+                        specular = brdf_coated(n, v, map_dir, s_, diffuse);
+                        radiance = radiance + path_weight * (diffuse + specular) * dot(n, map_dir) * map_light *
+                                                  mis(map_pdf, bsdf_pdf(n, v, map_dir, s_));
+                    }
                     M3<R> frame = basis(n);
                     V3<R> l;
                     if (rand() < coat_share(s_)) {
@@ -1121,6 +1217,10 @@ template <class R> struct Ctx {
                     R light_phase = phase_hg(dot(dir, light_dir));
                     radiance = radiance + path_weight * albedo * light_phase * light * light_t *
                                               (area_light ? mis(light_pdf, light_phase) : R(1));
+                    if (map_pdf > R(0)) {  // pt.comp:524-527
+                        R phase = phase_hg(dot(dir, map_dir));
+                        radiance = radiance + path_weight * albedo * phase * map_light * mis(map_pdf, phase);
+                    }
                     throughput = throughput * albedo;
                     volumes++;
                     origin = idx;
@@ -1826,6 +1926,23 @@ void oracle_post(const oracle_scene* s, const vvr_uniforms* u, const float* hdr,
             uint8_t* o = out + 4 * ((size_t)y * w + x);
             o[0] = unorm8(c[0]), o[1] = unorm8(c[1]), o[2] = unorm8(c[2]), o[3] = 255;
         }
+}
+
+void oracle_sample_env(const oracle_scene* s, const vvr_uniforms* u, uint32_t seed, uint32_t n, float* dirs,
+                       float* pdfs) {
+    Ctx<float> c(s, u);
+    c.rng = seed;
+    for (uint32_t i = 0; i < n; i++) {
+        float pdf;
+        V3<float> d = c.sample_env(pdf);
+        dirs[3 * i] = d.x, dirs[3 * i + 1] = d.y, dirs[3 * i + 2] = d.z;
+        pdfs[i] = pdf;
+    }
+}
+
+float oracle_env_pdf(const oracle_scene* s, const vvr_uniforms* u, const float d[3]) {
+    Ctx<float> c(s, u);
+    return c.env_pdf(V3<float>(d[0], d[1], d[2]));
 }
 
 void oracle_pt(const oracle_scene* s, const vvr_uniforms* u, const uint32_t* pixels, size_t n, int precision,

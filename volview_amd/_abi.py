@@ -119,6 +119,10 @@ class ServerStats(Structure):
                 ("iterations", c_uint64), ("hbm_in_use", c_uint64), ("hbm_peak", c_uint64)]
 
 
+class EnvMap(Structure):
+    _fields_ = [("width", c_uint32), ("height", c_uint32), ("rgb", POINTER(c_float))]
+
+
 SIGNATURES = [
     ("vvr_label_params_default", None, [POINTER(LabelParams)]),
     ("vvr_lighting_default", None, [POINTER(Lighting)]),
@@ -149,6 +153,8 @@ SIGNATURES = [
     ("vvr_renderer_read_accum", c_int, [c_void_p, POINTER(c_float), POINTER(c_float), POINTER(c_float)]),
     ("vvr_renderer_read_denoised", c_int, [c_void_p, POINTER(c_float)]),
     ("vvr_renderer_read_rgba", c_int, [c_void_p, POINTER(c_uint8)]),
+    ("vvr_renderer_set_environment_map", c_int, [c_void_p, POINTER(EnvMap)]),
+    ("vvr_renderer_read_environment", c_int, [c_void_p, POINTER(c_float), POINTER(c_float), c_size_t]),
     ("vvr_renderer_read_exposure", c_int, [c_void_p, POINTER(c_float)]),
     ("vvr_render_to_rgba", c_int, [c_void_p, POINTER(FrameParams), c_uint32, c_uint32, c_uint32, POINTER(c_uint8)]),
     ("vvr_renderer_read_volume", c_int, [c_void_p, c_int, c_void_p, c_size_t]),

@@ -2,6 +2,8 @@
 // A path's random numbers come from one PCG stream (pt.comp:41-53) seeded per
 // (pixel, sample index) exactly as the reference seeds a frame of one path.
 #pragma once
+#include <hip/hip_fp16.h>
+
 #include "shading.hpp"
 
 namespace vvr {
@@ -27,8 +29,100 @@ __device__ __forceinline__ Rng seed_rng(uint32_t px, uint32_t py, uint32_t path_
     return r;
 }
 
-// pt.comp:56-82, for the dome and the studio (no light map).
-__device__ inline V3 environment(const vvr_uniforms* u, V3 d) {
+// ---- light map (common.glsl:457-485, pt.comp:192-226) ------------------------------
+// Equirectangular, fixed to the camera like the key light: the centre column
+// ahead, the top row above, turned by env.x.
+
+__device__ __forceinline__ void env_uv(const vvr_uniforms* u, V3 d, float& uu, float& vv) {
+    const V3 f = normalize(V3(u->forward[0], u->forward[1], u->forward[2]));
+    const V3 r = normalize(V3(u->right[0], u->right[1], u->right[2]));
+    const V3 up = normalize(V3(u->up[0], u->up[1], u->up[2]));
+    const float phi = atan2f(dot(d, r), dot(d, f));
+    const float theta = acosf(clampf(dot(d, up), -1.0f, 1.0f));
+    const float x = 0.5f + (phi + u->env[0]) / (2.0f * kPi);
+    uu = x - floorf(x);
+    vv = theta / kPi;
+}
+
+__device__ __forceinline__ V3 env_direction(const vvr_uniforms* u, float uu, float vv) {
+    const float phi = (uu - 0.5f) * 2.0f * kPi - u->env[0];
+    const float theta = vv * kPi;
+    const V3 f = normalize(V3(u->forward[0], u->forward[1], u->forward[2]));
+    const V3 r = normalize(V3(u->right[0], u->right[1], u->right[2]));
+    const V3 up = normalize(V3(u->up[0], u->up[1], u->up[2]));
+    return sinf(theta) * (sinf(phi) * r + cosf(phi) * f) + cosf(theta) * up;
+}
+
+__device__ __forceinline__ V3 env_texel(const uint2* img, int w, int x, int y) {
+    const uint2 t = img[y * w + x];
+    return V3(__half2float(__ushort_as_half((unsigned short)(t.x & 0xffffu))),
+              __half2float(__ushort_as_half((unsigned short)(t.x >> 16))),
+              __half2float(__ushort_as_half((unsigned short)(t.y & 0xffffu))));
+}
+
+// The renderer's linear clamp-to-edge sampler on an RGBA16F image at (uu, vv).
+__device__ __forceinline__ V3 env_image(const uint2* img, int w, int h, float uu, float vv) {
+    const float tx = uu * float(w) - 0.5f, ty = vv * float(h) - 0.5f;
+    const float fx = floorf(tx), fy = floorf(ty);
+    const float ax = tx - fx, ay = ty - fy;
+    const int x0 = min(max(int(fx), 0), w - 1), x1 = min(max(int(fx) + 1, 0), w - 1);
+    const int y0 = min(max(int(fy), 0), h - 1), y1 = min(max(int(fy) + 1, 0), h - 1);
+    return vmix(vmix(env_texel(img, w, x0, y0), env_texel(img, w, x1, y0), ax),
+                vmix(env_texel(img, w, x0, y1), env_texel(img, w, x1, y1), ax), ay);
+}
+
+__device__ __forceinline__ V3 env_map_radiance(const DevScene& s, V3 d) {  // common.glsl:477-480
+    float uu, vv;
+    env_uv(s.u, d, uu, vv);
+    return V3(s.u->ambient[0], s.u->ambient[1], s.u->ambient[2]) * env_image(s.env_rad, s.env_w, s.env_h, uu, vv);
+}
+
+__device__ __forceinline__ V3 env_map_diffuse(const DevScene& s, V3 n) {  // common.glsl:482-485
+    float uu, vv;
+    env_uv(s.u, n, uu, vv);
+    return V3(s.u->ambient[0], s.u->ambient[1], s.u->ambient[2]) * env_image(s.env_dif, 32, 16, uu, vv);
+}
+
+// Solid-angle pdf of drawing direction d from the light map (pt.comp:193-199).
+__device__ __forceinline__ float env_pdf(const DevScene& s, V3 d) {
+    float uu, vv;
+    env_uv(s.u, d, uu, vv);
+    const int tx = min(int(uu * float(s.env_w)), s.env_w - 1), ty = min(int(vv * float(s.env_h)), s.env_h - 1);
+    const float p = s.env_table[ty * (s.env_w + 1) + tx].y;
+    return p * float(s.env_w * s.env_h) / (2.0f * kPi * kPi * gmax(sinf(vv * kPi), 1e-6f));
+}
+
+// A direction drawn proportionally to the map's luminance: a row by binary
+// search of the marginal CDF, then a column of the row's CDF (pt.comp:203-226).
+__device__ inline V3 sample_env(const DevScene& s, Rng& rng, float& pdf) {
+    const int w = s.env_w, h = s.env_h;
+    float xi = rng.rand();
+    int lo = 0, hi = h - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if (s.env_table[mid * (w + 1) + w].x >= xi) hi = mid;
+        else lo = mid + 1;
+    }
+    const int y = lo;
+    xi = rng.rand();
+    lo = 0, hi = w - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if (s.env_table[y * (w + 1) + mid].x >= xi) hi = mid;
+        else lo = mid + 1;
+    }
+    const float ju = rng.rand();
+    const float jv = rng.rand();
+    const float uu = (float(lo) + ju) / float(w), vv = (float(y) + jv) / float(h);
+    const float p = s.env_table[y * (w + 1) + lo].y;
+    pdf = p * float(w * h) / (2.0f * kPi * kPi * gmax(sinf(vv * kPi), 1e-6f));
+    return env_direction(s.u, uu, vv);
+}
+
+// pt.comp:56-82: the light map, the studio or the dome.
+__device__ inline V3 environment(const DevScene& s, V3 d) {
+    const vvr_uniforms* u = s.u;
+    if (u->lighting[3] == 2u) return env_map_radiance(s, d);
     const V3 amb(u->ambient[0], u->ambient[1], u->ambient[2]);
     if (u->lighting[3] == 1u) {
         const V3 f = normalize(V3(u->forward[0], u->forward[1], u->forward[2]));
@@ -253,6 +347,7 @@ struct FrameConsts {
     V3 light;
     float cos_light, light_pdf;
     bool area_light;
+    bool env_map;  // Environment::Map with a light map (pt.comp:395)
     M3 ti;
     V3 eye_idx;
 };
@@ -268,6 +363,7 @@ __device__ __forceinline__ FrameConsts frame_consts(const vvr_uniforms* u) {
     c.cos_light = cosf(u->material[3]);
     c.area_light = u->material[3] > 1e-4f;
     c.light_pdf = c.area_light ? 1.0f / (2.0f * kPi * (1.0f - c.cos_light)) : 0.0f;
+    c.env_map = u->lighting[3] == 2u;
     c.ti = to_index(u);
     c.eye_idx = world_to_index_point(u, V3(u->eye[0], u->eye[1], u->eye[2]));
     return c;
@@ -332,6 +428,9 @@ struct EventState {
     bool surface_event;
     V3 light_dir;    // key-light sample
     V3 light_t;      // its transmittance
+    V3 map_dir;      // light-map sample (pt.comp:467-476)
+    float map_pdf;   // 0: none
+    V3 map_light;    // its radiance / pdf x transmittance
 };
 
 // pt.comp:427-428: whether the bounce loop runs another iteration.
@@ -340,15 +439,17 @@ __device__ __forceinline__ bool path_continues(const FrameConsts& fc, const Path
 }
 
 // pt.comp:434-447: what a path that left the volume adds: the background guides
-// on bounce 0, else the environment and the key light seen along the ray.
-__device__ __forceinline__ void path_escape(const vvr_uniforms* u, const FrameConsts& fc, PathState& st) {
+// on bounce 0, else the environment (MIS-weighted against the light-map
+// samples) and the key light seen along the ray.
+__device__ __forceinline__ void path_escape(const DevScene& sc, const FrameConsts& fc, PathState& st) {
     if (st.bounce == 0u) {
         st.background = 1.0f;
         st.albedo = V3(1.0f);
         st.depth = 1e4f;
     } else {
+        const float weight = fc.env_map ? mis(st.last_pdf, env_pdf(sc, st.dir)) : 1.0f;
         const V3 path_weight = st.throughput * st.spectral / dot(st.spectral, V3(1.0f / 3.0f));
-        st.radiance = st.radiance + path_weight * environment(u, st.dir);
+        st.radiance = st.radiance + path_weight * environment(sc, st.dir) * weight;
         if (fc.area_light && dot(st.dir, fc.ldir) >= fc.cos_light)
             st.radiance = st.radiance + path_weight * fc.light * fc.light_pdf * mis(st.last_pdf, fc.light_pdf);
     }
@@ -362,7 +463,7 @@ __device__ __forceinline__ bool stage_extend(const DevScene& sc, const FrameCons
     if (free_flight(sc, st.origin, st.dir, st.bounce == 0u, st.hero, st.spectral, ev.idx, ev.s, ev.cap_plane, ev.cut,
                     st.rng))
         return true;
-    path_escape(sc.u, fc, st);
+    path_escape(sc, fc, st);
     return false;
 }
 
@@ -385,9 +486,17 @@ __device__ __forceinline__ void stage_shade(const DevScene& sc, const FrameConst
     ev.light_dir = sample_light(u, fc.light_frame, st.rng);
 }
 
-// pt.comp:466: the shadow ray.
+// pt.comp:466-476: the shadow ray, then with a light map a light-map sample and
+// its shadow ray.
 __device__ __forceinline__ void stage_shadow(const DevScene& sc, PathState& st, EventState& ev) {
     ev.light_t = transmittance(sc, ev.idx, ev.light_dir, st.rng);
+    ev.map_pdf = 0.0f;
+    ev.map_dir = ev.map_light = V3(0.0f);
+    if (sc.u->lighting[3] == 2u) {
+        ev.map_dir = sample_env(sc, st.rng, ev.map_pdf);
+        ev.map_light = env_map_radiance(sc, ev.map_dir) / gmax(ev.map_pdf, 1e-8f);
+        if (ev.map_pdf > 0.0f) ev.map_light = ev.map_light * transmittance(sc, ev.idx, ev.map_dir, st.rng);
+    }
 }
 
 // pt.comp:478-544: next-event estimate, the next direction, Russian roulette.
@@ -405,6 +514,11 @@ __device__ __forceinline__ bool stage_scatter(const DevScene& sc, const FrameCon
         const float light_weight = fc.area_light ? mis(fc.light_pdf, bsdf_pdf(n, v, ev.light_dir, s)) : 1.0f;
         st.radiance = st.radiance + path_weight * (diffuse + specular) * gmax(dot(n, ev.light_dir), 0.0f) * fc.light *
                                         ev.light_t * light_weight;
+        if (ev.map_pdf > 0.0f && dot(n, ev.map_dir) > 0.0f) {  // pt.comp:485-489
+            specular = brdf_coated(n, v, ev.map_dir, s, diffuse);
+            st.radiance = st.radiance + path_weight * (diffuse + specular) * dot(n, ev.map_dir) * ev.map_light *
+                                            mis(ev.map_pdf, bsdf_pdf(n, v, ev.map_dir, s));
+        }
         const M3 frame = basis(n);
         V3 l;
         if (st.rng.rand() < coat_share(s)) {
@@ -430,6 +544,10 @@ __device__ __forceinline__ bool stage_scatter(const DevScene& sc, const FrameCon
         const float light_phase = phase_hg(u, dot(st.dir, ev.light_dir));
         st.radiance = st.radiance + path_weight * albedo * light_phase * fc.light * ev.light_t *
                                         (fc.area_light ? mis(fc.light_pdf, light_phase) : 1.0f);
+        if (ev.map_pdf > 0.0f) {  // pt.comp:524-527
+            const float phase = phase_hg(u, dot(st.dir, ev.map_dir));
+            st.radiance = st.radiance + path_weight * albedo * phase * ev.map_light * mis(ev.map_pdf, phase);
+        }
         st.throughput = st.throughput * albedo;
         st.volumes++;
         st.origin = ev.idx;

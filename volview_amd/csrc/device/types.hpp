@@ -56,6 +56,12 @@ struct DevScene {
     const uint2* __restrict__ shadow;
     const uint16_t* __restrict__ ao;
     int sdims[3], adims[3];
+    // Light map (Environment::Map; null without one): RGBA16F radiance env_w x
+    // env_h, RG32F sampling table (env_w + 1) x env_h, RGBA16F diffuse 32 x 16.
+    const uint2* __restrict__ env_rad;
+    const float2* __restrict__ env_table;
+    const uint2* __restrict__ env_dif;
+    int env_w, env_h;
 };
 
 // One ray-marched frame of a batched launch: its tiles are workgroups
@@ -100,6 +106,8 @@ struct WfSlots {
     float4* c0;        // chunk sums: radiance, background
     float4* c1;        //             albedo, luminance^2
     float4* c2;        //             normal, depth
+    float4* e_map;     // light-map sample: direction, pdf
+    float4* e_maplt;   // its radiance / pdf x transmittance
     uint16_t* req;     // request of the slot's path (engines over several requests)
 };
 

@@ -173,7 +173,8 @@ Geometry make_geometry(const vvr_volume& v) {
 }
 
 void pack_uniforms(const Geometry& g, const vvr_frame_params& p, const float bucket_max[64], bool has_labels,
-                   uint32_t accumulated, uint32_t width, uint32_t height, uint64_t frame, vvr_uniforms* u) {
+                   uint32_t accumulated, uint32_t width, uint32_t height, uint64_t frame, vvr_uniforms* u,
+                   const float* env_peak_u) {
     std::memset(u, 0, sizeof(*u));
     const vvr_camera& cam = p.camera;
     F3 right, up;
@@ -183,8 +184,9 @@ void pack_uniforms(const Geometry& g, const vvr_frame_params& p, const float buc
     const vvr_lighting& light = p.lighting;
     const F3 ld = light_direction(light, cam);
     const vvr_label_params& labels = p.labels;
-    // No light map can be set: Map falls back to the studio (renderer.rs:1521-1524).
-    const uint32_t environment = light.environment == VVR_ENV_MAP ? uint32_t(VVR_ENV_STUDIO) : light.environment;
+    // Map without a light map set falls back to the studio (renderer.rs:1524-1527).
+    const uint32_t environment =
+        light.environment == VVR_ENV_MAP && !env_peak_u ? uint32_t(VVR_ENV_STUDIO) : light.environment;
 
     std::memcpy(u->world_to_index, g.world_to_index, sizeof(u->world_to_index));
     const F3 f = forward(cam);
@@ -257,7 +259,12 @@ void pack_uniforms(const Geometry& g, const vvr_frame_params& p, const float buc
         u->path[2] = std::max(p.path_tracing.samples_per_frame, 1u);
     }
     u->path[3] = p.skip_empty_space ? 1u : 0u;
-    u->env[0] = 0.0f;  // light-map rotation: no light map
+    // The light map's brightest column turned to the key light's azimuth (renderer.rs:1531-1534).
+    u->env[0] = 0.0f;
+    if (env_peak_u) {
+        const float key_phi = std::atan2(std::sin(light.azimuth), -std::cos(light.azimuth));
+        u->env[0] = (*env_peak_u - 0.5f) * 6.28318530717958647692f - key_phi + light.env_rotation;
+    }
     u->env[1] = p.smooth_sampling ? 1.0f : 0.0f;
     u->env[2] = light.anisotropy;
     u->env[3] = light.chromatic;

@@ -61,8 +61,22 @@ struct OpacityKey {
 OpacityKey opacity_key(const vvr_label_params& l, uint64_t generation, bool has_labels, float min_spacing);
 
 // The uniform fill of Renderer::record (renderer.rs:1516-1660).
+// env_peak_u: the light map's brightest column (environment.rs:113-125), null without a map.
 void pack_uniforms(const Geometry& g, const vvr_frame_params& p, const float bucket_max[64], bool has_labels,
-                   uint32_t accumulated, uint32_t width, uint32_t height, uint64_t frame, vvr_uniforms* u);
+                   uint32_t accumulated, uint32_t width, uint32_t height, uint64_t frame, vvr_uniforms* u,
+                   const float* env_peak_u = nullptr);
+
+// EnvironmentMap::prepare and upload_environment (environment.rs:56-180,
+// renderer.rs:2224-2290): a width x height linear RGB light map, rows from the
+// zenith, as the shaders read it.
+struct EnvTables {
+    uint32_t width = 0, height = 0;
+    std::vector<uint16_t> radiance;  // RGBA16F bits, mean radiance 1 over the sphere
+    std::vector<float> table;        // (width + 1) x height RG: conditional CDF + texel probability; marginal
+    std::vector<uint16_t> diffuse;   // 32 x 16 RGBA16F bits: cosine-weighted mean radiance
+    float peak_u = 0.0f;             // image column of the brightest region
+};
+EnvTables prepare_environment(uint32_t width, uint32_t height, const float* rgb);
 
 // Seeded synthetic phantoms.
 void phantom_ct(const uint32_t dims[3], uint64_t seed, int16_t* hu);

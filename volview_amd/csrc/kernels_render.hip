@@ -120,7 +120,11 @@ __device__ __forceinline__ void dvr_pixel(const DevScene& sc, uint32_t px, uint3
             const V3 surface = (specular + diffuse) * (1.0f - s.subsurface) * nl * shadow + diffuse * s.subsurface * diffused;
             const V3 volume = s.albedo / (4.0f * kPi) * vmix(V3(shadow), diffused, s.subsurface);
             const V3 direct = vmix(volume, surface, surfaceness) * light;
-            const V3 ambient = s.albedo * ambient_light * ao;
+            // dvr.comp:135-139: a light map's diffuse response where surfaces face it.
+            const V3 ambient_env = u->lighting[3] == 2u
+                                       ? vmix(ambient_light, env_map_diffuse(sc, n), surfaceness * (1.0f - s.subsurface))
+                                       : ambient_light;
+            const V3 ambient = s.albedo * ambient_env * ao;
             radiance = radiance + transmittance * alpha * (direct + ambient);
             transmittance *= 1.0f - alpha;
         }

@@ -550,8 +550,16 @@ __global__ void __launch_bounds__(64) k_wf_shadow(const PtRequest* __restrict__ 
         EventState ev;
         ev.idx = xyz(S.e_idx[slot]);
         ev.light_dir = xyz(S.e_light[slot]);
-        for_each_request<kBatched>(true, req, [&](uint32_t r) { stage_shadow(reqs[r].scene, st, ev); });
+        bool map = false;
+        for_each_request<kBatched>(true, req, [&](uint32_t r) {
+            stage_shadow(reqs[r].scene, st, ev);
+            map = reqs[r].scene.u->lighting[3] == 2u;
+        });
         S.e_lt[slot] = f4(ev.light_t, 0.0f);
+        if (map) {
+            S.e_map[slot] = f4(ev.map_dir, ev.map_pdf);
+            S.e_maplt[slot] = f4(ev.map_light, 0.0f);
+        }
         S.p_dir[slot].w = bitsf(st.rng.state);
     }
 }
@@ -580,6 +588,11 @@ __global__ void __launch_bounds__(64) k_wf_scatter(const PtRequest* __restrict__
         for_each_request<kBatched>(active, ref.req, [&](uint32_t r) {
             const PtRequest& R = reqs[r];
             const FrameConsts fc = frame_consts(R.scene.u);
+            ev.map_pdf = 0.0f;
+            if (fc.env_map) {
+                const float4 m = S.e_map[slot], ml = S.e_maplt[slot];
+                ev.map_dir = xyz(m), ev.map_pdf = m.w, ev.map_light = xyz(ml);
+            }
             more = stage_scatter(R.scene, fc, st, ev) && path_continues(fc, st);
         });
         if (active) store_path(S, slot, st, ref);

@@ -144,7 +144,8 @@ void record(vvr_renderer* r, uint64_t frame, const vvr_frame_params* p) {
     }
     const int slot = int(frame % 2);
     vvr_uniforms u;
-    pack_uniforms(r->geo, *p, r->tf.bucket_max, r->has_labels, r->accumulated, r->width, r->height, frame, &u);
+    pack_uniforms(r->geo, *p, r->tf.bucket_max, r->has_labels, r->accumulated, r->width, r->height, frame, &u,
+                  r->env_peak());
     const vvr_uniforms* du = r->upload_uniforms(u, slot);
     r->prepare(*p, u, du, r->stream);
     const DevScene sc = r->scene(du);
@@ -228,6 +229,46 @@ int vvr_renderer_set_transfer_function(vvr_renderer* r, const vvr_tf* tf) {
         check(hipStreamSynchronize(r->stream), "device wait");
         r->upload_tf(*tf);
         r->generation++;
+    });
+}
+
+int vvr_renderer_set_environment_map(vvr_renderer* r, const vvr_env_map* map) {
+    return guarded([&] {
+        if (!r) throw Error{VVR_EINVAL, "null argument"};
+        if (map && (!map->rgb || map->width < 2 || map->height < 1 || map->width > 16384 || map->height > 8192))
+            throw Error{VVR_EINVAL, "light map must be 2..16384 x 1..8192 RGB texels"};
+        r->use_device();
+        check(hipStreamSynchronize(r->stream), "device wait");
+        dfree(r->env_rad), dfree(r->env_table), dfree(r->env_dif);
+        r->env_rad = nullptr, r->env_table = nullptr, r->env_dif = nullptr;
+        r->has_env = false;
+        r->env_w = r->env_h = 0;
+        r->acc_valid = false;  // renderer.rs:1299: accumulation restarts
+        if (!map) return;
+        const EnvTables t = prepare_environment(map->width, map->height, map->rgb);
+        r->env_rad = dalloc<uint2>(size_t(t.width) * t.height, "light map");
+        r->env_table = dalloc<float2>(size_t(t.width + 1) * t.height, "light map sampling");
+        r->env_dif = dalloc<uint2>(32 * 16, "light map diffuse");
+        check(hipMemcpy(r->env_rad, t.radiance.data(), t.radiance.size() * 2, hipMemcpyHostToDevice), "upload");
+        check(hipMemcpy(r->env_table, t.table.data(), t.table.size() * 4, hipMemcpyHostToDevice), "upload");
+        check(hipMemcpy(r->env_dif, t.diffuse.data(), t.diffuse.size() * 2, hipMemcpyHostToDevice), "upload");
+        r->env_w = int(t.width), r->env_h = int(t.height);
+        r->env_peak_u = t.peak_u;
+        r->has_env = true;
+    });
+}
+
+int vvr_renderer_read_environment(vvr_renderer* r, float* peak_u, float* table, size_t table_floats) {
+    return guarded([&] {
+        if (!r || !peak_u) throw Error{VVR_EINVAL, "null argument"};
+        if (!r->has_env) throw Error{VVR_ESTATE, "no light map is set"};
+        *peak_u = r->env_peak_u;
+        if (table) {
+            const size_t n = 2 * size_t(r->env_w + 1) * r->env_h;
+            if (table_floats != n) throw Error{VVR_EINVAL, "sampling table size mismatch"};
+            r->use_device();
+            check(hipMemcpy(table, r->env_table, n * 4, hipMemcpyDeviceToHost), "read-back");
+        }
     });
 }
 

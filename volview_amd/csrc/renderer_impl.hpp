@@ -87,13 +87,14 @@ struct WfEngine {
         if (n <= slots) return;
         dfree(slot_mem);
         slot_mem = nullptr, slots = 0;
-        constexpr int kF4 = 15, kQueues = 8;
+        constexpr int kF4 = 17, kQueues = 8;
         const size_t f4_bytes = size_t(n) * sizeof(float4), q_bytes = (size_t(n) * 4 + 255) / 256 * 256;
         const size_t req_bytes = (size_t(n) * 2 + 255) / 256 * 256, bin_bytes = 4 * kWfBins * 4;
         slot_mem = dalloc<uint8_t>(kF4 * f4_bytes + kQueues * q_bytes + req_bytes + 256 + bin_bytes, "path slots");
         uint8_t* p = slot_mem;
-        float4** f4s[kF4] = {&S.p_origin, &S.p_dir, &S.p_thr, &S.p_spec, &S.p_rad, &S.e_idx, &S.e_s0, &S.e_s1,
-                             &S.e_s2,     &S.e_n,   &S.e_light, &S.e_lt, &S.c0,   &S.c1,   &S.c2};
+        float4** f4s[kF4] = {&S.p_origin, &S.p_dir, &S.p_thr, &S.p_spec, &S.p_rad, &S.e_idx,
+                             &S.e_s0,     &S.e_s1,  &S.e_s2,  &S.e_n,    &S.e_light, &S.e_lt,
+                             &S.c0,       &S.c1,    &S.c2,    &S.e_map,  &S.e_maplt};
         for (auto f : f4s) *f = reinterpret_cast<float4*>(p), p += f4_bytes;
         uint32_t** qs[kQueues] = {&Q.ff[0],   &Q.ff[1],   &Q.ev,      &Q.end[0],
                                   &Q.end[1], &Q.idle[0], &Q.idle[1], &Q.sorted};
@@ -260,6 +261,14 @@ struct vvr_renderer {
     uint2* shadow = nullptr;   // RGBA16F at ceil(dims / 2)
     uint16_t* ao = nullptr;    // R16F at ceil(dims / 4)
     std::optional<LightKey> shadow_key, ao_key;
+    // Light map of Environment::Map (set_environment_map), absent by default.
+    uint2* env_rad = nullptr;
+    float2* env_table = nullptr;
+    uint2* env_dif = nullptr;
+    int env_w = 0, env_h = 0;
+    bool has_env = false;
+    float env_peak_u = 0.0f;
+    const float* env_peak() const { return has_env ? &env_peak_u : nullptr; }
 
     uint32_t width = 0, height = 0;
     uint16_t* hdr = nullptr;
@@ -382,6 +391,8 @@ struct vvr_renderer {
         s.shadow = shadow;
         s.ao = ao;
         for (int a = 0; a < 3; a++) s.sdims[a] = shadow_dims(a), s.adims[a] = ao_dims(a);
+        s.env_rad = env_rad, s.env_table = env_table, s.env_dif = env_dif;
+        s.env_w = env_w, s.env_h = env_h;
         return s;
     }
 
@@ -454,7 +465,7 @@ struct vvr_renderer {
         free_frame();
         wf.reset();
         dfree(density), dfree(normals), dfree(majorant), dfree(labels), dfree(mask), dfree(mask_b), dfree(soft);
-        dfree(shadow), dfree(ao);
+        dfree(shadow), dfree(ao), dfree(env_rad), dfree(env_table), dfree(env_dif);
         dfree(lut), dfree(preint), dfree(d_uniforms[0]), dfree(d_uniforms[1]), dfree(queue), dfree(exposure);
         if (h_uniforms) (void)hipHostFree(h_uniforms);
         for (auto& e : slot_done)

@@ -236,7 +236,8 @@ void vvr_server::admit(std::vector<Job*>& fresh_dvr) {
         if (j->pt && (entries_in_use >= kTable || n_active >= uint32_t(kWfMaxActive))) continue;
         vvr_renderer* v = j->vol;
         vvr_uniforms packed;
-        pack_uniforms(v->geo, j->params, v->tf.bucket_max, v->has_labels, 0, j->width, j->height, j->id, &packed);
+        pack_uniforms(v->geo, j->params, v->tf.bucket_max, v->has_labels, 0, j->width, j->height, j->id, &packed,
+                      v->env_peak());
         // A derived volume is rebuilt only while no admitted request reads it:
         // masks and majorant wait for every reader of the volume, the lighting
         // volumes for the ray-marched requests that read them.

@@ -10,7 +10,7 @@ from ctypes import POINTER, byref, c_double, c_float, c_int16, c_uint8, c_uint16
 import numpy as np
 
 from . import _abi
-from ._abi import (Camera, ControlPoint, FrameParams, LabelParams, Lighting, PathTracing, SoftGrid, TF, Uniforms,
+from ._abi import (Camera, ControlPoint, EnvMap, FrameParams, LabelParams, Lighting, PathTracing, SoftGrid, TF, Uniforms,
                    Volume, check, load)
 
 
@@ -201,6 +201,27 @@ class Renderer:
         out = np.zeros((self.height, self.width, 4), np.float32)
         check(self._lib.vvr_renderer_read_hdr(self.handle, _fptr(out)))
         return out
+
+    def set_environment_map(self, rgb):
+        """Renderer::set_environment_map: an (h, w, 3) linear RGB light map, rows
+        from the zenith, or None to remove it."""
+        if rgb is None:
+            check(self._lib.vvr_renderer_set_environment_map(self.handle, None))
+            self._env = None
+            return
+        a = np.ascontiguousarray(rgb, np.float32)
+        m = EnvMap(a.shape[1], a.shape[0], a.ctypes.data_as(POINTER(c_float)))
+        check(self._lib.vvr_renderer_set_environment_map(self.handle, byref(m)))
+        self._env = a.shape[:2]
+
+    def read_environment(self):
+        """(peak_u, sampling table (h, w + 1, 2)) of the set light map."""
+        h, w = self._env
+        table = np.zeros((h, w + 1, 2), np.float32)
+        peak = c_float()
+        check(self._lib.vvr_renderer_read_environment(self.handle, byref(peak), table.ctypes.data_as(POINTER(c_float)),
+                                                       table.size))
+        return peak.value, table
 
     def read_rgba(self) -> np.ndarray:
         """The tone-mapped RGBA8 display image of the last record()."""
