@@ -320,7 +320,7 @@ __device__ inline bool free_flight(const DevScene& sc, V3 origin, V3 dir, bool s
 }
 
 // Ratio tracking with Russian roulette (pt.comp:339-371).
-__device__ inline V3 transmittance(const DevScene& sc, V3 origin, V3 dir, Rng& rng) {
+__device__ __forceinline__ V3 transmittance(const DevScene& sc, V3 origin, V3 dir, Rng& rng) {
     const vvr_uniforms* u = sc.u;
     Flight f = flight_begin(u, origin, dir);
     V3 result(1.0f);
@@ -486,17 +486,21 @@ __device__ __forceinline__ void stage_shade(const DevScene& sc, const FrameConst
     ev.light_dir = sample_light(u, fc.light_frame, st.rng);
 }
 
-// pt.comp:466-476: the shadow ray, then with a light map a light-map sample and
-// its shadow ray.
+// pt.comp:466: the shadow ray.
 __device__ __forceinline__ void stage_shadow(const DevScene& sc, PathState& st, EventState& ev) {
     ev.light_t = transmittance(sc, ev.idx, ev.light_dir, st.rng);
+}
+
+// pt.comp:467-476: with a light map, a light-map sample and its shadow ray
+// (after the key light's, in the stream's order). A stage of its own so frames
+// without a map do not carry its registers.
+__device__ __forceinline__ void stage_shadow_map(const DevScene& sc, PathState& st, EventState& ev) {
     ev.map_pdf = 0.0f;
     ev.map_dir = ev.map_light = V3(0.0f);
-    if (sc.u->lighting[3] == 2u) {
-        ev.map_dir = sample_env(sc, st.rng, ev.map_pdf);
-        ev.map_light = env_map_radiance(sc, ev.map_dir) / gmax(ev.map_pdf, 1e-8f);
-        if (ev.map_pdf > 0.0f) ev.map_light = ev.map_light * transmittance(sc, ev.idx, ev.map_dir, st.rng);
-    }
+    if (sc.u->lighting[3] != 2u) return;
+    ev.map_dir = sample_env(sc, st.rng, ev.map_pdf);
+    ev.map_light = env_map_radiance(sc, ev.map_dir) / gmax(ev.map_pdf, 1e-8f);
+    if (ev.map_pdf > 0.0f) ev.map_light = ev.map_light * transmittance(sc, ev.idx, ev.map_dir, st.rng);
 }
 
 // pt.comp:478-544: next-event estimate, the next direction, Russian roulette.
@@ -573,6 +577,7 @@ __device__ __forceinline__ bool path_bounce(const DevScene& sc, const FrameConst
     if (!stage_extend(sc, fc, st, ev)) return false;
     stage_shade(sc, fc, st, ev);
     stage_shadow(sc, st, ev);
+    stage_shadow_map(sc, st, ev);
     return stage_scatter(sc, fc, st, ev);
 }
 

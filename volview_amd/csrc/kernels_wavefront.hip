@@ -550,17 +550,34 @@ __global__ void __launch_bounds__(64) k_wf_shadow(const PtRequest* __restrict__ 
         EventState ev;
         ev.idx = xyz(S.e_idx[slot]);
         ev.light_dir = xyz(S.e_light[slot]);
+        for_each_request<kBatched>(true, req, [&](uint32_t r) { stage_shadow(reqs[r].scene, st, ev); });
+        S.e_lt[slot] = f4(ev.light_t, 0.0f);
+        S.p_dir[slot].w = bitsf(st.rng.state);
+    }
+}
+
+// Stage 3b, for frames with a light map: its sample and shadow ray.
+template <bool kBatched>
+__global__ void __launch_bounds__(64) k_wf_shadow_map(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
+                                                      const uint32_t* __restrict__ in) {
+    const uint32_t n = Q.counts[kWfEV];
+    for (uint32_t q = blockIdx.x * 64u + __lane_id(); q < n; q += gridDim.x * 64u) {
+        const uint32_t slot = in[q];
+        const uint32_t req = fbits(S.p_rad[slot].w) >> 16;
+        PathState st;
+        st.rng.state = fbits(S.p_dir[slot].w);
+        EventState ev;
+        ev.idx = xyz(S.e_idx[slot]);
         bool map = false;
         for_each_request<kBatched>(true, req, [&](uint32_t r) {
-            stage_shadow(reqs[r].scene, st, ev);
             map = reqs[r].scene.u->lighting[3] == 2u;
+            if (map) stage_shadow_map(reqs[r].scene, st, ev);
         });
-        S.e_lt[slot] = f4(ev.light_t, 0.0f);
         if (map) {
             S.e_map[slot] = f4(ev.map_dir, ev.map_pdf);
             S.e_maplt[slot] = f4(ev.map_light, 0.0f);
+            S.p_dir[slot].w = bitsf(st.rng.state);
         }
-        S.p_dir[slot].w = bitsf(st.rng.state);
     }
 }
 
@@ -664,7 +681,7 @@ namespace {
 // each queue binned by request first.
 template <bool kBatched>
 void wf_stages(const PtRequest* reqs, const WfSlots& S, const WfQueues& Q, uint32_t cur, const WfGrid& g, bool sort,
-               hipStream_t st) {
+               bool env_map, hipStream_t st) {
     const uint32_t* ff = Q.ff[cur];
     const uint32_t* ev = Q.ev;
     if (sort) {
@@ -684,19 +701,20 @@ void wf_stages(const PtRequest* reqs, const WfSlots& S, const WfQueues& Q, uint3
     }
     hipLaunchKernelGGL(k_wf_shade<kBatched>, dim3(g.shade), dim3(64), 0, st, reqs, S, Q, ev);
     hipLaunchKernelGGL(k_wf_shadow<kBatched>, dim3(g.shadow), dim3(64), 0, st, reqs, S, Q, ev);
+    if (env_map) hipLaunchKernelGGL(k_wf_shadow_map<kBatched>, dim3(g.shadow), dim3(64), 0, st, reqs, S, Q, ev);
     hipLaunchKernelGGL(k_wf_scatter<kBatched>, dim3(g.scatter), dim3(64), 0, st, reqs, S, Q, ev, cur);
 }
 
 }  // namespace
 
 void wf_iteration(const PtRequest* reqs, uint32_t n_reqs, uint32_t total_items, const WfSlots& S, const WfQueues& Q,
-                  uint32_t cur, const WfGrid& g, hipStream_t st) {
+                  uint32_t cur, const WfGrid& g, bool env_map, hipStream_t st) {
     if (n_reqs > 1) {
         hipLaunchKernelGGL(k_wf_regen<true>, dim3(g.regen), dim3(64), 0, st, reqs, n_reqs, total_items, S, Q, cur);
-        wf_stages<true>(reqs, S, Q, cur, g, n_reqs <= kWfBins, st);
+        wf_stages<true>(reqs, S, Q, cur, g, n_reqs <= kWfBins, env_map, st);
     } else {
         hipLaunchKernelGGL(k_wf_regen<false>, dim3(g.regen), dim3(64), 0, st, reqs, n_reqs, total_items, S, Q, cur);
-        wf_stages<false>(reqs, S, Q, cur, g, false, st);
+        wf_stages<false>(reqs, S, Q, cur, g, false, env_map, st);
     }
 }
 
@@ -706,9 +724,9 @@ void wf_init_served(const WfSlots& S, const WfQueues& Q, uint32_t slots, hipStre
 }
 
 void wf_iteration_served(const PtRequest* reqs, const WfActive& A, const WfServed& V, const WfSlots& S,
-                         const WfQueues& Q, uint32_t cur, const WfGrid& g, hipStream_t st) {
+                         const WfQueues& Q, uint32_t cur, const WfGrid& g, bool env_map, hipStream_t st) {
     hipLaunchKernelGGL(k_wf_regen_served, dim3(g.regen), dim3(64), 0, st, reqs, A, V, S, Q, cur);
-    wf_stages<true>(reqs, S, Q, cur, g, true, st);
+    wf_stages<true>(reqs, S, Q, cur, g, true, env_map, st);
 }
 
 void wf_resolve(const PtRequest& r, hipStream_t st) {

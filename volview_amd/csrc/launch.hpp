@@ -77,13 +77,14 @@ struct WfGrid {
 WfGrid wf_grid(int cus);
 // All `slots` slots empty on the end queue of iteration 0, item queue at 0.
 void wf_init(const WfSlots& S, const WfQueues& Q, uint32_t slots, hipStream_t st);
+// env_map: a request in flight renders with a light map (adds its stage).
 void wf_iteration(const PtRequest* reqs, uint32_t n_reqs, uint32_t total_items, const WfSlots& S, const WfQueues& Q,
-                  uint32_t cur, const WfGrid& g, hipStream_t st);
+                  uint32_t cur, const WfGrid& g, bool env_map, hipStream_t st);
 // Served engine: all slots idle; one iteration drawing work from the active
 // requests of the table `reqs` (see k_wf_regen_served).
 void wf_init_served(const WfSlots& S, const WfQueues& Q, uint32_t slots, hipStream_t st);
 void wf_iteration_served(const PtRequest* reqs, const WfActive& A, const WfServed& V, const WfSlots& S,
-                         const WfQueues& Q, uint32_t cur, const WfGrid& g, hipStream_t st);
+                         const WfQueues& Q, uint32_t cur, const WfGrid& g, bool env_map, hipStream_t st);
 // Chunk sums -> accumulation, guides and HDR images of one request.
 void wf_resolve(const PtRequest& r, hipStream_t st);
 

@@ -158,7 +158,7 @@ void record(vvr_renderer* r, uint64_t frame, const vvr_frame_params* p) {
             q.scene = sc;
             q.accum = r->accum, q.albedo = r->albedo, q.normal = r->normal, q.hdr = r->hdr;
             WfEngine::shape(q, r->width, r->height, first, spf, r->pt_chunk);
-            r->timed(VVR_KERNEL_PT, [&] { r->wf->run(&q, 1, r->stream); });
+            r->timed(VVR_KERNEL_PT, [&] { r->wf->run(&q, 1, r->stream, u.lighting[3] == 2u); });
         } else if (r->pt_kernel == vvr_renderer::kSimple) {
             r->timed(VVR_KERNEL_PT, [&] {
                 launch_pt(sc, r->accum, r->albedo, r->normal, r->hdr, r->width, r->height, first, spf, r->stream);

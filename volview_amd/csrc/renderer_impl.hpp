@@ -136,7 +136,8 @@ struct WfEngine {
 
     // Renders the requests (scene, images and shape set) on `st`. Returns once
     // every path is traced or queued to be; the resolves are queued last.
-    void run(const PtRequest* reqs, uint32_t n, hipStream_t st) {
+    // env_map: some request renders with a light map.
+    void run(const PtRequest* reqs, uint32_t n, hipStream_t st, bool env_map = false) {
         if (n == 0) return;
         size_t partial_f4 = 0;
         uint64_t items = 0;
@@ -165,7 +166,7 @@ struct WfEngine {
         constexpr uint32_t kBatch = 4;
         uint32_t iter = 0;
         auto launch_batch = [&](int k) {
-            for (uint32_t j = 0; j < kBatch; j++, iter++) wf_iteration(d_reqs, n, base, S, Q, iter & 1u, grid, st);
+            for (uint32_t j = 0; j < kBatch; j++, iter++) wf_iteration(d_reqs, n, base, S, Q, iter & 1u, grid, env_map, st);
             check(hipGetLastError(), "wavefront iteration");
             check(hipMemcpyAsync(&h_live[k], Q.counts + kWfLIVE, 4, hipMemcpyDeviceToHost, st), "live count");
             check(hipEventRecord(live_ev[k], st), "hipEventRecord");
