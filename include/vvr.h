@@ -308,6 +308,41 @@ int vvr_renderer_set_environment_map(vvr_renderer* r, const vvr_env_map* map);
  * checks, its (width + 1) x height x 2 sampling table (table may be NULL). */
 int vvr_renderer_read_environment(vvr_renderer* r, float* peak_u, float* table, size_t table_floats);
 
+/* ---- the learned denoiser --------------------------------------------------------
+ * No counterpart in the reference (which calls Intel OIDN there, renderer.rs:351-357):
+ * a convolutional U-Net over the path tracer's accumulation and guide images,
+ * trained on the GPU by this library's bf16 MFMA kernels; the a-trous filter is
+ * its fallback and comparison point. */
+typedef struct vvr_unet vvr_unet;
+typedef struct vvr_unet_config {
+    uint32_t levels;       /* 2..5 resolution levels */
+    uint32_t channels[5];  /* per level, multiples of 16 */
+    uint32_t seed;         /* of the He-normal initial weights */
+} vvr_unet_config;
+int vvr_unet_create(vvr_ctx* ctx, const vvr_unet_config* config, vvr_unet** out);
+void vvr_unet_destroy(vvr_unet* net);
+/* Parameters, FP32, per layer W[cout][3x3][cin] then b[cout], layers in the order
+ * encoder (two per level), decoder (two per level, deepest first), output. */
+size_t vvr_unet_num_params(const vvr_unet* net);
+int vvr_unet_get_params(vvr_unet* net, float* out, size_t count);
+int vvr_unet_set_params(vvr_unet* net, const float* params, size_t count);
+/* The gradients of the last train_step's loss, in the parameter order. */
+int vvr_unet_get_grads(vvr_unet* net, float* out, size_t count);
+/* One step on `batch` frames of width x height (multiples of 2^(levels-1)):
+ * host float4 images of `samples` paths (accumulation, albedo and normal guides,
+ * as vvr_renderer_read_accum returns them) and the reference radiance (float4,
+ * rgb used). Computes the L1 loss of the log irradiance and the gradients; with
+ * lr > 0 takes an Adam step. */
+int vvr_unet_train_step(vvr_unet* net, uint32_t batch, uint32_t width, uint32_t height, const float* accum,
+                        const float* albedo, const float* normal, float samples, const float* target, float lr,
+                        float* loss);
+/* The denoised frame (rgba floats: radiance, background share) of host images. */
+int vvr_unet_denoise(vvr_unet* net, uint32_t width, uint32_t height, const float* accum, const float* albedo,
+                     const float* normal, float samples, float* rgba);
+/* Frames recorded with VVR_DENOISE_LEARNED run `net` (owned by the caller, kept
+ * alive while attached; NULL detaches). */
+int vvr_renderer_set_denoiser(vvr_renderer* r, vvr_unet* net);
+
 /* ---- measurement ------------------------------------------------------------ */
 
 enum { VVR_KERNEL_MAJORANT = 0, VVR_KERNEL_NORMALS = 1, VVR_KERNEL_MASK = 2, VVR_KERNEL_DVR = 3, VVR_KERNEL_PT = 4,

@@ -119,6 +119,10 @@ class ServerStats(Structure):
                 ("iterations", c_uint64), ("hbm_in_use", c_uint64), ("hbm_peak", c_uint64)]
 
 
+class UnetConfig(Structure):
+    _fields_ = [("levels", c_uint32), ("channels", c_uint32 * 5), ("seed", c_uint32)]
+
+
 class EnvMap(Structure):
     _fields_ = [("width", c_uint32), ("height", c_uint32), ("rgb", POINTER(c_float))]
 
@@ -154,6 +158,17 @@ SIGNATURES = [
     ("vvr_renderer_read_denoised", c_int, [c_void_p, POINTER(c_float)]),
     ("vvr_renderer_read_rgba", c_int, [c_void_p, POINTER(c_uint8)]),
     ("vvr_renderer_set_environment_map", c_int, [c_void_p, POINTER(EnvMap)]),
+    ("vvr_unet_create", c_int, [c_void_p, POINTER(UnetConfig), POINTER(c_void_p)]),
+    ("vvr_unet_destroy", None, [c_void_p]),
+    ("vvr_unet_num_params", c_size_t, [c_void_p]),
+    ("vvr_unet_get_params", c_int, [c_void_p, POINTER(c_float), c_size_t]),
+    ("vvr_unet_set_params", c_int, [c_void_p, POINTER(c_float), c_size_t]),
+    ("vvr_unet_get_grads", c_int, [c_void_p, POINTER(c_float), c_size_t]),
+    ("vvr_unet_train_step", c_int, [c_void_p, c_uint32, c_uint32, c_uint32, POINTER(c_float), POINTER(c_float),
+                                    POINTER(c_float), c_float, POINTER(c_float), c_float, POINTER(c_float)]),
+    ("vvr_unet_denoise", c_int, [c_void_p, c_uint32, c_uint32, POINTER(c_float), POINTER(c_float), POINTER(c_float),
+                                 c_float, POINTER(c_float)]),
+    ("vvr_renderer_set_denoiser", c_int, [c_void_p, c_void_p]),
     ("vvr_renderer_read_environment", c_int, [c_void_p, POINTER(c_float), POINTER(c_float), c_size_t]),
     ("vvr_renderer_read_exposure", c_int, [c_void_p, POINTER(c_float)]),
     ("vvr_render_to_rgba", c_int, [c_void_p, POINTER(FrameParams), c_uint32, c_uint32, c_uint32, POINTER(c_uint8)]),

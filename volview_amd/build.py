@@ -24,8 +24,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 CXX = os.environ.get("CXX", "g++")
 
 HIP_SOURCES = ["kernels_volume.hip", "kernels_render.hip", "kernels_wavefront.hip", "kernels_lighting.hip",
-               "kernels_denoise.hip", "kernels_post.hip"]
-HOST_SOURCES = ["transfer.cpp", "frame.cpp", "phantom.cpp", "renderer.cpp", "server.cpp", "environment.cpp"]
+               "kernels_denoise.hip", "kernels_post.hip", "kernels_unet.hip"]
+HOST_SOURCES = ["transfer.cpp", "frame.cpp", "phantom.cpp", "renderer.cpp", "server.cpp", "environment.cpp", "unet.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", f"-I{PKG.parent / 'include'}"]
 DEVICE = [

@@ -1,0 +1,385 @@
+// The learned denoiser's convolutional U-Net: bf16 MFMA forward and backward
+// kernels with FP32 accumulation, and the elementwise kernels around them.
+//
+// Activations are NHWC bf16 (channels fastest, counts multiples of 16); a
+// tensor argument is (pointer, channel stride, channel offset) so that
+// concatenations are channel slices of one buffer. Weights of a 3x3 layer are
+// bf16 [co_pad][9][ci] (co_pad = co rounded up to 32, zero rows), taps in
+// row-major order of (dy, dx) in {-1, 0, 1}^2; zero padding at image borders.
+//
+//   k_conv3x3     out = act(conv(in) + bias): implicit GEMM, M = pixels, N = out
+//                 channels, K = 9 x in channels, one v_mfma_f32_32x32x16_bf16 per
+//                 16 K. A wave owns 32 consecutive pixels and up to 4 blocks of
+//                 32 output channels; the A fragment (8 channels of one pixel per
+//                 lane) is loaded once per K step and reused across the blocks.
+//                 Also the input gradient, with flipped transposed weights and
+//                 the mask of a ReLU'd input fused into the epilogue.
+//   k_conv_wgrad  dW[co][tap][ci] += sum over pixels dY[p][co] x[p + tap][ci],
+//                 db[co] += sum dY: M = out channels, N = in channels, K = pixels.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "launch.hpp"
+
+namespace vvr {
+
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
+    const uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return uint16_t((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    return uint16_t((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+__device__ __forceinline__ bf16x8 load8(const uint16_t* p) {
+    const uint4 v = *reinterpret_cast<const uint4*>(p);
+    return __builtin_bit_cast(bf16x8, v);
+}
+__device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u)); }
+
+// Row of MFMA output register r for lane-half hi (0 or 1): D[8(r/4) + 4 hi + r%4][lane % 32].
+__device__ __forceinline__ int drow(int r, int hi) { return 8 * (r >> 2) + 4 * hi + (r & 3); }
+
+template <int NCOB>
+__global__ void __launch_bounds__(256) k_conv3x3(UnetConv c) {
+    const int lane = int(threadIdx.x & 63u), wave = int(threadIdx.x >> 6);
+    const int col = lane & 31, hi = lane >> 5;
+    const int npix = c.n * c.h * c.w;
+    const int p = int(blockIdx.x) * 128 + wave * 32 + col;  // this lane's A row (pixel)
+    const int cob0 = int(blockIdx.y) * NCOB;                 // first 32-channel output block
+    int px = 0, py = 0, pn = 0;
+    if (p < npix) {
+        px = p % c.w;
+        py = (p / c.w) % c.h;
+        pn = p / (c.w * c.h);
+    }
+    f32x16 acc[NCOB];
+#pragma unroll
+    for (int b = 0; b < NCOB; b++) acc[b] = f32x16{};
+    const int nb = min(NCOB, c.co_blocks - cob0);
+    for (int tap = 0; tap < 9; tap++) {
+        const int yy = py + tap / 3 - 1, xx = px + tap % 3 - 1;
+        const bool inside = p < npix && yy >= 0 && yy < c.h && xx >= 0 && xx < c.w;
+        const uint16_t* src = c.in + (size_t((pn * c.h + yy) * c.w + xx)) * c.in_cs + c.in_co + 8 * hi;
+        for (int ci = 0; ci < c.cin; ci += 16) {
+            const bf16x8 a = inside ? load8(src + ci) : zero8();
+#pragma unroll
+            for (int b = 0; b < NCOB; b++) {
+                if (b < nb) {
+                    const int co = (cob0 + b) * 32 + col;
+                    const bf16x8 wv = load8(c.wts + (size_t(co) * 9 + tap) * c.cin + ci + 8 * hi);
+                    acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, wv, acc[b], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // Epilogue: lane holds output channel (cob0 + b) * 32 + col of 16 pixels.
+    const int pbase = int(blockIdx.x) * 128 + wave * 32;
+#pragma unroll
+    for (int b = 0; b < NCOB; b++) {
+        if (b >= nb) continue;
+        const int co = (cob0 + b) * 32 + col;
+        if (co >= c.cout) continue;
+        const float bias = c.bias ? c.bias[co] : 0.0f;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int q = pbase + drow(r, hi);
+            if (q >= npix) continue;
+            float v = acc[b][r] + bias;
+            if (c.relu) v = fmaxf(v, 0.0f);
+            if (c.mask && bf2f(c.mask[size_t(q) * c.mask_cs + c.mask_co + co]) <= 0.0f) v = 0.0f;
+            if (c.out) c.out[size_t(q) * c.out_cs + c.out_co + co] = f2bf(v);
+            if (c.out_f32) c.out_f32[size_t(q) * c.cout + co] = v;
+        }
+    }
+}
+
+// Weight gradient. One wave per (tap, 32 output channels, 32 input channels)
+// and chunk of pixels; K = 16 pixels per MFMA, A = dY^T (lane: output channel,
+// 8 consecutive pixels), B = shifted x (lane: input channel, 8 pixels).
+__global__ void __launch_bounds__(64) k_conv_wgrad(UnetWgrad g) {
+    const int lane = int(threadIdx.x), col = lane & 31, hi = lane >> 5;
+    const int job = int(blockIdx.y);  // (tap, co block, ci block)
+    const int tap = job % 9, cob = (job / 9) % g.co_blocks, cib = job / (9 * g.co_blocks);
+    const int npix = g.n * g.h * g.w;
+    const int p0 = int(blockIdx.x) * g.chunk, p1 = min(p0 + g.chunk, npix);
+    const int co = cob * 32 + col, ci = cib * 32 + col;
+    const bool co_ok = co < g.cout, ci_ok = ci < g.cin;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    f32x16 acc{};
+    float bsum = 0.0f;
+    for (int k = p0; k < p1; k += 16) {
+        bf16x8 a, b;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int p = k + 8 * hi + j;
+            uint16_t va = 0, vb = 0;
+            if (p < p1) {
+                if (co_ok) va = g.dy[size_t(p) * g.dy_cs + co];
+                const int x = p % g.w, y = (p / g.w) % g.h, n = p / (g.w * g.h);
+                const int xx = x + dx, yy = y + dy;
+                if (ci_ok && xx >= 0 && xx < g.w && yy >= 0 && yy < g.h)
+                    vb = g.x[size_t((n * g.h + yy) * g.w + xx) * g.x_cs + g.x_co + ci];
+            }
+            a[j] = __builtin_bit_cast(__bf16, va);
+            b[j] = __builtin_bit_cast(__bf16, vb);
+            if (tap == 0 && cib == 0) bsum += bf2f(va);
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    // D[row = out channel][col = in channel].
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int o = cob * 32 + drow(r, hi);
+        if (o < g.cout && ci_ok) atomicAdd(&g.dw[(size_t(o) * 9 + tap) * g.cin + ci], acc[r]);
+    }
+    if (tap == 0 && cib == 0 && co_ok) {
+        bsum += __shfl_xor(bsum, 32);
+        if (hi == 0) atomicAdd(&g.db[co], bsum);
+    }
+}
+
+// Flipped transposed weights for the input gradient: wt[ci][8 - tap][co] = w[co][tap][ci],
+// with ci padded to 32 rows and co to the stored count.
+__global__ void k_wtrans(const uint16_t* __restrict__ w, int cout, int cin, uint16_t* __restrict__ wt, int cout_pad16,
+                         int cin_pad32) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= cin_pad32 * 9 * cout_pad16) return;
+    const int co = i % cout_pad16, tap = (i / cout_pad16) % 9, ci = i / (cout_pad16 * 9);
+    wt[i] = (ci < cin && co < cout) ? w[(size_t(co) * 9 + (8 - tap)) * cin + ci] : uint16_t(0);
+}
+
+// Average 2x2 pooling: out (n, h/2, w/2) from in (n, h, w), channels c.
+__global__ void k_pool2(UnetEw e) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    const int ow = e.w / 2, oh = e.h / 2;
+    if (i >= e.n * oh * ow * e.c) return;
+    const int ch = i % e.c, q = i / e.c, x = q % ow, y = (q / ow) % oh, n = q / (ow * oh);
+    float s = 0.0f;
+    for (int k = 0; k < 4; k++) {
+        const size_t src = size_t((n * e.h + 2 * y + (k >> 1)) * e.w + 2 * x + (k & 1));
+        s += bf2f(e.a[src * e.a_cs + e.a_co + ch]);
+    }
+    e.out[size_t(q) * e.out_cs + e.out_co + ch] = f2bf(0.25f * s);
+}
+
+// Nearest 2x upsampling: out (n, 2h, 2w) from in (n, h, w) (h, w: the input's).
+__global__ void k_up2(UnetEw e) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    const int ow = 2 * e.w, oh = 2 * e.h;
+    if (i >= e.n * oh * ow * e.c) return;
+    const int ch = i % e.c, q = i / e.c, x = q % ow, y = (q / ow) % oh, n = q / (ow * oh);
+    const size_t src = size_t((n * e.h + y / 2) * e.w + x / 2);
+    e.out[size_t(q) * e.out_cs + e.out_co + ch] = e.a[src * e.a_cs + e.a_co + ch];
+}
+
+// Channel-slice copy at one resolution.
+__global__ void k_copy(UnetEw e) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= e.n * e.h * e.w * e.c) return;
+    const int ch = i % e.c, q = i / e.c;
+    e.out[size_t(q) * e.out_cs + e.out_co + ch] = e.a[size_t(q) * e.a_cs + e.a_co + ch];
+}
+
+// Gradient at a ReLU'd tensor T (n, h, w): mask(T > 0) x (a at the same pixel
+// + the 2x2 pooling's share of b at half resolution + the sum of c's 2x2
+// children at double resolution), each term optional.
+__global__ void k_grad_combine(UnetEw e) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= e.n * e.h * e.w * e.c) return;
+    const int ch = i % e.c, q = i / e.c, x = q % e.w, y = (q / e.w) % e.h, n = q / (e.w * e.h);
+    float s = 0.0f;
+    if (e.a) s += bf2f(e.a[size_t(q) * e.a_cs + e.a_co + ch]);
+    if (e.b) {
+        const size_t src = size_t((n * (e.h / 2) + y / 2) * (e.w / 2) + x / 2);
+        s += 0.25f * bf2f(e.b[src * e.b_cs + e.b_co + ch]);
+    }
+    if (e.c_in) {
+        for (int k = 0; k < 4; k++) {
+            const size_t src = size_t((n * 2 * e.h + 2 * y + (k >> 1)) * (2 * e.w) + 2 * x + (k & 1));
+            s += bf2f(e.c_in[src * e.c_cs + e.c_co + ch]);
+        }
+    }
+    if (e.mask && bf2f(e.mask[size_t(q) * e.mask_cs + e.mask_co + ch]) <= 0.0f) s = 0.0f;
+    e.out[size_t(q) * e.out_cs + e.out_co + ch] = f2bf(s);
+}
+
+// L1 loss in the log domain over 3 channels and its gradient into a 16-channel
+// bf16 buffer (channels 3.. zero): loss += |pred - target| / count.
+__global__ void k_l1_loss(const float* __restrict__ pred, const float* __restrict__ target, int npix,
+                          uint16_t* __restrict__ grad, float* __restrict__ loss, float scale) {
+    const int q = int(blockIdx.x * blockDim.x + threadIdx.x);
+    float l = 0.0f;
+    if (q < npix) {
+        for (int ch = 0; ch < 16; ch++) {
+            float g = 0.0f;
+            if (ch < 3) {
+                const float d = pred[size_t(q) * 3 + ch] - target[size_t(q) * 3 + ch];
+                l += fabsf(d);
+                g = d > 0.0f ? scale : (d < 0.0f ? -scale : 0.0f);
+            }
+            grad[size_t(q) * 16 + ch] = f2bf(g);
+        }
+    }
+    // One atomic per wave.
+    for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
+    if ((threadIdx.x & 63u) == 0u) atomicAdd(loss, l * scale);
+}
+
+// Adam (Kingma and Ba 2015) on FP32 master weights, then their bf16 copy.
+__global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
+                       const float* __restrict__ g, uint16_t* __restrict__ w_bf, int n, float lr, float b1, float b2,
+                       float eps, float c1, float c2) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi, v[i] = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wi;
+    if (w_bf) w_bf[i] = f2bf(wi);
+}
+
+__global__ void k_f32_to_bf16(const float* __restrict__ a, uint16_t* __restrict__ out, int n) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) out[i] = f2bf(a[i]);
+}
+
+// Network input from a path-traced frame's accumulation and guide images (n
+// samples): log1p of the demodulated irradiance (as the a-trous filter
+// demodulates, denoise.comp:37-39), the mean albedo, the mean unit normal, the
+// background share; 16 channels (10 used). Images are (h, w) float4 row-major;
+// the input tensor is (hp, wp) >= (h, w), zero outside.
+__global__ void k_unet_input(const float4* __restrict__ accum, const float4* __restrict__ albedo,
+                             const float4* __restrict__ normal, int w, int h, float n, uint16_t* __restrict__ x,
+                             int wp, int hp, float* __restrict__ alb_out) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= wp * hp) return;
+    const int px = i % wp, py = i / wp;
+    float f[16] = {};
+    float ar = 1.0f, ag = 1.0f, ab = 1.0f;
+    if (px < w && py < h) {
+        const size_t p = size_t(py) * w + px;
+        const float4 a = accum[p], g = albedo[p], nd = normal[p];
+        ar = fmaxf(g.x / n, 0.02f), ag = fmaxf(g.y / n, 0.02f), ab = fmaxf(g.z / n, 0.02f);
+        f[0] = log1pf(fmaxf(a.x / n, 0.0f) / ar);
+        f[1] = log1pf(fmaxf(a.y / n, 0.0f) / ag);
+        f[2] = log1pf(fmaxf(a.z / n, 0.0f) / ab);
+        f[3] = ar, f[4] = ag, f[5] = ab;
+        const float gx = nd.x / n, gy = nd.y / n, gz = nd.z / n;
+        const float len = sqrtf(gx * gx + gy * gy + gz * gz);
+        if (len > 1e-3f) f[6] = gx / len, f[7] = gy / len, f[8] = gz / len;
+        f[9] = a.w / n;
+    }
+#pragma unroll
+    for (int k = 0; k < 16; k++) x[size_t(i) * 16 + k] = f2bf(f[k]);
+    if (alb_out) {
+        alb_out[size_t(i) * 3] = ar, alb_out[size_t(i) * 3 + 1] = ag, alb_out[size_t(i) * 3 + 2] = ab;
+    }
+}
+
+// Training target: log1p of the reference radiance demodulated by the noisy
+// frame's albedo (the network's output is modulated by the same albedo).
+__global__ void k_unet_target(const float4* __restrict__ t, const float* __restrict__ alb, int npix,
+                              float* __restrict__ out) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= npix) return;
+    const float4 v = t[i];
+    out[size_t(i) * 3] = log1pf(fmaxf(v.x, 0.0f) / alb[size_t(i) * 3]);
+    out[size_t(i) * 3 + 1] = log1pf(fmaxf(v.y, 0.0f) / alb[size_t(i) * 3 + 1]);
+    out[size_t(i) * 3 + 2] = log1pf(fmaxf(v.z, 0.0f) / alb[size_t(i) * 3 + 2]);
+}
+
+// The denoised frame: expm1 of the predicted log irradiance times the mean
+// albedo, the background share as alpha, into RGBA16F.
+__global__ void k_unet_output(const float* __restrict__ pred, const float* __restrict__ alb, int wp,
+                              const float4* __restrict__ accum, float n, int w, int h, uint16_t* __restrict__ hdr) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= w * h) return;
+    const int px = i % w, py = i / w;
+    const size_t q = size_t(py) * wp + px;
+    float c[4];
+    for (int k = 0; k < 3; k++) c[k] = expm1f(fmaxf(pred[q * 3 + k], 0.0f)) * alb[q * 3 + k];
+    c[3] = accum[i].w / n;
+    uint2 word;
+    word.x = uint32_t(__half_as_ushort(__float2half_rn(c[0]))) | (uint32_t(__half_as_ushort(__float2half_rn(c[1]))) << 16);
+    word.y = uint32_t(__half_as_ushort(__float2half_rn(c[2]))) | (uint32_t(__half_as_ushort(__float2half_rn(c[3]))) << 16);
+    reinterpret_cast<uint2*>(hdr)[i] = word;
+}
+
+int blocks(long n, int t = 256) { return int((n + t - 1) / t); }
+
+}  // namespace
+
+void launch_conv3x3(const UnetConv& c, hipStream_t st) {
+    const int npix = c.n * c.h * c.w;
+    UnetConv k = c;
+    k.co_blocks = (c.cout + 31) / 32;
+    const int groups = (k.co_blocks + 3) / 4;
+    const dim3 grid((npix + 127) / 128, groups);
+    if (k.co_blocks == 1) hipLaunchKernelGGL(k_conv3x3<1>, dim3(grid.x, k.co_blocks), dim3(256), 0, st, k);
+    else if (k.co_blocks == 2) hipLaunchKernelGGL(k_conv3x3<2>, dim3(grid.x, 1), dim3(256), 0, st, k);
+    else if (k.co_blocks == 3) hipLaunchKernelGGL(k_conv3x3<3>, dim3(grid.x, 1), dim3(256), 0, st, k);
+    else hipLaunchKernelGGL(k_conv3x3<4>, grid, dim3(256), 0, st, k);
+}
+
+void launch_conv_wgrad(const UnetWgrad& g, hipStream_t st) {
+    UnetWgrad k = g;
+    k.co_blocks = (g.cout + 31) / 32;
+    const int npix = g.n * g.h * g.w;
+    const int chunks = (npix + k.chunk - 1) / k.chunk;
+    const dim3 grid(chunks, 9 * k.co_blocks * ((g.cin + 31) / 32));
+    hipLaunchKernelGGL(k_conv_wgrad, grid, dim3(64), 0, st, k);
+}
+
+void launch_wtrans(const uint16_t* w, int cout, int cin, uint16_t* wt, hipStream_t st) {
+    const int cout16 = (cout + 15) / 16 * 16, cin32 = (cin + 31) / 32 * 32;
+    hipLaunchKernelGGL(k_wtrans, dim3(blocks(long(cin32) * 9 * cout16)), dim3(256), 0, st, w, cout, cin, wt, cout16,
+                       cin32);
+}
+
+void launch_unet_ew(int op, const UnetEw& e, hipStream_t st) {
+    long n = long(e.n) * e.h * e.w * e.c;
+    if (op == kUnetPool) n = long(e.n) * (e.h / 2) * (e.w / 2) * e.c;
+    if (op == kUnetUp) n = long(e.n) * (2 * e.h) * (2 * e.w) * e.c;
+    auto k = op == kUnetPool ? k_pool2 : op == kUnetUp ? k_up2 : op == kUnetCopy ? k_copy : k_grad_combine;
+    hipLaunchKernelGGL(k, dim3(blocks(n)), dim3(256), 0, st, e);
+}
+
+void launch_l1_loss(const float* pred, const float* target, int npix, uint16_t* grad, float* loss, hipStream_t st) {
+    hipLaunchKernelGGL(k_l1_loss, dim3(blocks(npix)), dim3(256), 0, st, pred, target, npix, grad, loss,
+                       1.0f / (3.0f * float(npix)));
+}
+
+void launch_adam(float* w, float* m, float* v, const float* g, uint16_t* w_bf, int n, float lr, int step,
+                 hipStream_t st) {
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f - powf(b1, float(step)), c2 = 1.0f - powf(b2, float(step));
+    hipLaunchKernelGGL(k_adam, dim3(blocks(n)), dim3(256), 0, st, w, m, v, g, w_bf, n, lr, b1, b2, 1e-8f, c1, c2);
+}
+
+void launch_f32_to_bf16(const float* a, uint16_t* out, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_f32_to_bf16, dim3(blocks(n)), dim3(256), 0, st, a, out, n);
+}
+
+void launch_unet_input(const float4* accum, const float4* albedo, const float4* normal, int w, int h, float n,
+                       uint16_t* x, int wp, int hp, float* alb_out, hipStream_t st) {
+    hipLaunchKernelGGL(k_unet_input, dim3(blocks(long(wp) * hp)), dim3(256), 0, st, accum, albedo, normal, w, h, n, x,
+                       wp, hp, alb_out);
+}
+
+void launch_unet_target(const float4* t, const float* alb, int npix, float* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_unet_target, dim3(blocks(npix)), dim3(256), 0, st, t, alb, npix, out);
+}
+
+void launch_unet_output(const float* pred, const float* alb, int wp, const float4* accum, float n, int w, int h,
+                        uint16_t* hdr, hipStream_t st) {
+    hipLaunchKernelGGL(k_unet_output, dim3(blocks(long(w) * h)), dim3(256), 0, st, pred, alb, wp, accum, n, w, h, hdr);
+}
+
+}  // namespace vvr

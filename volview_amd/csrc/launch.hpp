@@ -48,6 +48,57 @@ void launch_ao(const DevScene& s, uint16_t* out, hipStream_t st);
 void launch_post(const vvr_uniforms* du, const uint16_t* hdr, uint32_t width, uint32_t height, uint16_t* bloom,
                  float* exposure, uint32_t* out, hipStream_t st);
 size_t bloom_bytes(uint32_t width, uint32_t height);
+// ---- the learned denoiser's U-Net (kernels_unet.hip) ----------------------------
+// Tensors are NHWC bf16 given as (pointer, channel stride, channel offset).
+struct UnetConv {  // out = act(conv3x3(in) + bias), optionally masked by mask > 0
+    const uint16_t* in;
+    int in_cs, in_co, cin;  // cin: multiple of 16
+    const uint16_t* wts;    // [co rounded up to 32][9][cin]
+    const float* bias;      // [cout] or null
+    uint16_t* out;          // bf16 output or null
+    int out_cs, out_co, cout;
+    float* out_f32;  // [pixels][cout] or null
+    const uint16_t* mask;
+    int mask_cs, mask_co;
+    int n, h, w, relu, co_blocks;
+};
+struct UnetWgrad {  // dw[co][tap][ci] += ..., db[co] += ... (FP32)
+    const uint16_t* x;
+    int x_cs, x_co, cin;
+    const uint16_t* dy;
+    int dy_cs, cout;
+    float* dw;
+    float* db;
+    int n, h, w, chunk, co_blocks;
+};
+struct UnetEw {  // pooling, upsampling, slice copies, gradient sums
+    const uint16_t* a;
+    int a_cs, a_co;
+    const uint16_t* b;
+    int b_cs, b_co;
+    const uint16_t* c_in;
+    int c_cs, c_co;
+    const uint16_t* mask;
+    int mask_cs, mask_co;
+    uint16_t* out;
+    int out_cs, out_co;
+    int n, h, w, c;
+};
+enum { kUnetPool = 0, kUnetUp = 1, kUnetCopy = 2, kUnetCombine = 3 };
+void launch_conv3x3(const UnetConv& c, hipStream_t st);
+void launch_conv_wgrad(const UnetWgrad& g, hipStream_t st);
+void launch_wtrans(const uint16_t* w, int cout, int cin, uint16_t* wt, hipStream_t st);
+void launch_unet_ew(int op, const UnetEw& e, hipStream_t st);
+void launch_l1_loss(const float* pred, const float* target, int npix, uint16_t* grad, float* loss, hipStream_t st);
+void launch_adam(float* w, float* m, float* v, const float* g, uint16_t* w_bf, int n, float lr, int step,
+                 hipStream_t st);
+void launch_f32_to_bf16(const float* a, uint16_t* out, int n, hipStream_t st);
+void launch_unet_input(const float4* accum, const float4* albedo, const float4* normal, int w, int h, float n,
+                       uint16_t* x, int wp, int hp, float* alb_out, hipStream_t st);
+void launch_unet_target(const float4* t, const float* alb, int npix, float* out, hipStream_t st);
+void launch_unet_output(const float* pred, const float* alb, int wp, const float4* accum, float n, int w, int h,
+                        uint16_t* hdr, hipStream_t st);
+
 void launch_denoise(const float4* accum, const float4* albedo, const float4* normal, float4* scratch, uint16_t* hdr,
                     uint32_t width, uint32_t height, float samples, hipStream_t st);
 

@@ -132,6 +132,8 @@ void record(vvr_renderer* r, uint64_t frame, const vvr_frame_params* p) {
     if (!r->hdr) throw Error{VVR_ESTATE, "resize() must be called before record()"};
     validate_frame(*p);
     const bool pt = p->path_tracing_enabled != 0;
+    if (pt && p->path_tracing.denoiser == VVR_DENOISE_LEARNED && !r->unet)
+        throw Error{VVR_EINVAL, "no learned denoiser is attached (vvr_renderer_set_denoiser)"};
     r->use_device();
 
     // Accumulation restarts when anything but the denoiser changes (renderer.rs:1535-1544).
@@ -173,12 +175,17 @@ void record(vvr_renderer* r, uint64_t frame, const vvr_frame_params* p) {
         r->accumulated += spf;
         // renderer.rs:1795-1797: the a-trous denoiser replaces the HDR image with
         // its filtered version of everything accumulated so far.
-        r->denoised = p->path_tracing.denoiser == VVR_DENOISE_ATROUS;
-        if (r->denoised) {
+        r->denoised = p->path_tracing.denoiser != VVR_DENOISE_OFF;
+        if (p->path_tracing.denoiser == VVR_DENOISE_ATROUS) {
             if (!r->dn_scratch) r->dn_scratch = dalloc<float4>(4 * size_t(r->width) * r->height, "denoiser images");
             r->timed(VVR_KERNEL_DENOISE, [&] {
                 launch_denoise(r->accum, r->albedo, r->normal, r->dn_scratch, r->hdr, r->width, r->height,
                                float(r->accumulated), r->stream);
+            });
+        } else if (p->path_tracing.denoiser == VVR_DENOISE_LEARNED) {
+            r->timed(VVR_KERNEL_DENOISE, [&] {
+                unet_denoise_frame(r->unet, r->accum, r->albedo, r->normal, float(r->accumulated), r->width, r->height,
+                                   r->hdr, r->stream);
             });
         }
     } else {
@@ -255,6 +262,15 @@ int vvr_renderer_set_environment_map(vvr_renderer* r, const vvr_env_map* map) {
         r->env_w = int(t.width), r->env_h = int(t.height);
         r->env_peak_u = t.peak_u;
         r->has_env = true;
+    });
+}
+
+int vvr_renderer_set_denoiser(vvr_renderer* r, vvr_unet* net) {
+    return guarded([&] {
+        if (!r) throw Error{VVR_EINVAL, "null argument"};
+        r->use_device();
+        check(hipStreamSynchronize(r->stream), "device wait");
+        r->unet = net;
     });
 }
 

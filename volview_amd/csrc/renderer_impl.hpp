@@ -216,8 +216,7 @@ inline LightKey light_key(const OpacityKey& opacity, const vvr_uniforms& u, bool
 
 inline void validate_frame(const vvr_frame_params& p) {
     const bool pt = p.path_tracing_enabled != 0;
-    if (pt && p.path_tracing.denoiser > VVR_DENOISE_ATROUS)
-        throw Error{VVR_EINVAL, "the learned denoiser is not available: use VVR_DENOISE_ATROUS or VVR_DENOISE_OFF"};
+    if (pt && p.path_tracing.denoiser > VVR_DENOISE_LEARNED) throw Error{VVR_EINVAL, "unknown denoiser"};
     if (!(p.step_mm > 0.0f)) throw Error{VVR_EINVAL, "step_mm must be positive"};
     if (p.labels.mode > VVR_LABELS_HYBRID) throw Error{VVR_EINVAL, "unknown label mode"};
 }
@@ -230,6 +229,10 @@ using namespace vvr;
 struct vvr_ctx {
     int device;
 };
+
+// Runs the learned denoiser on a path-traced frame's images into its HDR image (unet.cpp).
+void unet_denoise_frame(vvr_unet* n, const float4* accum, const float4* albedo, const float4* normal, float samples,
+                        uint32_t width, uint32_t height, uint16_t* hdr, hipStream_t st);
 
 struct vvr_renderer {
     int device = 0;
@@ -281,6 +284,7 @@ struct vvr_renderer {
     uint16_t* bloom = nullptr;     // glow images A, B (quarter), C, D (eighth resolution), RGBA16F
     float* exposure = nullptr;     // adapted auto exposure, persists across frames
     bool denoised = false;         // the HDR image holds the denoiser's output
+    vvr_unet* unet = nullptr;      // the learned denoiser (VVR_DENOISE_LEARNED), owned by the caller
     // Path-tracing engine: the wavefront engine (default), or with
     // VVR_PT_KERNEL=persistent / simple the persistent megakernel or one lane per pixel.
     enum { kWavefront, kPersistent, kSimple } pt_kernel = kWavefront;

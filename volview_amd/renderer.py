@@ -223,6 +223,12 @@ class Renderer:
                                                        table.size))
         return peak.value, table
 
+    def set_denoiser(self, net):
+        """Frames with VVR_DENOISE_LEARNED run the learned denoiser `net` (a
+        volview_amd.unet.Unet, kept alive by the caller; None detaches)."""
+        check(self._lib.vvr_renderer_set_denoiser(self.handle, net.handle if net is not None else None))
+        self._denoiser = net
+
     def read_rgba(self) -> np.ndarray:
         """The tone-mapped RGBA8 display image of the last record()."""
         out = np.zeros((self.height, self.width, 4), np.uint8)
