@@ -9,8 +9,8 @@
 - A small HBM budget makes requests wait for admission; all still finish and
   the budget is never exceeded once more than one request is admitted.
 - Requests the renderer would reject are rejected at submission.
-- A path-traced request with the a-trous denoiser delivers denoised frames,
-  equal to its renderer's.
+- Path-traced requests with the a-trous or the learned denoiser deliver
+  denoised frames, equal to their renderer's.
 - Cancelling ends a queued request at once, and a path-traced one in flight
   with its last delivered frame.
 """
@@ -20,6 +20,7 @@ import pytest
 from volview_amd import _abi, phantom, workloads
 from volview_amd.renderer import Renderer, TransferFunction
 from volview_amd.server import Server
+from volview_amd.unet import Unet
 
 pytestmark = pytest.mark.gpu
 
@@ -38,12 +39,14 @@ def volumes(ctx):
 def mixed_requests(a, b, n):
     dvr_a = workloads.dvr_params(a, 0)
     pt_a = workloads.pt_params(a, 0, 8, bounces=4, camera_index=1)
+    pt_learned = workloads.pt_params(a, 0, 8, bounces=4, camera_index=3)
+    pt_learned.path_tracing.denoiser = _abi.DENOISE_LEARNED  # the network attached to `a`
     pt_b = workloads.pt_params(b, n, 12, bounces=4, camera_index=2)
     pt_b.path_tracing.denoiser = _abi.DENOISE_ATROUS  # each delivered frame denoised
     dvr_b = workloads.dvr_params(b, 1)
     workloads.organ_label_params(dvr_b, n, 7)  # same labels as pt_b: one opacity key per volume
     return [(a, dvr_a, 64, 48), (a, pt_a, 32, 24), (b, pt_b, 24, 32), (b, dvr_b, 40, 40),
-            (a, workloads.dvr_params(a, 3), 17, 29)]
+            (a, workloads.dvr_params(a, 3), 17, 29), (a, pt_learned, 36, 20)]
 
 
 def render_alone(r, p, w, h, increment):
@@ -77,6 +80,8 @@ def drain(srv, ids, timeout_polls=600, cancelled=()):
 
 def test_served_frames_equal_alone(ctx):
     a, b, n = volumes(ctx)
+    net = Unet.pretrained(ctx)
+    a.set_denoiser(net)
     reqs = mixed_requests(a, b, n)
     inc = 4
     srv = Server(ctx, pt_increment=inc)
@@ -105,6 +110,8 @@ def test_served_frames_equal_alone(ctx):
 
 def test_budget_queues_requests_and_all_finish(ctx):
     a, b, n = volumes(ctx)
+    net = Unet.pretrained(ctx)
+    a.set_denoiser(net)
     reqs = mixed_requests(a, b, n) * 3
     # The largest request: the denoised path-traced one, its images, two
     # increments' chunk sums in flight and the denoiser's four images.

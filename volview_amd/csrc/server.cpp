@@ -107,6 +107,7 @@ size_t chunk_sum_bytes(const Job& j) {  // one increment's chunk sums
 }
 
 bool denoised(const Job& j) { return j.pt && j.params.path_tracing.denoiser == VVR_DENOISE_ATROUS; }
+bool learned(const Job& j) { return j.pt && j.params.path_tracing.denoiser == VVR_DENOISE_LEARNED; }
 
 size_t request_bytes(const Job& j) {
     const size_t px = size_t(j.width) * j.height;
@@ -475,6 +476,10 @@ void vvr_server::advance_jobs() {
                 launch_denoise(j->accum, j->albedo, j->normal, j->dn_scratch, j->hdr, j->width, j->height,
                                float(j->spp_resolved), stream);
                 check(hipGetLastError(), "denoiser");
+            } else if (learned(*j)) {  // the network attached to the request's renderer
+                unet_denoise_frame(j->vol->unet, j->accum, j->albedo, j->normal, float(j->spp_resolved), j->width,
+                                   j->height, j->hdr, stream);
+                check(hipGetLastError(), "learned denoiser");
             }
             read_back(j, j->spp_resolved);
         }
@@ -626,6 +631,9 @@ int vvr_server_submit(vvr_server* s, const vvr_request* rq, uint64_t* id) {
         if (!rq->width || !rq->height || rq->width > 16384 || rq->height > 16384)
             throw Error{VVR_EINVAL, "image size must be 1..16384 per axis"};
         validate_frame(*rq->params);
+        if (rq->params->path_tracing_enabled && rq->params->path_tracing.denoiser == VVR_DENOISE_LEARNED &&
+            !rq->volume->unet)
+            throw Error{VVR_EINVAL, "no learned denoiser is attached to the request's renderer"};
         auto j = std::make_unique<Job>();
         j->vol = rq->volume;
         j->params = *rq->params;

@@ -422,12 +422,14 @@ int vvr_unet_denoise(vvr_unet* n, uint32_t width, uint32_t height, const float* 
     return guarded([&] {
         if (!n || !accum || !albedo || !normal || !rgba) throw Error{VVR_EINVAL, "null argument"};
         check(hipSetDevice(n->device), "hipSetDevice");
-        n->shape(1, int(height), int(width));
-        upload_batch(n, 1, width, height, accum, albedo, normal, samples, nullptr);
-        n->forward();
         const size_t pix = size_t(width) * height;
+        n->stage(pix);
+        const size_t bytes = pix * sizeof(float4);
+        check(hipMemcpyAsync(n->h_accum, accum, bytes, hipMemcpyHostToDevice, n->stream), "upload");
+        check(hipMemcpyAsync(n->h_albedo, albedo, bytes, hipMemcpyHostToDevice, n->stream), "upload");
+        check(hipMemcpyAsync(n->h_normal, normal, bytes, hipMemcpyHostToDevice, n->stream), "upload");
         uint16_t* hdr = dalloc<uint16_t>(4 * pix, "U-Net frame");
-        launch_unet_output(n->pred, n->alb, int(width), n->h_accum, samples, int(width), int(height), hdr, n->stream);
+        unet_denoise_frame(n, n->h_accum, n->h_albedo, n->h_normal, samples, width, height, hdr, n->stream);
         std::vector<uint16_t> h(4 * pix);
         hipError_t e = hipMemcpyAsync(h.data(), hdr, 8 * pix, hipMemcpyDeviceToHost, n->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(n->stream);

@@ -10,6 +10,7 @@ library's own bf16 MFMA kernels.
 from __future__ import annotations
 
 from ctypes import POINTER, byref, c_float, c_void_p
+from pathlib import Path
 
 import numpy as np
 
@@ -60,6 +61,14 @@ class Unet:
         out = np.zeros((h, w, 4), np.float32)
         check(self._lib.vvr_unet_denoise(self.handle, w, h, _f(a), _f(b), _f(c), samples, _f(out)))
         return out
+
+    @classmethod
+    def pretrained(cls, ctx: Context) -> "Unet":
+        """The network trained by volview_amd.train_denoiser (5 levels, 32-128
+        channels; weights shipped in volview_amd/data/denoiser_unet.npy)."""
+        net = cls(ctx, (32, 48, 64, 96, 128))
+        net.set_params(np.load(Path(__file__).parent / "data" / "denoiser_unet.npy").astype(np.float32))
+        return net
 
     def save(self, path):
         np.save(path, self.params())
