@@ -359,6 +359,52 @@ def run_serving(ctx, seconds: float, grace: float = 5.0, increment: int = 16, sl
             "window_seconds": window, "plain": plain, "served": served}
 
 
+def unet_flops(channels, width, height) -> float:
+    """Algorithmic FLOPs of one U-Net pass over a width x height frame: 2 x 9 x
+    c_in x c_out per pixel of each 3x3 layer at its level's resolution (the
+    output layer with its 3 real channels)."""
+    L = len(channels)
+    per_level = [0.0] * L
+    for l in range(L):
+        per_level[l] += 9 * (16 if l == 0 else channels[l - 1]) * channels[l] + 9 * channels[l] * channels[l]
+    for l in range(L - 1):
+        per_level[l] += 9 * (channels[l + 1] + channels[l]) * channels[l] + 9 * channels[l] * channels[l]
+    per_level[0] += 9 * channels[0] * 3
+    return 2.0 * sum(m * width * height / 4 ** l for l, m in enumerate(per_level))
+
+
+def run_denoisers(r, params, size: int, frames: int = 5) -> dict:
+    """Per-frame cost of the a-trous and the learned denoiser on a 4-path frame
+    of the PT-1 scene (HIP events around each denoiser's launches)."""
+    from volview_amd.unet import Unet
+    net = Unet.pretrained(r.ctx)
+    r.set_denoiser(net)
+    out = {"frame": f"{size}^2, 4 paths per frame"}
+    p = _abi.FrameParams.from_buffer_copy(params)
+    p.path_tracing.samples_per_frame = 4
+    for name, mode in (("atrous", _abi.DENOISE_ATROUS), ("learned", _abi.DENOISE_LEARNED)):
+        p.path_tracing.denoiser = mode
+        r.record(1000, p)  # warm-up
+        r.synchronize()
+        r.set_timing(True)
+        for f in range(frames):
+            r.record(1001 + f, p)
+        r.synchronize()
+        ms, n = r.kernel_time(_abi.KERNEL_DENOISE)
+        pt_ms, pt_n = r.kernel_time(_abi.KERNEL_PT)
+        r.set_timing(False)
+        out[f"{name}_ms"] = round(ms / max(n, 1), 3)
+        out["pt_4spp_ms"] = round(pt_ms / max(pt_n, 1), 3)
+        log(f"denoiser {name}: {ms / max(n, 1):.3f} ms per frame")
+    flops = unet_flops(net.channels, size, size)
+    tflops = flops / (out["learned_ms"] * 1e-3) / 1e12
+    out["learned_roofline"] = {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                               "frac": round(tflops / 2500.0, 5), "flops": flops}
+    r.set_denoiser(None)
+    net.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -370,6 +416,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip Batched-32 (plain vs served)")
+    ap.add_argument("--no-denoisers", action="store_true", help="skip the per-frame denoiser timings")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the Poisson serving trace (0: skip it)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -443,6 +490,8 @@ def main() -> int:
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, scene.params, args.cpu_seconds)
+    if not args.no_denoisers:
+        out["denoisers"] = run_denoisers(r, scene.params, args.size)
     r.close()
     if not args.no_batched:
         out["batched"] = run_batched(ctx)

@@ -5,7 +5,8 @@
   alone by its renderer (path-traced ones with samples_per_frame = the
   server's increment, frame after frame): work items carry their request, each
   path its own seed, and each pixel's paths are summed in path order.
-- Progressive requests deliver one frame per increment, in order.
+- Progressive requests deliver frames in order, at most one per increment
+  (increments that finish together deliver the newest), the last with every path.
 - A small HBM budget makes requests wait for admission; all still finish and
   the budget is never exceeded once more than one request is admitted.
 - Requests the renderer would reject are rejected at submission.
@@ -91,8 +92,9 @@ def test_served_frames_equal_alone(ctx):
     for k, i in enumerate(ids):
         r, p, w, h = reqs[k]
         spp = [c.spp for c in got[i]]
-        if p.path_tracing_enabled:
-            assert spp == list(range(inc, p.path_tracing.samples_per_frame + 1, inc)), spp
+        if p.path_tracing_enabled:  # increments that finish together deliver their newest frame
+            assert spp == sorted(set(spp)) and all(x % inc == 0 for x in spp), spp
+            assert spp[-1] == p.path_tracing.samples_per_frame, spp
         else:
             assert spp == [0]
         assert all(c.tag == k for c in got[i])
@@ -156,7 +158,7 @@ def test_cancel_queued_and_running(ctx):
         first = next((c for c in cs if c.id == running), None)
         if first is not None:
             break
-    assert first is not None and first.spp == 4 and not first.final_frame
+    assert first is not None and first.spp % 4 == 0 and 4 <= first.spp < 400 and not first.final_frame
     srv.cancel(queued)  # max_requests = 1 keeps it queued behind the running request
     srv.cancel(running)
     got = drain(srv, [running, queued], cancelled=(queued,))

@@ -297,12 +297,18 @@ __device__ __forceinline__ bool flight_step(const DevScene& sc, Flight& f, Rng& 
     return true;
 }
 
-// Delta tracking against the brick bounds (pt.comp:300-334).
-__device__ inline bool free_flight(const DevScene& sc, V3 origin, V3 dir, bool smooth_density, int hero, V3& spectral,
-                                   V3& idx, Sample& s, int& cap_plane, float& cut, Rng& rng) {
+// How a bounded run of a tracking loop ended.
+enum Flow : int { kFlowExit = 0, kFlowHit = 1, kFlowSuspended = 2 };
+
+// Delta tracking against the brick bounds (pt.comp:300-334) from the loop state
+// `f`, stopping before iteration `stop` (kFlowSuspended; `f`, `spectral` and
+// `rng` then resume it exactly where it stopped).
+__device__ inline Flow free_flight_steps(const DevScene& sc, Flight& f, V3 dir, bool smooth_density, int hero,
+                                         V3& spectral, V3& idx, Sample& s, int& cap_plane, float& cut, Rng& rng,
+                                         int stop) {
     const vvr_uniforms* u = sc.u;
-    Flight f = flight_begin(u, origin, dir);
-    for (; f.i < 4096 && f.tt < f.t1; f.i++) {
+    const int end = min(stop, 4096);
+    for (; f.i < end && f.tt < f.t1; f.i++) {
         float majorant;
         if (!flight_step(sc, f, rng, majorant)) continue;
         idx = f.origin + f.d_idx * f.tt;
@@ -312,22 +318,33 @@ __device__ inline bool free_flight(const DevScene& sc, V3 origin, V3 dir, bool s
         const float sh = component(sigma, hero);
         if (rng.rand() * majorant < sh) {
             spectral = spectral * (sigma / sh);
-            return true;
+            return kFlowHit;
         }
         spectral = spectral * (vmax(V3(majorant) - sigma, V3(0.0f)) / (majorant - sh));
     }
-    return false;
+    return f.i < 4096 && f.tt < f.t1 ? kFlowSuspended : kFlowExit;
 }
 
-// Ratio tracking with Russian roulette (pt.comp:339-371).
-__device__ __forceinline__ V3 transmittance(const DevScene& sc, V3 origin, V3 dir, Rng& rng) {
+__device__ inline bool free_flight(const DevScene& sc, V3 origin, V3 dir, bool smooth_density, int hero, V3& spectral,
+                                   V3& idx, Sample& s, int& cap_plane, float& cut, Rng& rng) {
+    Flight f = flight_begin(sc.u, origin, dir);
+    return free_flight_steps(sc, f, dir, smooth_density, hero, spectral, idx, s, cap_plane, cut, rng, 4096) ==
+           kFlowHit;
+}
+
+// Ratio tracking with Russian roulette (pt.comp:339-371) from the loop state
+// `f` and running `result`, stopping before iteration `stop` (false; resumable
+// as free_flight_steps). True once the transmittance is final.
+__device__ __forceinline__ bool transmittance_steps(const DevScene& sc, Flight& f, V3& result, Rng& rng, int stop) {
     const vvr_uniforms* u = sc.u;
-    Flight f = flight_begin(u, origin, dir);
-    V3 result(1.0f);
-    for (; f.i < 4096 && f.tt < f.t1; f.i++) {
+    const int end = min(stop, 4096);
+    for (; f.i < end && f.tt < f.t1; f.i++) {
         const float peak = gmax(result.x, gmax(result.y, result.z));
         if (peak < 0.1f) {
-            if (rng.rand() * 0.1f >= peak) return V3(0.0f);
+            if (rng.rand() * 0.1f >= peak) {
+                result = V3(0.0f);
+                return true;
+            }
             result = result * (0.1f / peak);
         }
         float majorant;
@@ -335,6 +352,13 @@ __device__ __forceinline__ V3 transmittance(const DevScene& sc, V3 origin, V3 di
         const Sample m = sample_point(sc, f.origin + f.d_idx * f.tt);
         result = result * vmax(1.0f - path_sigma(u, m, m.extinction) / majorant, V3(0.0f));
     }
+    return !(f.i < 4096 && f.tt < f.t1);
+}
+
+__device__ __forceinline__ V3 transmittance(const DevScene& sc, V3 origin, V3 dir, Rng& rng) {
+    Flight f = flight_begin(sc.u, origin, dir);
+    V3 result(1.0f);
+    transmittance_steps(sc, f, result, rng, 4096);
     return result;
 }
 
@@ -465,6 +489,18 @@ __device__ __forceinline__ bool stage_extend(const DevScene& sc, const FrameCons
         return true;
     path_escape(sc, fc, st);
     return false;
+}
+
+// stage_extend as a bounded run from the flight state `f` (flight_begin of the
+// path's ray, or a suspended run's state): at most up to iteration `stop`.
+__device__ __forceinline__ Flow stage_extend_steps(const DevScene& sc, const FrameConsts& fc, PathState& st,
+                                                   EventState& ev, Flight& f, int stop) {
+    ev.cap_plane = -1;
+    ev.cut = 0.0f;
+    const Flow flow = free_flight_steps(sc, f, st.dir, st.bounce == 0u, st.hero, st.spectral, ev.idx, ev.s,
+                                        ev.cap_plane, ev.cut, st.rng, stop);
+    if (flow == kFlowExit) path_escape(sc, fc, st);
+    return flow;
 }
 
 // pt.comp:448-466 up to the shadow ray: shading frame, surface texture, the

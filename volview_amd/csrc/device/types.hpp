@@ -109,6 +109,8 @@ struct WfSlots {
     float4* e_map;     // light-map sample: direction, pdf
     float4* e_maplt;   // its radiance / pdf x transmittance
     uint16_t* req;     // request of the slot's path (engines over several requests)
+    uint2* p_flight;   // a suspended tracking loop: distance along the ray (f32 bits), iteration
+                       // (free flight; the shadow ray's, whose running transmittance is in e_lt)
 };
 
 // Counters in WfQueues::counts: queue lengths, the item queue head and the live
@@ -123,6 +125,11 @@ enum {
     kWfLIVE = 6,
     kWfIDLE0 = 7,
     kWfIDLE1 = 8,
+    kWfRF0 = 9,
+    kWfRF1 = 10,
+    kWfSH0 = 11,
+    kWfSH1 = 12,
+    kWfSC = 13,
     kWfCounts = 16
 };
 
@@ -131,6 +138,9 @@ struct WfQueues {
     uint32_t* ev;       // slots at an event (shade, shadow, scatter)
     uint32_t* end[2];   // slots whose path ended (or that have no item)
     uint32_t* idle[2];  // served engine: slots waiting for work
+    uint32_t* rf[2];    // slots whose free flight was suspended (resumed by the next extend)
+    uint32_t* sh[2];    // slots whose shadow ray was suspended (resumed by the next shadow stage)
+    uint32_t* sc;       // slots whose shadow ray is done (light-map stage, scatter)
     uint32_t* counts;   // kWfCounts counters
     // Engines over several requests: ff or ev binned by request, so a wave's
     // lanes mostly share a request, and the bins' counts and cursors

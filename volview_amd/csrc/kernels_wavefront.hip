@@ -8,6 +8,12 @@
 //   shadow   ratio-tracked shadow ray
 //   scatter  next-event estimate, next direction, roulette -> extend or end
 //
+// A free flight or shadow ray runs at most a set number of tracking iterations
+// per launch; one that is not done is suspended (its loop state stored in its
+// slot) and resumed by the same stage of the next iteration. Lanes then do not
+// idle behind the longest ray of their wave, and each launch's waves are
+// compacted anew. A path's random numbers are drawn in the same order either way.
+//
 // Each kernel keeps few registers live and runs coherent waves; queues are
 // compacted with a wave ballot, one atomic per wave and the lanes' mbcnt ranks.
 // A path draws its random numbers in the same order as the megakernel, so it
@@ -276,6 +282,9 @@ __global__ void __launch_bounds__(64) k_wf_regen(const PtRequest* __restrict__ r
         Q.counts[kWfFF0 + nxt] = 0u;
         Q.counts[kWfEV] = 0u;
         Q.counts[kWfEND0 + nxt] = 0u;
+        Q.counts[kWfRF0 + nxt] = 0u;
+        Q.counts[kWfSH0 + nxt] = 0u;
+        Q.counts[kWfSC] = 0u;
     }
     if (kBatched && blockIdx.x == 0) clear_bins(Q);
     const uint32_t n = Q.counts[kWfEND0 + cur];
@@ -376,6 +385,9 @@ __global__ void __launch_bounds__(64) k_wf_regen_served(const PtRequest* __restr
         Q.counts[kWfFF0 + nxt] = 0u;
         Q.counts[kWfEV] = 0u;
         Q.counts[kWfEND0 + nxt] = 0u;
+        Q.counts[kWfRF0 + nxt] = 0u;
+        Q.counts[kWfSH0 + nxt] = 0u;
+        Q.counts[kWfSC] = 0u;
     }
     if (blockIdx.x == 0) clear_bins(Q);
     const uint32_t n_end = Q.counts[kWfEND0 + cur];
@@ -469,28 +481,40 @@ __global__ void __launch_bounds__(64) k_wf_regen_served(const PtRequest* __restr
     }
 }
 
-// Stage 1: free flight.
+// Stage 1: free flight: the new flights of `in`, then the suspended ones of
+// rf[cur]; each runs at most `steps` tracking iterations.
 template <bool kBatched>
-__global__ void __launch_bounds__(64) k_wf_extend(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
-                                                  const uint32_t* __restrict__ in, uint32_t cur) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_wf_extend(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
+                                                  const uint32_t* __restrict__ in, uint32_t cur, int steps) {
     const uint32_t nxt = cur ^ 1u;
-    const uint32_t n = Q.counts[kWfFF0 + cur];
+    const uint32_t n_new = Q.counts[kWfFF0 + cur];
+    const uint32_t n = n_new + Q.counts[kWfRF0 + cur];
     for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
         const uint32_t q = base + __lane_id();
         const bool active = q < n;
-        const uint32_t slot = active ? in[q] : 0u;
-        bool hit = false;
+        const bool resumed = q >= n_new;
+        const uint32_t slot = active ? (resumed ? Q.rf[cur][q - n_new] : in[q]) : 0u;
+        Flow flow = kFlowExit;
         PathState st;
         SlotRef ref{0u, 0u, 0u};
         EventState ev;
-        if (active) load_path(S, slot, st, ref);
+        uint2 fl = make_uint2(0u, 0u);
+        if (active) {
+            load_path(S, slot, st, ref);
+            if (resumed) fl = S.p_flight[slot];
+        }
         for_each_request<kBatched>(active, ref.req, [&](uint32_t r) {
             const PtRequest& R = reqs[r];
-            hit = stage_extend(R.scene, frame_consts(R.scene.u), st, ev);
+            Flight f = flight_begin(R.scene.u, st.origin, st.dir);
+            if (resumed) f.tt = bitsf(fl.x), f.i = int(fl.y);
+            flow = stage_extend_steps(R.scene, frame_consts(R.scene.u), st, ev, f, f.i + steps);
+            fl = make_uint2(fbits(f.tt), uint32_t(f.i));
         });
         if (active) {
-            if (hit) {
+            if (flow == kFlowHit) {
                 store_event(S, slot, ev);
+            } else if (flow == kFlowSuspended) {
+                S.p_flight[slot] = fl;
             } else {
                 S.p_rad[slot] = f4(st.radiance, bitsf(ref.k | (ref.req << 16)));
                 if (st.bounce == 0u) {  // background: pt.comp:434-438
@@ -502,14 +526,15 @@ __global__ void __launch_bounds__(64) k_wf_extend(const PtRequest* __restrict__ 
             S.p_dir[slot].w = bitsf(st.rng.state);
             S.p_spec[slot] = f4(st.spectral, bitsf(ref.item));
         }
-        wave_push(active && hit, slot, Q.ev, &Q.counts[kWfEV]);
-        wave_push(active && !hit, slot, Q.end[nxt], &Q.counts[kWfEND0 + nxt]);
+        wave_push(active && flow == kFlowHit, slot, Q.ev, &Q.counts[kWfEV]);
+        wave_push(active && flow == kFlowSuspended, slot, Q.rf[nxt], &Q.counts[kWfRF0 + nxt]);
+        wave_push(active && flow == kFlowExit, slot, Q.end[nxt], &Q.counts[kWfEND0 + nxt]);
     }
 }
 
 // Stage 2: shading frame and light sample.
 template <bool kBatched>
-__global__ void __launch_bounds__(64) k_wf_shade(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_wf_shade(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
                                                  const uint32_t* __restrict__ in) {
     const uint32_t n = Q.counts[kWfEV];
     for (uint32_t q = blockIdx.x * 64u + __lane_id(); q < n; q += gridDim.x * 64u) {
@@ -537,30 +562,59 @@ __global__ void __launch_bounds__(64) k_wf_shade(const PtRequest* __restrict__ r
     }
 }
 
-// Stage 3: shadow ray.
+// Stage 3: shadow ray: the events of `in` (new rays), then the suspended rays
+// of sh[cur]; each runs at most `steps` tracking iterations. Finished rays go
+// on to the light-map stage and scatter (queue sc).
 template <bool kBatched>
 __global__ void __launch_bounds__(64) k_wf_shadow(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
-                                                  const uint32_t* __restrict__ in) {
-    const uint32_t n = Q.counts[kWfEV];
-    for (uint32_t q = blockIdx.x * 64u + __lane_id(); q < n; q += gridDim.x * 64u) {
-        const uint32_t slot = in[q];
-        const uint32_t req = fbits(S.p_rad[slot].w) >> 16;
-        PathState st;
-        st.rng.state = fbits(S.p_dir[slot].w);
-        EventState ev;
-        ev.idx = xyz(S.e_idx[slot]);
-        ev.light_dir = xyz(S.e_light[slot]);
-        for_each_request<kBatched>(true, req, [&](uint32_t r) { stage_shadow(reqs[r].scene, st, ev); });
-        S.e_lt[slot] = f4(ev.light_t, 0.0f);
-        S.p_dir[slot].w = bitsf(st.rng.state);
+                                                  const uint32_t* __restrict__ in, uint32_t cur, int steps) {
+    const uint32_t nxt = cur ^ 1u;
+    const uint32_t n_new = Q.counts[kWfEV];
+    const uint32_t n = n_new + Q.counts[kWfSH0 + cur];
+    for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
+        const uint32_t q = base + __lane_id();
+        const bool active = q < n;
+        const bool resumed = q >= n_new;
+        const uint32_t slot = active ? (resumed ? Q.sh[cur][q - n_new] : in[q]) : 0u;
+        uint32_t req = 0u;
+        Rng rng;
+        rng.state = 0u;
+        V3 origin(0.0f), dir(0.0f), result(1.0f);
+        uint2 fl = make_uint2(0u, 0u);
+        bool done = true;
+        if (active) {
+            req = fbits(S.p_rad[slot].w) >> 16;
+            rng.state = fbits(S.p_dir[slot].w);
+            origin = xyz(S.e_idx[slot]);
+            dir = xyz(S.e_light[slot]);
+            if (resumed) {
+                result = xyz(S.e_lt[slot]);
+                fl = S.p_flight[slot];
+            }
+        }
+        for_each_request<kBatched>(active, req, [&](uint32_t r) {
+            const DevScene& sc = reqs[r].scene;
+            Flight f = flight_begin(sc.u, origin, dir);
+            if (resumed) f.tt = bitsf(fl.x), f.i = int(fl.y);
+            done = transmittance_steps(sc, f, result, rng, f.i + steps);
+            fl = make_uint2(fbits(f.tt), uint32_t(f.i));
+        });
+        if (active) {
+            S.e_lt[slot] = f4(result, 0.0f);
+            if (!done) S.p_flight[slot] = fl;
+            S.p_dir[slot].w = bitsf(rng.state);
+        }
+        wave_push(active && done, slot, Q.sc, &Q.counts[kWfSC]);
+        wave_push(active && !done, slot, Q.sh[nxt], &Q.counts[kWfSH0 + nxt]);
     }
 }
 
-// Stage 3b, for frames with a light map: its sample and shadow ray.
+// Stage 3b, for frames with a light map: its sample and shadow ray (for the
+// paths whose key-light shadow ray is done).
 template <bool kBatched>
 __global__ void __launch_bounds__(64) k_wf_shadow_map(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
                                                       const uint32_t* __restrict__ in) {
-    const uint32_t n = Q.counts[kWfEV];
+    const uint32_t n = Q.counts[kWfSC];
     for (uint32_t q = blockIdx.x * 64u + __lane_id(); q < n; q += gridDim.x * 64u) {
         const uint32_t slot = in[q];
         const uint32_t req = fbits(S.p_rad[slot].w) >> 16;
@@ -581,12 +635,13 @@ __global__ void __launch_bounds__(64) k_wf_shadow_map(const PtRequest* __restric
     }
 }
 
-// Stage 4: next-event estimate, next direction, roulette.
+// Stage 4: next-event estimate, next direction, roulette (for the paths whose
+// shadow rays are done).
 template <bool kBatched>
 __global__ void __launch_bounds__(64) k_wf_scatter(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
                                                    const uint32_t* __restrict__ in, uint32_t cur) {
     const uint32_t nxt = cur ^ 1u;
-    const uint32_t n = Q.counts[kWfEV];
+    const uint32_t n = Q.counts[kWfSC];
     for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
         const uint32_t q = base + __lane_id();
         const bool active = q < n;
@@ -666,12 +721,18 @@ void wf_init(const WfSlots& S, const WfQueues& Q, uint32_t slots, hipStream_t st
 
 WfGrid wf_grid(int cus) {
     WfGrid g;
-    g.regen = waves_for(k_wf_regen<true>, cus, 32);
-    g.extend = waves_for(k_wf_extend<true>, cus, 32);
-    g.shade = waves_for(k_wf_shade<true>, cus, 32);
-    g.shadow = waves_for(k_wf_shadow<true>, cus, 32);
-    g.scatter = waves_for(k_wf_scatter<true>, cus, 32);
+    g.single = WfStageGrid{waves_for(k_wf_regen<false>, cus, 32), waves_for(k_wf_extend<false>, cus, 32),
+                           waves_for(k_wf_shade<false>, cus, 32), waves_for(k_wf_shadow<false>, cus, 32),
+                           waves_for(k_wf_scatter<false>, cus, 32)};
+    // The served regen's occupancy bounds the batched regen grid too (it runs in its place).
+    g.batched = WfStageGrid{std::min(waves_for(k_wf_regen<true>, cus, 32), waves_for(k_wf_regen_served, cus, 32)),
+                            waves_for(k_wf_extend<true>, cus, 32), waves_for(k_wf_shade<true>, cus, 32),
+                            waves_for(k_wf_shadow<true>, cus, 32), waves_for(k_wf_scatter<true>, cus, 32)};
     g.sort = cus * 2;
+    // Measured at PT-1 (512 x 512 x 400, 1024^2, 64 paths): 16 / 32 gives 220
+    // Msamples/s against 124 unsuspended; 8-24 / 16-48 are within 3 %.
+    g.flight_steps = 16;
+    g.shadow_steps = 32;
     return g;
 }
 
@@ -680,29 +741,30 @@ namespace {
 // Extend over ff[cur], then shade, shadow and scatter over ev; with `sort`,
 // each queue binned by request first.
 template <bool kBatched>
-void wf_stages(const PtRequest* reqs, const WfSlots& S, const WfQueues& Q, uint32_t cur, const WfGrid& g, bool sort,
+void wf_stages(const PtRequest* reqs, const WfSlots& S, const WfQueues& Q, uint32_t cur, const WfGrid& grid, bool sort,
                bool env_map, hipStream_t st) {
+    const WfStageGrid& g = kBatched ? grid.batched : grid.single;
     const uint32_t* ff = Q.ff[cur];
     const uint32_t* ev = Q.ev;
     if (sort) {
         uint32_t* hist = Q.bins;
-        hipLaunchKernelGGL(k_wf_bin_count, dim3(g.sort), dim3(256), 0, st, ff, Q.counts + kWfFF0 + cur, S.req, hist);
-        hipLaunchKernelGGL(k_wf_bin_scatter, dim3(g.sort), dim3(256), 0, st, ff, Q.counts + kWfFF0 + cur, S.req, hist,
+        hipLaunchKernelGGL(k_wf_bin_count, dim3(grid.sort), dim3(256), 0, st, ff, Q.counts + kWfFF0 + cur, S.req, hist);
+        hipLaunchKernelGGL(k_wf_bin_scatter, dim3(grid.sort), dim3(256), 0, st, ff, Q.counts + kWfFF0 + cur, S.req, hist,
                            hist + kWfBins, Q.sorted);
         ff = Q.sorted;
     }
-    hipLaunchKernelGGL(k_wf_extend<kBatched>, dim3(g.extend), dim3(64), 0, st, reqs, S, Q, ff, cur);
+    hipLaunchKernelGGL(k_wf_extend<kBatched>, dim3(g.extend), dim3(64), 0, st, reqs, S, Q, ff, cur, grid.flight_steps);
     if (sort) {
         uint32_t* hist = Q.bins + 2 * kWfBins;
-        hipLaunchKernelGGL(k_wf_bin_count, dim3(g.sort), dim3(256), 0, st, ev, Q.counts + kWfEV, S.req, hist);
-        hipLaunchKernelGGL(k_wf_bin_scatter, dim3(g.sort), dim3(256), 0, st, ev, Q.counts + kWfEV, S.req, hist,
+        hipLaunchKernelGGL(k_wf_bin_count, dim3(grid.sort), dim3(256), 0, st, ev, Q.counts + kWfEV, S.req, hist);
+        hipLaunchKernelGGL(k_wf_bin_scatter, dim3(grid.sort), dim3(256), 0, st, ev, Q.counts + kWfEV, S.req, hist,
                            hist + kWfBins, Q.sorted);
         ev = Q.sorted;
     }
     hipLaunchKernelGGL(k_wf_shade<kBatched>, dim3(g.shade), dim3(64), 0, st, reqs, S, Q, ev);
-    hipLaunchKernelGGL(k_wf_shadow<kBatched>, dim3(g.shadow), dim3(64), 0, st, reqs, S, Q, ev);
-    if (env_map) hipLaunchKernelGGL(k_wf_shadow_map<kBatched>, dim3(g.shadow), dim3(64), 0, st, reqs, S, Q, ev);
-    hipLaunchKernelGGL(k_wf_scatter<kBatched>, dim3(g.scatter), dim3(64), 0, st, reqs, S, Q, ev, cur);
+    hipLaunchKernelGGL(k_wf_shadow<kBatched>, dim3(g.shadow), dim3(64), 0, st, reqs, S, Q, ev, cur, grid.shadow_steps);
+    if (env_map) hipLaunchKernelGGL(k_wf_shadow_map<kBatched>, dim3(g.shadow), dim3(64), 0, st, reqs, S, Q, Q.sc);
+    hipLaunchKernelGGL(k_wf_scatter<kBatched>, dim3(g.scatter), dim3(64), 0, st, reqs, S, Q, Q.sc, cur);
 }
 
 }  // namespace
@@ -710,10 +772,10 @@ void wf_stages(const PtRequest* reqs, const WfSlots& S, const WfQueues& Q, uint3
 void wf_iteration(const PtRequest* reqs, uint32_t n_reqs, uint32_t total_items, const WfSlots& S, const WfQueues& Q,
                   uint32_t cur, const WfGrid& g, bool env_map, hipStream_t st) {
     if (n_reqs > 1) {
-        hipLaunchKernelGGL(k_wf_regen<true>, dim3(g.regen), dim3(64), 0, st, reqs, n_reqs, total_items, S, Q, cur);
+        hipLaunchKernelGGL(k_wf_regen<true>, dim3(g.batched.regen), dim3(64), 0, st, reqs, n_reqs, total_items, S, Q, cur);
         wf_stages<true>(reqs, S, Q, cur, g, n_reqs <= kWfBins, env_map, st);
     } else {
-        hipLaunchKernelGGL(k_wf_regen<false>, dim3(g.regen), dim3(64), 0, st, reqs, n_reqs, total_items, S, Q, cur);
+        hipLaunchKernelGGL(k_wf_regen<false>, dim3(g.single.regen), dim3(64), 0, st, reqs, n_reqs, total_items, S, Q, cur);
         wf_stages<false>(reqs, S, Q, cur, g, false, env_map, st);
     }
 }
@@ -725,7 +787,7 @@ void wf_init_served(const WfSlots& S, const WfQueues& Q, uint32_t slots, hipStre
 
 void wf_iteration_served(const PtRequest* reqs, const WfActive& A, const WfServed& V, const WfSlots& S,
                          const WfQueues& Q, uint32_t cur, const WfGrid& g, bool env_map, hipStream_t st) {
-    hipLaunchKernelGGL(k_wf_regen_served, dim3(g.regen), dim3(64), 0, st, reqs, A, V, S, Q, cur);
+    hipLaunchKernelGGL(k_wf_regen_served, dim3(g.batched.regen), dim3(64), 0, st, reqs, A, V, S, Q, cur);
     wf_stages<true>(reqs, S, Q, cur, g, true, env_map, st);
 }
 

@@ -121,9 +121,16 @@ void launch_pt_persistent(const DevScene& s, float4* accum, float4* albedo, floa
 // Wavefront path tracer (kernels_wavefront.hip). One iteration runs the five
 // stages over the queues; `reqs` is a device array of n_reqs requests with
 // item ranges [item_base, item_base + n_items) covering [0, total_items).
-struct WfGrid {
+struct WfStageGrid {
     int regen, extend, shade, shadow, scatter;  // single-wave workgroups per stage
-    int sort;                                    // 256-lane workgroups of the queue sort
+};
+struct WfGrid {
+    WfStageGrid single, batched;  // one request / several (their kernels' occupancies differ)
+    int sort;                     // 256-lane workgroups of the queue sort
+    // Tracking-loop iterations a free flight / shadow ray runs per launch before
+    // it is suspended and re-queued, so that a wave's lanes do not idle behind
+    // its longest ray (4096: never suspended).
+    int flight_steps, shadow_steps;
 };
 WfGrid wf_grid(int cus);
 // All `slots` slots empty on the end queue of iteration 0, item queue at 0.

@@ -69,6 +69,8 @@ struct WfEngine {
         int cus = 0;
         check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "device attribute");
         grid = wf_grid(std::max(cus, 1));
+        grid.flight_steps = int(std::min(env_u32("VVR_PT_FLIGHT_STEPS", uint32_t(grid.flight_steps)), 4096u));
+        grid.shadow_steps = int(std::min(env_u32("VVR_PT_SHADOW_STEPS", uint32_t(grid.shadow_steps)), 4096u));
         // 4M slots (about 1 GiB): enough paths in flight that the long random
         // walks of a frame's last iterations still fill the device.
         max_slots = env_u32("VVR_PT_SLOTS", 1u << 22);
@@ -87,17 +89,20 @@ struct WfEngine {
         if (n <= slots) return;
         dfree(slot_mem);
         slot_mem = nullptr, slots = 0;
-        constexpr int kF4 = 17, kQueues = 8;
+        constexpr int kF4 = 17, kQueues = 13;
         const size_t f4_bytes = size_t(n) * sizeof(float4), q_bytes = (size_t(n) * 4 + 255) / 256 * 256;
         const size_t req_bytes = (size_t(n) * 2 + 255) / 256 * 256, bin_bytes = 4 * kWfBins * 4;
-        slot_mem = dalloc<uint8_t>(kF4 * f4_bytes + kQueues * q_bytes + req_bytes + 256 + bin_bytes, "path slots");
+        const size_t flight_bytes = size_t(n) * sizeof(uint2);
+        slot_mem = dalloc<uint8_t>(kF4 * f4_bytes + flight_bytes + kQueues * q_bytes + req_bytes + 256 + bin_bytes,
+                                   "path slots");
         uint8_t* p = slot_mem;
         float4** f4s[kF4] = {&S.p_origin, &S.p_dir, &S.p_thr, &S.p_spec, &S.p_rad, &S.e_idx,
                              &S.e_s0,     &S.e_s1,  &S.e_s2,  &S.e_n,    &S.e_light, &S.e_lt,
                              &S.c0,       &S.c1,    &S.c2,    &S.e_map,  &S.e_maplt};
         for (auto f : f4s) *f = reinterpret_cast<float4*>(p), p += f4_bytes;
-        uint32_t** qs[kQueues] = {&Q.ff[0],   &Q.ff[1],   &Q.ev,      &Q.end[0],
-                                  &Q.end[1], &Q.idle[0], &Q.idle[1], &Q.sorted};
+        S.p_flight = reinterpret_cast<uint2*>(p), p += flight_bytes;
+        uint32_t** qs[kQueues] = {&Q.ff[0], &Q.ff[1], &Q.ev,    &Q.end[0], &Q.end[1], &Q.idle[0], &Q.idle[1],
+                                  &Q.sorted, &Q.rf[0], &Q.rf[1], &Q.sh[0],  &Q.sh[1],  &Q.sc};
         for (auto q : qs) *q = reinterpret_cast<uint32_t*>(p), p += q_bytes;
         S.req = reinterpret_cast<uint16_t*>(p), p += req_bytes;
         Q.counts = reinterpret_cast<uint32_t*>(p), p += 256;

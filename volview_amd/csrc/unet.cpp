@@ -12,6 +12,7 @@
 // (demodulated by the noisy frame's own albedo guide).
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <random>
 
 #include "renderer_impl.hpp"
@@ -46,6 +47,11 @@ using namespace vvr;
 struct vvr_unet {
     int device = 0;
     hipStream_t stream = nullptr;
+    // Every use of the network's buffers is serialised: host calls under `mu`,
+    // device work after `busy`, the event recorded after the previous use on
+    // whichever stream it ran (the network's own or a renderer's).
+    std::mutex mu;
+    hipEvent_t busy = nullptr;
     int L = 0;
     int ch[5] = {};
     std::vector<Layer> layers;  // enc (2 per level), dec (2 per level, deepest first), out
@@ -69,12 +75,18 @@ struct vvr_unet {
     explicit vvr_unet(int dev) : device(dev) {}
     ~vvr_unet() {
         (void)hipSetDevice(device);
+        if (busy) (void)hipEventSynchronize(busy);
         if (stream) (void)hipStreamSynchronize(stream);
+        if (busy) (void)hipEventDestroy(busy);
         dfree(w), dfree(g), dfree(m), dfree(v), dfree(w_bf), dfree(wt_bf), dfree(act), dfree(grad);
         dfree(pred), dfree(target), dfree(alb), dfree(loss), dfree(h_accum), dfree(h_albedo), dfree(h_normal);
         dfree(h_target);
         if (stream) (void)hipStreamDestroy(stream);
     }
+
+    // Device work on `s` starts after the previous use / the next use waits for `s`.
+    void acquire(hipStream_t s) { check(hipStreamWaitEvent(s, busy, 0), "hipStreamWaitEvent"); }
+    void release(hipStream_t s) { check(hipEventRecord(busy, s), "hipEventRecord"); }
 
     int enc(int l, int k) const { return 2 * l + k; }
     int dec(int l, int k) const { return 2 * L + 2 * (L - 2 - l) + k; }
@@ -119,6 +131,7 @@ struct vvr_unet {
         const size_t pix0 = size_t(b) * h * w;
         const size_t total = walk(nullptr, nullptr);
         if (pix0 > cap_pix || total > cap_elems) {
+            check(hipEventSynchronize(busy), "device wait");
             check(hipStreamSynchronize(stream), "device wait");
             dfree(act), dfree(grad), dfree(pred), dfree(target), dfree(alb);
             act = grad = nullptr, pred = target = alb = nullptr;
@@ -222,6 +235,8 @@ struct vvr_unet {
 
     void stage(size_t pix) {
         if (pix <= stage_pix) return;
+        check(hipEventSynchronize(busy), "device wait");
+        check(hipStreamSynchronize(stream), "device wait");
         dfree(h_accum), dfree(h_albedo), dfree(h_normal), dfree(h_target);
         h_accum = dalloc<float4>(pix, "U-Net staging");
         h_albedo = dalloc<float4>(pix, "U-Net staging");
@@ -242,6 +257,7 @@ void create_unet(vvr_ctx* ctx, const vvr_unet_config* cfg, vvr_unet** out) {
     check(hipSetDevice(ctx->device), "hipSetDevice");
     auto n = std::make_unique<vvr_unet>(ctx->device);
     check(hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking), "hipStreamCreate");
+    check(hipEventCreateWithFlags(&n->busy, hipEventDisableTiming), "hipEventCreate");
     n->L = int(cfg->levels);
     for (int l = 0; l < n->L; l++) n->ch[l] = int(cfg->channels[l]);
     const int L = n->L;
@@ -346,6 +362,22 @@ void upload_batch(vvr_unet* n, uint32_t batch, uint32_t w, uint32_t h, const flo
     }
 }
 
+// The network on a frame's accumulation and guide images into an RGBA16F image,
+// padding the frame to the network's multiple; on stream `st`, the lock held.
+void denoise_frame(vvr_unet* n, const float4* accum, const float4* albedo, const float4* normal, float samples,
+                   uint32_t width, uint32_t height, uint16_t* hdr, hipStream_t st) {
+    const int m = 1 << (n->L - 1);
+    const int wp = int((width + m - 1) / m * m), hp = int((height + m - 1) / m * m);
+    n->shape(1, hp, wp);
+    // Run on the renderer's stream so the frame's order holds.
+    hipStream_t saved = n->stream;
+    n->stream = st;
+    launch_unet_input(accum, albedo, normal, int(width), int(height), samples, n->x0.p, wp, hp, n->alb, st);
+    n->forward();
+    launch_unet_output(n->pred, n->alb, wp, accum, samples, int(width), int(height), hdr, st);
+    n->stream = saved;
+}
+
 }  // namespace
 
 extern "C" {
@@ -361,7 +393,9 @@ size_t vvr_unet_num_params(const vvr_unet* n) { return n ? abi_params(n) : 0; }
 int vvr_unet_get_params(vvr_unet* n, float* out, size_t count) {
     return guarded([&] {
         if (!n || !out) throw Error{VVR_EINVAL, "null argument"};
+        std::lock_guard<std::mutex> lock(n->mu);
         check(hipSetDevice(n->device), "hipSetDevice");
+        check(hipEventSynchronize(n->busy), "device wait");
         check(hipStreamSynchronize(n->stream), "device wait");
         copy_out(n, n->w, out, count);
     });
@@ -370,7 +404,9 @@ int vvr_unet_get_params(vvr_unet* n, float* out, size_t count) {
 int vvr_unet_get_grads(vvr_unet* n, float* out, size_t count) {
     return guarded([&] {
         if (!n || !out) throw Error{VVR_EINVAL, "null argument"};
+        std::lock_guard<std::mutex> lock(n->mu);
         check(hipSetDevice(n->device), "hipSetDevice");
+        check(hipEventSynchronize(n->busy), "device wait");
         check(hipStreamSynchronize(n->stream), "device wait");
         copy_out(n, n->g, out, count);
     });
@@ -380,7 +416,9 @@ int vvr_unet_set_params(vvr_unet* n, const float* in, size_t count) {
     return guarded([&] {
         if (!n || !in) throw Error{VVR_EINVAL, "null argument"};
         if (count != abi_params(n)) throw Error{VVR_EINVAL, "parameter count mismatch"};
+        std::lock_guard<std::mutex> lock(n->mu);
         check(hipSetDevice(n->device), "hipSetDevice");
+        check(hipEventSynchronize(n->busy), "device wait");
         check(hipStreamSynchronize(n->stream), "device wait");
         std::vector<float> h(n->n_params, 0.0f);
         each_param(n, [&](size_t a, size_t p, size_t c) { std::memcpy(h.data() + p, in + a, c * 4); });
@@ -399,7 +437,9 @@ int vvr_unet_train_step(vvr_unet* n, uint32_t batch, uint32_t width, uint32_t he
     return guarded([&] {
         if (!n || !accum || !albedo || !normal || !target) throw Error{VVR_EINVAL, "null argument"};
         if (!batch || !width || !height || !(samples > 0.0f)) throw Error{VVR_EINVAL, "empty batch"};
+        std::lock_guard<std::mutex> lock(n->mu);
         check(hipSetDevice(n->device), "hipSetDevice");
+        n->acquire(n->stream);
         n->shape(int(batch), int(height), int(width));
         upload_batch(n, batch, width, height, accum, albedo, normal, samples, target);
         n->forward();
@@ -412,6 +452,7 @@ int vvr_unet_train_step(vvr_unet* n, uint32_t batch, uint32_t width, uint32_t he
         }
         float l = 0.0f;
         check(hipMemcpyAsync(&l, n->loss, 4, hipMemcpyDeviceToHost, n->stream), "loss read-back");
+        n->release(n->stream);
         check(hipStreamSynchronize(n->stream), "train step");
         if (loss) *loss = l;
     });
@@ -421,15 +462,18 @@ int vvr_unet_denoise(vvr_unet* n, uint32_t width, uint32_t height, const float* 
                      const float* normal, float samples, float* rgba) {
     return guarded([&] {
         if (!n || !accum || !albedo || !normal || !rgba) throw Error{VVR_EINVAL, "null argument"};
+        std::lock_guard<std::mutex> lock(n->mu);
         check(hipSetDevice(n->device), "hipSetDevice");
         const size_t pix = size_t(width) * height;
         n->stage(pix);
+        n->acquire(n->stream);
         const size_t bytes = pix * sizeof(float4);
         check(hipMemcpyAsync(n->h_accum, accum, bytes, hipMemcpyHostToDevice, n->stream), "upload");
         check(hipMemcpyAsync(n->h_albedo, albedo, bytes, hipMemcpyHostToDevice, n->stream), "upload");
         check(hipMemcpyAsync(n->h_normal, normal, bytes, hipMemcpyHostToDevice, n->stream), "upload");
         uint16_t* hdr = dalloc<uint16_t>(4 * pix, "U-Net frame");
-        unet_denoise_frame(n, n->h_accum, n->h_albedo, n->h_normal, samples, width, height, hdr, n->stream);
+        denoise_frame(n, n->h_accum, n->h_albedo, n->h_normal, samples, width, height, hdr, n->stream);
+        n->release(n->stream);
         std::vector<uint16_t> h(4 * pix);
         hipError_t e = hipMemcpyAsync(h.data(), hdr, 8 * pix, hipMemcpyDeviceToHost, n->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(n->stream);
@@ -441,18 +485,12 @@ int vvr_unet_denoise(vvr_unet* n, uint32_t width, uint32_t height, const float* 
 
 }  // extern "C"
 
-// Runs the network on a renderer's accumulation and guide images into its HDR
-// image (VVR_DENOISE_LEARNED), padding the frame to the network's multiple.
+// Runs the learned denoiser on a renderer's images into its HDR image
+// (VVR_DENOISE_LEARNED), on the renderer's or the server's stream.
 void unet_denoise_frame(vvr_unet* n, const float4* accum, const float4* albedo, const float4* normal, float samples,
                         uint32_t width, uint32_t height, uint16_t* hdr, hipStream_t st) {
-    const int m = 1 << (n->L - 1);
-    const int wp = int((width + m - 1) / m * m), hp = int((height + m - 1) / m * m);
-    n->shape(1, hp, wp);
-    // Run on the renderer's stream so the frame's order holds.
-    hipStream_t saved = n->stream;
-    n->stream = st;
-    launch_unet_input(accum, albedo, normal, int(width), int(height), samples, n->x0.p, wp, hp, n->alb, st);
-    n->forward();
-    launch_unet_output(n->pred, n->alb, wp, accum, samples, int(width), int(height), hdr, st);
-    n->stream = saved;
+    std::lock_guard<std::mutex> lock(n->mu);
+    n->acquire(st);
+    denoise_frame(n, accum, albedo, normal, samples, width, height, hdr, st);
+    n->release(st);
 }
