@@ -419,6 +419,7 @@ def main() -> int:
     ap.add_argument("--no-denoisers", action="store_true", help="skip the per-frame denoiser timings")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the Poisson serving trace (0: skip it)")
+    ap.add_argument("--serve-slots", type=int, default=1 << 20, help="path slots of the server in the trace")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB mebibytes device to device (a plain copy kernel whose HBM counters "
                          "scale the path tracer's under rocprofv3 --pmc)")
@@ -496,7 +497,7 @@ def main() -> int:
     if not args.no_batched:
         out["batched"] = run_batched(ctx)
     if args.serve_seconds > 0:
-        out["serving"] = run_serving(ctx, args.serve_seconds)
+        out["serving"] = run_serving(ctx, args.serve_seconds, slots=args.serve_slots)
     print(json.dumps(out), flush=True)
     return 0
 

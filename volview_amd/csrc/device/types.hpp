@@ -110,7 +110,8 @@ struct WfSlots {
     float4* e_maplt;   // its radiance / pdf x transmittance
     uint16_t* req;     // request of the slot's path (engines over several requests)
     uint2* p_flight;   // a suspended tracking loop: distance along the ray (f32 bits), iteration
-                       // (free flight; the shadow ray's, whose running transmittance is in e_lt)
+                       // (free flight: iteration kWfNewFlight for a flight not begun; the shadow
+                       // ray's, whose running transmittance is in e_lt)
 };
 
 // Counters in WfQueues::counts: queue lengths, the item queue head and the live
@@ -125,8 +126,8 @@ enum {
     kWfLIVE = 6,
     kWfIDLE0 = 7,
     kWfIDLE1 = 8,
-    kWfRF0 = 9,
-    kWfRF1 = 10,
+    kWfSORTFF = 9,   // entries of the sorted flight queue, padding included
+    kWfSORTEV = 10,  // entries of the sorted event queue, padding included
     kWfSH0 = 11,
     kWfSH1 = 12,
     kWfSC = 13,
@@ -138,18 +139,22 @@ struct WfQueues {
     uint32_t* ev;       // slots at an event (shade, shadow, scatter)
     uint32_t* end[2];   // slots whose path ended (or that have no item)
     uint32_t* idle[2];  // served engine: slots waiting for work
-    uint32_t* rf[2];    // slots whose free flight was suspended (resumed by the next extend)
     uint32_t* sh[2];    // slots whose shadow ray was suspended (resumed by the next shadow stage)
     uint32_t* sc;       // slots whose shadow ray is done (light-map stage, scatter)
     uint32_t* counts;   // kWfCounts counters
-    // Engines over several requests: ff or ev binned by request, so a wave's
-    // lanes mostly share a request, and the bins' counts and cursors
-    // (kWfBins each for the ff sort, then for the ev sort).
+    // Engines over several requests: ff or ev binned by request, each request's
+    // run padded with kWfNone to a multiple of 64 entries so that every wave
+    // holds one request, and the bins' counts and cursors (kWfBins each for the
+    // ff sort, then for the ev sort).
     uint32_t* sorted;
     uint32_t* bins;
 };
 
 constexpr uint32_t kWfBins = 256;  // request indices the queue sort bins
+constexpr uint32_t kWfNone = 0xffffffffu;
+constexpr uint32_t kWfNewFlight = 0xffffffffu;  // p_flight.y of a free flight not begun
+// Entries of the sorted queue: a queue of `slots` entries plus the padding.
+constexpr uint32_t wf_sorted_capacity(uint32_t slots) { return slots + kWfBins * 64u; }
 
 // Requests the served engine draws work items from this iteration: entries of
 // its request table, passed by value.

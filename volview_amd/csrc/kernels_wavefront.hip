@@ -30,7 +30,7 @@ namespace vvr {
 
 namespace {
 
-constexpr uint32_t kNone = 0xffffffffu;
+constexpr uint32_t kNone = kWfNone;
 
 __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
 __device__ __forceinline__ float bitsf(uint32_t u) { return __uint_as_float(u); }
@@ -101,6 +101,23 @@ __device__ __forceinline__ void for_each_request(bool on, uint32_t req, F&& f) {
         if (mine) f(r);
         pending = pending && !mine;
     }
+}
+
+// Runs f(r) for the lanes with `on` set, r the request of the wave's first
+// such lane, read into a scalar register. For waves of one request only: the
+// sorted queues (each request's run padded to whole waves) and single-request
+// engines. Unlike for_each_request, f is inlined once and not in a loop, which
+// keeps the register-heavy stages at two waves per SIMD without spills.
+template <bool kBatched, class F>
+__device__ __forceinline__ void with_wave_request(bool on, uint32_t req, F&& f) {
+    if (!kBatched) {
+        if (on) f(0u);
+        return;
+    }
+    const uint64_t m = __ballot(on);
+    if (!m) return;
+    const uint32_t r = __builtin_amdgcn_readfirstlane(__shfl(req, __ffsll((unsigned long long)m) - 1));
+    if (on) f(r);
 }
 
 __device__ __forceinline__ uint32_t pack_counters(const PathState& st) {
@@ -217,16 +234,19 @@ __global__ void __launch_bounds__(256) k_wf_bin_count(const uint32_t* __restrict
 
 // Pass 2: each workgroup ranks a chunk of entries within their requests in
 // LDS, reserves each request's run with one atomic and writes the slots out.
+// Runs start at multiples of 64 entries; workgroup 0 fills each run's tail up
+// to the next multiple with kNone and stores the padded total in *out_count.
 constexpr uint32_t kSortPer = 16;  // entries per lane and chunk
 __global__ void __launch_bounds__(256) k_wf_bin_scatter(const uint32_t* __restrict__ in,
                                                         const uint32_t* __restrict__ count,
                                                         const uint16_t* __restrict__ req,
                                                         const uint32_t* __restrict__ hist, uint32_t* __restrict__ cursor,
-                                                        uint32_t* __restrict__ out) {
+                                                        uint32_t* __restrict__ out, uint32_t* __restrict__ out_count) {
     __shared__ uint32_t start[kWfBins], h[kWfBins], run[kWfBins];
     const uint32_t t = threadIdx.x;
-    // Exclusive prefix of the counts: where each request's run starts.
-    const uint32_t mine = hist[t];
+    // Exclusive prefix of the padded counts: where each request's run starts.
+    const uint32_t filled = hist[t];
+    const uint32_t mine = (filled + 63u) & ~63u;
     start[t] = mine;
     __syncthreads();
     for (uint32_t off = 1; off < kWfBins; off <<= 1) {
@@ -234,6 +254,10 @@ __global__ void __launch_bounds__(256) k_wf_bin_scatter(const uint32_t* __restri
         __syncthreads();
         start[t] += x;
         __syncthreads();
+    }
+    if (blockIdx.x == 0) {
+        if (t == kWfBins - 1u) *out_count = start[t];
+        for (uint32_t k = start[t] - mine + filled; k < start[t]; k++) out[k] = kNone;
     }
     start[t] -= mine;
     const uint32_t n = *count;
@@ -282,7 +306,6 @@ __global__ void __launch_bounds__(64) k_wf_regen(const PtRequest* __restrict__ r
         Q.counts[kWfFF0 + nxt] = 0u;
         Q.counts[kWfEV] = 0u;
         Q.counts[kWfEND0 + nxt] = 0u;
-        Q.counts[kWfRF0 + nxt] = 0u;
         Q.counts[kWfSH0 + nxt] = 0u;
         Q.counts[kWfSC] = 0u;
     }
@@ -349,6 +372,7 @@ __global__ void __launch_bounds__(64) k_wf_regen(const PtRequest* __restrict__ r
         if (active) {
             if (push) {
                 store_path(S, slot, st, ref);
+                S.p_flight[slot] = make_uint2(0u, kWfNewFlight);
                 S.c0[slot] = c0, S.c1[slot] = c1, S.c2[slot] = c2;
                 if (kBatched) S.req[slot] = uint16_t(ref.req);
             } else {
@@ -385,7 +409,6 @@ __global__ void __launch_bounds__(64) k_wf_regen_served(const PtRequest* __restr
         Q.counts[kWfFF0 + nxt] = 0u;
         Q.counts[kWfEV] = 0u;
         Q.counts[kWfEND0 + nxt] = 0u;
-        Q.counts[kWfRF0 + nxt] = 0u;
         Q.counts[kWfSH0 + nxt] = 0u;
         Q.counts[kWfSC] = 0u;
     }
@@ -470,6 +493,7 @@ __global__ void __launch_bounds__(64) k_wf_regen_served(const PtRequest* __restr
         if (active) {
             if (push) {
                 store_path(S, slot, st, ref);
+                S.p_flight[slot] = make_uint2(0u, kWfNewFlight);
                 S.c0[slot] = c0, S.c1[slot] = c1, S.c2[slot] = c2;
                 S.req[slot] = uint16_t(ref.req);
             } else {
@@ -481,32 +505,32 @@ __global__ void __launch_bounds__(64) k_wf_regen_served(const PtRequest* __restr
     }
 }
 
-// Stage 1: free flight: the new flights of `in`, then the suspended ones of
-// rf[cur]; each runs at most `steps` tracking iterations.
+// Stage 1: free flight over the `*n_in` entries of `in` (kNone: padding): new
+// flights, and flights a previous launch suspended (p_flight); each runs at
+// most `steps` tracking iterations, and one that is not done goes back on ff.
 template <bool kBatched>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_wf_extend(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
-                                                  const uint32_t* __restrict__ in, uint32_t cur, int steps) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+k_wf_extend(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q, const uint32_t* __restrict__ in,
+            const uint32_t* __restrict__ n_in, uint32_t cur, int steps) {
     const uint32_t nxt = cur ^ 1u;
-    const uint32_t n_new = Q.counts[kWfFF0 + cur];
-    const uint32_t n = n_new + Q.counts[kWfRF0 + cur];
+    const uint32_t n = *n_in;
     for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
         const uint32_t q = base + __lane_id();
-        const bool active = q < n;
-        const bool resumed = q >= n_new;
-        const uint32_t slot = active ? (resumed ? Q.rf[cur][q - n_new] : in[q]) : 0u;
+        const uint32_t slot = q < n ? in[q] : kNone;
+        const bool active = slot != kNone;
         Flow flow = kFlowExit;
         PathState st;
         SlotRef ref{0u, 0u, 0u};
         EventState ev;
-        uint2 fl = make_uint2(0u, 0u);
+        uint2 fl = make_uint2(0u, kWfNewFlight);
         if (active) {
             load_path(S, slot, st, ref);
-            if (resumed) fl = S.p_flight[slot];
+            fl = S.p_flight[slot];
         }
-        for_each_request<kBatched>(active, ref.req, [&](uint32_t r) {
+        with_wave_request<kBatched>(active, ref.req, [&](uint32_t r) {
             const PtRequest& R = reqs[r];
             Flight f = flight_begin(R.scene.u, st.origin, st.dir);
-            if (resumed) f.tt = bitsf(fl.x), f.i = int(fl.y);
+            if (fl.y != kWfNewFlight) f.tt = bitsf(fl.x), f.i = int(fl.y);
             flow = stage_extend_steps(R.scene, frame_consts(R.scene.u), st, ev, f, f.i + steps);
             fl = make_uint2(fbits(f.tt), uint32_t(f.i));
         });
@@ -527,55 +551,66 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
             S.p_spec[slot] = f4(st.spectral, bitsf(ref.item));
         }
         wave_push(active && flow == kFlowHit, slot, Q.ev, &Q.counts[kWfEV]);
-        wave_push(active && flow == kFlowSuspended, slot, Q.rf[nxt], &Q.counts[kWfRF0 + nxt]);
+        wave_push(active && flow == kFlowSuspended, slot, Q.ff[nxt], &Q.counts[kWfFF0 + nxt]);
         wave_push(active && flow == kFlowExit, slot, Q.end[nxt], &Q.counts[kWfEND0 + nxt]);
     }
 }
 
-// Stage 2: shading frame and light sample.
+// Stage 2: shading frame and light sample, over the `*n_in` entries of `in`
+// (kNone: padding).
 template <bool kBatched>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_wf_shade(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
-                                                 const uint32_t* __restrict__ in) {
-    const uint32_t n = Q.counts[kWfEV];
-    for (uint32_t q = blockIdx.x * 64u + __lane_id(); q < n; q += gridDim.x * 64u) {
-        const uint32_t slot = in[q];
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+k_wf_shade(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q, const uint32_t* __restrict__ in,
+           const uint32_t* __restrict__ n_in) {
+    const uint32_t n = *n_in;
+    for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
+        const uint32_t q = base + __lane_id();
+        const uint32_t slot = q < n ? in[q] : kNone;
+        const bool active = slot != kNone;
         PathState st;  // the stage reads the direction, bounce and random stream
-        const float4 d = S.p_dir[slot];
-        st.dir = xyz(d);
-        st.rng.state = fbits(d.w);
-        st.bounce = fbits(S.p_thr[slot].w) >> 18;
-        const uint32_t req = fbits(S.p_rad[slot].w) >> 16;
         EventState ev;
-        load_event(S, slot, ev);
-        for_each_request<kBatched>(true, req, [&](uint32_t r) {
+        uint32_t req = 0u;
+        if (active) {
+            const float4 d = S.p_dir[slot];
+            st.dir = xyz(d);
+            st.rng.state = fbits(d.w);
+            st.bounce = fbits(S.p_thr[slot].w) >> 18;
+            req = fbits(S.p_rad[slot].w) >> 16;
+            load_event(S, slot, ev);
+        }
+        with_wave_request<kBatched>(active, req, [&](uint32_t r) {
             const PtRequest& R = reqs[r];
             stage_shade(R.scene, frame_consts(R.scene.u), st, ev);
         });
-        store_event(S, slot, ev);
-        S.e_n[slot] = f4(ev.n, 0.0f);
-        S.e_light[slot] = f4(ev.light_dir, 0.0f);
-        S.p_dir[slot].w = bitsf(st.rng.state);
-        if (st.bounce == 0u) {  // guides of the first event (pt.comp:459-462)
-            add4(&S.c1[slot], f4(st.albedo, 0.0f));
-            add4(&S.c2[slot], f4(st.normal, st.depth));
+        if (active) {
+            store_event(S, slot, ev);
+            S.e_n[slot] = f4(ev.n, 0.0f);
+            S.e_light[slot] = f4(ev.light_dir, 0.0f);
+            S.p_dir[slot].w = bitsf(st.rng.state);
+            if (st.bounce == 0u) {  // guides of the first event (pt.comp:459-462)
+                add4(&S.c1[slot], f4(st.albedo, 0.0f));
+                add4(&S.c2[slot], f4(st.normal, st.depth));
+            }
         }
     }
 }
 
-// Stage 3: shadow ray: the events of `in` (new rays), then the suspended rays
-// of sh[cur]; each runs at most `steps` tracking iterations. Finished rays go
-// on to the light-map stage and scatter (queue sc).
+// Stage 3: shadow ray: the events among the `*n_in` entries of `in` (new rays;
+// kNone: padding), then the suspended rays of sh[cur]; each runs at most
+// `steps` tracking iterations. Finished rays go on to the light-map stage and
+// scatter (queue sc).
 template <bool kBatched>
 __global__ void __launch_bounds__(64) k_wf_shadow(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
-                                                  const uint32_t* __restrict__ in, uint32_t cur, int steps) {
+                                                  const uint32_t* __restrict__ in, const uint32_t* __restrict__ n_in,
+                                                  uint32_t cur, int steps) {
     const uint32_t nxt = cur ^ 1u;
-    const uint32_t n_new = Q.counts[kWfEV];
+    const uint32_t n_new = *n_in;
     const uint32_t n = n_new + Q.counts[kWfSH0 + cur];
     for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
         const uint32_t q = base + __lane_id();
-        const bool active = q < n;
         const bool resumed = q >= n_new;
-        const uint32_t slot = active ? (resumed ? Q.sh[cur][q - n_new] : in[q]) : 0u;
+        const uint32_t slot = q < n ? (resumed ? Q.sh[cur][q - n_new] : in[q]) : kNone;
+        const bool active = slot != kNone;
         uint32_t req = 0u;
         Rng rng;
         rng.state = 0u;
@@ -638,7 +673,7 @@ __global__ void __launch_bounds__(64) k_wf_shadow_map(const PtRequest* __restric
 // Stage 4: next-event estimate, next direction, roulette (for the paths whose
 // shadow rays are done).
 template <bool kBatched>
-__global__ void __launch_bounds__(64) k_wf_scatter(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_wf_scatter(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
                                                    const uint32_t* __restrict__ in, uint32_t cur) {
     const uint32_t nxt = cur ^ 1u;
     const uint32_t n = Q.counts[kWfSC];
@@ -667,7 +702,10 @@ __global__ void __launch_bounds__(64) k_wf_scatter(const PtRequest* __restrict__
             }
             more = stage_scatter(R.scene, fc, st, ev) && path_continues(fc, st);
         });
-        if (active) store_path(S, slot, st, ref);
+        if (active) {
+            store_path(S, slot, st, ref);
+            if (more) S.p_flight[slot].y = kWfNewFlight;
+        }
         wave_push(active && more, slot, Q.ff[nxt], &Q.counts[kWfFF0 + nxt]);
         wave_push(active && !more, slot, Q.end[nxt], &Q.counts[kWfEND0 + nxt]);
     }
@@ -745,24 +783,30 @@ void wf_stages(const PtRequest* reqs, const WfSlots& S, const WfQueues& Q, uint3
                bool env_map, hipStream_t st) {
     const WfStageGrid& g = kBatched ? grid.batched : grid.single;
     const uint32_t* ff = Q.ff[cur];
+    const uint32_t* n_ff = Q.counts + kWfFF0 + cur;
     const uint32_t* ev = Q.ev;
+    const uint32_t* n_ev = Q.counts + kWfEV;
     if (sort) {
         uint32_t* hist = Q.bins;
-        hipLaunchKernelGGL(k_wf_bin_count, dim3(grid.sort), dim3(256), 0, st, ff, Q.counts + kWfFF0 + cur, S.req, hist);
-        hipLaunchKernelGGL(k_wf_bin_scatter, dim3(grid.sort), dim3(256), 0, st, ff, Q.counts + kWfFF0 + cur, S.req, hist,
-                           hist + kWfBins, Q.sorted);
+        hipLaunchKernelGGL(k_wf_bin_count, dim3(grid.sort), dim3(256), 0, st, ff, n_ff, S.req, hist);
+        hipLaunchKernelGGL(k_wf_bin_scatter, dim3(grid.sort), dim3(256), 0, st, ff, n_ff, S.req, hist,
+                           hist + kWfBins, Q.sorted, Q.counts + kWfSORTFF);
         ff = Q.sorted;
+        n_ff = Q.counts + kWfSORTFF;
     }
-    hipLaunchKernelGGL(k_wf_extend<kBatched>, dim3(g.extend), dim3(64), 0, st, reqs, S, Q, ff, cur, grid.flight_steps);
+    hipLaunchKernelGGL(k_wf_extend<kBatched>, dim3(g.extend), dim3(64), 0, st, reqs, S, Q, ff, n_ff, cur,
+                       grid.flight_steps);
     if (sort) {
         uint32_t* hist = Q.bins + 2 * kWfBins;
-        hipLaunchKernelGGL(k_wf_bin_count, dim3(grid.sort), dim3(256), 0, st, ev, Q.counts + kWfEV, S.req, hist);
-        hipLaunchKernelGGL(k_wf_bin_scatter, dim3(grid.sort), dim3(256), 0, st, ev, Q.counts + kWfEV, S.req, hist,
-                           hist + kWfBins, Q.sorted);
+        hipLaunchKernelGGL(k_wf_bin_count, dim3(grid.sort), dim3(256), 0, st, ev, n_ev, S.req, hist);
+        hipLaunchKernelGGL(k_wf_bin_scatter, dim3(grid.sort), dim3(256), 0, st, ev, n_ev, S.req, hist,
+                           hist + kWfBins, Q.sorted, Q.counts + kWfSORTEV);
         ev = Q.sorted;
+        n_ev = Q.counts + kWfSORTEV;
     }
-    hipLaunchKernelGGL(k_wf_shade<kBatched>, dim3(g.shade), dim3(64), 0, st, reqs, S, Q, ev);
-    hipLaunchKernelGGL(k_wf_shadow<kBatched>, dim3(g.shadow), dim3(64), 0, st, reqs, S, Q, ev, cur, grid.shadow_steps);
+    hipLaunchKernelGGL(k_wf_shade<kBatched>, dim3(g.shade), dim3(64), 0, st, reqs, S, Q, ev, n_ev);
+    hipLaunchKernelGGL(k_wf_shadow<kBatched>, dim3(g.shadow), dim3(64), 0, st, reqs, S, Q, ev, n_ev, cur,
+                       grid.shadow_steps);
     if (env_map) hipLaunchKernelGGL(k_wf_shadow_map<kBatched>, dim3(g.shadow), dim3(64), 0, st, reqs, S, Q, Q.sc);
     hipLaunchKernelGGL(k_wf_scatter<kBatched>, dim3(g.scatter), dim3(64), 0, st, reqs, S, Q, Q.sc, cur);
 }
@@ -773,7 +817,7 @@ void wf_iteration(const PtRequest* reqs, uint32_t n_reqs, uint32_t total_items, 
                   uint32_t cur, const WfGrid& g, bool env_map, hipStream_t st) {
     if (n_reqs > 1) {
         hipLaunchKernelGGL(k_wf_regen<true>, dim3(g.batched.regen), dim3(64), 0, st, reqs, n_reqs, total_items, S, Q, cur);
-        wf_stages<true>(reqs, S, Q, cur, g, n_reqs <= kWfBins, env_map, st);
+        wf_stages<true>(reqs, S, Q, cur, g, true, env_map, st);  // n_reqs <= kWfBins (WfEngine::run)
     } else {
         hipLaunchKernelGGL(k_wf_regen<false>, dim3(g.single.regen), dim3(64), 0, st, reqs, n_reqs, total_items, S, Q, cur);
         wf_stages<false>(reqs, S, Q, cur, g, false, env_map, st);

@@ -89,11 +89,13 @@ struct WfEngine {
         if (n <= slots) return;
         dfree(slot_mem);
         slot_mem = nullptr, slots = 0;
-        constexpr int kF4 = 17, kQueues = 13;
+        constexpr int kF4 = 17, kQueues = 10;
         const size_t f4_bytes = size_t(n) * sizeof(float4), q_bytes = (size_t(n) * 4 + 255) / 256 * 256;
+        const size_t sorted_bytes = (size_t(wf_sorted_capacity(n)) * 4 + 255) / 256 * 256;
         const size_t req_bytes = (size_t(n) * 2 + 255) / 256 * 256, bin_bytes = 4 * kWfBins * 4;
         const size_t flight_bytes = size_t(n) * sizeof(uint2);
-        slot_mem = dalloc<uint8_t>(kF4 * f4_bytes + flight_bytes + kQueues * q_bytes + req_bytes + 256 + bin_bytes,
+        slot_mem = dalloc<uint8_t>(kF4 * f4_bytes + flight_bytes + kQueues * q_bytes + sorted_bytes + req_bytes + 256 +
+                                       bin_bytes,
                                    "path slots");
         uint8_t* p = slot_mem;
         float4** f4s[kF4] = {&S.p_origin, &S.p_dir, &S.p_thr, &S.p_spec, &S.p_rad, &S.e_idx,
@@ -101,9 +103,10 @@ struct WfEngine {
                              &S.c0,       &S.c1,    &S.c2,    &S.e_map,  &S.e_maplt};
         for (auto f : f4s) *f = reinterpret_cast<float4*>(p), p += f4_bytes;
         S.p_flight = reinterpret_cast<uint2*>(p), p += flight_bytes;
-        uint32_t** qs[kQueues] = {&Q.ff[0], &Q.ff[1], &Q.ev,    &Q.end[0], &Q.end[1], &Q.idle[0], &Q.idle[1],
-                                  &Q.sorted, &Q.rf[0], &Q.rf[1], &Q.sh[0],  &Q.sh[1],  &Q.sc};
+        uint32_t** qs[kQueues] = {&Q.ff[0],   &Q.ff[1],   &Q.ev,    &Q.end[0], &Q.end[1],
+                                  &Q.idle[0], &Q.idle[1], &Q.sh[0], &Q.sh[1],  &Q.sc};
         for (auto q : qs) *q = reinterpret_cast<uint32_t*>(p), p += q_bytes;
+        Q.sorted = reinterpret_cast<uint32_t*>(p), p += sorted_bytes;
         S.req = reinterpret_cast<uint16_t*>(p), p += req_bytes;
         Q.counts = reinterpret_cast<uint32_t*>(p), p += 256;
         Q.bins = reinterpret_cast<uint32_t*>(p);
@@ -151,6 +154,7 @@ struct WfEngine {
             items += reqs[i].n_items;
         }
         if (items >= (1ull << 31)) throw Error{VVR_ELIMIT, "too many paths in one frame (2^31 work items)"};
+        if (n > kWfBins) throw Error{VVR_ELIMIT, "too many requests in one run"};
         reserve(n, partial_f4);
         const uint32_t want = uint32_t(std::min<uint64_t>(max_slots, (items + 63) / 64 * 64));
         reserve_slots(want);
