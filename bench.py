@@ -95,19 +95,25 @@ def copy_probe(nbytes: int) -> None:
             raise RuntimeError(f"copy probe: {what} failed ({rc})")
 
 
-def measured_traffic(args) -> int | None:
+def measured_traffic(args, shape: dict) -> int | None:
     """HBM bytes of one path-tracing pass at the default PT-1 configuration, as
     measured by rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a copy of
-    known size (profiles/pt1_wavefront_hbm_traffic.md); None for other sizes."""
+    known size (profiles/pt1_wavefront_hbm_traffic.md); None for other sizes or
+    another engine shape than the profiled one."""
     if (args.spp, args.size, tuple(args.dims)) != (64, 1024, (512, 512, 400)):
         return None
     prof = ROOT / "profiles" / "pt1_wavefront_hbm_traffic.md"
     if not prof.exists():
         return None
+    traffic, profiled = None, None
     for line in prof.read_text().splitlines():
         if line.startswith("traffic_bytes_per_pass:"):
-            return int(line.split(":")[1])
-    return None
+            traffic = int(line.split(":")[1])
+        if line.startswith("shape:"):
+            profiled = [int(x) for x in line.split(":")[1].split()]
+    if profiled != [shape["slots"], shape["flight_steps"], shape["shadow_steps"]]:
+        return None
+    return traffic
 
 
 def log(msg: str) -> None:
@@ -417,6 +423,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip Batched-32 (plain vs served)")
     ap.add_argument("--no-denoisers", action="store_true", help="skip the per-frame denoiser timings")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default shape instead of the autotuner's (vvr_renderer_autotune)")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the Poisson serving trace (0: skip it)")
     ap.add_argument("--serve-slots", type=int, default=1 << 20, help="path slots of the server in the trace")
@@ -438,6 +446,11 @@ def main() -> int:
     if args.copy_probe:
         copy_probe(args.copy_probe << 20)
         print(f"copy probe: {args.copy_probe} MiB", file=sys.stderr, flush=True)
+    shape = r.pt_shape()
+    if args.autotune:  # untimed: the autotuner's trial frames of the workload itself
+        t_tune = time.perf_counter()
+        shape = r.autotune(scene.params, trials=1)
+        print(f"autotune {time.perf_counter() - t_tune:.1f} s: {shape}", file=sys.stderr, flush=True)
 
     frame = 0
     for _ in range(args.warmup):
@@ -482,11 +495,11 @@ def main() -> int:
         "config": {"workload": f"PT-1 plain: {args.dims[0]}x{args.dims[1]}x{args.dims[2]} int16 phantom, "
                                f"{scene.n_labels} organs (isolate, soft margins), {args.size}^2, {args.spp} spp, "
                                "8 bounces, HG g=0.8, one request at a time",
-                   "mode": "plain"},
+                   "mode": "plain", "shape": shape, "shape_from": "autotune" if args.autotune else "defaults"},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic(args),
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic(args, shape),
                      "kernel": "PT pass (k_wf_* iterations + k_wf_resolve)", "kernel_ms": round(avg_kernel_s * 1e3, 3), "algorithmic_bytes": algo},
     }
     if not args.no_cpu:

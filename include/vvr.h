@@ -208,7 +208,8 @@ int vvr_frame_uniforms(const vvr_volume* volume, const vvr_tf* tf, const vvr_fra
 enum { VVR_ABI_VOLUME = 0, VVR_ABI_CONTROL_POINT = 1, VVR_ABI_TF = 2, VVR_ABI_SOFT_GRID = 3, VVR_ABI_CAMERA = 4,
        VVR_ABI_LABEL_PARAMS = 5, VVR_ABI_LIGHTING = 6, VVR_ABI_CLIPPING = 7, VVR_ABI_PATH_TRACING = 8,
        VVR_ABI_FRAME_PARAMS = 9, VVR_ABI_UNIFORMS = 10, VVR_ABI_SERVER_CONFIG = 11, VVR_ABI_REQUEST = 12,
-       VVR_ABI_COMPLETION = 13, VVR_ABI_SERVER_STATS = 14 };
+       VVR_ABI_COMPLETION = 13, VVR_ABI_SERVER_STATS = 14, VVR_ABI_PT_SHAPE = 15, VVR_ABI_ENV_MAP = 16,
+       VVR_ABI_UNET_CONFIG = 17 };
 /* sizeof of an ABI struct, for binding checks (0 for an unknown id). */
 size_t vvr_abi_sizeof(int which);
 
@@ -352,6 +353,26 @@ int vvr_renderer_set_timing(vvr_renderer* r, int enabled);
 /* Total milliseconds and launches of kernel `which` since timing was enabled. */
 int vvr_renderer_kernel_time(vvr_renderer* r, int which, double* total_ms, uint64_t* launches);
 
+/* ---- shape of the wavefront path tracer, and its autotuner -------------------
+ * No counterpart in the reference, whose megakernel runs one pixel per lane. */
+typedef struct vvr_pt_shape {
+    uint32_t slots;        /* paths in flight, 64 .. 2^26, rounded up to 64 (0: keep) */
+    uint32_t flight_steps; /* tracking iterations a free flight runs per launch before it is
+                              suspended and resumed in the next iteration, 1 .. 4096 (0: keep) */
+    uint32_t shadow_steps; /* the same for a shadow ray (0: keep) */
+} vvr_pt_shape;
+int vvr_renderer_get_pt_shape(vvr_renderer* r, vvr_pt_shape* out);
+/* Frames are the same with any shape; only their speed changes. */
+int vvr_renderer_set_pt_shape(vvr_renderer* r, const vvr_pt_shape* shape);
+/* Times the path-tracing frames of `params` (path_tracing_enabled, the renderer
+ * resized) with candidate shapes and keeps the fastest, written to `chosen`: the
+ * current shape, step pairs (8,16) (12,24) (16,32) (24,48) (32,64) at the current
+ * slots, then 1, 2 and 4 Mi slots with the best pair. Each candidate's time is the
+ * least of `trials` frames (HIP events); a candidate replaces the best so far only
+ * when it is 2 % faster. The trial frames are recorded as frames 1, 2, ... and
+ * accumulate like any others. -1 for a renderer not on the wavefront engine. */
+int vvr_renderer_autotune(vvr_renderer* r, const vvr_frame_params* params, uint32_t trials, vvr_pt_shape* chosen);
+
 /* ---- serving: many requests on one device ------------------------------------
  * No counterpart in the reference, which renders one frame per renderer at a
  * time (the plain loop of offscreen.rs:32-36). A server keeps the volumes of
@@ -369,6 +390,8 @@ typedef struct vvr_server_config {
     uint32_t slots;        /* path slots of the wavefront engine (0: 4,194,304) */
     uint32_t pt_increment; /* paths per pixel per delivered frame of a path-traced request (0: all at once) */
     uint32_t max_requests; /* requests admitted at once, at most 256 (0: 256) */
+    uint32_t flight_steps; /* vvr_pt_shape's, for the server's engine (0: 16) */
+    uint32_t shadow_steps; /* (0: 32) */
 } vvr_server_config;
 
 typedef struct vvr_request {

@@ -21,7 +21,8 @@ VOL_DENSITY, VOL_LABELS, VOL_NORMALS, VOL_MASK, VOL_MAJORANT, VOL_SHADOW, VOL_AO
 KERNEL_MAJORANT, KERNEL_NORMALS, KERNEL_MASK, KERNEL_DVR, KERNEL_PT, KERNEL_LIGHTING, KERNEL_DENOISE, KERNEL_POST = (
     0, 1, 2, 3, 4, 5, 6, 7)
 ABI_STRUCTS = ['Volume', 'ControlPoint', 'TF', 'SoftGrid', 'Camera', 'LabelParams', 'Lighting', 'Clipping',
-               'PathTracing', 'FrameParams', 'Uniforms', 'ServerConfig', 'Request', 'Completion', 'ServerStats']
+               'PathTracing', 'FrameParams', 'Uniforms', 'ServerConfig', 'Request', 'Completion', 'ServerStats',
+               'PtShape', 'EnvMap', 'UnetConfig']
 
 
 class Volume(Structure):
@@ -100,7 +101,12 @@ assert ctypes.sizeof(Uniforms) == 21200
 
 # (name, restype, argtypes) of every exported entry point of include/vvr.h.
 class ServerConfig(Structure):
-    _fields_ = [("hbm_budget", c_uint64), ("slots", c_uint32), ("pt_increment", c_uint32), ("max_requests", c_uint32)]
+    _fields_ = [("hbm_budget", c_uint64), ("slots", c_uint32), ("pt_increment", c_uint32), ("max_requests", c_uint32),
+                ("flight_steps", c_uint32), ("shadow_steps", c_uint32)]
+
+
+class PtShape(Structure):
+    _fields_ = [("slots", c_uint32), ("flight_steps", c_uint32), ("shadow_steps", c_uint32)]
 
 
 class Request(Structure):
@@ -177,6 +183,9 @@ SIGNATURES = [
     ("vvr_renderer_read_uniforms", c_int, [c_void_p, c_void_p, c_size_t]),
     ("vvr_renderer_set_timing", c_int, [c_void_p, c_int]),
     ("vvr_renderer_kernel_time", c_int, [c_void_p, c_int, POINTER(c_double), POINTER(c_uint64)]),
+    ("vvr_renderer_get_pt_shape", c_int, [c_void_p, POINTER(PtShape)]),
+    ("vvr_renderer_set_pt_shape", c_int, [c_void_p, POINTER(PtShape)]),
+    ("vvr_renderer_autotune", c_int, [c_void_p, POINTER(FrameParams), c_uint32, POINTER(PtShape)]),
     ("vvr_phantom_ct", c_int, [POINTER(c_uint32), c_uint64, POINTER(c_int16)]),
     ("vvr_phantom_organs", c_int, [POINTER(c_uint32), c_uint64, c_uint32, POINTER(c_uint8), POINTER(c_uint8),
                                    POINTER(c_uint16)]),

@@ -433,6 +433,9 @@ extern "C" size_t vvr_abi_sizeof(int which) {
         case VVR_ABI_REQUEST: return sizeof(vvr_request);
         case VVR_ABI_COMPLETION: return sizeof(vvr_completion);
         case VVR_ABI_SERVER_STATS: return sizeof(vvr_server_stats);
+        case VVR_ABI_PT_SHAPE: return sizeof(vvr_pt_shape);
+        case VVR_ABI_ENV_MAP: return sizeof(vvr_env_map);
+        case VVR_ABI_UNET_CONFIG: return sizeof(vvr_unet_config);
         default: return 0;
     }
 }

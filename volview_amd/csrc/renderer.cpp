@@ -127,6 +127,69 @@ void set_labels(vvr_renderer* r, const uint8_t* labels, size_t n, const vvr_soft
     r->generation++;
 }
 
+WfEngine& wavefront(vvr_renderer* r) {
+    if (!r->wf) r->wf = std::make_unique<WfEngine>(r->device);
+    return *r->wf;
+}
+
+void record(vvr_renderer* r, uint64_t frame, const vvr_frame_params* p);
+
+// The autotuner: coordinate search over the step bounds, then the slots, each
+// candidate timed by HIP events around whole frames on the renderer's stream.
+void autotune(vvr_renderer* r, const vvr_frame_params* p, uint32_t trials, vvr_pt_shape* chosen) {
+    if (!r || !p || !chosen) throw Error{VVR_EINVAL, "null argument"};
+    if (!p->path_tracing_enabled) throw Error{VVR_EINVAL, "the autotuner times path-traced frames"};
+    if (r->pt_kernel != vvr_renderer::kWavefront) throw Error{VVR_EINVAL, "the renderer is not on the wavefront engine"};
+    r->use_device();
+    trials = std::max(trials, 1u);
+    WfEngine& e = wavefront(r);
+    hipEvent_t a = nullptr, b = nullptr;
+    check(hipEventCreate(&a), "hipEventCreate");
+    check(hipEventCreate(&b), "hipEventCreate");
+    uint64_t frame = 1;
+    auto frame_ms = [&]() {
+        check(hipEventRecord(a, r->stream), "hipEventRecord");
+        record(r, frame++, p);
+        check(hipEventRecord(b, r->stream), "hipEventRecord");
+        check(hipEventSynchronize(b), "autotune frame");
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+        return ms;
+    };
+    auto time_shape = [&](const vvr_pt_shape& s) {
+        e.set_shape(s);
+        float best = 1e30f;
+        for (uint32_t t = 0; t < trials; t++) best = std::min(best, frame_ms());
+        return best;
+    };
+    try {
+        frame_ms();  // derived volumes, slot arrays
+        // The current shape is the incumbent; a candidate replaces the best so far
+        // only when it is 2 % faster, so that timing noise does not move the choice.
+        vvr_pt_shape best = e.shape();
+        float best_ms = time_shape(best);
+        auto consider = [&](const vvr_pt_shape& s) {
+            if (s.slots == best.slots && s.flight_steps == best.flight_steps && s.shadow_steps == best.shadow_steps)
+                return;
+            const float ms = time_shape(s);
+            if (ms < 0.98f * best_ms) best_ms = ms, best = s;
+        };
+        const uint32_t steps[][2] = {{8, 16}, {12, 24}, {16, 32}, {24, 48}, {32, 64}};
+        const uint32_t slots0 = best.slots;
+        for (const auto& st : steps) consider(vvr_pt_shape{slots0, st[0], st[1]});
+        const vvr_pt_shape tuned = best;
+        for (uint32_t slots : {1u << 20, 1u << 21, 1u << 22}) consider(vvr_pt_shape{slots, tuned.flight_steps, tuned.shadow_steps});
+        e.set_shape(best);
+        *chosen = best;
+    } catch (...) {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        throw;
+    }
+    check(hipEventDestroy(a), "hipEventDestroy");
+    check(hipEventDestroy(b), "hipEventDestroy");
+}
+
 void record(vvr_renderer* r, uint64_t frame, const vvr_frame_params* p) {
     if (!r || !p) throw Error{VVR_EINVAL, "null argument"};
     if (!r->hdr) throw Error{VVR_ESTATE, "resize() must be called before record()"};
@@ -155,7 +218,7 @@ void record(vvr_renderer* r, uint64_t frame, const vvr_frame_params* p) {
         const uint32_t spf = std::max(p->path_tracing.samples_per_frame, 1u);
         const uint32_t first = r->accumulated;
         if (r->pt_kernel == vvr_renderer::kWavefront) {
-            if (!r->wf) r->wf = std::make_unique<WfEngine>(r->device);
+            wavefront(r);
             PtRequest q{};
             q.scene = sc;
             q.accum = r->accum, q.albedo = r->albedo, q.normal = r->normal, q.hdr = r->hdr;
@@ -313,6 +376,26 @@ int vvr_renderer_resize(vvr_renderer* r, uint32_t width, uint32_t height) {
 
 int vvr_renderer_record(vvr_renderer* r, uint64_t frame, const vvr_frame_params* params) {
     return guarded([&] { record(r, frame, params); });
+}
+
+int vvr_renderer_get_pt_shape(vvr_renderer* r, vvr_pt_shape* out) {
+    return guarded([&] {
+        if (!r || !out) throw Error{VVR_EINVAL, "null argument"};
+        r->use_device();
+        *out = wavefront(r).shape();
+    });
+}
+
+int vvr_renderer_set_pt_shape(vvr_renderer* r, const vvr_pt_shape* shape) {
+    return guarded([&] {
+        if (!r || !shape) throw Error{VVR_EINVAL, "null argument"};
+        r->use_device();
+        wavefront(r).set_shape(*shape);
+    });
+}
+
+int vvr_renderer_autotune(vvr_renderer* r, const vvr_frame_params* params, uint32_t trials, vvr_pt_shape* chosen) {
+    return guarded([&] { autotune(r, params, trials, chosen); });
 }
 
 int vvr_renderer_synchronize(vvr_renderer* r) {

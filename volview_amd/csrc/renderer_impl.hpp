@@ -130,6 +130,18 @@ struct WfEngine {
         }
     }
 
+    // The shape the next run uses (vvr_pt_shape; zero fields keep theirs).
+    void set_shape(const vvr_pt_shape& s) {
+        if (s.flight_steps > 4096 || s.shadow_steps > 4096) throw Error{VVR_EINVAL, "steps must be 1..4096"};
+        if (s.slots && (s.slots < 64 || s.slots > (1u << 26))) throw Error{VVR_EINVAL, "slots must be 64..2^26"};
+        if (s.slots) max_slots = (s.slots + 63u) / 64u * 64u;
+        if (s.flight_steps) grid.flight_steps = int(s.flight_steps);
+        if (s.shadow_steps) grid.shadow_steps = int(s.shadow_steps);
+    }
+    vvr_pt_shape shape() const {
+        return vvr_pt_shape{max_slots, uint32_t(grid.flight_steps), uint32_t(grid.shadow_steps)};
+    }
+
     // Items of `samples` paths per pixel at `chunk` paths per item.
     static void shape(PtRequest& q, uint32_t width, uint32_t height, uint32_t first, uint32_t samples,
                       uint32_t chunk) {

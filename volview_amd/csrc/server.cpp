@@ -591,7 +591,7 @@ void create_server(vvr_ctx* ctx, const vvr_server_config* cfg, vvr_server** out)
     if (!c.hbm_budget) c.hbm_budget = uint64_t(64) << 30;
     if (!c.max_requests || c.max_requests > kMaxJobs) c.max_requests = kMaxJobs;
     s->cfg = c;
-    s->engine.max_slots = c.slots ? c.slots : s->engine.max_slots;
+    s->engine.set_shape(vvr_pt_shape{c.slots, c.flight_steps, c.shadow_steps});
     check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
     s->engine.reserve_slots(s->engine.max_slots);
     s->engine.reserve(1, 1);

@@ -10,8 +10,8 @@ from ctypes import POINTER, byref, c_double, c_float, c_int16, c_uint8, c_uint16
 import numpy as np
 
 from . import _abi
-from ._abi import (Camera, ControlPoint, EnvMap, FrameParams, LabelParams, Lighting, PathTracing, SoftGrid, TF, Uniforms,
-                   Volume, check, load)
+from ._abi import (Camera, ControlPoint, EnvMap, FrameParams, LabelParams, Lighting, PathTracing, PtShape, SoftGrid, TF,
+                   Uniforms, Volume, check, load)
 
 
 def _fptr(a: np.ndarray):
@@ -279,6 +279,23 @@ class Renderer:
         u = Uniforms()
         check(self._lib.vvr_renderer_read_uniforms(self.handle, byref(u), ctypes.sizeof(u)))
         return u
+
+    def pt_shape(self) -> dict:
+        """The wavefront path tracer's shape: slots, flight_steps, shadow_steps."""
+        sh = PtShape()
+        check(self._lib.vvr_renderer_get_pt_shape(self.handle, byref(sh)))
+        return {"slots": sh.slots, "flight_steps": sh.flight_steps, "shadow_steps": sh.shadow_steps}
+
+    def set_pt_shape(self, slots: int = 0, flight_steps: int = 0, shadow_steps: int = 0):
+        """Sets the shape's non-zero fields (frames are the same with any shape)."""
+        check(self._lib.vvr_renderer_set_pt_shape(self.handle, byref(PtShape(slots, flight_steps, shadow_steps))))
+
+    def autotune(self, params: FrameParams, trials: int = 1) -> dict:
+        """vvr_renderer_autotune: times `params`' frames over candidate shapes and
+        keeps the fastest; returns it. The trial frames accumulate."""
+        sh = PtShape()
+        check(self._lib.vvr_renderer_autotune(self.handle, byref(params), trials, byref(sh)))
+        return {"slots": sh.slots, "flight_steps": sh.flight_steps, "shadow_steps": sh.shadow_steps}
 
     def set_timing(self, enabled: bool):
         check(self._lib.vvr_renderer_set_timing(self.handle, 1 if enabled else 0))
