@@ -4,9 +4,10 @@
 //   k_bloom_down4   quarter-resolution Karis average of 4x4 HDR pixels, with
 //                   the covered share (pass 0)
 //   k_meter         auto exposure: the covered pixels' mean log luminance,
-//                   adapted 15 % a frame in log space (pass 1). One wave: lane t
-//                   sums texels t, t + 64, ... in order, then a tree, as the
-//                   reference's 64-invocation group does; loads run 8 ahead.
+//                   adapted 15 % a frame in log space (pass 1). Lane t of one
+//                   wave sums texels t, t + 64, ... in order, then a tree, as the
+//                   reference's 64-invocation group does; the whole workgroup
+//                   computes the terms into LDS ahead of the sums.
 //   k_blur9         9-tap Gaussian along x or y (passes 2, 3, 5, 6)
 //   k_bloom_down2   eighth-resolution 2x2 average (pass 4)
 //   k_tonemap       exposure, bloom mix, vignette, ACES / AgX, background,
@@ -57,34 +58,47 @@ __global__ void __launch_bounds__(64) k_bloom_down4(const uint2* __restrict__ hd
     store_h4(a, size_t(y) * qw + x, sr / weights, sg / weights, sb / weights, coverage / 16.0f);
 }
 
-// bloom.comp:60-98
-__global__ void __launch_bounds__(64) k_meter(const uint2* __restrict__ a, int n, float* __restrict__ exposure) {
-    __shared__ float ls[64], ws[64];
+// bloom.comp:60-98. The sums keep the reference's order: lane t of one 64-lane
+// group adds texels t, t + 64, ... in turn. The terms (coverage x log luminance)
+// do not depend on that order, so the workgroup's 16 waves compute a chunk of
+// them into LDS, and the first wave then adds its lanes' runs from LDS.
+constexpr int kMeterThreads = 1024, kMeterPer = 8, kMeterChunk = kMeterThreads * kMeterPer;
+__global__ void __launch_bounds__(kMeterThreads) k_meter(const uint2* __restrict__ a, int n,
+                                                         float* __restrict__ exposure) {
+    __shared__ float2 terms[kMeterChunk];  // 64 KiB
     const int t = int(threadIdx.x);
     float s = 0.0f, wsum = 0.0f;
-    constexpr int kAhead = 8;
-    for (int i0 = t; i0 < n; i0 += 64 * kAhead) {
-        float4 c[kAhead];
+    for (int c0 = 0; c0 < n; c0 += kMeterChunk) {
 #pragma unroll
-        for (int k = 0; k < kAhead; k++) {
-            const int i = i0 + 64 * k;
-            c[k] = i < n ? load_h4(a, size_t(i)) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        }
-#pragma unroll
-        for (int k = 0; k < kAhead; k++)
-            if (i0 + 64 * k < n) {
-                s += c[k].w * logf(fmaxf(lumf(c[k].x, c[k].y, c[k].z), 1e-4f));
-                wsum += c[k].w;
+        for (int k = 0; k < kMeterPer; k++) {
+            const int j = k * kMeterThreads + t;
+            float2 v = make_float2(0.0f, 0.0f);
+            if (c0 + j < n) {
+                const float4 c = load_h4(a, size_t(c0 + j));
+                const float term = c.w * logf(fmaxf(lumf(c.x, c.y, c.z), 1e-4f));
+                v = make_float2(term, c.w);
             }
-    }
-    ls[t] = s, ws[t] = wsum;
-    __syncthreads();
-    for (int stride = 32; stride > 0; stride >>= 1) {
-        if (t < stride) ls[t] += ls[t + stride], ws[t] += ws[t + stride];
+            terms[j] = v;
+        }
+        __syncthreads();
+        if (t < 64) {
+            const int m = min(kMeterChunk, n - c0);
+            for (int j = t; j < m; j += 64) {
+                const float2 v = terms[j];
+                s += v.x;
+                wsum += v.y;
+            }
+        }
         __syncthreads();
     }
+    if (t >= 64) return;
+    // The tree within the first wave: lane t adds lane t + stride's sums.
+    for (int stride = 32; stride > 0; stride >>= 1) {
+        const float o = __shfl_down(s, stride), ow = __shfl_down(wsum, stride);
+        if (t < stride) s += o, wsum += ow;
+    }
     if (t == 0) {
-        const float target = ws[0] > 1.0f ? fminf(fmaxf(0.24f / expf(ls[0] / ws[0]), 0.25f), 4.0f) : 1.0f;
+        const float target = wsum > 1.0f ? fminf(fmaxf(0.24f / expf(s / wsum), 0.25f), 4.0f) : 1.0f;
         const float prev = *exposure;
         *exposure = prev <= 0.0f ? target : expf(mixf(logf(prev), logf(target), 0.15f));
     }
@@ -234,7 +248,7 @@ void launch_post(const vvr_uniforms* du, const uint16_t* hdr, uint32_t width, ui
     uint2* D = C + size_t(ew) * eh;
     const uint2* H = reinterpret_cast<const uint2*>(hdr);
     hipLaunchKernelGGL(k_bloom_down4, tiles8(qw, qh), dim3(64), 0, st, H, w, h, A, qw, qh);
-    hipLaunchKernelGGL(k_meter, dim3(1), dim3(64), 0, st, A, qw * qh, exposure);
+    hipLaunchKernelGGL(k_meter, dim3(1), dim3(kMeterThreads), 0, st, A, qw * qh, exposure);
     hipLaunchKernelGGL(k_blur9, tiles8(qw, qh), dim3(64), 0, st, A, B, qw, qh, 1);
     hipLaunchKernelGGL(k_blur9, tiles8(qw, qh), dim3(64), 0, st, B, A, qw, qh, 0);
     hipLaunchKernelGGL(k_bloom_down2, tiles8(ew, eh), dim3(64), 0, st, A, qw, qh, C, ew, eh);
