@@ -61,21 +61,31 @@ __global__ void __launch_bounds__(256) k_conv3x3(UnetConv c) {
 #pragma unroll
     for (int b = 0; b < NCOB; b++) acc[b] = f32x16{};
     const int nb = min(NCOB, c.co_blocks - cob0);
-    for (int tap = 0; tap < 9; tap++) {
-        const int yy = py + tap / 3 - 1, xx = px + tap % 3 - 1;
+    // K steps of 16 (tap-major, then input channels), each step's fragments
+    // loaded one step ahead so that their latency overlaps the MFMAs.
+    const int csteps = c.cin >> 4, total = 9 * csteps;
+    int lt = 0, lc = 0;  // tap and channel step of the next load
+    bf16x8 a_nx, w_nx[NCOB];
+    auto load_step = [&]() {
+        const int yy = py + lt / 3 - 1, xx = px + lt % 3 - 1;
         const bool inside = p < npix && yy >= 0 && yy < c.h && xx >= 0 && xx < c.w;
-        const uint16_t* src = c.in + (size_t((pn * c.h + yy) * c.w + xx)) * c.in_cs + c.in_co + 8 * hi;
-        for (int ci = 0; ci < c.cin; ci += 16) {
-            const bf16x8 a = inside ? load8(src + ci) : zero8();
+        const int ci = 16 * lc + 8 * hi;
+        a_nx = inside ? load8(c.in + size_t((pn * c.h + yy) * c.w + xx) * c.in_cs + c.in_co + ci) : zero8();
 #pragma unroll
-            for (int b = 0; b < NCOB; b++) {
-                if (b < nb) {
-                    const int co = (cob0 + b) * 32 + col;
-                    const bf16x8 wv = load8(c.wts + (size_t(co) * 9 + tap) * c.cin + ci + 8 * hi);
-                    acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, wv, acc[b], 0, 0, 0);
-                }
-            }
-        }
+        for (int b = 0; b < NCOB; b++)
+            w_nx[b] = b < nb ? load8(c.wts + (size_t((cob0 + b) * 32 + col) * 9 + lt) * c.cin + ci) : zero8();
+        if (++lc == csteps) lc = 0, lt++;
+    };
+    load_step();
+    for (int s = 0; s < total; s++) {
+        const bf16x8 a = a_nx;
+        bf16x8 w[NCOB];
+#pragma unroll
+        for (int b = 0; b < NCOB; b++) w[b] = w_nx[b];
+        if (s + 1 < total) load_step();
+#pragma unroll
+        for (int b = 0; b < NCOB; b++)
+            if (b < nb) acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, w[b], acc[b], 0, 0, 0);
     }
     // Epilogue: lane holds output channel (cob0 + b) * 32 + col of 16 pixels.
     const int pbase = int(blockIdx.x) * 128 + wave * 32;
@@ -153,36 +163,63 @@ __global__ void k_wtrans(const uint16_t* __restrict__ w, int cout, int cin, uint
     wt[i] = (ci < cin && co < cout) ? w[(size_t(co) * 9 + (8 - tap)) * cin + ci] : uint16_t(0);
 }
 
+// The elementwise passes work on 8 channels (16 bytes) per thread: channel
+// counts, strides and offsets are multiples of 16.
+struct Vec8 {
+    float v[8];
+};
+__device__ __forceinline__ Vec8 ld8(const uint16_t* p) {
+    const uint4 u = *reinterpret_cast<const uint4*>(p);
+    const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+    Vec8 r;
+#pragma unroll
+    for (int k = 0; k < 4; k++) r.v[2 * k] = bf2f(uint16_t(w[k] & 0xffffu)), r.v[2 * k + 1] = bf2f(uint16_t(w[k] >> 16));
+    return r;
+}
+__device__ __forceinline__ void st8(uint16_t* p, const Vec8& r) {
+    uint32_t w[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) w[k] = uint32_t(f2bf(r.v[2 * k])) | (uint32_t(f2bf(r.v[2 * k + 1])) << 16);
+    *reinterpret_cast<uint4*>(p) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
 // Average 2x2 pooling: out (n, h/2, w/2) from in (n, h, w), channels c.
 __global__ void k_pool2(UnetEw e) {
     const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
-    const int ow = e.w / 2, oh = e.h / 2;
-    if (i >= e.n * oh * ow * e.c) return;
-    const int ch = i % e.c, q = i / e.c, x = q % ow, y = (q / ow) % oh, n = q / (ow * oh);
-    float s = 0.0f;
+    const int ow = e.w / 2, oh = e.h / 2, c8 = e.c >> 3;
+    if (i >= e.n * oh * ow * c8) return;
+    const int ch = (i % c8) * 8, q = i / c8, x = q % ow, y = (q / ow) % oh, n = q / (ow * oh);
+    Vec8 s{};
     for (int k = 0; k < 4; k++) {
         const size_t src = size_t((n * e.h + 2 * y + (k >> 1)) * e.w + 2 * x + (k & 1));
-        s += bf2f(e.a[src * e.a_cs + e.a_co + ch]);
+        const Vec8 t = ld8(e.a + src * e.a_cs + e.a_co + ch);
+#pragma unroll
+        for (int j = 0; j < 8; j++) s.v[j] += t.v[j];
     }
-    e.out[size_t(q) * e.out_cs + e.out_co + ch] = f2bf(0.25f * s);
+#pragma unroll
+    for (int j = 0; j < 8; j++) s.v[j] *= 0.25f;
+    st8(e.out + size_t(q) * e.out_cs + e.out_co + ch, s);
 }
 
 // Nearest 2x upsampling: out (n, 2h, 2w) from in (n, h, w) (h, w: the input's).
 __global__ void k_up2(UnetEw e) {
     const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
-    const int ow = 2 * e.w, oh = 2 * e.h;
-    if (i >= e.n * oh * ow * e.c) return;
-    const int ch = i % e.c, q = i / e.c, x = q % ow, y = (q / ow) % oh, n = q / (ow * oh);
+    const int ow = 2 * e.w, oh = 2 * e.h, c8 = e.c >> 3;
+    if (i >= e.n * oh * ow * c8) return;
+    const int ch = (i % c8) * 8, q = i / c8, x = q % ow, y = (q / ow) % oh, n = q / (ow * oh);
     const size_t src = size_t((n * e.h + y / 2) * e.w + x / 2);
-    e.out[size_t(q) * e.out_cs + e.out_co + ch] = e.a[src * e.a_cs + e.a_co + ch];
+    *reinterpret_cast<uint4*>(e.out + size_t(q) * e.out_cs + e.out_co + ch) =
+        *reinterpret_cast<const uint4*>(e.a + src * e.a_cs + e.a_co + ch);
 }
 
 // Channel-slice copy at one resolution.
 __global__ void k_copy(UnetEw e) {
     const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= e.n * e.h * e.w * e.c) return;
-    const int ch = i % e.c, q = i / e.c;
-    e.out[size_t(q) * e.out_cs + e.out_co + ch] = e.a[size_t(q) * e.a_cs + e.a_co + ch];
+    const int c8 = e.c >> 3;
+    if (i >= e.n * e.h * e.w * c8) return;
+    const int ch = (i % c8) * 8, q = i / c8;
+    *reinterpret_cast<uint4*>(e.out + size_t(q) * e.out_cs + e.out_co + ch) =
+        *reinterpret_cast<const uint4*>(e.a + size_t(q) * e.a_cs + e.a_co + ch);
 }
 
 // Gradient at a ReLU'd tensor T (n, h, w): mask(T > 0) x (a at the same pixel
@@ -190,22 +227,32 @@ __global__ void k_copy(UnetEw e) {
 // children at double resolution), each term optional.
 __global__ void k_grad_combine(UnetEw e) {
     const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= e.n * e.h * e.w * e.c) return;
-    const int ch = i % e.c, q = i / e.c, x = q % e.w, y = (q / e.w) % e.h, n = q / (e.w * e.h);
-    float s = 0.0f;
-    if (e.a) s += bf2f(e.a[size_t(q) * e.a_cs + e.a_co + ch]);
+    const int c8 = e.c >> 3;
+    if (i >= e.n * e.h * e.w * c8) return;
+    const int ch = (i % c8) * 8, q = i / c8, x = q % e.w, y = (q / e.w) % e.h, n = q / (e.w * e.h);
+    Vec8 s{};
+    if (e.a) s = ld8(e.a + size_t(q) * e.a_cs + e.a_co + ch);
     if (e.b) {
         const size_t src = size_t((n * (e.h / 2) + y / 2) * (e.w / 2) + x / 2);
-        s += 0.25f * bf2f(e.b[src * e.b_cs + e.b_co + ch]);
+        const Vec8 t = ld8(e.b + src * e.b_cs + e.b_co + ch);
+#pragma unroll
+        for (int j = 0; j < 8; j++) s.v[j] += 0.25f * t.v[j];
     }
     if (e.c_in) {
         for (int k = 0; k < 4; k++) {
             const size_t src = size_t((n * 2 * e.h + 2 * y + (k >> 1)) * (2 * e.w) + 2 * x + (k & 1));
-            s += bf2f(e.c_in[src * e.c_cs + e.c_co + ch]);
+            const Vec8 t = ld8(e.c_in + src * e.c_cs + e.c_co + ch);
+#pragma unroll
+            for (int j = 0; j < 8; j++) s.v[j] += t.v[j];
         }
     }
-    if (e.mask && bf2f(e.mask[size_t(q) * e.mask_cs + e.mask_co + ch]) <= 0.0f) s = 0.0f;
-    e.out[size_t(q) * e.out_cs + e.out_co + ch] = f2bf(s);
+    if (e.mask) {
+        const Vec8 m = ld8(e.mask + size_t(q) * e.mask_cs + e.mask_co + ch);
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (m.v[j] <= 0.0f) s.v[j] = 0.0f;
+    }
+    st8(e.out + size_t(q) * e.out_cs + e.out_co + ch, s);
 }
 
 // L1 loss in the log domain over 3 channels and its gradient into a 16-channel
@@ -320,11 +367,15 @@ void launch_conv3x3(const UnetConv& c, hipStream_t st) {
     const int npix = c.n * c.h * c.w;
     UnetConv k = c;
     k.co_blocks = (c.cout + 31) / 32;
-    const int groups = (k.co_blocks + 3) / 4;
-    const dim3 grid((npix + 127) / 128, groups);
-    if (k.co_blocks == 1) hipLaunchKernelGGL(k_conv3x3<1>, dim3(grid.x, k.co_blocks), dim3(256), 0, st, k);
-    else if (k.co_blocks == 2) hipLaunchKernelGGL(k_conv3x3<2>, dim3(grid.x, 1), dim3(256), 0, st, k);
-    else if (k.co_blocks == 3) hipLaunchKernelGGL(k_conv3x3<3>, dim3(grid.x, 1), dim3(256), 0, st, k);
+    // Output blocks per wave: up to 4 (A fragments reused across them), fewer
+    // when that leaves under 2048 waves (deep levels: latency wants waves).
+    const int pix_waves = (npix + 31) / 32;
+    int ncob = std::min(4, k.co_blocks);
+    while (ncob > 1 && pix_waves * ((k.co_blocks + ncob - 1) / ncob) < 2048) ncob--;
+    const dim3 grid((npix + 127) / 128, (k.co_blocks + ncob - 1) / ncob);
+    if (ncob == 1) hipLaunchKernelGGL(k_conv3x3<1>, grid, dim3(256), 0, st, k);
+    else if (ncob == 2) hipLaunchKernelGGL(k_conv3x3<2>, grid, dim3(256), 0, st, k);
+    else if (ncob == 3) hipLaunchKernelGGL(k_conv3x3<3>, grid, dim3(256), 0, st, k);
     else hipLaunchKernelGGL(k_conv3x3<4>, grid, dim3(256), 0, st, k);
 }
 
@@ -344,9 +395,9 @@ void launch_wtrans(const uint16_t* w, int cout, int cin, uint16_t* wt, hipStream
 }
 
 void launch_unet_ew(int op, const UnetEw& e, hipStream_t st) {
-    long n = long(e.n) * e.h * e.w * e.c;
-    if (op == kUnetPool) n = long(e.n) * (e.h / 2) * (e.w / 2) * e.c;
-    if (op == kUnetUp) n = long(e.n) * (2 * e.h) * (2 * e.w) * e.c;
+    long n = long(e.n) * e.h * e.w * (e.c / 8);  // threads: 8 channels each
+    if (op == kUnetPool) n = long(e.n) * (e.h / 2) * (e.w / 2) * (e.c / 8);
+    if (op == kUnetUp) n = long(e.n) * (2 * e.h) * (2 * e.w) * (e.c / 8);
     auto k = op == kUnetPool ? k_pool2 : op == kUnetUp ? k_up2 : op == kUnetCopy ? k_copy : k_grad_combine;
     hipLaunchKernelGGL(k, dim3(blocks(n)), dim3(256), 0, st, e);
 }
