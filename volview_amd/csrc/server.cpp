@@ -128,10 +128,16 @@ struct vvr_server {
     PtRequest* d_table = nullptr;
     PtRequest* h_table = nullptr;  // pinned staging, one per table entry
     uint32_t* d_ctr = nullptr;     // per entry, kWfCounterStride apart: head block, then done block
-    uint32_t* h_ctr = nullptr;     // pinned read-back of both blocks
-    hipEvent_t ctr_ev = nullptr;
-    uint64_t batch_seq = 0;        // wavefront batches launched
-    uint64_t snap_seq = 0;         // the batch the read-back follows
+    // Pinned read-backs of both counter blocks, two in turn: the pass after a
+    // batch is queued before the host waits for the batch's counters.
+    uint32_t* h_ctr_buf[2] = {nullptr, nullptr};
+    hipEvent_t ctr_ev[2] = {nullptr, nullptr};
+    uint64_t ctr_seq[2] = {0, 0};    // the batch each read-back follows
+    bool ctr_pending[2] = {false, false};
+    int ctr_next = 0;
+    const uint32_t* h_ctr = nullptr;  // the newest landed read-back
+    uint64_t batch_seq = 0;           // wavefront batches launched
+    uint64_t snap_seq = 0;            // the batch h_ctr follows
     uint32_t cur = 0, idle_in = 0;
     bool revive = false;
     Entry table[kTable];
@@ -551,17 +557,31 @@ void vvr_server::loop() {
             if (!fresh_dvr.empty()) launch_dvr(fresh_dvr);
             stop_cancelled();
             if (entries_in_use > 0) {
-                // One short batch per pass, waited for before the next pass: a
-                // ray-marched request admitted meanwhile waits for at most this
-                // batch (the host's turn-around between batches is ~0.1 ms).
+                // One short batch per pass, queued before the host waits for the
+                // previous batch's counters, so the device runs this batch while
+                // the host acts on the last one (ray-marched requests admitted
+                // meanwhile wait for at most two batches).
                 pt_batch();
-                check(hipMemcpyAsync(h_ctr, d_ctr, 2 * kTable * kWfCounterStride * 4, hipMemcpyDeviceToHost, stream),
+                const int b = ctr_next;
+                ctr_next ^= 1;
+                check(hipMemcpyAsync(h_ctr_buf[b], d_ctr, 2 * kTable * kWfCounterStride * 4, hipMemcpyDeviceToHost,
+                                     stream),
                       "counter read-back");
-                snap_seq = batch_seq - 1;
-                check(hipEventRecord(ctr_ev, stream), "hipEventRecord");
-                check(hipEventSynchronize(ctr_ev), "counter read-back");
+                check(hipEventRecord(ctr_ev[b], stream), "hipEventRecord");
+                ctr_seq[b] = batch_seq - 1;
+                ctr_pending[b] = true;
+                const int w = ctr_pending[b ^ 1] ? (b ^ 1) : b;
+                check(hipEventSynchronize(ctr_ev[w]), "counter read-back");
+                ctr_pending[w] = false;
+                h_ctr = h_ctr_buf[w];
+                snap_seq = ctr_seq[w];
                 check_entries();
             } else {
+                for (int k = 0; k < 2; k++)
+                    if (ctr_pending[k]) {  // no entries left: the read-back only needs to land
+                        check(hipEventSynchronize(ctr_ev[k]), "counter read-back");
+                        ctr_pending[k] = false;
+                    }
                 bool reading = false;
                 {
                     std::lock_guard<std::mutex> lk(mu);
@@ -599,8 +619,12 @@ void create_server(vvr_ctx* ctx, const vvr_server_config* cfg, vvr_server** out)
     s->h_table = halloc<PtRequest>(kTable, "request table staging");
     s->d_ctr = dalloc<uint32_t>(2 * kTable * kWfCounterStride, "request counters");
     check(hipMemsetAsync(s->d_ctr, 0, 2 * kTable * kWfCounterStride * 4, s->stream), "counter reset");
-    s->h_ctr = halloc<uint32_t>(2 * kTable * kWfCounterStride, "counter read-back");
-    check(hipEventCreateWithFlags(&s->ctr_ev, hipEventDisableTiming), "hipEventCreate");
+    for (int k = 0; k < 2; k++) {
+        s->h_ctr_buf[k] = halloc<uint32_t>(2 * kTable * kWfCounterStride, "counter read-back");
+        std::memset(s->h_ctr_buf[k], 0, 2 * kTable * kWfCounterStride * 4);
+        check(hipEventCreateWithFlags(&s->ctr_ev[k], hipEventDisableTiming), "hipEventCreate");
+    }
+    s->h_ctr = s->h_ctr_buf[0];
     for (uint32_t k = 0; k < kDvrRing; k++) {
         s->d_jobs[k] = dalloc<DvrJob>(64, "ray-marching jobs");
         s->h_jobs[k] = halloc<DvrJob>(64, "ray-marching job staging");
@@ -763,8 +787,9 @@ void vvr_server_destroy(vvr_server* s) {
         if (j->read_ev) (void)hipEventDestroy(j->read_ev);
     }
     dfree(s->d_table), dfree(s->d_ctr), dfree(s->d_uniforms);
-    hfree(s->h_table), hfree(s->h_uniforms), hfree(s->h_ctr);
-    if (s->ctr_ev) (void)hipEventDestroy(s->ctr_ev);
+    hfree(s->h_table), hfree(s->h_uniforms), hfree(s->h_ctr_buf[0]), hfree(s->h_ctr_buf[1]);
+    for (auto& e : s->ctr_ev)
+        if (e) (void)hipEventDestroy(e);
     for (uint32_t k = 0; k < kDvrRing; k++) {
         dfree(s->d_jobs[k]);
         hfree(s->h_jobs[k]);
