@@ -360,6 +360,9 @@ typedef struct vvr_pt_shape {
     uint32_t flight_steps; /* tracking iterations a free flight runs per launch before it is
                               suspended and resumed in the next iteration, 1 .. 4096 (0: keep) */
     uint32_t shadow_steps; /* the same for a shadow ray (0: keep) */
+    uint32_t tail_paths;   /* once every work item of a frame is handed out and fewer paths than
+                              this are in flight, they run to their ends in one launch
+                              (UINT32_MAX: never; 0: keep) */
 } vvr_pt_shape;
 int vvr_renderer_get_pt_shape(vvr_renderer* r, vvr_pt_shape* out);
 /* Frames are the same with any shape; only their speed changes. */
@@ -367,7 +370,8 @@ int vvr_renderer_set_pt_shape(vvr_renderer* r, const vvr_pt_shape* shape);
 /* Times the path-tracing frames of `params` (path_tracing_enabled, the renderer
  * resized) with candidate shapes and keeps the fastest, written to `chosen`: the
  * current shape, step pairs (8,16) (12,24) (16,32) (24,48) (32,64) at the current
- * slots, then 1, 2 and 4 Mi slots with the best pair. Each candidate's time is the
+ * slots, then 1, 2 and 4 Mi slots with the best pair, then tail thresholds of 32,
+ * 64, 128 and 256 Ki paths and none with the best so far. Each candidate's time is the
  * least of `trials` frames (HIP events); a candidate replaces the best so far only
  * when it is 2 % faster. The trial frames are recorded as frames 1, 2, ... and
  * accumulate like any others. -1 for a renderer not on the wavefront engine. */

@@ -1,8 +1,9 @@
 """The wavefront path tracer's shape (vvr_pt_shape) and its autotuner, on the GPU.
 
 - Frames are bit-identical whatever the shape: tracking loops suspended after
-  1-3 iterations, or never, and a few hundred or a million slots, in the
-  renderer and in the server.
+  1-7 iterations, or never, a few hundred or a million slots, and the frame's
+  tail run in one launch from the first iteration on (every path of a small
+  frame), late, or never, in the renderer and in the server.
 - The autotuner picks one of its candidates, leaves it set, and the renderer's
   frames afterwards equal a default renderer's.
 - Shapes out of range are rejected.
@@ -40,15 +41,18 @@ def test_frames_equal_for_every_shape(ctx):
     p = workloads.pt_params(r, n, 8, bounces=8, camera_index=2)
     w, h = 48, 40
     if not any(k in os.environ for k in ("VVR_PT_SLOTS", "VVR_PT_FLIGHT_STEPS", "VVR_PT_SHADOW_STEPS")):
-        assert r.pt_shape() == {"slots": 1 << 22, "flight_steps": 16, "shadow_steps": 32}
+        assert r.pt_shape() == {"slots": 1 << 22, "flight_steps": 16, "shadow_steps": 32, "tail_paths": 1 << 16}
     want = frame(r, p, w, h)
-    for shape in ({"slots": 192, "flight_steps": 1, "shadow_steps": 1}, {"flight_steps": 3, "shadow_steps": 2},
-                  {"slots": 1 << 20, "flight_steps": 4096, "shadow_steps": 4096}):
+    never = Renderer.TAIL_NEVER
+    for shape in ({"slots": 192, "flight_steps": 1, "shadow_steps": 1, "tail_paths": never},
+                  {"flight_steps": 3, "shadow_steps": 2},
+                  {"slots": 1 << 20, "flight_steps": 4096, "shadow_steps": 4096, "tail_paths": 64},
+                  {"flight_steps": 5, "shadow_steps": 7, "tail_paths": 1 << 30}):
         r.set_pt_shape(**shape)
         got = frame(r, p, w, h)
         for a, b in zip(got, want):
             assert np.array_equal(a, b), shape
-    assert r.pt_shape() == {"slots": 1 << 20, "flight_steps": 4096, "shadow_steps": 4096}
+    assert r.pt_shape() == {"slots": 1 << 20, "flight_steps": 5, "shadow_steps": 7, "tail_paths": 1 << 30}
     assert np.abs(want[0][..., :3]).max() > 0
 
 
@@ -61,6 +65,7 @@ def test_autotune_keeps_a_candidate_and_frames(ctx):
     assert (chosen["flight_steps"], chosen["shadow_steps"]) in {(8, 16), (12, 24), (16, 32), (24, 48), (32, 64)}
     # (the default shape is the incumbent, and is one of the candidates)
     assert chosen["slots"] in {1 << 20, 1 << 21, 1 << 22}
+    assert chosen["tail_paths"] in {1 << 15, 1 << 16, 1 << 17, 1 << 18, Renderer.TAIL_NEVER}
     assert r.pt_shape() == chosen
     assert r.accumulated_samples() > 4  # the trial frames accumulated
     r2, _ = scene(ctx)

@@ -106,7 +106,7 @@ class ServerConfig(Structure):
 
 
 class PtShape(Structure):
-    _fields_ = [("slots", c_uint32), ("flight_steps", c_uint32), ("shadow_steps", c_uint32)]
+    _fields_ = [("slots", c_uint32), ("flight_steps", c_uint32), ("shadow_steps", c_uint32), ("tail_paths", c_uint32)]
 
 
 class Request(Structure):

@@ -711,6 +711,98 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     }
 }
 
+// The tail of a single-request frame, once every work item is handed out and
+// few paths remain: each path still in flight runs to its end in one launch,
+// a lane per slot, then the rest of its chunk. The stages run in the order the
+// iterations would run them, with the same functions, so every path draws the
+// same random numbers and the chunk sums get the same additions in the same
+// order: the frame is the same as with iterations to the end. Entries: the
+// ended paths of end[cur] (regen's turn), the suspended shadow rays of sh[cur]
+// and the flights of ff[cur] (new or suspended).
+__global__ void __launch_bounds__(64) k_wf_finish(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
+                                                  uint32_t cur) {
+    const PtRequest& R = reqs[0];
+    const DevScene& sc = R.scene;
+    const FrameConsts fc = frame_consts(sc.u);
+    const uint32_t n_end = Q.counts[kWfEND0 + cur], n_sh = Q.counts[kWfSH0 + cur];
+    const uint32_t n = n_end + n_sh + Q.counts[kWfFF0 + cur];
+    enum { kFlight, kShadow, kScatter, kEnded };
+    for (uint32_t q = blockIdx.x * 64u + __lane_id(); q < n; q += gridDim.x * 64u) {
+        int phase;
+        uint32_t slot;
+        if (q < n_end) slot = Q.end[cur][q], phase = kEnded;
+        else if (q < n_end + n_sh) slot = Q.sh[cur][q - n_end], phase = kShadow;
+        else slot = Q.ff[cur][q - n_end - n_sh], phase = kFlight;
+        PathState st;
+        SlotRef ref;
+        load_path(S, slot, st, ref);
+        if (ref.item == kNone) continue;  // an idle slot
+        float4 c0 = S.c0[slot], c1 = S.c1[slot], c2 = S.c2[slot];
+        bool resumed = phase == kFlight && S.p_flight[slot].y != kWfNewFlight;
+        EventState ev;
+        if (phase == kShadow) {  // as k_wf_shadow, without a bound
+            load_event(S, slot, ev);
+            ev.n = xyz(S.e_n[slot]);
+            ev.light_dir = xyz(S.e_light[slot]);
+            V3 result = xyz(S.e_lt[slot]);
+            const uint2 fl = S.p_flight[slot];
+            Flight f = flight_begin(sc.u, ev.idx, ev.light_dir);
+            f.tt = bitsf(fl.x), f.i = int(fl.y);
+            transmittance_steps(sc, f, result, st.rng, 4096);
+            ev.light_t = result;
+            phase = kScatter;
+        }
+        for (;;) {
+            if (phase == kFlight) {  // k_wf_extend, then k_wf_shade and k_wf_shadow
+                Flight f = flight_begin(sc.u, st.origin, st.dir);
+                if (resumed) {
+                    const uint2 fl = S.p_flight[slot];
+                    f.tt = bitsf(fl.x), f.i = int(fl.y);
+                    resumed = false;
+                }
+                if (stage_extend_steps(sc, fc, st, ev, f, 4096) == kFlowHit) {
+                    stage_shade(sc, fc, st, ev);
+                    if (st.bounce == 0u) {
+                        c1 = make_float4(c1.x + st.albedo.x, c1.y + st.albedo.y, c1.z + st.albedo.z, c1.w + 0.0f);
+                        c2 = make_float4(c2.x + st.normal.x, c2.y + st.normal.y, c2.z + st.normal.z, c2.w + st.depth);
+                    }
+                    stage_shadow(sc, st, ev);
+                    phase = kScatter;
+                } else {
+                    if (st.bounce == 0u) {  // background: pt.comp:434-438
+                        c0 = make_float4(c0.x + 0.f, c0.y + 0.f, c0.z + 0.f, c0.w + 1.0f);
+                        c1 = make_float4(c1.x + 1.0f, c1.y + 1.0f, c1.z + 1.0f, c1.w + 0.f);
+                        c2 = make_float4(c2.x + 0.f, c2.y + 0.f, c2.z + 0.f, c2.w + 1e4f);
+                    }
+                    phase = kEnded;
+                }
+            }
+            if (phase == kScatter) {  // k_wf_shadow_map, k_wf_scatter
+                ev.map_pdf = 0.0f;
+                if (fc.env_map) stage_shadow_map(sc, st, ev);
+                phase = stage_scatter(sc, fc, st, ev) && path_continues(fc, st) ? kFlight : kEnded;
+                continue;
+            }
+            // k_wf_regen: the path's radiance into its chunk's sums (pt.comp:546-551),
+            // then the chunk's next path or its partial sums.
+            const V3 r = path_end(st);
+            const float lum = dot(r, V3(0.2126f, 0.7152f, 0.0722f));
+            c0 = make_float4(c0.x + r.x, c0.y + r.y, c0.z + r.z, c0.w);
+            c1.w += lum * lum;
+            ref.k++;
+            uint32_t px, py, ci, pi, path_index, n_paths;
+            item_coords(R, ref.item, ref.k, px, py, ci, pi, path_index, n_paths);
+            if (ref.k >= n_paths) {
+                store_partial(R, ci, pi, c0, c1, c2);
+                break;
+            }
+            path_begin(sc, fc, px, py, path_index, st);
+            phase = kFlight;
+        }
+        S.p_spec[slot] = make_float4(0.f, 0.f, 0.f, bitsf(kNone));  // the slot is idle
+    }
+}
+
 // The chunk sums of every pixel of a request, added in chunk order to the
 // accumulation and guide images; the running mean to the HDR image (pt.comp:554-560).
 __global__ void k_wf_resolve(PtRequest R) {
@@ -771,6 +863,7 @@ WfGrid wf_grid(int cus) {
     // Msamples/s against 124 unsuspended; 8-24 / 16-48 are within 3 %.
     g.flight_steps = 16;
     g.shadow_steps = 32;
+    g.tail_paths = 1u << 16;
     return g;
 }
 
@@ -833,6 +926,11 @@ void wf_iteration_served(const PtRequest* reqs, const WfActive& A, const WfServe
                          const WfQueues& Q, uint32_t cur, const WfGrid& g, bool env_map, hipStream_t st) {
     hipLaunchKernelGGL(k_wf_regen_served, dim3(g.batched.regen), dim3(64), 0, st, reqs, A, V, S, Q, cur);
     wf_stages<true>(reqs, S, Q, cur, g, true, env_map, st);
+}
+
+void wf_finish(const PtRequest* reqs, const WfSlots& S, const WfQueues& Q, uint32_t cur, uint32_t paths,
+               hipStream_t st) {
+    hipLaunchKernelGGL(k_wf_finish, dim3(std::max(1u, (paths + 63u) / 64u)), dim3(64), 0, st, reqs, S, Q, cur);
 }
 
 void wf_resolve(const PtRequest& r, hipStream_t st) {

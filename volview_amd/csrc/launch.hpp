@@ -131,6 +131,9 @@ struct WfGrid {
     // it is suspended and re-queued, so that a wave's lanes do not idle behind
     // its longest ray (4096: never suspended).
     int flight_steps, shadow_steps;
+    // Single-request engines: the live paths under which the frame's tail runs
+    // in one launch (wf_finish); 0: never.
+    uint32_t tail_paths;
 };
 WfGrid wf_grid(int cus);
 // All `slots` slots empty on the end queue of iteration 0, item queue at 0.
@@ -144,6 +147,11 @@ void wf_init_served(const WfSlots& S, const WfQueues& Q, uint32_t slots, hipStre
 void wf_iteration_served(const PtRequest* reqs, const WfActive& A, const WfServed& V, const WfSlots& S,
                          const WfQueues& Q, uint32_t cur, const WfGrid& g, bool env_map, hipStream_t st);
 // Chunk sums -> accumulation, guides and HDR images of one request.
+// Single-request engine, every item handed out: runs the paths still in flight
+// (at most `paths`) to their ends in one launch, after the iterations queued so
+// far; `cur` is the next iteration's.
+void wf_finish(const PtRequest* reqs, const WfSlots& S, const WfQueues& Q, uint32_t cur, uint32_t paths,
+               hipStream_t st);
 void wf_resolve(const PtRequest& r, hipStream_t st);
 
 }  // namespace vvr

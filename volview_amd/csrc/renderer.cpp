@@ -169,16 +169,21 @@ void autotune(vvr_renderer* r, const vvr_frame_params* p, uint32_t trials, vvr_p
         vvr_pt_shape best = e.shape();
         float best_ms = time_shape(best);
         auto consider = [&](const vvr_pt_shape& s) {
-            if (s.slots == best.slots && s.flight_steps == best.flight_steps && s.shadow_steps == best.shadow_steps)
+            if (s.slots == best.slots && s.flight_steps == best.flight_steps && s.shadow_steps == best.shadow_steps &&
+                s.tail_paths == best.tail_paths)
                 return;
             const float ms = time_shape(s);
             if (ms < 0.98f * best_ms) best_ms = ms, best = s;
         };
         const uint32_t steps[][2] = {{8, 16}, {12, 24}, {16, 32}, {24, 48}, {32, 64}};
-        const uint32_t slots0 = best.slots;
-        for (const auto& st : steps) consider(vvr_pt_shape{slots0, st[0], st[1]});
-        const vvr_pt_shape tuned = best;
-        for (uint32_t slots : {1u << 20, 1u << 21, 1u << 22}) consider(vvr_pt_shape{slots, tuned.flight_steps, tuned.shadow_steps});
+        const vvr_pt_shape s0 = best;
+        for (const auto& st : steps) consider(vvr_pt_shape{s0.slots, st[0], st[1], s0.tail_paths});
+        const vvr_pt_shape s1 = best;
+        for (uint32_t slots : {1u << 20, 1u << 21, 1u << 22})
+            consider(vvr_pt_shape{slots, s1.flight_steps, s1.shadow_steps, s1.tail_paths});
+        const vvr_pt_shape s2 = best;
+        for (uint32_t tail : {1u << 15, 1u << 16, 1u << 17, 1u << 18, UINT32_MAX})
+            consider(vvr_pt_shape{s2.slots, s2.flight_steps, s2.shadow_steps, tail});
         e.set_shape(best);
         *chosen = best;
     } catch (...) {

@@ -71,6 +71,7 @@ struct WfEngine {
         grid = wf_grid(std::max(cus, 1));
         grid.flight_steps = int(std::min(env_u32("VVR_PT_FLIGHT_STEPS", uint32_t(grid.flight_steps)), 4096u));
         grid.shadow_steps = int(std::min(env_u32("VVR_PT_SHADOW_STEPS", uint32_t(grid.shadow_steps)), 4096u));
+        if (const char* v = std::getenv("VVR_PT_TAIL")) grid.tail_paths = uint32_t(std::strtoul(v, nullptr, 10));  // 0: never
         // 4M slots (about 1 GiB): enough paths in flight that the long random
         // walks of a frame's last iterations still fill the device.
         max_slots = env_u32("VVR_PT_SLOTS", 1u << 22);
@@ -137,9 +138,11 @@ struct WfEngine {
         if (s.slots) max_slots = (s.slots + 63u) / 64u * 64u;
         if (s.flight_steps) grid.flight_steps = int(s.flight_steps);
         if (s.shadow_steps) grid.shadow_steps = int(s.shadow_steps);
+        if (s.tail_paths) grid.tail_paths = s.tail_paths == UINT32_MAX ? 0u : s.tail_paths;
     }
     vvr_pt_shape shape() const {
-        return vvr_pt_shape{max_slots, uint32_t(grid.flight_steps), uint32_t(grid.shadow_steps)};
+        return vvr_pt_shape{max_slots, uint32_t(grid.flight_steps), uint32_t(grid.shadow_steps),
+                            grid.tail_paths ? grid.tail_paths : UINT32_MAX};
     }
 
     // Items of `samples` paths per pixel at `chunk` paths per item.
@@ -197,6 +200,13 @@ struct WfEngine {
         for (int k = 0;; k ^= 1) {
             check(hipEventSynchronize(live_ev[k]), "wavefront progress");
             if (h_live[k] == 0) break;
+            // Fewer live slots than were filled: every item is handed out (regen
+            // leaves a slot idle only then). Few paths left: finish them in one launch.
+            if (n == 1 && h_live[k] < std::min(want, grid.tail_paths)) {
+                wf_finish(d_reqs, S, Q, iter & 1u, h_live[k], st);
+                check(hipGetLastError(), "wavefront tail");
+                break;
+            }
             launch_batch(k);
         }
         for (uint32_t i = 0; i < n; i++) wf_resolve(h_reqs[i], st);

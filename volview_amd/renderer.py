@@ -280,22 +280,31 @@ class Renderer:
         check(self._lib.vvr_renderer_read_uniforms(self.handle, byref(u), ctypes.sizeof(u)))
         return u
 
+    TAIL_NEVER = 0xFFFFFFFF
+
+    @staticmethod
+    def _shape(sh) -> dict:
+        return {"slots": sh.slots, "flight_steps": sh.flight_steps, "shadow_steps": sh.shadow_steps,
+                "tail_paths": sh.tail_paths}
+
     def pt_shape(self) -> dict:
-        """The wavefront path tracer's shape: slots, flight_steps, shadow_steps."""
+        """The wavefront path tracer's shape: slots, flight_steps, shadow_steps,
+        tail_paths (TAIL_NEVER: no tail launch)."""
         sh = PtShape()
         check(self._lib.vvr_renderer_get_pt_shape(self.handle, byref(sh)))
-        return {"slots": sh.slots, "flight_steps": sh.flight_steps, "shadow_steps": sh.shadow_steps}
+        return self._shape(sh)
 
-    def set_pt_shape(self, slots: int = 0, flight_steps: int = 0, shadow_steps: int = 0):
+    def set_pt_shape(self, slots: int = 0, flight_steps: int = 0, shadow_steps: int = 0, tail_paths: int = 0):
         """Sets the shape's non-zero fields (frames are the same with any shape)."""
-        check(self._lib.vvr_renderer_set_pt_shape(self.handle, byref(PtShape(slots, flight_steps, shadow_steps))))
+        check(self._lib.vvr_renderer_set_pt_shape(self.handle,
+                                                  byref(PtShape(slots, flight_steps, shadow_steps, tail_paths))))
 
     def autotune(self, params: FrameParams, trials: int = 1) -> dict:
         """vvr_renderer_autotune: times `params`' frames over candidate shapes and
         keeps the fastest; returns it. The trial frames accumulate."""
         sh = PtShape()
         check(self._lib.vvr_renderer_autotune(self.handle, byref(params), trials, byref(sh)))
-        return {"slots": sh.slots, "flight_steps": sh.flight_steps, "shadow_steps": sh.shadow_steps}
+        return self._shape(sh)
 
     def set_timing(self, enabled: bool):
         check(self._lib.vvr_renderer_set_timing(self.handle, 1 if enabled else 0))
