@@ -111,7 +111,7 @@ def measured_traffic(args, shape: dict) -> int | None:
             traffic = int(line.split(":")[1])
         if line.startswith("shape:"):
             profiled = [int(x) for x in line.split(":")[1].split()]
-    if profiled != [shape["slots"], shape["flight_steps"], shape["shadow_steps"]]:
+    if profiled != [shape["slots"], shape["flight_steps"], shape["shadow_steps"], shape["tail_paths"]]:
         return None
     return traffic
 

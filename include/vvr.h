@@ -246,7 +246,9 @@ int vvr_renderer_default_camera(const vvr_renderer* r, vvr_camera* out);
 /* Renderer::bounds / min_spacing / center / diagonal fields (renderer.rs:546-551). */
 int vvr_renderer_geometry(const vvr_renderer* r, float bounds_lo[3], float bounds_hi[3], float* min_spacing,
                           float center[3], float* diagonal);
-/* Renderer::destroy (renderer.rs:2072-2125). */
+/* Renderer::destroy (renderer.rs:2072-2125). A renderer that server requests
+ * still read (queued or running) is deleted when the last of them ends; the
+ * handle must not be used after this call either way. */
 void vvr_renderer_destroy(vvr_renderer* r);
 
 /* ---- read-back (synchronising) --------------------------------------------- */

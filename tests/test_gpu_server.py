@@ -14,6 +14,8 @@
   denoised frames, equal to their renderer's.
 - Cancelling ends a queued request at once, and a path-traced one in flight
   with its last delivered frame.
+- A renderer destroyed while its requests are queued or running is deleted
+  only when the last of them ends.
 """
 import numpy as np
 import pytest
@@ -173,6 +175,28 @@ def test_cancel_queued_and_running(ctx):
         srv.read_hdr(queued, 16, 16)
     img = srv.read_hdr(running, 32, 32)
     assert np.isfinite(img).all()
+    srv.release(running)
+    srv.release(queued)
+    srv.close()
+
+
+def test_renderer_destroyed_while_served(ctx):
+    """A renderer destroyed while its requests are running or queued stays
+    alive until the last of them ends: the running one delivers its frames,
+    the queued one can still be cancelled, and nothing reads freed memory."""
+    a, b, n = volumes(ctx)
+    srv = Server(ctx, pt_increment=4, max_requests=1)
+    p = workloads.pt_params(a, 0, 24, bounces=4)
+    running = srv.submit(a, p, 32, 24)
+    queued = srv.submit(b, workloads.dvr_params(b, 0), 16, 16)
+    a.close()  # its request is in flight: deletion waits for it
+    b.close()  # its request waits for admission
+    srv.cancel(queued)
+    got = drain(srv, [running, queued], cancelled=(queued,))
+    last = got[running][-1]
+    assert last.final_frame and last.status == 0 and last.spp == 24
+    img = srv.read_hdr(running, 32, 24)
+    assert np.isfinite(img).all() and np.abs(img[..., :3]).max() > 0
     srv.release(running)
     srv.release(queued)
     srv.close()

@@ -434,7 +434,17 @@ int vvr_renderer_geometry(const vvr_renderer* r, float lo[3], float hi[3], float
     });
 }
 
-void vvr_renderer_destroy(vvr_renderer* r) { delete r; }
+void vvr_renderer_destroy(vvr_renderer* r) {
+    if (!r) return;
+    {
+        std::lock_guard<std::mutex> lk(r->hold_mu);
+        if (r->holds > 0) {  // server requests still read it: the last one deletes it
+            r->orphaned = true;
+            return;
+        }
+    }
+    delete r;
+}
 
 int vvr_renderer_read_hdr(vvr_renderer* r, float* rgba) {
     return guarded([&] {

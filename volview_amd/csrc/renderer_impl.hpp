@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <optional>
 #include <string>
 #include <vector>
@@ -261,6 +262,11 @@ struct vvr_ctx {
     int device;
 };
 
+// A server request's hold on its renderer; let_go returns true when the caller
+// must delete the renderer (destroyed meanwhile and this was the last hold).
+inline void hold_renderer(vvr_renderer* r);
+inline bool let_go_renderer(vvr_renderer* r);
+
 // Runs the learned denoiser on a path-traced frame's images into its HDR image (unet.cpp).
 void unet_denoise_frame(vvr_unet* n, const float4* accum, const float4* albedo, const float4* normal, float samples,
                         uint32_t width, uint32_t height, uint16_t* hdr, hipStream_t st);
@@ -268,6 +274,11 @@ void unet_denoise_frame(vvr_unet* n, const float4* accum, const float4* albedo, 
 struct vvr_renderer {
     int device = 0;
     hipStream_t stream = nullptr;
+    // Server requests that read this renderer hold it: a renderer destroyed
+    // while held is deleted when its last request lets go (vvr_renderer_destroy).
+    std::mutex hold_mu;
+    int holds = 0;
+    bool orphaned = false;
     Geometry geo{};
     BrickGrid grid{};
     int mdims[3] = {0, 0, 0};
@@ -530,3 +541,13 @@ template <class F> int guarded(F f) {
 }
 
 }  // namespace vvr
+
+inline void hold_renderer(vvr_renderer* r) {
+    std::lock_guard<std::mutex> lk(r->hold_mu);
+    r->holds++;
+}
+
+inline bool let_go_renderer(vvr_renderer* r) {
+    std::lock_guard<std::mutex> lk(r->hold_mu);
+    return --r->holds == 0 && r->orphaned;
+}

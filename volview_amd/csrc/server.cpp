@@ -63,6 +63,7 @@ void hfree(void* p) {
 struct Job {
     uint64_t id = 0, tag = 0;
     vvr_renderer* vol = nullptr;
+    bool holds = false;  // holds `vol` (hold_renderer) until the request is done with it
     vvr_frame_params params{};
     uint32_t width = 0, height = 0;
     bool pt = false;
@@ -188,6 +189,16 @@ struct vvr_server {
     void drop_active(uint32_t e);
     void free_entry(int e);
     void finish(Job* j, int status, const std::string& msg, const vvr_completion* last = nullptr);
+    // The request is done with its renderer. A renderer destroyed meanwhile is
+    // deleted here by its last request, after the server's work on it.
+    void let_go(Job* j) {
+        if (!j->holds) return;
+        j->holds = false;
+        if (let_go_renderer(j->vol)) {
+            (void)hipStreamSynchronize(stream);
+            delete j->vol;
+        }
+    }
     void release_memory(Job* j);
     vvr_completion record(const Job* j, int status, uint32_t spp, bool final_frame) const {
         vvr_completion c{};
@@ -217,6 +228,7 @@ void vvr_server::release_memory(Job* j) {
 // that sees the completion finds the request finished.
 void vvr_server::finish(Job* j, int status, const std::string& msg, const vvr_completion* last) {
     release_memory(j);
+    let_go(j);
     std::lock_guard<std::mutex> lk(mu);
     j->state = Job::kDone;
     j->status = status;
@@ -237,6 +249,11 @@ void vvr_server::admit(std::vector<Job*>& fresh_dvr) {
         candidates.assign(queue.begin(), queue.end());
     }
     for (Job* j : candidates) {
+        // Under the lock from here to its removal from the queue: a request
+        // cancelled meanwhile has left the queue, and its renderer may be gone.
+        std::unique_lock<std::mutex> lk(mu);
+        const auto qi = std::find(queue.begin(), queue.end(), j);
+        if (qi == queue.end()) continue;
         const size_t need = request_bytes(*j);
         if (running.size() >= cfg.max_requests || free_uslots.empty()) break;
         if (stats.hbm_in_use + need > cfg.hbm_budget && !running.empty()) continue;
@@ -251,10 +268,8 @@ void vvr_server::admit(std::vector<Job*>& fresh_dvr) {
         if (v->needs_opacity_rebuild(j->params) && readers[v] > 0) continue;
         if (!j->pt && v->needs_lighting_rebuild(j->params, packed) && light_readers[v] > 0) continue;
         const bool rebuild = v->needs_prepare(j->params, packed);
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            queue.erase(std::find(queue.begin(), queue.end(), j));
-        }
+        queue.erase(qi);
+        lk.unlock();
         try {
             j->uslot = free_uslots.back();
             free_uslots.pop_back();
@@ -682,6 +697,8 @@ int vvr_server_submit(vvr_server* s, const vvr_request* rq, uint64_t* id) {
         if (!s->fatal.empty()) throw Error{VVR_EDEVICE, "server failed: " + s->fatal};
         j->id = s->next_id++;
         if (id) *id = j->id;
+        hold_renderer(j->vol);
+        j->holds = true;
         s->queue.push_back(j.get());
         s->jobs[j->id] = std::move(j);
         s->stats.submitted++;
@@ -736,6 +753,7 @@ int vvr_server_cancel(vvr_server* s, uint64_t id) {
         auto q = std::find(s->queue.begin(), s->queue.end(), j);
         if (q != s->queue.end()) {
             s->queue.erase(q);
+            s->let_go(j);
             j->state = Job::kDone;
             j->status = VVR_ECANCELED;
             j->error = "cancelled";
@@ -773,7 +791,10 @@ void vvr_server_destroy(vvr_server* s) {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stopping = true;
         // Requests not admitted yet are dropped.
-        for (Job* j : s->queue) j->state = Job::kDone;
+        for (Job* j : s->queue) {
+            j->state = Job::kDone;
+            s->let_go(j);  // never admitted: no work of the server reads its renderer
+        }
         s->queue.clear();
         s->cv_submit.notify_all();
     }
@@ -783,6 +804,7 @@ void vvr_server_destroy(vvr_server* s) {
     for (auto& kv : s->jobs) {
         Job* j = kv.second.get();
         if (j->state == Job::kRunning) s->release_memory(j);
+        s->let_go(j);
         hfree(j->h_hdr);
         if (j->read_ev) (void)hipEventDestroy(j->read_ev);
     }

@@ -28,6 +28,7 @@ class Server:
         self._lib = load()
         self.ctx = ctx
         cfg = ServerConfig(hbm_budget, slots, pt_increment, max_requests, flight_steps, shadow_steps)
+        self._held = {}  # request id -> its renderer, kept alive while the server may read it
         h = c_void_p()
         check(self._lib.vvr_server_create(ctx.handle, byref(cfg), byref(h)))
         self.handle = h
@@ -36,6 +37,7 @@ class Server:
         rq = Request(renderer.handle, pointer(params), width, height, tag)  # copied by the call
         rid = c_uint64()
         check(self._lib.vvr_server_submit(self.handle, byref(rq), byref(rid)))
+        self._held[int(rid.value)] = renderer
         return int(rid.value)
 
     def poll(self, cap: int = 256, wait_ms: int = 0) -> list[Completion]:
@@ -54,6 +56,7 @@ class Server:
 
     def release(self, rid: int):
         check(self._lib.vvr_server_release(self.handle, rid))
+        self._held.pop(rid, None)
 
     def stats(self) -> dict:
         s = ServerStats()
@@ -64,6 +67,7 @@ class Server:
         if getattr(self, "handle", None):
             self._lib.vvr_server_destroy(self.handle)
             self.handle = None
+        self._held = {}
 
     def __del__(self):
         try:
