@@ -90,28 +90,42 @@ struct PtRequest {
 
 // Path slots of the wavefront engine, structure of float4 arrays (one float4
 // per slot per field; the w lanes carry packed integers).
+// One field of the path slots, read and written as field[slot]. The fields a
+// stage touches together are interleaved slot by slot: each slot's fields form
+// a record of `stride` elements, so one lane's loads of a path fall in one or
+// two cache lines (queues visit slots in no order that would coalesce lanes).
+template <class T> struct SlotField {
+    T* base;
+    uint32_t stride;
+    __device__ __forceinline__ T& operator[](uint32_t i) const { return base[size_t(i) * stride]; }
+};
+
+// Records: a path's (11 float4s: its state, chunk sums, suspended loop, light-map
+// sample) and its event's (8 float4s, one 128-byte line).
+constexpr uint32_t kWfPathRecord = 11, kWfEventRecord = 8;
+
 struct WfSlots {
-    float4* p_origin;  // origin (index space), last_pdf
-    float4* p_dir;     // direction (world), rng state
-    float4* p_thr;     // throughput, hero | surfaces << 2 | volumes << 8 | bounce << 18
-    float4* p_spec;    // spectral ratios, global item index
-    float4* p_rad;     // radiance, k | request << 16
-    float4* e_idx;     // event position, cut
-    float4* e_s0;      // sample albedo, extinction
-    float4* e_s1;      // roughness, metalness, subsurface, coat
-    float4* e_s2;      // density, label | ghost << 8 | (cap_plane + 1) << 9 | surface event << 12
-    float4* e_n;       // shading normal
-    float4* e_light;   // key-light direction
-    float4* e_lt;      // its transmittance
-    float4* c0;        // chunk sums: radiance, background
-    float4* c1;        //             albedo, luminance^2
-    float4* c2;        //             normal, depth
-    float4* e_map;     // light-map sample: direction, pdf
-    float4* e_maplt;   // its radiance / pdf x transmittance
-    uint16_t* req;     // request of the slot's path (engines over several requests)
-    uint2* p_flight;   // a suspended tracking loop: distance along the ray (f32 bits), iteration
-                       // (free flight: iteration kWfNewFlight for a flight not begun; the shadow
-                       // ray's, whose running transmittance is in e_lt)
+    SlotField<float4> p_origin;  // origin (index space), last_pdf
+    SlotField<float4> p_dir;     // direction (world), rng state
+    SlotField<float4> p_thr;     // throughput, hero | surfaces << 2 | volumes << 8 | bounce << 18
+    SlotField<float4> p_spec;    // spectral ratios, global item index
+    SlotField<float4> p_rad;     // radiance, k | request << 16
+    SlotField<uint2> p_flight;   // a suspended tracking loop: distance along the ray (f32 bits), iteration
+                                 // (free flight: iteration kWfNewFlight for a flight not begun; the shadow
+                                 // ray's, whose running transmittance is in e_lt)
+    SlotField<float4> c0;        // chunk sums: radiance, background
+    SlotField<float4> c1;        //             albedo, luminance^2
+    SlotField<float4> c2;        //             normal, depth
+    SlotField<float4> e_map;     // light-map sample: direction, pdf
+    SlotField<float4> e_maplt;   // its radiance / pdf x transmittance
+    SlotField<float4> e_idx;     // event position, cut
+    SlotField<float4> e_s0;      // sample albedo, extinction
+    SlotField<float4> e_s1;      // roughness, metalness, subsurface, coat
+    SlotField<float4> e_s2;      // density, label | ghost << 8 | (cap_plane + 1) << 9 | surface event << 12
+    SlotField<float4> e_n;       // shading normal
+    SlotField<float4> e_light;   // key-light direction
+    SlotField<float4> e_lt;      // its transmittance
+    uint16_t* req;               // request of the slot's path (engines over several requests)
 };
 
 // Counters in WfQueues::counts: queue lengths, the item queue head and the live
