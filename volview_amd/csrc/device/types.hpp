@@ -100,9 +100,9 @@ template <class T> struct SlotField {
     __device__ __forceinline__ T& operator[](uint32_t i) const { return base[size_t(i) * stride]; }
 };
 
-// Records: a path's (11 float4s: its state, chunk sums, suspended loop, light-map
-// sample) and its event's (8 float4s, one 128-byte line).
-constexpr uint32_t kWfPathRecord = 11, kWfEventRecord = 8;
+// Records of 8 float4s, one 128-byte line each: a path's state and suspended
+// loop (what extend reads), its chunk sums and light-map sample, its event.
+constexpr uint32_t kWfPathRecord = 8, kWfSumsRecord = 8, kWfEventRecord = 8;
 
 struct WfSlots {
     SlotField<float4> p_origin;  // origin (index space), last_pdf

@@ -93,11 +93,12 @@ struct WfEngine {
         slot_mem = nullptr, slots = 0;
         constexpr int kQueues = 10;
         const size_t path_bytes = size_t(n) * kWfPathRecord * sizeof(float4);
+        const size_t sums_bytes = size_t(n) * kWfSumsRecord * sizeof(float4);
         const size_t event_bytes = size_t(n) * kWfEventRecord * sizeof(float4);
         const size_t q_bytes = (size_t(n) * 4 + 255) / 256 * 256;
         const size_t sorted_bytes = (size_t(wf_sorted_capacity(n)) * 4 + 255) / 256 * 256;
         const size_t req_bytes = (size_t(n) * 2 + 255) / 256 * 256, bin_bytes = 4 * kWfBins * 4;
-        slot_mem = dalloc<uint8_t>(path_bytes + event_bytes + kQueues * q_bytes + sorted_bytes + req_bytes + 256 +
+        slot_mem = dalloc<uint8_t>(path_bytes + sums_bytes + event_bytes + kQueues * q_bytes + sorted_bytes + req_bytes + 256 +
                                        bin_bytes,
                                    "path slots");
         uint8_t* p = slot_mem;
@@ -105,13 +106,16 @@ struct WfEngine {
         p += path_bytes;
         float4* events = reinterpret_cast<float4*>(p);
         p += event_bytes;
+        float4* sums = reinterpret_cast<float4*>(p);
+        p += sums_bytes;
+        auto sums_field = [&](int k) { return SlotField<float4>{sums + k, kWfSumsRecord}; };
         auto path_field = [&](int k) { return SlotField<float4>{paths + k, kWfPathRecord}; };
         auto event_field = [&](int k) { return SlotField<float4>{events + k, kWfEventRecord}; };
         S.p_origin = path_field(0), S.p_dir = path_field(1), S.p_thr = path_field(2), S.p_spec = path_field(3);
         S.p_rad = path_field(4);
         S.p_flight = SlotField<uint2>{reinterpret_cast<uint2*>(paths + 5), 2 * kWfPathRecord};
-        S.c0 = path_field(6), S.c1 = path_field(7), S.c2 = path_field(8), S.e_map = path_field(9);
-        S.e_maplt = path_field(10);
+        S.c0 = sums_field(0), S.c1 = sums_field(1), S.c2 = sums_field(2), S.e_map = sums_field(3);
+        S.e_maplt = sums_field(4);
         S.e_idx = event_field(0), S.e_s0 = event_field(1), S.e_s1 = event_field(2), S.e_s2 = event_field(3);
         S.e_n = event_field(4), S.e_light = event_field(5), S.e_lt = event_field(6);
         uint32_t** qs[kQueues] = {&Q.ff[0],   &Q.ff[1],   &Q.ev,    &Q.end[0], &Q.end[1],
