@@ -365,6 +365,75 @@ def run_serving(ctx, seconds: float, grace: float = 5.0, increment: int = 16, sl
             "window_seconds": window, "plain": plain, "served": served}
 
 
+def dvr_frame_bytes(dims, width, height) -> int:
+    """Algorithmic bytes of one DVR-1 frame with lighting off: density (2 B) and
+    normals (4 B) per voxel, the majorant grid, the RGBA16F image."""
+    n = dims[0] * dims[1] * dims[2]
+    bricks = -(-dims[0] // 8) * -(-dims[1] // 8) * -(-dims[2] // 8)
+    return 6 * n + 4 * bricks + 8 * width * height
+
+
+def majorant_bytes(dims, label_mode: bool) -> int:
+    """Algorithmic bytes of the majorant build: density (2 B), the mask in label
+    modes (4 B) per voxel, one f32 per 8^3 brick."""
+    n = dims[0] * dims[1] * dims[2]
+    bricks = -(-dims[0] // 8) * -(-dims[1] // 8) * -(-dims[2] // 8)
+    return (6 if label_mode else 2) * n + 4 * bricks
+
+
+def roofline_gbs(nbytes: int, ms: float) -> dict:
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "algorithmic_bytes": nbytes}
+
+
+def run_dvr(ctx, frames: int = 40) -> dict:
+    """DVR-1 (256^3 phantom, bone preset, 512^2, cubic sampling, lighting off):
+    frames/s, the ray marcher's HIP-event time per frame and its roofline, wall
+    latency of record + read-back; the majorant build's time on first use."""
+    dims, size = (256, 256, 256), (512, 512)
+    sc = workloads.dvr_scene(ctx, dims=dims, size=size)
+    r = sc.renderer
+    r.set_timing(True)
+    r.record(1, sc.params)  # builds normals and the majorant
+    r.synchronize()
+    maj_ms, _ = r.kernel_time(_abi.KERNEL_MAJORANT)
+    pt_majorant_ms, _ = r.kernel_time(_abi.KERNEL_MAJORANT)
+    r.set_timing(True)
+    lat = []
+    for f in range(frames):
+        t = time.perf_counter()
+        r.record(2 + f, sc.params)
+        r.read_hdr()
+        lat.append(time.perf_counter() - t)
+    dvr_ms, n = r.kernel_time(_abi.KERNEL_DVR)
+    r.set_timing(False)
+    r.close()
+    per = dvr_ms / max(n, 1)
+    # With the shadow and occlusion volumes (built once, then read by every frame).
+    lit = workloads.dvr_scene(ctx, dims=dims, size=size, lighting=True)
+    rl = lit.renderer
+    rl.set_timing(True)
+    rl.record(1, lit.params)
+    rl.synchronize()
+    light_ms, _ = rl.kernel_time(_abi.KERNEL_LIGHTING)
+    rl.set_timing(True)
+    for f in range(frames):
+        rl.record(2 + f, lit.params)
+    rl.synchronize()
+    lit_ms, n_lit = rl.kernel_time(_abi.KERNEL_DVR)
+    rl.set_timing(False)
+    rl.close()
+    log(f"dvr-1: {per:.3f} ms per frame ({lit_ms / max(n_lit, 1):.3f} lit), majorant {maj_ms:.3f} ms")
+    return {"workload": "DVR-1: 256^3 int16 phantom, bone preset, 512^2, step 0.5 mm, cubic sampling, lighting off",
+            "ms_per_frame": round(per, 3), "frames_per_s": round(1e3 / per, 1),
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                           "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
+            "roofline": roofline_gbs(dvr_frame_bytes(dims, *size), per),
+            "lit": {"ms_per_frame": round(lit_ms / max(n_lit, 1), 3), "lighting_volumes_ms": round(light_ms, 3)},
+            "majorant": {"ms": round(maj_ms, 4), "roofline": roofline_gbs(majorant_bytes(dims, False), maj_ms)}}
+
+
 def unet_flops(channels, width, height) -> float:
     """Algorithmic FLOPs of one U-Net pass over a width x height frame: 2 x 9 x
     c_in x c_out per pixel of each 3x3 layer at its level's resolution (the
@@ -423,6 +492,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip Batched-32 (plain vs served)")
     ap.add_argument("--no-denoisers", action="store_true", help="skip the per-frame denoiser timings")
+    ap.add_argument("--no-dvr", action="store_true", help="skip the DVR-1 frame timings")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default shape instead of the autotuner's (vvr_renderer_autotune)")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
@@ -446,6 +516,7 @@ def main() -> int:
     if args.copy_probe:
         copy_probe(args.copy_probe << 20)
         print(f"copy probe: {args.copy_probe} MiB", file=sys.stderr, flush=True)
+    r.set_timing(True)  # the first record builds the masks and the majorant (autotune's, warm-up's or the first step's)
     shape = r.pt_shape()
     if args.autotune:  # untimed: the autotuner's trial frames of the workload itself
         t_tune = time.perf_counter()
@@ -458,6 +529,7 @@ def main() -> int:
         r.record(frame, scene.params)
         r.synchronize()
         print(f"warmup frame {frame} done", file=sys.stderr, flush=True)
+    pt_majorant_ms, _ = r.kernel_time(_abi.KERNEL_MAJORANT)
     r.set_timing(True)
     lat = []
     r.synchronize()
@@ -472,6 +544,7 @@ def main() -> int:
     r.synchronize()
     elapsed = time.perf_counter() - t0
     pt_ms, pt_launches = r.kernel_time(_abi.KERNEL_PT)
+    pt_majorant_ms = max(pt_majorant_ms, r.kernel_time(_abi.KERNEL_MAJORANT)[0])
     r.set_timing(False)
 
     samples = args.size * args.size * args.spp * args.steps
@@ -504,9 +577,14 @@ def main() -> int:
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, scene.params, args.cpu_seconds)
+    out["majorant"] = {"workload": f"PT-1 volume {args.dims[0]}x{args.dims[1]}x{args.dims[2]}, isolate label mode",
+                       "ms": round(pt_majorant_ms, 4),
+                       "roofline": roofline_gbs(majorant_bytes(args.dims, True), pt_majorant_ms)}
     if not args.no_denoisers:
         out["denoisers"] = run_denoisers(r, scene.params, args.size)
     r.close()
+    if not args.no_dvr:
+        out["dvr"] = run_dvr(ctx)
     if not args.no_batched:
         out["batched"] = run_batched(ctx)
     if args.serve_seconds > 0:
