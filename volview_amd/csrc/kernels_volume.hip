@@ -273,34 +273,66 @@ __global__ void __launch_bounds__(kBlock) k_majorant(const uint16_t* __restrict_
     __shared__ uint32_t s_minmax[kCells * kCells * kCells];  // min | max << 16
     __shared__ uint16_t s_mask[kCells * kCells * kCells];
     const int tbx = blockIdx.x * kTile, tby = blockIdx.y * kTile, tbz = blockIdx.z * kTile;
-    // Cell c covers voxels 2c, 2c + 1; the tile's first cell is 4 * tb - 1.
+    // Cell c covers voxels 2c, 2c + 1; the tile's first cell is 4 * tb - 1. A
+    // 4^3 micro-brick (64 contiguous voxels) holds 2^3 cells: the tile's cells
+    // lie in micro-bricks 2 tb - 1 .. 2 tb + 2 kTile, one per thread at a time,
+    // read with 16-byte loads.
     const int c0x = 4 * tbx - 1, c0y = 4 * tby - 1, c0z = 4 * tbz - 1;
-    for (int c = threadIdx.x; c < kCells * kCells * kCells; c += kBlock) {
-        const int cx = c % kCells, cy = (c / kCells) % kCells, cz = c / (kCells * kCells);
-        const int vx = 2 * (c0x + cx), vy = 2 * (c0y + cy), vz = 2 * (c0z + cz);
-        uint32_t lo = 0xffffu, hi = 0u, mm = 0u;
-        for (int k = 0; k < 2; k++) {
-            const int z = vz + k;
-            if (z < 0 || z >= g.nz) continue;
-            for (int j = 0; j < 2; j++) {
-                const int y = vy + j;
-                if (y < 0 || y >= g.ny) continue;
-                for (int i = 0; i < 2; i++) {
-                    const int x = vx + i;
-                    if (x < 0 || x >= g.nx) continue;
-                    const uint32_t a = g.at(x, y, z);
-                    const uint32_t v = d[a];
-                    lo = min(lo, v);
-                    hi = max(hi, v);
-                    if (mode != 0) {
-                        const uint32_t m = mask[a];
-                        mm = max(mm, max(m & 0xffffu, m >> 16));
+    constexpr int kMb = 2 * kTile + 2;  // micro-bricks per axis
+    for (int t = threadIdx.x; t < kMb * kMb * kMb; t += kBlock) {
+        const int mbx = 2 * tbx - 1 + t % kMb, mby = 2 * tby - 1 + (t / kMb) % kMb, mbz = 2 * tbz - 1 + t / (kMb * kMb);
+        uint32_t lo[8], hi[8], mm[8];
+#pragma unroll
+        for (int c = 0; c < 8; c++) lo[c] = 0xffffu, hi[c] = 0u, mm[c] = 0u;
+        if (mbx >= 0 && mby >= 0 && mbz >= 0 && mbx < g.bx && mby < g.by && mbz < g.bz) {
+            const uint32_t base = g.at(4 * mbx, 4 * mby, 4 * mbz);  // the micro-brick's first voxel
+            // Voxel v = dx + 4 dy + 16 dz of the micro-brick; inside the volume?
+            auto inside = [&](int v) {
+                return 4 * mbx + (v & 3) < g.nx && 4 * mby + ((v >> 2) & 3) < g.ny && 4 * mbz + (v >> 4) < g.nz;
+            };
+            auto cell = [](int v) { return ((v >> 1) & 1) | (((v >> 3) & 1) << 1) | ((v >> 5) << 2); };
+            const uint4* dv = reinterpret_cast<const uint4*>(d + base);
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const uint4 w4 = dv[q];
+                const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        const int v = 8 * q + 2 * k + h;
+                        if (!inside(v)) continue;
+                        const uint32_t x = h ? (w[k] >> 16) : (w[k] & 0xffffu);
+                        const int c = cell(v);
+                        lo[c] = min(lo[c], x);
+                        hi[c] = max(hi[c], x);
+                    }
+            }
+            if (mode != 0) {
+                const uint4* mv = reinterpret_cast<const uint4*>(mask + base);
+#pragma unroll
+                for (int q = 0; q < 16; q++) {
+                    const uint4 w4 = mv[q];
+                    const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const int v = 4 * q + k;
+                        if (!inside(v)) continue;
+                        const int c = cell(v);
+                        mm[c] = max(mm[c], max(w[k] & 0xffffu, w[k] >> 16));
                     }
                 }
             }
         }
-        s_minmax[c] = lo | (hi << 16);
-        s_mask[c] = uint16_t(mm);
+        // The micro-brick's cells that lie in the tile (all of them but halo halves).
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const int cx = 2 * mbx + (c & 1) - c0x, cy = 2 * mby + ((c >> 1) & 1) - c0y, cz = 2 * mbz + (c >> 2) - c0z;
+            if (cx < 0 || cy < 0 || cz < 0 || cx >= kCells || cy >= kCells || cz >= kCells) continue;
+            const int i = cx + kCells * (cy + kCells * cz);
+            s_minmax[i] = lo[c] | (hi[c] << 16);
+            s_mask[i] = uint16_t(mm[c]);
+        }
     }
     __syncthreads();
     if (threadIdx.x >= kTile * kTile * kTile) return;
