@@ -6,7 +6,8 @@ pinned by finite differences).
   the oracle's evaluated at the same bf16-rounded weights and input: the loss
   within 1 %, each layer's gradient with cosine > 0.98 and norm within 10 %
   (activations are rounded to bf16 at every layer on the GPU).
-- The denoised frame equals the oracle's forward pass within bf16 rounding.
+- The denoised frame equals the oracle's forward pass within bf16 rounding, on a
+  small frame and on a 384 x 400 one.
 - Training on the GPU fits a fixed batch of path-traced frames: the loss drops
   by more than a third in 150 Adam steps.
 """
@@ -74,11 +75,12 @@ def test_gradients_equal_oracle(ctx):
             assert cos > 0.98 and 0.9 < ratio < 1.1, (i, part, cos, ratio)
 
 
-def test_denoised_frame_equals_oracle_forward(ctx):
+@pytest.mark.parametrize("h,w", [(32, 48), (384, 400)])  # the larger frame: fewer output blocks per wave
+def test_denoised_frame_equals_oracle_forward(ctx, h, w):
     rng = np.random.default_rng(12)
     channels = (16, 32)
     net = Unet(ctx, channels, seed=7)
-    accum, albedo, normal, n, target = synthetic(rng, 1, 32, 48)
+    accum, albedo, normal, n, target = synthetic(rng, 1, h, w)
     got = net.denoise(accum[0], albedo[0], normal[0], n)
     p, x, alb, _ = oracle_inputs(net.params(), channels, accum, albedo, normal, n, target)
     out, _ = U.forward(p, channels, x)
