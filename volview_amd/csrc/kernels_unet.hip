@@ -108,6 +108,103 @@ __global__ void __launch_bounds__(256) k_conv3x3(UnetConv c) {
     }
 }
 
+// k_conv3x3 over 2-D tiles of 8 x 32 output pixels (4 waves, each 2 rows of 32)
+// for layers with many pixels. Per chunk of 32 input channels the workgroup
+// stages in LDS the input tile with a one-pixel halo (10 x 34 pixels) and the
+// chunk's weights for all 9 taps, then runs the 9 taps' MFMAs from LDS: each
+// input pixel is fetched once per chunk instead of once per tap. LDS rows are
+// padded to 80 bytes so that the 16-byte fragment reads of 8 lanes cover the 32
+// banks once.
+constexpr int kTileH = 8, kTileW = 32, kChunk = 32, kRow = kChunk + 8;  // kRow: bf16s per LDS row
+template <int NCOB>
+__global__ void __launch_bounds__(256) k_conv3x3_tile(UnetConv c, int co_groups) {
+    __shared__ uint16_t s_in[(kTileH + 2) * (kTileW + 2) * kRow];
+    __shared__ uint16_t s_w[9 * NCOB * 32 * kRow];
+    const int lane = int(threadIdx.x & 63u), wave = int(threadIdx.x >> 6);
+    const int col = lane & 31, hi = lane >> 5;
+    const int x0 = int(blockIdx.x) * kTileW, y0 = int(blockIdx.y) * kTileH;
+    const int n = int(blockIdx.z) / co_groups, cob0 = (int(blockIdx.z) % co_groups) * NCOB;
+    const int nb = min(NCOB, c.co_blocks - cob0);
+    f32x16 acc[2][NCOB];
+#pragma unroll
+    for (int pb = 0; pb < 2; pb++)
+#pragma unroll
+        for (int b = 0; b < NCOB; b++) acc[pb][b] = f32x16{};
+    for (int kc = 0; kc < c.cin; kc += kChunk) {
+        const int cc = min(kChunk, c.cin - kc);  // 16 or 32
+        const int parts = cc >> 3;                // 16-byte parts per pixel or weight row
+        // The input tile: pixel (y0 - 1 + ty, x0 - 1 + tx), zero outside the image.
+        for (int k = int(threadIdx.x); k < (kTileH + 2) * (kTileW + 2) * parts; k += 256) {
+            const int pix = k / parts, part = k - pix * parts;
+            const int ty = pix / (kTileW + 2), tx = pix - ty * (kTileW + 2);
+            const int y = y0 - 1 + ty, x = x0 - 1 + tx;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (y >= 0 && y < c.h && x >= 0 && x < c.w)
+                v = *reinterpret_cast<const uint4*>(c.in + size_t((n * c.h + y) * c.w + x) * c.in_cs + c.in_co + kc +
+                                                    part * 8);
+            *reinterpret_cast<uint4*>(s_in + pix * kRow + part * 8) = v;
+        }
+        // The weights: row (tap, b * 32 + r) holds input channels kc .. kc + cc.
+        for (int k = int(threadIdx.x); k < 9 * NCOB * 32 * parts; k += 256) {
+            const int row = k / parts, part = k - row * parts;
+            const int tap = row / (NCOB * 32), r = row - tap * (NCOB * 32);
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (r < nb * 32)
+                v = *reinterpret_cast<const uint4*>(c.wts + (size_t(cob0 * 32 + r) * 9 + tap) * c.cin + kc + part * 8);
+            *reinterpret_cast<uint4*>(s_w + row * kRow + part * 8) = v;
+        }
+        __syncthreads();
+        for (int tap = 0; tap < 9; tap++) {
+            const int dy = tap / 3, dx = tap % 3;  // tile coordinates are shifted by the halo
+            for (int ci = 0; ci < cc; ci += 16) {
+                bf16x8 a[2];
+#pragma unroll
+                for (int pb = 0; pb < 2; pb++) {
+                    const int ty = 2 * wave + pb + dy, tx = col + dx;
+                    a[pb] = __builtin_bit_cast(
+                        bf16x8, *reinterpret_cast<const uint4*>(s_in + (ty * (kTileW + 2) + tx) * kRow + ci + 8 * hi));
+                }
+#pragma unroll
+                for (int b = 0; b < NCOB; b++) {
+                    if (b < nb) {
+                        const bf16x8 w = __builtin_bit_cast(
+                            bf16x8,
+                            *reinterpret_cast<const uint4*>(s_w + (tap * NCOB * 32 + b * 32 + col) * kRow + ci + 8 * hi));
+#pragma unroll
+                        for (int pb = 0; pb < 2; pb++)
+                            acc[pb][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[pb], w, acc[pb][b], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        __syncthreads();  // before the next chunk overwrites the tiles
+    }
+    // Epilogue: D row drow(r, hi) is pixel x0 + drow of row y0 + 2 wave + pb.
+#pragma unroll
+    for (int pb = 0; pb < 2; pb++) {
+        const int y = y0 + 2 * wave + pb;
+        if (y >= c.h) continue;
+#pragma unroll
+        for (int b = 0; b < NCOB; b++) {
+            if (b >= nb) continue;
+            const int co = (cob0 + b) * 32 + col;
+            if (co >= c.cout) continue;
+            const float bias = c.bias ? c.bias[co] : 0.0f;
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int x = x0 + drow(r, hi);
+                if (x >= c.w) continue;
+                const size_t q = size_t(n * c.h + y) * c.w + x;
+                float v = acc[pb][b][r] + bias;
+                if (c.relu) v = fmaxf(v, 0.0f);
+                if (c.mask && bf2f(c.mask[q * c.mask_cs + c.mask_co + co]) <= 0.0f) v = 0.0f;
+                if (c.out) c.out[q * c.out_cs + c.out_co + co] = f2bf(v);
+                if (c.out_f32) c.out_f32[q * c.cout + co] = v;
+            }
+        }
+    }
+}
+
 // Weight gradient. One wave per (tap, 32 output channels, 32 input channels)
 // and chunk of pixels; K = 16 pixels per MFMA, A = dY^T (lane: output channel,
 // 8 consecutive pixels), B = shifted x (lane: input channel, 8 pixels).
@@ -369,6 +466,17 @@ void launch_conv3x3(const UnetConv& c, hipStream_t st) {
     k.co_blocks = (c.cout + 31) / 32;
     // Output blocks per wave: up to 4 (A fragments reused across them), fewer
     // when that leaves under 2048 waves (deep levels: latency wants waves).
+    // Many pixels: 2-D tiles with the input and weights in LDS, 1 or 2 output
+    // blocks per workgroup, while that leaves 2048 waves.
+    const int tiles = ((c.w + kTileW - 1) / kTileW) * ((c.h + kTileH - 1) / kTileH) * c.n;
+    for (int nc : {2, 1}) {
+        const int groups = (k.co_blocks + nc - 1) / nc;
+        if (nc > k.co_blocks || tiles * 4 * groups < 2048) continue;
+        const dim3 grid((c.w + kTileW - 1) / kTileW, (c.h + kTileH - 1) / kTileH, c.n * groups);
+        if (nc == 2) hipLaunchKernelGGL(k_conv3x3_tile<2>, grid, dim3(256), 0, st, k, groups);
+        else hipLaunchKernelGGL(k_conv3x3_tile<1>, grid, dim3(256), 0, st, k, groups);
+        return;
+    }
     const int pix_waves = (npix + 31) / 32;
     int ncob = std::min(4, k.co_blocks);
     while (ncob > 1 && pix_waves * ((k.co_blocks + ncob - 1) / ncob) < 2048) ncob--;
