@@ -51,7 +51,7 @@ def render_once(r, params, w=8, h=8):
     r.synchronize()
 
 
-@pytest.mark.parametrize("dims", [(256, 256, 256), (77, 50, 43)])
+@pytest.mark.parametrize("dims", [(256, 256, 256), (77, 50, 43), (5, 3, 1), (9, 1, 17), (1, 1, 1)])
 def test_majorant_mode_off_bit_exact(ctx, oracle, dims):
     hu = phantom.ct(dims, 1)
     tf = TransferFunction.preset(0)
@@ -62,7 +62,8 @@ def test_majorant_mode_off_bit_exact(ctx, oracle, dims):
     u = oracle_uniforms(oracle, r, hu, p, tf)
     want = oracle.Scene(oracle.hu_to_unorm16(hu), tf=tf).majorant(u)
     assert np.array_equal(got, want)
-    assert (got > 0).mean() < 0.9 and (got == 0).any()  # bricks are skipped
+    if min(dims) > 8:
+        assert (got > 0).mean() < 0.9 and (got == 0).any()  # bricks are skipped
 
 
 def organ_scene(ctx, dims, seed, mode, soft=True, **label_kw):
