@@ -275,64 +275,82 @@ __global__ void __launch_bounds__(kBlock) k_majorant(const uint16_t* __restrict_
     const int tbx = blockIdx.x * kTile, tby = blockIdx.y * kTile, tbz = blockIdx.z * kTile;
     // Cell c covers voxels 2c, 2c + 1; the tile's first cell is 4 * tb - 1. A
     // 4^3 micro-brick (64 contiguous voxels) holds 2^3 cells: the tile's cells
-    // lie in micro-bricks 2 tb - 1 .. 2 tb + 2 kTile, one per thread at a time,
-    // read with 16-byte loads.
+    // lie in micro-bricks 2 tb - 1 .. 2 tb + 2 kTile. Eight neighbouring lanes
+    // read one micro-brick with one 16-byte load each (density; two of mask),
+    // so a wave's loads cover whole lines, and combine their cells by shuffles.
+    //   density chunk k (voxels 8k .. 8k + 7): rows dy = 2 (k & 1) + {0, 1} of
+    //     dz = k >> 1, i.e. cells cx = 0, 1 of cy = k & 1, cz = k >> 2; lane
+    //     k ^ 2 holds the cells' other dz;
+    //   mask chunks k and k + 8 (4 voxels each): row dy = k & 3 of dz = k >> 2
+    //     and 2 + (k >> 2), i.e. cells cx = 0, 1 of cy = (k >> 1) & 1 and
+    //     cz = 0, 1; lanes k ^ 1 and k ^ 4 hold the cells' other rows.
     const int c0x = 4 * tbx - 1, c0y = 4 * tby - 1, c0z = 4 * tbz - 1;
     constexpr int kMb = 2 * kTile + 2;  // micro-bricks per axis
-    for (int t = threadIdx.x; t < kMb * kMb * kMb; t += kBlock) {
-        const int mbx = 2 * tbx - 1 + t % kMb, mby = 2 * tby - 1 + (t / kMb) % kMb, mbz = 2 * tbz - 1 + t / (kMb * kMb);
-        uint32_t lo[8], hi[8], mm[8];
-#pragma unroll
-        for (int c = 0; c < 8; c++) lo[c] = 0xffffu, hi[c] = 0u, mm[c] = 0u;
+    static_assert((kMb * kMb * kMb * 8) % 64 == 0, "every wave runs the loop's iterations whole");
+    for (int t = threadIdx.x; t < kMb * kMb * kMb * 8; t += kBlock) {
+        const int m = t >> 3, k = t & 7;
+        const int mbx = 2 * tbx - 1 + m % kMb, mby = 2 * tby - 1 + (m / kMb) % kMb, mbz = 2 * tbz - 1 + m / (kMb * kMb);
+        uint32_t lo[2] = {0xffffu, 0xffffu}, hi[2] = {0u, 0u}, mm[4] = {0u, 0u, 0u, 0u};  // mm[cx + 2 cz]
         if (mbx >= 0 && mby >= 0 && mbz >= 0 && mbx < g.bx && mby < g.by && mbz < g.bz) {
             const uint32_t base = g.at(4 * mbx, 4 * mby, 4 * mbz);  // the micro-brick's first voxel
             // Voxel v = dx + 4 dy + 16 dz of the micro-brick; inside the volume?
             auto inside = [&](int v) {
                 return 4 * mbx + (v & 3) < g.nx && 4 * mby + ((v >> 2) & 3) < g.ny && 4 * mbz + (v >> 4) < g.nz;
             };
-            auto cell = [](int v) { return ((v >> 1) & 1) | (((v >> 3) & 1) << 1) | ((v >> 5) << 2); };
-            const uint4* dv = reinterpret_cast<const uint4*>(d + base);
+            const uint4 w4 = reinterpret_cast<const uint4*>(d + base)[k];
+            const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
 #pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const uint4 w4 = dv[q];
-                const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
-#pragma unroll
-                for (int k = 0; k < 4; k++)
-#pragma unroll
-                    for (int h = 0; h < 2; h++) {
-                        const int v = 8 * q + 2 * k + h;
-                        if (!inside(v)) continue;
-                        const uint32_t x = h ? (w[k] >> 16) : (w[k] & 0xffffu);
-                        const int c = cell(v);
-                        lo[c] = min(lo[c], x);
-                        hi[c] = max(hi[c], x);
-                    }
+            for (int j = 0; j < 8; j++) {
+                const int v = 8 * k + j;
+                if (!inside(v)) continue;
+                const uint32_t x = (j & 1) ? (w[j >> 1] >> 16) : (w[j >> 1] & 0xffffu);
+                const int cx = (j >> 1) & 1;
+                lo[cx] = min(lo[cx], x);
+                hi[cx] = max(hi[cx], x);
             }
             if (mode != 0) {
                 const uint4* mv = reinterpret_cast<const uint4*>(mask + base);
 #pragma unroll
-                for (int q = 0; q < 16; q++) {
-                    const uint4 w4 = mv[q];
-                    const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
+                for (int half = 0; half < 2; half++) {
+                    const int q = k + 8 * half;
+                    const uint4 m4 = mv[q];
+                    const uint32_t mw[4] = {m4.x, m4.y, m4.z, m4.w};
 #pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        const int v = 4 * q + k;
-                        if (!inside(v)) continue;
-                        const int c = cell(v);
-                        mm[c] = max(mm[c], max(w[k] & 0xffffu, w[k] >> 16));
+                    for (int dx = 0; dx < 4; dx++) {
+                        if (!inside(4 * q + dx)) continue;
+                        uint32_t& c = mm[(dx >> 1) + 2 * half];
+                        c = max(c, max(mw[dx] & 0xffffu, mw[dx] >> 16));
                     }
                 }
             }
         }
-        // The micro-brick's cells that lie in the tile (all of them but halo halves).
 #pragma unroll
-        for (int c = 0; c < 8; c++) {
-            const int cx = 2 * mbx + (c & 1) - c0x, cy = 2 * mby + ((c >> 1) & 1) - c0y, cz = 2 * mbz + (c >> 2) - c0z;
-            if (cx < 0 || cy < 0 || cz < 0 || cx >= kCells || cy >= kCells || cz >= kCells) continue;
-            const int i = cx + kCells * (cy + kCells * cz);
-            s_minmax[i] = lo[c] | (hi[c] << 16);
-            s_mask[i] = uint16_t(mm[c]);
+        for (int c = 0; c < 2; c++) {
+            lo[c] = min(lo[c], uint32_t(__shfl_xor(int(lo[c]), 2)));
+            hi[c] = max(hi[c], uint32_t(__shfl_xor(int(hi[c]), 2)));
         }
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            mm[c] = max(mm[c], uint32_t(__shfl_xor(int(mm[c]), 1)));
+            mm[c] = max(mm[c], uint32_t(__shfl_xor(int(mm[c]), 4)));
+        }
+        // Owners write the micro-brick's cells that lie in the tile (all of
+        // them but halo halves): lanes 0, 1, 4, 5 the density cells, lanes 0, 2
+        // the mask cells.
+        auto in_tile = [&](int cx, int cy, int cz, int& i) {
+            cx += 2 * mbx - c0x, cy += 2 * mby - c0y, cz += 2 * mbz - c0z;
+            i = cx + kCells * (cy + kCells * cz);
+            return cx >= 0 && cy >= 0 && cz >= 0 && cx < kCells && cy < kCells && cz < kCells;
+        };
+        int i;
+        if ((k & 2) == 0)
+#pragma unroll
+            for (int cx = 0; cx < 2; cx++)
+                if (in_tile(cx, k & 1, k >> 2, i)) s_minmax[i] = lo[cx] | (hi[cx] << 16);
+        if ((k & 5) == 0)
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                if (in_tile(c & 1, (k >> 1) & 1, c >> 1, i)) s_mask[i] = uint16_t(mm[c]);
     }
     __syncthreads();
     if (threadIdx.x >= kTile * kTile * kTile) return;
