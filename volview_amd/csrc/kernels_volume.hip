@@ -73,43 +73,66 @@ __device__ __forceinline__ uint32_t snorm8_bits(float f) {
 }
 
 // normals.comp:14-37: 3x3x3 Sobel, world gradient, outward normal and surfaceness.
-__global__ void k_normals(const uint16_t* __restrict__ d, uint32_t* __restrict__ out, BrickGrid g,
-                          const vvr_uniforms* __restrict__ u) {
-    const uint32_t o = blockIdx.x * kBlock + threadIdx.x;
-    if (o >= g.padded()) return;
-    int x, y, z;
-    brick_coords(g, o, x, y, z);
-    if (x >= g.nx || y >= g.ny || z >= g.nz) {
-        out[o] = 0u;
-        return;
+// A workgroup covers 2 x 2 x 2 micro-bricks (8^3 voxels). It first stages the
+// tile's densities with a one-voxel halo in LDS (10^3 values, each the clamped
+// texel as dtexel reads it), then every thread runs the stencil for two voxels
+// from LDS with the same arithmetic in the same order. Each wave writes one
+// micro-brick's 64 contiguous outputs.
+constexpr int kNormTile = 8;                // voxels per axis
+constexpr int kNormHalo = kNormTile + 2;  // with the halo
+__global__ void __launch_bounds__(kBlock) k_normals(const uint16_t* __restrict__ d, uint32_t* __restrict__ out,
+                                                    BrickGrid g, const vvr_uniforms* __restrict__ u) {
+    __shared__ float s_d[kNormHalo * kNormHalo * kNormHalo];
+    const int x0 = int(blockIdx.x) * kNormTile, y0 = int(blockIdx.y) * kNormTile, z0 = int(blockIdx.z) * kNormTile;
+    for (int i = threadIdx.x; i < kNormHalo * kNormHalo * kNormHalo; i += kBlock) {
+        const int hx = i % kNormHalo, hy = (i / kNormHalo) % kNormHalo, hz = i / (kNormHalo * kNormHalo);
+        s_d[i] = dtexel(d, g, x0 - 1 + hx, y0 - 1 + hy, z0 - 1 + hz);
     }
-    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
-    for (int a = -1; a <= 1; a++) {
-        for (int b = -1; b <= 1; b++) {
-            const float w = float((2 - abs(a)) * (2 - abs(b)));
-            gx += w * (dtexel(d, g, x + 1, y + a, z + b) - dtexel(d, g, x - 1, y + a, z + b));
-            gy += w * (dtexel(d, g, x + a, y + 1, z + b) - dtexel(d, g, x + a, y - 1, z + b));
-            gz += w * (dtexel(d, g, x + a, y + b, z + 1) - dtexel(d, g, x + a, y + b, z - 1));
-        }
-    }
-    gx /= 32.0f;
-    gy /= 32.0f;
-    gz /= 32.0f;
+    __syncthreads();
     const float* m = u->world_to_index;
-    // transpose(mat3(world_to_index)) * g
-    const float wx = m[0] * gx + m[1] * gy + m[2] * gz;
-    const float wy = m[4] * gx + m[5] * gy + m[6] * gz;
-    const float wz = m[8] * gx + m[9] * gy + m[10] * gz;
-    const float len = sqrtf(wx * wx + wy * wy + wz * wz);
-    float nx = 0.0f, ny = 0.0f, nz = 0.0f;
-    if (len > 1e-7f) {
-        nx = -wx / len;
-        ny = -wy / len;
-        nz = -wz / len;
+#pragma unroll 1
+    for (int half = 0; half < 2; half++) {
+        const int mb = (threadIdx.x >> 6) + 4 * half, w = threadIdx.x & 63;  // micro-brick of the tile, voxel in it
+        const int mbx = 2 * int(blockIdx.x) + (mb & 1), mby = 2 * int(blockIdx.y) + ((mb >> 1) & 1),
+                  mbz = 2 * int(blockIdx.z) + (mb >> 2);
+        if (mbx >= g.bx || mby >= g.by || mbz >= g.bz) continue;  // no such micro-brick (whole waves)
+        const int x = 4 * mbx + (w & 3), y = 4 * mby + ((w >> 2) & 3), z = 4 * mbz + (w >> 4);
+        const uint32_t o = g.at(x, y, z);
+        if (x >= g.nx || y >= g.ny || z >= g.nz) {
+            out[o] = 0u;
+            continue;
+        }
+        // dtexel(x + i, y + j, z + k) for |i|, |j|, |k| <= 1.
+        auto t = [&](int i, int j, int k) {
+            return s_d[(x - x0 + 1 + i) + kNormHalo * ((y - y0 + 1 + j) + kNormHalo * (z - z0 + 1 + k))];
+        };
+        float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+        for (int a = -1; a <= 1; a++) {
+            for (int b = -1; b <= 1; b++) {
+                const float wt = float((2 - abs(a)) * (2 - abs(b)));
+                gx += wt * (t(1, a, b) - t(-1, a, b));
+                gy += wt * (t(a, 1, b) - t(a, -1, b));
+                gz += wt * (t(a, b, 1) - t(a, b, -1));
+            }
+        }
+        gx /= 32.0f;
+        gy /= 32.0f;
+        gz /= 32.0f;
+        // transpose(mat3(world_to_index)) * g
+        const float wx = m[0] * gx + m[1] * gy + m[2] * gz;
+        const float wy = m[4] * gx + m[5] * gy + m[6] * gz;
+        const float wz = m[8] * gx + m[9] * gy + m[10] * gz;
+        const float len = sqrtf(wx * wx + wy * wy + wz * wz);
+        float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+        if (len > 1e-7f) {
+            nx = -wx / len;
+            ny = -wy / len;
+            nz = -wz / len;
+        }
+        float sness = len * 400.0f;
+        sness = sness < 0.0f ? 0.0f : (sness > 1.0f ? 1.0f : sness);
+        out[o] = snorm8_bits(nx) | (snorm8_bits(ny) << 8) | (snorm8_bits(nz) << 16) | (snorm8_bits(sness) << 24);
     }
-    float sness = len * 400.0f;
-    sness = sness < 0.0f ? 0.0f : (sness > 1.0f ? 1.0f : sness);
-    out[o] = snorm8_bits(nx) | (snorm8_bits(ny) << 8) | (snorm8_bits(nz) << 16) | (snorm8_bits(sness) << 24);
 }
 
 __device__ __forceinline__ uint32_t unorm16_bits(float f) {
@@ -406,7 +429,8 @@ void launch_debrick(const void* in, void* out, const BrickGrid& g, int elem_byte
 
 void launch_normals(const uint16_t* density, uint32_t* normals, const BrickGrid& g, const vvr_uniforms* u,
                     hipStream_t st) {
-    hipLaunchKernelGGL(k_normals, dim3(blocks_for(g.padded())), dim3(kBlock), 0, st, density, normals, g, u);
+    const dim3 grid(unsigned(g.bx + 1) / 2, unsigned(g.by + 1) / 2, unsigned(g.bz + 1) / 2);
+    hipLaunchKernelGGL(k_normals, grid, dim3(kBlock), 0, st, density, normals, g, u);
 }
 
 void launch_mask_binary(const uint16_t* density, const uint8_t* labels, SoftGrid soft, int margin, uint32_t* dst,
