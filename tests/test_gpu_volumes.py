@@ -84,6 +84,41 @@ def organ_scene(ctx, dims, seed, mode, soft=True, **label_kw):
     return hu, labels, sg, tf, r, p
 
 
+@pytest.mark.parametrize("soft_dims,src_axis,flip", [((23, 17, 29), (2, 0, 1), (1, 0, 1)),
+                                                     ((70, 41, 30), (1, 2, 0), (0, 1, 0))])
+def test_mask_soft_grid_other_shape(ctx, oracle, soft_dims, src_axis, flip):
+    """A soft grid coarser than the volume (its node densities staged once)
+    and one with more nodes than the volume has voxels (fetched per window),
+    each following permuted and flipped volume axes, match the oracle."""
+    dims = (36, 30, 26)
+    hu = phantom.ct(dims, 9)
+    labels, _, _, n = phantom.organs(dims, 9, 12)
+    rng = np.random.default_rng(sum(soft_dims))
+    sd0, sd1, sd2 = soft_dims
+    sg = np.zeros((sd2, sd1, sd0, 4), np.uint8)
+    sg[..., 0] = rng.integers(0, n + 1, sg.shape[:3])
+    sg[..., 1] = rng.integers(0, n + 1, sg.shape[:3])
+    sg[..., 2] = rng.integers(0, 80, sg.shape[:3])
+    sg[..., 3] = rng.random(sg.shape[:3]) < 0.05
+    tf = TransferFunction.preset(0)
+    r = Renderer(ctx, hu, tf)
+    r.set_labels(labels, sg, soft_dims=soft_dims, src_axis=src_axis, flip=flip)
+    p = frame_params(hu)
+    p.labels.mode = _abi.LABELS_ISOLATE
+    p.labels.colors[2][3] = 0.0  # a hidden label
+    p.labels.ghost[4] = 1
+    render_once(r, p)
+    got = r.read_volume(_abi.VOL_MASK)
+    u = oracle_uniforms(oracle, r, hu, p, tf, has_labels=True)
+    sc = oracle.Scene(oracle.hu_to_unorm16(hu), tf=tf, labels=labels)
+    flags = flip[0] | (flip[1] << 1) | (flip[2] << 2) | 8
+    want = sc.mask(u, sg, soft_dims, (*src_axis, flags), margin=0, radius=1)
+    diff = np.abs(got.astype(np.int64) - want.astype(np.int64))
+    assert diff.max() <= 1, diff.max()
+    assert (diff > 0).mean() < 5e-3
+    assert got[..., 0].min() < got[..., 0].max() and got[..., 1].max() > 0  # neither constant nor unghosted
+
+
 @pytest.mark.parametrize("mode", [_abi.LABELS_TINT, _abi.LABELS_ISOLATE, _abi.LABELS_HYBRID])
 def test_mask_binary_path_bit_exact(ctx, oracle, mode):
     hu, labels, sg, tf, r, p = organ_scene(ctx, (70, 64, 58), 4, mode, soft=False)

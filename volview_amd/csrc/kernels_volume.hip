@@ -171,9 +171,67 @@ __device__ __forceinline__ float density_linear(const uint16_t* __restrict__ d, 
     return c0 * (1.0f - az) + c1 * az;
 }
 
+// The soft grid's mapping (mask.comp:40-60): for grid axis a, the volume axis
+// it follows, that axis's extent `can` and the grid-per-voxel scale `to_grid`.
+struct SoftAxes {
+    int axis[3];
+    float can[3], to_grid[3];
+};
+__device__ __forceinline__ SoftAxes soft_axes(const SoftGrid& soft, const BrickGrid& g) {
+    SoftAxes s;
+    const int dims[3] = {g.nx, g.ny, g.nz};
+    for (int a = 0; a < 3; a++) {
+        s.axis[a] = soft.push[a];
+        s.can[a] = float(dims[s.axis[a]]);
+        const float den = fmaxf(s.can[a] - 1.0f, 1.0f);
+        s.to_grid[a] = (float(soft.dims[a]) - 1.0f) / den;
+    }
+    return s;
+}
+
+// The density at grid node n, trilinear at the node's volume position.
+__device__ __forceinline__ float soft_node_density(const uint16_t* __restrict__ d, const BrickGrid& g,
+                                                   const SoftGrid& soft, const SoftAxes& s, const int n[3]) {
+    float v[3];
+    for (int a = 0; a < 3; a++) {
+        const float c = float(n[a]) / s.to_grid[a];
+        v[s.axis[a]] = ((soft.push[3] >> a) & 1) != 0 ? s.can[a] - 1.0f - c : c;
+    }
+    return density_linear(d, g, v[0], v[1], v[2]);
+}
+
+// Every grid node's density, once, for k_mask_binary's 27-node windows (which
+// would otherwise fetch it 27 times per voxel); grid axis 0 fastest.
+__global__ void k_soft_node_density(const uint16_t* __restrict__ d, SoftGrid soft, BrickGrid g,
+                                    float* __restrict__ there) {
+    const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
+    const size_t n01 = size_t(soft.dims[0]) * size_t(soft.dims[1]);
+    if (i >= n01 * size_t(soft.dims[2])) return;
+    const int n[3] = {int(i % size_t(soft.dims[0])), int((i / size_t(soft.dims[0])) % size_t(soft.dims[1])),
+                      int(i / n01)};
+    there[i] = soft_node_density(d, g, soft, soft_axes(soft, g), n);
+}
+
+// Per label: bit 0 solid in the frame's mode, bit 1 ghosted, bit 2 shown.
+constexpr uint32_t kLabelSolid = 1u, kLabelGhosted = 2u, kLabelShown = 4u;
+
 // mask.comp:107-138 (binary_masks), with soft_coord / soft_masks (mask.comp:40-100).
-__global__ void k_mask_binary(const uint16_t* __restrict__ d, const uint8_t* __restrict__ labels, SoftGrid soft,
-                              int margin, uint32_t* __restrict__ out, BrickGrid g, const vvr_uniforms* __restrict__ u) {
+// `there` holds every soft-grid node's density (k_soft_node_density), or is
+// null and each window fetches it.
+__global__ void __launch_bounds__(kBlock) k_mask_binary(const uint16_t* __restrict__ d,
+                                                        const uint8_t* __restrict__ labels, SoftGrid soft,
+                                                        const float* __restrict__ there_of, int margin,
+                                                        uint32_t* __restrict__ out, BrickGrid g,
+                                                        const vvr_uniforms* __restrict__ u) {
+    __shared__ uint8_t s_label[256];
+    const uint32_t mode = uint32_t(u->labels[0]);
+    static_assert(kBlock == 256, "one label per thread");
+    {
+        const uint32_t l = threadIdx.x;
+        s_label[l] = uint8_t((solid(u, l, mode) ? kLabelSolid : 0u) | (ghosted(u, l) ? kLabelGhosted : 0u) |
+                             (shown(u, l) ? kLabelShown : 0u));
+    }
+    __syncthreads();
     const uint32_t o = blockIdx.x * kBlock + threadIdx.x;
     if (o >= g.padded()) return;
     int x, y, z;
@@ -182,30 +240,28 @@ __global__ void k_mask_binary(const uint16_t* __restrict__ d, const uint8_t* __r
         out[o] = 0u;
         return;
     }
-    const uint32_t mode = uint32_t(u->labels[0]);
+    auto has = [&](uint32_t l, uint32_t bit) { return (s_label[l] & bit) != 0u; };
     const uint32_t label = labels[o];
-    float m0 = solid(u, label, mode) ? 1.0f : 0.0f;
-    float m1 = ghosted(u, label) ? 1.0f : 0.0f;
+    float m0 = has(label, kLabelSolid) ? 1.0f : 0.0f;
+    float m1 = has(label, kLabelGhosted) ? 1.0f : 0.0f;
     if ((soft.push[3] & 8) != 0 && label < 250u) {
         const int p[3] = {x, y, z};
-        const int axis[3] = {soft.push[0], soft.push[1], soft.push[2]};
-        const int dims[3] = {g.nx, g.ny, g.nz};
-        float can[3], xs[3], to_grid[3];
+        const SoftAxes sa = soft_axes(soft, g);
+        float xs[3];
         for (int a = 0; a < 3; a++) {
-            can[a] = float(dims[axis[a]]);
-            const float v = float(p[axis[a]]);
-            xs[a] = ((soft.push[3] >> a) & 1) != 0 ? can[a] - 1.0f - v : v;
-            const float den = fmaxf(can[a] - 1.0f, 1.0f);
-            to_grid[a] = (float(soft.dims[a]) - 1.0f) / den;
+            const float v = float(p[sa.axis[a]]);
+            xs[a] = ((soft.push[3] >> a) & 1) != 0 ? sa.can[a] - 1.0f - v : v;
+            const float den = fmaxf(sa.can[a] - 1.0f, 1.0f);
             xs[a] = xs[a] * (float(soft.dims[a]) - 1.0f) / den;
         }
-        auto node = [&](int a, int b, int c) {
-            return soft.nodes[size_t(a) + size_t(soft.dims[0]) * (size_t(b) + size_t(soft.dims[1]) * c)];
+        auto node_index = [&](int a, int b, int c) {
+            return size_t(a) + size_t(soft.dims[0]) * (size_t(b) + size_t(soft.dims[1]) * c);
         };
         int cx = int(roundf(xs[0])), cy = int(roundf(xs[1])), cz = int(roundf(xs[2]));
-        const bool overridden =
-            (node(min(max(cx, 0), soft.dims[0] - 1), min(max(cy, 0), soft.dims[1] - 1), min(max(cz, 0), soft.dims[2] - 1)) >>
-             24) != 0u;
+        const bool overridden = (soft.nodes[node_index(min(max(cx, 0), soft.dims[0] - 1),
+                                                       min(max(cy, 0), soft.dims[1] - 1),
+                                                       min(max(cz, 0), soft.dims[2] - 1))] >>
+                                 24) != 0u;
         if (!overridden) {
             const float here = dtexel(d, g, x, y, z);
             const float kSigmaCt = 40.0f / 4095.0f;
@@ -215,20 +271,17 @@ __global__ void k_mask_binary(const uint16_t* __restrict__ d, const uint8_t* __r
                     for (int dx = -1; dx <= 1; dx++) {
                         const int n[3] = {min(max(cx + dx, 0), soft.dims[0] - 1), min(max(cy + dy, 0), soft.dims[1] - 1),
                                           min(max(cz + dz, 0), soft.dims[2] - 1)};
-                        float dd[3], v[3];
+                        float dd[3];
                         for (int a = 0; a < 3; a++) dd[a] = float(n[a]) - xs[a];
                         const float w_space = expf(-(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]) / (2.0f * 0.6f * 0.6f));
-                        for (int a = 0; a < 3; a++) {
-                            const float c = float(n[a]) / to_grid[a];
-                            v[axis[a]] = ((soft.push[3] >> a) & 1) != 0 ? can[a] - 1.0f - c : c;
-                        }
-                        const float there = density_linear(d, g, v[0], v[1], v[2]);
+                        const size_t ni = node_index(n[0], n[1], n[2]);
+                        const float there = there_of ? there_of[ni] : soft_node_density(d, g, soft, sa, n);
                         const float w_ct = expf(-(here - there) * (here - there) / (2.0f * kSigmaCt * kSigmaCt));
-                        const uint32_t t = node(n[0], n[1], n[2]);
+                        const uint32_t t = soft.nodes[ni];
                         const uint32_t best = t & 0xffu, second = (t >> 8) & 0xffu;
                         const float mg = float((t >> 16) & 0xffu) / 16.0f;
-                        const float sc0 = soft_score(solid(u, best, mode), solid(u, second, mode), mg);
-                        const float sc1 = soft_score(ghosted(u, best), ghosted(u, second), mg);
+                        const float sc0 = soft_score(has(best, kLabelSolid), has(second, kLabelSolid), mg);
+                        const float sc1 = soft_score(has(best, kLabelGhosted), has(second, kLabelGhosted), mg);
                         const float w = w_space * w_ct + 1e-6f * w_space;
                         s0 += w * sc0;
                         s1 += w * sc1;
@@ -247,7 +300,7 @@ __global__ void k_mask_binary(const uint16_t* __restrict__ d, const uint8_t* __r
                     const int qx = min(max(x + a, 0), g.nx - 1), qy = min(max(y + b, 0), g.ny - 1),
                               qz = min(max(z + c, 0), g.nz - 1);
                     const uint32_t l = labels[g.at(qx, qy, qz)];
-                    if (l != 0u && (!shown(u, l) || ghosted(u, l))) {
+                    if (l != 0u && (!has(l, kLabelShown) || has(l, kLabelGhosted))) {
                         m0 = 0.0f;
                         break;
                     }
@@ -434,9 +487,16 @@ void launch_normals(const uint16_t* density, uint32_t* normals, const BrickGrid&
 }
 
 void launch_mask_binary(const uint16_t* density, const uint8_t* labels, SoftGrid soft, int margin, uint32_t* dst,
-                        const BrickGrid& g, const vvr_uniforms* u, hipStream_t st) {
-    hipLaunchKernelGGL(k_mask_binary, dim3(blocks_for(g.padded())), dim3(kBlock), 0, st, density, labels, soft, margin,
-                       dst, g, u);
+                        void* scratch, size_t scratch_bytes, const BrickGrid& g, const vvr_uniforms* u,
+                        hipStream_t st) {
+    float* there = nullptr;
+    const size_t nodes = size_t(soft.dims[0]) * size_t(soft.dims[1]) * size_t(soft.dims[2]);
+    if ((soft.push[3] & 8) != 0 && nodes * sizeof(float) <= scratch_bytes) {
+        there = static_cast<float*>(scratch);
+        hipLaunchKernelGGL(k_soft_node_density, dim3(blocks_for(nodes)), dim3(kBlock), 0, st, density, soft, g, there);
+    }
+    hipLaunchKernelGGL(k_mask_binary, dim3(blocks_for(g.padded())), dim3(kBlock), 0, st, density, labels, soft,
+                       static_cast<const float*>(there), margin, dst, g, u);
 }
 
 void launch_mask_blur(const uint32_t* src, uint32_t* dst, int axis, int radius, const BrickGrid& g, hipStream_t st) {

@@ -23,8 +23,11 @@ struct SoftGrid {
     int dims[3];
     int push[4];            // source axes, flip bits | 8 when present
 };
+// `scratch` (scratch_bytes, not `dst`) may be overwritten: it holds the soft
+// grid's node densities when they fit (4 bytes per node).
 void launch_mask_binary(const uint16_t* density, const uint8_t* labels, SoftGrid soft, int margin, uint32_t* dst,
-                        const BrickGrid& g, const vvr_uniforms* u, hipStream_t st);
+                        void* scratch, size_t scratch_bytes, const BrickGrid& g, const vvr_uniforms* u,
+                        hipStream_t st);
 // mask.comp, passes 1-3: binomial blur of radius r along `axis`.
 void launch_mask_blur(const uint32_t* src, uint32_t* dst, int axis, int radius, const BrickGrid& g, hipStream_t st);
 

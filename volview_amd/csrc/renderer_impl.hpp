@@ -400,8 +400,12 @@ struct vvr_renderer {
         };
         if (opacity.mode != VVR_LABELS_OFF && (!mask_key || *mask_key != opacity)) {
             // Binary masks into B, then blur x (B -> A), y (A -> B), z (B -> A) (renderer.rs:1671-1694).
+            // A, rewritten by the first blur, holds the soft grid's node densities meanwhile.
             const int radius = int(opacity.smoothing);
-            run(VVR_KERNEL_MASK, [&] { launch_mask_binary(density, labels, soft_grid, opacity.margin, mask_b, g, du, st); });
+            run(VVR_KERNEL_MASK, [&] {
+                launch_mask_binary(density, labels, soft_grid, opacity.margin, mask_b, mask,
+                                   g.padded() * sizeof(uint32_t), g, du, st);
+            });
             run(VVR_KERNEL_MASK, [&] { launch_mask_blur(mask_b, mask, 0, radius, g, st); });
             run(VVR_KERNEL_MASK, [&] { launch_mask_blur(mask, mask_b, 1, radius, g, st); });
             run(VVR_KERNEL_MASK, [&] { launch_mask_blur(mask_b, mask, 2, radius, g, st); });
