@@ -116,6 +116,10 @@ __global__ void __launch_bounds__(256) k_conv3x3(UnetConv c) {
 // padded to 80 bytes so that the 16-byte fragment reads of 8 lanes cover the 32
 // banks once.
 constexpr int kTileH = 8, kTileW = 32, kChunk = 32, kRow = kChunk + 8;  // kRow: bf16s per LDS row
+// 16-byte loads per thread for a chunk's input tile and weights (4 per pixel or row).
+constexpr int kInLoads = ((kTileH + 2) * (kTileW + 2) * (kChunk / 8) + 255) / 256;
+template <int NCOB>
+constexpr int kWtLoads = (9 * NCOB * 32 * (kChunk / 8) + 255) / 256;
 template <int NCOB>
 __global__ void __launch_bounds__(256) k_conv3x3_tile(UnetConv c, int co_groups) {
     __shared__ uint16_t s_in[(kTileH + 2) * (kTileW + 2) * kRow];
@@ -133,30 +137,65 @@ __global__ void __launch_bounds__(256) k_conv3x3_tile(UnetConv c, int co_groups)
     for (int kc = 0; kc < c.cin; kc += kChunk) {
         const int cc = min(kChunk, c.cin - kc);  // 16 or 32
         const int parts = cc >> 3;                // 16-byte parts per pixel or weight row
-        // The input tile: pixel (y0 - 1 + ty, x0 - 1 + tx), zero outside the image.
-        for (int k = int(threadIdx.x); k < (kTileH + 2) * (kTileW + 2) * parts; k += 256) {
-            const int pix = k / parts, part = k - pix * parts;
-            const int ty = pix / (kTileW + 2), tx = pix - ty * (kTileW + 2);
-            const int y = y0 - 1 + ty, x = x0 - 1 + tx;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (y >= 0 && y < c.h && x >= 0 && x < c.w)
-                v = *reinterpret_cast<const uint4*>(c.in + size_t((n * c.h + y) * c.w + x) * c.in_cs + c.in_co + kc +
-                                                    part * 8);
-            *reinterpret_cast<uint4*>(s_in + pix * kRow + part * 8) = v;
+        // The input tile: pixel (y0 - 1 + ty, x0 - 1 + tx), zero outside the
+        // image. Every thread issues all its loads before its first LDS store,
+        // so their latencies overlap.
+        const int in_items = (kTileH + 2) * (kTileW + 2) * parts;
+        uint4 vin[kInLoads];
+#pragma unroll
+        for (int j = 0; j < kInLoads; j++) {
+            const int k = int(threadIdx.x) + 256 * j;
+            vin[j] = make_uint4(0u, 0u, 0u, 0u);
+            if (k < in_items) {
+                const int pix = k / parts, part = k - pix * parts;
+                const int ty = pix / (kTileW + 2), tx = pix - ty * (kTileW + 2);
+                const int y = y0 - 1 + ty, x = x0 - 1 + tx;
+                if (y >= 0 && y < c.h && x >= 0 && x < c.w)
+                    vin[j] = *reinterpret_cast<const uint4*>(c.in + size_t((n * c.h + y) * c.w + x) * c.in_cs +
+                                                             c.in_co + kc + part * 8);
+            }
         }
         // The weights: row (tap, b * 32 + r) holds input channels kc .. kc + cc.
-        for (int k = int(threadIdx.x); k < 9 * NCOB * 32 * parts; k += 256) {
-            const int row = k / parts, part = k - row * parts;
-            const int tap = row / (NCOB * 32), r = row - tap * (NCOB * 32);
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (r < nb * 32)
-                v = *reinterpret_cast<const uint4*>(c.wts + (size_t(cob0 * 32 + r) * 9 + tap) * c.cin + kc + part * 8);
-            *reinterpret_cast<uint4*>(s_w + row * kRow + part * 8) = v;
+        const int w_items = 9 * NCOB * 32 * parts;
+        uint4 vw[kWtLoads<NCOB>];
+#pragma unroll
+        for (int j = 0; j < kWtLoads<NCOB>; j++) {
+            const int k = int(threadIdx.x) + 256 * j;
+            vw[j] = make_uint4(0u, 0u, 0u, 0u);
+            if (k < w_items) {
+                const int row = k / parts, part = k - row * parts;
+                const int tap = row / (NCOB * 32), r = row - tap * (NCOB * 32);
+                if (r < nb * 32)
+                    vw[j] = *reinterpret_cast<const uint4*>(c.wts + (size_t(cob0 * 32 + r) * 9 + tap) * c.cin + kc +
+                                                            part * 8);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kInLoads; j++) {
+            const int k = int(threadIdx.x) + 256 * j;
+            if (k < in_items) {
+                const int pix = k / parts, part = k - pix * parts;
+                *reinterpret_cast<uint4*>(s_in + pix * kRow + part * 8) = vin[j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kWtLoads<NCOB>; j++) {
+            const int k = int(threadIdx.x) + 256 * j;
+            if (k < w_items) {
+                const int row = k / parts, part = k - row * parts;
+                *reinterpret_cast<uint4*>(s_w + row * kRow + part * 8) = vw[j];
+            }
         }
         __syncthreads();
+        // The 9 taps' MFMAs, unrolled (the chunk's second 16 channels behind a
+        // uniform test) so that the fragments' LDS reads can be issued ahead
+        // of the MFMAs using them.
+#pragma unroll
         for (int tap = 0; tap < 9; tap++) {
             const int dy = tap / 3, dx = tap % 3;  // tile coordinates are shifted by the halo
-            for (int ci = 0; ci < cc; ci += 16) {
+#pragma unroll
+            for (int ci = 0; ci < kChunk; ci += 16) {
+                if (ci >= cc) break;
                 bf16x8 a[2];
 #pragma unroll
                 for (int pb = 0; pb < 2; pb++) {
