@@ -8,6 +8,9 @@ pinned by finite differences).
   (activations are rounded to bf16 at every layer on the GPU).
 - The denoised frame equals the oracle's forward pass within bf16 rounding, on a
   small frame and on a 380 x 400 one (2-D tiled convolutions, partial tiles).
+- Layers of at most 32 input and output channels on large frames run in
+  persistent workgroups that walk tiles; the frame equals, bit for bit, the one
+  computed with one workgroup per tile (VVR_UNET_STREAM=0).
 - Training on the GPU fits a fixed batch of path-traced frames: the loss drops
   by more than a third in 150 Adam steps.
 """
@@ -90,6 +93,18 @@ def test_denoised_frame_equals_oracle_forward(ctx, h, w):
     print(f"forward: {bad.mean():.4f} of channels beyond bf16 tolerance, worst {err.max():.4g}")
     assert bad.mean() < 0.01
     assert np.allclose(got[..., 3], accum[0, ..., 3] / n, atol=1e-3)
+
+
+def test_persistent_convolutions_equal_per_tile(ctx, monkeypatch):
+    rng = np.random.default_rng(5)
+    net = Unet.pretrained(ctx)
+    accum, albedo, normal, n, _ = synthetic(rng, 1, 264, 1000)  # partial tiles on both axes
+    monkeypatch.setenv("VVR_UNET_STREAM", "1")
+    streamed = net.denoise(accum[0], albedo[0], normal[0], n)
+    monkeypatch.setenv("VVR_UNET_STREAM", "0")
+    per_tile = net.denoise(accum[0], albedo[0], normal[0], n)
+    assert np.array_equal(streamed, per_tile)
+    assert np.isfinite(streamed).all() and streamed[..., :3].max() > 0
 
 
 def test_training_fits_a_batch(ctx):

@@ -19,6 +19,8 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "launch.hpp"
 
 namespace vvr {
@@ -120,105 +122,110 @@ constexpr int kTileH = 8, kTileW = 32, kChunk = 32, kRow = kChunk + 8;  // kRow:
 constexpr int kInLoads = ((kTileH + 2) * (kTileW + 2) * (kChunk / 8) + 255) / 256;
 template <int NCOB>
 constexpr int kWtLoads = (9 * NCOB * 32 * (kChunk / 8) + 255) / 256;
+constexpr int kTileIn = (kTileH + 2) * (kTileW + 2) * kRow;  // bf16s of the staged input tile
+
+// The tile's input channels kc .. kc + 8 parts into registers: pixel
+// (y0 - 1 + ty, x0 - 1 + tx), zero outside the image. All loads are issued
+// before any is used, so their latencies overlap.
+__device__ __forceinline__ void tile_input_load(const UnetConv& c, int n, int x0, int y0, int kc, int parts,
+                                                uint4 (&v)[kInLoads]) {
+    const int items = (kTileH + 2) * (kTileW + 2) * parts;
+#pragma unroll
+    for (int j = 0; j < kInLoads; j++) {
+        const int k = int(threadIdx.x) + 256 * j;
+        v[j] = make_uint4(0u, 0u, 0u, 0u);
+        if (k < items) {
+            const int pix = k / parts, part = k - pix * parts;
+            const int ty = pix / (kTileW + 2), tx = pix - ty * (kTileW + 2);
+            const int y = y0 - 1 + ty, x = x0 - 1 + tx;
+            if (y >= 0 && y < c.h && x >= 0 && x < c.w)
+                v[j] = *reinterpret_cast<const uint4*>(c.in + size_t((n * c.h + y) * c.w + x) * c.in_cs + c.in_co +
+                                                       kc + part * 8);
+        }
+    }
+}
+__device__ __forceinline__ void tile_input_store(int parts, const uint4 (&v)[kInLoads], uint16_t* s_in) {
+    const int items = (kTileH + 2) * (kTileW + 2) * parts;
+#pragma unroll
+    for (int j = 0; j < kInLoads; j++) {
+        const int k = int(threadIdx.x) + 256 * j;
+        if (k < items) {
+            const int pix = k / parts, part = k - pix * parts;
+            *reinterpret_cast<uint4*>(s_in + pix * kRow + part * 8) = v[j];
+        }
+    }
+}
+
+// The chunk's weights into LDS: row (tap, b * 32 + r) holds input channels kc .. kc + 8 parts.
 template <int NCOB>
-__global__ void __launch_bounds__(256) k_conv3x3_tile(UnetConv c, int co_groups) {
-    __shared__ uint16_t s_in[(kTileH + 2) * (kTileW + 2) * kRow];
-    __shared__ uint16_t s_w[9 * NCOB * 32 * kRow];
+__device__ __forceinline__ void tile_weights_stage(const UnetConv& c, int cob0, int nb, int kc, int parts,
+                                                   uint16_t* s_w) {
+    const int items = 9 * NCOB * 32 * parts;
+    uint4 v[kWtLoads<NCOB>];
+#pragma unroll
+    for (int j = 0; j < kWtLoads<NCOB>; j++) {
+        const int k = int(threadIdx.x) + 256 * j;
+        v[j] = make_uint4(0u, 0u, 0u, 0u);
+        if (k < items) {
+            const int row = k / parts, part = k - row * parts;
+            const int tap = row / (NCOB * 32), r = row - tap * (NCOB * 32);
+            if (r < nb * 32)
+                v[j] = *reinterpret_cast<const uint4*>(c.wts + (size_t(cob0 * 32 + r) * 9 + tap) * c.cin + kc +
+                                                       part * 8);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kWtLoads<NCOB>; j++) {
+        const int k = int(threadIdx.x) + 256 * j;
+        if (k < items) {
+            const int row = k / parts, part = k - row * parts;
+            *reinterpret_cast<uint4*>(s_w + row * kRow + part * 8) = v[j];
+        }
+    }
+}
+
+// The 9 taps' MFMAs over cc (16 or 32) staged channels, unrolled (the second
+// 16 channels behind a uniform test) so that the fragments' LDS reads can be
+// issued ahead of the MFMAs using them. Wave w computes rows 2w, 2w + 1.
+template <int NCOB>
+__device__ __forceinline__ void tile_mfma(const uint16_t* s_in, const uint16_t* s_w, int nb, int cc,
+                                          f32x16 (&acc)[2][NCOB]) {
     const int lane = int(threadIdx.x & 63u), wave = int(threadIdx.x >> 6);
     const int col = lane & 31, hi = lane >> 5;
-    const int x0 = int(blockIdx.x) * kTileW, y0 = int(blockIdx.y) * kTileH;
-    const int n = int(blockIdx.z) / co_groups, cob0 = (int(blockIdx.z) % co_groups) * NCOB;
-    const int nb = min(NCOB, c.co_blocks - cob0);
-    f32x16 acc[2][NCOB];
 #pragma unroll
-    for (int pb = 0; pb < 2; pb++)
+    for (int tap = 0; tap < 9; tap++) {
+        const int dy = tap / 3, dx = tap % 3;  // tile coordinates are shifted by the halo
 #pragma unroll
-        for (int b = 0; b < NCOB; b++) acc[pb][b] = f32x16{};
-    for (int kc = 0; kc < c.cin; kc += kChunk) {
-        const int cc = min(kChunk, c.cin - kc);  // 16 or 32
-        const int parts = cc >> 3;                // 16-byte parts per pixel or weight row
-        // The input tile: pixel (y0 - 1 + ty, x0 - 1 + tx), zero outside the
-        // image. Every thread issues all its loads before its first LDS store,
-        // so their latencies overlap.
-        const int in_items = (kTileH + 2) * (kTileW + 2) * parts;
-        uint4 vin[kInLoads];
+        for (int ci = 0; ci < kChunk; ci += 16) {
+            if (ci >= cc) break;
+            bf16x8 a[2];
 #pragma unroll
-        for (int j = 0; j < kInLoads; j++) {
-            const int k = int(threadIdx.x) + 256 * j;
-            vin[j] = make_uint4(0u, 0u, 0u, 0u);
-            if (k < in_items) {
-                const int pix = k / parts, part = k - pix * parts;
-                const int ty = pix / (kTileW + 2), tx = pix - ty * (kTileW + 2);
-                const int y = y0 - 1 + ty, x = x0 - 1 + tx;
-                if (y >= 0 && y < c.h && x >= 0 && x < c.w)
-                    vin[j] = *reinterpret_cast<const uint4*>(c.in + size_t((n * c.h + y) * c.w + x) * c.in_cs +
-                                                             c.in_co + kc + part * 8);
+            for (int pb = 0; pb < 2; pb++) {
+                const int ty = 2 * wave + pb + dy, tx = col + dx;
+                a[pb] = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const uint4*>(s_in + (ty * (kTileW + 2) + tx) * kRow + ci + 8 * hi));
             }
-        }
-        // The weights: row (tap, b * 32 + r) holds input channels kc .. kc + cc.
-        const int w_items = 9 * NCOB * 32 * parts;
-        uint4 vw[kWtLoads<NCOB>];
 #pragma unroll
-        for (int j = 0; j < kWtLoads<NCOB>; j++) {
-            const int k = int(threadIdx.x) + 256 * j;
-            vw[j] = make_uint4(0u, 0u, 0u, 0u);
-            if (k < w_items) {
-                const int row = k / parts, part = k - row * parts;
-                const int tap = row / (NCOB * 32), r = row - tap * (NCOB * 32);
-                if (r < nb * 32)
-                    vw[j] = *reinterpret_cast<const uint4*>(c.wts + (size_t(cob0 * 32 + r) * 9 + tap) * c.cin + kc +
-                                                            part * 8);
-            }
-        }
+            for (int b = 0; b < NCOB; b++) {
+                if (b < nb) {
+                    const bf16x8 w = __builtin_bit_cast(
+                        bf16x8,
+                        *reinterpret_cast<const uint4*>(s_w + (tap * NCOB * 32 + b * 32 + col) * kRow + ci + 8 * hi));
 #pragma unroll
-        for (int j = 0; j < kInLoads; j++) {
-            const int k = int(threadIdx.x) + 256 * j;
-            if (k < in_items) {
-                const int pix = k / parts, part = k - pix * parts;
-                *reinterpret_cast<uint4*>(s_in + pix * kRow + part * 8) = vin[j];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < kWtLoads<NCOB>; j++) {
-            const int k = int(threadIdx.x) + 256 * j;
-            if (k < w_items) {
-                const int row = k / parts, part = k - row * parts;
-                *reinterpret_cast<uint4*>(s_w + row * kRow + part * 8) = vw[j];
-            }
-        }
-        __syncthreads();
-        // The 9 taps' MFMAs, unrolled (the chunk's second 16 channels behind a
-        // uniform test) so that the fragments' LDS reads can be issued ahead
-        // of the MFMAs using them.
-#pragma unroll
-        for (int tap = 0; tap < 9; tap++) {
-            const int dy = tap / 3, dx = tap % 3;  // tile coordinates are shifted by the halo
-#pragma unroll
-            for (int ci = 0; ci < kChunk; ci += 16) {
-                if (ci >= cc) break;
-                bf16x8 a[2];
-#pragma unroll
-                for (int pb = 0; pb < 2; pb++) {
-                    const int ty = 2 * wave + pb + dy, tx = col + dx;
-                    a[pb] = __builtin_bit_cast(
-                        bf16x8, *reinterpret_cast<const uint4*>(s_in + (ty * (kTileW + 2) + tx) * kRow + ci + 8 * hi));
-                }
-#pragma unroll
-                for (int b = 0; b < NCOB; b++) {
-                    if (b < nb) {
-                        const bf16x8 w = __builtin_bit_cast(
-                            bf16x8,
-                            *reinterpret_cast<const uint4*>(s_w + (tap * NCOB * 32 + b * 32 + col) * kRow + ci + 8 * hi));
-#pragma unroll
-                        for (int pb = 0; pb < 2; pb++)
-                            acc[pb][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[pb], w, acc[pb][b], 0, 0, 0);
-                    }
+                    for (int pb = 0; pb < 2; pb++)
+                        acc[pb][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[pb], w, acc[pb][b], 0, 0, 0);
                 }
             }
         }
-        __syncthreads();  // before the next chunk overwrites the tiles
     }
-    // Epilogue: D row drow(r, hi) is pixel x0 + drow of row y0 + 2 wave + pb.
+}
+
+// Epilogue: D row drow(r, hi) is pixel x0 + drow of row y0 + 2 wave + pb.
+template <int NCOB>
+__device__ __forceinline__ void tile_epilogue(const UnetConv& c, int n, int x0, int y0, int cob0, int nb,
+                                              const f32x16 (&acc)[2][NCOB]) {
+    const int lane = int(threadIdx.x & 63u), wave = int(threadIdx.x >> 6);
+    const int col = lane & 31, hi = lane >> 5;
 #pragma unroll
     for (int pb = 0; pb < 2; pb++) {
         const int y = y0 + 2 * wave + pb;
@@ -241,6 +248,72 @@ __global__ void __launch_bounds__(256) k_conv3x3_tile(UnetConv c, int co_groups)
                 if (c.out_f32) c.out_f32[q * c.cout + co] = v;
             }
         }
+    }
+}
+
+template <int NCOB>
+__global__ void __launch_bounds__(256) k_conv3x3_tile(UnetConv c, int co_groups) {
+    __shared__ uint16_t s_in[kTileIn];
+    __shared__ uint16_t s_w[9 * NCOB * 32 * kRow];
+    const int x0 = int(blockIdx.x) * kTileW, y0 = int(blockIdx.y) * kTileH;
+    const int n = int(blockIdx.z) / co_groups, cob0 = (int(blockIdx.z) % co_groups) * NCOB;
+    const int nb = min(NCOB, c.co_blocks - cob0);
+    f32x16 acc[2][NCOB];
+#pragma unroll
+    for (int pb = 0; pb < 2; pb++)
+#pragma unroll
+        for (int b = 0; b < NCOB; b++) acc[pb][b] = f32x16{};
+    for (int kc = 0; kc < c.cin; kc += kChunk) {
+        const int cc = min(kChunk, c.cin - kc);  // 16 or 32
+        const int parts = cc >> 3;                // 16-byte parts per pixel or weight row
+        uint4 vin[kInLoads];
+        tile_input_load(c, n, x0, y0, kc, parts, vin);
+        tile_weights_stage<NCOB>(c, cob0, nb, kc, parts, s_w);
+        tile_input_store(parts, vin, s_in);
+        __syncthreads();
+        tile_mfma<NCOB>(s_in, s_w, nb, cc, acc);
+        __syncthreads();  // before the next chunk overwrites the tiles
+    }
+    tile_epilogue<NCOB>(c, n, x0, y0, cob0, nb, acc);
+}
+
+// k_conv3x3_tile for layers of at most 32 input and 32 output channels (one
+// chunk, one output block), with persistent workgroups: a workgroup stages the
+// weights once, then walks tiles t = blockIdx.x, + gridDim.x, ... and loads the
+// next tile's input into registers while the current tile's MFMAs and
+// epilogue run. Tile t is column t % tiles_x, row (t / tiles_x) % tiles_y,
+// image t / (tiles_x tiles_y).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+k_conv3x3_stream(UnetConv c, int tiles_x, int tiles_y, int n_tiles) {
+    __shared__ uint16_t s_in[kTileIn];
+    __shared__ uint16_t s_w[9 * 32 * kRow];
+    int t = int(blockIdx.x);  // the grid has at most n_tiles workgroups
+    const int cc = c.cin, parts = cc >> 3;
+    tile_weights_stage<1>(c, 0, 1, 0, parts, s_w);
+    auto origin = [&](int tile, int& n, int& x0, int& y0) {
+        x0 = (tile % tiles_x) * kTileW;
+        y0 = ((tile / tiles_x) % tiles_y) * kTileH;
+        n = tile / (tiles_x * tiles_y);
+    };
+    int n, x0, y0;
+    origin(t, n, x0, y0);
+    uint4 vin[kInLoads];
+    tile_input_load(c, n, x0, y0, 0, parts, vin);
+    for (;;) {
+        tile_input_store(parts, vin, s_in);  // every wave is past the last tile's MFMAs
+        __syncthreads();
+        const int next = t + int(gridDim.x);
+        int nn = 0, nx0 = 0, ny0 = 0;
+        if (next < n_tiles) {  // uniform
+            origin(next, nn, nx0, ny0);
+            tile_input_load(c, nn, nx0, ny0, 0, parts, vin);
+        }
+        f32x16 acc[2][1] = {{f32x16{}}, {f32x16{}}};
+        tile_mfma<1>(s_in, s_w, 1, cc, acc);
+        tile_epilogue<1>(c, n, x0, y0, 0, 1, acc);
+        if (next >= n_tiles) break;
+        __syncthreads();  // before the next tile overwrites s_in
+        t = next, n = nn, x0 = nx0, y0 = ny0;
     }
 }
 
@@ -507,7 +580,23 @@ void launch_conv3x3(const UnetConv& c, hipStream_t st) {
     // when that leaves under 2048 waves (deep levels: latency wants waves).
     // Many pixels: 2-D tiles with the input and weights in LDS, 1 or 2 output
     // blocks per workgroup, while that leaves 2048 waves.
-    const int tiles = ((c.w + kTileW - 1) / kTileW) * ((c.h + kTileH - 1) / kTileH) * c.n;
+    const int tiles_x = (c.w + kTileW - 1) / kTileW, tiles_y = (c.h + kTileH - 1) / kTileH;
+    const int tiles = tiles_x * tiles_y * c.n;
+    const char* stream_env = std::getenv("VVR_UNET_STREAM");  // "0": per-tile workgroups only
+    if (k.co_blocks == 1 && c.cin <= kChunk && tiles * 4 >= 2048 && !(stream_env && stream_env[0] == '0')) {
+        // One chunk and one output block: persistent workgroups, three per CU
+        // (their LDS), each walking tiles.
+        static const int cus = [] {
+            int dev = 0, n = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+                n = 256;
+            return n;
+        }();
+        const int grid = std::min(tiles, 3 * cus);
+        hipLaunchKernelGGL(k_conv3x3_stream, dim3(grid), dim3(256), 0, st, k, tiles_x, tiles_y, tiles);
+        return;
+    }
     for (int nc : {2, 1}) {
         const int groups = (k.co_blocks + nc - 1) / nc;
         if (nc > k.co_blocks || tiles * 4 * groups < 2048) continue;
