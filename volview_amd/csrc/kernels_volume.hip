@@ -6,6 +6,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
+
+#include <cmath>
+
 #include "launch.hpp"
 
 namespace vvr {
@@ -309,28 +313,36 @@ __global__ void __launch_bounds__(kBlock) k_mask_binary(const uint16_t* __restri
     out[o] = unorm16_bits(m0) | (unorm16_bits(m1) << 16);
 }
 
-// mask.comp:152-170: binomial weights C(2r, r + k) / 4^r along one axis.
-__global__ void k_mask_blur(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, int axis, int r, BrickGrid g) {
-    const uint32_t o = blockIdx.x * kBlock + threadIdx.x;
-    if (o >= g.padded()) return;
-    int x, y, z;
-    brick_coords(g, o, x, y, z);
+// mask.comp:152-170: binomial weights C(2r, r + k) / 4^r along one axis. The
+// weights c_j (c_0 = 1, c_j+1 = c_j (2r - j) / (j + 1), each step rounded to
+// f32) and 4^r come from the host, computed with the same f32 operations. A
+// workgroup covers 4 micro-bricks along x, one per wave, so a lane's voxel
+// follows from its indices without divisions.
+constexpr int kMaxBlurTaps = 17;  // radius <= 8
+struct BlurTaps {
+    float c[kMaxBlurTaps];
+    float e;
+};
+__global__ void __launch_bounds__(kBlock) k_mask_blur(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                                      int axis, int r, BrickGrid g, BlurTaps taps) {
+    const int mbx = int(blockIdx.x) * (kBlock / 64) + int(threadIdx.x >> 6), w = int(threadIdx.x & 63u);
+    if (mbx >= g.bx) return;
+    const uint32_t o = uint32_t(mbx) * 64u + uint32_t(w) + blockIdx.y * g.ystride + blockIdx.z * g.zstride;
+    const int x = 4 * mbx + (w & 3), y = 4 * int(blockIdx.y) + ((w >> 2) & 3), z = 4 * int(blockIdx.z) + (w >> 4);
     if (x >= g.nx || y >= g.ny || z >= g.nz) {
         dst[o] = 0u;
         return;
     }
-    float s0 = 0.0f, s1 = 0.0f, c = 1.0f;
+    float s0 = 0.0f, s1 = 0.0f;
+    const int n = axis == 0 ? g.nx : (axis == 1 ? g.ny : g.nz);
     for (int j = 0; j <= 2 * r; j++) {
         int q[3] = {x, y, z};
-        const int n = axis == 0 ? g.nx : (axis == 1 ? g.ny : g.nz);
         q[axis] = min(max(q[axis] + j - r, 0), n - 1);
         const uint32_t v = src[g.at(q[0], q[1], q[2])];
-        s0 += c * (float(v & 0xffffu) / 65535.0f);
-        s1 += c * (float(v >> 16) / 65535.0f);
-        c = c * float(2 * r - j) / float(j + 1);
+        s0 += taps.c[j] * (float(v & 0xffffu) / 65535.0f);
+        s1 += taps.c[j] * (float(v >> 16) / 65535.0f);
     }
-    const float e = exp2f(float(2 * r));
-    dst[o] = unorm16_bits(s0 / e) | (unorm16_bits(s1 / e) << 16);
+    dst[o] = unorm16_bits(s0 / taps.e) | (unorm16_bits(s1 / taps.e) << 16);
 }
 
 // majorant.comp:26-71. A workgroup covers 4 x 4 x 4 bricks. Every brick's window
@@ -500,7 +512,16 @@ void launch_mask_binary(const uint16_t* density, const uint8_t* labels, SoftGrid
 }
 
 void launch_mask_blur(const uint32_t* src, uint32_t* dst, int axis, int radius, const BrickGrid& g, hipStream_t st) {
-    hipLaunchKernelGGL(k_mask_blur, dim3(blocks_for(g.padded())), dim3(kBlock), 0, st, src, dst, axis, radius, g);
+    if (radius < 0 || 2 * radius + 1 > kMaxBlurTaps) throw std::invalid_argument("mask blur radius out of range");
+    BlurTaps taps{};
+    float c = 1.0f;
+    for (int j = 0; j <= 2 * radius; j++) {
+        taps.c[j] = c;
+        c = c * float(2 * radius - j) / float(j + 1);
+    }
+    taps.e = std::ldexp(1.0f, 2 * radius);
+    const dim3 grid(unsigned(g.bx + kBlock / 64 - 1) / (kBlock / 64), unsigned(g.by), unsigned(g.bz));
+    hipLaunchKernelGGL(k_mask_blur, grid, dim3(kBlock), 0, st, src, dst, axis, radius, g, taps);
 }
 
 void launch_majorant(const uint16_t* density, const uint32_t* mask, float* out, const BrickGrid& g, int mode,
