@@ -56,7 +56,7 @@ def render_once(r, params, w=8, h=8):
     r.synchronize()
 
 
-@pytest.mark.parametrize("dims", [(256, 256, 256), (77, 50, 43), (5, 3, 1), (9, 1, 17), (1, 1, 1)])
+@pytest.mark.parametrize("dims", [(512, 256, 256), (256, 256, 256), (77, 50, 43), (5, 3, 1), (9, 1, 17), (1, 1, 1)])
 def test_majorant_mode_off_bit_exact(ctx, oracle, dims):
     hu = phantom.ct(dims, 1)
     tf = TransferFunction.preset(0)
@@ -69,6 +69,19 @@ def test_majorant_mode_off_bit_exact(ctx, oracle, dims):
     assert np.array_equal(got, want)
     if min(dims) > 8:
         assert (got > 0).mean() < 0.9 and (got == 0).any()  # bricks are skipped
+
+
+def test_majorant_label_mode_large_tiles(ctx, oracle):
+    """Label mode on a volume of many majorant workgroups: the majorant over the
+    GPU's own mask equals the oracle's bit for bit."""
+    hu, labels, sg, tf, r, p = organ_scene(ctx, (512, 256, 256), 3, _abi.LABELS_ISOLATE)
+    render_once(r, p)
+    u = oracle_uniforms(oracle, r, hu, p, tf, has_labels=True)
+    sc = oracle.Scene(oracle.hu_to_unorm16(hu), tf=tf, labels=labels)
+    sc.set("mask", r.read_volume(_abi.VOL_MASK))
+    got = r.read_majorant()
+    assert np.array_equal(got, sc.majorant(u))
+    assert (got == 0).any() and (got > 0).any()
 
 
 def organ_scene(ctx, dims, seed, mode, soft=True, **label_kw):
