@@ -387,8 +387,12 @@ int vvr_renderer_autotune(vvr_renderer* r, const vvr_frame_params* params, uint3
  * items of one shared wavefront engine that admits new requests while others
  * are in flight (continuous batching) up to an HBM budget. Each request's result
  * is the same as rendering it alone on its renderer with samples_per_frame =
- * pt_increment, frame after frame. A renderer attached to a running request
- * must not be recorded on, resized, relabelled or destroyed meanwhile. */
+ * pt_increment, frame after frame. A request holds its renderer from
+ * submission to its final frame; meanwhile record, resize, set_labels,
+ * set_transfer_function, set_environment_map and set_denoiser on that renderer
+ * return VVR_ESTATE, and destroy defers the deletion to the last request.
+ * Work recorded on the renderer before the submission completes before the
+ * server touches the renderer's volumes. */
 typedef struct vvr_server vvr_server;
 
 typedef struct vvr_server_config {
@@ -433,8 +437,11 @@ int vvr_server_submit(vvr_server* s, const vvr_request* request, uint64_t* id);
  * for the first. *n is the number returned. */
 int vvr_server_poll(vvr_server* s, vvr_completion* out, uint32_t cap, uint32_t* n, uint32_t wait_ms);
 /* The request's latest delivered frame as float RGBA (linear radiance +
- * transmittance or background), width * height * 4 floats. */
-int vvr_server_read_hdr(vvr_server* s, uint64_t id, float* rgba);
+ * transmittance or background): n_floats must equal width * height * 4 of the
+ * request. Frames land in one of two pinned buffers in turn, so a read never
+ * sees a frame that is still being copied, and a read racing the next frame's
+ * delivery returns either frame whole. */
+int vvr_server_read_hdr(vvr_server* s, uint64_t id, float* rgba, size_t n_floats);
 /* Stops a request: one not admitted yet ends with status VVR_ECANCELED. A
  * path-traced one in flight stops handing out paths at once, abandons the
  * increment in flight and ends with a final completion whose frame is the last

@@ -7,6 +7,8 @@
 #pragma once
 #include <cmath>
 #include <cstdint>
+#include <cstring>
+#include <type_traits>
 
 namespace oracle {
 
@@ -35,6 +37,75 @@ template <class R> inline V3<R> operator-(V3<R> a) { return {-a.x, -a.y, -a.z}; 
 template <class R> inline R gmin(R a, R b) { return b < a ? b : a; }
 template <class R> inline R gmax(R a, R b) { return a < b ? b : a; }
 template <class R> inline R clampr(R x, R lo, R hi) { return gmin(gmax(x, lo), hi); }
+// GLSL's transcendental built-ins on the render path. GLSL leaves their precision
+// to the implementation (log 3 ulp, exp 3 + 2|x| ulp). The FP32 oracle fixes one
+// model for each, which the kernels repeat operation for operation:
+//  - log and exp: float algorithms. Cody and Waite's reduction to a mantissa m in
+//    [sqrt(1/2), sqrt(2)), log(m) = 2 atanh(s) with s = (m - 1) / (m + 1) by its
+//    series to s^9 (<= 2 ulp); exp by reduction to r = x - k ln 2 and the Taylor
+//    polynomial of e^r to r^7 (<= 1 ulp), scaled by two exact powers of two;
+//  - pow, sin, cos, acos, atan2: evaluated in double and rounded once (the
+//    correctly rounded value, barring ties closer than a double ulp).
+// The FP64 oracle uses the double functions throughout.
+inline float f32_from_bits(uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+inline uint32_t f32_bits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+constexpr float kLn2Hi = 0.693145751953125f;  // ln 2 with 11 trailing zero bits
+constexpr float kLn2Lo = 1.42860677e-06f;     // ln 2 - kLn2Hi
+
+inline float vlog_f32(float x) {
+    if (!(x > 0.0f)) return x == 0.0f ? -INFINITY : NAN;
+    if (x == INFINITY) return x;
+    int e = 0;
+    if (x < 1.17549435e-38f) x *= 16777216.0f, e = -24;
+    const uint32_t b = f32_bits(x);
+    e += int(b >> 23) - 127;
+    float m = f32_from_bits((b & 0x7fffffu) | 0x3f800000u);
+    if (m > 1.41421356f) m = m * 0.5f, e += 1;
+    const float f = m - 1.0f;
+    const float s = f / (2.0f + f);
+    const float z = s * s;
+    const float poly = z * (0.666666667f + z * (0.4f + z * (0.285714286f + z * 0.222222222f)));
+    const float lm = (s + s) + s * poly;
+    const float fe = float(e);
+    return fe * kLn2Hi + (fe * kLn2Lo + lm);
+}
+
+inline float vexp_f32(float x) {
+    if (x != x) return x;
+    if (x > 88.7228394f) return INFINITY;
+    if (x < -103.972084f) return 0.0f;
+    const float kf = std::rint(x * 1.44269504f);
+    const float r = (x - kf * kLn2Hi) - kf * kLn2Lo;
+    const float p =
+        1.0f + r * (1.0f + r * (0.5f + r * (0.166666667f + r * (0.0416666667f + r * (0.00833333333f +
+                                                                                      r * (0.00138888889f +
+                                                                                           r * 0.000198412698f))))));
+    const int k = int(kf), k1 = k / 2, k2 = k - k1;
+    return p * f32_from_bits(uint32_t(k1 + 127) << 23) * f32_from_bits(uint32_t(k2 + 127) << 23);
+}
+
+template <class R> inline R vlog(R x) {
+    if constexpr (std::is_same_v<R, float>) return vlog_f32(x);
+    else return std::log(x);
+}
+template <class R> inline R vexp(R x) {
+    if constexpr (std::is_same_v<R, float>) return vexp_f32(x);
+    else return std::exp(x);
+}
+template <class R> inline R cr_pow(R x, R y) { return R(std::pow(double(x), double(y))); }
+template <class R> inline R cr_sin(R x) { return R(std::sin(double(x))); }
+template <class R> inline R cr_cos(R x) { return R(std::cos(double(x))); }
+template <class R> inline R cr_acos(R x) { return R(std::acos(double(x))); }
+template <class R> inline R cr_atan2(R y, R x) { return R(std::atan2(double(y), double(x))); }
+
 template <class R> inline R mixr(R a, R b, R t) { return a * (R(1) - t) + b * t; }
 template <class R> inline R fract(R x) { return x - std::floor(x); }
 template <class R> inline R smoothstep(R e0, R e1, R x) {

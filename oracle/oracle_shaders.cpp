@@ -445,7 +445,7 @@ template <class R> struct Ctx {
         R k = a * R(0.5);
         R g = (nv / (nv * (R(1) - k) + k)) * (nl / (nl * (R(1) - k) + k));
         V3<R> f0 = vmix(V3<R>(R(0.028)), albedo, metalness);
-        V3<R> f = f0 + (R(1) - f0) * std::pow(R(1) - vh, R(5));
+        V3<R> f = f0 + (R(1) - f0) * cr_pow(R(1) - vh, R(5));
         V3<R> specular = d * g * f / (R(4) * nv * nl);
         diffuse = (R(1) - f) * (R(1) - metalness) * albedo / PI();
         return specular;
@@ -462,7 +462,7 @@ template <class R> struct Ctx {
         R nv = gmax(dot(n, v), R(1e-4));
         R nl = gmax(dot(n, l), R(1e-4));
         R a = coat_roughness(s_) * coat_roughness(s_);
-        R f = s_.coat * (R(0.02) + R(0.98) * std::pow(R(1) - gmax(dot(v, h), R(0)), R(5)));
+        R f = s_.coat * (R(0.02) + R(0.98) * cr_pow(R(1) - gmax(dot(v, h), R(0)), R(5)));
         R coat = ggx_distribution(gmax(dot(n, h), R(0)), a * a) * smith(nv, nl, a) * f / (R(4) * nv * nl);
         diffuse = diffuse * (R(1) - f);
         return specular * (R(1) - f) + V3<R>(coat);
@@ -513,7 +513,7 @@ template <class R> struct Ctx {
             R g_len = length(g);
             if (!skin || n_len <= R(1e-3)) n = g_len > R(1e-5) ? (-g) / g_len : n;
             surfaceness = clampr(g_len * R(4), R(0), R(1));
-            R edge_on = std::pow(R(1) - std::fabs(dot(n, v)), R(3));
+            R edge_on = cr_pow(R(1) - std::fabs(dot(n, v)), R(3));
             return surfaceness * mixr(R(u->labels[3]), R(1), edge_on);
         }
         return R(1);
@@ -711,7 +711,7 @@ template <class R> struct Ctx {
                 prev_density = density;
                 s_ = labelled(s_, idx);
                 if (s_.extinction <= R(0)) continue;
-                R alpha = R(1) - std::exp(-s_.extinction * step_mm);
+                R alpha = R(1) - vexp(-s_.extinction * step_mm);
                 if (segment && alpha > R(0.2) && transmittance > R(0.5) && s_.label == 0u &&
                     std::fabs(density - front_density) > R(1e-4)) {
                     idx = idx - d_idx * (step_mm * (R(1) - preint_half_depth(front_density, density)));
@@ -732,7 +732,7 @@ template <class R> struct Ctx {
                 R surfaceness;
                 R extinction_scale = surface_frame(idx, s_, v, cap_plane, cut, R(1.5) * step_mm, n, surfaceness);
                 apply_surface_texture(idx, s_);
-                if (s_.ghost) alpha = R(1) - std::exp(-s_.extinction * extinction_scale * step_mm);
+                if (s_.ghost) alpha = R(1) - vexp(-s_.extinction * extinction_scale * step_mm);
                 // dvr.comp:107-119: lighting volumes looked up just outside the surface,
                 // the diffused light a little below it.
                 M3<R> ti = to_index();
@@ -784,7 +784,7 @@ template <class R> struct Ctx {
         V3<R> mid = (here + prev) * R(0.5) + V3<R>(R(0.5));
         V3<R> mid_tex(mid.x / sd.x, mid.y / sd.y, mid.z / sd.z);
         Sample m = sample_point(tex_to_index(mid_tex));
-        R direct = light_linear(0, prev, 3) * std::exp(-m.extinction * step);
+        R direct = light_linear(0, prev, 3) * vexp(-m.extinction * step);
         R au[3] = {0, 0, 0}, av[3] = {0, 0, 0};
         au[u_axis] = R(u->subsurface[2]);
         av[v_axis] = R(u->subsurface[2]);
@@ -800,8 +800,8 @@ template <class R> struct Ctx {
         R mx = gmax(gmax(m.albedo.x, gmax(m.albedo.y, m.albedo.z)), R(1e-4));
         V3<R> chroma = m.albedo / mx;
         V3<R> ind_ext = (V3<R>(R(1.2)) - chroma) * (m.extinction * R(u->subsurface[1]));
-        V3<R> indirect(gathered.x * std::exp(-ind_ext.x * step), gathered.y * std::exp(-ind_ext.y * step),
-                       gathered.z * std::exp(-ind_ext.z * step));
+        V3<R> indirect(gathered.x * vexp(-ind_ext.x * step), gathered.y * vexp(-ind_ext.y * step),
+                       gathered.z * vexp(-ind_ext.z * step));
         out[0] = indirect.x, out[1] = indirect.y, out[2] = indirect.z, out[3] = direct;
     }
 
@@ -816,10 +816,10 @@ template <class R> struct Ctx {
             R z = R(1) - (R(i) + R(0.5)) * (R(2) / R(16));
             R r = std::sqrt(R(1) - z * z);
             R phi = R(i) * R(2.39996323f);
-            V3<R> d = ti.mul(V3<R>(r * std::cos(phi), r * std::sin(phi), z));
+            V3<R> d = ti.mul(V3<R>(r * cr_cos(phi), r * cr_sin(phi), z));
             R depth = 0;
             for (int j = 0; j < 8; j++) depth += sample_point(center + d * ((R(j) + R(0.5)) * dt)).extinction * dt;
-            visible += std::exp(-depth);
+            visible += vexp(-depth);
         }
         return gmin(R(2) * visible / R(16), R(1));
     }
@@ -839,8 +839,8 @@ template <class R> struct Ctx {
     V3<R> basis_u() const { return normalize(V3<R>(u->up[0], u->up[1], u->up[2])); }
     void env_uv(V3<R> d, R& uu, R& vv) const {
         V3<R> f = basis_f(), r = basis_r(), up = basis_u();
-        R phi = std::atan2(dot(d, r), dot(d, f));
-        R theta = std::acos(clampr(dot(d, up), R(-1), R(1)));
+        R phi = cr_atan2(dot(d, r), dot(d, f));
+        R theta = cr_acos(clampr(dot(d, up), R(-1), R(1)));
         R x = R(0.5) + (phi + R(u->env[0])) / (R(2) * PI());
         uu = x - std::floor(x);
         vv = theta / PI();
@@ -849,7 +849,7 @@ template <class R> struct Ctx {
         R phi = (uu - R(0.5)) * R(2) * PI() - R(u->env[0]);
         R theta = vv * PI();
         V3<R> f = basis_f(), r = basis_r(), up = basis_u();
-        return std::sin(theta) * (std::sin(phi) * r + std::cos(phi) * f) + std::cos(theta) * up;
+        return cr_sin(theta) * (cr_sin(phi) * r + cr_cos(phi) * f) + cr_cos(theta) * up;
     }
     // Linear clamp-to-edge lookup of an RGBA16F image (the renderer's sampler).
     V3<R> image_rgb(const uint16_t* img, int w, int h, R uu, R vv) const {
@@ -884,7 +884,7 @@ template <class R> struct Ctx {
         env_uv(d, uu, vv);
         int tx = std::min((int)(uu * R(w)), w - 1), ty = std::min((int)(vv * R(h)), h - 1);
         R p = table(tx, ty, 1);
-        return p * R(w * h) / (R(2) * PI() * PI() * gmax(std::sin(vv * PI()), R(1e-6)));
+        return p * R(w * h) / (R(2) * PI() * PI() * gmax(cr_sin(vv * PI()), R(1e-6)));
     }
     V3<R> sample_env(R& pdf) {
         int w = (int)s->env_w, h = (int)s->env_h;
@@ -907,7 +907,7 @@ template <class R> struct Ctx {
         R jv = rand();
         R uu = (R(lo) + ju) / R(w), vv = (R(y) + jv) / R(h);
         R p = table(lo, y, 1);
-        pdf = p * R(w * h) / (R(2) * PI() * PI() * gmax(std::sin(vv * PI()), R(1e-6)));
+        pdf = p * R(w * h) / (R(2) * PI() * PI() * gmax(cr_sin(vv * PI()), R(1e-6)));
         return env_direction(uu, vv);
     }
 
@@ -921,8 +921,8 @@ template <class R> struct Ctx {
             R key_side = dot(V3<R>(u->light_dir[0], u->light_dir[1], u->light_dir[2]), r) >= R(0) ? R(1) : R(-1);
             V3<R> fill = normalize(((-f) - (R(0.9) * key_side) * r) - R(0.1) * up);
             V3<R> rim = normalize((f + R(0.8) * up) - (R(0.4) * key_side) * r);
-            R fill_lobe = R(0.5) * std::pow(gmax(dot(d, fill), R(0)), R(6));
-            R rim_lobe = R(2.5) * std::pow(gmax(dot(d, rim), R(0)), R(24));
+            R fill_lobe = R(0.5) * cr_pow(gmax(dot(d, fill), R(0)), R(6));
+            R rim_lobe = R(2.5) * cr_pow(gmax(dot(d, rim), R(0)), R(24));
             V3<R> studio = V3<R>(R(0.02)) + fill_lobe * V3<R>(R(0.95), R(0.98), R(1.05)) +
                            rim_lobe * V3<R>(R(0.9), R(0.95), R(1.1));
             return amb * studio * (R(1) / R(0.106));
@@ -943,11 +943,11 @@ template <class R> struct Ctx {
         return m;
     }
     V3<R> sample_light(const M3<R>& frame) {  // pt.comp:96-102
-        R cos_max = std::cos(R(u->material[3]));
+        R cos_max = cr_cos(R(u->material[3]));
         R cos_t = mixr(cos_max, R(1), rand());
         R sin_t = std::sqrt(gmax(R(0), R(1) - cos_t * cos_t));
         R phi = R(2) * PI() * rand();
-        return frame.mul(V3<R>(sin_t * std::cos(phi), sin_t * std::sin(phi), cos_t));
+        return frame.mul(V3<R>(sin_t * cr_cos(phi), sin_t * cr_sin(phi), cos_t));
     }
     static V3<R> single_scattering_albedo(V3<R> a) {  // pt.comp:119-123
         a = vclamp(a, R(0), R(0.999));
@@ -988,7 +988,7 @@ template <class R> struct Ctx {
         }
         R sin_t = std::sqrt(gmax(R(0), R(1) - cos_t * cos_t));
         R phi = R(2) * PI() * rand();
-        return basis(dir).mul(V3<R>(sin_t * std::cos(phi), sin_t * std::sin(phi), cos_t));
+        return basis(dir).mul(V3<R>(sin_t * cr_cos(phi), sin_t * cr_sin(phi), cos_t));
     }
     static R mis(R a, R b) { return a * a / gmax(a * a + b * b, R(1e-20)); }
     static R coat_share(const Sample& s_) { return R(0.5) * s_.coat; }
@@ -1004,7 +1004,7 @@ template <class R> struct Ctx {
         R xi = rand();
         R cos_t = std::sqrt((R(1) - xi) / (R(1) + (a2 - R(1)) * xi));
         R sin_t = std::sqrt(gmax(R(0), R(1) - cos_t * cos_t));
-        V3<R> h = frame.mul(V3<R>(sin_t * std::cos(phi), sin_t * std::sin(phi), cos_t));
+        V3<R> h = frame.mul(V3<R>(sin_t * cr_cos(phi), sin_t * cr_sin(phi), cos_t));
         return reflect(-v, h);
     }
     static R bsdf_pdf(V3<R> n, V3<R> v, V3<R> l, const Sample& s_) {
@@ -1042,7 +1042,7 @@ template <class R> struct Ctx {
                 tt = exit;
                 continue;
             }
-            R next = tt - std::log(R(1) - rand()) / majorant;
+            R next = tt - vlog(R(1) - rand()) / majorant;
             if (next >= exit) {
                 tt = exit;
                 continue;
@@ -1080,7 +1080,7 @@ template <class R> struct Ctx {
                 tt = exit;
                 continue;
             }
-            R next = tt - std::log(R(1) - rand()) / majorant;
+            R next = tt - vlog(R(1) - rand()) / majorant;
             if (next >= exit) {
                 tt = exit;
                 continue;
@@ -1102,7 +1102,7 @@ template <class R> struct Ctx {
         V3<R> ldir(u->light_dir[0], u->light_dir[1], u->light_dir[2]);
         M3<R> light_frame = basis(ldir);
         V3<R> light(R(u->light_dir[3]));
-        R cos_light = std::cos(R(u->material[3]));
+        R cos_light = cr_cos(R(u->material[3]));
         bool area_light = u->material[3] > 1e-4f;
         R light_pdf = area_light ? R(1) / (R(2) * PI() * (R(1) - cos_light)) : R(0);
         M3<R> ti = to_index();
@@ -1126,7 +1126,7 @@ template <class R> struct Ctx {
                 R phi = R(2) * PI() * rand();
                 V3<R> rr = normalize(V3<R>(u->right[0], u->right[1], u->right[2]));
                 V3<R> uu = normalize(V3<R>(u->up[0], u->up[1], u->up[2]));
-                eye = eye + r * (std::cos(phi) * rr + std::sin(phi) * uu);
+                eye = eye + r * (cr_cos(phi) * rr + cr_sin(phi) * uu);
                 dir = normalize(focal - eye);
                 origin = world_to_index_point(eye);
             }
@@ -1201,7 +1201,7 @@ template <class R> struct Ctx {
                     } else {
                         R r = std::sqrt(rand());
                         R phi = R(2) * PI() * rand();
-                        l = frame.mul(V3<R>(r * std::cos(phi), r * std::sin(phi), std::sqrt(gmax(R(0), R(1) - r * r))));
+                        l = frame.mul(V3<R>(r * cr_cos(phi), r * cr_sin(phi), std::sqrt(gmax(R(0), R(1) - r * r))));
                     }
                     R nl = dot(n, l);
                     if (nl <= R(0)) break;

@@ -104,12 +104,12 @@ def pt_scene(ctx, labels_on, dims=(64, 64, 48), seed=12):
     return hu, tf, r, labels, n
 
 
-def oracle_pt_sum(oracle, r, hu, tf, n, labels, w, h, first, count, precision=32):
+def oracle_pt_sum(oracle, r, hu, tf, n, labels, w, h, first, count, precision=32, bounces=8):
     """The oracle's sums over sample indices first .. first + count - 1, each traced
     as a reference frame of one path."""
     want = [np.zeros((w * h, 4), np.float64) for _ in range(3)]
     for k in range(first, first + count):
-        q = workloads.pt_params(r, n, 1, bounces=4, g=0.8, camera_index=2)
+        q = workloads.pt_params(r, n, 1, bounces=bounces, g=0.8, camera_index=2)
         u, sc = oracle_for(oracle, r, hu, tf, q, w, h, has_labels=labels is not None, accumulated=k, labels=labels)
         for acc, part in zip(want, sc.pt(u, all_pixels(w, h), precision)):
             acc += part
@@ -121,18 +121,21 @@ def test_pt_traces_the_oracles_paths(ctx, oracle, labels_on):
     """Same random streams: nearly every pixel's sum agrees to 1e-3. A formula
     error would move every pixel; only float-rounding flips may move a few."""
     hu, tf, r, labels, n = pt_scene(ctx, labels_on)
-    w, h, spp = 32, 32, 3
+    w, h, spp = 32, 32, 4
     r.resize(w, h)
-    r.record(1, workloads.pt_params(r, n, spp, bounces=4, g=0.8, camera_index=2))
+    r.record(1, workloads.pt_params(r, n, spp, bounces=8, g=0.8, camera_index=2))
     acc, alb, nrm = r.read_accum()
     assert r.accumulated_samples() == spp
     want = oracle_pt_sum(oracle, r, hu, tf, n, labels, w, h, 0, spp)
-    for what, got, ref, limit in (("radiance", acc, want[0], 0.10), ("albedo guide", alb[..., :3], want[1][:, :3], 0.03),
-                                  ("normal guide", nrm, want[2], 0.03)):
-        err = np.abs(got.reshape(ref.shape) - ref)
+    for what, got, ref in (("radiance", acc, want[0]), ("albedo guide", alb[..., :3], want[1][:, :3]),
+                           ("normal guide", nrm, want[2])):
+        got = got.reshape(ref.shape).astype(np.float64)
+        err = np.abs(got - ref)
         off = np.any(err > 1e-3 * (1 + np.abs(ref)), axis=-1)
-        print(f"{what}: {off.mean():.4f} of pixels differ (limit {limit})")
-        assert off.mean() < limit, f"{what}: {off.sum()} of {off.size} pixels differ"
+        rel = np.abs(got.sum(0) - ref.sum(0)) / np.maximum(np.abs(ref).sum(0), 1e-9)
+        print(f"{what}: {off.mean():.4f} of pixels differ (limit 0.02); frame relative difference {rel.max():.2e}")
+        assert off.mean() < 0.02, f"{what}: {off.sum()} of {off.size} pixels differ"
+        assert rel[:3].max() < 0.01, f"{what}: the frame's sums differ by {rel}"
     hdr = r.read_hdr().reshape(-1, 4)
     assert np.allclose(hdr, (acc.reshape(-1, 4) / spp).astype(np.float16).astype(np.float32), rtol=2e-3, atol=1e-4)
 
@@ -148,7 +151,7 @@ def test_pt_converged_frames(ctx, oracle, labels_on):
     hu, tf, r, labels, n = pt_scene(ctx, labels_on)
     w, h, spp = 16, 16, 256
     r.resize(w, h)
-    r.record(1, workloads.pt_params(r, n, spp, bounces=4, g=0.8, camera_index=2))
+    r.record(1, workloads.pt_params(r, n, spp, bounces=8, g=0.8, camera_index=2))
     acc, alb, _ = r.read_accum()
     # The FP64 oracle on independent sample indices: two independent estimates.
     f64 = oracle_pt_sum(oracle, r, hu, tf, n, labels, w, h, spp, spp, precision=64)
@@ -170,6 +173,38 @@ def test_pt_converged_frames(ctx, oracle, labels_on):
     # The whole frame's mean luminance too.
     zt = (mg.sum() - mo.sum()) / np.sqrt(vg.sum() + vo.sum())
     assert abs(zt) < 4, zt
+
+
+@pytest.mark.parametrize("labels_on", [True, False])
+def test_pt_converged_relative_rmse(ctx, oracle, labels_on):
+    """Converged frames: 65,536 paths per pixel on the GPU against a 16,384-path
+    frame of the FP64 oracle on an independent stream (the reference's seeding of
+    one frame of many paths at another accumulated count), 8 bounces, g = 0.8.
+    The per-pixel luminance relative RMSE may exceed 1 % only by the noise the
+    two estimates carry, predicted from their luminance second moments:
+    mean((a - b)^2) <= (1 %)^2 mean(b^2) + 2 mean(var a + var b). Between two
+    independent oracle frames the left side measured 0.9-1.35 x the noise term."""
+    hu, tf, r, labels, n = pt_scene(ctx, labels_on)
+    w, h, spp, ref_spp = 16, 16, 65536, 16384
+    r.resize(w, h)
+    r.record(1, workloads.pt_params(r, n, spp, bounces=8, g=0.8, camera_index=2))
+    acc, alb, _ = r.read_accum()
+    q = workloads.pt_params(r, n, ref_spp, bounces=8, g=0.8, camera_index=2)
+    u, sc = oracle_for(oracle, r, hu, tf, q, w, h, has_labels=labels is not None, accumulated=1 << 20, labels=labels)
+    oa, ob, _ = sc.pt(u, all_pixels(w, h), 64)
+    lum_w = np.array([0.2126, 0.7152, 0.0722])
+
+    def stats(rad, lum2, k):
+        mean = rad[:, :3].astype(np.float64) @ lum_w / k
+        var = np.maximum(lum2.astype(np.float64) / k - mean ** 2, 0.0) / k
+        return mean, var
+
+    a, va = stats(acc.reshape(-1, 4), alb.reshape(-1, 4)[:, 3], spp)
+    b, vb = stats(oa.reshape(-1, 4), ob.reshape(-1, 4)[:, 3], ref_spp)
+    mse, ref2, noise = np.mean((a - b) ** 2), np.mean(b ** 2), np.mean(va + vb)
+    rel, floor = np.sqrt(mse / ref2), np.sqrt(noise / ref2)
+    print(f"relative RMSE {rel:.4f}, predicted noise {floor:.4f}, excess {np.sqrt(max(mse - noise, 0) / ref2):.4f}")
+    assert mse <= 1e-4 * ref2 + 2.0 * noise, (rel, floor)
 
 
 def test_pt_accumulates_across_frames(ctx):
@@ -210,3 +245,19 @@ def test_record_before_resize_and_unsupported_denoiser_fail(ctx):
     assert e.value.code == _abi.VVR_EINVAL
     with pytest.raises(_abi.VvrError):
         r.set_labels(np.zeros(10, np.uint8))
+
+
+def test_frame_over_the_item_limit_is_refused(ctx):
+    """Work items are numbered in 32 bits: a frame of 2^31 items or more (8192^2
+    pixels at 64 paths) fails with VVR_ELIMIT before anything is queued, and
+    the renderer keeps working."""
+    hu = phantom.ct((32, 32, 32), 1)
+    r = Renderer(ctx, hu, TransferFunction.preset(0))
+    r.resize(8192, 8192)
+    with pytest.raises(_abi.VvrError) as e:
+        r.record(1, workloads.pt_params(r, 0, 64, bounces=2))
+    assert e.value.code == _abi.VVR_ELIMIT
+    r.resize(16, 16)
+    r.record(2, workloads.pt_params(r, 0, 2, bounces=2))
+    acc, _, _ = r.read_accum()
+    assert np.isfinite(acc).all() and r.accumulated_samples() == 2

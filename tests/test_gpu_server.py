@@ -200,3 +200,33 @@ def test_renderer_destroyed_while_served(ctx):
     srv.release(running)
     srv.release(queued)
     srv.close()
+
+
+def test_renderer_calls_refused_while_held(ctx):
+    """A request holds its renderer from submission to its final frame: record,
+    resize and the set_* calls on it fail with VVR_ESTATE meanwhile (they would
+    rewrite volumes, tables or images the server's kernels read), and work again
+    once the request has ended."""
+    a, _, _ = volumes(ctx)
+    a.resize(16, 16)
+    srv = Server(ctx, pt_increment=4, max_requests=1)
+    rid = srv.submit(a, workloads.pt_params(a, 0, 64, bounces=4), 32, 24)
+    labels = np.zeros((36, 40, 48), np.uint8)
+    refused = {
+        "record": lambda: a.record(1, workloads.dvr_params(a, 0)),
+        "resize": lambda: a.resize(8, 8),
+        "set_labels": lambda: a.set_labels(labels),
+        "set_transfer_function": lambda: a.set_transfer_function(TransferFunction.preset(1)),
+        "set_environment_map": lambda: a.set_environment_map(np.ones((4, 8, 3), np.float32)),
+        "set_denoiser": lambda: a.set_denoiser(None),
+    }
+    for name, call in refused.items():
+        with pytest.raises(_abi.VvrError) as e:
+            call()
+        assert e.value.code == _abi.VVR_ESTATE, name
+    got = drain(srv, [rid])
+    assert got[rid][-1].final_frame and got[rid][-1].spp == 64
+    srv.release(rid)
+    a.record(1, workloads.dvr_params(a, 0))  # the hold is gone
+    a.set_transfer_function(TransferFunction.preset(1))
+    srv.close()

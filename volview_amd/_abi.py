@@ -195,7 +195,7 @@ SIGNATURES = [
     ("vvr_server_create", c_int, [c_void_p, POINTER(ServerConfig), POINTER(c_void_p)]),
     ("vvr_server_submit", c_int, [c_void_p, POINTER(Request), POINTER(c_uint64)]),
     ("vvr_server_poll", c_int, [c_void_p, POINTER(Completion), c_uint32, POINTER(c_uint32), c_uint32]),
-    ("vvr_server_read_hdr", c_int, [c_void_p, c_uint64, POINTER(c_float)]),
+    ("vvr_server_read_hdr", c_int, [c_void_p, c_uint64, POINTER(c_float), c_size_t]),
     ("vvr_server_cancel", c_int, [c_void_p, c_uint64]),
     ("vvr_server_release", c_int, [c_void_p, c_uint64]),
     ("vvr_server_get_stats", c_int, [c_void_p, POINTER(ServerStats)]),

@@ -2,7 +2,8 @@
 
     python -m volview_amd.build          # or: from volview_amd.build import build; build()
 
-Every HIP source is compiled for gfx950 with packed-FP32 instructions disabled;
+Every HIP source is compiled for gfx950 with packed-FP32 instructions disabled and
+without floating-point contraction;
 host sources are C++17, compiled by the host compiler. Objects go to volview_amd/_build/, the library to
 volview_amd/libvvr.so.
 """
@@ -32,6 +33,10 @@ DEVICE = [
     f"--offload-arch={ARCH}",
     # Packed FP32 (v_pk_*_f32) stays off: some of its forms have given wrong results.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # No contraction of a * b + c into one FMA: every operation is rounded on its
+    # own, in source order, as in the FP32 oracle, so the path tracer's random
+    # walks take the oracle's decisions (explicit fmaf calls stay fused).
+    "-ffp-contract=off",
 ]
 
 

@@ -37,8 +37,8 @@ __device__ __forceinline__ void env_uv(const vvr_uniforms* u, V3 d, float& uu, f
     const V3 f = normalize(V3(u->forward[0], u->forward[1], u->forward[2]));
     const V3 r = normalize(V3(u->right[0], u->right[1], u->right[2]));
     const V3 up = normalize(V3(u->up[0], u->up[1], u->up[2]));
-    const float phi = atan2f(dot(d, r), dot(d, f));
-    const float theta = acosf(clampf(dot(d, up), -1.0f, 1.0f));
+    const float phi = cr_atan2(dot(d, r), dot(d, f));
+    const float theta = cr_acos(clampf(dot(d, up), -1.0f, 1.0f));
     const float x = 0.5f + (phi + u->env[0]) / (2.0f * kPi);
     uu = x - floorf(x);
     vv = theta / kPi;
@@ -50,7 +50,7 @@ __device__ __forceinline__ V3 env_direction(const vvr_uniforms* u, float uu, flo
     const V3 f = normalize(V3(u->forward[0], u->forward[1], u->forward[2]));
     const V3 r = normalize(V3(u->right[0], u->right[1], u->right[2]));
     const V3 up = normalize(V3(u->up[0], u->up[1], u->up[2]));
-    return sinf(theta) * (sinf(phi) * r + cosf(phi) * f) + cosf(theta) * up;
+    return cr_sin(theta) * (cr_sin(phi) * r + cr_cos(phi) * f) + cr_cos(theta) * up;
 }
 
 __device__ __forceinline__ V3 env_texel(const uint2* img, int w, int x, int y) {
@@ -89,7 +89,7 @@ __device__ __forceinline__ float env_pdf(const DevScene& s, V3 d) {
     env_uv(s.u, d, uu, vv);
     const int tx = min(int(uu * float(s.env_w)), s.env_w - 1), ty = min(int(vv * float(s.env_h)), s.env_h - 1);
     const float p = s.env_table[ty * (s.env_w + 1) + tx].y;
-    return p * float(s.env_w * s.env_h) / (2.0f * kPi * kPi * gmax(sinf(vv * kPi), 1e-6f));
+    return p * float(s.env_w * s.env_h) / (2.0f * kPi * kPi * gmax(cr_sin(vv * kPi), 1e-6f));
 }
 
 // A direction drawn proportionally to the map's luminance: a row by binary
@@ -115,7 +115,7 @@ __device__ inline V3 sample_env(const DevScene& s, Rng& rng, float& pdf) {
     const float jv = rng.rand();
     const float uu = (float(lo) + ju) / float(w), vv = (float(y) + jv) / float(h);
     const float p = s.env_table[y * (w + 1) + lo].y;
-    pdf = p * float(w * h) / (2.0f * kPi * kPi * gmax(sinf(vv * kPi), 1e-6f));
+    pdf = p * float(w * h) / (2.0f * kPi * kPi * gmax(cr_sin(vv * kPi), 1e-6f));
     return env_direction(s.u, uu, vv);
 }
 
@@ -131,8 +131,8 @@ __device__ inline V3 environment(const DevScene& s, V3 d) {
         const float key_side = dot(V3(u->light_dir[0], u->light_dir[1], u->light_dir[2]), r) >= 0.0f ? 1.0f : -1.0f;
         const V3 fill = normalize(((-f) - (0.9f * key_side) * r) - 0.1f * up);
         const V3 rim = normalize((f + 0.8f * up) - (0.4f * key_side) * r);
-        const float fill_lobe = 0.5f * powf(gmax(dot(d, fill), 0.0f), 6.0f);
-        const float rim_lobe = 2.5f * powf(gmax(dot(d, rim), 0.0f), 24.0f);
+        const float fill_lobe = 0.5f * cr_pow(gmax(dot(d, fill), 0.0f), 6.0f);
+        const float rim_lobe = 2.5f * cr_pow(gmax(dot(d, rim), 0.0f), 24.0f);
         const V3 studio = V3(0.02f) + fill_lobe * V3(0.95f, 0.98f, 1.05f) + rim_lobe * V3(0.9f, 0.95f, 1.1f);
         return amb * studio * (1.0f / 0.106f);
     }
@@ -152,11 +152,11 @@ __device__ __forceinline__ M3 basis(V3 n) {  // pt.comp:85-92
 }
 
 __device__ __forceinline__ V3 sample_light(const vvr_uniforms* u, const M3& frame, Rng& rng) {  // pt.comp:96-102
-    const float cos_max = cosf(u->material[3]);
+    const float cos_max = cr_cos(u->material[3]);
     const float cos_t = mixf(cos_max, 1.0f, rng.rand());
     const float sin_t = sqrtf(gmax(0.0f, 1.0f - cos_t * cos_t));
     const float phi = 2.0f * kPi * rng.rand();
-    return frame.mul(V3(sin_t * cosf(phi), sin_t * sinf(phi), cos_t));
+    return frame.mul(V3(sin_t * cr_cos(phi), sin_t * cr_sin(phi), cos_t));
 }
 
 __device__ __forceinline__ V3 single_scattering_albedo(V3 a) {  // pt.comp:119-123
@@ -204,7 +204,7 @@ __device__ __forceinline__ V3 sample_hg(const vvr_uniforms* u, V3 dir, Rng& rng)
     }
     const float sin_t = sqrtf(gmax(0.0f, 1.0f - cos_t * cos_t));
     const float phi = 2.0f * kPi * rng.rand();
-    return basis(dir).mul(V3(sin_t * cosf(phi), sin_t * sinf(phi), cos_t));
+    return basis(dir).mul(V3(sin_t * cr_cos(phi), sin_t * cr_sin(phi), cos_t));
 }
 
 __device__ __forceinline__ float mis(float a, float b) { return a * a / gmax(a * a + b * b, 1e-20f); }
@@ -223,7 +223,7 @@ __device__ __forceinline__ V3 sample_ggx(const M3& frame, V3 v, float a, Rng& rn
     const float xi = rng.rand();
     const float cos_t = sqrtf((1.0f - xi) / (1.0f + (a2 - 1.0f) * xi));
     const float sin_t = sqrtf(gmax(0.0f, 1.0f - cos_t * cos_t));
-    const V3 h = frame.mul(V3(sin_t * cosf(phi), sin_t * sinf(phi), cos_t));
+    const V3 h = frame.mul(V3(sin_t * cr_cos(phi), sin_t * cr_sin(phi), cos_t));
     return reflect(-v, h);
 }
 
@@ -288,7 +288,7 @@ __device__ __forceinline__ bool flight_step(const DevScene& sc, Flight& f, Rng& 
         f.tt = exit;
         return false;
     }
-    const float next = f.tt - logf(1.0f - rng.rand()) / majorant;
+    const float next = f.tt - vlog(1.0f - rng.rand()) / majorant;
     if (next >= exit) {
         f.tt = exit;
         return false;
@@ -384,7 +384,7 @@ __device__ __forceinline__ FrameConsts frame_consts(const vvr_uniforms* u) {
     c.ldir = V3(u->light_dir[0], u->light_dir[1], u->light_dir[2]);
     c.light_frame = basis(c.ldir);
     c.light = V3(u->light_dir[3]);
-    c.cos_light = cosf(u->material[3]);
+    c.cos_light = cr_cos(u->material[3]);
     c.area_light = u->material[3] > 1e-4f;
     c.light_pdf = c.area_light ? 1.0f / (2.0f * kPi * (1.0f - c.cos_light)) : 0.0f;
     c.env_map = u->lighting[3] == 2u;
@@ -424,7 +424,7 @@ __device__ inline void path_begin(const DevScene& sc, const FrameConsts& fc, uin
         const float phi = 2.0f * kPi * st.rng.rand();
         const V3 rr = normalize(V3(u->right[0], u->right[1], u->right[2]));
         const V3 uu = normalize(V3(u->up[0], u->up[1], u->up[2]));
-        eye = eye + r * (cosf(phi) * rr + sinf(phi) * uu);
+        eye = eye + r * (cr_cos(phi) * rr + cr_sin(phi) * uu);
         dir = normalize(focal - eye);
         origin = world_to_index_point(u, eye);
     }
@@ -568,7 +568,7 @@ __device__ __forceinline__ bool stage_scatter(const DevScene& sc, const FrameCon
         } else {
             const float r = sqrtf(st.rng.rand());
             const float phi = 2.0f * kPi * st.rng.rand();
-            l = frame.mul(V3(r * cosf(phi), r * sinf(phi), sqrtf(gmax(0.0f, 1.0f - r * r))));
+            l = frame.mul(V3(r * cr_cos(phi), r * cr_sin(phi), sqrtf(gmax(0.0f, 1.0f - r * r))));
         }
         const float nl = dot(n, l);
         if (nl <= 0.0f) return false;
@@ -605,46 +605,10 @@ __device__ __forceinline__ bool stage_scatter(const DevScene& sc, const FrameCon
     return true;
 }
 
-// One iteration of the bounce loop (pt.comp:427-545). Returns false once the
-// path has ended (then the firefly clamp of path_end applies).
-__device__ __forceinline__ bool path_bounce(const DevScene& sc, const FrameConsts& fc, PathState& st) {
-    if (!path_continues(fc, st)) return false;
-    EventState ev;
-    if (!stage_extend(sc, fc, st, ev)) return false;
-    stage_shade(sc, fc, st, ev);
-    stage_shadow(sc, st, ev);
-    stage_shadow_map(sc, st, ev);
-    return stage_scatter(sc, fc, st, ev);
-}
-
 // Firefly clamp (pt.comp:546-548): the radiance the path adds.
 __device__ __forceinline__ V3 path_end(const PathState& st) {
     const float peak = gmax(st.radiance.x, gmax(st.radiance.y, st.radiance.z));
     return peak > 20.0f ? st.radiance * (20.0f / peak) : st.radiance;
 }
-
-// What the paths of one pixel add to the accumulation and guide images (pt.comp:399-552).
-struct PixelSums {
-    V3 radiance;
-    float background;
-    V3 albedo;
-    float lum2;  // sum of squared luminances (albedo guide alpha)
-    V3 normal;
-    float depth;
-    __device__ __forceinline__ void clear() {
-        radiance = albedo = normal = V3(0.0f);
-        background = lum2 = depth = 0.0f;
-    }
-    __device__ __forceinline__ void add(const PathState& st) {
-        const V3 r = path_end(st);
-        radiance = radiance + r;
-        background += st.background;
-        albedo = albedo + st.albedo;
-        normal = normal + st.normal;
-        depth += st.depth;
-        const float lum = dot(r, V3(0.2126f, 0.7152f, 0.0722f));
-        lum2 += lum * lum;
-    }
-};
 
 }  // namespace vvr

@@ -22,6 +22,22 @@ __device__ __forceinline__ int fidx(float f) { return int(fminf(fmaxf(f, -1.0e8f
 
 constexpr float kInv65535 = 1.0f / 65535.0f;
 
+// A unorm16 texel's value c / 65535, correctly rounded as a division would give
+// it, without the division: the product with the rounded reciprocal is off by
+// one ulp for 512 of the 65,536 codes, and one Newton correction of its
+// residual (exact in an FMA) fixes every one of them.
+__device__ __forceinline__ float unorm16_value(float c) {
+    const float p = c * kInv65535;
+    return fmaf(fmaf(-p, 65535.0f, c), kInv65535, p);
+}
+// A snorm8 texel's value max(c / 127, -1), correctly rounded likewise (16 of the
+// 256 codes need the correction).
+__device__ __forceinline__ float snorm8_value(float c) {
+    constexpr float kInv127 = 1.0f / 127.0f;
+    const float p = c * kInv127;
+    return gmax(fmaf(fmaf(-p, 127.0f, c), kInv127, p), -1.0f);
+}
+
 // ---- linear fetches (common.glsl:76-78 and the mask / normal images) -----------
 
 __device__ __forceinline__ float density_at_index(const DevScene& s, V3 idx) {
@@ -33,11 +49,12 @@ __device__ __forceinline__ float density_at_index(const DevScene& s, V3 idx) {
     const uint32_t Y0 = g.yoff(iclamp(y0, 0, g.ny - 1)), Y1 = g.yoff(iclamp(y0 + 1, 0, g.ny - 1));
     const uint32_t Z0 = g.zoff(iclamp(z0, 0, g.nz - 1)), Z1 = g.zoff(iclamp(z0 + 1, 0, g.nz - 1));
     const uint16_t* d = s.density;
-    const float c000 = d[X0 + Y0 + Z0], c100 = d[X1 + Y0 + Z0], c010 = d[X0 + Y1 + Z0], c110 = d[X1 + Y1 + Z0];
-    const float c001 = d[X0 + Y0 + Z1], c101 = d[X1 + Y0 + Z1], c011 = d[X0 + Y1 + Z1], c111 = d[X1 + Y1 + Z1];
+    auto v = [&](uint32_t i) { return unorm16_value(float(d[i])); };
+    const float c000 = v(X0 + Y0 + Z0), c100 = v(X1 + Y0 + Z0), c010 = v(X0 + Y1 + Z0), c110 = v(X1 + Y1 + Z0);
+    const float c001 = v(X0 + Y0 + Z1), c101 = v(X1 + Y0 + Z1), c011 = v(X0 + Y1 + Z1), c111 = v(X1 + Y1 + Z1);
     const float c00 = mixf(c000, c100, ax), c10 = mixf(c010, c110, ax);
     const float c01 = mixf(c001, c101, ax), c11 = mixf(c011, c111, ax);
-    return mixf(mixf(c00, c10, ay), mixf(c01, c11, ay), az) * kInv65535;
+    return mixf(mixf(c00, c10, ay), mixf(c01, c11, ay), az);
 }
 
 struct F2 {
@@ -59,11 +76,12 @@ __device__ __forceinline__ F2 mask_at(const DevScene& s, V3 idx) {
 #pragma unroll
     for (int ch = 0; ch < 2; ch++) {
         const int sh = 16 * ch;
-        const float c00 = mixf(float((t[0] >> sh) & 0xffffu), float((t[1] >> sh) & 0xffffu), ax);
-        const float c10 = mixf(float((t[2] >> sh) & 0xffffu), float((t[3] >> sh) & 0xffffu), ax);
-        const float c01 = mixf(float((t[4] >> sh) & 0xffffu), float((t[5] >> sh) & 0xffffu), ax);
-        const float c11 = mixf(float((t[6] >> sh) & 0xffffu), float((t[7] >> sh) & 0xffffu), ax);
-        const float v = mixf(mixf(c00, c10, ay), mixf(c01, c11, ay), az) * kInv65535;
+        auto c = [&](int k) { return unorm16_value(float((t[k] >> sh) & 0xffffu)); };
+        const float c00 = mixf(c(0), c(1), ax);
+        const float c10 = mixf(c(2), c(3), ax);
+        const float c01 = mixf(c(4), c(5), ax);
+        const float c11 = mixf(c(6), c(7), ax);
+        const float v = mixf(mixf(c00, c10, ay), mixf(c01, c11, ay), az);
         if (ch == 0) out.r = v;
         else out.g = v;
     }
@@ -80,7 +98,7 @@ struct F4 {
 };
 
 __device__ __forceinline__ float snorm8(uint32_t word, int ch) {
-    return gmax(float(int8_t((word >> (8 * ch)) & 0xffu)) * (1.0f / 127.0f), -1.0f);
+    return snorm8_value(float(int8_t((word >> (8 * ch)) & 0xffu)));
 }
 
 __device__ __forceinline__ F4 normal_at(const DevScene& s, V3 idx) {
@@ -176,13 +194,13 @@ __device__ __forceinline__ WindowAddr window_addr(const BrickGrid& g, int ix, in
     return w;
 }
 
-// One z slice (4 x 4 raw values) of the window.
+// One z slice (4 x 4 texel values) of the window.
 __device__ __forceinline__ void load_slice(const uint16_t* __restrict__ d, const WindowAddr& w, int z, float v[4][4]) {
 #pragma unroll
     for (int y = 0; y < 4; y++) {
         const uint32_t base = w.y[y] + w.z[z];
 #pragma unroll
-        for (int x = 0; x < 4; x++) v[y][x] = float(d[w.x[x] + base]);
+        for (int x = 0; x < 4; x++) v[y][x] = unorm16_value(float(d[w.x[x] + base]));
     }
 }
 // The slice's 4 rows lerped along x at tap t.
@@ -194,9 +212,9 @@ __device__ __forceinline__ void slice_x(const float v[4][4], Tap t, float out[4]
 __device__ __forceinline__ float tap_weight(Tap t, int z) {
     return z == t.o ? 1.0f - t.a : (z == t.o + 1 ? t.a : 0.0f);
 }
-// acc + c w, fused like the second product of mixf, so that summing the slices
-// in order reproduces mixf of the two weighted ones exactly.
-__device__ __forceinline__ float accum_tap(float acc, float c, float w) { return fmaf(c, w, acc); }
+// acc + c w, rounded like the terms of mixf, so that summing the slices in order
+// (zero weights add an exact zero) reproduces mixf of the two weighted ones exactly.
+__device__ __forceinline__ float accum_tap(float acc, float c, float w) { return acc + c * w; }
 
 // cubic_fetch's final combination (common.glsl:121) of the 8 trilinear values.
 __device__ __forceinline__ float fold8(const float T[2][2][2], float gx0, float gx1, float gy0, float gy1, float gz0,
@@ -237,7 +255,7 @@ __device__ inline float density_cubic(const DevScene& s, V3 idx) {
                 T[a][b][1] = accum_tap(T[a][b][1], c, wz[1]);
             }
     }
-    return fold8(T, tx.g0, tx.g1, ty.g0, ty.g1, tz.g0, tz.g1) * kInv65535;
+    return fold8(T, tx.g0, tx.g1, ty.g0, ty.g1, tz.g0, tz.g1);
 }
 
 // density_gradient_cubic (common.glsl:130-136), per index unit: the 24 trilinear
@@ -280,7 +298,7 @@ __device__ inline V3 density_gradient_cubic(const DevScene& s, V3 idx) {
     const float gx = fold8(GX, tx.dg0, tx.dg1, ty.g0, ty.g1, tz.g0, tz.g1);
     const float gy = fold8(GY, tx.g0, tx.g1, ty.dg0, ty.dg1, tz.g0, tz.g1);
     const float gz = fold8(GZ, tx.g0, tx.g1, ty.g0, ty.g1, tz.dg0, tz.dg1);
-    return V3(gx * kInv65535, gy * kInv65535, gz * kInv65535);
+    return V3(gx, gy, gz);
 }
 
 // density_view (common.glsl:160-162).
@@ -525,12 +543,12 @@ __device__ inline V3 mask_gradient(const DevScene& s, V3 idx, int ch) {
         Z[k] = g.zoff(iclamp(z0 - 1 + k, 0, g.nz - 1));
     }
     const int sh = 16 * ch;
-    auto t = [&](int i, int j, int k) { return float((s.mask[X[i] + Y[j] + Z[k]] >> sh) & 0xffffu); };
+    auto t = [&](int i, int j, int k) { return unorm16_value(float((s.mask[X[i] + Y[j] + Z[k]] >> sh) & 0xffffu)); };
     // Trilinear value with lower corner (i, j, k) in window indices: lerp x, y, z.
     auto tri = [&](float c000, float c100, float c010, float c110, float c001, float c101, float c011, float c111) {
         const float c00 = mixf(c000, c100, ax), c10 = mixf(c010, c110, ax);
         const float c01 = mixf(c001, c101, ax), c11 = mixf(c011, c111, ax);
-        return mixf(mixf(c00, c10, ay), mixf(c01, c11, ay), az) * kInv65535;
+        return mixf(mixf(c00, c10, ay), mixf(c01, c11, ay), az);
     };
     // x differences: rows y, z in {1, 2}, x in 0..3.
     float vx[2][2][4];

@@ -31,7 +31,7 @@ __device__ __forceinline__ float4 shadow_pass(const DevScene& s, int px, int py,
     const V3 mid = (here + prev) * 0.5f + V3(0.5f);
     const Sample m = sample_point(s, tex_to_index(u, V3(mid.x / sd.x, mid.y / sd.y, mid.z / sd.z)));
     const float4 at_prev = shadow_linear(s, prev, k);
-    const float direct = at_prev.w * expf(-m.extinction * step);
+    const float direct = at_prev.w * vexp(-m.extinction * step);
     // The diffused light gathered from a disc of the previous slice (radius
     // subsurface.z shadow texels along the slice's two axes).
     const float r = u->subsurface[2];
@@ -48,8 +48,8 @@ __device__ __forceinline__ float4 shadow_pass(const DevScene& s, int px, int py,
     const float mx = gmax(gmax(m.albedo.x, gmax(m.albedo.y, m.albedo.z)), 1e-4f);
     const V3 chroma = m.albedo / mx;
     const V3 ext = (V3(1.2f) - chroma) * (m.extinction * u->subsurface[1]);
-    return make_float4(gathered.x * expf(-ext.x * step), gathered.y * expf(-ext.y * step),
-                       gathered.z * expf(-ext.z * step), direct);
+    return make_float4(gathered.x * vexp(-ext.x * step), gathered.y * vexp(-ext.y * step),
+                       gathered.z * vexp(-ext.z * step), direct);
 }
 
 // Threads run along the slice's axis with the smaller memory stride (x where
@@ -84,10 +84,10 @@ __global__ void __launch_bounds__(64) k_ao(DevScene s, uint16_t* __restrict__ ou
         const float zc = 1.0f - (float(d) + 0.5f) * (2.0f / 16.0f);
         const float r = sqrtf(1.0f - zc * zc);
         const float phi = float(d) * 2.39996323f;
-        const V3 dir = ti.mul(V3(r * cosf(phi), r * sinf(phi), zc));
+        const V3 dir = ti.mul(V3(r * cr_cos(phi), r * cr_sin(phi), zc));
         float depth = 0.0f;
         for (int j = 0; j < 8; j++) depth += sample_point(s, center + dir * ((float(j) + 0.5f) * dt)).extinction * dt;
-        visible += expf(-depth);
+        visible += vexp(-depth);
     }
     out[i] = __half_as_ushort(__float2half_rn(gmin(2.0f * visible / 16.0f, 1.0f)));
 }

@@ -109,18 +109,6 @@ void launch_denoise(const float4* accum, const float4* albedo, const float4* nor
 // tile_base; total_tiles = the last job's tile_base + its tile count).
 void launch_dvr_batched(const DvrJob* jobs, uint32_t n_jobs, uint32_t total_tiles, hipStream_t st);
 
-// pt.comp for `samples` paths per pixel at path indices first_path ..: adds to
-// accum/albedo/normal (reset when first_path == 0) and writes hdr.
-void launch_pt(const DevScene& s, float4* accum, float4* albedo, float4* normal, uint16_t* hdr, uint32_t width,
-               uint32_t height, uint32_t first_path, uint32_t samples, hipStream_t st);
-
-// The same frame by persistent waves pulling pixels from `queue` (one uint32,
-// zero at launch): `waves` single-wave workgroups.
-int pt_persistent_waves_per_cu();
-void launch_pt_persistent(const DevScene& s, float4* accum, float4* albedo, float4* normal, uint16_t* hdr,
-                          uint32_t width, uint32_t height, uint32_t first_path, uint32_t samples, uint32_t* queue,
-                          uint32_t waves, hipStream_t st);
-
 // Wavefront path tracer (kernels_wavefront.hip). One iteration runs the five
 // stages over the queues; `reqs` is a device array of n_reqs requests with
 // item ranges [item_base, item_base + n_items) covering [0, total_items).

@@ -12,6 +12,15 @@ namespace {
 // spirit of maxImageDimension3D (renderer.rs:561-567).
 constexpr uint32_t kMaxDim = 8192;
 
+// Server requests read a held renderer's volumes, tables and light map from the
+// server's stream at any time: calls that change them are refused meanwhile.
+void refuse_if_held(vvr_renderer* r, const char* what) {
+    std::lock_guard<std::mutex> lk(r->hold_mu);
+    if (r->holds > 0)
+        throw Error{VVR_ESTATE, std::string(what) + " on a renderer that server requests still read (" +
+                                    std::to_string(r->holds) + " queued or running)"};
+}
+
 void new_renderer(vvr_ctx* ctx, const vvr_volume* vol, const vvr_tf* tf, vvr_renderer** out) {
     if (!ctx || !vol || !out) throw Error{VVR_EINVAL, "null argument"};
     if (!vol->hu) throw Error{VVR_EINVAL, "volume has no voxel data"};
@@ -38,19 +47,10 @@ void new_renderer(vvr_ctx* ctx, const vvr_volume* vol, const vvr_tf* tf, vvr_ren
     check(hipHostMalloc((void**)&r->h_uniforms, 2 * sizeof(vvr_uniforms), hipHostMallocDefault), "hipHostMalloc");
     for (auto& d : r->d_uniforms) d = dalloc<vvr_uniforms>(1, "uniforms");
     r->lut = dalloc<float4>(kLutSize, "transfer function");
-    r->queue = dalloc<uint32_t>(1, "work queue");
     // Auto exposure, adapted over frames: 0 = not yet metered (renderer.rs:791-818).
     r->exposure = dalloc<float>(1, "exposure");
     check(hipMemsetAsync(r->exposure, 0, sizeof(float), r->stream), "hipMemset");
-    {
-        int cus = 0;
-        check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, r->device), "device attribute");
-        r->pt_waves = uint32_t(std::max(cus, 1) * pt_persistent_waves_per_cu());
-        const char* k = std::getenv("VVR_PT_KERNEL");
-        if (k && std::strcmp(k, "simple") == 0) r->pt_kernel = vvr_renderer::kSimple;
-        if (k && std::strcmp(k, "persistent") == 0) r->pt_kernel = vvr_renderer::kPersistent;
-        r->pt_chunk = env_u32("VVR_PT_CHUNK", 1);
-    }
+    r->pt_chunk = env_u32("VVR_PT_CHUNK", 1);
     r->preint = dalloc<float4>(kLutSize, "pre-integration table");
     r->upload_tf(*tf);
 
@@ -90,6 +90,7 @@ void set_labels(vvr_renderer* r, const uint8_t* labels, size_t n, const vvr_soft
         for (int a = 0; a < 3; a++)
             if (soft->src_axis[a] > 2) throw Error{VVR_EINVAL, "soft label grid axis out of range"};
     }
+    refuse_if_held(r, "set_labels");
     r->use_device();
     check(hipStreamSynchronize(r->stream), "device wait");  // device_wait_idle (renderer.rs:1256)
     const BrickGrid& g = r->grid;
@@ -139,7 +140,6 @@ void record(vvr_renderer* r, uint64_t frame, const vvr_frame_params* p);
 void autotune(vvr_renderer* r, const vvr_frame_params* p, uint32_t trials, vvr_pt_shape* chosen) {
     if (!r || !p || !chosen) throw Error{VVR_EINVAL, "null argument"};
     if (!p->path_tracing_enabled) throw Error{VVR_EINVAL, "the autotuner times path-traced frames"};
-    if (r->pt_kernel != vvr_renderer::kWavefront) throw Error{VVR_EINVAL, "the renderer is not on the wavefront engine"};
     r->use_device();
     trials = std::max(trials, 1u);
     WfEngine& e = wavefront(r);
@@ -199,6 +199,7 @@ void record(vvr_renderer* r, uint64_t frame, const vvr_frame_params* p) {
     if (!r || !p) throw Error{VVR_EINVAL, "null argument"};
     if (!r->hdr) throw Error{VVR_ESTATE, "resize() must be called before record()"};
     validate_frame(*p);
+    refuse_if_held(r, "record");
     const bool pt = p->path_tracing_enabled != 0;
     if (pt && p->path_tracing.denoiser == VVR_DENOISE_LEARNED && !r->unet)
         throw Error{VVR_EINVAL, "no learned denoiser is attached (vvr_renderer_set_denoiser)"};
@@ -222,24 +223,12 @@ void record(vvr_renderer* r, uint64_t frame, const vvr_frame_params* p) {
     if (pt) {
         const uint32_t spf = std::max(p->path_tracing.samples_per_frame, 1u);
         const uint32_t first = r->accumulated;
-        if (r->pt_kernel == vvr_renderer::kWavefront) {
-            wavefront(r);
-            PtRequest q{};
-            q.scene = sc;
-            q.accum = r->accum, q.albedo = r->albedo, q.normal = r->normal, q.hdr = r->hdr;
-            WfEngine::shape(q, r->width, r->height, first, spf, r->pt_chunk);
-            r->timed(VVR_KERNEL_PT, [&] { r->wf->run(&q, 1, r->stream, u.lighting[3] == 2u); });
-        } else if (r->pt_kernel == vvr_renderer::kSimple) {
-            r->timed(VVR_KERNEL_PT, [&] {
-                launch_pt(sc, r->accum, r->albedo, r->normal, r->hdr, r->width, r->height, first, spf, r->stream);
-            });
-        } else {
-            check(hipMemsetAsync(r->queue, 0, sizeof(uint32_t), r->stream), "queue reset");
-            r->timed(VVR_KERNEL_PT, [&] {
-                launch_pt_persistent(sc, r->accum, r->albedo, r->normal, r->hdr, r->width, r->height, first, spf,
-                                     r->queue, r->pt_waves, r->stream);
-            });
-        }
+        PtRequest q{};
+        q.scene = sc;
+        q.accum = r->accum, q.albedo = r->albedo, q.normal = r->normal, q.hdr = r->hdr;
+        WfEngine::shape(q, r->width, r->height, first, spf, r->pt_chunk);  // throws before anything is queued
+        wavefront(r);
+        r->timed(VVR_KERNEL_PT, [&] { r->wf->run(&q, 1, r->stream, u.lighting[3] == 2u); });
         r->accumulated += spf;
         // renderer.rs:1795-1797: the a-trous denoiser replaces the HDR image with
         // its filtered version of everything accumulated so far.
@@ -300,6 +289,7 @@ int vvr_renderer_set_transfer_function(vvr_renderer* r, const vvr_tf* tf) {
     return guarded([&] {
         if (!r) throw Error{VVR_EINVAL, "null argument"};
         validate_tf(tf);
+        refuse_if_held(r, "set_transfer_function");
         r->use_device();
         check(hipStreamSynchronize(r->stream), "device wait");
         r->upload_tf(*tf);
@@ -312,6 +302,7 @@ int vvr_renderer_set_environment_map(vvr_renderer* r, const vvr_env_map* map) {
         if (!r) throw Error{VVR_EINVAL, "null argument"};
         if (map && (!map->rgb || map->width < 2 || map->height < 1 || map->width > 16384 || map->height > 8192))
             throw Error{VVR_EINVAL, "light map must be 2..16384 x 1..8192 RGB texels"};
+        refuse_if_held(r, "set_environment_map");
         r->use_device();
         check(hipStreamSynchronize(r->stream), "device wait");
         dfree(r->env_rad), dfree(r->env_table), dfree(r->env_dif);
@@ -336,6 +327,7 @@ int vvr_renderer_set_environment_map(vvr_renderer* r, const vvr_env_map* map) {
 int vvr_renderer_set_denoiser(vvr_renderer* r, vvr_unet* net) {
     return guarded([&] {
         if (!r) throw Error{VVR_EINVAL, "null argument"};
+        refuse_if_held(r, "set_denoiser");
         r->use_device();
         check(hipStreamSynchronize(r->stream), "device wait");
         r->unet = net;
@@ -361,6 +353,7 @@ int vvr_renderer_resize(vvr_renderer* r, uint32_t width, uint32_t height) {
         if (!r) throw Error{VVR_EINVAL, "null argument"};
         if (!width || !height || width > 16384 || height > 16384)
             throw Error{VVR_EINVAL, "image size must be 1..16384 per axis"};
+        refuse_if_held(r, "resize");
         r->use_device();
         check(hipStreamSynchronize(r->stream), "device wait");
         r->free_frame();

@@ -159,7 +159,9 @@ struct WfEngine {
                             grid.tail_paths ? grid.tail_paths : UINT32_MAX};
     }
 
-    // Items of `samples` paths per pixel at `chunk` paths per item.
+    // Items of `samples` paths per pixel at `chunk` paths per item. Work items are
+    // numbered in 32 bits: a request of 2^31 items or more is refused here, before
+    // anything is allocated or queued.
     static void shape(PtRequest& q, uint32_t width, uint32_t height, uint32_t first, uint32_t samples,
                       uint32_t chunk) {
         q.width = width, q.height = height;
@@ -168,7 +170,11 @@ struct WfEngine {
         q.first_path = first, q.samples = samples;
         q.chunk = std::min(std::max(chunk, 1u), samples);
         q.n_chunks = (samples + q.chunk - 1) / q.chunk;
-        q.n_items = q.n_chunks * q.n_px;
+        const uint64_t items = uint64_t(q.n_chunks) * q.n_px;
+        if (items >= (1ull << 31))
+            throw Error{VVR_ELIMIT, "too many paths in one frame: " + std::to_string(items) +
+                                        " work items (the limit is 2^31; lower samples_per_frame)"};
+        q.n_items = uint32_t(items);
     }
 
     // Renders the requests (scene, images and shape set) on `st`. Returns once
@@ -179,6 +185,8 @@ struct WfEngine {
         size_t partial_f4 = 0;
         uint64_t items = 0;
         for (uint32_t i = 0; i < n; i++) {
+            if (uint64_t(reqs[i].n_chunks) * reqs[i].n_px != reqs[i].n_items)
+                throw Error{VVR_ELIMIT, "request shape out of range"};
             partial_f4 += size_t(reqs[i].n_chunks) * 3u * reqs[i].n_px;
             items += reqs[i].n_items;
         }
@@ -340,13 +348,8 @@ struct vvr_renderer {
     float* exposure = nullptr;     // adapted auto exposure, persists across frames
     bool denoised = false;         // the HDR image holds the denoiser's output
     vvr_unet* unet = nullptr;      // the learned denoiser (VVR_DENOISE_LEARNED), owned by the caller
-    // Path-tracing engine: the wavefront engine (default), or with
-    // VVR_PT_KERNEL=persistent / simple the persistent megakernel or one lane per pixel.
-    enum { kWavefront, kPersistent, kSimple } pt_kernel = kWavefront;
-    std::unique_ptr<WfEngine> wf;
-    uint32_t pt_chunk = 1;       // paths per wavefront work item (VVR_PT_CHUNK)
-    uint32_t* queue = nullptr;   // work queue of the persistent path tracer
-    uint32_t pt_waves = 0;       // its grid: waves resident on the device
+    std::unique_ptr<WfEngine> wf;  // the wavefront path tracer, created on the first path-traced frame
+    uint32_t pt_chunk = 1;         // paths per wavefront work item (VVR_PT_CHUNK)
     uint32_t accumulated = 0;
     bool acc_valid = false;
     vvr_frame_params acc_params{};
@@ -530,7 +533,7 @@ struct vvr_renderer {
         wf.reset();
         dfree(density), dfree(normals), dfree(majorant), dfree(labels), dfree(mask), dfree(mask_b), dfree(soft);
         dfree(shadow), dfree(ao), dfree(env_rad), dfree(env_table), dfree(env_dif);
-        dfree(lut), dfree(preint), dfree(d_uniforms[0]), dfree(d_uniforms[1]), dfree(queue), dfree(exposure);
+        dfree(lut), dfree(preint), dfree(d_uniforms[0]), dfree(d_uniforms[1]), dfree(exposure);
         if (h_uniforms) (void)hipHostFree(h_uniforms);
         for (auto& e : slot_done)
             if (e) (void)hipEventDestroy(e);

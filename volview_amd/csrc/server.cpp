@@ -90,8 +90,15 @@ struct Job {
     float4 *accum = nullptr, *albedo = nullptr, *normal = nullptr, *partial = nullptr;
     float4* dn_scratch = nullptr;  // a-trous denoiser images (4 per pixel), when the request asks for it
     uint16_t* hdr = nullptr;
-    uint16_t* h_hdr = nullptr;  // pinned copy of the newest frame
+    // Pinned copies of the newest frames, two in turn: read-backs land in the
+    // buffer that is not the front one, and become the front once complete.
+    // Fields below are guarded by the server's mutex.
+    uint16_t* h_hdr[2] = {nullptr, nullptr};
+    int front = -1;              // the buffer holding the newest delivered frame
+    int landing = -1;            // the buffer the read-back in flight writes
+    int host_readers[2] = {0, 0};  // vvr_server_read_hdr calls copying out of each buffer
     hipEvent_t read_ev = nullptr;
+    hipEvent_t vol_ready = nullptr;  // the renderer's stream at submission: work recorded before it
 };
 
 // One increment of a path-traced request in the wavefront engine.
@@ -105,6 +112,14 @@ struct Entry {
 
 size_t chunk_sum_bytes(const Job& j) {  // one increment's chunk sums
     return size_t(j.increment) * 3u * ((j.width + 7) / 8) * ((j.height + 7) / 8) * 64u * sizeof(float4);
+}
+
+// A finished request's pinned frames and events.
+void free_host_side(Job* j) {
+    for (auto& b : j->h_hdr) hfree(b), b = nullptr;
+    if (j->read_ev) (void)hipEventDestroy(j->read_ev);
+    if (j->vol_ready) (void)hipEventDestroy(j->vol_ready);
+    j->read_ev = j->vol_ready = nullptr;
 }
 
 bool denoised(const Job& j) { return j.pt && j.params.path_tracing.denoiser == VVR_DENOISE_ATROUS; }
@@ -163,7 +178,7 @@ struct vvr_server {
 
     // requests
     std::mutex mu;
-    std::condition_variable cv_submit, cv_done;
+    std::condition_variable cv_submit, cv_done, cv_read;
     std::deque<Job*> queue;  // submitted, not admitted
     std::unordered_map<uint64_t, std::unique_ptr<Job>> jobs;
     std::vector<Job*> running;  // admitted, not finished
@@ -189,6 +204,7 @@ struct vvr_server {
     void drop_active(uint32_t e);
     void free_entry(int e);
     void finish(Job* j, int status, const std::string& msg, const vvr_completion* last = nullptr);
+    void fail_all(int code, const std::string& msg);
     // The request is done with its renderer. A renderer destroyed meanwhile is
     // deleted here by its last request, after the server's work on it.
     void let_go(Job* j) {
@@ -286,8 +302,11 @@ void vvr_server::admit(std::vector<Job*>& fresh_dvr) {
                     dalloc<uint8_t>(std::min(j->n_inc, kInFlight) * chunk_sum_bytes(*j), "request chunk sums"));
                 if (denoised(*j)) j->dn_scratch = dalloc<float4>(4 * px, "request denoiser images");
             }
-            j->h_hdr = halloc<uint16_t>(4 * px, "frame read-back");
+            for (auto& b : j->h_hdr) b = halloc<uint16_t>(4 * px, "frame read-back");
             if (!j->read_ev) check(hipEventCreateWithFlags(&j->read_ev, hipEventDisableTiming), "hipEventCreate");
+            // Whatever the caller recorded on the renderer before submitting may
+            // still read or write its volumes: the server's work waits for it.
+            check(hipStreamWaitEvent(stream, j->vol_ready, 0), "hipStreamWaitEvent");
             j->bytes = need;
             {
                 std::lock_guard<std::mutex> lk(mu);
@@ -392,7 +411,16 @@ void vvr_server::launch_dvr(const std::vector<Job*>& batch) {
 // launch on the stream (and after any earlier read-back into the same buffer).
 void vvr_server::read_back(Job* j, uint32_t spp) {
     const size_t bytes = size_t(j->width) * j->height * 8;
-    check(hipMemcpyAsync(j->h_hdr, j->hdr, bytes, hipMemcpyDeviceToHost, stream), "frame read-back");
+    int target;
+    {
+        // The buffer that is not the front one; a caller still copying out of it
+        // (it was the front when the caller started) finishes first.
+        std::unique_lock<std::mutex> lk(mu);
+        target = j->front < 0 ? 0 : j->front ^ 1;
+        cv_read.wait(lk, [&] { return j->host_readers[target] == 0; });
+        j->landing = target;
+    }
+    check(hipMemcpyAsync(j->h_hdr[target], j->hdr, bytes, hipMemcpyDeviceToHost, stream), "frame read-back");
     check(hipEventRecord(j->read_ev, stream), "hipEventRecord");
     j->read_pending = true;
     j->read_spp = spp;
@@ -541,6 +569,11 @@ void vvr_server::poll_reads() {
         if (q == hipErrorNotReady) continue;
         check(q, "frame read-back");
         j->read_pending = false;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            j->front = j->landing;
+            j->landing = -1;
+        }
         j->spp_delivered = j->read_spp;
         const bool last = !j->pt || j->read_spp >= j->spp_total;
         const vvr_completion c = record(j, VVR_OK, j->read_spp, last);
@@ -608,11 +641,36 @@ void vvr_server::loop() {
             poll_reads();
         }
     } catch (const Error& e) {
-        std::lock_guard<std::mutex> lk(mu);
-        fatal = e.msg;
-        for (Job* j : running) completions.push_back(record(j, e.code, 0, true));
-        cv_done.notify_all();
+        fail_all(e.code, e.msg);
+    } catch (const std::bad_alloc&) {
+        fail_all(VVR_ENOMEM, "host allocation failed");
+    } catch (...) {
+        fail_all(VVR_EDEVICE, "unexpected internal error");
     }
+}
+
+// The loop failed: every admitted and queued request ends with the error (and
+// can be released), and later submissions are refused.
+void vvr_server::fail_all(int code, const std::string& msg) {
+    (void)hipStreamSynchronize(stream);
+    std::vector<Job*> snap;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        fatal = msg;
+        snap = running;
+    }
+    for (Job* j : snap) finish(j, code, msg);
+    std::lock_guard<std::mutex> lk(mu);
+    for (Job* j : queue) {
+        let_go(j);
+        j->state = Job::kDone;
+        j->status = code;
+        j->error = msg;
+        completions.push_back(record(j, code, 0, true));
+        stats.completed++;
+    }
+    queue.clear();
+    cv_done.notify_all();
 }
 
 namespace {
@@ -691,6 +749,9 @@ int vvr_server_submit(vvr_server* s, const vvr_request* rq, uint64_t* id) {
         }
         if (size_t need = request_bytes(*j); need > s->cfg.hbm_budget)
             throw Error{VVR_ELIMIT, "request needs " + std::to_string(need) + " bytes, over the server's HBM budget"};
+        check(hipSetDevice(s->device), "hipSetDevice");
+        check(hipEventCreateWithFlags(&j->vol_ready, hipEventDisableTiming), "hipEventCreate");
+        check(hipEventRecord(j->vol_ready, j->vol->stream), "hipEventRecord");
         j->t_submit = now_ns();
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->stopping) throw Error{VVR_ESTATE, "server is stopping"};
@@ -721,24 +782,33 @@ int vvr_server_poll(vvr_server* s, vvr_completion* out, uint32_t cap, uint32_t* 
     });
 }
 
-int vvr_server_read_hdr(vvr_server* s, uint64_t id, float* rgba) {
+int vvr_server_read_hdr(vvr_server* s, uint64_t id, float* rgba, size_t n_floats) {
     return guarded([&] {
         if (!s || !rgba) throw Error{VVR_EINVAL, "null argument"};
         Job* j = nullptr;
+        int f = -1;
         {
             std::lock_guard<std::mutex> lk(s->mu);
             auto it = s->jobs.find(id);
             if (it == s->jobs.end()) throw Error{VVR_EINVAL, "unknown request"};
             j = it->second.get();
             const bool delivered = j->pt ? j->spp_delivered > 0 : (j->state == Job::kDone && j->status == VVR_OK);
-            if (!delivered) {
+            if (!delivered || j->front < 0) {
                 if (j->state == Job::kDone && j->status != VVR_OK)
                     throw Error{j->status, "the request failed: " + j->error};
                 throw Error{VVR_ESTATE, "the request has no delivered frame"};
             }
+            if (n_floats != size_t(j->width) * j->height * 4)
+                throw Error{VVR_EINVAL, "buffer holds " + std::to_string(n_floats) + " floats, the frame " +
+                                            std::to_string(size_t(j->width) * j->height * 4)};
+            f = j->front;
+            j->host_readers[f]++;  // no read-back lands in this buffer until the copy below is done
         }
-        const size_t n = size_t(j->width) * j->height * 4;
-        for (size_t i = 0; i < n; i++) rgba[i] = f16_bits_to_f32(j->h_hdr[i]);
+        const uint16_t* src = j->h_hdr[f];
+        for (size_t i = 0; i < n_floats; i++) rgba[i] = f16_bits_to_f32(src[i]);
+        std::lock_guard<std::mutex> lk(s->mu);
+        j->host_readers[f]--;
+        s->cv_read.notify_all();
     });
 }
 
@@ -770,9 +840,10 @@ int vvr_server_release(vvr_server* s, uint64_t id) {
         std::lock_guard<std::mutex> lk(s->mu);
         auto it = s->jobs.find(id);
         if (it == s->jobs.end()) throw Error{VVR_EINVAL, "unknown request"};
-        if (it->second->state != Job::kDone) throw Error{VVR_ESTATE, "the request has not finished"};
-        hfree(it->second->h_hdr);
-        if (it->second->read_ev) (void)hipEventDestroy(it->second->read_ev);
+        Job* j = it->second.get();
+        if (j->state != Job::kDone) throw Error{VVR_ESTATE, "the request has not finished"};
+        if (j->host_readers[0] || j->host_readers[1]) throw Error{VVR_ESTATE, "the request's frame is being read"};
+        free_host_side(j);
         s->jobs.erase(it);
     });
 }
@@ -805,8 +876,7 @@ void vvr_server_destroy(vvr_server* s) {
         Job* j = kv.second.get();
         if (j->state == Job::kRunning) s->release_memory(j);
         s->let_go(j);
-        hfree(j->h_hdr);
-        if (j->read_ev) (void)hipEventDestroy(j->read_ev);
+        free_host_side(j);
     }
     dfree(s->d_table), dfree(s->d_ctr), dfree(s->d_uniforms);
     hfree(s->h_table), hfree(s->h_uniforms), hfree(s->h_ctr_buf[0]), hfree(s->h_ctr_buf[1]);

@@ -4,13 +4,14 @@ volumes rendered side by side on one device.
     srv = Server(ctx, pt_increment=16)
     rid = srv.submit(renderer, params, 1024, 1024, tag=7)
     for c in srv.poll(wait_ms=100): ...            # frame completions
-    img = srv.read_hdr(rid, 1024, 1024)
+    img = srv.read_hdr(rid)                        # float32 [1024, 1024, 4]
     srv.release(rid)
 
 A request's frames equal those of its renderer recording the same parameters
 with samples_per_frame = pt_increment, frame after frame (ray-marched frames
-equal one record()). The renderers stay owned by the caller; one attached to a
-request in flight must not be recorded on, resized or relabelled meanwhile.
+equal one record()). The renderers stay owned by the caller; while a request
+holds one (from submission to its final frame), record, resize and the set_*
+calls on it are refused with VVR_ESTATE.
 """
 from __future__ import annotations
 
@@ -29,6 +30,7 @@ class Server:
         self.ctx = ctx
         cfg = ServerConfig(hbm_budget, slots, pt_increment, max_requests, flight_steps, shadow_steps)
         self._held = {}  # request id -> its renderer, kept alive while the server may read it
+        self._dims = {}  # request id -> (width, height) of its frames
         h = c_void_p()
         check(self._lib.vvr_server_create(ctx.handle, byref(cfg), byref(h)))
         self.handle = h
@@ -38,6 +40,7 @@ class Server:
         rid = c_uint64()
         check(self._lib.vvr_server_submit(self.handle, byref(rq), byref(rid)))
         self._held[int(rid.value)] = renderer
+        self._dims[int(rid.value)] = (int(width), int(height))
         return int(rid.value)
 
     def poll(self, cap: int = 256, wait_ms: int = 0) -> list[Completion]:
@@ -46,9 +49,14 @@ class Server:
         check(self._lib.vvr_server_poll(self.handle, out, cap, byref(n), wait_ms))
         return [out[i] for i in range(n.value)]
 
-    def read_hdr(self, rid: int, width: int, height: int) -> np.ndarray:
-        img = np.zeros((height, width, 4), np.float32)
-        check(self._lib.vvr_server_read_hdr(self.handle, rid, img.ctypes.data_as(POINTER(c_float))))
+    def read_hdr(self, rid: int, width: int | None = None, height: int | None = None) -> np.ndarray:
+        """The request's newest delivered frame, sized from its submission;
+        width and height, when given, must match it."""
+        w, h = self._dims.get(rid, (width or 0, height or 0))
+        if (width is not None and width != w) or (height is not None and height != h):
+            raise ValueError(f"request {rid} renders {w} x {h} frames, not {width} x {height}")
+        img = np.zeros((h, w, 4), np.float32)
+        check(self._lib.vvr_server_read_hdr(self.handle, rid, img.ctypes.data_as(POINTER(c_float)), img.size))
         return img
 
     def cancel(self, rid: int):
@@ -57,6 +65,7 @@ class Server:
     def release(self, rid: int):
         check(self._lib.vvr_server_release(self.handle, rid))
         self._held.pop(rid, None)
+        self._dims.pop(rid, None)
 
     def stats(self) -> dict:
         s = ServerStats()
@@ -68,6 +77,7 @@ class Server:
             self._lib.vvr_server_destroy(self.handle)
             self.handle = None
         self._held = {}
+        self._dims = {}
 
     def __del__(self):
         try:
