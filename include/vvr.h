@@ -342,6 +342,15 @@ int vvr_unet_train_step(vvr_unet* net, uint32_t batch, uint32_t width, uint32_t 
 /* The denoised frame (rgba floats: radiance, background share) of host images. */
 int vvr_unet_denoise(vvr_unet* net, uint32_t width, uint32_t height, const float* accum, const float* albedo,
                      const float* normal, float samples, float* rgba);
+/* No counterpart in the reference. For callers that cannot allocate device or
+ * host memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid it): vvr_unet_denoise, same inputs and bit-identical output, with every
+ * buffer of the call in a static device array of 64 MiB per GPU that the library
+ * holds while loaded. Enough for the 5-level 32-128-channel network on frames of
+ * about 96 Ki pixels (256 x 384); larger frames fail with VVR_ELIMIT. Calls on one
+ * device share the array, so calls that overlap on one device race. */
+int vvr_unet_denoise_noalloc(vvr_unet* net, uint32_t width, uint32_t height, const float* accum,
+                             const float* albedo, const float* normal, float samples, float* rgba);
 /* Frames recorded with VVR_DENOISE_LEARNED run `net` (owned by the caller, kept
  * alive while attached; NULL detaches). */
 int vvr_renderer_set_denoiser(vvr_renderer* r, vvr_unet* net);

@@ -174,6 +174,8 @@ SIGNATURES = [
                                     POINTER(c_float), c_float, POINTER(c_float), c_float, POINTER(c_float)]),
     ("vvr_unet_denoise", c_int, [c_void_p, c_uint32, c_uint32, POINTER(c_float), POINTER(c_float), POINTER(c_float),
                                  c_float, POINTER(c_float)]),
+    ("vvr_unet_denoise_noalloc", c_int, [c_void_p, c_uint32, c_uint32, POINTER(c_float), POINTER(c_float),
+                                         POINTER(c_float), c_float, POINTER(c_float)]),
     ("vvr_renderer_set_denoiser", c_int, [c_void_p, c_void_p]),
     ("vvr_renderer_read_environment", c_int, [c_void_p, POINTER(c_float), POINTER(c_float), c_size_t]),
     ("vvr_renderer_read_exposure", c_int, [c_void_p, POINTER(c_float)]),

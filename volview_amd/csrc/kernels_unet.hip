@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
+#include <mutex>
 
 #include "launch.hpp"
 
@@ -667,6 +668,36 @@ void launch_unet_target(const float4* t, const float* alb, int npix, float* out,
 void launch_unet_output(const float* pred, const float* alb, int wp, const float4* accum, float n, int w, int h,
                         uint16_t* hdr, hipStream_t st) {
     hipLaunchKernelGGL(k_unet_output, dim3(blocks(long(w) * h)), dim3(256), 0, st, pred, alb, wp, accum, n, w, h, hdr);
+}
+
+// ---- scratch of the allocation-free denoise (vvr_unet_denoise_noalloc) -------------
+//
+// A file-scope device array, one copy per GPU for as long as the library is
+// loaded. Host code never passes its name to a kernel or a copy (on the host the
+// name stands for nothing on the device): it uses the address hipGetSymbolAddress
+// returns for the current device, looked up once per device and kept.
+constexpr size_t kUnetScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) uint8_t scratch[kUnetScratchBytes];
+
+hipError_t unet_scratch(int device, void** ptr, size_t* bytes) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void* per_device[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!per_device[device]) {
+        int current = -1;
+        hipError_t e = hipGetDevice(&current);
+        if (e != hipSuccess) return e;
+        if (current != device) return hipErrorInvalidDevice;  // the caller makes `device` current
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        per_device[device] = p;
+    }
+    *ptr = per_device[device];
+    *bytes = kUnetScratchBytes;
+    return hipSuccess;
 }
 
 }  // namespace vvr

@@ -99,6 +99,9 @@ void launch_f32_to_bf16(const float* a, uint16_t* out, int n, hipStream_t st);
 void launch_unet_input(const float4* accum, const float4* albedo, const float4* normal, int w, int h, float n,
                        uint16_t* x, int wp, int hp, float* alb_out, hipStream_t st);
 void launch_unet_target(const float4* t, const float* alb, int npix, float* out, hipStream_t st);
+// The allocation-free denoise's scratch on `device` (which must be current): a
+// static device array of *bytes bytes, its address looked up once per device.
+hipError_t unet_scratch(int device, void** ptr, size_t* bytes);
 void launch_unet_output(const float* pred, const float* alb, int wp, const float4* accum, float n, int w, int h,
                         uint16_t* hdr, hipStream_t st);
 

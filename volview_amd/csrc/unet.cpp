@@ -123,6 +123,33 @@ struct vvr_unet {
         return used;
     }
 
+    // The forward pass's views only (no gradients) for one image of h x w within
+    // `a` (or, with a null base, only counts); returns the elements used.
+    size_t walk_forward(uint16_t* a, int h, int w) {
+        const size_t pix0 = size_t(h) * w;
+        auto px = [&](int l) { return pix0 >> (2 * l); };
+        size_t used = 0;
+        auto take = [&](View& v, size_t pix, int c) {
+            v = View{a ? a + used : nullptr, c, 0, c};
+            used += pix * size_t(c);
+        };
+        take(x0, px(0), 16);
+        for (int l = 0; l < L; l++) {
+            const int c = ch[l];
+            take(ea[l], px(l), c);
+            if (l > 0) take(p[l], px(l), ch[l - 1]);
+            if (l <= L - 2) {
+                take(cat[l], px(l), ch[l + 1] + c);
+                e[l] = View{cat[l].p, ch[l + 1] + c, ch[l + 1], c};
+                take(da[l], px(l), c);
+                take(d[l], px(l), c);
+            } else {
+                take(e[l], px(l), c);
+            }
+        }
+        return used;
+    }
+
     // Buffers for b images of h x w (multiples of 2^(L-1)).
     void shape(int b, int h, int w) {
         if (h % (1 << (L - 1)) || w % (1 << (L - 1)))
@@ -480,6 +507,76 @@ int vvr_unet_denoise(vvr_unet* n, uint32_t width, uint32_t height, const float* 
         dfree(hdr);
         check(e, "denoised read-back");
         for (size_t i = 0; i < 4 * pix; i++) rgba[i] = f16_bits_to_f32(h[i]);
+    });
+}
+
+// vvr_unet_denoise without any allocation: every buffer of the call (staged
+// inputs, activations, network output, the RGBA16F frame) lies in the library's
+// static per-device scratch; the frame's read-back is widened to float in the
+// caller's buffer. Calls on one device share that scratch, so calls that overlap
+// on one device race; the network's own lock orders calls on one network.
+int vvr_unet_denoise_noalloc(vvr_unet* n, uint32_t width, uint32_t height, const float* accum, const float* albedo,
+                             const float* normal, float samples, float* rgba) {
+    return guarded([&] {
+        if (!n || !accum || !albedo || !normal || !rgba) throw Error{VVR_EINVAL, "null argument"};
+        if (!width || !height || width > 16384 || height > 16384)
+            throw Error{VVR_EINVAL, "frame size must be 1..16384 per axis"};
+        std::lock_guard<std::mutex> lock(n->mu);
+        check(hipSetDevice(n->device), "hipSetDevice");
+        void* base = nullptr;
+        size_t cap = 0;
+        check(unet_scratch(n->device, &base, &cap), "denoiser scratch");
+        const int m = 1 << (n->L - 1);
+        const int wp = int((width + m - 1) / m * m), hp = int((height + m - 1) / m * m);
+        const size_t pix = size_t(width) * height, pix0 = size_t(wp) * hp;
+        auto align = [](size_t b) { return (b + 255) / 256 * 256; };
+        const size_t stage_b = align(pix * sizeof(float4)), hdr_b = align(pix * 8), f3_b = align(pix0 * 3 * 4);
+        const size_t act_b = align(n->walk_forward(nullptr, hp, wp) * 2);
+        const size_t need = 3 * stage_b + hdr_b + 2 * f3_b + act_b;
+        if (need > cap)
+            throw Error{VVR_ELIMIT, "a " + std::to_string(width) + " x " + std::to_string(height) + " frame needs " +
+                                        std::to_string(need) + " bytes of scratch; the allocation-free denoiser has " +
+                                        std::to_string(cap) + " (use vvr_unet_denoise)"};
+        uint8_t* q = static_cast<uint8_t*>(base);
+        float4* in_accum = reinterpret_cast<float4*>(q);
+        float4* in_albedo = reinterpret_cast<float4*>(q + stage_b);
+        float4* in_normal = reinterpret_cast<float4*>(q + 2 * stage_b);
+        uint16_t* hdr = reinterpret_cast<uint16_t*>(q + 3 * stage_b);
+        float* pred = reinterpret_cast<float*>(q + 3 * stage_b + hdr_b);
+        float* alb = reinterpret_cast<float*>(q + 3 * stage_b + hdr_b + f3_b);
+        uint16_t* act = reinterpret_cast<uint16_t*>(q + 3 * stage_b + hdr_b + 2 * f3_b);
+        n->acquire(n->stream);
+        const size_t bytes = pix * sizeof(float4);
+        check(hipMemcpyAsync(in_accum, accum, bytes, hipMemcpyHostToDevice, n->stream), "upload");
+        check(hipMemcpyAsync(in_albedo, albedo, bytes, hipMemcpyHostToDevice, n->stream), "upload");
+        check(hipMemcpyAsync(in_normal, normal, bytes, hipMemcpyHostToDevice, n->stream), "upload");
+        // The network's views and output pointers point into the scratch for this
+        // call; the next call of any other entry point lays them out again.
+        float* const saved_pred = n->pred;
+        float* const saved_alb = n->alb;
+        n->B = 1, n->H = hp, n->W = wp;
+        n->walk_forward(act, hp, wp);
+        n->pred = pred, n->alb = alb;
+        launch_unet_input(in_accum, in_albedo, in_normal, int(width), int(height), samples, n->x0.p, wp, hp, alb,
+                          n->stream);
+        n->forward();
+        launch_unet_output(pred, alb, wp, in_accum, samples, int(width), int(height), hdr, n->stream);
+        n->pred = saved_pred, n->alb = saved_alb;
+        hipError_t e = hipGetLastError();
+        n->release(n->stream);
+        // RGBA16F into the first half of the caller's buffer, then widened in place
+        // from the end: float i covers halves 2i and 2i + 1, read before it (byte
+        // copies, so the compiler keeps that order).
+        uint8_t* out = reinterpret_cast<uint8_t*>(rgba);
+        if (e == hipSuccess) e = hipMemcpyAsync(out, hdr, 8 * pix, hipMemcpyDeviceToHost, n->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(n->stream);
+        check(e, "denoised read-back");
+        for (size_t i = 4 * pix; i-- > 0;) {
+            uint16_t h;
+            std::memcpy(&h, out + 2 * i, 2);
+            const float f = f16_bits_to_f32(h);
+            std::memcpy(out + 4 * i, &f, 4);
+        }
     });
 }
 

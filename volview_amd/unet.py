@@ -62,6 +62,15 @@ class Unet:
         check(self._lib.vvr_unet_denoise(self.handle, w, h, _f(a), _f(b), _f(c), samples, _f(out)))
         return out
 
+    def denoise_noalloc(self, accum, albedo, normal, samples: float) -> np.ndarray:
+        """denoise() through the allocation-free entry point (the library's static
+        per-device scratch; frames up to about 96 Ki pixels)."""
+        a, b, c = (np.ascontiguousarray(x, np.float32) for x in (accum, albedo, normal))
+        h, w = a.shape[:2]
+        out = np.zeros((h, w, 4), np.float32)
+        check(self._lib.vvr_unet_denoise_noalloc(self.handle, w, h, _f(a), _f(b), _f(c), samples, _f(out)))
+        return out
+
     @classmethod
     def pretrained(cls, ctx: Context) -> "Unet":
         """The network trained by volview_amd.train_denoiser (5 levels, 32-128
