@@ -100,6 +100,33 @@ def load() -> ctypes.CDLL:
                                            P(c_float), P(c_float)]),
             "oracle_transmittance": (None, [P(OracleScene), P(_abi.Uniforms), P(c_float), P(c_float), c_uint32,
                                             P(c_float)]),
+            "oracle_brdf": (None, [P(c_float), P(c_float), P(c_float), P(c_float), c_float, c_float, c_float, c_int,
+                                   P(c_float), P(c_float)]),
+            "oracle_smith": (ctypes.c_double, [ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+            "oracle_bsdf_pdf": (c_float, [P(c_float), P(c_float), P(c_float), c_float, c_float, c_float]),
+            "oracle_bsdf_sample": (None, [P(c_float), P(c_float), c_float, c_float, c_float, c_uint32, c_uint32,
+                                          P(c_float)]),
+            "oracle_sample_light": (None, [P(_abi.Uniforms), P(c_float), c_uint32, c_uint32, P(c_float)]),
+            "oracle_mis": (c_float, [c_float, c_float]),
+            "oracle_hash3": (None, [P(c_int32), P(c_float)]),
+            "oracle_gradient_noise": (c_float, [P(c_float), c_int]),
+            "oracle_fbm": (c_float, [P(c_float), c_int]),
+            "oracle_cellular": (None, [P(c_float), c_int, P(c_float), P(c_float), P(c_int32)]),
+            "oracle_surface_height": (c_float, [P(c_float), P(c_float), c_int]),
+            "oracle_index_to_world": (None, [P(_abi.Uniforms), P(c_float), P(c_float)]),
+            "oracle_surface_texture": (None, [P(_abi.Uniforms), c_uint32, P(c_float), P(c_float), c_float, c_float,
+                                              P(c_float), P(c_float), P(c_float)]),
+            "oracle_bump_normal": (None, [P(_abi.Uniforms), c_uint32, P(c_float), P(c_float), P(c_float)]),
+            "oracle_drawn_label_near": (c_int, [P(OracleScene), P(_abi.Uniforms), P(c_float), c_int, P(c_uint32)]),
+            "oracle_with_label": (c_float, [P(OracleScene), P(_abi.Uniforms), c_float, c_uint32, c_int, P(c_float)]),
+            "oracle_apply_labels": (c_int, [P(OracleScene), P(_abi.Uniforms), P(c_float), P(c_float), P(c_int)]),
+            "oracle_clip_depth": (c_float, [P(_abi.Uniforms), P(c_float), P(c_int)]),
+            "oracle_tf_segment": (None, [P(OracleScene), P(_abi.Uniforms), c_float, c_float, c_int, P(c_float)]),
+            "oracle_preint_half_depth": (c_float, [P(OracleScene), P(_abi.Uniforms), c_float, c_float, c_int]),
+            "oracle_surface_frame": (c_float, [P(OracleScene), P(_abi.Uniforms), P(c_float), c_uint32, c_int,
+                                               P(c_float), c_int, c_float, c_float, c_int, P(c_float), P(c_float)]),
+            "oracle_tissue_sigma": (None, [P(_abi.Uniforms), c_uint32, c_float, P(c_float)]),
+            "oracle_pixel_stream": (None, [P(_abi.Uniforms), c_uint32, c_uint32, c_uint32, P(c_uint32)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -279,6 +306,39 @@ class Scene:
         v = np.ascontiguousarray(d, np.float32)
         return float(load().oracle_env_pdf(byref(self.c), byref(u), _p(v, c_float)))
 
+    def drawn_label_near(self, u, idx, ghost=False):
+        """(found, label) of the 21-candidate search at idx."""
+        lab = c_uint32()
+        found = load().oracle_drawn_label_near(byref(self.c), byref(u), _f3(idx), int(ghost), byref(lab))
+        return bool(found), int(lab.value)
+
+    def with_label(self, u, density, label, ghost=False):
+        """with_label of tf_point(density): (albedo, extinction)."""
+        alb = (c_float * 3)()
+        ext = load().oracle_with_label(byref(self.c), byref(u), density, label, int(ghost), alb)
+        return np.array(alb, np.float64), float(ext)
+
+    def apply_labels(self, u, idx):
+        """(label, extinction, ghost) of apply_labels at idx on a sample of density 0.5."""
+        ext, ghost = c_float(), c_int()
+        lab = load().oracle_apply_labels(byref(self.c), byref(u), _f3(idx), byref(ext), byref(ghost))
+        return int(lab), ext.value, bool(ghost.value)
+
+    def tf_segment(self, u, front, back, precision=64) -> np.ndarray:
+        out = (c_float * 4)()
+        load().oracle_tf_segment(byref(self.c), byref(u), front, back, precision, out)
+        return np.array(out, np.float64)
+
+    def preint_half_depth(self, u, front, back, precision=64) -> float:
+        return float(load().oracle_preint_half_depth(byref(self.c), byref(u), front, back, precision))
+
+    def surface_frame(self, u, idx, v, label=0, ghost=False, cap_plane=-1, cut=0.0, cap_depth=0.0, precision=64):
+        """(normal, surfaceness, extinction scale) of surface_frame."""
+        n, sf = (c_float * 3)(), c_float()
+        scale = load().oracle_surface_frame(byref(self.c), byref(u), _f3(idx), label, int(ghost), _f3(v), cap_plane,
+                                            cut, cap_depth, precision, n, byref(sf))
+        return np.array(n, np.float64), sf.value, float(scale)
+
     def shadow_dims(self):
         return tuple((n + 1) // 2 for n in self.dims)
 
@@ -426,3 +486,100 @@ def f32_to_f16(f: float) -> int:
 
 def f16_to_f32(h: int) -> float:
     return float(load().oracle_f16_to_f32(h))
+
+
+# ---- building blocks (shading, labels, clipping, sampling) ---------------------------------
+
+def _f3(a):
+    return (c_float * 3)(*[float(x) for x in a])
+
+
+def brdf(n, v, l, albedo, roughness, metalness=0.0, coat=0.0, precision=64):
+    """brdf_coated (brdf when coat = 0): (specular rgb, diffuse rgb)."""
+    sp, df = (c_float * 3)(), (c_float * 3)()
+    load().oracle_brdf(_f3(n), _f3(v), _f3(l), _f3(albedo), roughness, metalness, coat, precision, sp, df)
+    return np.array(sp, np.float64), np.array(df, np.float64)
+
+
+def smith(nv, nl, a) -> float:
+    return float(load().oracle_smith(nv, nl, a))
+
+
+def bsdf_pdf(n, v, l, roughness, metalness=0.0, coat=0.0) -> float:
+    return float(load().oracle_bsdf_pdf(_f3(n), _f3(v), _f3(l), roughness, metalness, coat))
+
+
+def bsdf_sample(n, v, roughness, metalness, coat, seed, count) -> np.ndarray:
+    out = np.zeros((count, 3), np.float32)
+    load().oracle_bsdf_sample(_f3(n), _f3(v), roughness, metalness, coat, seed, count, _p(out, c_float))
+    return out
+
+
+def sample_light(u, direction, seed, count) -> np.ndarray:
+    out = np.zeros((count, 3), np.float32)
+    load().oracle_sample_light(byref(u), _f3(direction), seed, count, _p(out, c_float))
+    return out
+
+
+def mis(a, b) -> float:
+    return float(load().oracle_mis(a, b))
+
+
+def hash3(c) -> np.ndarray:
+    out = (c_float * 3)()
+    load().oracle_hash3((c_int32 * 3)(*[int(x) for x in c]), out)
+    return np.array(out, np.float32)
+
+
+def gradient_noise(p, precision=32) -> float:
+    return float(load().oracle_gradient_noise(_f3(p), precision))
+
+
+def fbm(p, precision=32) -> float:
+    return float(load().oracle_fbm(_f3(p), precision))
+
+
+def cellular(p, precision=32):
+    nearest, border, cell = c_float(), c_float(), (c_int32 * 3)()
+    load().oracle_cellular(_f3(p), precision, byref(nearest), byref(border), cell)
+    return nearest.value, border.value, tuple(cell)
+
+
+def surface_height(p, params, precision=32) -> float:
+    return float(load().oracle_surface_height(_f3(p), (c_float * 4)(*params), precision))
+
+
+def index_to_world(u, idx) -> np.ndarray:
+    out = (c_float * 3)()
+    load().oracle_index_to_world(byref(u), _f3(idx), out)
+    return np.array(out, np.float64)
+
+
+def surface_texture(u, label, idx, albedo, roughness, coat):
+    a, r, c = (c_float * 3)(), c_float(), c_float()
+    load().oracle_surface_texture(byref(u), label, _f3(idx), _f3(albedo), roughness, coat, a, byref(r), byref(c))
+    return np.array(a, np.float64), r.value, c.value
+
+
+def bump_normal(u, label, idx, n) -> np.ndarray:
+    out = (c_float * 3)()
+    load().oracle_bump_normal(byref(u), label, _f3(idx), _f3(n), out)
+    return np.array(out, np.float64)
+
+
+def clip_depth(u, idx):
+    plane = c_int()
+    d = load().oracle_clip_depth(byref(u), _f3(idx), byref(plane))
+    return float(d), plane.value
+
+
+def pixel_stream(u, px, py, n) -> np.ndarray:
+    out = np.zeros(n, np.uint32)
+    load().oracle_pixel_stream(byref(u), px, py, n, _p(out, c_uint32))
+    return out
+
+
+def tissue_sigma(u, label, extinction) -> np.ndarray:
+    out = (c_float * 3)()
+    load().oracle_tissue_sigma(byref(u), label, extinction, out)
+    return np.array(out, np.float64)

@@ -134,6 +134,57 @@ int oracle_free_flight(const oracle_scene* s, const vvr_uniforms* u, const float
 void oracle_transmittance(const oracle_scene* s, const vvr_uniforms* u, const float origin_idx[3], const float dir[3],
                           uint32_t seed, float out[3]);
 
+/* Shading library (shading.glsl): brdf_coated's specular and diffuse parts for a
+ * surface of these materials (brdf alone when coat = 0); smith; the path
+ * tracer's lobe pdf (bsdf_pdf) and lobe sampling (pt.comp:491-502), n samples from
+ * a PCG stream seeded `seed`; sample_light in the frame of `dir`; mis. */
+void oracle_brdf(const float n[3], const float v[3], const float l[3], const float albedo[3], float roughness,
+                 float metalness, float coat, int precision, float spec[3], float diffuse[3]);
+double oracle_smith(double nv, double nl, double a);
+float oracle_bsdf_pdf(const float n[3], const float v[3], const float l[3], float roughness, float metalness,
+                      float coat);
+void oracle_bsdf_sample(const float n[3], const float v[3], float roughness, float metalness, float coat,
+                        uint32_t seed, uint32_t count, float* dirs);
+void oracle_sample_light(const vvr_uniforms* u, const float dir[3], uint32_t seed, uint32_t count, float* dirs);
+float oracle_mis(float a, float b);
+/* Procedural texture (shading.glsl:175-342): pcg3d hash, gradient noise, fbm,
+ * cellular noise, relief height; the texture's albedo / roughness / coat
+ * change and the bump-mapped normal of label `label` at index idx. */
+void oracle_hash3(const int32_t c[3], float out[3]);
+float oracle_gradient_noise(const float p[3], int precision);
+float oracle_fbm(const float p[3], int precision);
+void oracle_cellular(const float p[3], int precision, float* nearest, float* border, int32_t cell[3]);
+float oracle_surface_height(const float p[3], const float params[4], int precision);
+void oracle_index_to_world(const vvr_uniforms* u, const float idx[3], float out[3]);
+void oracle_surface_texture(const vvr_uniforms* u, uint32_t label, const float idx[3], const float albedo[3],
+                            float roughness, float coat, float out_albedo[3], float* out_roughness, float* out_coat);
+void oracle_bump_normal(const vvr_uniforms* u, uint32_t label, const float idx[3], const float n[3], float out[3]);
+/* Labels (common.glsl:232-351): the 21-candidate search (1 and the label when
+ * found), with_label of a transfer-function sample of `density` (albedo out,
+ * extinction returned), apply_labels at idx (label returned). */
+int oracle_drawn_label_near(const oracle_scene* s, const vvr_uniforms* u, const float idx[3], int ghost,
+                            uint32_t* label);
+float oracle_with_label(const oracle_scene* s, const vvr_uniforms* u, float density, uint32_t label, int ghost,
+                        float albedo[3]);
+int oracle_apply_labels(const oracle_scene* s, const vvr_uniforms* u, const float idx[3], float* extinction,
+                        int* ghost);
+/* clip_depth (common.glsl:357-373): mm inside the cut, the bounding plane. */
+float oracle_clip_depth(const vvr_uniforms* u, const float idx[3], int* plane);
+/* Pre-integration (common.glsl:195-230): tf_segment's albedo and extinction,
+ * preint_half_depth. */
+void oracle_tf_segment(const oracle_scene* s, const vvr_uniforms* u, float front, float back, int precision,
+                       float out[4]);
+float oracle_preint_half_depth(const oracle_scene* s, const vvr_uniforms* u, float front, float back, int precision);
+/* surface_frame (shading.glsl:73-144) of a sample of `label` (ghost or not) seen
+ * from v: normal, surfaceness, and the returned extinction scale. */
+float oracle_surface_frame(const oracle_scene* s, const vvr_uniforms* u, const float idx[3], uint32_t label,
+                           int ghost, const float v[3], int cap_plane, float cut, float cap_depth, int precision,
+                           float n[3], float* surfaceness);
+/* tissue_sigma (shading.glsl:161-168) of a sample of `label` with `extinction`. */
+void oracle_tissue_sigma(const vvr_uniforms* u, uint32_t label, float extinction, float out[3]);
+/* The PCG stream of pixel (px, py) after pt.comp's seeding and warm-up draw. */
+void oracle_pixel_stream(const vvr_uniforms* u, uint32_t px, uint32_t py, uint32_t n, uint32_t* out);
+
 #ifdef __cplusplus
 }
 #endif

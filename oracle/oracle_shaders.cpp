@@ -832,6 +832,7 @@ template <class R> struct Ctx {
         return (word >> 22u) ^ word;
     }
     R rand() { return R(pcg() >> 8) * R(1.0 / 16777216.0); }
+    static R rng_real() { return R(0); }  // the real type, for generic callers
 
     // ---- light map (common.glsl:459-485, pt.comp:193-226) --------------------------------
     V3<R> basis_f() const { return normalize(V3<R>(u->forward[0], u->forward[1], u->forward[2])); }
@@ -1007,6 +1008,16 @@ template <class R> struct Ctx {
         V3<R> h = frame.mul(V3<R>(sin_t * cr_cos(phi), sin_t * cr_sin(phi), cos_t));
         return reflect(-v, h);
     }
+    // The surface event's next direction (pt.comp:491-502): clear coat, GGX or
+    // cosine lobe, chosen in that order with their shares.
+    V3<R> sample_bsdf(V3<R> n, V3<R> v, const Sample& s_) {
+        M3<R> frame = basis(n);
+        if (rand() < coat_share(s_)) return sample_ggx(frame, v, coat_roughness(s_) * coat_roughness(s_));
+        if (rand() < specular_share(s_)) return sample_ggx(frame, v, gmax(s_.roughness * s_.roughness, R(2e-3)));
+        R r = std::sqrt(rand());
+        R phi = R(2) * PI() * rand();
+        return frame.mul(V3<R>(r * cr_cos(phi), r * cr_sin(phi), std::sqrt(gmax(R(0), R(1) - r * r))));
+    }
     static R bsdf_pdf(V3<R> n, V3<R> v, V3<R> l, const Sample& s_) {
         R nl = dot(n, l);
         if (nl <= R(0)) return R(0);
@@ -1092,9 +1103,12 @@ template <class R> struct Ctx {
         return result;
     }
 
-    void pt_pixel(uint32_t px, uint32_t py, float acc_out[4], float alb_out[4], float nrm_out[4]) {
+    void seed_pixel(uint32_t px, uint32_t py) {  // pt.comp:378-379
         rng = (px * 1973u + py * 9277u + u->path[0] * 26699u) | 1u;
         pcg();
+    }
+    void pt_pixel(uint32_t px, uint32_t py, float acc_out[4], float alb_out[4], float nrm_out[4]) {
+        seed_pixel(px, py);
         uint32_t samples = u->path[2];
         uint32_t max_bounces = u->path[1];
         uint32_t volume_limit = (uint32_t)(float(32u) / (1.0f - clampr(u->env[2], 0.0f, 0.95f)));
@@ -1192,17 +1206,7 @@ template <class R> struct Ctx {
                         radiance = radiance + path_weight * (diffuse + specular) * dot(n, map_dir) * map_light *
                                                   mis(map_pdf, bsdf_pdf(n, v, map_dir, s_));
                     }
-                    M3<R> frame = basis(n);
-                    V3<R> l;
-                    if (rand() < coat_share(s_)) {
-                        l = sample_ggx(frame, v, coat_roughness(s_) * coat_roughness(s_));
-                    } else if (rand() < specular_share(s_)) {
-                        l = sample_ggx(frame, v, gmax(s_.roughness * s_.roughness, R(2e-3)));
-                    } else {
-                        R r = std::sqrt(rand());
-                        R phi = R(2) * PI() * rand();
-                        l = frame.mul(V3<R>(r * cr_cos(phi), r * cr_sin(phi), std::sqrt(gmax(R(0), R(1) - r * r))));
-                    }
+                    V3<R> l = sample_bsdf(n, v, s_);
                     R nl = dot(n, l);
                     if (nl <= R(0)) break;
                     R pdf = bsdf_pdf(n, v, l, s_);
@@ -2055,6 +2059,198 @@ void oracle_transmittance(const oracle_scene* s, const vvr_uniforms* u, const fl
     c.rng = seed;
     V3<float> r = c.transmittance(V3<float>(origin_idx[0], origin_idx[1], origin_idx[2]), V3<float>(dir[0], dir[1], dir[2]));
     out[0] = r.x, out[1] = r.y, out[2] = r.z;
+}
+
+// ---- building blocks of the shading, label, clipping and sampling code -------------
+}  // extern "C"
+
+namespace {
+template <class R> V3<R> v3(const float* a) { return V3<R>(R(a[0]), R(a[1]), R(a[2])); }
+template <class R> void put3(V3<R> v, float* out) { out[0] = float(v.x), out[1] = float(v.y), out[2] = float(v.z); }
+template <class R> typename Ctx<R>::Sample material(const float albedo[3], float roughness, float metalness, float coat) {
+    typename Ctx<R>::Sample s{};
+    s.albedo = v3<R>(albedo);
+    s.roughness = R(roughness), s.metalness = R(metalness), s.coat = R(coat);
+    return s;
+}
+template <class R>
+void brdf_t(const float n[3], const float v[3], const float l[3], const float albedo[3], float roughness,
+            float metalness, float coat, float spec[3], float diffuse[3]) {
+    V3<R> d;
+    V3<R> sp = Ctx<R>::brdf_coated(v3<R>(n), v3<R>(v), v3<R>(l), material<R>(albedo, roughness, metalness, coat), d);
+    put3(sp, spec), put3(d, diffuse);
+}
+}  // namespace
+
+extern "C" {
+
+void oracle_brdf(const float n[3], const float v[3], const float l[3], const float albedo[3], float roughness,
+                 float metalness, float coat, int precision, float spec[3], float diffuse[3]) {
+    if (precision == 64) brdf_t<double>(n, v, l, albedo, roughness, metalness, coat, spec, diffuse);
+    else brdf_t<float>(n, v, l, albedo, roughness, metalness, coat, spec, diffuse);
+}
+
+double oracle_smith(double nv, double nl, double a) { return Ctx<double>::smith(nv, nl, a); }
+
+float oracle_bsdf_pdf(const float n[3], const float v[3], const float l[3], float roughness, float metalness,
+                      float coat) {
+    const float alb[3] = {1, 1, 1};
+    return float(Ctx<double>::bsdf_pdf(v3<double>(n), v3<double>(v), v3<double>(l),
+                                       material<double>(alb, roughness, metalness, coat)));
+}
+
+void oracle_bsdf_sample(const float n[3], const float v[3], float roughness, float metalness, float coat,
+                        uint32_t seed, uint32_t count, float* dirs) {
+    oracle_scene sc{};
+    vvr_uniforms un{};
+    Ctx<double> c(&sc, &un);
+    c.rng = seed;
+    const float alb[3] = {1, 1, 1};
+    auto s_ = material<double>(alb, roughness, metalness, coat);
+    for (uint32_t i = 0; i < count; i++) put3(c.sample_bsdf(v3<double>(n), v3<double>(v), s_), dirs + 3 * i);
+}
+
+void oracle_sample_light(const vvr_uniforms* u, const float dir[3], uint32_t seed, uint32_t count, float* dirs) {
+    oracle_scene sc{};
+    Ctx<double> c(&sc, u);
+    c.rng = seed;
+    auto frame = Ctx<double>::basis(v3<double>(dir));
+    for (uint32_t i = 0; i < count; i++) put3(c.sample_light(frame), dirs + 3 * i);
+}
+
+float oracle_mis(float a, float b) { return float(Ctx<double>::mis(a, b)); }
+
+void oracle_hash3(const int32_t c[3], float out[3]) {
+    put3(Ctx<float>::hash3(V3<int>(c[0], c[1], c[2])), out);
+}
+
+float oracle_gradient_noise(const float p[3], int precision) {
+    return precision == 64 ? float(Ctx<double>::gradient_noise(v3<double>(p))) : Ctx<float>::gradient_noise(v3<float>(p));
+}
+
+float oracle_fbm(const float p[3], int precision) {
+    return precision == 64 ? float(Ctx<double>::fbm(v3<double>(p))) : Ctx<float>::fbm(v3<float>(p));
+}
+
+void oracle_cellular(const float p[3], int precision, float* nearest, float* border, int32_t cell[3]) {
+    V3<int> c;
+    if (precision == 64) {
+        double nd, bd;
+        Ctx<double>::cellular(v3<double>(p), nd, bd, c);
+        *nearest = float(nd), *border = float(bd);
+    } else {
+        Ctx<float>::cellular(v3<float>(p), *nearest, *border, c);
+    }
+    cell[0] = c.x, cell[1] = c.y, cell[2] = c.z;
+}
+
+float oracle_surface_height(const float p[3], const float params[4], int precision) {
+    return precision == 64 ? float(Ctx<double>::surface_height(v3<double>(p), params))
+                           : Ctx<float>::surface_height(v3<float>(p), params);
+}
+
+void oracle_index_to_world(const vvr_uniforms* u, const float idx[3], float out[3]) {
+    oracle_scene sc{};
+    Ctx<double> c(&sc, u);
+    put3(c.index_to_world(v3<double>(idx)), out);
+}
+
+void oracle_surface_texture(const vvr_uniforms* u, uint32_t label, const float idx[3], const float albedo[3],
+                            float roughness, float coat, float out_albedo[3], float* out_roughness, float* out_coat) {
+    oracle_scene sc{};
+    Ctx<double> c(&sc, u);
+    auto s_ = material<double>(albedo, roughness, 0.0f, coat);
+    s_.label = label;
+    c.apply_surface_texture(v3<double>(idx), s_);
+    put3(s_.albedo, out_albedo);
+    *out_roughness = float(s_.roughness), *out_coat = float(s_.coat);
+}
+
+void oracle_bump_normal(const vvr_uniforms* u, uint32_t label, const float idx[3], const float n[3], float out[3]) {
+    oracle_scene sc{};
+    Ctx<double> c(&sc, u);
+    const float alb[3] = {1, 1, 1};
+    auto s_ = material<double>(alb, 0.5f, 0.0f, 0.0f);
+    s_.label = label;
+    put3(c.bump_normal(v3<double>(idx), s_, v3<double>(n)), out);
+}
+
+int oracle_drawn_label_near(const oracle_scene* s, const vvr_uniforms* u, const float idx[3], int ghost,
+                            uint32_t* label) {
+    Ctx<float> c(s, u);
+    return c.drawn_label_near(v3<float>(idx), uint32_t(u->labels[0]), ghost != 0, *label) ? 1 : 0;
+}
+
+float oracle_with_label(const oracle_scene* s, const vvr_uniforms* u, float density, uint32_t label, int ghost,
+                        float albedo[3]) {
+    Ctx<double> c(s, u);
+    auto s_ = c.with_label(c.tf_point(density), label, uint32_t(u->labels[0]), ghost != 0);
+    put3(s_.albedo, albedo);
+    return float(s_.extinction);
+}
+
+int oracle_apply_labels(const oracle_scene* s, const vvr_uniforms* u, const float idx[3], float* extinction,
+                        int* ghost) {
+    Ctx<float> c(s, u);
+    auto s_ = c.apply_labels(c.tf_point(0.5f), v3<float>(idx));
+    *extinction = s_.extinction;
+    *ghost = s_.ghost ? 1 : 0;
+    return int(s_.label);
+}
+
+float oracle_clip_depth(const vvr_uniforms* u, const float idx[3], int* plane) {
+    oracle_scene sc{};
+    Ctx<double> c(&sc, u);
+    return float(c.clip_depth(v3<double>(idx), *plane));
+}
+
+void oracle_tf_segment(const oracle_scene* s, const vvr_uniforms* u, float front, float back, int precision,
+                       float out[4]) {
+    if (precision == 64) {
+        Ctx<double> c(s, u);
+        auto t = c.tf_segment(front, back);
+        put3(t.albedo, out), out[3] = float(t.extinction);
+    } else {
+        Ctx<float> c(s, u);
+        auto t = c.tf_segment(front, back);
+        put3(t.albedo, out), out[3] = t.extinction;
+    }
+}
+
+float oracle_preint_half_depth(const oracle_scene* s, const vvr_uniforms* u, float front, float back, int precision) {
+    if (precision == 64) return float(Ctx<double>(s, u).preint_half_depth(front, back));
+    return Ctx<float>(s, u).preint_half_depth(front, back);
+}
+
+float oracle_surface_frame(const oracle_scene* s, const vvr_uniforms* u, const float idx[3], uint32_t label,
+                           int ghost, const float v[3], int cap_plane, float cut, float cap_depth, int precision,
+                           float n[3], float* surfaceness) {
+    auto run = [&](auto c) {
+        using R = decltype(c.rng_real());
+        typename decltype(c)::Sample s_{};
+        s_.label = label, s_.ghost = ghost != 0;
+        V3<R> nn;
+        R sf;
+        R scale = c.surface_frame(v3<R>(idx), s_, v3<R>(v), cap_plane, R(cut), R(cap_depth), nn, sf);
+        put3(nn, n), *surfaceness = float(sf);
+        return float(scale);
+    };
+    return precision == 64 ? run(Ctx<double>(s, u)) : run(Ctx<float>(s, u));
+}
+
+void oracle_tissue_sigma(const vvr_uniforms* u, uint32_t label, float extinction, float out[3]) {
+    oracle_scene sc{};
+    Ctx<double> c(&sc, u);
+    Ctx<double>::Sample s_{};
+    s_.label = label, s_.extinction = extinction;
+    put3(c.tissue_sigma(s_), out);
+}
+
+void oracle_pixel_stream(const vvr_uniforms* u, uint32_t px, uint32_t py, uint32_t n, uint32_t* out) {
+    oracle_scene sc{};
+    Ctx<float> c(&sc, u);
+    c.seed_pixel(px, py);
+    for (uint32_t i = 0; i < n; i++) out[i] = c.pcg();
 }
 
 }  // extern "C"

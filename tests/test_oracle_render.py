@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 from helpers import all_pixels, flat_tf, frame_params, oracle_scene, uniforms
-from volview_amd.renderer import TransferFunction
+from volview_amd.renderer import TransferFunction, make_volume
 
 
 def test_dvr_empty_volume(oracle):
@@ -143,3 +143,99 @@ def test_single_scattering_albedo_map(oracle):
     a = np.linspace(0, 0.999, 50)
     s = np.array([oracle.single_scattering_albedo((v, v, v))[0] for v in a])
     assert np.all(np.diff(s) > 0) and np.all(s >= a - 1e-6)  # multiple scattering darkens
+
+
+def primary_transmittance(u, sigma_rgb, width, height, sub=8):
+    """Per-pixel mean of exp(-sigma_c d) over the pixel's area, d the primary
+    ray's chord (mm) through the volume box [-0.5, dims - 0.5] (the camera of
+    common.glsl:439-442 and the slab test of common.glsl:408-417)."""
+    w2i = np.array(u.world_to_index, np.float64).reshape(4, 4).T
+    eye = np.array(u.eye[:3], np.float64)
+    f, r, up = (np.array(v[:3], np.float64) for v in (u.forward, u.right, u.up))
+    o = w2i[:3, :3] @ eye + w2i[:3, 3]
+    lo, hi = -0.5, np.array(u.dims[:3], np.float64) - 0.5
+    out = np.zeros((height * width, 3))
+    j = (np.arange(sub) + 0.5) / sub
+    for py in range(height):
+        for px in range(width):
+            acc = np.zeros(3)
+            for jy in j:
+                for jx in j:
+                    nx, ny = (px + jx) / width * 2 - 1, (py + jy) / height * 2 - 1
+                    d = f + nx * r - ny * up
+                    d /= np.linalg.norm(d)
+                    di = w2i[:3, :3] @ d
+                    with np.errstate(divide="ignore"):
+                        t0, t1 = (lo - o) / di, (hi - o) / di
+                    tn = max(np.minimum(t0, t1).max(), 0.0)
+                    tf_ = np.maximum(t0, t1).min()
+                    acc += np.exp(-np.asarray(sigma_rgb) * max(tf_ - tn, 0.0))
+            out[py * width + px] = acc / sub ** 2
+    return out
+
+
+@pytest.mark.parametrize("g,chromatic", [(0.0, 0.0), (0.8, 0.0), (0.8, 1.0)])
+def test_pt_white_furnace(oracle, g, chromatic):
+    """White furnace (pt.comp:373-561): a white, purely scattering medium lit only
+    by a uniform light map. Each channel c of a primary ray reaches the medium
+    with probability 1 - T_c (T_c = exp(-sigma_c d) along the ray), and from
+    there all of the map's radiance L comes back whatever the scattering: the
+    expected radiance per path is L (1 - T_c). That holds only if next-event
+    estimation of the map and the escaped paths split the light by the power
+    heuristic without double counting, the map's pdf matches its sampling, the
+    phase function matches its sampling, and the spectral (hero-channel)
+    tracking weights are unbiased: with chromatic 1 the extinction differs per
+    channel (tissue chroma). Without chroma, radiance + L background = L per
+    path as well. Key light off; no surfaces (a constant volume has no
+    gradient). FP64 oracle, 16 x 16 pixels x 512 paths."""
+    hu = np.full((24, 24, 24), 200, np.int16)
+    sigma = 0.12
+    tf = flat_tf(sigma, albedo=(1.0, 1.0, 1.0))
+    p = frame_params(hu, path_tracing=True, spp=512, bounces=8, g=g)
+    p.lighting.intensity = 0.0
+    p.lighting.environment = 2  # the light map
+    p.lighting.chromatic = chromatic
+    p.lighting.subsurface = 0.4
+    rad, table, dif, peak = oracle.env_prepare(np.ones((8, 16, 3), np.float32))
+    u = oracle.pack_uniforms(make_volume(hu), p, oracle.bucket_max(tf), False, 0, 16, 16, 1, env_peak_u=peak)
+    sc = oracle_scene(oracle, hu, tf, u)
+    sc.set_env(rad, table, dif)
+    acc, _, _ = sc.pt(u, all_pixels(16, 16), 64)
+    L = np.array(u.ambient[:3], np.float64) * oracle.f16_to_f32(int(rad[0, 0, 0]))
+    # pt.comp:132-139: chroma = mix(1, (0.62, 0.77, 1), clamp(2 subsurface, 0, 1) chromatic).
+    k = min(max(2 * u.subsurface[0], 0.0), 1.0) * u.env[3]
+    chroma = 1 + k * (np.array([0.62, 0.77, 1.0]) - 1)
+    T = primary_transmittance(u, sigma * chroma, 16, 16)
+    per_pixel = acc[:, :3] / 512.0
+    expected = L * (1 - T)
+    mean, want = per_pixel.mean(0), expected.mean(0)
+    se = (per_pixel - expected).std(0) / np.sqrt(len(per_pixel))  # the paths' noise about each pixel's expectation
+    print("furnace", mean / want, "se", se / want, "background", acc[:, 3].mean() / 512, "T", T.mean(0))
+    assert 0.05 < acc[:, 3].mean() / 512 < 0.95  # both escaping and scattered primary rays
+    assert np.all(np.abs(mean - want) < 5 * se) and np.all(np.abs(mean / want - 1) < 0.01)
+    if chromatic == 0.0:
+        total = (acc[:, :3] + acc[:, 3:4] * L) / 512.0
+        assert np.all(np.abs(total.mean(0) / L - 1) < 5 * total.std(0) / np.sqrt(len(total)) / L + 1e-9)
+
+
+def test_dvr_brick_skipping_only_moves_samples(oracle):
+    """Empty-space skipping (dvr.comp:47-57) only jumps over bricks whose bound is
+    0: on a real phantom (the DVR-1 recipe at 96^3) the frame with skipping on
+    equals the frame with it off up to the shift of the sample positions after a
+    jump, a fraction of the 0.5-voxel step (measured 2.4 % relative RMSE, from
+    thin bone edges that the shifted samples hit or miss)."""
+    from volview_amd import phantom
+    hu = phantom.ct((96, 96, 96), 1)
+    tf = TransferFunction.preset(0)
+    imgs = []
+    for skip in (1, 0):
+        p = frame_params(hu, yaw=0.6, pitch=0.3)
+        p.skip_empty_space = skip
+        u = uniforms(oracle, hu, tf, p, width=48, height=48)
+        sc = oracle_scene(oracle, hu, tf, u)
+        imgs.append(sc.dvr(u, all_pixels(48, 48)).astype(np.float64))
+    on, off = imgs
+    rel = np.sqrt(np.mean((on[:, :3] - off[:, :3]) ** 2)) / np.sqrt(np.mean(off[:, :3] ** 2))
+    assert rel < 0.04 and np.abs(on[:, 3] - off[:, 3]).mean() < 5e-3
+    assert abs((on[:, 3] == 1.0).mean() - (off[:, 3] == 1.0).mean()) < 0.002
+    assert (off[:, 3] < 0.5).mean() > 0.05

@@ -1,0 +1,678 @@
+"""Pins of the oracle's shading library, label search, clipping, pre-integration,
+procedural texture and path-sampling building blocks (shading.glsl, common.glsl,
+pt.comp), each by a check that a plausible mistake (a dropped term, a wrong
+sign, constant or index, a transposed operand) fails: closed forms at special
+configurations, symmetries, integrals over the sphere, brute force over small
+inputs, and independent implementations of the published algorithms the
+reference names (PCG-RXS-M-XS and pcg3d, Jarzynski and Olano 2020).
+"""
+import numpy as np
+import pytest
+
+from helpers import flat_tf, frame_params, oracle_scene
+from volview_amd import _abi
+from volview_amd.renderer import TransferFunction, make_volume
+
+U32 = 0xFFFFFFFF
+
+
+def unit(v):
+    v = np.asarray(v, np.float64)
+    return v / np.linalg.norm(v)
+
+
+def hemisphere(rng, n, count):
+    """Uniform directions on the hemisphere around n."""
+    d = rng.normal(size=(count, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d[d @ n < 0] *= -1
+    return d
+
+
+# ---- BRDF (shading.glsl:4-58) -------------------------------------------------------------
+
+@pytest.mark.parametrize("roughness", [0.02, 0.3, 0.7, 1.0])
+@pytest.mark.parametrize("metalness", [0.0, 0.4, 1.0])
+def test_brdf_at_normal_incidence(oracle, roughness, metalness):
+    """v = l = n: GGX D = 1 / (pi a^2), Smith G = 1, Schlick F = F0 =
+    mix(0.028, albedo, metalness): specular F0 / (4 pi a^2), a = max(r^2, 2e-3);
+    diffuse (1 - F0)(1 - metalness) albedo / pi."""
+    albedo = np.array([0.9, 0.5, 0.2])
+    n = [0.0, 0.0, 1.0]
+    spec, diff = oracle.brdf(n, n, n, albedo, roughness, metalness)
+    a = max(roughness ** 2, 2e-3)
+    f0 = 0.028 * (1 - metalness) + albedo * metalness
+    assert np.allclose(spec, f0 / (4 * np.pi * a * a), rtol=1e-5)
+    assert np.allclose(diff, (1 - f0) * (1 - metalness) * albedo / np.pi, rtol=1e-5, atol=1e-9)
+
+
+def test_brdf_reciprocity_and_schlick(oracle):
+    """f(v, l) = f(l, v) for both parts, and the diffuse part is
+    (1 - F(v.h)) albedo / pi with Schlick's F = F0 + (1 - F0)(1 - v.h)^5, which
+    goes to 1 at grazing half-vectors."""
+    rng = np.random.default_rng(4)
+    n = unit([0.2, -0.3, 0.9])
+    albedo = np.array([0.7, 0.6, 0.4])
+    for v, l in zip(hemisphere(rng, n, 60), hemisphere(rng, n, 60)):
+        v, l = (x.astype(np.float32).astype(np.float64) for x in (v, l))  # the oracle's inputs
+        if v @ n < 0.05 or l @ n < 0.05:
+            continue
+        s1, d1 = oracle.brdf(n, v, l, albedo, 0.45, 0.0)
+        s2, d2 = oracle.brdf(n, l, v, albedo, 0.45, 0.0)
+        assert np.allclose(s1, s2, rtol=1e-5) and np.allclose(d1, d2, rtol=1e-5)
+        vh = max(v @ unit(v + l), 0.0)
+        f = 0.028 + 0.972 * (1 - vh) ** 5
+        assert np.allclose(d1, (1 - f) * albedo / np.pi, rtol=1e-5)
+    # v and l nearly opposite and grazing: v.h -> 0, F -> 1, almost no diffuse.
+    v = unit([1.0, 0.0, 1e-3]).astype(np.float32).astype(np.float64)
+    l = unit([-1.0, 1e-3, 1e-3]).astype(np.float32).astype(np.float64)
+    _, d = oracle.brdf([0, 0, 1], v, l, albedo, 0.5, 0.0)
+    vh = v @ unit(v + l)
+    assert vh < 0.005
+    assert np.allclose(d, (1 - (0.028 + 0.972 * (1 - vh) ** 5)) * albedo / np.pi, rtol=1e-3)
+    assert np.all(d < 0.01 * albedo)
+
+
+@pytest.mark.parametrize("coat", [0.0, 0.8])
+@pytest.mark.parametrize("metalness", [0.0, 1.0])
+def test_brdf_white_furnace_bound(oracle, coat, metalness):
+    """White furnace of the surface: the directional albedo of a white surface,
+    the integral of (specular + diffuse) n.l over the hemisphere (importance
+    sampled with the path tracer's own lobe sampling and pdf), stays below 1 for
+    views within 40 degrees of the normal at every roughness, and is > 0.95 for
+    a rough white dielectric seen face-on. (Lambert x (1 - F) plus GGX with
+    Schlick-Smith G is not energy conserving at grazing views of smooth
+    surfaces: there the reference's model reflects more than it receives.)"""
+    n = np.array([0.0, 0.0, 1.0])
+    white = np.ones(3)
+    for rough in (0.15, 0.5, 1.0):
+        for theta in (0.0, 0.7):
+            v = np.array([np.sin(theta), 0.0, np.cos(theta)])
+            ls = oracle.bsdf_sample(n, v, rough, metalness, coat, 7, 6000).astype(np.float64)
+            tot = np.zeros(3)
+            for l in ls[ls @ n > 0]:
+                s, d = oracle.brdf(n, v, l, white, rough, metalness, coat)
+                tot += (s + d) * (l @ n) / oracle.bsdf_pdf(n, v, l, rough, metalness, coat)
+            albedo = tot / len(ls)
+            assert np.all(albedo < 1.01), (rough, theta, albedo)
+            if theta == 0.0 and metalness == 0.0 and rough >= 0.5:
+                assert np.all(albedo > 0.95), (rough, albedo)
+
+
+def test_smith_closed_forms(oracle):
+    """Smith-Schlick G with k = a / 2: 1 at a -> 0, n.v n.l at a = 2 (k = 1),
+    (2 nv / (1 + nv)) (2 nl / (1 + nl)) at a = 1; symmetric in nv, nl."""
+    for nv, nl in ((0.3, 0.8), (1.0, 1.0), (0.05, 0.6)):
+        assert oracle.smith(nv, nl, 1e-12) == pytest.approx(1.0, abs=1e-9)
+        assert oracle.smith(nv, nl, 2.0) == pytest.approx(nv * nl, rel=1e-12)
+        assert oracle.smith(nv, nl, 1.0) == pytest.approx(4 * nv * nl / ((1 + nv) * (1 + nl)), rel=1e-12)
+        assert oracle.smith(nv, nl, 0.37) == oracle.smith(nl, nv, 0.37)
+
+
+@pytest.mark.parametrize("roughness", [0.1, 0.4, 0.9])
+def test_clear_coat_at_normal_incidence(oracle, roughness):
+    """Clear coat (shading.glsl:37-58) at v = l = n: Fresnel F0 = 0.02 times the
+    coat weight; the base layer is scaled by 1 - F; the coat lobe is
+    D(1) F / 4 with the coat roughness clamp(r / 2, 0.1, 0.25) squared."""
+    albedo = np.array([0.8, 0.4, 0.3])
+    n = [0.0, 0.0, 1.0]
+    coat = 0.7
+    base_s, base_d = oracle.brdf(n, n, n, albedo, roughness, 0.0, 0.0)
+    s, d = oracle.brdf(n, n, n, albedo, roughness, 0.0, coat)
+    f = coat * 0.02
+    ac = min(max(0.5 * roughness, 0.1), 0.25) ** 2
+    lobe = f / (4 * np.pi * ac * ac)
+    assert np.allclose(d, base_d * (1 - f), rtol=1e-5)
+    assert np.allclose(s, base_s * (1 - f) + lobe, rtol=1e-5)
+
+
+# ---- lobe sampling and pdfs (pt.comp:96-102, 188-273, 491-502) --------------------------
+
+@pytest.mark.parametrize("material", [(0.3, 0.0, 0.0), (0.6, 1.0, 0.0), (0.2, 0.0, 0.9), (0.9, 0.5, 0.5)])
+def test_bsdf_samples_follow_their_pdf(oracle, material):
+    """Directions drawn by the surface event's lobe sampling have bsdf_pdf as
+    their density: over the directions above the surface, E[f(l) / pdf(l)]
+    equals the integral of f for f = 1 (2 pi) and f = n.l (pi), and the pdf
+    integrates over the hemisphere to the share of samples that stay above it
+    (GGX reflections can fall below)."""
+    rough, metal, coat = material
+    n = unit([0.1, 0.2, 1.0])
+    v = unit([0.5, -0.2, 0.8])
+    ls = oracle.bsdf_sample(n, v, rough, metal, coat, 99, 60000).astype(np.float64)
+    above = ls @ n > 0
+    pdf = np.array([oracle.bsdf_pdf(n, v, l, rough, metal, coat) for l in ls[above]])
+    assert np.all(pdf > 0)
+    est1 = np.sum(1.0 / pdf) / len(ls)
+    est_cos = np.sum((ls[above] @ n) / pdf) / len(ls)
+    assert est_cos == pytest.approx(np.pi, rel=0.03)
+    assert est1 == pytest.approx(2 * np.pi, rel=0.06)
+    if coat == 0.0:  # (a clear coat's lobe is too sharp for a uniform estimate)
+        rng = np.random.default_rng(1)
+        hs = hemisphere(rng, n, 40000)
+        mass = np.mean([oracle.bsdf_pdf(n, v, l, rough, metal, coat) for l in hs]) * 2 * np.pi
+        assert mass == pytest.approx(above.mean(), abs=0.03)
+
+
+def test_light_samples_fill_the_cone(oracle):
+    """sample_light: uniform over the cone of half-angle light_size around the
+    light direction: every sample inside it, mean cosine (1 + cos max) / 2, and
+    azimuths uniform."""
+    u = _abi.Uniforms()
+    u.material[3] = 0.2
+    d = unit([0.3, -0.5, 0.8])
+    ls = oracle.sample_light(u, d, 5, 50000).astype(np.float64)
+    c = ls @ d
+    cos_max = np.cos(0.2)
+    assert np.all(c >= cos_max - 1e-6) and np.allclose(np.linalg.norm(ls, axis=1), 1, atol=1e-5)
+    assert c.mean() == pytest.approx((1 + cos_max) / 2, abs=2e-4)
+    perp = ls - np.outer(c, d)
+    ref = unit(np.cross(d, [1.0, 0.0, 0.0]))
+    ang = np.arctan2(perp @ np.cross(d, ref), perp @ ref)
+    hist, _ = np.histogram(ang, bins=8, range=(-np.pi, np.pi))
+    assert np.all(np.abs(hist / len(ls) - 1 / 8) < 0.01)
+
+
+def test_mis_power_heuristic(oracle):
+    """mis(a, b) = a^2 / (a^2 + b^2): the two strategies' weights sum to 1, one
+    strategy alone has weight 1."""
+    for a, b in ((0.3, 2.0), (5.0, 0.1), (1.0, 1.0)):
+        assert oracle.mis(a, b) + oracle.mis(b, a) == pytest.approx(1.0, abs=1e-6)
+        assert oracle.mis(a, b) == pytest.approx(a * a / (a * a + b * b), rel=1e-6)
+    assert oracle.mis(0.7, 0.0) == 1.0 and oracle.mis(0.0, 0.7) == 0.0
+
+
+# ---- PCG (pt.comp:41-53, 378-379) -------------------------------------------------------
+
+def pcg_rxs_m_xs(state, n):
+    """PCG-RXS-M-XS 32 (O'Neill 2014) with the LCG multiplier 747796405 and
+    increment 2891336453, as published for GPU hashing by Jarzynski and Olano
+    (2020): output of the state before each step."""
+    out = []
+    for _ in range(n):
+        word = (((state >> ((state >> 28) + 4)) ^ state) * 277803737) & U32
+        out.append(((word >> 22) ^ word) & U32)
+        state = (state * 747796405 + 2891336453) & U32
+    return np.array(out, np.uint32), state
+
+
+def test_pcg_known_answers(oracle):
+    for seed in (1, 0x9E3779B9, 12345, U32):
+        want, _ = pcg_rxs_m_xs(seed, 64)
+        assert np.array_equal(oracle.pcg_sequence(seed, 64), want)
+        r = oracle.rand_sequence(seed, 64)
+        assert np.array_equal(r, (want >> 8).astype(np.float32) / np.float32(16777216.0))
+
+
+def test_pixel_seed(oracle):
+    """Each pixel's stream starts at (x 1973 + y 9277 + accumulated 26699) | 1
+    with one draw discarded."""
+    for px, py, acc in ((0, 0, 0), (5, 9, 0), (1023, 511, 63), (17, 3, 1 << 20)):
+        u = _abi.Uniforms()
+        u.path[0] = acc
+        seed = ((px * 1973 + py * 9277 + acc * 26699) & U32) | 1
+        want, _ = pcg_rxs_m_xs(seed, 9)
+        assert np.array_equal(oracle.pixel_stream(u, px, py, 8), want[1:])
+
+
+# ---- procedural texture (shading.glsl:175-342) ------------------------------------------
+
+def pcg3d(c):
+    """pcg3d (Jarzynski and Olano 2020) on a signed integer cell, scaled to [0, 1)."""
+    v = [(int(x) & U32) * 1664525 + 1013904223 & U32 for x in c]
+    for _ in range(2):
+        v[0] = (v[0] + v[1] * v[2]) & U32
+        v[1] = (v[1] + v[2] * v[0]) & U32
+        v[2] = (v[2] + v[0] * v[1]) & U32
+        if _ == 0:
+            v = [x ^ (x >> 16) for x in v]
+    return np.array(v, np.float64) / 4294967296.0
+
+
+def test_hash3_is_pcg3d(oracle):
+    rng = np.random.default_rng(3)
+    for c in list(rng.integers(-5000, 5000, (200, 3))) + [(0, 0, 0), (-1, -1, -1), (7919, 7919, 7919)]:
+        assert np.array_equal(oracle.hash3(c), pcg3d(c).astype(np.float32)), c
+
+
+def test_gradient_noise_lattice_and_continuity(oracle):
+    """Perlin gradient noise is 0 at every lattice point, continuous across cell
+    faces (the quintic fade), and bounded by 1.7 sqrt(3) / 2 x ... in practice
+    well inside [-1.7, 1.7]."""
+    rng = np.random.default_rng(5)
+    for c in rng.integers(-40, 40, (50, 3)):
+        assert oracle.gradient_noise(c.astype(np.float64)) == 0.0
+    for _ in range(100):
+        p = rng.uniform(-20, 20, 3)
+        a = rng.integers(0, 3)
+        p[a] = np.floor(p[a]) + 1.0
+        lo, hi = p.copy(), p.copy()
+        lo[a] -= 1e-4
+        hi[a] += 1e-4
+        assert abs(oracle.gradient_noise(lo, 64) - oracle.gradient_noise(hi, 64)) < 2e-3
+    vals = [oracle.gradient_noise(rng.uniform(-50, 50, 3)) for _ in range(2000)]
+    assert max(abs(v) for v in vals) < 1.7 and np.std(vals) > 0.15
+    f = [oracle.fbm(rng.uniform(-50, 50, 3)) for _ in range(500)]
+    assert max(abs(v) for v in f) < 1.7
+
+
+def test_cellular_is_the_nearest_feature_point(oracle):
+    """Worley noise with one feature point per unit cell at cell + pcg3d(cell):
+    brute force over the 27 neighbouring cells gives the nearest distance and
+    its cell; the border value is the distance from p to the bisecting plane of
+    the two nearest feature points, >= 0."""
+    rng = np.random.default_rng(11)
+    for _ in range(150):
+        p = rng.uniform(-30, 30, 3)
+        base = np.floor(p).astype(int)
+        pts, cells = [], []
+        for dz in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    c = base + (dx, dy, dz)
+                    pts.append(c + pcg3d(c))
+                    cells.append(tuple(c))
+        d = np.linalg.norm(np.array(pts) - p, axis=1)
+        order = np.argsort(d)
+        nearest, border, cell = oracle.cellular(p, 64)
+        assert nearest == pytest.approx(d[order[0]], abs=1e-5)
+        assert cell == cells[order[0]]
+        p1, p2 = np.array(pts)[order[0]], np.array(pts)[order[1]]
+        plane = (np.dot(p - (p1 + p2) / 2, unit(p2 - p1)))
+        assert border == pytest.approx(-plane, abs=1e-4) and border >= -1e-6
+
+
+def texture_uniforms(pattern, amount, relief, size):
+    u = _abi.Uniforms()
+    m = np.eye(4, dtype=np.float32)
+    m[3, :3] = (3.0, -2.0, 5.0)  # world_to_index: idx = world + t (column-major translation)
+    u.world_to_index[:] = tuple(m.ravel())
+    u.label_surface[7][:] = (pattern, amount, relief, size)
+    return u
+
+
+@pytest.mark.parametrize("pattern", [1.0, 2.0])
+def test_surface_texture_identities(oracle, pattern):
+    """With no colour variation the texture leaves the albedo as it is; labels
+    without a texture, and label 0, are not touched at all; the relief tilts a
+    unit normal to another unit normal, not at all with relief 0."""
+    rng = np.random.default_rng(13)
+    alb = np.array([0.6, 0.5, 0.4])
+    u0 = texture_uniforms(pattern, 0.0, 1.0, 5.0)
+    u1 = texture_uniforms(pattern, 0.3, 1.0, 5.0)
+    flat = texture_uniforms(pattern, 0.3, 0.0, 5.0)
+    changed = 0
+    for _ in range(40):
+        idx = rng.uniform(0, 60, 3)
+        a, r, c = oracle.surface_texture(u0, 7, idx, alb, 0.5, 0.5)
+        assert np.allclose(a, alb, atol=1e-6)
+        a1, r1, c1 = oracle.surface_texture(u1, 7, idx, alb, 0.5, 0.5)
+        changed += not np.allclose(a1, alb)
+        for lab in (0, 8):
+            a2, r2, c2 = oracle.surface_texture(u1, lab, idx, alb, 0.5, 0.5)
+            assert np.array_equal(a2, alb.astype(np.float32)) and r2 == 0.5 and c2 == 0.5
+        n = unit(rng.normal(size=3))
+        b = oracle.bump_normal(u1, 7, idx, n)
+        assert np.linalg.norm(b) == pytest.approx(1.0, abs=1e-5)
+        assert np.allclose(oracle.bump_normal(flat, 7, idx, n), n, atol=1e-6)
+    assert changed > 30
+
+
+def test_index_to_world_inverts_the_volume_transform(oracle):
+    """index_to_world (the adjugate inverse of world_to_index, shading.glsl:312)
+    maps the index of a patient point back to it, for rotated, anisotropic grids."""
+    rng = np.random.default_rng(17)
+    for _ in range(5):
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        hu = np.zeros((6, 5, 4), np.int16)
+        vol = make_volume(hu, spacing=tuple(rng.uniform(0.5, 2.0, 3)), origin=tuple(rng.uniform(-50, 50, 3)),
+                          direction=tuple(q.T.ravel()))
+        p = frame_params(hu)
+        u = oracle.pack_uniforms(vol, p, np.zeros(64, np.float32), False, 0, 8, 8, 1)
+        w2i = np.array(u.world_to_index, np.float64).reshape(4, 4).T
+        for _ in range(10):
+            world = rng.uniform(-80, 80, 3)
+            idx = w2i[:3, :3] @ world + w2i[:3, 3]
+            assert np.allclose(oracle.index_to_world(u, idx), world, atol=2e-3)
+
+
+# ---- labels (common.glsl:232-351) ---------------------------------------------------------
+
+def label_uniforms(dims, mode, shown=(), ghosted=(), hidden=()):
+    u = _abi.Uniforms()
+    u.dims[0], u.dims[1], u.dims[2] = dims
+    u.labels[0] = mode
+    u.labels[2] = 0.5
+    for lab in range(256):
+        u.label_colors[lab][:] = (0.5, 0.5, 0.5, 0.0)
+    for lab in shown:
+        u.label_colors[lab][3] = 1.0
+    for lab in ghosted:
+        u.label_colors[lab][3] = 2.0
+    return u
+
+
+def candidate_ranks(idx, dims):
+    """The reference's candidates for idx (common.glsl:302-324), in their order:
+    the nearest voxel, the 8 corners of the cell containing idx (x fastest), then
+    the nearest voxel's axis neighbours at +1, -1 (x, y, z), then +2, -2; all
+    clamped to the volume. Returns {voxel: first rank}."""
+    hi = np.array(dims) - 1
+    nearest = np.clip(np.floor(np.asarray(idx) + 0.5).astype(int), 0, hi)
+    base = np.floor(idx).astype(int)
+    cands = [nearest]
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                cands.append(base + (dx, dy, dz))
+    for dist in (1, 2):
+        for sign in (1, -1):
+            for a in range(3):
+                d = np.zeros(3, int)
+                d[a] = sign * dist
+                cands.append(nearest + d)
+    ranks = {}
+    for r, c in enumerate(cands):
+        ranks.setdefault(tuple(np.clip(c, 0, hi)), r)
+    return ranks
+
+
+def test_label_search_candidates_brute_force(oracle):
+    """One drawn voxel in a volume of undrawn labels: the search finds it from
+    idx exactly when it is one of the 21 candidates."""
+    dims = (7, 6, 5)
+    rng = np.random.default_rng(19)
+    u = label_uniforms(dims, _abi.LABELS_ISOLATE, shown=(3,))
+    hits = misses = 0
+    for _ in range(60):
+        labels = np.full(dims[::-1], 5, np.uint8)  # label 5: hidden
+        vox = tuple(int(rng.integers(0, d)) for d in dims)
+        labels[vox[2], vox[1], vox[0]] = 3
+        sc = oracle.Scene(np.zeros(dims[::-1], np.uint16), tf=flat_tf(0.1), labels=labels)
+        for _ in range(40):
+            idx = rng.uniform(-1.5, np.array(dims) + 0.5)
+            found, lab = sc.drawn_label_near(u, idx)
+            want = vox in candidate_ranks(idx, dims)
+            assert found == want, (idx, vox)
+            if found:
+                assert lab == 3
+                hits += 1
+            else:
+                misses += 1
+    assert hits > 100 and misses > 100
+
+
+def test_label_search_order(oracle):
+    """Two drawn voxels among the candidates with different labels: the search
+    returns the one the reference's order reaches first."""
+    dims = (8, 8, 8)
+    rng = np.random.default_rng(23)
+    u = label_uniforms(dims, _abi.LABELS_ISOLATE, shown=(1, 2))
+    checked = 0
+    for _ in range(300):
+        idx = rng.uniform(1.0, 6.0, 3)
+        ranks = candidate_ranks(idx, dims)
+        vox = list(ranks)
+        a, b = rng.choice(len(vox), 2, replace=False)
+        labels = np.zeros(dims[::-1], np.uint8)
+        labels[vox[a][::-1]] = 1
+        labels[vox[b][::-1]] = 2
+        sc = oracle.Scene(np.zeros(dims[::-1], np.uint16), tf=flat_tf(0.1), labels=labels)
+        found, lab = sc.drawn_label_near(u, idx)
+        assert found and lab == (1 if ranks[vox[a]] < ranks[vox[b]] else 2)
+        checked += 1
+    assert checked == 300
+
+
+def test_label_search_solid_and_ghost_rules(oracle):
+    """Which labels count (common.glsl:232-247): in tint mode label 0 and shown
+    labels are solid; in isolate mode only shown labels other than 0; hidden and
+    ghosted labels are never solid; ghost searches find only ghosted labels."""
+    dims = (3, 3, 3)
+    for value, mode, ghost, want in ((0, _abi.LABELS_TINT, False, True), (0, _abi.LABELS_ISOLATE, False, False),
+                                     (4, _abi.LABELS_ISOLATE, False, True), (9, _abi.LABELS_ISOLATE, False, False),
+                                     (6, _abi.LABELS_ISOLATE, False, False), (6, _abi.LABELS_ISOLATE, True, True),
+                                     (4, _abi.LABELS_HYBRID, True, False), (6, _abi.LABELS_TINT, False, False)):
+        u = label_uniforms(dims, mode, shown=(4,), ghosted=(6,))
+        labels = np.full(dims[::-1], value, np.uint8)
+        sc = oracle.Scene(np.zeros(dims[::-1], np.uint16), tf=flat_tf(0.1), labels=labels)
+        found, lab = sc.drawn_label_near(u, [1.2, 0.8, 1.0], ghost)
+        assert found == want, (value, mode, ghost)
+
+
+def smoothstep(e0, e1, x):
+    t = min(max((x - e0) / (e1 - e0), 0.0), 1.0)
+    return t * t * (3 - 2 * t)
+
+
+def test_with_label_modes(oracle):
+    """with_label (common.glsl:253-290): tint mixes the transfer function's albedo
+    toward the label colour by `tint`; isolate takes the label's colour and the
+    label extinction; hybrid ramps that extinction by smoothstep(start, end,
+    density) and brightens the albedo by 1 + 0.2 t, t = clamp((density -
+    median) / IQR, -1.5, 1.5)."""
+    dims = (2, 2, 2)
+    tf = flat_tf(0.3, albedo=(0.2, 0.4, 0.6))
+    sc = oracle.Scene(np.zeros(dims[::-1], np.uint16), tf=tf, labels=np.zeros(dims[::-1], np.uint8))
+    color = np.array([0.9, 0.3, 0.1])
+    u = label_uniforms(dims, _abi.LABELS_TINT, shown=(5,))
+    u.label_colors[5][:3] = tuple(color)
+    u.labels[1] = 0.25
+    alb, ext = sc.with_label(u, 0.4, 5)
+    assert np.allclose(alb, 0.75 * np.array([0.2, 0.4, 0.6]) + 0.25 * color, atol=2e-3) and ext == pytest.approx(0.3,
+                                                                                                               rel=2e-3)
+    u.labels[0] = _abi.LABELS_ISOLATE
+    u.labels[2] = 0.8
+    alb, ext = sc.with_label(u, 0.4, 5)
+    assert np.allclose(alb, color, atol=1e-6) and ext == pytest.approx(0.8)
+    u.labels[0] = _abi.LABELS_HYBRID
+    start, end, median, iqr = 0.25, 0.31, 0.28, 0.02  # median below the bone density (300 HU)
+    u.label_density[5][:] = (start, end, median, iqr)
+    for dens in (0.2, 0.25, 0.265, 0.28, 0.295, 0.31, 0.5):
+        alb, ext = sc.with_label(u, dens, 5)
+        assert ext == pytest.approx(0.8 * smoothstep(start, end, dens), abs=2e-6), dens
+        t = min(max((dens - median) / iqr, -1.5), 1.5)
+        assert np.allclose(alb, np.clip(color * (1 + 0.2 * t), 0, 1), atol=2e-6), dens
+    # Bone (median above (300 + 1024) / 4095): thin, less dense parts take a warm
+    # tint, mix(1, (0.92, 0.6, 0.5), 1 - smoothstep(-1.5, -0.5, t)).
+    u.label_density[5][:] = (0.3, 0.36, 0.4, 0.05)
+    for dens in (0.3, 0.33, 0.36, 0.45):
+        alb, _ = sc.with_label(u, dens, 5)
+        t = min(max((dens - 0.4) / 0.05, -1.5), 1.5)
+        thin = 1 - smoothstep(-1.5, -0.5, t)
+        tint = 1 + thin * (np.array([0.92, 0.6, 0.5]) - 1)
+        assert np.allclose(alb, np.clip(color * (1 + 0.2 * t), 0, 1) * tint, atol=2e-6), dens
+    alb, ext = sc.with_label(u, 0.5, 5, ghost=True)  # ghosts keep the plain label extinction
+    assert ext == pytest.approx(0.8) and np.allclose(alb, color, atol=1e-6)
+
+
+def test_apply_labels_mask_thresholds(oracle):
+    """apply_labels (common.glsl:326-351): a point is drawn solid where the
+    smoothed mask's first channel is >= 0.5 (0.25 in hybrid mode), ghosted where
+    it is below and the second channel >= 0.5, else not at all (isolate)."""
+    dims = (4, 4, 4)
+    labels = np.full(dims[::-1], 2, np.uint8)
+    labels[:, :, 2:] = 3
+    for mode in (_abi.LABELS_ISOLATE, _abi.LABELS_HYBRID):
+        for r, g, want in ((0.6, 0.0, (2, False)), (0.3, 0.0, (2, False) if mode == _abi.LABELS_HYBRID else None),
+                           (0.2, 0.7, (3, True)), (0.2, 0.2, None)):
+            mask = np.zeros(dims[::-1] + (2,), np.uint16)
+            mask[..., 0] = round(r * 65535)
+            mask[..., 1] = round(g * 65535)
+            u = label_uniforms(dims, mode, shown=(2,), ghosted=(3,))
+            sc = oracle.Scene(np.zeros(dims[::-1], np.uint16), tf=flat_tf(0.1), labels=labels)
+            sc.set("mask", mask)
+            lab, ext, ghost = sc.apply_labels(u, [0.9, 1.1, 1.0])
+            if want is None:
+                assert ext == 0.0
+            else:
+                assert (lab, ghost) == want and ext > 0
+
+
+# ---- clipping (common.glsl:357-373, renderer.rs:1449-1466) -------------------------------
+
+def test_clip_depth_is_the_signed_distance(oracle):
+    """With one plane the depth is the signed patient-space distance (mm) into
+    the removed side: world[axis] - position cutting the positive side,
+    position - world[axis] the negative one; 'any' takes the deepest plane,
+    'all' the shallowest, each reporting its plane."""
+    rng = np.random.default_rng(29)
+    hu = np.zeros((10, 12, 14), np.int16)
+    spacing = (0.8, 1.25, 2.0)
+    vol = make_volume(hu, spacing=spacing, origin=(-5.0, 7.0, -11.0))
+    p = frame_params(hu)
+    planes = [(0, True, 1.5), (1, False, 12.0), (2, True, -3.0)]
+    for k, (axis, pos_side, pos) in enumerate(planes):
+        cp = p.clipping.planes[k]
+        cp.enabled, cp.plane, cp.cut_positive, cp.position = 1, axis, int(pos_side), pos
+    for all_ in (0, 1):
+        p.clipping.all = all_
+        u = oracle.pack_uniforms(vol, p, np.zeros(64, np.float32), False, 0, 8, 8, 1)
+        for _ in range(50):
+            idx = rng.uniform(-0.5, 12.0, 3)
+            world = np.array(vol.origin) + np.array(spacing) * idx
+            d = np.array([(world[a] - q) if s else (q - world[a]) for a, s, q in planes])
+            depth, plane = oracle.clip_depth(u, idx)
+            want = d.min() if all_ else d.max()
+            assert depth == pytest.approx(want, abs=1e-4)
+            assert d[plane] == pytest.approx(want, abs=1e-4)
+
+
+# ---- pre-integration (common.glsl:195-230, transfer.rs:103-117) -------------------------
+
+def ramp_tf():
+    """Extinction linear in HU over the whole range, albedo constant."""
+    return TransferFunction([(-1024.0, (0.5, 0.6, 0.7), 0.0), (3071.0, (0.5, 0.6, 0.7), 4.0)])
+
+
+def test_tf_segment_is_the_mean_over_the_segment(oracle):
+    """tf_segment: the mean extinction between the front and back densities, from
+    the cumulative integrals: for extinction linear in density, the extinction at
+    the midpoint; for a constant transfer function, the constant; equal ends
+    fall back to the point sample; the albedo is the extinction-weighted mean."""
+    hu = np.zeros((4, 4, 4), np.int16)
+    for tf, f in ((ramp_tf(), lambda d: 4.0 * d), (flat_tf(0.7, (0.2, 0.3, 0.4)), lambda d: 0.7)):
+        p = frame_params(hu)
+        p.preintegrate = 1
+        u = oracle.pack_uniforms(make_volume(hu), p, oracle.bucket_max(tf), False, 0, 8, 8, 1)
+        sc = oracle_scene(oracle, hu, tf, u)
+        for front, back in ((0.1, 0.6), (0.8, 0.3), (0.45, 0.4501), (0.2, 0.2)):
+            seg = sc.tf_segment(u, front, back)
+            assert seg[3] == pytest.approx(f(0.5 * (front + back)), rel=2e-3, abs=2e-3), (front, back)
+            point = np.array(tf.points[0][1])
+            assert np.allclose(seg[:3], point, atol=2e-3)
+
+
+def test_preint_half_depth_closed_form(oracle):
+    """preint_half_depth: the fraction x of the segment where the optical depth
+    reaches half of the segment's, by bisection (6 halvings, 1/128): 1/2 for a
+    constant extinction; for extinction e(d) = 4 d linear along the segment,
+    the root of the quadratic e0 x + (e1 - e0) x^2 / 2 = (e0 + e1) / 4."""
+    hu = np.zeros((4, 4, 4), np.int16)
+    for tf, lin in ((ramp_tf(), True), (flat_tf(0.7), False)):
+        p = frame_params(hu)
+        p.preintegrate = 1
+        u = oracle.pack_uniforms(make_volume(hu), p, oracle.bucket_max(tf), False, 0, 8, 8, 1)
+        sc = oracle_scene(oracle, hu, tf, u)
+        for front, back in ((0.1, 0.7), (0.05, 0.9), (0.6, 0.2)):
+            x = sc.preint_half_depth(u, front, back)
+            if lin:
+                e0, e1 = 4 * front, 4 * back
+                a, b, c = (e1 - e0) / 2, e0, -(e0 + e1) / 4
+                want = (-b + np.sqrt(b * b - 4 * a * c)) / (2 * a)
+            else:
+                want = 0.5
+            assert x == pytest.approx(want, abs=1 / 128 + 2e-3), (front, back)
+
+
+# ---- surface frame (shading.glsl:73-144) --------------------------------------------------
+
+def test_surface_frame_on_a_ramp(oracle):
+    """A density ramp along a patient axis, on a rotated, anisotropic grid: the
+    frame's normal is minus the ramp's patient-space gradient direction (the
+    cubic gradient mapped through transpose(world_to_index)) and its
+    surfaceness is the Sobel normal's clamp(400 |g|, 0, 1) = 1 for a steep ramp."""
+    rng = np.random.default_rng(31)
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    dims = (16, 16, 16)
+    spacing = (1.0, 1.3, 0.7)
+    k, j, i = np.mgrid[0:dims[2], 0:dims[1], 0:dims[0]]
+    world = np.stack([i * spacing[0], j * spacing[1], k * spacing[2]], -1) @ q.T
+    grad_dir = unit([0.3, 0.8, -0.5])
+    slope = 150.0  # HU per mm: steep, so that the integer HU steps bend the gradient little
+    hu = np.clip(np.rint(slope * (world - world[8, 8, 8]) @ grad_dir + 1000.0), -1024, 3071).astype(np.int16)
+    vol = make_volume(hu, spacing=spacing, origin=(0.0, 0.0, 0.0), direction=tuple(q.T.ravel()))
+    tf = flat_tf(0.2)
+    p = frame_params(hu)
+    u = oracle.pack_uniforms(vol, p, oracle.bucket_max(tf), False, 0, 8, 8, 1)
+    sc = oracle_scene(oracle, hu, tf, u)
+    inside = 0
+    for _ in range(60):
+        idx = rng.uniform(4, 11, 3)
+        n, surf, scale = sc.surface_frame(u, idx, -grad_dir)
+        window = hu[int(idx[2]) - 2:int(idx[2]) + 4, int(idx[1]) - 2:int(idx[1]) + 4, int(idx[0]) - 2:int(idx[0]) + 4]
+        if window.min() <= -1024 or window.max() >= 3071:  # the cubic support reaches a clipped voxel
+            continue
+        inside += 1
+        assert np.allclose(n, -grad_dir, atol=5e-3), (n, -grad_dir)
+        assert surf == pytest.approx(1.0) and scale == 1.0
+    assert inside > 10
+
+
+def test_surface_frame_ghost_silhouette_and_cap(oracle):
+    """Ghosted labels keep their silhouette: the extinction scale is surfaceness
+    x mix(ghost opacity, 1, (1 - |n.v|)^3), from the ghost mask's gradient: the
+    opacity alone face-on, surfaceness edge-on. A cut face (cap plane reached
+    within cap_depth) takes the plane's normal with surfaceness 1."""
+    dims = (12, 12, 12)
+    hu = np.zeros(dims[::-1], np.int16)
+    tf = flat_tf(0.2)
+    p = frame_params(hu)
+    p.labels.mode = _abi.LABELS_ISOLATE
+    p.labels.ghost_opacity = 0.15
+    u = oracle.pack_uniforms(make_volume(hu), p, oracle.bucket_max(tf), True, 0, 8, 8, 1)
+    labels = np.full(dims[::-1], 4, np.uint8)
+    sc = oracle_scene(oracle, hu, tf, u, labels=labels, mask=np.zeros(dims[::-1] + (2,), np.uint16))
+    mask = np.zeros(dims[::-1] + (2,), np.uint16)
+    mask[..., 1] = (np.arange(dims[0]) * 5000).astype(np.uint16)[None, None, :]  # ghost mask rising along +x
+    sc.set("mask", mask)
+    idx = [5.5, 6.0, 6.0]
+    g_len = 0.5 * 2 * 5000 / 65535
+    surf_want = min(g_len * 4, 1.0)
+    for v, edge in (([-1.0, 0.0, 0.0], 0.0), ([0.0, 1.0, 0.0], 1.0), (unit([1.0, 1.0, 0.0]), (1 - 0.5 ** 0.5) ** 3)):
+        n, surf, scale = sc.surface_frame(u, idx, v, label=4, ghost=True)
+        assert np.allclose(n, [-1.0, 0.0, 0.0], atol=1e-5)
+        assert surf == pytest.approx(surf_want, rel=1e-4)
+        assert scale == pytest.approx(surf_want * (0.15 + (1 - 0.15) * edge), rel=1e-4)
+    p.clipping.planes[0].enabled = 1
+    p.clipping.planes[0].plane = 2
+    u = oracle.pack_uniforms(make_volume(hu), p, oracle.bucket_max(tf), True, 0, 8, 8, 1)
+    n, surf, _ = sc.surface_frame(u, idx, [0, 0, 1.0], label=4, ghost=False, cap_plane=0, cut=-0.2, cap_depth=0.75)
+    assert np.allclose(n, [0.0, 0.0, -1.0]) and surf == 1.0
+    n2, _, _ = sc.surface_frame(u, idx, [0, 0, 1.0], label=4, ghost=False, cap_plane=0, cut=-1.0, cap_depth=0.75)
+    assert not np.allclose(n2, [0.0, 0.0, -1.0])
+
+
+def test_tissue_sigma(oracle):
+    """Measured tissue (shading.glsl:161-168): the extinction is mu_s' / (1 - g)
+    (g clamped to 0.95), grey at the green channel's value without chroma,
+    scaled down keeping the channels' ratios when its largest channel exceeds the
+    resolution limit, and thinned by the occupancy extinction / label opacity
+    (clamped to [0, 1])."""
+    mu = np.array([0.9, 1.4, 2.3])
+    u = _abi.Uniforms()
+    u.label_optics[3][:] = (*mu, 1.0)
+    u.labels[2] = 0.5
+    u.tissue[1] = 100.0
+    u.env[3] = 1.0
+    for g in (0.0, 0.5, 0.8, 0.99):
+        u.env[2] = g
+        assert np.allclose(oracle.tissue_sigma(u, 3, 0.5), mu / (1 - min(g, 0.95)), rtol=1e-6)
+    u.env[2] = 0.0
+    u.env[3] = 0.0
+    assert np.allclose(oracle.tissue_sigma(u, 3, 0.5), [1.4] * 3, rtol=1e-6)
+    u.env[3] = 1.0
+    u.tissue[1] = 1.15
+    assert np.allclose(oracle.tissue_sigma(u, 3, 0.5), mu * 1.15 / 2.3, rtol=1e-6)
+    u.tissue[1] = 100.0
+    assert np.allclose(oracle.tissue_sigma(u, 3, 0.2), mu * 0.4, rtol=1e-6)
+    assert np.allclose(oracle.tissue_sigma(u, 3, 0.9), mu, rtol=1e-6)
