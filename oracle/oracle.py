@@ -125,6 +125,7 @@ def load() -> ctypes.CDLL:
             "oracle_preint_half_depth": (c_float, [P(OracleScene), P(_abi.Uniforms), c_float, c_float, c_int]),
             "oracle_surface_frame": (c_float, [P(OracleScene), P(_abi.Uniforms), P(c_float), c_uint32, c_int,
                                                P(c_float), c_int, c_float, c_float, c_int, P(c_float), P(c_float)]),
+            "oracle_vlog_vexp": (None, [P(c_float), c_size_t, P(c_float), P(c_float)]),
             "oracle_tissue_sigma": (None, [P(_abi.Uniforms), c_uint32, c_float, P(c_float)]),
             "oracle_pixel_stream": (None, [P(_abi.Uniforms), c_uint32, c_uint32, c_uint32, P(c_uint32)]),
         }
@@ -583,3 +584,11 @@ def tissue_sigma(u, label, extinction) -> np.ndarray:
     out = (c_float * 3)()
     load().oracle_tissue_sigma(byref(u), label, extinction, out)
     return np.array(out, np.float64)
+
+
+def vlog_vexp(x: np.ndarray):
+    """The FP32 oracle's log and exp of each value."""
+    a = np.ascontiguousarray(x, np.float32)
+    lo, ex = np.zeros_like(a), np.zeros_like(a)
+    load().oracle_vlog_vexp(_p(a, c_float), a.size, _p(lo, c_float), _p(ex, c_float))
+    return lo, ex

@@ -2238,6 +2238,10 @@ float oracle_surface_frame(const oracle_scene* s, const vvr_uniforms* u, const f
     return precision == 64 ? run(Ctx<double>(s, u)) : run(Ctx<float>(s, u));
 }
 
+void oracle_vlog_vexp(const float* x, size_t n, float* log_out, float* exp_out) {
+    for (size_t i = 0; i < n; i++) log_out[i] = vlog(x[i]), exp_out[i] = vexp(x[i]);
+}
+
 void oracle_tissue_sigma(const vvr_uniforms* u, uint32_t label, float extinction, float out[3]) {
     oracle_scene sc{};
     Ctx<double> c(&sc, u);

@@ -676,3 +676,29 @@ def test_tissue_sigma(oracle):
     u.tissue[1] = 100.0
     assert np.allclose(oracle.tissue_sigma(u, 3, 0.2), mu * 0.4, rtol=1e-6)
     assert np.allclose(oracle.tissue_sigma(u, 3, 0.9), mu, rtol=1e-6)
+
+
+def ulps(a, b):
+    """Distance in units in the last place between float32 arrays of one sign."""
+    ia = np.abs(a.astype(np.float32)).view(np.int32).astype(np.int64)
+    ib = np.abs(b.astype(np.float32)).view(np.int32).astype(np.int64)
+    return np.abs(ia - ib)
+
+
+def test_log_and_exp_models(oracle):
+    """The render path's log and exp (Cody-Waite reduction; atanh series and
+    Taylor polynomial) are within 2 ulp (log) and 1 ulp (exp) of the correctly
+    rounded values, over every 97th positive float and exp's whole finite range,
+    and exact at log(1) = 0, exp(0) = 1; log(0) = -inf, exp underflows to 0."""
+    bits = np.arange(1, 0x7F800000, 97, dtype=np.int64).astype(np.uint32)
+    x = bits.view(np.float32)
+    lo, _ = oracle.vlog_vexp(x)
+    want = np.log(x.astype(np.float64)).astype(np.float32)
+    same_sign = np.sign(lo) == np.sign(want)
+    assert np.all(ulps(lo[same_sign], want[same_sign]) <= 2)
+    assert np.all(np.abs(lo[~same_sign]) < 1e-7)  # around x = 1
+    y = np.linspace(-103.9, 88.7, 400001, dtype=np.float32)
+    _, ex = oracle.vlog_vexp(y)
+    assert np.all(ulps(ex, np.exp(y.astype(np.float64)).astype(np.float32)) <= 1)
+    lo, ex = oracle.vlog_vexp(np.array([1.0, 0.0, -120.0, 0.0], np.float32))
+    assert lo[0] == 0.0 and lo[1] == -np.inf and ex[2] == 0.0 and ex[3] == 1.0
