@@ -44,6 +44,21 @@ def pt_frame_bytes(dims, width, height) -> int:
     return (2 + 4 + 1 + 4) * nvox + 4 * nbricks + 3 * 2 * 16 * width * height
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo)."""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads() -> int:
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
 def cpu_baseline(scene, params, seconds: float) -> dict:
     """The FP32 CPU oracle (a port of pt.comp, OpenMP over all host cores) on
     random pixels of the same frame, one path each, for about `seconds`."""
@@ -69,9 +84,39 @@ def cpu_baseline(scene, params, seconds: float) -> dict:
         done += batch
         if dt < seconds / 20:
             batch *= 2
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": done / t_total / 1e6, "unit": "Msamples/s", "cores": cores, "kind": "port",
+    return {"value": done / t_total / 1e6, "unit": "Msamples/s", "cores": cpu_threads(), "kind": "port",
+            "cpu": cpu_model(),
             "sample": f"{done} paths (1 per random pixel) of the PT-1 frame, {t_total:.1f} s, FP32 C++ oracle"}
+
+
+def dvr_cpu_baseline(scene, seconds: float) -> dict:
+    """The FP32 CPU oracle of dvr.comp (OpenMP over the host cores) on random
+    pixels of the DVR-1 frame for about `seconds`, as ms per 512^2 frame."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle  # noqa: E402
+    oracle.load()
+    r = scene.renderer
+    tf = TransferFunction.preset(0)
+    u = oracle.pack_uniforms(make_volume(scene.hu), scene.params, oracle.bucket_max(tf), False, 0, scene.width,
+                             scene.height, 1)
+    sc = oracle.Scene(oracle.hu_to_unorm16(scene.hu), tf=tf)
+    sc.set("normals", r.read_volume(_abi.VOL_NORMALS))
+    sc.set("majorant", r.read_majorant())
+    rng = np.random.default_rng(6)
+    done, t_total, batch = 0, 0.0, 1024
+    while t_total < seconds:
+        px = np.stack([rng.integers(0, scene.width, batch), rng.integers(0, scene.height, batch)], 1)
+        t0 = time.perf_counter()
+        sc.dvr(u, px.astype(np.uint32), 32)
+        dt = time.perf_counter() - t0
+        t_total += dt
+        done += batch
+        if dt < seconds / 20:
+            batch *= 2
+    per_frame = t_total / done * scene.width * scene.height * 1e3
+    return {"value": round(per_frame, 1), "unit": "ms per 512^2 frame", "cores": cpu_threads(), "kind": "port",
+            "cpu": cpu_model(),
+            "sample": f"{done} random pixels of the DVR-1 frame, {t_total:.1f} s, FP32 C++ oracle, extrapolated"}
 
 
 def copy_probe(nbytes: int) -> None:
@@ -365,6 +410,49 @@ def run_serving(ctx, seconds: float, grace: float = 5.0, increment: int = 16, sl
             "window_seconds": window, "plain": plain, "served": served}
 
 
+def served_quality(ctx, reference_spp: int = 4096, budget: int = 256, increment: int = 16) -> dict:
+    """The serving trace's progressive requests against a converged frame: the
+    first path-traced request of the trace served to its whole budget (256
+    paths, in 16-path frames) and the same request rendered with 4096 paths by
+    its renderer; relative RMSE of the served frame (linear RGB and luminance)."""
+    from volview_amd import phantom
+    from volview_amd.renderer import Renderer
+    from volview_amd.server import Server
+    vols, reqs = workloads.serving_trace(5.0)
+    q = next(q for q in reqs if q.path_traced)
+    dims, seed = vols[q.volume]
+    hu = phantom.ct(dims, seed)
+    labels, soft, _, n = phantom.organs(dims, seed, 40)
+    r = Renderer(ctx, hu, TransferFunction.preset(0))
+    r.set_labels(labels, soft)
+    p = workloads.request_params(r, q, n)
+    p.path_tracing.samples_per_frame = budget
+    srv = Server(ctx, pt_increment=increment)
+    rid = srv.submit(r, p, q.width, q.height)
+    drain(srv, [rid], 300.0)
+    served = srv.read_hdr(rid).astype(np.float64)
+    srv.release(rid)
+    srv.close()
+    ref = _abi.FrameParams.from_buffer_copy(p)
+    ref.path_tracing.samples_per_frame = 512  # frames of 512 paths accumulate to the reference
+    r.resize(q.width, q.height)
+    t = time.perf_counter()
+    for f in range(reference_spp // 512):
+        r.record(1 + f, ref)
+    conv = r.read_hdr().astype(np.float64)
+    t_ref = time.perf_counter() - t
+    r.close()
+    lum = np.array([0.2126, 0.7152, 0.0722])
+    rel = lambda a, b: float(np.sqrt(np.mean((a - b) ** 2) / np.mean(b ** 2)))
+    out = {"request": f"{q.width}x{q.height}, volume {dims[0]}^3 with {n} organs, {budget} paths served in "
+                      f"{increment}-path frames", "reference_paths": reference_spp,
+           "rel_rmse_rgb": round(rel(served[..., :3], conv[..., :3]), 4),
+           "rel_rmse_luminance": round(rel(served[..., :3] @ lum, conv[..., :3] @ lum), 4),
+           "reference_render_s": round(t_ref, 2)}
+    log(f"served quality: {out}")
+    return out
+
+
 def dvr_frame_bytes(dims, width, height) -> int:
     """Algorithmic bytes of one DVR-1 frame with lighting off: density (2 B) and
     normals (4 B) per voxel, the majorant grid, the RGBA16F image."""
@@ -387,7 +475,7 @@ def roofline_gbs(nbytes: int, ms: float) -> dict:
             "frac": round(achieved / HBM_PEAK_GBS, 4), "algorithmic_bytes": nbytes}
 
 
-def run_dvr(ctx, frames: int = 40) -> dict:
+def run_dvr(ctx, frames: int = 40, cpu_seconds: float = 0.0) -> dict:
     """DVR-1 (256^3 phantom, bone preset, 512^2, cubic sampling, lighting off):
     frames/s, the ray marcher's HIP-event time per frame and its roofline, wall
     latency of record + read-back; the majorant build's time (the median of 9
@@ -415,6 +503,7 @@ def run_dvr(ctx, frames: int = 40) -> dict:
         lat.append(time.perf_counter() - t)
     dvr_ms, n = r.kernel_time(_abi.KERNEL_DVR)
     r.set_timing(False)
+    cpu = dvr_cpu_baseline(sc, cpu_seconds) if cpu_seconds > 0 else None
     r.close()
     per = dvr_ms / max(n, 1)
     # With the shadow and occlusion volumes (built once, then read by every frame).
@@ -438,7 +527,8 @@ def run_dvr(ctx, frames: int = 40) -> dict:
                            "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
             "roofline": roofline_gbs(dvr_frame_bytes(dims, *size), per),
             "lit": {"ms_per_frame": round(lit_ms / max(n_lit, 1), 3), "lighting_volumes_ms": round(light_ms, 3)},
-            "majorant": {"ms": round(maj_ms, 4), "roofline": roofline_gbs(majorant_bytes(dims, False), maj_ms)}}
+            "majorant": {"ms": round(maj_ms, 4), "roofline": roofline_gbs(majorant_bytes(dims, False), maj_ms)},
+            "cpu_baseline": cpu}
 
 
 def unet_flops(channels, width, height) -> float:
@@ -502,8 +592,12 @@ def main() -> int:
     ap.add_argument("--no-dvr", action="store_true", help="skip the DVR-1 frame timings")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default shape instead of the autotuner's (vvr_renderer_autotune)")
-    ap.add_argument("--serve-seconds", type=float, default=20.0,
-                    help="length of the Poisson serving trace (0: skip it)")
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
+                    help="length of the Poisson serving trace at 20 requests/s (0: skip it)")
+    ap.add_argument("--no-quality", action="store_true",
+                    help="skip the served frame's relative RMSE against a 4096-path frame")
+    ap.add_argument("--only", choices=["dvr", "denoisers"], default=None,
+                    help="run only this part (for profiling): no PT-1 steps and no JSON line")
     ap.add_argument("--serve-slots", type=int, default=1 << 20, help="path slots of the server in the trace")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB mebibytes device to device (a plain copy kernel whose HBM counters "
@@ -516,6 +610,9 @@ def main() -> int:
 
     t_setup = time.perf_counter()
     ctx = Context(0)
+    if args.only == "dvr":
+        print(json.dumps({"dvr": run_dvr(ctx)}), flush=True)
+        return 0
     scene = workloads.pt_scene(ctx, dims=tuple(args.dims), size=(args.size, args.size), spp=args.spp)
     r = scene.renderer
     print(f"setup {time.perf_counter() - t_setup:.1f} s: {scene.n_labels} organs", file=sys.stderr, flush=True)
@@ -587,15 +684,20 @@ def main() -> int:
     out["majorant"] = {"workload": f"PT-1 volume {args.dims[0]}x{args.dims[1]}x{args.dims[2]}, isolate label mode",
                        "ms": round(pt_majorant_ms, 4),
                        "roofline": roofline_gbs(majorant_bytes(args.dims, True), pt_majorant_ms)}
-    if not args.no_denoisers:
+    if not args.no_denoisers or args.only == "denoisers":
         out["denoisers"] = run_denoisers(r, scene.params, args.size)
     r.close()
+    if args.only == "denoisers":
+        print(json.dumps({"denoisers": out["denoisers"]}), flush=True)
+        return 0
     if not args.no_dvr:
-        out["dvr"] = run_dvr(ctx)
+        out["dvr"] = run_dvr(ctx, cpu_seconds=0.0 if args.no_cpu else 8.0)
     if not args.no_batched:
         out["batched"] = run_batched(ctx)
     if args.serve_seconds > 0:
         out["serving"] = run_serving(ctx, args.serve_seconds, slots=args.serve_slots)
+    if not args.no_quality:
+        out["served_quality"] = served_quality(ctx)
     print(json.dumps(out), flush=True)
     return 0
 
