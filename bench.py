@@ -461,12 +461,15 @@ def dvr_frame_bytes(dims, width, height) -> int:
     return 6 * n + 4 * bricks + 8 * width * height
 
 
-def majorant_bytes(dims, label_mode: bool) -> int:
-    """Algorithmic bytes of the majorant build: density (2 B), the mask in label
-    modes (4 B) per voxel, one f32 per 8^3 brick."""
+def majorant_bytes(dims, mode: int) -> int:
+    """Algorithmic bytes of the majorant build: per voxel the inputs its bound
+    depends on, read once (the density, 2 B, without labels and in tint mode;
+    the mask, 4 B, in the label modes: isolate and hybrid bounds are the label
+    opacity wherever the mask is non-zero), and one f32 per 8^3 brick."""
     n = dims[0] * dims[1] * dims[2]
     bricks = -(-dims[0] // 8) * -(-dims[1] // 8) * -(-dims[2] // 8)
-    return (6 if label_mode else 2) * n + 4 * bricks
+    per_voxel = {_abi.LABELS_OFF: 2, _abi.LABELS_TINT: 6}.get(mode, 4)
+    return per_voxel * n + 4 * bricks
 
 
 def roofline_gbs(nbytes: int, ms: float) -> dict:
@@ -527,7 +530,8 @@ def run_dvr(ctx, frames: int = 40, cpu_seconds: float = 0.0) -> dict:
                            "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
             "roofline": roofline_gbs(dvr_frame_bytes(dims, *size), per),
             "lit": {"ms_per_frame": round(lit_ms / max(n_lit, 1), 3), "lighting_volumes_ms": round(light_ms, 3)},
-            "majorant": {"ms": round(maj_ms, 4), "roofline": roofline_gbs(majorant_bytes(dims, False), maj_ms)},
+            "majorant": {"ms": round(maj_ms, 4),
+                         "roofline": roofline_gbs(majorant_bytes(dims, _abi.LABELS_OFF), maj_ms)},
             "cpu_baseline": cpu}
 
 
@@ -683,7 +687,7 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(scene, scene.params, args.cpu_seconds)
     out["majorant"] = {"workload": f"PT-1 volume {args.dims[0]}x{args.dims[1]}x{args.dims[2]}, isolate label mode",
                        "ms": round(pt_majorant_ms, 4),
-                       "roofline": roofline_gbs(majorant_bytes(args.dims, True), pt_majorant_ms)}
+                       "roofline": roofline_gbs(majorant_bytes(args.dims, _abi.LABELS_ISOLATE), pt_majorant_ms)}
     if not args.no_denoisers or args.only == "denoisers":
         out["denoisers"] = run_denoisers(r, scene.params, args.size)
     r.close()

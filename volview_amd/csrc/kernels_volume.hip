@@ -345,128 +345,224 @@ __global__ void __launch_bounds__(kBlock) k_mask_blur(const uint32_t* __restrict
     dst[o] = unorm16_bits(s0 / taps.e) | (unorm16_bits(s1 / taps.e) << 16);
 }
 
-// majorant.comp:26-71. A workgroup covers 4 x 4 x 4 bricks. Every brick's window
-// [8b - 2, 8b + 9] (clamped to the volume) is a union of 6^3 aligned 2^3-voxel
-// cells, so the group first reduces the 18^3 cells of its bricks and their
-// halo into LDS, then each brick reduces its 216 cells. Densities stay integer:
-// int(d * 64) of d = c / 65535 equals floor(64 c / 65535) for every c (no
-// quotient lies within f32 rounding of a bucket edge), and mask max > 0 iff the
-// largest raw mask value is > 0.
-constexpr int kTile = 4;                 // bricks per axis per workgroup
-constexpr int kCells = 4 * kTile + 2;  // 18 cells per axis: 4 per brick and 1 of halo each side
+// ---- majorant (majorant.comp) ----------------------------------------------------------
+//
+// Brick b (8^3 voxels) bounds the extinction over the voxels 8b - 2 .. 8b + 9 of
+// each axis, clamped to the volume (majorant.comp:26-71): the 2^3-voxel cells
+// 4b - 1 .. 4b + 4. Along an axis, micro-brick (4 voxels) 2j + 1 covers cells
+// 4j + 2 and 4j + 3, both in brick j, the second also in brick j + 1; micro-brick
+// 2j covers cells 4j (also in brick j - 1) and 4j + 1. So each micro-brick feeds
+// its own brick with all its cells and, per axis, one neighbour with one layer.
+//
+// k_majorant_cells visits every 4^3 micro-brick (64 contiguous voxels) once:
+// eight lanes read it with one 16-byte load each (two of the mask). Each lane
+// folds its voxels into the micro-brick's 2^3 cells, the lanes combine by
+// shuffles, then
+//  - modes off and tint: the cells' density min | max << 16 go to a 32-byte
+//    record per micro-brick, and (tint) a byte with bit c set when a mask value
+//    (r or g) of cell c is non-zero; k_majorant_bricks reduces each brick's
+//    window from a tile of records staged in LDS and writes the bound;
+//  - isolate and hybrid: the bound is the label opacity wherever a mask value
+//    of the window is non-zero, else 0, so with the grid cleared to 0 the lanes
+//    write the opacity straight into the (up to 8) bricks whose windows hold a
+//    non-zero cell of the micro-brick (equal values; the order of the stores
+//    does not matter). The density is not read.
+// Voxels of partial micro-bricks outside the volume do not count.
 
-__global__ void __launch_bounds__(kBlock) k_majorant(const uint16_t* __restrict__ d, const uint32_t* __restrict__ mask,
-                                                     float* __restrict__ out, BrickGrid g, int mode, int mx, int my,
-                                                     int mz, const vvr_uniforms* __restrict__ u) {
-    __shared__ uint32_t s_minmax[kCells * kCells * kCells];  // min | max << 16
-    __shared__ uint16_t s_mask[kCells * kCells * kCells];
-    const int tbx = blockIdx.x * kTile, tby = blockIdx.y * kTile, tbz = blockIdx.z * kTile;
-    // Cell c covers voxels 2c, 2c + 1; the tile's first cell is 4 * tb - 1. A
-    // 4^3 micro-brick (64 contiguous voxels) holds 2^3 cells: the tile's cells
-    // lie in micro-bricks 2 tb - 1 .. 2 tb + 2 kTile. Eight neighbouring lanes
-    // read one micro-brick with one 16-byte load each (density; two of mask),
-    // so a wave's loads cover whole lines, and combine their cells by shuffles.
-    //   density chunk k (voxels 8k .. 8k + 7): rows dy = 2 (k & 1) + {0, 1} of
-    //     dz = k >> 1, i.e. cells cx = 0, 1 of cy = k & 1, cz = k >> 2; lane
-    //     k ^ 2 holds the cells' other dz;
-    //   mask chunks k and k + 8 (4 voxels each): row dy = k & 3 of dz = k >> 2
-    //     and 2 + (k >> 2), i.e. cells cx = 0, 1 of cy = (k >> 1) & 1 and
-    //     cz = 0, 1; lanes k ^ 1 and k ^ 4 hold the cells' other rows.
-    const int c0x = 4 * tbx - 1, c0y = 4 * tby - 1, c0z = 4 * tbz - 1;
-    constexpr int kMb = 2 * kTile + 2;  // micro-bricks per axis
-    static_assert((kMb * kMb * kMb * 8) % 64 == 0, "every wave runs the loop's iterations whole");
-    for (int t = threadIdx.x; t < kMb * kMb * kMb * 8; t += kBlock) {
-        const int m = t >> 3, k = t & 7;
-        const int mbx = 2 * tbx - 1 + m % kMb, mby = 2 * tby - 1 + (m / kMb) % kMb, mbz = 2 * tbz - 1 + m / (kMb * kMb);
-        uint32_t lo[2] = {0xffffu, 0xffffu}, hi[2] = {0u, 0u}, mm[4] = {0u, 0u, 0u, 0u};  // mm[cx + 2 cz]
-        if (mbx >= 0 && mby >= 0 && mbz >= 0 && mbx < g.bx && mby < g.by && mbz < g.bz) {
-            const uint32_t base = g.at(4 * mbx, 4 * mby, 4 * mbz);  // the micro-brick's first voxel
-            // Voxel v = dx + 4 dy + 16 dz of the micro-brick; inside the volume?
-            auto inside = [&](int v) {
-                return 4 * mbx + (v & 3) < g.nx && 4 * mby + ((v >> 2) & 3) < g.ny && 4 * mbz + (v >> 4) < g.nz;
-            };
-            const uint4 w4 = reinterpret_cast<const uint4*>(d + base)[k];
-            const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
+__global__ void __launch_bounds__(kBlock) k_fill_f32(float* __restrict__ out, uint32_t n, float v) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) out[i] = v;
+}
+
+// Micro-bricks per group of eight lanes (their loads are issued before any is
+// used). One measured fastest: 83 us on the PT-1 volume against 89 with four,
+// where fewer lanes are left to cover the latency.
+constexpr int kMajRun = 1;
+
+__global__ void __launch_bounds__(kBlock) k_majorant_cells(const uint16_t* __restrict__ d,
+                                                           const uint32_t* __restrict__ mask,
+                                                           uint32_t* __restrict__ cell_mm, uint8_t* __restrict__ cell_nz,
+                                                           float* __restrict__ out, BrickGrid g, int mode, int mx,
+                                                           int my, int mz, const vvr_uniforms* __restrict__ u) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t m0 = (t >> 3) * kMajRun, k = t & 7u;
+    const uint32_t nmb = uint32_t(g.bx) * uint32_t(g.by) * uint32_t(g.bz);
+    if (m0 >= nmb) return;  // whole groups of eight lanes, so the shuffles below stay within live groups
+    // Loads of the group's micro-bricks m0 .. m0 + kMajRun - 1 (those that exist), stored in this
+    // order, 64 voxels each: density chunk k (16 bytes), mask chunks k and k + 8.
+    uint4 dw[kMajRun], mw[kMajRun][2];
+#pragma unroll
+    for (int i = 0; i < kMajRun; i++) {
+        const uint32_t m = m0 + i;
+        if (m >= nmb) break;
+        const size_t base = size_t(m) * 64u;
+        if (mode <= 1) dw[i] = reinterpret_cast<const uint4*>(d + base)[k];
+        if (mode != 0) {
+            mw[i][0] = reinterpret_cast<const uint4*>(mask + base)[k];
+            mw[i][1] = reinterpret_cast<const uint4*>(mask + base)[k + 8];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kMajRun; i++) {
+        const uint32_t m = m0 + i;
+        if (m >= nmb) break;  // uniform over the group
+        const int mbx = int(m % uint32_t(g.bx)), mby = int((m / uint32_t(g.bx)) % uint32_t(g.by));
+        const int mbz = int(m / (uint32_t(g.bx) * uint32_t(g.by)));
+        const int rx = g.nx - 4 * mbx, ry = g.ny - 4 * mby, rz = g.nz - 4 * mbz;  // voxels left per axis, >= 1
+        // Voxel v = dx + 4 dy + 16 dz of the micro-brick lies inside the volume.
+        auto inside = [&](int v) { return (v & 3) < rx && ((v >> 2) & 3) < ry && (v >> 4) < rz; };
+        if (mode <= 1) {
+            // Chunk k holds voxels 8k .. 8k + 7: dz = k >> 1, dy = 2 (k & 1) + (j >> 2), dx = j & 3, i.e.
+            // cells cx = 0, 1 of cy = k & 1, cz = k >> 2; lane k ^ 2 holds the same cells' other dz.
+            const uint32_t w[4] = {dw[i].x, dw[i].y, dw[i].z, dw[i].w};
+            uint32_t lo[2] = {0xffffu, 0xffffu}, hi[2] = {0u, 0u};
 #pragma unroll
             for (int j = 0; j < 8; j++) {
-                const int v = 8 * k + j;
-                if (!inside(v)) continue;
+                if (!inside(int(8 * k) + j)) continue;
                 const uint32_t x = (j & 1) ? (w[j >> 1] >> 16) : (w[j >> 1] & 0xffffu);
                 const int cx = (j >> 1) & 1;
                 lo[cx] = min(lo[cx], x);
                 hi[cx] = max(hi[cx], x);
             }
-            if (mode != 0) {
-                const uint4* mv = reinterpret_cast<const uint4*>(mask + base);
 #pragma unroll
-                for (int half = 0; half < 2; half++) {
-                    const int q = k + 8 * half;
-                    const uint4 m4 = mv[q];
-                    const uint32_t mw[4] = {m4.x, m4.y, m4.z, m4.w};
-#pragma unroll
-                    for (int dx = 0; dx < 4; dx++) {
-                        if (!inside(4 * q + dx)) continue;
-                        uint32_t& c = mm[(dx >> 1) + 2 * half];
-                        c = max(c, max(mw[dx] & 0xffffu, mw[dx] >> 16));
-                    }
-                }
+            for (int c = 0; c < 2; c++) {
+                lo[c] = min(lo[c], uint32_t(__shfl_xor(int(lo[c]), 2)));
+                hi[c] = max(hi[c], uint32_t(__shfl_xor(int(hi[c]), 2)));
+            }
+            if ((k & 2u) == 0u) {  // lanes 0, 1, 4, 5: cells 2 (k & 1) + 4 (k >> 2) + {0, 1}
+                const uint32_t c0 = 2u * (k & 1u) + 4u * (k >> 2);
+                reinterpret_cast<uint2*>(cell_mm + size_t(m) * 8u + c0)[0] =
+                    make_uint2(lo[0] | (hi[0] << 16), lo[1] | (hi[1] << 16));
             }
         }
+        if (mode == 0) continue;
+        // Mask chunks q = k and k + 8 (four texels each): dz = q >> 2, dy = q & 3, i.e. cells
+        // cx = dx >> 1 of cy = (q >> 1) & 1, cz = q >> 3.
+        uint32_t bits = 0u;
 #pragma unroll
-        for (int c = 0; c < 2; c++) {
-            lo[c] = min(lo[c], uint32_t(__shfl_xor(int(lo[c]), 2)));
-            hi[c] = max(hi[c], uint32_t(__shfl_xor(int(hi[c]), 2)));
+        for (int half = 0; half < 2; half++) {
+            const int q = int(k) + 8 * half;
+            const uint32_t tw[4] = {mw[i][half].x, mw[i][half].y, mw[i][half].z, mw[i][half].w};
+#pragma unroll
+            for (int dx = 0; dx < 4; dx++)
+                if (tw[dx] != 0u && inside(4 * q + dx)) bits |= 1u << ((dx >> 1) + 2 * ((q >> 1) & 1) + 4 * (q >> 3));
         }
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            mm[c] = max(mm[c], uint32_t(__shfl_xor(int(mm[c]), 1)));
-            mm[c] = max(mm[c], uint32_t(__shfl_xor(int(mm[c]), 4)));
+        bits |= uint32_t(__shfl_xor(int(bits), 1));
+        bits |= uint32_t(__shfl_xor(int(bits), 2));
+        bits |= uint32_t(__shfl_xor(int(bits), 4));
+        if (mode == 1) {
+            if (k == 0u) cell_nz[m] = uint8_t(bits);
+            continue;
         }
-        // Owners write the micro-brick's cells that lie in the tile (all of
-        // them but halo halves): lanes 0, 1, 4, 5 the density cells, lanes 0, 2
-        // the mask cells.
-        auto in_tile = [&](int cx, int cy, int cz, int& i) {
-            cx += 2 * mbx - c0x, cy += 2 * mby - c0y, cz += 2 * mbz - c0z;
-            i = cx + kCells * (cy + kCells * cz);
-            return cx >= 0 && cy >= 0 && cz >= 0 && cx < kCells && cy < kCells && cz < kCells;
-        };
-        int i;
-        if ((k & 2) == 0)
+        // Isolate / hybrid: lane k feeds the brick that is the neighbour along the axes
+        // of k's set bits (own brick along the others) with the cells it holds there.
+        const int mb[3] = {mbx, mby, mbz};
+        int brick[3];
+        uint32_t sel = 0xffu;  // cells of the micro-brick in that brick's window
 #pragma unroll
-            for (int cx = 0; cx < 2; cx++)
-                if (in_tile(cx, k & 1, k >> 2, i)) s_minmax[i] = lo[cx] | (hi[cx] << 16);
-        if ((k & 5) == 0)
+        for (int a = 0; a < 3; a++) {
+            const bool nb = (k >> a) & 1u;
+            const bool odd = mb[a] & 1;
+            brick[a] = (mb[a] >> 1) + (nb ? (odd ? 1 : -1) : 0);
+            if (nb) {  // one layer: the upper cell of an odd micro-brick, the lower of an even one
+                const uint32_t layer = odd ? 1u : 0u;
+                uint32_t keep = 0u;
 #pragma unroll
-            for (int c = 0; c < 4; c++)
-                if (in_tile(c & 1, (k >> 1) & 1, c >> 1, i)) s_mask[i] = uint16_t(mm[c]);
+                for (int c = 0; c < 8; c++)
+                    if (uint32_t((c >> a) & 1) == layer) keep |= 1u << c;
+                sel &= keep;
+            }
+        }
+        if ((bits & sel) && brick[0] >= 0 && brick[1] >= 0 && brick[2] >= 0 && brick[0] < mx && brick[1] < my &&
+            brick[2] < mz)
+            out[size_t(brick[0]) + size_t(mx) * (size_t(brick[1]) + size_t(my) * size_t(brick[2]))] = u->labels[2];
+    }
+}
+
+// Modes off and tint: bounds from the cell records. A workgroup takes a tile of
+// 4^3 bricks, stages the records of their 10^3-micro-brick window in LDS, and
+// four lanes reduce each brick's 4^3 micro-bricks (16 each: one z layer of the
+// window) before lane 0 of the four writes the bound. (Tiles of 2^3 bricks in
+// one-wave workgroups measured slower: 16 against 13 us on the DVR-1 volume.)
+constexpr int kMajTile = 4;
+constexpr int kMajMb = 2 * kMajTile + 2;  // micro-bricks per axis of a tile's windows
+constexpr int kMajThreads = 256;
+
+__global__ void __launch_bounds__(kMajThreads) k_majorant_bricks(const uint32_t* __restrict__ cell_mm,
+                                                                 const uint8_t* __restrict__ cell_nz,
+                                                                 float* __restrict__ out, BrickGrid g, int mode, int mx,
+                                                                 int my, int mz, const vvr_uniforms* __restrict__ u) {
+    __shared__ uint4 s_rec[kMajMb * kMajMb * kMajMb][2];
+    __shared__ uint8_t s_nz[kMajMb * kMajMb * kMajMb];
+    __shared__ float s_bucket[64];  // the transfer function's 64 bucket maxima, read by every brick's range
+    if (threadIdx.x < 64) s_bucket[threadIdx.x] = u->tf_bucket_max[threadIdx.x / 4][threadIdx.x % 4];
+    const int tbx = blockIdx.x * kMajTile, tby = blockIdx.y * kMajTile, tbz = blockIdx.z * kMajTile;
+    const int m0x = 2 * tbx - 1, m0y = 2 * tby - 1, m0z = 2 * tbz - 1;
+    // Every load of the staging first, then the LDS stores.
+    constexpr int kN = kMajMb * kMajMb * kMajMb * 2, kIter = (kN + kMajThreads - 1) / kMajThreads;
+    uint4 v[kIter];
+    uint32_t nzv[kIter];
+#pragma unroll
+    for (int it = 0; it < kIter; it++) {
+        const int i = threadIdx.x + it * kMajThreads;
+        v[it] = make_uint4(0xffffu, 0xffffu, 0xffffu, 0xffffu);  // no voxels: min 0xffff, max 0
+        nzv[it] = 0u;
+        if (i >= kN) continue;
+        const int r = i >> 1, h = i & 1;
+        const int mbx = m0x + r % kMajMb, mby = m0y + (r / kMajMb) % kMajMb, mbz = m0z + r / (kMajMb * kMajMb);
+        if (mbx >= 0 && mby >= 0 && mbz >= 0 && mbx < g.bx && mby < g.by && mbz < g.bz) {
+            const size_t m = size_t(mbx) + size_t(g.bx) * (size_t(mby) + size_t(g.by) * size_t(mbz));
+            v[it] = reinterpret_cast<const uint4*>(cell_mm + m * 8u)[h];
+            if (mode == 1 && h == 0) nzv[it] = cell_nz[m];
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < kIter; it++) {
+        const int i = threadIdx.x + it * kMajThreads;
+        if (i >= kN) continue;
+        s_rec[i >> 1][i & 1] = v[it];
+        if ((i & 1) == 0) s_nz[i >> 1] = uint8_t(nzv[it]);
     }
     __syncthreads();
-    if (threadIdx.x >= kTile * kTile * kTile) return;
-    const int lx = threadIdx.x % kTile, ly = (threadIdx.x / kTile) % kTile, lz = threadIdx.x / (kTile * kTile);
+    const int lb = threadIdx.x >> 2, part = threadIdx.x & 3;  // brick of the tile, quarter of its window
+    const int lx = lb % kMajTile, ly = (lb / kMajTile) % kMajTile, lz = lb / (kMajTile * kMajTile);
+    auto window = [](int i) { return i == 0 ? 2u : (i == 3 ? 1u : 3u); };  // cells inside, per axis
+    uint32_t lo = 0xffffu, hi = 0u, nz = 0u;
+    const int wz_i = part;
+    const uint32_t wz = window(wz_i);
+#pragma unroll
+    for (int wy_i = 0; wy_i < 4; wy_i++)
+#pragma unroll
+        for (int wx_i = 0; wx_i < 4; wx_i++) {
+            const uint32_t wx = window(wx_i), wy = window(wy_i);
+            uint32_t sel = 0u;
+#pragma unroll
+            for (int c = 0; c < 8; c++)
+                if (((wx >> (c & 1)) & 1u) && ((wy >> ((c >> 1) & 1)) & 1u) && ((wz >> (c >> 2)) & 1u)) sel |= 1u << c;
+            const int r = (2 * lx + wx_i) + kMajMb * ((2 * ly + wy_i) + kMajMb * (2 * lz + wz_i));
+            const uint4 a = s_rec[r][0], b = s_rec[r][1];
+            const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+            for (int c = 0; c < 8; c++)
+                if ((sel >> c) & 1u) lo = min(lo, w[c] & 0xffffu), hi = max(hi, w[c] >> 16);
+            nz |= uint32_t(s_nz[r]) & sel;
+        }
+#pragma unroll
+    for (int o = 1; o < 4; o <<= 1) {
+        lo = min(lo, uint32_t(__shfl_xor(int(lo), o)));
+        hi = max(hi, uint32_t(__shfl_xor(int(hi), o)));
+        nz |= uint32_t(__shfl_xor(int(nz), o));
+    }
     const int bx = tbx + lx, by = tby + ly, bz = tbz + lz;
-    if (bx >= mx || by >= my || bz >= mz) return;
-    uint32_t lo = 0xffffu, hi = 0u, mm = 0u;
-    for (int k = 0; k < 6; k++)
-        for (int j = 0; j < 6; j++)
-            for (int i = 0; i < 6; i++) {
-                const int c = (4 * lx + i) + kCells * ((4 * ly + j) + kCells * (4 * lz + k));
-                const uint32_t v = s_minmax[c];
-                lo = min(lo, v & 0xffffu);
-                hi = max(hi, v >> 16);
-                mm = max(mm, uint32_t(s_mask[c]));
-            }
-    float bound;
+    if (part != 0 || bx >= mx || by >= my || bz >= mz) return;
     auto tf_max = [&](uint32_t clo, uint32_t chi) {
         const int first = int((64u * clo) / 65535u), last = int((64u * chi) / 65535u);
-        float m = 0.0f;
-        for (int b = min(first, 63); b <= min(last, 63); b++) m = fmaxf(m, u->tf_bucket_max[b / 4][b % 4]);
-        return m;
+        float mv = 0.0f;
+        for (int b = min(first, 63); b <= min(last, 63); b++) mv = fmaxf(mv, s_bucket[b]);
+        return mv;
     };
-    if (mode == 0) bound = tf_max(lo, hi);
-    else if (mm == 0u) bound = 0.0f;
-    else if (mode == 1) bound = tf_max(lo, hi);
-    else bound = u->labels[2];
-    out[size_t(bx) + size_t(mx) * (size_t(by) + size_t(my) * bz)] = bound;
+    const float bound = (mode == 1 && nz == 0u) ? 0.0f : tf_max(lo, hi);
+    out[size_t(bx) + size_t(mx) * (size_t(by) + size_t(my) * size_t(bz))] = bound;
 }
 
 inline unsigned blocks_for(size_t n) { return unsigned((n + kBlock - 1) / kBlock); }
@@ -525,10 +621,19 @@ void launch_mask_blur(const uint32_t* src, uint32_t* dst, int axis, int radius, 
 }
 
 void launch_majorant(const uint16_t* density, const uint32_t* mask, float* out, const BrickGrid& g, int mode,
-                     const vvr_uniforms* u, hipStream_t st) {
+                     const vvr_uniforms* u, uint32_t* cell_mm, uint8_t* cell_nz, hipStream_t st) {
     const int mx = (g.nx + 7) / 8, my = (g.ny + 7) / 8, mz = (g.nz + 7) / 8;
-    const dim3 grid((mx + kTile - 1) / kTile, (my + kTile - 1) / kTile, (mz + kTile - 1) / kTile);
-    hipLaunchKernelGGL(k_majorant, grid, dim3(kBlock), 0, st, density, mask, out, g, mode, mx, my, mz, u);
+    const size_t nmb = size_t(g.bx) * g.by * g.bz, bricks = size_t(mx) * my * mz;
+    if (mode >= 2)
+        hipLaunchKernelGGL(k_fill_f32, dim3(blocks_for(bricks)), dim3(kBlock), 0, st, out, uint32_t(bricks), 0.0f);
+    const size_t groups = (nmb + kMajRun - 1) / kMajRun;
+    hipLaunchKernelGGL(k_majorant_cells, dim3(blocks_for(groups * 8)), dim3(kBlock), 0, st, density, mask, cell_mm,
+                       cell_nz, out, g, mode, mx, my, mz, u);
+    if (mode <= 1) {
+        const dim3 grid((mx + kMajTile - 1) / kMajTile, (my + kMajTile - 1) / kMajTile, (mz + kMajTile - 1) / kMajTile);
+        hipLaunchKernelGGL(k_majorant_bricks, grid, dim3(kMajThreads), 0, st, cell_mm, cell_nz, out, g, mode, mx, my,
+                           mz, u);
+    }
 }
 
 }  // namespace vvr

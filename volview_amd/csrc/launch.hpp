@@ -33,7 +33,7 @@ void launch_mask_blur(const uint32_t* src, uint32_t* dst, int axis, int radius, 
 
 // majorant.comp: `mask` may be null when mode is 0.
 void launch_majorant(const uint16_t* density, const uint32_t* mask, float* out, const BrickGrid& g, int mode,
-                     const vvr_uniforms* u, hipStream_t st);
+                     const vvr_uniforms* u, uint32_t* cell_mm, uint8_t* cell_nz, hipStream_t st);
 
 // dvr.comp: hdr = RGBA16F (width x height x 4 halves).
 void launch_dvr(const DevScene& s, uint16_t* hdr, uint32_t width, uint32_t height, hipStream_t st);

@@ -307,6 +307,8 @@ struct vvr_renderer {
     uint16_t* density = nullptr;
     uint32_t* normals = nullptr;
     float* majorant = nullptr;
+    uint32_t* maj_cells = nullptr;  // majorant build (modes off, tint): per micro-brick, its 2^3 cells' density
+    uint8_t* maj_nz = nullptr;      // min | max << 16, and (tint) a bit per cell: some mask value is non-zero
     uint8_t* labels = nullptr;
     uint32_t* mask = nullptr;    // image A: what rendering samples
     uint32_t* mask_b = nullptr;  // image B: blur partner
@@ -415,9 +417,13 @@ struct vvr_renderer {
             mask_key = opacity;
         }
         if (!majorant_key || *majorant_key != opacity) {
+            // Cell records of the two-pass build, allocated on the first build that needs them.
+            const size_t nmb = size_t(g.bx) * g.by * g.bz;
+            if (opacity.mode <= VVR_LABELS_TINT && !maj_cells) maj_cells = dalloc<uint32_t>(8 * nmb, "majorant cells");
+            if (opacity.mode == VVR_LABELS_TINT && !maj_nz) maj_nz = dalloc<uint8_t>(nmb, "majorant mask cells");
             run(VVR_KERNEL_MAJORANT, [&] {
                 launch_majorant(density, opacity.mode != VVR_LABELS_OFF ? mask : nullptr, majorant, g, int(opacity.mode),
-                                du, st);
+                                du, maj_cells, maj_nz, st);
             });
             majorant_key = opacity;
         }
@@ -532,6 +538,7 @@ struct vvr_renderer {
         free_frame();
         wf.reset();
         dfree(density), dfree(normals), dfree(majorant), dfree(labels), dfree(mask), dfree(mask_b), dfree(soft);
+        dfree(maj_cells), dfree(maj_nz);
         dfree(shadow), dfree(ao), dfree(env_rad), dfree(env_table), dfree(env_dif);
         dfree(lut), dfree(preint), dfree(d_uniforms[0]), dfree(d_uniforms[1]), dfree(exposure);
         if (h_uniforms) (void)hipHostFree(h_uniforms);
