@@ -236,7 +236,8 @@ def run_batched(ctx) -> dict:
             "speedup": round(t_plain / t_served, 2)}
 
 
-def run_serving(ctx, seconds: float, grace: float = 5.0, increment: int = 16, slots: int = 1 << 20) -> dict:
+def run_serving(ctx, seconds: float, grace: float = 5.0, increment: int = 16, slots: int = 1 << 20,
+                iterations: int = 0, plain_side: bool = True) -> dict:
     """The serving trace in real time, twice: by a plain loop (one frame at a
     time, record + synchronize: ray-marched requests first, then one increment
     of the oldest progressive request) and by the server (all at once). A new
@@ -284,66 +285,67 @@ def run_serving(ctx, seconds: float, grace: float = 5.0, increment: int = 16, sl
         f"{sum(q.path_traced for q in reqs)} path-traced")
 
     # ---- plain ----
-    from collections import deque
-    dvr_lat, pt_first, pt_paths, frame, useful = [], [], 0, 0, 0
-    dvr_q, pt_fifo, active = deque(), deque(), {}
-    i, t0 = 0, time.perf_counter()
-    while True:
-        now = time.perf_counter() - t0
-        if now >= seconds + grace:
-            break
-        while i < len(reqs) and reqs[i].arrival <= now:
-            q = reqs[i]
-            if q.path_traced:
-                if q.volume not in active:
-                    pt_fifo.append(q.volume)
+    plain, window = None, seconds + grace
+    if plain_side:
+        from collections import deque
+        dvr_lat, pt_first, pt_paths, frame, useful = [], [], 0, 0, 0
+        dvr_q, pt_fifo, active = deque(), deque(), {}
+        i, t0 = 0, time.perf_counter()
+        while True:
+            now = time.perf_counter() - t0
+            if now >= seconds + grace:
+                break
+            while i < len(reqs) and reqs[i].arrival <= now:
+                q = reqs[i]
+                if q.path_traced:
+                    if q.volume not in active:
+                        pt_fifo.append(q.volume)
+                    else:
+                        old, old_done = active[q.volume]
+                        useful += old_done
+                        if old_done == 0:  # replaced before its first frame
+                            pt_first.append(q.arrival - reqs[old].arrival)
+                    active[q.volume] = [i, 0]
                 else:
-                    old, old_done = active[q.volume]
-                    useful += old_done
-                    if old_done == 0:  # replaced before its first frame
-                        pt_first.append(q.arrival - reqs[old].arrival)
-                active[q.volume] = [i, 0]
+                    dvr_q.append(i)
+                i += 1
+            frame += 1
+            if dvr_q:
+                k = dvr_q.popleft()
+                r = dvr_r[reqs[k].volume]
+                r.record(frame, plain_params[k])
+                r.synchronize()
+                dvr_lat.append(time.perf_counter() - t0 - reqs[k].arrival)
+            elif pt_fifo:
+                v = pt_fifo.popleft()
+                k, done = active[v]
+                pt_r[v].record(frame, plain_params[k])
+                pt_r[v].synchronize()
+                done += increment
+                pt_paths += 1024 * 1024 * increment
+                if done == increment:
+                    pt_first.append(time.perf_counter() - t0 - reqs[k].arrival)
+                if done < reqs[k].spp:
+                    active[v][1] = done
+                    pt_fifo.append(v)
+                else:
+                    useful += done
+                    del active[v]
+            elif i < len(reqs):
+                time.sleep(max(0.0, reqs[i].arrival - now))
             else:
-                dvr_q.append(i)
-            i += 1
-        frame += 1
-        if dvr_q:
-            k = dvr_q.popleft()
-            r = dvr_r[reqs[k].volume]
-            r.record(frame, plain_params[k])
-            r.synchronize()
-            dvr_lat.append(time.perf_counter() - t0 - reqs[k].arrival)
-        elif pt_fifo:
-            v = pt_fifo.popleft()
-            k, done = active[v]
-            pt_r[v].record(frame, plain_params[k])
-            pt_r[v].synchronize()
-            done += increment
-            pt_paths += 1024 * 1024 * increment
-            if done == increment:
-                pt_first.append(time.perf_counter() - t0 - reqs[k].arrival)
-            if done < reqs[k].spp:
-                active[v][1] = done
-                pt_fifo.append(v)
-            else:
-                useful += done
-                del active[v]
-        elif i < len(reqs):
-            time.sleep(max(0.0, reqs[i].arrival - now))
-        else:
-            break
-    span = time.perf_counter() - t0
-    dvr_lat += [span - reqs[k].arrival for k in dvr_q]  # still waiting
-    pt_first += [span - reqs[k].arrival for k, done in active.values() if done == 0]
-    useful += sum(done for _, done in active.values())
-    window = seconds + grace
-    plain = {"dvr_latency_ms": percentiles(dvr_lat), "pt_first_frame_ms": percentiles(pt_first),
-             "pt_msamples_s": round(pt_paths / window / 1e6, 2), "pt_paths": pt_paths,
-             "pt_useful_msamples_s": round(useful * 1024 * 1024 / window / 1e6, 2), "busy_seconds": round(span, 2)}
-    log(f"serving plain: {plain}")
+                break
+        span = time.perf_counter() - t0
+        dvr_lat += [span - reqs[k].arrival for k in dvr_q]  # still waiting
+        pt_first += [span - reqs[k].arrival for k, done in active.values() if done == 0]
+        useful += sum(done for _, done in active.values())
+        plain = {"dvr_latency_ms": percentiles(dvr_lat), "pt_first_frame_ms": percentiles(pt_first),
+                 "pt_msamples_s": round(pt_paths / window / 1e6, 2), "pt_paths": pt_paths,
+                 "pt_useful_msamples_s": round(useful * 1024 * 1024 / window / 1e6, 2), "busy_seconds": round(span, 2)}
+        log(f"serving plain: {plain}")
 
     # ---- served ----
-    srv = Server(ctx, pt_increment=increment, slots=slots)
+    srv = Server(ctx, pt_increment=increment, slots=slots, iterations=iterations)
     ids, kind, current, finals = {}, {}, {}, set()
     first, dvr_done, pt_paths = {}, {}, 0
     i, t0, t0_ns = 0, time.perf_counter(), time.monotonic_ns()  # the library stamps frames on the same clock
@@ -603,6 +605,8 @@ def main() -> int:
     ap.add_argument("--only", choices=["dvr", "denoisers"], default=None,
                     help="run only this part (for profiling): no PT-1 steps and no JSON line")
     ap.add_argument("--serve-slots", type=int, default=1 << 20, help="path slots of the server in the trace")
+    ap.add_argument("--serve-iterations", type=int, default=0,
+                    help="wavefront iterations per pass of the server in the trace (0: its default)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB mebibytes device to device (a plain copy kernel whose HBM counters "
                          "scale the path tracer's under rocprofv3 --pmc)")
@@ -699,7 +703,8 @@ def main() -> int:
     if not args.no_batched:
         out["batched"] = run_batched(ctx)
     if args.serve_seconds > 0:
-        out["serving"] = run_serving(ctx, args.serve_seconds, slots=args.serve_slots)
+        out["serving"] = run_serving(ctx, args.serve_seconds, slots=args.serve_slots,
+                                     iterations=args.serve_iterations)
     if not args.no_quality:
         out["served_quality"] = served_quality(ctx)
     print(json.dumps(out), flush=True)

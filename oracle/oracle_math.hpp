@@ -44,8 +44,9 @@ template <class R> inline R clampr(R x, R lo, R hi) { return gmin(gmax(x, lo), h
 //    [sqrt(1/2), sqrt(2)), log(m) = 2 atanh(s) with s = (m - 1) / (m + 1) by its
 //    series to s^9 (<= 2 ulp); exp by reduction to r = x - k ln 2 and the Taylor
 //    polynomial of e^r to r^7 (<= 1 ulp), scaled by two exact powers of two;
-//  - pow, sin, cos, acos, atan2: evaluated in double and rounded once (the
-//    correctly rounded value, barring ties closer than a double ulp).
+//  - pow(x, y) = exp(y log x) with those two (GLSL defines it through exp2 and log2);
+//  - sin, cos, acos, atan2: evaluated in double and rounded once (the correctly
+//    rounded value, barring ties closer than a double ulp).
 // The FP64 oracle uses the double functions throughout.
 inline float f32_from_bits(uint32_t u) {
     float f;
@@ -100,7 +101,10 @@ template <class R> inline R vexp(R x) {
     if constexpr (std::is_same_v<R, float>) return vexp_f32(x);
     else return std::exp(x);
 }
-template <class R> inline R cr_pow(R x, R y) { return R(std::pow(double(x), double(y))); }
+template <class R> inline R vpow(R x, R y) {  // GLSL's exp2(y log2 x), as exp(y log x)
+    if constexpr (std::is_same_v<R, float>) return vexp_f32(y * vlog_f32(x));
+    else return std::pow(x, y);
+}
 template <class R> inline R cr_sin(R x) { return R(std::sin(double(x))); }
 template <class R> inline R cr_cos(R x) { return R(std::cos(double(x))); }
 template <class R> inline R cr_acos(R x) { return R(std::acos(double(x))); }

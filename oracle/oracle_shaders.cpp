@@ -445,7 +445,7 @@ template <class R> struct Ctx {
         R k = a * R(0.5);
         R g = (nv / (nv * (R(1) - k) + k)) * (nl / (nl * (R(1) - k) + k));
         V3<R> f0 = vmix(V3<R>(R(0.028)), albedo, metalness);
-        V3<R> f = f0 + (R(1) - f0) * cr_pow(R(1) - vh, R(5));
+        V3<R> f = f0 + (R(1) - f0) * vpow(R(1) - vh, R(5));
         V3<R> specular = d * g * f / (R(4) * nv * nl);
         diffuse = (R(1) - f) * (R(1) - metalness) * albedo / PI();
         return specular;
@@ -462,7 +462,7 @@ template <class R> struct Ctx {
         R nv = gmax(dot(n, v), R(1e-4));
         R nl = gmax(dot(n, l), R(1e-4));
         R a = coat_roughness(s_) * coat_roughness(s_);
-        R f = s_.coat * (R(0.02) + R(0.98) * cr_pow(R(1) - gmax(dot(v, h), R(0)), R(5)));
+        R f = s_.coat * (R(0.02) + R(0.98) * vpow(R(1) - gmax(dot(v, h), R(0)), R(5)));
         R coat = ggx_distribution(gmax(dot(n, h), R(0)), a * a) * smith(nv, nl, a) * f / (R(4) * nv * nl);
         diffuse = diffuse * (R(1) - f);
         return specular * (R(1) - f) + V3<R>(coat);
@@ -513,7 +513,7 @@ template <class R> struct Ctx {
             R g_len = length(g);
             if (!skin || n_len <= R(1e-3)) n = g_len > R(1e-5) ? (-g) / g_len : n;
             surfaceness = clampr(g_len * R(4), R(0), R(1));
-            R edge_on = cr_pow(R(1) - std::fabs(dot(n, v)), R(3));
+            R edge_on = vpow(R(1) - std::fabs(dot(n, v)), R(3));
             return surfaceness * mixr(R(u->labels[3]), R(1), edge_on);
         }
         return R(1);
@@ -922,8 +922,8 @@ template <class R> struct Ctx {
             R key_side = dot(V3<R>(u->light_dir[0], u->light_dir[1], u->light_dir[2]), r) >= R(0) ? R(1) : R(-1);
             V3<R> fill = normalize(((-f) - (R(0.9) * key_side) * r) - R(0.1) * up);
             V3<R> rim = normalize((f + R(0.8) * up) - (R(0.4) * key_side) * r);
-            R fill_lobe = R(0.5) * cr_pow(gmax(dot(d, fill), R(0)), R(6));
-            R rim_lobe = R(2.5) * cr_pow(gmax(dot(d, rim), R(0)), R(24));
+            R fill_lobe = R(0.5) * vpow(gmax(dot(d, fill), R(0)), R(6));
+            R rim_lobe = R(2.5) * vpow(gmax(dot(d, rim), R(0)), R(24));
             V3<R> studio = V3<R>(R(0.02)) + fill_lobe * V3<R>(R(0.95), R(0.98), R(1.05)) +
                            rim_lobe * V3<R>(R(0.9), R(0.95), R(1.1));
             return amb * studio * (R(1) / R(0.106));

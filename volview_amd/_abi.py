@@ -102,7 +102,7 @@ assert ctypes.sizeof(Uniforms) == 21200
 # (name, restype, argtypes) of every exported entry point of include/vvr.h.
 class ServerConfig(Structure):
     _fields_ = [("hbm_budget", c_uint64), ("slots", c_uint32), ("pt_increment", c_uint32), ("max_requests", c_uint32),
-                ("flight_steps", c_uint32), ("shadow_steps", c_uint32)]
+                ("flight_steps", c_uint32), ("shadow_steps", c_uint32), ("iterations", c_uint32)]
 
 
 class PtShape(Structure):

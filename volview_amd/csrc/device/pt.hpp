@@ -131,8 +131,8 @@ __device__ inline V3 environment(const DevScene& s, V3 d) {
         const float key_side = dot(V3(u->light_dir[0], u->light_dir[1], u->light_dir[2]), r) >= 0.0f ? 1.0f : -1.0f;
         const V3 fill = normalize(((-f) - (0.9f * key_side) * r) - 0.1f * up);
         const V3 rim = normalize((f + 0.8f * up) - (0.4f * key_side) * r);
-        const float fill_lobe = 0.5f * cr_pow(gmax(dot(d, fill), 0.0f), 6.0f);
-        const float rim_lobe = 2.5f * cr_pow(gmax(dot(d, rim), 0.0f), 24.0f);
+        const float fill_lobe = 0.5f * vpow(gmax(dot(d, fill), 0.0f), 6.0f);
+        const float rim_lobe = 2.5f * vpow(gmax(dot(d, rim), 0.0f), 24.0f);
         const V3 studio = V3(0.02f) + fill_lobe * V3(0.95f, 0.98f, 1.05f) + rim_lobe * V3(0.9f, 0.95f, 1.1f);
         return amb * studio * (1.0f / 0.106f);
     }

@@ -22,7 +22,7 @@ __device__ __forceinline__ V3 brdf(V3 n, V3 v, V3 l, V3 albedo, float roughness,
     const float k = a * 0.5f;
     const float g = (nv / (nv * (1.0f - k) + k)) * (nl / (nl * (1.0f - k) + k));
     const V3 f0 = vmix(V3(0.028f), albedo, metalness);
-    const V3 f = f0 + (1.0f - f0) * cr_pow(1.0f - vh, 5.0f);
+    const V3 f = f0 + (1.0f - f0) * vpow(1.0f - vh, 5.0f);
     const V3 specular = d * g * f / (4.0f * nv * nl);
     diffuse = (1.0f - f) * (1.0f - metalness) * albedo / kPi;
     return specular;
@@ -43,7 +43,7 @@ __device__ __forceinline__ V3 brdf_coated(V3 n, V3 v, V3 l, const Sample& s, V3&
     const float nv = gmax(dot(n, v), 1e-4f);
     const float nl = gmax(dot(n, l), 1e-4f);
     const float a = coat_roughness(s) * coat_roughness(s);
-    const float f = s.coat * (0.02f + 0.98f * cr_pow(1.0f - gmax(dot(v, h), 0.0f), 5.0f));
+    const float f = s.coat * (0.02f + 0.98f * vpow(1.0f - gmax(dot(v, h), 0.0f), 5.0f));
     const float coat = ggx_distribution(gmax(dot(n, h), 0.0f), a * a) * smith(nv, nl, a) * f / (4.0f * nv * nl);
     diffuse = diffuse * (1.0f - f);
     return specular * (1.0f - f) + V3(coat);
@@ -127,7 +127,7 @@ __device__ inline float surface_frame(const DevScene& sc, V3 idx, const Sample& 
         const float g_len = length(g);
         if (!skin || n_len <= 1e-3f) n = g_len > 1e-5f ? (-g) / g_len : n;
         surfaceness = clampf(g_len * 4.0f, 0.0f, 1.0f);
-        const float edge_on = cr_pow(1.0f - fabsf(dot(n, v)), 3.0f);
+        const float edge_on = vpow(1.0f - fabsf(dot(n, v)), 3.0f);
         return surfaceness * mixf(u->labels[3], 1.0f, edge_on);
     }
     return 1.0f;

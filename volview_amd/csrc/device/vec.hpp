@@ -44,8 +44,9 @@ __device__ __forceinline__ float fractf(float x) { return x - floorf(x); }
 // [sqrt(1/2), sqrt(2)) with log(m) = 2 atanh(s), s = (m - 1) / (m + 1), by its
 // series to s^9 (<= 2 ulp); exp by reduction to r = x - k ln 2, |r| <= ln 2 / 2,
 // and its Taylor polynomial to r^7 (<= 1 ulp), scaled by two exact powers of two.
-// The others are evaluated in double and rounded once to float: the correctly
-// rounded value except within a double ulp of a float rounding boundary.
+// pow is exp(y log x) with those two. sin, cos, acos and atan2 are evaluated in
+// double and rounded once to float: the correctly rounded value except within a
+// double ulp of a float rounding boundary.
 constexpr float kLn2Hi = 0.693145751953125f;  // ln 2 with 11 trailing zero bits: k ln2_hi is exact
 constexpr float kLn2Lo = 1.42860677e-06f;     // ln 2 - kLn2Hi
 
@@ -81,7 +82,8 @@ __device__ __forceinline__ float vexp(float x) {
     return p * __uint_as_float(uint32_t(k1 + 127) << 23) * __uint_as_float(uint32_t(k2 + 127) << 23);
 }
 
-__device__ __forceinline__ float cr_pow(float x, float y) { return float(pow(double(x), double(y))); }
+// GLSL defines pow(x, y) as exp2(y log2(x)); here exp(y log(x)) with the two above.
+__device__ __forceinline__ float vpow(float x, float y) { return vexp(y * vlog(x)); }
 __device__ __forceinline__ float cr_sin(float x) { return float(sin(double(x))); }
 __device__ __forceinline__ float cr_cos(float x) { return float(cos(double(x))); }
 __device__ __forceinline__ float cr_acos(float x) { return float(acos(double(x))); }

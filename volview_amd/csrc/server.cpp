@@ -37,7 +37,6 @@ namespace {
 constexpr uint32_t kTable = 256;    // request-table entries (path-traced increments in flight)
 static_assert(kTable <= kWfBins, "the queue sort bins paths by request-table entry");
 constexpr uint32_t kMaxJobs = 256;  // requests admitted at once
-constexpr uint32_t kBatch = 2;      // wavefront iterations per pass
 constexpr uint32_t kDvrRing = 8;    // batched ray-marching job tables in flight
 constexpr uint32_t kInFlight = 2;   // increments of one request in flight
 
@@ -445,7 +444,7 @@ void vvr_server::pt_batch() {
     bool env_map = false;
     for (uint32_t e = 0; e < kTable && !env_map; e++)
         env_map = table[e].job && !table[e].complete && h_uniforms[table[e].job->uslot].lighting[3] == 2u;
-    for (uint32_t it = 0; it < kBatch; it++) {
+    for (uint32_t it = 0; it < cfg.iterations; it++) {
         V.idle_in = idle_in;
         V.revive = revive ? 1u : 0u;
         if (revive)
@@ -458,7 +457,7 @@ void vvr_server::pt_batch() {
     }
     batch_seq++;
     std::lock_guard<std::mutex> lk(mu);
-    stats.iterations += kBatch;
+    stats.iterations += cfg.iterations;
 }
 
 // The counters have landed: increments that handed out all their items leave
@@ -683,6 +682,8 @@ void create_server(vvr_ctx* ctx, const vvr_server_config* cfg, vvr_server** out)
     vvr_server_config c = cfg ? *cfg : vvr_server_config{};
     if (!c.hbm_budget) c.hbm_budget = uint64_t(64) << 30;
     if (!c.max_requests || c.max_requests > kMaxJobs) c.max_requests = kMaxJobs;
+    if (!c.iterations) c.iterations = 2;
+    if (c.iterations > 64) throw Error{VVR_EINVAL, "iterations per pass must be 1..64"};
     s->cfg = c;
     s->engine.set_shape(vvr_pt_shape{c.slots, c.flight_steps, c.shadow_steps});
     check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");

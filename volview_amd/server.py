@@ -25,10 +25,10 @@ from .renderer import Context, Renderer
 
 class Server:
     def __init__(self, ctx: Context, hbm_budget: int = 0, slots: int = 0, pt_increment: int = 0,
-                 max_requests: int = 0, flight_steps: int = 0, shadow_steps: int = 0):
+                 max_requests: int = 0, flight_steps: int = 0, shadow_steps: int = 0, iterations: int = 0):
         self._lib = load()
         self.ctx = ctx
-        cfg = ServerConfig(hbm_budget, slots, pt_increment, max_requests, flight_steps, shadow_steps)
+        cfg = ServerConfig(hbm_budget, slots, pt_increment, max_requests, flight_steps, shadow_steps, iterations)
         self._held = {}  # request id -> its renderer, kept alive while the server may read it
         self._dims = {}  # request id -> (width, height) of its frames
         h = c_void_p()
