@@ -236,7 +236,7 @@ def run_batched(ctx) -> dict:
             "speedup": round(t_plain / t_served, 2)}
 
 
-def run_serving(ctx, seconds: float, grace: float = 5.0, increment: int = 16, slots: int = 1 << 20,
+def run_serving(ctx, seconds: float, grace: float = 5.0, increment: int = 16, slots: int = 2 << 20,
                 iterations: int = 0, plain_side: bool = True) -> dict:
     """The serving trace in real time, twice: by a plain loop (one frame at a
     time, record + synchronize: ray-marched requests first, then one increment
@@ -349,7 +349,7 @@ def run_serving(ctx, seconds: float, grace: float = 5.0, increment: int = 16, sl
     ids, kind, current, finals = {}, {}, {}, set()
     first, dvr_done, pt_paths = {}, {}, 0
     i, t0, t0_ns = 0, time.perf_counter(), time.monotonic_ns()  # the library stamps frames on the same clock
-    arrival, replaced, frames, history, replaced_at = {}, {}, {}, {}, {}
+    arrival, replaced, frames, history, replaced_at, admitted = {}, {}, {}, {}, {}, {}
 
     def take(cs):
         nonlocal pt_paths
@@ -360,6 +360,7 @@ def run_serving(ctx, seconds: float, grace: float = 5.0, increment: int = 16, sl
                 continue
             at = (c.frame_ns - t0_ns) / 1e9 - arrival[c.id]  # from the scheduled arrival, as for plain
             if kind[c.id]:
+                admitted.setdefault(c.id, (c.admit_ns - c.submit_ns) / 1e9)
                 if c.spp > frames.get(c.id, (0, 0))[1]:
                     pt_paths += 1024 * 1024 * (c.spp - frames.get(c.id, (0, 0))[1])
                     frames[c.id] = ((c.frame_ns - t0_ns) / 1e9, c.spp)
@@ -402,7 +403,9 @@ def run_serving(ctx, seconds: float, grace: float = 5.0, increment: int = 16, sl
     served = {"dvr_latency_ms": percentiles(dvr_lat), "pt_first_frame_ms": percentiles(pt_first),
               "pt_msamples_s": round(pt_paths / window / 1e6, 2), "pt_paths": pt_paths,
               "pt_useful_msamples_s": round(useful * 1024 * 1024 / window / 1e6, 2),
-              "busy_seconds": round(span, 2), "slots": srv_slots, "hbm_peak_bytes": stats["hbm_peak"]}
+              "busy_seconds": round(span, 2), "slots": srv_slots, "hbm_peak_bytes": stats["hbm_peak"],
+              "pt_paths_abandoned": stats["pt_paths_abandoned"],
+              "pt_admission_ms": percentiles(list(admitted.values()))}
     log(f"serving served: {served}")
     for r in pt_r + dvr_r:
         r.close()
@@ -604,7 +607,7 @@ def main() -> int:
                     help="skip the served frame's relative RMSE against a 4096-path frame")
     ap.add_argument("--only", choices=["dvr", "denoisers"], default=None,
                     help="run only this part (for profiling): no PT-1 steps and no JSON line")
-    ap.add_argument("--serve-slots", type=int, default=1 << 20, help="path slots of the server in the trace")
+    ap.add_argument("--serve-slots", type=int, default=2 << 20, help="path slots of the server in the trace")
     ap.add_argument("--serve-iterations", type=int, default=0,
                     help="wavefront iterations per pass of the server in the trace (0: its default)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",

@@ -435,10 +435,12 @@ typedef struct vvr_completion {
 
 typedef struct vvr_server_stats {
     uint64_t submitted, completed, frames;
-    uint64_t pt_paths;       /* paths traced */
+    uint64_t pt_paths;       /* paths traced into delivered frames */
     uint64_t iterations;     /* wavefront iterations */
     uint64_t hbm_in_use;     /* bytes charged to admitted requests */
     uint64_t hbm_peak;
+    uint64_t pt_paths_abandoned; /* paths begun for increments of cancelled requests that never resolved
+                                    (traced to their ends, or dropped in flight at the cancellation) */
 } vvr_server_stats;
 
 int vvr_server_create(vvr_ctx* ctx, const vvr_server_config* config, vvr_server** out);

@@ -122,7 +122,8 @@ class Completion(Structure):
 
 class ServerStats(Structure):
     _fields_ = [("submitted", c_uint64), ("completed", c_uint64), ("frames", c_uint64), ("pt_paths", c_uint64),
-                ("iterations", c_uint64), ("hbm_in_use", c_uint64), ("hbm_peak", c_uint64)]
+                ("iterations", c_uint64), ("hbm_in_use", c_uint64), ("hbm_peak", c_uint64),
+                ("pt_paths_abandoned", c_uint64)]
 
 
 class UnetConfig(Structure):

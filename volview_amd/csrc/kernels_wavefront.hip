@@ -219,7 +219,8 @@ __device__ __forceinline__ void clear_bins(const WfQueues& Q) {
 // Counting sort of a queue by the request of each slot's path, so that the
 // stages' waves mostly hold paths of one request (each distinct request in a
 // wave costs the stage one more pass). The order within a request is
-// arbitrary: no result depends on queue order. Pass 1: requests' counts.
+// arbitrary: no result depends on queue order. kNone entries (slots a served
+// tail took) are left out. Pass 1: requests' counts.
 __global__ void __launch_bounds__(256) k_wf_bin_count(const uint32_t* __restrict__ in,
                                                       const uint32_t* __restrict__ count,
                                                       const uint16_t* __restrict__ req, uint32_t* __restrict__ hist) {
@@ -227,7 +228,10 @@ __global__ void __launch_bounds__(256) k_wf_bin_count(const uint32_t* __restrict
     h[threadIdx.x] = 0u;
     __syncthreads();
     const uint32_t n = *count;
-    for (uint32_t q = blockIdx.x * 256u + threadIdx.x; q < n; q += gridDim.x * 256u) atomicAdd(&h[req[in[q]] & (kWfBins - 1u)], 1u);
+    for (uint32_t q = blockIdx.x * 256u + threadIdx.x; q < n; q += gridDim.x * 256u) {
+        const uint32_t slot = in[q];
+        if (slot != kNone) atomicAdd(&h[req[slot] & (kWfBins - 1u)], 1u);
+    }
     __syncthreads();
     if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
 }
@@ -270,8 +274,8 @@ __global__ void __launch_bounds__(256) k_wf_bin_scatter(const uint32_t* __restri
 #pragma unroll
         for (uint32_t j = 0; j < kSortPer; j++) {
             const uint32_t q = c + j * 256u + t;
-            if (q < n) {
-                slot[j] = in[q];
+            slot[j] = q < n ? in[q] : kNone;
+            if (slot[j] != kNone) {
                 b[j] = req[slot[j]] & (kWfBins - 1u);
                 rank[j] = atomicAdd(&h[b[j]], 1u);
             }
@@ -281,7 +285,7 @@ __global__ void __launch_bounds__(256) k_wf_bin_scatter(const uint32_t* __restri
         __syncthreads();
 #pragma unroll
         for (uint32_t j = 0; j < kSortPer; j++)
-            if (c + j * 256u + t < n) out[run[b[j]] + rank[j]] = slot[j];
+            if (slot[j] != kNone) out[run[b[j]] + rank[j]] = slot[j];
         __syncthreads();
     }
 }
@@ -711,95 +715,172 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     }
 }
 
-// The tail of a single-request frame, once every work item is handed out and
-// few paths remain: each path still in flight runs to its end in one launch,
-// a lane per slot, then the rest of its chunk. The stages run in the order the
-// iterations would run them, with the same functions, so every path draws the
-// same random numbers and the chunk sums get the same additions in the same
-// order: the frame is the same as with iterations to the end. Entries: the
-// ended paths of end[cur] (regen's turn), the suspended shadow rays of sh[cur]
-// and the flights of ff[cur] (new or suspended).
-__global__ void __launch_bounds__(64) k_wf_finish(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
-                                                  uint32_t cur) {
-    const PtRequest& R = reqs[0];
+// Where a path waits between two iterations: a flight to begin or resume
+// (ff), a suspended shadow ray (sh) or its end (end, for regen).
+enum TailPhase : uint32_t { kFlight, kShadow, kScatter, kEnded };
+
+// Runs the path in `slot`, waiting at `phase`, to its end, then the rest of its
+// chunk, in one lane: the stages run in the order the iterations would run
+// them, with the same functions, so the path draws the same random numbers and
+// the chunk sums get the same additions in the same order as with iterations
+// to the end. The chunk's sums are stored and the slot is left idle. False for
+// an idle slot.
+__device__ __forceinline__ bool run_to_end(const PtRequest& R, const WfSlots& S, uint32_t slot, uint32_t phase) {
     const DevScene& sc = R.scene;
     const FrameConsts fc = frame_consts(sc.u);
+    PathState st;
+    SlotRef ref;
+    load_path(S, slot, st, ref);
+    if (ref.item == kNone) return false;  // an idle slot
+    float4 c0 = S.c0[slot], c1 = S.c1[slot], c2 = S.c2[slot];
+    bool resumed = phase == kFlight && S.p_flight[slot].y != kWfNewFlight;
+    EventState ev;
+    if (phase == kShadow) {  // as k_wf_shadow, without a bound
+        load_event(S, slot, ev);
+        ev.n = xyz(S.e_n[slot]);
+        ev.light_dir = xyz(S.e_light[slot]);
+        V3 result = xyz(S.e_lt[slot]);
+        const uint2 fl = S.p_flight[slot];
+        Flight f = flight_begin(sc.u, ev.idx, ev.light_dir);
+        f.tt = bitsf(fl.x), f.i = int(fl.y);
+        transmittance_steps(sc, f, result, st.rng, 4096);
+        ev.light_t = result;
+        phase = kScatter;
+    }
+    for (;;) {
+        if (phase == kFlight) {  // k_wf_extend, then k_wf_shade and k_wf_shadow
+            Flight f = flight_begin(sc.u, st.origin, st.dir);
+            if (resumed) {
+                const uint2 fl = S.p_flight[slot];
+                f.tt = bitsf(fl.x), f.i = int(fl.y);
+                resumed = false;
+            }
+            if (stage_extend_steps(sc, fc, st, ev, f, 4096) == kFlowHit) {
+                stage_shade(sc, fc, st, ev);
+                if (st.bounce == 0u) {
+                    c1 = make_float4(c1.x + st.albedo.x, c1.y + st.albedo.y, c1.z + st.albedo.z, c1.w + 0.0f);
+                    c2 = make_float4(c2.x + st.normal.x, c2.y + st.normal.y, c2.z + st.normal.z, c2.w + st.depth);
+                }
+                stage_shadow(sc, st, ev);
+                phase = kScatter;
+            } else {
+                if (st.bounce == 0u) {  // background: pt.comp:434-438
+                    c0 = make_float4(c0.x + 0.f, c0.y + 0.f, c0.z + 0.f, c0.w + 1.0f);
+                    c1 = make_float4(c1.x + 1.0f, c1.y + 1.0f, c1.z + 1.0f, c1.w + 0.f);
+                    c2 = make_float4(c2.x + 0.f, c2.y + 0.f, c2.z + 0.f, c2.w + 1e4f);
+                }
+                phase = kEnded;
+            }
+        }
+        if (phase == kScatter) {  // k_wf_shadow_map, k_wf_scatter
+            ev.map_pdf = 0.0f;
+            if (fc.env_map) stage_shadow_map(sc, st, ev);
+            phase = stage_scatter(sc, fc, st, ev) && path_continues(fc, st) ? kFlight : kEnded;
+            continue;
+        }
+        // k_wf_regen: the path's radiance into its chunk's sums (pt.comp:546-551),
+        // then the chunk's next path or its partial sums.
+        const V3 r = path_end(st);
+        const float lum = dot(r, V3(0.2126f, 0.7152f, 0.0722f));
+        c0 = make_float4(c0.x + r.x, c0.y + r.y, c0.z + r.z, c0.w);
+        c1.w += lum * lum;
+        ref.k++;
+        uint32_t px, py, ci, pi, path_index, n_paths;
+        item_coords(R, ref.item, ref.k, px, py, ci, pi, path_index, n_paths);
+        if (ref.k >= n_paths) {
+            store_partial(R, ci, pi, c0, c1, c2);
+            break;
+        }
+        path_begin(sc, fc, px, py, path_index, st);
+        phase = kFlight;
+    }
+    S.p_spec[slot] = make_float4(0.f, 0.f, 0.f, bitsf(kNone));  // the slot is idle
+    return true;
+}
+
+// The tail of a single-request frame, once every work item is handed out and
+// few paths remain: each path still in flight runs to its end in one launch,
+// a lane per slot. Entries: the ended paths of end[cur] (regen's turn), the
+// suspended shadow rays of sh[cur] and the flights of ff[cur] (new or suspended).
+__global__ void __launch_bounds__(64) k_wf_finish(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
+                                                  uint32_t cur) {
     const uint32_t n_end = Q.counts[kWfEND0 + cur], n_sh = Q.counts[kWfSH0 + cur];
     const uint32_t n = n_end + n_sh + Q.counts[kWfFF0 + cur];
-    enum { kFlight, kShadow, kScatter, kEnded };
     for (uint32_t q = blockIdx.x * 64u + __lane_id(); q < n; q += gridDim.x * 64u) {
-        int phase;
-        uint32_t slot;
+        uint32_t phase, slot;
         if (q < n_end) slot = Q.end[cur][q], phase = kEnded;
         else if (q < n_end + n_sh) slot = Q.sh[cur][q - n_end], phase = kShadow;
         else slot = Q.ff[cur][q - n_end - n_sh], phase = kFlight;
-        PathState st;
-        SlotRef ref;
-        load_path(S, slot, st, ref);
-        if (ref.item == kNone) continue;  // an idle slot
-        float4 c0 = S.c0[slot], c1 = S.c1[slot], c2 = S.c2[slot];
-        bool resumed = phase == kFlight && S.p_flight[slot].y != kWfNewFlight;
-        EventState ev;
-        if (phase == kShadow) {  // as k_wf_shadow, without a bound
-            load_event(S, slot, ev);
-            ev.n = xyz(S.e_n[slot]);
-            ev.light_dir = xyz(S.e_light[slot]);
-            V3 result = xyz(S.e_lt[slot]);
-            const uint2 fl = S.p_flight[slot];
-            Flight f = flight_begin(sc.u, ev.idx, ev.light_dir);
-            f.tt = bitsf(fl.x), f.i = int(fl.y);
-            transmittance_steps(sc, f, result, st.rng, 4096);
-            ev.light_t = result;
-            phase = kScatter;
+        run_to_end(reqs[0], S, slot, phase);
+    }
+}
+
+// Served engine, between two iterations: the paths of request-table entry `e`
+// leave the wavefront. Its flights on ff[cur] and suspended shadow rays on
+// sh[cur] have their queue entries set to kNone (padding, which the stages and
+// the queue sort skip); its ended paths on end[cur] stay for regen. Without
+// `drop`, up to `cap` of them are listed for k_wf_tail (slot | phase << 30;
+// beyond `cap` they stay queued). With `drop` (a cancelled request), their
+// slots are left idle on end[cur] and their work items counted in `done`.
+__global__ void __launch_bounds__(256) k_wf_tail_gather(WfSlots S, WfQueues Q, uint32_t cur, uint32_t e, bool drop,
+                                                        uint32_t* __restrict__ list, uint32_t* __restrict__ count,
+                                                        uint32_t cap, uint32_t* __restrict__ done) {
+    const uint32_t n_sh = Q.counts[kWfSH0 + cur];
+    const uint32_t n = n_sh + Q.counts[kWfFF0 + cur];
+    const uint32_t lane = __lane_id();
+    for (uint32_t base = blockIdx.x * 256u + (threadIdx.x & ~63u); base < n; base += gridDim.x * 256u) {
+        const uint32_t q = base + lane;
+        uint32_t* at = nullptr;
+        uint32_t slot = kNone;
+        if (q < n) {
+            at = q < n_sh ? &Q.sh[cur][q] : &Q.ff[cur][q - n_sh];
+            slot = *at;
         }
-        for (;;) {
-            if (phase == kFlight) {  // k_wf_extend, then k_wf_shade and k_wf_shadow
-                Flight f = flight_begin(sc.u, st.origin, st.dir);
-                if (resumed) {
-                    const uint2 fl = S.p_flight[slot];
-                    f.tt = bitsf(fl.x), f.i = int(fl.y);
-                    resumed = false;
-                }
-                if (stage_extend_steps(sc, fc, st, ev, f, 4096) == kFlowHit) {
-                    stage_shade(sc, fc, st, ev);
-                    if (st.bounce == 0u) {
-                        c1 = make_float4(c1.x + st.albedo.x, c1.y + st.albedo.y, c1.z + st.albedo.z, c1.w + 0.0f);
-                        c2 = make_float4(c2.x + st.normal.x, c2.y + st.normal.y, c2.z + st.normal.z, c2.w + st.depth);
-                    }
-                    stage_shadow(sc, st, ev);
-                    phase = kScatter;
-                } else {
-                    if (st.bounce == 0u) {  // background: pt.comp:434-438
-                        c0 = make_float4(c0.x + 0.f, c0.y + 0.f, c0.z + 0.f, c0.w + 1.0f);
-                        c1 = make_float4(c1.x + 1.0f, c1.y + 1.0f, c1.z + 1.0f, c1.w + 0.f);
-                        c2 = make_float4(c2.x + 0.f, c2.y + 0.f, c2.z + 0.f, c2.w + 1e4f);
-                    }
-                    phase = kEnded;
-                }
+        const bool mine = slot != kNone && S.req[slot] == e;
+        if (drop) {
+            if (mine) {
+                *at = kNone;
+                S.p_spec[slot] = make_float4(0.f, 0.f, 0.f, bitsf(kNone));
             }
-            if (phase == kScatter) {  // k_wf_shadow_map, k_wf_scatter
-                ev.map_pdf = 0.0f;
-                if (fc.env_map) stage_shadow_map(sc, st, ev);
-                phase = stage_scatter(sc, fc, st, ev) && path_continues(fc, st) ? kFlight : kEnded;
-                continue;
+            wave_push(mine, slot, Q.end[cur], &Q.counts[kWfEND0 + cur]);
+            const uint64_t m = __ballot(mine);
+            if (m && int(lane) == __ffsll((unsigned long long)m) - 1) atomicAdd(done, uint32_t(__popcll(m)));
+        } else {
+            const uint32_t i = wave_take(mine, count);
+            if (mine && i < cap) {
+                list[i] = slot | ((q < n_sh ? uint32_t(kShadow) : uint32_t(kFlight)) << 30);
+                *at = kNone;
             }
-            // k_wf_regen: the path's radiance into its chunk's sums (pt.comp:546-551),
-            // then the chunk's next path or its partial sums.
-            const V3 r = path_end(st);
-            const float lum = dot(r, V3(0.2126f, 0.7152f, 0.0722f));
-            c0 = make_float4(c0.x + r.x, c0.y + r.y, c0.z + r.z, c0.w);
-            c1.w += lum * lum;
-            ref.k++;
-            uint32_t px, py, ci, pi, path_index, n_paths;
-            item_coords(R, ref.item, ref.k, px, py, ci, pi, path_index, n_paths);
-            if (ref.k >= n_paths) {
-                store_partial(R, ci, pi, c0, c1, c2);
-                break;
-            }
-            path_begin(sc, fc, px, py, path_index, st);
-            phase = kFlight;
         }
-        S.p_spec[slot] = make_float4(0.f, 0.f, 0.f, bitsf(kNone));  // the slot is idle
+    }
+}
+
+// Served engine: the paths k_wf_tail_gather listed for entry `e` run to their
+// ends (run_to_end: the frame is the same as with iterations), each finished
+// work item counted in `done`. Runs on its own stream beside the iterations,
+// which no longer reach these slots.
+__global__ void __launch_bounds__(64) k_wf_tail(const PtRequest* __restrict__ reqs, WfSlots S, uint32_t e,
+                                                const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
+                                                uint32_t cap, uint32_t* __restrict__ done) {
+    const PtRequest& R = reqs[e];
+    const uint32_t n = min(*count, cap);
+    for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
+        const uint32_t q = base + __lane_id();
+        bool finished = false;
+        if (q < n) finished = run_to_end(R, S, list[q] & 0x3fffffffu, list[q] >> 30);
+        const uint64_t m = __ballot(finished);
+        if (m && int(__lane_id()) == __ffsll((unsigned long long)m) - 1) atomicAdd(done, uint32_t(__popcll(m)));
+    }
+}
+
+// Served engine, once k_wf_tail has ended: its slots, idle now, back on
+// end[cur], where regen gives them work or parks them.
+__global__ void __launch_bounds__(256) k_wf_tail_return(WfQueues Q, uint32_t cur, const uint32_t* __restrict__ list,
+                                                        const uint32_t* __restrict__ count, uint32_t cap) {
+    const uint32_t n = min(*count, cap);
+    for (uint32_t base = blockIdx.x * 256u + (threadIdx.x & ~63u); base < n; base += gridDim.x * 256u) {
+        const uint32_t q = base + __lane_id();
+        wave_push(q < n, q < n ? list[q] & 0x3fffffffu : 0u, Q.end[cur], &Q.counts[kWfEND0 + cur]);
     }
 }
 
@@ -931,6 +1012,24 @@ void wf_iteration_served(const PtRequest* reqs, const WfActive& A, const WfServe
 void wf_finish(const PtRequest* reqs, const WfSlots& S, const WfQueues& Q, uint32_t cur, uint32_t paths,
                hipStream_t st) {
     hipLaunchKernelGGL(k_wf_finish, dim3(std::max(1u, (paths + 63u) / 64u)), dim3(64), 0, st, reqs, S, Q, cur);
+}
+
+void wf_tail_gather(const WfSlots& S, const WfQueues& Q, uint32_t cur, uint32_t entry, bool drop, uint32_t* list,
+                    uint32_t* count, uint32_t cap, uint32_t* done, uint32_t slots, hipStream_t st) {
+    const uint32_t blocks = std::max(1u, std::min((slots + 255u) / 256u, 4096u));
+    hipLaunchKernelGGL(k_wf_tail_gather, dim3(blocks), dim3(256), 0, st, S, Q, cur, entry, drop, list, count, cap, done);
+}
+
+void wf_tail(const PtRequest* reqs, const WfSlots& S, uint32_t entry, const uint32_t* list, const uint32_t* count,
+             uint32_t cap, uint32_t* done, hipStream_t st) {
+    hipLaunchKernelGGL(k_wf_tail, dim3(std::max(1u, (cap + 63u) / 64u)), dim3(64), 0, st, reqs, S, entry, list, count,
+                       cap, done);
+}
+
+void wf_tail_return(const WfQueues& Q, uint32_t cur, const uint32_t* list, const uint32_t* count, uint32_t cap,
+                    hipStream_t st) {
+    hipLaunchKernelGGL(k_wf_tail_return, dim3(std::max(1u, (cap + 255u) / 256u)), dim3(256), 0, st, Q, cur, list,
+                       count, cap);
 }
 
 void wf_resolve(const PtRequest& r, hipStream_t st) {

@@ -146,6 +146,19 @@ void wf_iteration_served(const PtRequest* reqs, const WfActive& A, const WfServe
 // far; `cur` is the next iteration's.
 void wf_finish(const PtRequest* reqs, const WfSlots& S, const WfQueues& Q, uint32_t cur, uint32_t paths,
                hipStream_t st);
+// Served engine, after the iterations queued so far on `st` (`cur` the next
+// one's): the paths of request-table entry `entry` leave the queues. With
+// `drop` they are abandoned (slots idle, items counted in done); otherwise up to
+// `cap` are listed in `list` / *count (zeroed by the caller) for wf_tail.
+void wf_tail_gather(const WfSlots& S, const WfQueues& Q, uint32_t cur, uint32_t entry, bool drop, uint32_t* list,
+                    uint32_t* count, uint32_t cap, uint32_t* done, uint32_t slots, hipStream_t st);
+// Runs the listed paths to their ends (the same results as iterations), on a
+// stream of its own beside the iterations; finished items counted in done.
+void wf_tail(const PtRequest* reqs, const WfSlots& S, uint32_t entry, const uint32_t* list, const uint32_t* count,
+             uint32_t cap, uint32_t* done, hipStream_t st);
+// After wf_tail has ended: its slots back on the end queue of iteration `cur`.
+void wf_tail_return(const WfQueues& Q, uint32_t cur, const uint32_t* list, const uint32_t* count, uint32_t cap,
+                    hipStream_t st);
 void wf_resolve(const PtRequest& r, hipStream_t st);
 
 }  // namespace vvr

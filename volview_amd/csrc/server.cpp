@@ -8,13 +8,19 @@
 //      batched kernel, each followed by its read-back (ahead of the next
 //      path-tracing batch, so they wait for at most one short batch);
 //   3. runs a short batch of wavefront iterations for the path-traced requests
-//      and reads back every increment's item counters after it;
+//      and reads back every increment's item counters after it; an increment
+//      whose items are all handed out and whose paths in flight are few has
+//      those paths taken out of the queues and run to their ends in one launch
+//      on a second stream, beside the iterations (a tail launch), so that it
+//      does not wait for its longest paths at one stage per iteration;
 //   4. resolves, in order, each request's increments whose items have all
 //      finished, reads the newest frame back, and starts a request's next
 //      increment as soon as the current one has handed out all its items (two
 //      increments of a request may be in flight: one in its tail, one filling
 //      the slots), so a progressive request always offers work;
 //   5. turns read-backs that have landed into completion records.
+// A cancelled request stops handing out items at once and its paths in flight
+// are dropped from the queues after the batches already queued.
 // Requests that arrive meanwhile join at the next pass: they are admitted as
 // memory and table entries free up, and their paths share the iterations of
 // the requests already in flight (continuous batching). Work items are handed
@@ -27,6 +33,7 @@
 #include <map>
 #include <mutex>
 #include <thread>
+#include <tuple>
 #include <unordered_map>
 
 #include "renderer_impl.hpp"
@@ -107,6 +114,9 @@ struct Entry {
     uint64_t start_seq = 0;    // first batch whose counter read-back sees its reset counters
     bool handing_out = false;  // on the active list
     bool complete = false;     // every item finished
+    uint64_t touched = 0;      // order in which it first handed out items (0: not yet)
+    int tail = -1;             // the tail launch finishing its paths, while it runs
+    bool tailed = false;       // it had one
 };
 
 size_t chunk_sum_bytes(const Job& j) {  // one increment's chunk sums
@@ -153,9 +163,24 @@ struct vvr_server {
     const uint32_t* h_ctr = nullptr;  // the newest landed read-back
     uint64_t batch_seq = 0;           // wavefront batches launched
     uint64_t snap_seq = 0;            // the batch h_ctr follows
+    uint64_t touch_seq = 0;           // increments that have begun handing out items
     uint32_t cur = 0, idle_in = 0;
     bool revive = false;
     Entry table[kTable];
+    // Tail launches. An increment's paths advance one stage per iteration, so
+    // once its items are all handed out it would wait for its longest paths
+    // over many iterations; when few remain, they leave the queues and run to
+    // their ends in one launch on tail_stream, beside the iterations.
+    struct Tail {
+        int entry = -1;            // the table entry it finishes (-1: free)
+        uint32_t* list = nullptr;  // device: tail_cap listed slots, then the count
+        uint32_t* count = nullptr;
+        hipEvent_t gathered = nullptr, finished = nullptr;
+    };
+    static constexpr uint32_t kTails = 4;
+    Tail tails[kTails];
+    hipStream_t tail_stream = nullptr;
+    uint32_t tail_cap = 0;  // paths one tail takes, at most (0: no tails)
     uint32_t active[kWfMaxActive] = {};
     uint32_t n_active = 0;
     uint32_t entries_in_use = 0;
@@ -196,6 +221,8 @@ struct vvr_server {
     void launch_dvr(const std::vector<Job*>& batch);
     void pt_batch();
     void check_entries();
+    void start_tails();
+    void reap_tails();
     void advance_jobs();
     void stop_cancelled();
     void poll_reads();
@@ -426,14 +453,20 @@ void vvr_server::read_back(Job* j, uint32_t spp) {
 }
 
 // A short batch of wavefront iterations over the increments handing out work.
-// Priority: the lowest increment index first (a new request's first frame
-// before later frames of others), then the oldest request.
+// Priority: increments that have begun handing out items finish first, in the
+// order they began, so that an increment is never left half done while others
+// run (a request cancelled then would abandon the paths it already traced);
+// then the lowest increment index (a new request's first frame before later
+// frames of others), then the oldest request.
 void vvr_server::pt_batch() {
     auto key = [&](uint32_t e) {
         const Job* j = table[e].job;
-        return std::make_pair(j->inc_started, j->id);
+        const uint64_t t = table[e].touched;
+        return std::make_tuple(t ? t : ~uint64_t(0), j->inc_started, j->id);
     };
     std::sort(active, active + n_active, [&](uint32_t a, uint32_t b) { return key(a) < key(b); });
+    // The first increment in this order receives this batch's first items.
+    if (n_active && !table[active[0]].touched) table[active[0]].touched = ++touch_seq;
     WfActive A{};
     A.n = n_active;
     for (uint32_t i = 0; i < n_active; i++) A.r[i] = uint16_t(active[i]);
@@ -468,11 +501,65 @@ void vvr_server::check_entries() {
     for (uint32_t e = 0; e < kTable; e++) {
         Entry& en = table[e];
         if (!en.job || snap_seq < en.start_seq) continue;
+        // Items went to it while an increment ahead of it ran out (the host
+        // learns this from the counters only).
+        if (!en.touched && head[e * kWfCounterStride] > 0) en.touched = ++touch_seq;
         if (en.handing_out && head[e * kWfCounterStride] >= en.n_items) {
             en.handing_out = false;
             drop_active(e);
         }
-        if (!en.complete && done[e * kWfCounterStride] >= en.n_items) en.complete = true;
+        // Complete once every item finished and no tail still writes its sums
+        // (a reaped tail's work precedes later launches on the stream).
+        if (!en.complete && en.tail < 0 && done[e * kWfCounterStride] >= en.n_items) en.complete = true;
+    }
+}
+
+// Increments whose items are all handed out and whose paths in flight are
+// few: those paths leave the queues after the batches queued so far and run to
+// their ends on tail_stream (the counters say how many are left; a batch queued
+// since only lowers that).
+void vvr_server::start_tails() {
+    if (!tail_cap) return;
+    const uint32_t* done = h_ctr + kTable * kWfCounterStride;
+    for (uint32_t e = 0; e < kTable; e++) {
+        Entry& en = table[e];
+        if (!en.job || en.handing_out || en.complete || en.tailed || en.job->draining || snap_seq < en.start_seq)
+            continue;
+        const uint32_t left = en.n_items - std::min(done[e * kWfCounterStride], en.n_items);
+        if (left == 0 || left > tail_cap) continue;
+        uint32_t k = 0;
+        while (k < kTails && tails[k].entry >= 0) k++;
+        if (k == kTails) return;
+        Tail& t = tails[k];
+        uint32_t* d_done = d_ctr + (kTable + e) * kWfCounterStride;
+        check(hipMemsetAsync(t.count, 0, 4, stream), "tail count reset");
+        wf_tail_gather(engine.S, engine.Q, cur, e, false, t.list, t.count, tail_cap, d_done, engine.slots, stream);
+        check(hipGetLastError(), "tail gather");
+        check(hipEventRecord(t.gathered, stream), "hipEventRecord");
+        check(hipStreamWaitEvent(tail_stream, t.gathered, 0), "hipStreamWaitEvent");
+        wf_tail(d_table, engine.S, e, t.list, t.count, tail_cap, d_done, tail_stream);
+        check(hipGetLastError(), "tail launch");
+        check(hipEventRecord(t.finished, tail_stream), "hipEventRecord");
+        t.entry = int(e);
+        en.tail = int(k);
+        en.tailed = true;
+    }
+}
+
+// Tails that have ended: the stream's later work waits for them (their sums,
+// their slots), and their slots go back to the iterations.
+void vvr_server::reap_tails() {
+    for (uint32_t k = 0; k < kTails; k++) {
+        Tail& t = tails[k];
+        if (t.entry < 0) continue;
+        const hipError_t q = hipEventQuery(t.finished);
+        if (q == hipErrorNotReady) continue;
+        check(q, "tail launch");
+        check(hipStreamWaitEvent(stream, t.finished, 0), "hipStreamWaitEvent");
+        wf_tail_return(engine.Q, cur, t.list, t.count, tail_cap, stream);
+        check(hipGetLastError(), "tail return");
+        table[t.entry].tail = -1;
+        t.entry = -1;
     }
 }
 
@@ -493,10 +580,18 @@ void vvr_server::advance_jobs() {
             bool drained = j->entries.empty() || snap_seq >= j->drain_seq;
             for (int e : j->entries) {
                 const uint32_t handed = std::min(head[e * kWfCounterStride], table[e].n_items);
-                drained = drained && done[e * kWfCounterStride] >= handed;
+                drained = drained && done[e * kWfCounterStride] >= handed && table[e].tail < 0;
             }
             if (drained && !j->read_pending) {
-                for (int e : j->entries) free_entry(e);
+                uint64_t abandoned = 0;
+                for (int e : j->entries) {
+                    abandoned += std::min(done[e * kWfCounterStride], table[e].n_items);
+                    free_entry(e);
+                }
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    stats.pt_paths_abandoned += abandoned;
+                }
                 j->entries.clear();
                 const uint32_t spp = j->spp_delivered;
                 const vvr_completion c = record(j, spp > 0 ? VVR_OK : VVR_ECANCELED, spp, true);
@@ -551,6 +646,11 @@ void vvr_server::stop_cancelled() {
         for (int e : j->entries) {
             table[e].handing_out = false;
             drop_active(uint32_t(e));
+            // Its paths in flight are abandoned after the batches queued so far
+            // (those on the end queue finish at the next regen).
+            wf_tail_gather(engine.S, engine.Q, cur, uint32_t(e), true, nullptr, nullptr, 0,
+                           d_ctr + (kTable + e) * kWfCounterStride, engine.slots, stream);
+            check(hipGetLastError(), "dropping cancelled paths");
         }
     }
 }
@@ -623,6 +723,8 @@ void vvr_server::loop() {
                 h_ctr = h_ctr_buf[w];
                 snap_seq = ctr_seq[w];
                 check_entries();
+                reap_tails();
+                start_tails();
             } else {
                 for (int k = 0; k < 2; k++)
                     if (ctr_pending[k]) {  // no entries left: the read-back only needs to land
@@ -652,6 +754,7 @@ void vvr_server::loop() {
 // can be released), and later submissions are refused.
 void vvr_server::fail_all(int code, const std::string& msg) {
     (void)hipStreamSynchronize(stream);
+    if (tail_stream) (void)hipStreamSynchronize(tail_stream);
     std::vector<Job*> snap;
     {
         std::lock_guard<std::mutex> lk(mu);
@@ -703,6 +806,16 @@ void create_server(vvr_ctx* ctx, const vvr_server_config* cfg, vvr_server** out)
         s->d_jobs[k] = dalloc<DvrJob>(64, "ray-marching jobs");
         s->h_jobs[k] = halloc<DvrJob>(64, "ray-marching job staging");
         check(hipEventCreateWithFlags(&s->jobs_ev[k], hipEventDisableTiming), "hipEventCreate");
+    }
+    s->tail_cap = s->engine.grid.tail_paths;
+    if (s->tail_cap) {
+        check(hipStreamCreateWithFlags(&s->tail_stream, hipStreamNonBlocking), "hipStreamCreate");
+        for (auto& t : s->tails) {
+            t.list = dalloc<uint32_t>(size_t(s->tail_cap) + 64, "tail lists");
+            t.count = t.list + s->tail_cap;
+            check(hipEventCreateWithFlags(&t.gathered, hipEventDisableTiming), "hipEventCreate");
+            check(hipEventCreateWithFlags(&t.finished, hipEventDisableTiming), "hipEventCreate");
+        }
     }
     s->d_uniforms = dalloc<vvr_uniforms>(kMaxJobs, "request uniforms");
     s->h_uniforms = halloc<vvr_uniforms>(kMaxJobs, "request uniform staging");
@@ -873,6 +986,7 @@ void vvr_server_destroy(vvr_server* s) {
     if (s->worker.joinable()) s->worker.join();
     (void)hipSetDevice(s->device);
     (void)hipStreamSynchronize(s->stream);
+    if (s->tail_stream) (void)hipStreamSynchronize(s->tail_stream);
     for (auto& kv : s->jobs) {
         Job* j = kv.second.get();
         if (j->state == Job::kRunning) s->release_memory(j);
@@ -888,6 +1002,12 @@ void vvr_server_destroy(vvr_server* s) {
         hfree(s->h_jobs[k]);
         if (s->jobs_ev[k]) (void)hipEventDestroy(s->jobs_ev[k]);
     }
+    for (auto& t : s->tails) {
+        dfree(t.list);
+        if (t.gathered) (void)hipEventDestroy(t.gathered);
+        if (t.finished) (void)hipEventDestroy(t.finished);
+    }
+    if (s->tail_stream) (void)hipStreamDestroy(s->tail_stream);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
