@@ -621,6 +621,9 @@ def main() -> int:
 
     t_setup = time.perf_counter()
     ctx = Context(0)
+    if args.copy_probe:
+        copy_probe(args.copy_probe << 20)
+        print(f"copy probe: {args.copy_probe} MiB", file=sys.stderr, flush=True)
     if args.only == "dvr":
         print(json.dumps({"dvr": run_dvr(ctx)}), flush=True)
         return 0
@@ -628,9 +631,6 @@ def main() -> int:
     r = scene.renderer
     print(f"setup {time.perf_counter() - t_setup:.1f} s: {scene.n_labels} organs", file=sys.stderr, flush=True)
 
-    if args.copy_probe:
-        copy_probe(args.copy_probe << 20)
-        print(f"copy probe: {args.copy_probe} MiB", file=sys.stderr, flush=True)
     r.set_timing(True)  # the first record builds the masks and the majorant (autotune's, warm-up's or the first step's)
     shape = r.pt_shape()
     if args.autotune:  # untimed: the autotuner's trial frames of the workload itself

@@ -261,3 +261,30 @@ def test_frame_over_the_item_limit_is_refused(ctx):
     r.record(2, workloads.pt_params(r, 0, 2, bounces=2))
     acc, _, _ = r.read_accum()
     assert np.isfinite(acc).all() and r.accumulated_samples() == 2
+
+
+@pytest.mark.parametrize("mode", ["plain", "lit", "labels"])
+def test_dvr_passes_equal_one_loop(ctx, monkeypatch, mode):
+    """The ray marcher in passes (march, shading of the listed samples, per-pixel
+    sums, the long rays marching on) gives the one-loop frame bit for bit: the
+    DVR-1 recipe at 128^3, with lighting volumes, and with organ labels."""
+    dims = (128, 128, 128)
+    hu = phantom.ct(dims, 5)
+    r = Renderer(ctx, hu, TransferFunction.preset(2 if mode == "labels" else 0))
+    n = 0
+    if mode == "labels":
+        labels, soft, _, n = phantom.organs(dims, 5, 24)
+        r.set_labels(labels, soft)
+    p = workloads.dvr_params(r, 3, lighting=mode == "lit")
+    if n:
+        workloads.organ_label_params(p, n, 5, _abi.LABELS_ISOLATE)
+    r.resize(160, 144)
+    frames = {}
+    for env in ("1", "0"):
+        monkeypatch.setenv("VVR_DVR_PASSES", env)
+        r.record(1, p)
+        frames[env] = r.read_hdr()
+    a, b = frames["1"], frames["0"]
+    print(f"{mode}: {(a[..., 3] < 0.5).mean():.3f} of pixels opaque")
+    assert (a[..., 3] < 0.99).mean() > 0.02
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))

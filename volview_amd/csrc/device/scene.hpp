@@ -227,34 +227,41 @@ __device__ __forceinline__ float fold8(const float T[2][2][2], float gx0, float 
 }
 
 // density_cubic (common.glsl:124-127). The 8 trilinear fetches of cubic_fetch
-// (common.glsl:116-122) read only the 4^3 voxels around floor(idx). They are
-// evaluated in one pass over the window, a z slice at a time: each fetch lerps
-// along x, then y, then z as a texture unit does, the z lerp accumulated over
-// the slices.
+// (common.glsl:116-122) read only the 4^3 voxels around floor(idx): each lerps
+// along x, then y, then z as a texture unit does, here a z slice of the window
+// at a time. The value taps' lower texels sit at fixed window offsets: h0 -
+// (i - 1) = w1 / (w0 + w1) lies in [0.8, 1] and h1 - (i + 1) = w3 / (w2 + w3) in
+// [0, 0.2), so the low tap lerps offsets 0 and 1 and the high tap offsets 2 and
+// 3. (Where h0 rounds up to i, the texture unit's tap is offset 1 at weight 0;
+// the lerp of offsets 0 and 1 at weight 1 gives the same value, exactly.)
 __device__ inline float density_cubic(const DevScene& s, V3 idx) {
     const float ix = floorf(idx.x), iy = floorf(idx.y), iz = floorf(idx.z);
     const Taps1 tx = cubic_taps1(idx.x, ix), ty = cubic_taps1(idx.y, iy), tz = cubic_taps1(idx.z, iz);
-    const Tap xt[2] = {tap_of(tx.h0, ix), tap_of(tx.h1, ix)};
-    const Tap yt[2] = {tap_of(ty.h0, iy), tap_of(ty.h1, iy)};
-    const Tap zt[2] = {tap_of(tz.h0, iz), tap_of(tz.h1, iz)};
+    const float ax[2] = {tx.h0 - (ix - 1.0f), tx.h1 - (ix + 1.0f)};
+    const float ay[2] = {ty.h0 - (iy - 1.0f), ty.h1 - (iy + 1.0f)};
+    const float az[2] = {tz.h0 - (iz - 1.0f), tz.h1 - (iz + 1.0f)};
     const WindowAddr w = window_addr(s.g, fidx(ix), fidx(iy), fidx(iz));
-    float T[2][2][2] = {{{0.f, 0.f}, {0.f, 0.f}}, {{0.f, 0.f}, {0.f, 0.f}}};  // [x tap][y tap][z tap]
+    float Y[2][2][4];  // [x tap][y tap][z slice]: the fetches lerped along x and y
 #pragma unroll
     for (int z = 0; z < 4; z++) {
-        float v[4][4], r[2][4];
+        float v[4][4];
         load_slice(s.density, w, z, v);
-        slice_x(v, xt[0], r[0]);
-        slice_x(v, xt[1], r[1]);
-        const float wz[2] = {tap_weight(zt[0], z), tap_weight(zt[1], z)};
 #pragma unroll
-        for (int a = 0; a < 2; a++)
+        for (int a = 0; a < 2; a++) {
+            float r[4];
 #pragma unroll
-            for (int b = 0; b < 2; b++) {
-                const float c = lerp4(r[a][0], r[a][1], r[a][2], r[a][3], yt[b]);
-                T[a][b][0] = accum_tap(T[a][b][0], c, wz[0]);
-                T[a][b][1] = accum_tap(T[a][b][1], c, wz[1]);
-            }
+            for (int y = 0; y < 4; y++) r[y] = mixf(v[y][2 * a], v[y][2 * a + 1], ax[a]);
+#pragma unroll
+            for (int b = 0; b < 2; b++) Y[a][b][z] = mixf(r[2 * b], r[2 * b + 1], ay[b]);
+        }
     }
+    float T[2][2][2];  // [x tap][y tap][z tap]
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int c = 0; c < 2; c++) T[a][b][c] = mixf(Y[a][b][2 * c], Y[a][b][2 * c + 1], az[c]);
     return fold8(T, tx.g0, tx.g1, ty.g0, ty.g1, tz.g0, tz.g1);
 }
 

@@ -37,6 +37,22 @@ void launch_majorant(const uint16_t* density, const uint32_t* mask, float* out, 
 
 // dvr.comp: hdr = RGBA16F (width x height x 4 halves).
 void launch_dvr(const DevScene& s, uint16_t* hdr, uint32_t width, uint32_t height, hipStream_t st);
+// The same frame in passes: rounds of march (each pixel to its next
+// kDvrSamples visible samples), shading of the listed samples, per-pixel sums;
+// then the rest of the long rays in one loop. For frames without ghosted
+// labels; buffers of dvr_passes_bytes(width * height); `cus` sizes the shading.
+constexpr uint32_t kDvrSamples = 32;  // samples a pixel records per round
+constexpr uint32_t kDvrRounds = 1;
+struct DvrPasses {
+    float4* rec;        // kDvrSamples records of 2 vectors per pixel
+    float4* state;      // per pixel: the march state after a round
+    float4* radiance;   // per pixel: the light of its samples so far
+    uint32_t* list;     // the round's samples (pixel x kDvrSamples + k)
+    uint32_t* count;    // their number
+};
+inline size_t dvr_passes_bytes(size_t pixels) { return pixels * (kDvrSamples * 32 + 16 + 16 + kDvrSamples * 4) + 5 * 256; }
+void launch_dvr_passes(const DevScene& s, const DvrPasses& P, uint16_t* hdr, uint32_t width, uint32_t height,
+                       int cus, hipStream_t st);
 // Lighting volumes of the ray marcher (shadow.comp, ao.comp) into s.sdims /
 // s.adims texel arrays; `hu` is the host copy of the uniforms s.u points to.
 void shadow_sweep_axis(const vvr_uniforms& hu, int& axis, int& sign);

@@ -39,6 +39,8 @@ void new_renderer(vvr_ctx* ctx, const vvr_volume* vol, const vvr_tf* tf, vvr_ren
     auto r = std::make_unique<vvr_renderer>();
     r->device = ctx->device;
     r->use_device();
+    check(hipDeviceGetAttribute(&r->cus, hipDeviceAttributeMultiprocessorCount, r->device), "device attribute");
+    r->cus = std::max(r->cus, 1);
     r->geo = make_geometry(*vol);
     r->grid = g;
     for (int a = 0; a < 3; a++) r->mdims[a] = int((vol->dims[a] + kBrick - 1) / kBrick);
@@ -247,7 +249,13 @@ void record(vvr_renderer* r, uint64_t frame, const vvr_frame_params* p) {
         }
     } else {
         r->denoised = false;
-        r->timed(VVR_KERNEL_DVR, [&] { launch_dvr(sc, r->hdr, r->width, r->height, r->stream); });
+        if (r->marches_in_passes(u)) {
+            const DvrPasses& passes = r->passes();
+            r->timed(VVR_KERNEL_DVR,
+                     [&] { launch_dvr_passes(sc, passes, r->hdr, r->width, r->height, r->cus, r->stream); });
+        } else {
+            r->timed(VVR_KERNEL_DVR, [&] { launch_dvr(sc, r->hdr, r->width, r->height, r->stream); });
+        }
     }
     // Camera finishing and tone mapping into the RGBA8 output (renderer.rs:1801-1811).
     r->timed(VVR_KERNEL_POST, [&] {

@@ -521,8 +521,45 @@ struct vvr_renderer {
         check(hipMemcpy(preint, p.data(), kLutSize * sizeof(float4), hipMemcpyHostToDevice), "pre-integration upload");
     }
 
+    // Buffers of the ray marcher's passes at the frame size (launch_dvr_passes),
+    // allocated by the first frame that uses them.
+    uint8_t* dvr_mem = nullptr;
+    DvrPasses dvr_passes{};
+    int cus = 1;  // compute units of the device
+    // Whether a frame with uniforms `u` is ray marched in passes: not when a
+    // label may be drawn as a ghost, whose opacity the march cannot know (nor
+    // with VVR_DVR_PASSES=0, or buffers beyond 4 GiB).
+    bool marches_in_passes(const vvr_uniforms& u) const {
+        const char* env = std::getenv("VVR_DVR_PASSES");
+        if ((env && env[0] == '0') || dvr_passes_bytes(size_t(width) * height) > (size_t(4) << 30)) return false;
+        if (uint32_t(u.labels[0]) != VVR_LABELS_OFF)
+            for (int l = 1; l < 256; l++)
+                if (u.label_colors[l][3] > 1.5f) return false;
+        return true;
+    }
+    const DvrPasses& passes() {
+        if (!dvr_mem) {
+            const size_t px = size_t(width) * height;
+            dvr_mem = dalloc<uint8_t>(dvr_passes_bytes(px), "ray-marching passes");
+            uint8_t* p = dvr_mem;
+            auto take = [&](size_t bytes) {
+                uint8_t* q = p;
+                p += (bytes + 255) / 256 * 256;
+                return q;
+            };
+            dvr_passes.rec = reinterpret_cast<float4*>(take(px * kDvrSamples * 32));
+            dvr_passes.state = reinterpret_cast<float4*>(take(px * 16));
+            dvr_passes.radiance = reinterpret_cast<float4*>(take(px * 16));
+            dvr_passes.list = reinterpret_cast<uint32_t*>(take(px * kDvrSamples * 4));
+            dvr_passes.count = reinterpret_cast<uint32_t*>(take(4));
+        }
+        return dvr_passes;
+    }
+
     void free_frame() {
         dfree(hdr), dfree(accum), dfree(albedo), dfree(normal), dfree(dn_scratch), dfree(output), dfree(bloom);
+        dfree(dvr_mem);
+        dvr_mem = nullptr;
         hdr = nullptr, accum = albedo = normal = dn_scratch = nullptr;
         output = nullptr, bloom = nullptr;
         denoised = false;
