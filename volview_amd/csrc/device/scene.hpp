@@ -194,13 +194,24 @@ __device__ __forceinline__ WindowAddr window_addr(const BrickGrid& g, int ix, in
     return w;
 }
 
-// One z slice (4 x 4 texel values) of the window.
+// One z slice (4 x 4 texel values) of the window, from the unorm16 density or
+// from its float copy (the same values, without the conversion: the ray marcher
+// reads it; the path tracer's scattered first-bounce fetches run faster on the
+// 2-byte texels).
 __device__ __forceinline__ void load_slice(const uint16_t* __restrict__ d, const WindowAddr& w, int z, float v[4][4]) {
 #pragma unroll
     for (int y = 0; y < 4; y++) {
         const uint32_t base = w.y[y] + w.z[z];
 #pragma unroll
         for (int x = 0; x < 4; x++) v[y][x] = unorm16_value(float(d[w.x[x] + base]));
+    }
+}
+__device__ __forceinline__ void load_slice(const float* __restrict__ d, const WindowAddr& w, int z, float v[4][4]) {
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+        const uint32_t base = w.y[y] + w.z[z];
+#pragma unroll
+        for (int x = 0; x < 4; x++) v[y][x] = d[w.x[x] + base];
     }
 }
 // The slice's 4 rows lerped along x at tap t.
@@ -234,6 +245,7 @@ __device__ __forceinline__ float fold8(const float T[2][2][2], float gx0, float 
 // [0, 0.2), so the low tap lerps offsets 0 and 1 and the high tap offsets 2 and
 // 3. (Where h0 rounds up to i, the texture unit's tap is offset 1 at weight 0;
 // the lerp of offsets 0 and 1 at weight 1 gives the same value, exactly.)
+template <bool kFloatTexels = false>
 __device__ inline float density_cubic(const DevScene& s, V3 idx) {
     const float ix = floorf(idx.x), iy = floorf(idx.y), iz = floorf(idx.z);
     const Taps1 tx = cubic_taps1(idx.x, ix), ty = cubic_taps1(idx.y, iy), tz = cubic_taps1(idx.z, iz);
@@ -245,7 +257,8 @@ __device__ inline float density_cubic(const DevScene& s, V3 idx) {
 #pragma unroll
     for (int z = 0; z < 4; z++) {
         float v[4][4];
-        load_slice(s.density, w, z, v);
+        if (kFloatTexels) load_slice(s.density_f, w, z, v);
+        else load_slice(s.density, w, z, v);
 #pragma unroll
         for (int a = 0; a < 2; a++) {
             float r[4];
@@ -309,8 +322,10 @@ __device__ inline V3 density_gradient_cubic(const DevScene& s, V3 idx) {
 }
 
 // density_view (common.glsl:160-162).
+// (kFloatTexels: the cubic window from the float copy.)
+template <bool kFloatTexels = false>
 __device__ __forceinline__ float density_view(const DevScene& s, V3 idx) {
-    return s.u->env[1] > 0.5f ? density_cubic(s, idx) : density_at_index(s, idx);
+    return s.u->env[1] > 0.5f ? density_cubic<kFloatTexels>(s, idx) : density_at_index(s, idx);
 }
 
 // ---- transfer function (common.glsl:170-230) ------------------------------------------

@@ -42,6 +42,7 @@ inline BrickGrid make_brick_grid(int nx, int ny, int nz) {
 struct DevScene {
     BrickGrid g;
     const uint16_t* __restrict__ density;  // unorm16
+    const float* __restrict__ density_f;   // the same values as floats, c / 65535 (the cubic fetch's window)
     const uint8_t* __restrict__ labels;    // label per voxel
     const uint32_t* __restrict__ mask;     // RG16: r low half, g high half
     const uint32_t* __restrict__ normals;  // RGBA8 snorm, x in the low byte

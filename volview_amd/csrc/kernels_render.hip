@@ -64,7 +64,7 @@ __device__ __forceinline__ MarchState march_begin(const DevScene& sc, const DvrR
     m.prev_density = -1.0f;
     if (r.t0 < r.t1) {
         m.tt = r.t0 + ign(float(px), float(py)) * step_mm;
-        m.prev_density = density_view(sc, r.o_idx + r.d_idx * (m.tt - step_mm));
+        m.prev_density = density_view<true>(sc, r.o_idx + r.d_idx * (m.tt - step_mm));
     }
     return m;
 }
@@ -97,7 +97,7 @@ __device__ __forceinline__ bool dvr_march(const DevScene& sc, const DvrRay& r, M
             m.prev_density = -1.0f;
             continue;
         }
-        const float density = density_view(sc, idx);
+        const float density = density_view<true>(sc, idx);
         const float front_density = m.prev_density;
         const bool segment = preintegrate && front_density >= 0.0f;
         Sample s = segment ? tf_segment(sc, front_density, density) : tf_point(sc, density);
@@ -115,7 +115,7 @@ __device__ __forceinline__ bool dvr_march(const DevScene& sc, const DvrRay& r, M
             for (int i = 0; i < 5; i++) {
                 const float mid = 0.5f * (inside + outside);
                 const V3 at = idx + d_idx * mid;
-                const bool opaque = labelled(sc, tf_point(sc, density_view(sc, at)), at).extinction >= target;
+                const bool opaque = labelled(sc, tf_point(sc, density_view<true>(sc, at)), at).extinction >= target;
                 inside = opaque ? mid : inside;
                 outside = opaque ? outside : mid;
             }

@@ -10,6 +10,7 @@
 
 #include <cmath>
 
+#include "device/scene.hpp"
 #include "launch.hpp"
 
 namespace vvr {
@@ -35,7 +36,10 @@ __device__ __forceinline__ uint16_t hu_to_unorm16(int16_t hu) {
     return uint16_t(roundf(t * 65535.0f));
 }
 
-__global__ void k_upload_density(const int16_t* __restrict__ hu, uint16_t* __restrict__ out, BrickGrid g) {
+// The unorm16 density, and its values c / 65535 as floats (what the cubic
+// fetch's window reads).
+__global__ void k_upload_density(const int16_t* __restrict__ hu, uint16_t* __restrict__ out,
+                                 float* __restrict__ out_f, BrickGrid g) {
     const uint32_t o = blockIdx.x * kBlock + threadIdx.x;
     if (o >= g.padded()) return;
     int x, y, z;
@@ -43,6 +47,7 @@ __global__ void k_upload_density(const int16_t* __restrict__ hu, uint16_t* __res
     uint16_t v = 0;
     if (x < g.nx && y < g.ny && z < g.nz) v = hu_to_unorm16(hu[size_t(x) + size_t(g.nx) * (size_t(y) + size_t(g.ny) * z)]);
     out[o] = v;
+    out_f[o] = unorm16_value(float(v));
 }
 
 __global__ void k_upload_labels(const uint8_t* __restrict__ in, uint8_t* __restrict__ out, BrickGrid g) {
@@ -569,8 +574,8 @@ inline unsigned blocks_for(size_t n) { return unsigned((n + kBlock - 1) / kBlock
 
 }  // namespace
 
-void launch_upload_density(const int16_t* hu, uint16_t* out, const BrickGrid& g, hipStream_t st) {
-    hipLaunchKernelGGL(k_upload_density, dim3(blocks_for(g.padded())), dim3(kBlock), 0, st, hu, out, g);
+void launch_upload_density(const int16_t* hu, uint16_t* out, float* out_f, const BrickGrid& g, hipStream_t st) {
+    hipLaunchKernelGGL(k_upload_density, dim3(blocks_for(g.padded())), dim3(kBlock), 0, st, hu, out, out_f, g);
 }
 
 void launch_upload_labels(const uint8_t* labels, uint8_t* out, const BrickGrid& g, hipStream_t st) {

@@ -7,7 +7,8 @@
 namespace vvr {
 
 // HU (linear, int16) -> unorm16 micro-bricks (transfer.rs:121-124, renderer.rs:577).
-void launch_upload_density(const int16_t* hu, uint16_t* out, const BrickGrid& g, hipStream_t st);
+// HU -> the unorm16 density and its float values (c / 65535), bricked.
+void launch_upload_density(const int16_t* hu, uint16_t* out, float* out_f, const BrickGrid& g, hipStream_t st);
 // uint8 labels (linear) -> micro-bricks.
 void launch_upload_labels(const uint8_t* labels, uint8_t* out, const BrickGrid& g, hipStream_t st);
 // Micro-bricks -> linear, element size 1, 2 or 4 bytes.

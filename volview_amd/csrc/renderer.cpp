@@ -58,12 +58,13 @@ void new_renderer(vvr_ctx* ctx, const vvr_volume* vol, const vvr_tf* tf, vvr_ren
 
     const size_t nvox = size_t(vol->dims[0]) * vol->dims[1] * vol->dims[2];
     r->density = dalloc<uint16_t>(g.padded(), "density");
+    r->density_f = dalloc<float>(g.padded(), "density values");
     r->normals = dalloc<uint32_t>(g.padded(), "normals");
     r->majorant = dalloc<float>(size_t(r->mdims[0]) * r->mdims[1] * r->mdims[2], "extinction bounds");
     int16_t* hu = dalloc<int16_t>(nvox, "HU staging");
     hipError_t e = hipMemcpy(hu, vol->hu, nvox * sizeof(int16_t), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        launch_upload_density(hu, r->density, g, r->stream);
+        launch_upload_density(hu, r->density, r->density_f, g, r->stream);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(r->stream);

@@ -305,6 +305,7 @@ struct vvr_renderer {
     int mdims[3] = {0, 0, 0};
 
     uint16_t* density = nullptr;
+    float* density_f = nullptr;  // its values as floats (launch_upload_density)
     uint32_t* normals = nullptr;
     float* majorant = nullptr;
     uint32_t* maj_cells = nullptr;  // majorant build (modes off, tint): per micro-brick, its 2^3 cells' density
@@ -453,6 +454,7 @@ struct vvr_renderer {
         DevScene s;
         s.g = grid;
         s.density = density;
+        s.density_f = density_f;
         s.labels = labels;
         s.mask = mask;
         s.normals = normals;
@@ -574,7 +576,8 @@ struct vvr_renderer {
         }
         free_frame();
         wf.reset();
-        dfree(density), dfree(normals), dfree(majorant), dfree(labels), dfree(mask), dfree(mask_b), dfree(soft);
+        dfree(density), dfree(density_f), dfree(normals), dfree(majorant), dfree(labels), dfree(mask), dfree(mask_b),
+            dfree(soft);
         dfree(maj_cells), dfree(maj_nz);
         dfree(shadow), dfree(ao), dfree(env_rad), dfree(env_table), dfree(env_dif);
         dfree(lut), dfree(preint), dfree(d_uniforms[0]), dfree(d_uniforms[1]), dfree(exposure);
