@@ -39,6 +39,7 @@ __device__ __forceinline__ void tile_pixel(uint32_t& px, uint32_t& py) {
 // A camera ray of dvr.comp in index space (dvr.comp:23-40).
 struct DvrRay {
     V3 dir, o_idx, d_idx;
+    V3 inv_d;  // brick_exit_inverse(d_idx)
     float t0, t1;
 };
 __device__ __forceinline__ DvrRay dvr_ray(const vvr_uniforms* u, uint32_t px, uint32_t py) {
@@ -46,6 +47,7 @@ __device__ __forceinline__ DvrRay dvr_ray(const vvr_uniforms* u, uint32_t px, ui
     r.dir = camera_ray(u, float(px) + 0.5f, float(py) + 0.5f);
     r.o_idx = world_to_index_point(u, V3(u->eye[0], u->eye[1], u->eye[2]));
     r.d_idx = to_index(u).mul(r.dir);
+    r.inv_d = brick_exit_inverse(r.d_idx);
     intersect_volume(u, r.o_idx, r.d_idx, r.t0, r.t1);
     return r;
 }
@@ -87,7 +89,7 @@ __device__ __forceinline__ bool dvr_march(const DevScene& sc, const DvrRay& r, M
         if (stop()) return false;
         V3 idx = o_idx + d_idx * m.tt;
         if (brick_majorant(sc, idx) <= 0.0f) {
-            m.tt += brick_exit(idx, d_idx);
+            m.tt += brick_exit_fast(idx, r.inv_d);
             m.prev_density = -1.0f;
             continue;
         }
