@@ -211,7 +211,10 @@ def run_batched(ctx) -> dict:
     plain()  # warm-up: majorants, engines
     t_plain = plain()
     log(f"batched-32 plain {t_plain:.2f} s")
-    srv = Server(ctx)
+    t = time.perf_counter()  # untimed: the server's shape for this batch (vvr_server_autotune)
+    shape = Server.autotune(ctx, [(rs[q.volume], p, q.width, q.height) for q, p in zip(reqs, params)])
+    log(f"batched-32 server autotune {time.perf_counter() - t:.1f} s: {shape}")
+    srv = Server(ctx, **shape)
 
     def served() -> float:
         t0 = time.perf_counter()
@@ -233,6 +236,7 @@ def run_batched(ctx) -> dict:
     return {"workload": "Batched-32: 32 seeded phantoms of edge U{128..512}, images U{256..1024}^2, "
                         "16 DVR (bone preset) + 16 PT (U{1..64} spp, 8 bounces, g=0.8)",
             "pt_paths": pt_paths, "plain": side(t_plain), "served": side(t_served),
+            "server_shape": shape, "server_shape_from": "autotune",
             "speedup": round(t_plain / t_served, 2)}
 
 

@@ -444,6 +444,14 @@ typedef struct vvr_server_stats {
 } vvr_server_stats;
 
 int vvr_server_create(vvr_ctx* ctx, const vvr_server_config* config, vvr_server** out);
+/* Chooses the engine shape of a server for a request mix: renders `requests`
+ * (a sample of the mix, e.g. a batch it will serve; their frames are discarded)
+ * on trial servers over path slots (1, 2, 4, 8 Mi) and wavefront iterations per
+ * pass (1, 2, 4, 8), and writes the fastest into config->slots and
+ * config->iterations; the other fields are kept and used by every trial. The
+ * requests' renderers must stay alive for the call. No counterpart in the
+ * reference: it serves one request at a time. */
+int vvr_server_autotune(vvr_ctx* ctx, const vvr_request* requests, uint32_t n, vvr_server_config* config);
 /* Queues a request; returns at once with its id. Thread-safe. */
 int vvr_server_submit(vvr_server* s, const vvr_request* request, uint64_t* id);
 /* Up to `cap` frame completions (in completion order), waiting up to wait_ms

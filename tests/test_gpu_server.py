@@ -230,3 +230,40 @@ def test_renderer_calls_refused_while_held(ctx):
     a.record(1, workloads.dvr_params(a, 0))  # the hold is gone
     a.set_transfer_function(TransferFunction.preset(1))
     srv.close()
+
+
+def test_autotune_picks_a_shape_and_the_requests_still_render(ctx):
+    """vvr_server_autotune over a small mix returns one of its candidate shapes;
+    a server of that shape then renders the mix, each frame equal to the
+    request rendered alone."""
+    hu = phantom.ct((48, 48, 40), 3)
+    r = Renderer(ctx, hu, TransferFunction.preset(0))
+    reqs = []
+    for k in range(3):
+        p = workloads.pt_params(r, 0, 2 + k, bounces=4, camera_index=k)
+        reqs.append((r, p, 32, 24))
+    reqs.append((r, workloads.dvr_params(r, 1), 40, 32))
+    shape = Server.autotune(ctx, reqs)
+    assert shape["slots"] in (1 << 20, 2 << 20, 4 << 20, 8 << 20), shape
+    assert shape["iterations"] in (1, 2, 4, 8), shape
+    alone = []
+    for k, (_, p, w, h) in enumerate(reqs):
+        r.resize(w, h)
+        r.record(k + 1, p)
+        alone.append(r.read_hdr())
+    srv = Server(ctx, **shape)
+    ids = [srv.submit(rr, p, w, h) for rr, p, w, h in reqs]
+    done = set()
+    for _ in range(600):
+        for c in srv.poll(wait_ms=100):
+            assert c.status == 0
+            if c.final_frame:
+                done.add(c.id)
+        if done == set(ids):
+            break
+    assert done == set(ids)
+    for i, a in zip(ids, alone):
+        assert np.array_equal(srv.read_hdr(i).view(np.uint32), a.view(np.uint32))
+        srv.release(i)
+    srv.close()
+    r.close()

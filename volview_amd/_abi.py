@@ -202,6 +202,7 @@ SIGNATURES = [
     ("vvr_server_cancel", c_int, [c_void_p, c_uint64]),
     ("vvr_server_release", c_int, [c_void_p, c_uint64]),
     ("vvr_server_get_stats", c_int, [c_void_p, POINTER(ServerStats)]),
+    ("vvr_server_autotune", c_int, [c_void_p, POINTER(Request), c_uint32, POINTER(ServerConfig)]),
     ("vvr_server_destroy", None, [c_void_p]),
     ("vvr_last_error", c_char_p, []),
     ("vvr_version", c_char_p, []),

@@ -828,9 +828,72 @@ void create_server(vvr_ctx* ctx, const vvr_server_config* cfg, vvr_server** out)
     *out = raw;
 }
 
+// One trial of config `c`: the requests submitted at once to a server of their
+// own, timed from the first submission to the last request's final frame.
+double trial_seconds(vvr_ctx* ctx, const vvr_server_config& c, const vvr_request* rq, uint32_t n) {
+    vvr_server* raw = nullptr;
+    create_server(ctx, &c, &raw);
+    std::unique_ptr<vvr_server, void (*)(vvr_server*)> s(raw, vvr_server_destroy);
+    std::vector<uint64_t> ids(n);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t i = 0; i < n; i++) {
+        const int e = vvr_server_submit(s.get(), &rq[i], &ids[i]);
+        if (e != VVR_OK) throw Error{e, vvr_last_error()};
+    }
+    uint32_t left = n;
+    std::vector<vvr_completion> buf(64);
+    while (left > 0) {
+        uint32_t got = 0;
+        const int e = vvr_server_poll(s.get(), buf.data(), uint32_t(buf.size()), &got, 100);
+        if (e != VVR_OK) throw Error{e, vvr_last_error()};
+        for (uint32_t k = 0; k < got; k++) {
+            if (buf[k].status != VVR_OK) throw Error{buf[k].status, "a trial request failed"};
+            if (buf[k].final_frame) left--;
+        }
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::minutes(10))
+            throw Error{VVR_EDEVICE, "autotune trial did not finish within 10 minutes"};
+    }
+    const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (uint64_t id : ids) (void)vvr_server_release(s.get(), id);
+    return sec;
+}
+
+// The server's engine shape for a request mix: trials of the requests over the
+// path slots (1, 2, 4 and 8 Mi at two iterations per pass), then over the
+// iterations per pass (1, 2, 4, 8) at the best slot count; the fastest shape
+// goes into *cfg, whose other fields are kept. A first trial at *cfg readies the
+// volumes (masks, majorants) so that every trial does the same work.
+void autotune_server(vvr_ctx* ctx, const vvr_request* rq, uint32_t n, vvr_server_config* cfg) {
+    if (!ctx || !rq || !cfg || n == 0) throw Error{VVR_EINVAL, "autotune needs a context, requests and a config"};
+    vvr_server_config best = *cfg;
+    (void)trial_seconds(ctx, best, rq, n);
+    double best_s = 1e300;
+    for (uint32_t mi : {1u, 2u, 4u, 8u}) {
+        vvr_server_config c = *cfg;
+        c.slots = mi << 20;
+        c.iterations = 2;
+        const double t = trial_seconds(ctx, c, rq, n);
+        if (t < best_s) best_s = t, best = c;
+    }
+    const uint32_t slots = best.slots;
+    for (uint32_t it : {1u, 4u, 8u}) {
+        vvr_server_config c = *cfg;
+        c.slots = slots;
+        c.iterations = it;
+        const double t = trial_seconds(ctx, c, rq, n);
+        if (t < best_s) best_s = t, best = c;
+    }
+    cfg->slots = best.slots;
+    cfg->iterations = best.iterations;
+}
+
 }  // namespace
 
 extern "C" {
+
+int vvr_server_autotune(vvr_ctx* ctx, const vvr_request* requests, uint32_t n, vvr_server_config* config) {
+    return guarded([&] { autotune_server(ctx, requests, n, config); });
+}
 
 int vvr_server_create(vvr_ctx* ctx, const vvr_server_config* config, vvr_server** out) {
     return guarded([&] { create_server(ctx, config, out); });

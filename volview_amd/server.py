@@ -35,6 +35,21 @@ class Server:
         check(self._lib.vvr_server_create(ctx.handle, byref(cfg), byref(h)))
         self.handle = h
 
+    @staticmethod
+    def autotune(ctx: Context, requests, **config) -> dict:
+        """vvr_server_autotune: the engine shape (slots, iterations per pass) under
+        which `requests` ((renderer, params, width, height) tuples) finish fastest,
+        as keyword arguments for Server(ctx, ...) together with `config`."""
+        lib = load()
+        keep = [r[1] for r in requests]
+        arr = (Request * len(requests))(*[Request(r.handle, pointer(p), w, h, 0) for r, p, w, h in requests])
+        cfg = ServerConfig(config.get("hbm_budget", 0), config.get("slots", 0), config.get("pt_increment", 0),
+                           config.get("max_requests", 0), config.get("flight_steps", 0),
+                           config.get("shadow_steps", 0), config.get("iterations", 0))
+        check(lib.vvr_server_autotune(ctx.handle, arr, len(requests), byref(cfg)))
+        del keep
+        return dict(config, slots=int(cfg.slots), iterations=int(cfg.iterations))
+
     def submit(self, renderer: Renderer, params: FrameParams, width: int, height: int, tag: int = 0) -> int:
         rq = Request(renderer.handle, pointer(params), width, height, tag)  # copied by the call
         rid = c_uint64()
