@@ -126,6 +126,7 @@ def load() -> ctypes.CDLL:
             "oracle_surface_frame": (c_float, [P(OracleScene), P(_abi.Uniforms), P(c_float), c_uint32, c_int,
                                                P(c_float), c_int, c_float, c_float, c_int, P(c_float), P(c_float)]),
             "oracle_vlog_vexp": (None, [P(c_float), c_size_t, P(c_float), P(c_float)]),
+            "oracle_vpow": (None, [P(c_float), P(c_float), c_size_t, P(c_float)]),
             "oracle_tissue_sigma": (None, [P(_abi.Uniforms), c_uint32, c_float, P(c_float)]),
             "oracle_pixel_stream": (None, [P(_abi.Uniforms), c_uint32, c_uint32, c_uint32, P(c_uint32)]),
         }
@@ -584,6 +585,15 @@ def tissue_sigma(u, label, extinction) -> np.ndarray:
     out = (c_float * 3)()
     load().oracle_tissue_sigma(byref(u), label, extinction, out)
     return np.array(out, np.float64)
+
+
+def vpow(x: np.ndarray, y: np.ndarray) -> np.ndarray:
+    """The FP32 oracle's pow of each pair (the render path's vpow)."""
+    a = np.ascontiguousarray(x, np.float32)
+    b = np.ascontiguousarray(np.broadcast_to(y, a.shape), np.float32)
+    out = np.zeros_like(a)
+    load().oracle_vpow(_p(a, c_float), _p(b, c_float), a.size, _p(out, c_float))
+    return out
 
 
 def vlog_vexp(x: np.ndarray):

@@ -182,6 +182,8 @@ float oracle_surface_frame(const oracle_scene* s, const vvr_uniforms* u, const f
                            float n[3], float* surfaceness);
 /* The FP32 oracle's log and exp of n values (the render path's model of GLSL's). */
 void oracle_vlog_vexp(const float* x, size_t n, float* log_out, float* exp_out);
+/* The render path's pow (vec.hpp vpow) of each pair. */
+void oracle_vpow(const float* x, const float* y, size_t n, float* out);
 /* tissue_sigma (shading.glsl:161-168) of a sample of `label` with `extinction`. */
 void oracle_tissue_sigma(const vvr_uniforms* u, uint32_t label, float extinction, float out[3]);
 /* The PCG stream of pixel (px, py) after pt.comp's seeding and warm-up draw. */

@@ -101,9 +101,13 @@ template <class R> inline R vexp(R x) {
     if constexpr (std::is_same_v<R, float>) return vexp_f32(x);
     else return std::exp(x);
 }
-template <class R> inline R vpow(R x, R y) {  // GLSL's exp2(y log2 x), as exp(y log x)
-    if constexpr (std::is_same_v<R, float>) return vexp_f32(y * vlog_f32(x));
-    else return std::pow(x, y);
+// GLSL's pow (exp2(y log2 x), undefined for x < 0) as exp(y log x) of the base
+// clamped at 0: a base that rounding left just below 0 (1 - dot(v, h) of unit
+// vectors) gives 0, not NaN.
+template <class R> inline R vpow(R x, R y) {
+    const R b = x > R(0) ? x : R(0);
+    if constexpr (std::is_same_v<R, float>) return vexp_f32(y * vlog_f32(b));
+    else return std::pow(b, y);
 }
 template <class R> inline R cr_sin(R x) { return R(std::sin(double(x))); }
 template <class R> inline R cr_cos(R x) { return R(std::cos(double(x))); }

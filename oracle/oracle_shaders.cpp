@@ -2242,6 +2242,10 @@ void oracle_vlog_vexp(const float* x, size_t n, float* log_out, float* exp_out) 
     for (size_t i = 0; i < n; i++) log_out[i] = vlog(x[i]), exp_out[i] = vexp(x[i]);
 }
 
+void oracle_vpow(const float* x, const float* y, size_t n, float* out) {
+    for (size_t i = 0; i < n; i++) out[i] = vpow(x[i], y[i]);
+}
+
 void oracle_tissue_sigma(const vvr_uniforms* u, uint32_t label, float extinction, float out[3]) {
     oracle_scene sc{};
     Ctx<double> c(&sc, u);

@@ -702,3 +702,24 @@ def test_log_and_exp_models(oracle):
     assert np.all(ulps(ex, np.exp(y.astype(np.float64)).astype(np.float32)) <= 1)
     lo, ex = oracle.vlog_vexp(np.array([1.0, 0.0, -120.0, 0.0], np.float32))
     assert lo[0] == 0.0 and lo[1] == -np.inf and ex[2] == 0.0 and ex[3] == 1.0
+
+
+def test_pow_model(oracle):
+    """The render path's pow, exp(y log x) with those two, is within 2 + 3 |y ln x|
+    ulp of the correctly rounded x^y for the exponents the shaders use (3, 5, 6,
+    24) over bases in (0, 1]; 0 at a zero base; and 0, not NaN, for a base that
+    rounding left just below 0 (1 - dot(v, h) of unit vectors), where GLSL leaves
+    pow undefined."""
+    rng = np.random.default_rng(3)
+    x = rng.uniform(1e-3, 1.0, 200001).astype(np.float32)
+    for y in (3.0, 5.0, 6.0, 24.0):
+        got = oracle.vpow(x, np.float32(y))
+        want = (x.astype(np.float64) ** y).astype(np.float32)
+        err = ulps(got, want)
+        # log's relative error enters exp as an absolute error of about ulp |y ln x|
+        bound = 2 + 3 * np.abs(y * np.log(x.astype(np.float64)))
+        print(f"pow(x, {y}): worst {err.max()} ulp, worst share of the bound {(err / bound).max():.2f}")
+        assert np.all(err <= bound)
+    edge = oracle.vpow(np.array([0.0, -1.1920929e-07, -1e-30, 1.0], np.float32), np.float32(5.0))
+    assert edge[0] == 0.0 and edge[1] == 0.0 and edge[2] == 0.0 and edge[3] == 1.0
+    assert not np.isnan(edge).any()

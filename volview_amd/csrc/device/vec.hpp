@@ -82,8 +82,10 @@ __device__ __forceinline__ float vexp(float x) {
     return p * __uint_as_float(uint32_t(k1 + 127) << 23) * __uint_as_float(uint32_t(k2 + 127) << 23);
 }
 
-// GLSL defines pow(x, y) as exp2(y log2(x)); here exp(y log(x)) with the two above.
-__device__ __forceinline__ float vpow(float x, float y) { return vexp(y * vlog(x)); }
+// GLSL defines pow(x, y) as exp2(y log2(x)), undefined for x < 0; here exp(y
+// log(x)) with the two above, of the base clamped at 0: a base that rounding
+// left just below 0 (1 - dot(v, h) of unit vectors) gives 0, not NaN.
+__device__ __forceinline__ float vpow(float x, float y) { return vexp(y * vlog(x > 0.0f ? x : 0.0f)); }
 __device__ __forceinline__ float cr_sin(float x) { return float(sin(double(x))); }
 __device__ __forceinline__ float cr_cos(float x) { return float(cos(double(x))); }
 __device__ __forceinline__ float cr_acos(float x) { return float(acos(double(x))); }
