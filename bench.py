@@ -294,6 +294,7 @@ def run_serving(ctx, seconds: float, grace: float = 5.0, increment: int = 16, sl
         from collections import deque
         dvr_lat, pt_first, pt_paths, frame, useful = [], [], 0, 0, 0
         dvr_q, pt_fifo, active = deque(), deque(), {}
+        done_at = {}  # request -> [(time its frame completed, paths per pixel)]
         i, t0 = 0, time.perf_counter()
         while True:
             now = time.perf_counter() - t0
@@ -306,7 +307,8 @@ def run_serving(ctx, seconds: float, grace: float = 5.0, increment: int = 16, sl
                         pt_fifo.append(q.volume)
                     else:
                         old, old_done = active[q.volume]
-                        useful += old_done
+                        # the newest frame completed before the replacement arrived
+                        useful += max([d for t, d in done_at.get(old, []) if t <= q.arrival], default=0)
                         if old_done == 0:  # replaced before its first frame
                             pt_first.append(q.arrival - reqs[old].arrival)
                     active[q.volume] = [i, 0]
@@ -326,6 +328,7 @@ def run_serving(ctx, seconds: float, grace: float = 5.0, increment: int = 16, sl
                 pt_r[v].record(frame, plain_params[k])
                 pt_r[v].synchronize()
                 done += increment
+                done_at.setdefault(k, []).append((time.perf_counter() - t0, done))
                 pt_paths += 1024 * 1024 * increment
                 if done == increment:
                     pt_first.append(time.perf_counter() - t0 - reqs[k].arrival)
