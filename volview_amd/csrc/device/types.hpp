@@ -45,6 +45,12 @@ struct DevScene {
     const float* __restrict__ density_f;   // the same values as floats, c / 65535 (the cubic fetch's window)
     const uint8_t* __restrict__ labels;    // label per voxel
     const uint32_t* __restrict__ mask;     // RG16: r low half, g high half
+    // One bit per unit cell of the mask (the 8 clamped corners a sample there
+    // interpolates; cells 0..n per axis, in 4^3-cell micro-bricks of grid mg): 0
+    // when no corner reaches the frame's solid or ghost threshold, so that no
+    // label can be drawn there (null: not built).
+    const uint32_t* __restrict__ mask_cells;
+    BrickGrid mg;
     const uint32_t* __restrict__ normals;  // RGBA8 snorm, x in the low byte
     const float* __restrict__ majorant;    // ceil(dims / 8), x fastest
     int mx, my, mz;

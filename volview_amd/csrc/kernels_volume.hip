@@ -350,6 +350,34 @@ __global__ void __launch_bounds__(kBlock) k_mask_blur(const uint32_t* __restrict
     dst[o] = unorm16_bits(s0 / taps.e) | (unorm16_bits(s1 / taps.e) << 16);
 }
 
+// Mask cells (DevScene::mask_cells): one wave per 4^3-cell micro-brick of the
+// cell grid cg (n + 1 cells per axis; cell c covers the clamped corners c - 1
+// and c). A cell's bit is set when its corners' largest solid code reaches
+// solid_code or their largest ghost code reaches 32768 (0.5).
+__global__ void __launch_bounds__(64) k_mask_cells(const uint32_t* __restrict__ mask, BrickGrid g, BrickGrid cg,
+                                                   uint32_t solid_code, uint32_t* __restrict__ bits) {
+    const uint32_t mb = blockIdx.x;
+    const int bx = int(mb % uint32_t(cg.bx)), by = int((mb / uint32_t(cg.bx)) % uint32_t(cg.by));
+    const int bz = int(mb / (uint32_t(cg.bx) * uint32_t(cg.by)));
+    const int w = int(threadIdx.x);
+    const int cx = 4 * bx + (w & 3), cy = 4 * by + ((w >> 2) & 3), cz = 4 * bz + (w >> 4);
+    bool on = false;
+    if (cx <= g.nx && cy <= g.ny && cz <= g.nz) {
+        uint32_t mr = 0u, mgh = 0u;
+        for (int k = 0; k < 8; k++) {
+            const int x = min(max(cx - 1 + (k & 1), 0), g.nx - 1);
+            const int y = min(max(cy - 1 + ((k >> 1) & 1), 0), g.ny - 1);
+            const int z = min(max(cz - 1 + (k >> 2), 0), g.nz - 1);
+            const uint32_t v = mask[g.at(x, y, z)];
+            mr = max(mr, v & 0xffffu);
+            mgh = max(mgh, v >> 16);
+        }
+        on = mr >= solid_code || mgh >= 32768u;
+    }
+    const uint64_t m = __ballot(on);
+    if (w < 2) bits[size_t(mb) * 2u + uint32_t(w)] = uint32_t(m >> (32 * w));
+}
+
 // ---- majorant (majorant.comp) ----------------------------------------------------------
 //
 // Brick b (8^3 voxels) bounds the extinction over the voxels 8b - 2 .. 8b + 9 of
@@ -623,6 +651,13 @@ void launch_mask_blur(const uint32_t* src, uint32_t* dst, int axis, int radius, 
     taps.e = std::ldexp(1.0f, 2 * radius);
     const dim3 grid(unsigned(g.bx + kBlock / 64 - 1) / (kBlock / 64), unsigned(g.by), unsigned(g.bz));
     hipLaunchKernelGGL(k_mask_blur, grid, dim3(kBlock), 0, st, src, dst, axis, radius, g, taps);
+}
+
+void launch_mask_cells(const uint32_t* mask, const BrickGrid& g, int mode, uint32_t* bits, hipStream_t st) {
+    const BrickGrid cg = mask_cell_grid(g);
+    const uint32_t solid_code = mode == 3 ? 16384u : 32768u;  // 0.25 (hybrid) or 0.5, as unorm16 codes rounded up
+    hipLaunchKernelGGL(k_mask_cells, dim3(unsigned(size_t(cg.bx) * cg.by * cg.bz)), dim3(64), 0, st, mask, g, cg,
+                       solid_code, bits);
 }
 
 void launch_majorant(const uint16_t* density, const uint32_t* mask, float* out, const BrickGrid& g, int mode,
