@@ -471,20 +471,24 @@ __device__ inline bool drawn_label_near(const DevScene& s, V3 idx, uint32_t mode
 // thresholds by at least 7.6e-6 (a unorm16 step), which the trilinear lerps'
 // rounding cannot close, so mask_at would give m.r below the solid and m.g below
 // the ghost threshold.
-__device__ __forceinline__ bool mask_cell_may_draw(const DevScene& s, V3 idx) {
+__device__ __forceinline__ uint32_t mask_cell(const DevScene& s, V3 idx) {
     const int cx = iclamp(fidx(floorf(idx.x)), -1, s.g.nx - 1) + 1;
     const int cy = iclamp(fidx(floorf(idx.y)), -1, s.g.ny - 1) + 1;
     const int cz = iclamp(fidx(floorf(idx.z)), -1, s.g.nz - 1) + 1;
-    const uint32_t i = s.mg.at(cx, cy, cz);
-    return ((s.mask_cells[i >> 5] >> (i & 31u)) & 1u) != 0u;
+    return s.mg.at(cx, cy, cz);
 }
 
 __device__ __forceinline__ Sample apply_labels(const DevScene& s, Sample smp, V3 idx) {
     const uint32_t mode = uint32_t(s.u->labels[0]);
     if (mode == 0u) return smp;
-    if (s.mask_cells && !mask_cell_may_draw(s, idx)) {  // the branch below, without the loads
-        smp.extinction = 0.0f;
-        return smp;
+    if (s.mask_cells) {  // the branches below, without the mask and label loads
+        const uint32_t c = mask_cell(s, idx);
+        if (((s.mask_cells[c >> 5] >> (c & 31u)) & 1u) == 0u) {
+            smp.extinction = 0.0f;
+            return smp;
+        }
+        const uint32_t l = s.cell_labels[c];
+        if (l != 0u) return with_label(s, smp, l, mode, false);
     }
     const F2 m = mask_at(s, idx);
     uint32_t label = 0u;

@@ -51,6 +51,10 @@ struct DevScene {
     // label can be drawn there (null: not built).
     const uint32_t* __restrict__ mask_cells;
     BrickGrid mg;
+    // Per mask cell, in the same layout: L when all 8 corners are solid (at least
+    // the solid threshold) and carry the same label L, drawn in the frame's mode
+    // (then the label search returns L: its first candidate is a corner); else 0.
+    const uint8_t* __restrict__ cell_labels;
     const uint32_t* __restrict__ normals;  // RGBA8 snorm, x in the low byte
     const float* __restrict__ majorant;    // ceil(dims / 8), x fastest
     int mx, my, mz;

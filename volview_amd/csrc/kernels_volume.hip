@@ -350,32 +350,45 @@ __global__ void __launch_bounds__(kBlock) k_mask_blur(const uint32_t* __restrict
     dst[o] = unorm16_bits(s0 / taps.e) | (unorm16_bits(s1 / taps.e) << 16);
 }
 
-// Mask cells (DevScene::mask_cells): one wave per 4^3-cell micro-brick of the
-// cell grid cg (n + 1 cells per axis; cell c covers the clamped corners c - 1
-// and c). A cell's bit is set when its corners' largest solid code reaches
-// solid_code or their largest ghost code reaches 32768 (0.5).
-__global__ void __launch_bounds__(64) k_mask_cells(const uint32_t* __restrict__ mask, BrickGrid g, BrickGrid cg,
-                                                   uint32_t solid_code, uint32_t* __restrict__ bits) {
+// Mask cells (DevScene::mask_cells, cell_labels): one wave per 4^3-cell
+// micro-brick of the cell grid cg (n + 1 cells per axis; cell c covers the
+// clamped corners c - 1 and c). A cell's bit is set when its corners' largest
+// solid code reaches solid_code or their largest ghost code reaches 32768 (0.5);
+// its label byte is L when every corner's solid code reaches solid_code and
+// every corner has label L, drawn solid in `mode` (label 0 excepted), else 0.
+__global__ void __launch_bounds__(64) k_mask_cells(const uint32_t* __restrict__ mask,
+                                                   const uint8_t* __restrict__ labels, BrickGrid g, BrickGrid cg,
+                                                   uint32_t solid_code, uint32_t mode, const vvr_uniforms* __restrict__ u,
+                                                   uint32_t* __restrict__ bits, uint8_t* __restrict__ cell_labels) {
     const uint32_t mb = blockIdx.x;
     const int bx = int(mb % uint32_t(cg.bx)), by = int((mb / uint32_t(cg.bx)) % uint32_t(cg.by));
     const int bz = int(mb / (uint32_t(cg.bx) * uint32_t(cg.by)));
     const int w = int(threadIdx.x);
     const int cx = 4 * bx + (w & 3), cy = 4 * by + ((w >> 2) & 3), cz = 4 * bz + (w >> 4);
     bool on = false;
+    uint32_t uniform_label = 0u;
     if (cx <= g.nx && cy <= g.ny && cz <= g.nz) {
-        uint32_t mr = 0u, mgh = 0u;
+        uint32_t mr = 0u, mgh = 0u, lo = 0xffffu, l0 = 0u;
+        bool same = true;
         for (int k = 0; k < 8; k++) {
             const int x = min(max(cx - 1 + (k & 1), 0), g.nx - 1);
             const int y = min(max(cy - 1 + ((k >> 1) & 1), 0), g.ny - 1);
             const int z = min(max(cz - 1 + (k >> 2), 0), g.nz - 1);
-            const uint32_t v = mask[g.at(x, y, z)];
+            const uint32_t at = g.at(x, y, z);
+            const uint32_t v = mask[at];
             mr = max(mr, v & 0xffffu);
+            lo = min(lo, v & 0xffffu);
             mgh = max(mgh, v >> 16);
+            const uint32_t l = labels[at];
+            if (k == 0) l0 = l;
+            same = same && l == l0;
         }
         on = mr >= solid_code || mgh >= 32768u;
+        if (lo >= solid_code && same && l0 != 0u && solid(u, l0, mode)) uniform_label = l0;
     }
     const uint64_t m = __ballot(on);
     if (w < 2) bits[size_t(mb) * 2u + uint32_t(w)] = uint32_t(m >> (32 * w));
+    cell_labels[size_t(mb) * 64u + uint32_t(w)] = uint8_t(uniform_label);
 }
 
 // ---- majorant (majorant.comp) ----------------------------------------------------------
@@ -653,11 +666,12 @@ void launch_mask_blur(const uint32_t* src, uint32_t* dst, int axis, int radius, 
     hipLaunchKernelGGL(k_mask_blur, grid, dim3(kBlock), 0, st, src, dst, axis, radius, g, taps);
 }
 
-void launch_mask_cells(const uint32_t* mask, const BrickGrid& g, int mode, uint32_t* bits, hipStream_t st) {
+void launch_mask_cells(const uint32_t* mask, const uint8_t* labels, const BrickGrid& g, int mode,
+                       const vvr_uniforms* u, uint32_t* bits, uint8_t* cell_labels, hipStream_t st) {
     const BrickGrid cg = mask_cell_grid(g);
     const uint32_t solid_code = mode == 3 ? 16384u : 32768u;  // 0.25 (hybrid) or 0.5, as unorm16 codes rounded up
-    hipLaunchKernelGGL(k_mask_cells, dim3(unsigned(size_t(cg.bx) * cg.by * cg.bz)), dim3(64), 0, st, mask, g, cg,
-                       solid_code, bits);
+    hipLaunchKernelGGL(k_mask_cells, dim3(unsigned(size_t(cg.bx) * cg.by * cg.bz)), dim3(64), 0, st, mask, labels, g,
+                       cg, solid_code, uint32_t(mode), u, bits, cell_labels);
 }
 
 void launch_majorant(const uint16_t* density, const uint32_t* mask, float* out, const BrickGrid& g, int mode,

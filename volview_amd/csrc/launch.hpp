@@ -31,10 +31,12 @@ void launch_mask_binary(const uint16_t* density, const uint8_t* labels, SoftGrid
                         hipStream_t st);
 // mask.comp, passes 1-3: binomial blur of radius r along `axis`.
 void launch_mask_blur(const uint32_t* src, uint32_t* dst, int axis, int radius, const BrickGrid& g, hipStream_t st);
-// The cell grid of DevScene::mask_cells (n + 1 cells per axis) and its bits
-// (mask_cell_grid(g).padded() / 32 words) for label mode `mode`.
+// The cell grid of DevScene::mask_cells (n + 1 cells per axis), its bits
+// (mask_cell_grid(g).padded() / 32 words) and label bytes (padded() bytes) for
+// label mode `mode` and the label flags of `u`.
 inline BrickGrid mask_cell_grid(const BrickGrid& g) { return make_brick_grid(g.nx + 1, g.ny + 1, g.nz + 1); }
-void launch_mask_cells(const uint32_t* mask, const BrickGrid& g, int mode, uint32_t* bits, hipStream_t st);
+void launch_mask_cells(const uint32_t* mask, const uint8_t* labels, const BrickGrid& g, int mode,
+                       const vvr_uniforms* u, uint32_t* bits, uint8_t* cell_labels, hipStream_t st);
 
 // majorant.comp: `mask` may be null when mode is 0.
 void launch_majorant(const uint16_t* density, const uint32_t* mask, float* out, const BrickGrid& g, int mode,

@@ -101,6 +101,7 @@ void set_labels(vvr_renderer* r, const uint8_t* labels, size_t n, const vvr_soft
     if (!r->mask) r->mask = dalloc<uint32_t>(g.padded(), "label masks");
     if (!r->mask_b) r->mask_b = dalloc<uint32_t>(g.padded(), "label masks (blur)");
     if (!r->mask_cells) r->mask_cells = dalloc<uint32_t>(mask_cell_grid(g).padded() / 32, "label mask cells");
+    if (!r->cell_labels) r->cell_labels = dalloc<uint8_t>(mask_cell_grid(g).padded(), "label mask cell labels");
     uint8_t* staging = dalloc<uint8_t>(nvox, "label staging");
     hipError_t e = hipMemcpy(staging, labels, nvox, hipMemcpyHostToDevice);
     if (e == hipSuccess) {

@@ -314,6 +314,7 @@ struct vvr_renderer {
     uint32_t* mask = nullptr;    // image A: what rendering samples
     uint32_t* mask_b = nullptr;  // image B: blur partner
     uint32_t* mask_cells = nullptr;  // one bit per mask cell (launch_mask_cells)
+    uint8_t* cell_labels = nullptr;  // and its uniform drawn label, or 0
     uint32_t* soft = nullptr;
     SoftGrid soft_grid{};
     float4* lut = nullptr;
@@ -416,7 +417,8 @@ struct vvr_renderer {
             run(VVR_KERNEL_MASK, [&] { launch_mask_blur(mask_b, mask, 0, radius, g, st); });
             run(VVR_KERNEL_MASK, [&] { launch_mask_blur(mask, mask_b, 1, radius, g, st); });
             run(VVR_KERNEL_MASK, [&] { launch_mask_blur(mask_b, mask, 2, radius, g, st); });
-            run(VVR_KERNEL_MASK, [&] { launch_mask_cells(mask, g, int(opacity.mode), mask_cells, st); });
+            run(VVR_KERNEL_MASK,
+                [&] { launch_mask_cells(mask, labels, g, int(opacity.mode), du, mask_cells, cell_labels, st); });
             mask_key = opacity;
         }
         if (!majorant_key || *majorant_key != opacity) {
@@ -460,6 +462,7 @@ struct vvr_renderer {
         s.labels = labels;
         s.mask = mask;
         s.mask_cells = mask_cells;
+        s.cell_labels = cell_labels;
         s.mg = mask_cell_grid(grid);
         s.normals = normals;
         s.majorant = majorant;
@@ -580,7 +583,7 @@ struct vvr_renderer {
         }
         free_frame();
         wf.reset();
-        dfree(density), dfree(density_f), dfree(normals), dfree(majorant), dfree(labels), dfree(mask), dfree(mask_b), dfree(mask_cells),
+        dfree(density), dfree(density_f), dfree(normals), dfree(majorant), dfree(labels), dfree(mask), dfree(mask_b), dfree(mask_cells), dfree(cell_labels),
             dfree(soft);
         dfree(maj_cells), dfree(maj_nz);
         dfree(shadow), dfree(ao), dfree(env_rad), dfree(env_table), dfree(env_dif);
