@@ -127,6 +127,7 @@ def load() -> ctypes.CDLL:
                                                P(c_float), c_int, c_float, c_float, c_int, P(c_float), P(c_float)]),
             "oracle_vlog_vexp": (None, [P(c_float), c_size_t, P(c_float), P(c_float)]),
             "oracle_vpow": (None, [P(c_float), P(c_float), c_size_t, P(c_float)]),
+            "oracle_div6_mismatches": (ctypes.c_uint64, [c_uint32, c_uint32, c_uint32, ctypes.c_int]),
             "oracle_tissue_sigma": (None, [P(_abi.Uniforms), c_uint32, c_float, P(c_float)]),
             "oracle_pixel_stream": (None, [P(_abi.Uniforms), c_uint32, c_uint32, c_uint32, P(c_uint32)]),
         }
@@ -602,3 +603,11 @@ def vlog_vexp(x: np.ndarray):
     lo, ex = np.zeros_like(a), np.zeros_like(a)
     load().oracle_vlog_vexp(_p(a, c_float), a.size, _p(lo, c_float), _p(ex, c_float))
     return lo, ex
+
+
+def div6_mismatches(first: int, last: int, stride: int, mode: int) -> int:
+    """Bit patterns first..last (stride apart) where the kernels' division-free
+    x / 6 differs from the division: as dividends (mode 0) or as the fraction of
+    the cubic B-spline weights (mode 1); mode 2 compares the uncorrected
+    product x RN(1/6) instead."""
+    return int(load().oracle_div6_mismatches(first, last, stride, mode))

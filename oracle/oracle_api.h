@@ -184,6 +184,15 @@ float oracle_surface_frame(const oracle_scene* s, const vvr_uniforms* u, const f
 void oracle_vlog_vexp(const float* x, size_t n, float* log_out, float* exp_out);
 /* The render path's pow (vec.hpp vpow) of each pair. */
 void oracle_vpow(const float* x, const float* y, size_t n, float* out);
+/* The kernels' x / 6 without the division (scene.hpp div6_normal / div6: x / 2
+ * times RN(1/3) plus one FMA-exact correction; a true division for |x| < 2^-125),
+ * checked against x / 6.0f. mode 0: over the float bit patterns first .. last
+ * (stride apart) as dividends, through div6; mode 1: those patterns as the
+ * fraction f of the cubic weights w0..w3 (common.glsl:88-95), each numerator
+ * through the kernels' form; mode 2: the uncorrected product x RN(1/6) in place of
+ * the kernels' form (a check that the comparison can fail). Returns the number of
+ * mismatching bit patterns. */
+uint64_t oracle_div6_mismatches(uint32_t first, uint32_t last, uint32_t stride, int mode);
 /* tissue_sigma (shading.glsl:161-168) of a sample of `label` with `extinction`. */
 void oracle_tissue_sigma(const vvr_uniforms* u, uint32_t label, float extinction, float out[3]);
 /* The PCG stream of pixel (px, py) after pt.comp's seeding and warm-up draw. */

@@ -2246,6 +2246,36 @@ void oracle_vpow(const float* x, const float* y, size_t n, float* out) {
     for (size_t i = 0; i < n; i++) out[i] = vpow(x[i], y[i]);
 }
 
+// The kernels' division-free x / 6 (volview_amd/csrc/device/scene.hpp), restated
+// here only to check it against the division this oracle performs.
+static float kernel_div6_normal(float x) {
+    const float third = 1.0f / 3.0f;
+    const float h = x * 0.5f;
+    const float q = h * third;
+    return std::fma(std::fma(-q, 3.0f, h), third, q);
+}
+static float kernel_div6(float x) { return std::fabs(x) < 0x1p-125f ? x / 6.0f : kernel_div6_normal(x); }
+
+uint64_t oracle_div6_mismatches(uint32_t first, uint32_t last, uint32_t stride, int mode) {
+    uint64_t bad = 0;
+    auto differ = [](float a, float b) { return f32_bits(a) != f32_bits(b); };
+    for (uint64_t b = first; b <= last; b += stride) {
+        const float x = f32_from_bits(uint32_t(b));
+        if (mode == 0) {
+            bad += differ(kernel_div6(x), x / 6.0f);
+        } else if (mode == 2) {  // the product with RN(1/6) alone, without the correction
+            bad += differ(x * (1.0f / 6.0f), x / 6.0f);
+        } else {
+            const float f = x, f2 = f * f, f3 = f2 * f;
+            const float n0 = (1.0f - 3.0f * f) + 3.0f * f2 - f3, n1 = (4.0f - 6.0f * f2) + 3.0f * f3,
+                        n2 = ((1.0f + 3.0f * f) + 3.0f * f2) - 3.0f * f3;
+            bad += differ(kernel_div6_normal(n0), n0 / 6.0f) || differ(kernel_div6_normal(n1), n1 / 6.0f) ||
+                   differ(kernel_div6_normal(n2), n2 / 6.0f) || differ(kernel_div6(f3), f3 / 6.0f);
+        }
+    }
+    return bad;
+}
+
 void oracle_tissue_sigma(const vvr_uniforms* u, uint32_t label, float extinction, float out[3]) {
     oracle_scene sc{};
     Ctx<double> c(&sc, u);

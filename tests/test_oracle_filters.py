@@ -93,3 +93,19 @@ def test_trilinear_closed_form_and_clamp_to_edge(oracle):
                     acc += w * v[z, y, x]
         assert sc.density_linear(u, idx, 64) == pytest.approx(acc, abs=1e-7)
         assert sc.density_linear(u, idx, 32) == pytest.approx(acc, abs=2e-7)
+
+
+def test_kernels_division_free_cubic_weights(oracle):
+    """The kernels compute the cubic weights' x / 6 as (x / 2) RN(1/3) plus one
+    FMA-exact correction (a division below 2^-125). That form equals the
+    division for every dividend tried and for the weights of every fraction f
+    tried: all floats 101 apart of either sign, all subnormals up to 2^-140,
+    every 23rd f in [0, 1) and every 3rd f around 2^-42, where f^3 crosses
+    2^-125 (on the CPU all 2^31 positive dividends and all 2^30 fractions were
+    checked). The product with RN(1/6) alone differs, so the check can fail."""
+    assert oracle.div6_mismatches(0, 0x7F7FFFFF, 101, 0) == 0
+    assert oracle.div6_mismatches(0x80000000, 0xFF7FFFFF, 101, 0) == 0
+    assert oracle.div6_mismatches(1, 0x00080000, 1, 0) == 0
+    assert oracle.div6_mismatches(0, 0x3F7FFFFF, 23, 1) == 0
+    assert oracle.div6_mismatches(0x2A000000, 0x2B800000, 3, 1) == 0
+    assert oracle.div6_mismatches(0x3F800000, 0x40800000, 101, 2) > 1000
