@@ -263,11 +263,13 @@ def test_frame_over_the_item_limit_is_refused(ctx):
     assert np.isfinite(acc).all() and r.accumulated_samples() == 2
 
 
-@pytest.mark.parametrize("mode", ["plain", "lit", "labels"])
+@pytest.mark.parametrize("mode", ["plain", "lit", "labels", "preint", "clip"])
 def test_dvr_passes_equal_one_loop(ctx, monkeypatch, mode):
-    """The ray marcher in passes (march, shading of the listed samples, per-pixel
-    sums, the long rays marching on) gives the one-loop frame bit for bit: the
-    DVR-1 recipe at 128^3, with lighting volumes, and with organ labels."""
+    """The ray marcher in passes gives the one-loop frame bit for bit, with the
+    first round's march as parallel segments (walk, segments, merge) and as one
+    loop per ray: the DVR-1 recipe at 128^3, with lighting volumes, with organ
+    labels, pre-integrated (which marches without segments) and with a clip
+    plane."""
     dims = (128, 128, 128)
     hu = phantom.ct(dims, 5)
     r = Renderer(ctx, hu, TransferFunction.preset(2 if mode == "labels" else 0))
@@ -278,13 +280,19 @@ def test_dvr_passes_equal_one_loop(ctx, monkeypatch, mode):
     p = workloads.dvr_params(r, 3, lighting=mode == "lit")
     if n:
         workloads.organ_label_params(p, n, 5, _abi.LABELS_ISOLATE)
+    if mode == "preint":
+        p.preintegrate = 1
+    if mode == "clip":
+        cp = p.clipping.planes[0]
+        cp.enabled, cp.plane, cp.cut_positive, cp.position = 1, 0, 1, 4.0
     r.resize(160, 144)
     frames = {}
-    for env in ("1", "0"):
+    for env in ("1", "2", "0"):
         monkeypatch.setenv("VVR_DVR_PASSES", env)
         r.record(1, p)
         frames[env] = r.read_hdr()
-    a, b = frames["1"], frames["0"]
+    a, b, c = frames["1"], frames["2"], frames["0"]
     print(f"{mode}: {(a[..., 3] < 0.5).mean():.3f} of pixels opaque")
     assert (a[..., 3] < 0.99).mean() > 0.02
-    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert np.array_equal(b.view(np.uint32), c.view(np.uint32))
+    assert np.array_equal(a.view(np.uint32), c.view(np.uint32))

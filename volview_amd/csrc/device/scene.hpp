@@ -237,12 +237,48 @@ __device__ __forceinline__ void load_slice(const uint16_t* __restrict__ d, const
         for (int x = 0; x < 4; x++) v[y][x] = unorm16_value(float(load_at(d, w.x[x] + base)));
     }
 }
-__device__ __forceinline__ void load_slice(const float* __restrict__ d, const WindowAddr& w, int z, float v[4][4]) {
+
+// The window of the float copy in rows (DevScene::density_f): each of its 16
+// rows is 4 consecutive floats, read with one 16-byte load, when the window's
+// x range lies in the padded box (always inside the volume's box and one voxel
+// around it); else texel by texel with clamped indices, the same values.
+struct RowWindow {
+    const float* row[4][4];  // [z][y]: the padded row's texel x = -kRowPad
+    bool whole;              // the window's rows are 4 consecutive texels from x[0]
+    int x[4];                // the 4 clamped x offsets in the padded row
+};
+__device__ __forceinline__ RowWindow row_window(const DevScene& s, int ix, int iy, int iz) {
+    const BrickGrid& g = s.g;
+    const size_t rx = size_t(g.nx + 2 * kRowPad), ry = size_t(g.ny + 2 * kRowPad);
+    RowWindow w;
+    const int x0 = ix - 1;
+    w.whole = x0 >= -kRowPad && x0 + 3 <= g.nx - 1 + kRowPad;
 #pragma unroll
-    for (int y = 0; y < 4; y++) {
-        const uint32_t base = w.y[y] + w.z[z];
+    for (int k = 0; k < 4; k++) w.x[k] = iclamp(x0 + k, -kRowPad, g.nx - 1 + kRowPad) + kRowPad;
 #pragma unroll
-        for (int x = 0; x < 4; x++) v[y][x] = load_at(d, w.x[x] + base);
+    for (int c = 0; c < 4; c++) {
+        const size_t zz = size_t(iclamp(iz - 1 + c, -kRowPad, g.nz - 1 + kRowPad) + kRowPad);
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const size_t yy = size_t(iclamp(iy - 1 + b, -kRowPad, g.ny - 1 + kRowPad) + kRowPad);
+            w.row[c][b] = s.density_f + (zz * ry + yy) * rx;
+        }
+    }
+    return w;
+}
+__device__ __forceinline__ void load_slice(const RowWindow& w, int z, float v[4][4]) {
+    if (w.whole) {
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+            float4 t;
+            __builtin_memcpy(&t, w.row[z][y] + w.x[0], 16);
+            v[y][0] = t.x, v[y][1] = t.y, v[y][2] = t.z, v[y][3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int y = 0; y < 4; y++)
+#pragma unroll
+            for (int x = 0; x < 4; x++) v[y][x] = w.row[z][y][w.x[x]];
     }
 }
 // The slice's 4 rows lerped along x at tap t.
@@ -283,12 +319,15 @@ __device__ inline float density_cubic(const DevScene& s, V3 idx) {
     const float ax[2] = {tx.h0 - (ix - 1.0f), tx.h1 - (ix + 1.0f)};
     const float ay[2] = {ty.h0 - (iy - 1.0f), ty.h1 - (iy + 1.0f)};
     const float az[2] = {tz.h0 - (iz - 1.0f), tz.h1 - (iz + 1.0f)};
-    const WindowAddr w = window_addr<kFloatTexels ? 4 : 2>(s.g, fidx(ix), fidx(iy), fidx(iz));
+    WindowAddr w;
+    RowWindow rw;
+    if (kFloatTexels) rw = row_window(s, fidx(ix), fidx(iy), fidx(iz));
+    else w = window_addr<2>(s.g, fidx(ix), fidx(iy), fidx(iz));
     float Y[2][2][4];  // [x tap][y tap][z slice]: the fetches lerped along x and y
 #pragma unroll
     for (int z = 0; z < 4; z++) {
         float v[4][4];
-        if (kFloatTexels) load_slice(s.density_f, w, z, v);
+        if (kFloatTexels) load_slice(rw, z, v);
         else load_slice(s.density, w, z, v);
 #pragma unroll
         for (int a = 0; a < 2; a++) {
@@ -672,12 +711,20 @@ __device__ __forceinline__ void intersect_volume(const vvr_uniforms* u, V3 origi
     tf = gmin(gmin(tmax.x, tmax.y), tmax.z);
 }
 
-__device__ __forceinline__ float brick_majorant(const DevScene& s, V3 idx) {
-    if (s.u->path[3] == 0u) return 1e30f;
+// The majorant grid's brick of idx (clamped to the grid), or -1 when skipping
+// is off (every brick then bounds by 1e30).
+__device__ __forceinline__ int brick_index(const DevScene& s, V3 idx) {
+    if (s.u->path[3] == 0u) return -1;
     const V3 b = vfloor((idx + 0.5f) / 8.0f);
     const int bx = iclamp(fidx(b.x), 0, s.mx - 1), by = iclamp(fidx(b.y), 0, s.my - 1),
               bz = iclamp(fidx(b.z), 0, s.mz - 1);
-    return load_at(s.majorant, (uint32_t(bx) + uint32_t(s.mx) * (uint32_t(by) + uint32_t(s.my) * uint32_t(bz))) * 4u);
+    return int(uint32_t(bx) + uint32_t(s.mx) * (uint32_t(by) + uint32_t(s.my) * uint32_t(bz)));
+}
+__device__ __forceinline__ float brick_majorant_at(const DevScene& s, int b) {
+    return b < 0 ? 1e30f : load_at(s.majorant, uint32_t(b) * 4u);
+}
+__device__ __forceinline__ float brick_majorant(const DevScene& s, V3 idx) {
+    return brick_majorant_at(s, brick_index(s, idx));
 }
 
 // 1 / d_idx per axis, with brick_exit's guard against zero components.

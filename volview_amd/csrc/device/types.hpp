@@ -29,6 +29,14 @@ struct BrickGrid {
     VVR_HD size_t padded() const { return size_t(zstride) * size_t(bz); }
 };
 
+// The ray marcher's float copy of the density (DevScene::density_f): linear, x
+// fastest, with a border of kRowPad replicated texels on each side of every
+// axis, so that the 4 texels of any cubic-window row are consecutive.
+constexpr int kRowPad = 2;
+VVR_HD size_t row_volume_texels(int nx, int ny, int nz) {
+    return size_t(nx + 2 * kRowPad) * size_t(ny + 2 * kRowPad) * size_t(nz + 2 * kRowPad);
+}
+
 inline BrickGrid make_brick_grid(int nx, int ny, int nz) {
     BrickGrid g;
     g.nx = nx, g.ny = ny, g.nz = nz;
@@ -42,7 +50,7 @@ inline BrickGrid make_brick_grid(int nx, int ny, int nz) {
 struct DevScene {
     BrickGrid g;
     const uint16_t* __restrict__ density;  // unorm16
-    const float* __restrict__ density_f;   // the same values as floats, c / 65535 (the cubic fetch's window)
+    const float* __restrict__ density_f;   // the same values as floats, c / 65535, in rows (row_volume_texels)
     const uint8_t* __restrict__ labels;    // label per voxel
     const uint32_t* __restrict__ mask;     // RG16: r low half, g high half
     // One bit per unit cell of the mask (the 8 clamped corners a sample there

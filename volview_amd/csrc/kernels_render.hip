@@ -57,18 +57,36 @@ struct MarchState {
     float tt, transmittance, prev_density;
     bool refined;
 };
+// The first step's distance, jittered by the pixel's noise (dvr.comp:41-44), or
+// the far end for a ray that misses the volume.
+__device__ __forceinline__ float march_start(const DvrRay& r, float step_mm, uint32_t px, uint32_t py) {
+    return r.t0 < r.t1 ? r.t0 + ign(float(px), float(py)) * step_mm : r.t1;
+}
 __device__ __forceinline__ MarchState march_begin(const DevScene& sc, const DvrRay& r, uint32_t px, uint32_t py) {
     const float step_mm = sc.u->dims[3];
     MarchState m;
     m.transmittance = 1.0f;
     m.refined = false;
-    m.tt = r.t1;  // nothing to march
+    m.tt = march_start(r, step_mm, px, py);
     m.prev_density = -1.0f;
-    if (r.t0 < r.t1) {
-        m.tt = r.t0 + ign(float(px), float(py)) * step_mm;
-        m.prev_density = density_view<true>(sc, r.o_idx + r.d_idx * (m.tt - step_mm));
-    }
+    if (r.t0 < r.t1) m.prev_density = density_view<true>(sc, r.o_idx + r.d_idx * (m.tt - step_mm));
     return m;
+}
+
+// The first surface (dvr.comp:150-168): five bisection steps back over the
+// last step for where the extinction reaches half the sample's.
+__device__ __forceinline__ V3 first_surface(const DevScene& sc, V3 idx, V3 d_idx, float extinction) {
+    const float step_mm = sc.u->dims[3];
+    float inside = 0.0f, outside = -step_mm;
+    const float target = 0.5f * extinction;
+    for (int i = 0; i < 5; i++) {
+        const float mid = 0.5f * (inside + outside);
+        const V3 at = idx + d_idx * mid;
+        const bool opaque = labelled(sc, tf_point(sc, density_view<true>(sc, at)), at).extinction >= target;
+        inside = opaque ? mid : inside;
+        outside = opaque ? outside : mid;
+    }
+    return idx + d_idx * inside;
 }
 
 // dvr.comp:41-147 from state `m`: empty-space skipping (dvr.comp:118-125),
@@ -112,16 +130,7 @@ __device__ __forceinline__ bool dvr_march(const DevScene& sc, const DvrRay& r, M
             idx = idx - d_idx * (step_mm * (1.0f - preint_half_depth(sc, front_density, density)));
         } else if (!m.refined && alpha > 0.2f && m.transmittance > 0.9f) {
             m.refined = true;
-            float inside = 0.0f, outside = -step_mm;
-            const float target = 0.5f * s.extinction;
-            for (int i = 0; i < 5; i++) {
-                const float mid = 0.5f * (inside + outside);
-                const V3 at = idx + d_idx * mid;
-                const bool opaque = labelled(sc, tf_point(sc, density_view<true>(sc, at)), at).extinction >= target;
-                inside = opaque ? mid : inside;
-                outside = opaque ? outside : mid;
-            }
-            idx = idx + d_idx * inside;
+            idx = first_surface(sc, idx, d_idx, s.extinction);
         }
         m.transmittance *=
             1.0f - visit(idx, s, density, segment ? front_density : -1.0f, cap_plane, cut, alpha, m.transmittance);
@@ -232,8 +241,9 @@ __device__ __forceinline__ void list_push(bool push, uint32_t value, uint32_t* _
     if (push) list[base + lane_rank64(mask)] = value;
 }
 
+// Clears the sample list's count and the segmented march's item counter.
 __global__ void k_dvr_reset(uint32_t* __restrict__ count) {
-    if (threadIdx.x == 0u) *count = 0u;
+    if (threadIdx.x < 4u) count[threadIdx.x] = 0u;
 }
 
 // Pixel state after a round: (tt, transmittance, previous density, flags).
@@ -280,6 +290,216 @@ __global__ void __launch_bounds__(64) k_dvr_march(DevScene sc, DvrPasses P, uint
         [&] { return n == kDvrSamples; });
     const uint32_t flags = (m.refined ? kMarchRefined : 0u) | (done ? kMarchDone : 0u) | (n << 8) | (round << 16);
     P.state[pixel] = make_float4(m.tt, m.transmittance, m.prev_density, __uint_as_float(flags));
+}
+
+// ---- the first round's march as parallel segments -------------------------------------
+//
+// A ray's march is a chain of dependent steps, and the frame waits for its longest
+// rays: on DVR-1 a few hundred waves march ~200 cubic-fetch steps each while most
+// finish at once. The step positions depend only on the ray and the majorant
+// grid (a step into an empty brick jumps to its exit), not on what the samples
+// hold, so the loop is split: k_dvr_walk runs each ray's loop over the majorant
+// alone and keeps the distance at every kDvrSegIters-th iteration;
+// k_dvr_segments runs the full loop body for every (ray, segment) at once and
+// records the visible samples; k_dvr_merge takes each ray's records in march
+// order and does what the loop does with them (transmittance, the stop after
+// kDvrSamples samples, the first-surface bisection), so the records, states
+// and list equal k_dvr_march's bit for bit. Rays past kDvrSegMax segments, or
+// whose segment found the pool full, go on from that segment's state in
+// k_dvr_continue.
+
+constexpr uint32_t kSegFull = 15u;  // seg_out count of a segment that found the pool full
+
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) v = max(v, uint32_t(__shfl_xor(int(v), d)));
+    return v;
+}
+
+// Each ray's loop over the majorant alone: the distance at the start of every
+// segment; the tile's (tile, segment) items for k_dvr_segments.
+__global__ void __launch_bounds__(64) k_dvr_walk(DevScene sc, DvrPasses P, uint32_t width, uint32_t height) {
+    uint32_t px, py;
+    tile_pixel(px, py);
+    const uint32_t lane = threadIdx.x, npx = width * height;
+    uint32_t nseg = 0;
+    if (px < width && py < height) {
+        const uint32_t pixel = py * width + px;
+        const DvrRay r = dvr_ray(sc.u, px, py);
+        const float step_mm = sc.u->dims[3];
+        float tt = march_start(r, step_mm, px, py);
+        int cached = -2;  // the brick whose majorant `maj` holds (brick_index gives -1 and up)
+        float maj = 0.0f;
+        while (tt < r.t1 && nseg < kDvrSegMax) {
+            P.seg_tt[size_t(nseg) * npx + pixel] = tt;
+            nseg++;
+            for (uint32_t k = 0; k < kDvrSegIters && tt < r.t1; k++, tt += step_mm) {
+                const V3 idx = r.o_idx + r.d_idx * tt;
+                const int b = brick_index(sc, idx);
+                if (b != cached) maj = brick_majorant_at(sc, b), cached = b;
+                if (maj <= 0.0f) tt += brick_exit_fast(idx, r.inv_d);
+            }
+        }
+        P.walk[pixel] = make_float2(tt, __uint_as_float(nseg));
+    }
+    const uint32_t items = wave_max(nseg);
+    uint32_t base = 0;
+    if (lane == 0u && items) base = atomicAdd(P.seg_n, items);
+    base = uint32_t(__shfl(int(base), 0));
+    if (lane < items) P.seg_items[base + lane] = (blockIdx.y * gridDim.x + blockIdx.x) | (lane << 22);
+}
+
+__device__ __forceinline__ uint32_t wave_exclusive_sum(uint32_t v, uint32_t& total) {
+    uint32_t inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = uint32_t(__shfl_up(int(inc), d));
+        if (int(__lane_id()) >= d) inc += o;
+    }
+    total = uint32_t(__shfl(int(inc), 63));
+    return inc - v;
+}
+
+// The listed (tile, segment) items, one per wave at a time (wave w takes items
+// w, w + waves, ...): each ray of the tile marches that segment's loop
+// iterations and its visible samples go to the wave's own part of the pool,
+// contiguous per ray and segment. (Items and pool space taken from shared
+// counters made the waves wait on contended atomics: 3x slower.)
+__global__ void __launch_bounds__(64) k_dvr_segments(DevScene sc, DvrPasses P, uint32_t width, uint32_t height,
+                                                     uint32_t tiles_x) {
+    __shared__ float4 s_rec[kDvrSegIters][64];
+    const uint32_t lane = threadIdx.x, npx = width * height;
+    const vvr_uniforms* u = sc.u;
+    const float step_mm = u->dims[3];
+    const uint32_t items = P.seg_n[0];
+    const uint32_t chunk = P.pool_cap / gridDim.x;  // this wave's part of the pool
+    float4* __restrict__ part_pool = P.pool + size_t(blockIdx.x) * chunk;
+    uint32_t used = 0;
+    for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
+        const uint32_t e = P.seg_items[item], tile = e & 0x3fffffu, seg = e >> 22;
+        const uint32_t px = (tile % tiles_x) * 8u + (lane & 7u), py = (tile / tiles_x) * 8u + (lane >> 3);
+        const uint32_t pixel = py * width + px;
+        bool active = px < width && py < height;
+        if (active) active = seg < __float_as_uint(P.walk[pixel].y);
+        uint32_t n = 0;
+        if (active) {
+            const DvrRay r = dvr_ray(u, px, py);
+            float tt = P.seg_tt[size_t(seg) * npx + pixel];
+            for (uint32_t k = 0; k < kDvrSegIters && tt < r.t1; k++, tt += step_mm) {
+                const V3 idx = r.o_idx + r.d_idx * tt;
+                if (brick_majorant(sc, idx) <= 0.0f) {
+                    tt += brick_exit_fast(idx, r.inv_d);
+                    continue;
+                }
+                int cap_plane;
+                if (clip_depth(u, idx, cap_plane) > 0.0f) continue;
+                const float density = density_view<true>(sc, idx);
+                const Sample smp = labelled(sc, tf_point(sc, density), idx);
+                if (smp.extinction <= 0.0f) continue;
+                s_rec[n][lane] = make_float4(tt, smp.extinction, density, __uint_as_float(smp.label));
+                n++;
+            }
+        }
+        uint32_t total;
+        const uint32_t first = wave_exclusive_sum(n, total);
+        const uint32_t base = used;
+        const bool fits = base + total <= chunk;
+        if (fits) used += total;
+        if (active) {
+            if (fits) {
+                for (uint32_t q = 0; q < n; q++) part_pool[base + first + q] = s_rec[q][lane];
+                P.seg_out[size_t(seg) * npx + pixel] = ((blockIdx.x * chunk + base + first) << 4) | n;
+            } else {
+                P.seg_out[size_t(seg) * npx + pixel] = kSegFull;
+            }
+        }
+    }
+}
+
+// Each ray's records in march order, as dvr_march's loop takes its samples:
+// the same records, list entries and state as k_dvr_march (round 0). The
+// first-surface bisection only moves its sample's shading position, so it runs
+// after the loop, with the wave's lanes together.
+__global__ void __launch_bounds__(64) k_dvr_merge(DevScene sc, DvrPasses P, uint32_t width, uint32_t height) {
+    __shared__ uint32_t s_so[kDvrSegMax][64];  // every segment's pool entry, loaded before any is used
+    uint32_t px, py;
+    tile_pixel(px, py);
+    const bool valid = px < width && py < height;
+    const vvr_uniforms* u = sc.u;
+    const uint32_t lane = threadIdx.x, pixel = py * width + px, npx = width * height;
+    const float step_mm = u->dims[3];
+    uint32_t n = 0, refine_n = kDvrSamples;
+    float refine_ext = 0.0f;
+    DvrRay r;
+    if (valid) {
+        const float2 w = P.walk[pixel];
+        const uint32_t nseg = __float_as_uint(w.y);
+        uint32_t so[kDvrSegMax];
+#pragma unroll
+        for (uint32_t k = 0; k < kDvrSegMax; k++) so[k] = k < nseg ? P.seg_out[size_t(k) * npx + pixel] : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < kDvrSegMax; k++) s_so[k][lane] = so[k];
+        r = dvr_ray(u, px, py);
+        float4* __restrict__ rec = P.rec + size_t(pixel) * kDvrSamples * 2u;
+        float transmittance = 1.0f, tt = w.x;
+        bool refined = false, ended = false, done = false;
+        for (uint32_t seg = 0; seg < nseg && !ended; seg++) {
+            const uint32_t out = s_so[seg][lane];
+            if ((out & 15u) == kSegFull) {  // not marched: the one loop goes on from here
+                tt = P.seg_tt[size_t(seg) * npx + pixel];
+                ended = true;
+                break;
+            }
+            for (uint32_t q = 0; q < (out & 15u); q++) {
+                const float4 v = P.pool[(out >> 4) + q];
+                const float at = v.x, extinction = v.y, density = v.z;
+                const uint32_t label = __float_as_uint(v.w);
+                const V3 idx = r.o_idx + r.d_idx * at;
+                int cap_plane;
+                const float cut = clip_depth(u, idx, cap_plane);
+                const float alpha = 1.0f - vexp(-extinction * step_mm);
+                if (!refined && alpha > 0.2f && transmittance > 0.9f) {
+                    refined = true;
+                    refine_n = n, refine_ext = extinction;
+                }
+                float4* qr = rec + n * 2u;
+                qr[0] = make_float4(idx.x, idx.y, idx.z, transmittance * alpha);
+                qr[1] = make_float4(density, -1.0f, cut, __uint_as_float(label | (uint32_t(cap_plane + 1) << 8)));
+                n++;
+                transmittance *= 1.0f - alpha;
+                // The loop's checks before its next step: the ray ends when it is
+                // opaque (or past its end); after kDvrSamples samples it stops.
+                tt = at + step_mm;
+                if (!(transmittance > 0.01f) || !(tt < r.t1)) {
+                    done = ended = true;
+                    break;
+                }
+                if (n == kDvrSamples) {
+                    ended = true;
+                    break;
+                }
+            }
+        }
+        if (!ended) {  // every segment taken: done if the walk reached the ray's end
+            tt = w.x;
+            done = !(tt < r.t1);
+        }
+        const uint32_t flags = (refined ? kMarchRefined : 0u) | (done ? kMarchDone : 0u) | (n << 8);
+        P.state[pixel] = make_float4(tt, transmittance, -1.0f, __uint_as_float(flags));
+    }
+    // The pixel's samples on the shading list (one atomic per wave).
+    uint32_t total;
+    const uint32_t first = wave_exclusive_sum(n, total);
+    uint32_t base = 0;
+    if (lane == 0u && total) base = atomicAdd(P.count, total);
+    base = uint32_t(__shfl(int(base), 0));
+    for (uint32_t k = 0; k < n; k++) P.list[base + first + k] = pixel * kDvrSamples + k;
+    if (refine_n < kDvrSamples) {
+        float4* qr = P.rec + (size_t(pixel) * kDvrSamples + refine_n) * 2u;
+        const float4 a = qr[0];
+        const V3 idx = first_surface(sc, V3(a.x, a.y, a.z), r.d_idx, refine_ext);
+        qr[0] = make_float4(idx.x, idx.y, idx.z, a.w);
+    }
 }
 
 // The listed samples' light (direct + ambient), one per lane, over the
@@ -371,12 +591,19 @@ void launch_dvr(const DevScene& s, uint16_t* hdr, uint32_t width, uint32_t heigh
 }
 
 void launch_dvr_passes(const DevScene& s, const DvrPasses& P, uint16_t* hdr, uint32_t width, uint32_t height,
-                       int cus, hipStream_t st) {
+                       int cus, bool segmented, hipStream_t st) {
     const dim3 tiles((width + 7) / 8, (height + 7) / 8);
     const uint32_t shade_waves = std::max(std::min((width * height * kDvrSamples + 63u) / 64u, uint32_t(cus) * 16u), 1u);
     for (uint32_t round = 0; round < kDvrRounds; round++) {
         hipLaunchKernelGGL(k_dvr_reset, dim3(1), dim3(64), 0, st, P.count);
-        hipLaunchKernelGGL(k_dvr_march, tiles, dim3(64), 0, st, s, P, width, height, round);
+        if (round == 0 && segmented) {
+            hipLaunchKernelGGL(k_dvr_walk, tiles, dim3(64), 0, st, s, P, width, height);
+            hipLaunchKernelGGL(k_dvr_segments, dim3(uint32_t(std::max(cus, 1)) * 16u), dim3(64), 0, st, s, P, width,
+                               height, tiles.x);
+            hipLaunchKernelGGL(k_dvr_merge, tiles, dim3(64), 0, st, s, P, width, height);
+        } else {
+            hipLaunchKernelGGL(k_dvr_march, tiles, dim3(64), 0, st, s, P, width, height, round);
+        }
         hipLaunchKernelGGL(k_dvr_shade, dim3(shade_waves), dim3(64), 0, st, s, P, width);
         hipLaunchKernelGGL(k_dvr_resolve, tiles, dim3(64), 0, st, P, hdr, width, height, round);
     }

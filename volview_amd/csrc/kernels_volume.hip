@@ -38,8 +38,7 @@ __device__ __forceinline__ uint16_t hu_to_unorm16(int16_t hu) {
 
 // The unorm16 density, and its values c / 65535 as floats (what the cubic
 // fetch's window reads).
-__global__ void k_upload_density(const int16_t* __restrict__ hu, uint16_t* __restrict__ out,
-                                 float* __restrict__ out_f, BrickGrid g) {
+__global__ void k_upload_density(const int16_t* __restrict__ hu, uint16_t* __restrict__ out, BrickGrid g) {
     const uint32_t o = blockIdx.x * kBlock + threadIdx.x;
     if (o >= g.padded()) return;
     int x, y, z;
@@ -47,7 +46,17 @@ __global__ void k_upload_density(const int16_t* __restrict__ hu, uint16_t* __res
     uint16_t v = 0;
     if (x < g.nx && y < g.ny && z < g.nz) v = hu_to_unorm16(hu[size_t(x) + size_t(g.nx) * (size_t(y) + size_t(g.ny) * z)]);
     out[o] = v;
-    out_f[o] = unorm16_value(float(v));
+}
+
+// The float copy in rows (DevScene::density_f): texel (x, y, z) of the padded
+// box -kRowPad .. n - 1 + kRowPad holds the value of the clamped voxel.
+__global__ void k_upload_density_rows(const int16_t* __restrict__ hu, float* __restrict__ out, BrickGrid g) {
+    const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
+    const size_t rx = size_t(g.nx + 2 * kRowPad), ry = size_t(g.ny + 2 * kRowPad);
+    if (i >= row_volume_texels(g.nx, g.ny, g.nz)) return;
+    const int x = iclamp(int(i % rx) - kRowPad, 0, g.nx - 1), y = iclamp(int((i / rx) % ry) - kRowPad, 0, g.ny - 1),
+              z = iclamp(int(i / (rx * ry)) - kRowPad, 0, g.nz - 1);
+    out[i] = unorm16_value(float(hu_to_unorm16(hu[size_t(x) + size_t(g.nx) * (size_t(y) + size_t(g.ny) * z)])));
 }
 
 __global__ void k_upload_labels(const uint8_t* __restrict__ in, uint8_t* __restrict__ out, BrickGrid g) {
@@ -616,7 +625,9 @@ inline unsigned blocks_for(size_t n) { return unsigned((n + kBlock - 1) / kBlock
 }  // namespace
 
 void launch_upload_density(const int16_t* hu, uint16_t* out, float* out_f, const BrickGrid& g, hipStream_t st) {
-    hipLaunchKernelGGL(k_upload_density, dim3(blocks_for(g.padded())), dim3(kBlock), 0, st, hu, out, out_f, g);
+    hipLaunchKernelGGL(k_upload_density, dim3(blocks_for(g.padded())), dim3(kBlock), 0, st, hu, out, g);
+    hipLaunchKernelGGL(k_upload_density_rows, dim3(blocks_for(row_volume_texels(g.nx, g.ny, g.nz))), dim3(kBlock), 0,
+                       st, hu, out_f, g);
 }
 
 void launch_upload_labels(const uint8_t* labels, uint8_t* out, const BrickGrid& g, hipStream_t st) {

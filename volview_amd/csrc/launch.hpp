@@ -47,19 +47,46 @@ void launch_dvr(const DevScene& s, uint16_t* hdr, uint32_t width, uint32_t heigh
 // The same frame in passes: rounds of march (each pixel to its next
 // kDvrSamples visible samples), shading of the listed samples, per-pixel sums;
 // then the rest of the long rays in one loop. For frames without ghosted
-// labels; buffers of dvr_passes_bytes(width * height); `cus` sizes the shading.
+// labels; buffers of dvr_passes_bytes(width, height); `cus` sizes the shading.
+// With `segmented` (frames without pre-integration) the march of the first
+// round runs as segments of kDvrSegIters loop iterations, up to kDvrSegMax per
+// ray, marched in parallel: a walk over the majorant grid finds each segment's
+// start, the segments record their visible samples in a pool (kDvrPoolPerPixel
+// records per pixel on average), and a merge composites each ray's records in
+// march order.
 constexpr uint32_t kDvrSamples = 32;  // samples a pixel records per round
 constexpr uint32_t kDvrRounds = 1;
+constexpr uint32_t kDvrSegIters = 8;
+constexpr uint32_t kDvrSegMax = 32;
+constexpr uint32_t kDvrPoolPerPixel = 32;
 struct DvrPasses {
     float4* rec;        // kDvrSamples records of 2 vectors per pixel
     float4* state;      // per pixel: the march state after a round
     float4* radiance;   // per pixel: the light of its samples so far
     uint32_t* list;     // the round's samples (pixel x kDvrSamples + k)
     uint32_t* count;    // their number
+    // Segmented march: [segment][pixel] start distance and pool entry
+    // (first record << 4 | records, 15 = the pool was full); per pixel the
+    // walk's end (distance, segments); the pool of (distance, extinction,
+    // density, label) records.
+    float* seg_tt;
+    uint32_t* seg_out;
+    float2* walk;
+    float4* pool;
+    uint32_t pool_cap;  // records; each wave of k_dvr_segments fills its own equal part
+    // The (tile, segment) items that hold a marching ray (tile | segment << 22)
+    // and two counters: items listed, items taken.
+    uint32_t* seg_items;
+    uint32_t* seg_n;
 };
-inline size_t dvr_passes_bytes(size_t pixels) { return pixels * (kDvrSamples * 32 + 16 + 16 + kDvrSamples * 4) + 5 * 256; }
+inline size_t dvr_tiles(uint32_t width, uint32_t height) { return size_t((width + 7) / 8) * ((height + 7) / 8); }
+inline size_t dvr_passes_bytes(uint32_t width, uint32_t height) {
+    const size_t pixels = size_t(width) * height;
+    return pixels * (kDvrSamples * 32 + 16 + 16 + kDvrSamples * 4 + kDvrSegMax * 8 + 8 + kDvrPoolPerPixel * 16) +
+           dvr_tiles(width, height) * kDvrSegMax * 4 + 12 * 256;  // each buffer rounded up to 256 bytes
+}
 void launch_dvr_passes(const DevScene& s, const DvrPasses& P, uint16_t* hdr, uint32_t width, uint32_t height,
-                       int cus, hipStream_t st);
+                       int cus, bool segmented, hipStream_t st);
 // Lighting volumes of the ray marcher (shadow.comp, ao.comp) into s.sdims /
 // s.adims texel arrays; `hu` is the host copy of the uniforms s.u points to.
 void shadow_sweep_axis(const vvr_uniforms& hu, int& axis, int& sign);

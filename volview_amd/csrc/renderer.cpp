@@ -58,7 +58,7 @@ void new_renderer(vvr_ctx* ctx, const vvr_volume* vol, const vvr_tf* tf, vvr_ren
 
     const size_t nvox = size_t(vol->dims[0]) * vol->dims[1] * vol->dims[2];
     r->density = dalloc<uint16_t>(g.padded(), "density");
-    r->density_f = dalloc<float>(g.padded(), "density values");
+    r->density_f = dalloc<float>(row_volume_texels(g.nx, g.ny, g.nz), "density values");
     r->normals = dalloc<uint32_t>(g.padded(), "normals");
     r->majorant = dalloc<float>(size_t(r->mdims[0]) * r->mdims[1] * r->mdims[2], "extinction bounds");
     int16_t* hu = dalloc<int16_t>(nvox, "HU staging");
@@ -255,7 +255,10 @@ void record(vvr_renderer* r, uint64_t frame, const vvr_frame_params* p) {
         if (r->marches_in_passes(u)) {
             const DvrPasses& passes = r->passes();
             r->timed(VVR_KERNEL_DVR,
-                     [&] { launch_dvr_passes(sc, passes, r->hdr, r->width, r->height, r->cus, r->stream); });
+                     [&] {
+                         launch_dvr_passes(sc, passes, r->hdr, r->width, r->height, r->cus, r->segments_march(u),
+                                           r->stream);
+                     });
         } else {
             r->timed(VVR_KERNEL_DVR, [&] { launch_dvr(sc, r->hdr, r->width, r->height, r->stream); });
         }

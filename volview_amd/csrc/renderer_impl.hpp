@@ -540,16 +540,23 @@ struct vvr_renderer {
     // with VVR_DVR_PASSES=0, or buffers beyond 4 GiB).
     bool marches_in_passes(const vvr_uniforms& u) const {
         const char* env = std::getenv("VVR_DVR_PASSES");
-        if ((env && env[0] == '0') || dvr_passes_bytes(size_t(width) * height) > (size_t(4) << 30)) return false;
+        if ((env && env[0] == '0') || dvr_passes_bytes(width, height) > (size_t(4) << 30)) return false;
         if (uint32_t(u.labels[0]) != VVR_LABELS_OFF)
             for (int l = 1; l < 256; l++)
                 if (u.label_colors[l][3] > 1.5f) return false;
         return true;
     }
+    // Whether the march of such a frame runs as parallel segments: not for
+    // pre-integrated frames, whose samples need the previous step's density
+    // (nor with VVR_DVR_PASSES=2, the march as one loop per ray).
+    bool segments_march(const vvr_uniforms& u) const {
+        const char* env = std::getenv("VVR_DVR_PASSES");
+        return u.lighting[2] == 0u && !(env && env[0] == '2');
+    }
     const DvrPasses& passes() {
         if (!dvr_mem) {
             const size_t px = size_t(width) * height;
-            dvr_mem = dalloc<uint8_t>(dvr_passes_bytes(px), "ray-marching passes");
+            dvr_mem = dalloc<uint8_t>(dvr_passes_bytes(width, height), "ray-marching passes");
             uint8_t* p = dvr_mem;
             auto take = [&](size_t bytes) {
                 uint8_t* q = p;
@@ -560,7 +567,14 @@ struct vvr_renderer {
             dvr_passes.state = reinterpret_cast<float4*>(take(px * 16));
             dvr_passes.radiance = reinterpret_cast<float4*>(take(px * 16));
             dvr_passes.list = reinterpret_cast<uint32_t*>(take(px * kDvrSamples * 4));
-            dvr_passes.count = reinterpret_cast<uint32_t*>(take(4));
+            dvr_passes.count = reinterpret_cast<uint32_t*>(take(16));  // and the segment counters
+            dvr_passes.seg_n = dvr_passes.count + 2;
+            dvr_passes.seg_tt = reinterpret_cast<float*>(take(px * kDvrSegMax * 4));
+            dvr_passes.seg_out = reinterpret_cast<uint32_t*>(take(px * kDvrSegMax * 4));
+            dvr_passes.walk = reinterpret_cast<float2*>(take(px * 8));
+            dvr_passes.pool = reinterpret_cast<float4*>(take(px * kDvrPoolPerPixel * 16));
+            dvr_passes.seg_items = reinterpret_cast<uint32_t*>(take(dvr_tiles(width, height) * kDvrSegMax * 4));
+            dvr_passes.pool_cap = uint32_t(std::min<size_t>(px * kDvrPoolPerPixel, size_t(1) << 28));
         }
         return dvr_passes;
     }
