@@ -385,18 +385,26 @@ __global__ void __launch_bounds__(64) k_dvr_segments(DevScene sc, DvrPasses P, u
         if (active) {
             const DvrRay r = dvr_ray(u, px, py);
             float tt = P.seg_tt[size_t(seg) * npx + pixel];
-            for (uint32_t k = 0; k < kDvrSegIters && tt < r.t1; k++, tt += step_mm) {
+            // The majorant of the next step's brick is loaded before this step's
+            // work (a step into an empty brick loads it again from its exit).
+            float maj = brick_majorant(sc, r.o_idx + r.d_idx * tt);
+            for (uint32_t k = 0; k < kDvrSegIters && tt < r.t1; k++) {
                 const V3 idx = r.o_idx + r.d_idx * tt;
-                if (brick_majorant(sc, idx) <= 0.0f) {
+                if (maj <= 0.0f) {
                     tt += brick_exit_fast(idx, r.inv_d);
+                    tt += step_mm;
+                    maj = brick_majorant(sc, r.o_idx + r.d_idx * tt);
                     continue;
                 }
+                const float step_at = tt;
+                tt += step_mm;
+                maj = brick_majorant(sc, r.o_idx + r.d_idx * tt);
                 int cap_plane;
                 if (clip_depth(u, idx, cap_plane) > 0.0f) continue;
                 const float density = density_view<true>(sc, idx);
                 const Sample smp = labelled(sc, tf_point(sc, density), idx);
                 if (smp.extinction <= 0.0f) continue;
-                s_rec[n][lane] = make_float4(tt, smp.extinction, density, __uint_as_float(smp.label));
+                s_rec[n][lane] = make_float4(step_at, smp.extinction, density, __uint_as_float(smp.label));
                 n++;
             }
         }
@@ -598,7 +606,7 @@ void launch_dvr_passes(const DevScene& s, const DvrPasses& P, uint16_t* hdr, uin
         hipLaunchKernelGGL(k_dvr_reset, dim3(1), dim3(64), 0, st, P.count);
         if (round == 0 && segmented) {
             hipLaunchKernelGGL(k_dvr_walk, tiles, dim3(64), 0, st, s, P, width, height);
-            hipLaunchKernelGGL(k_dvr_segments, dim3(uint32_t(std::max(cus, 1)) * 16u), dim3(64), 0, st, s, P, width,
+            hipLaunchKernelGGL(k_dvr_segments, dim3(uint32_t(std::max(cus, 1)) * 32u), dim3(64), 0, st, s, P, width,
                                height, tiles.x);
             hipLaunchKernelGGL(k_dvr_merge, tiles, dim3(64), 0, st, s, P, width, height);
         } else {
