@@ -208,9 +208,29 @@ def run_batched(ctx) -> dict:
             rs[q.volume].synchronize()
         return time.perf_counter() - t0
 
+    def plain_one_path_frames() -> float:
+        """The reference's own loop (render_to_rgba, offscreen.rs:32-36) at its
+        default frame of one path per pixel (samples_per_frame = 1,
+        renderer.rs:363): a path-traced request of n paths is n frames, each
+        submitted and waited for. Its image equals the one-frame request's."""
+        t0 = time.perf_counter()
+        for k, (q, p) in enumerate(zip(reqs, params)):
+            if not q.path_traced:
+                rs[q.volume].record(k + 1, p)
+                rs[q.volume].synchronize()
+                continue
+            p1 = type(p).from_buffer_copy(p)
+            p1.path_tracing.samples_per_frame = 1
+            for f in range(q.spp):
+                rs[q.volume].record(f + 1, p1)
+                rs[q.volume].synchronize()
+        return time.perf_counter() - t0
+
     plain()  # warm-up: majorants, engines
     t_plain = plain()
     log(f"batched-32 plain {t_plain:.2f} s")
+    t_plain_ref = plain_one_path_frames()
+    log(f"batched-32 plain, one-path frames {t_plain_ref:.2f} s")
     t = time.perf_counter()  # untimed: the server's shape for this batch (vvr_server_autotune)
     shape = Server.autotune(ctx, [(rs[q.volume], p, q.width, q.height) for q, p in zip(reqs, params)])
     log(f"batched-32 server autotune {time.perf_counter() - t:.1f} s: {shape}")
@@ -236,8 +256,10 @@ def run_batched(ctx) -> dict:
     return {"workload": "Batched-32: 32 seeded phantoms of edge U{128..512}, images U{256..1024}^2, "
                         "16 DVR (bone preset) + 16 PT (U{1..64} spp, 8 bounces, g=0.8)",
             "pt_paths": pt_paths, "plain": side(t_plain), "served": side(t_served),
+            "plain_one_path_frames": side(t_plain_ref),
             "server_shape": shape, "server_shape_from": "autotune",
-            "speedup": round(t_plain / t_served, 2)}
+            "speedup": round(t_plain / t_served, 2),
+            "speedup_vs_one_path_frames": round(t_plain_ref / t_served, 2)}
 
 
 def run_serving(ctx, seconds: float, grace: float = 5.0, increment: int = 16, slots: int = 2 << 20,

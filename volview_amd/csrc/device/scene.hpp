@@ -40,16 +40,6 @@ __device__ __forceinline__ float snorm8_value(float c) {
 
 // ---- linear fetches (common.glsl:76-78 and the mask / normal images) -----------
 
-// A texel at a 32-bit byte offset from a volume's base. Bricked volumes hold at
-// most 2^30 texels of at most 4 bytes (checked when a renderer is created), so
-// the offset never wraps, and the load takes the kernel argument's base from
-// scalar registers instead of forming a 64-bit address per texel.
-template <class T>
-__device__ __forceinline__ T load_at(const T* __restrict__ base, uint32_t byte_offset) {
-    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_offset);
-}
-
-
 __device__ __forceinline__ float density_at_index(const DevScene& s, V3 idx) {
     const float fx = floorf(idx.x), fy = floorf(idx.y), fz = floorf(idx.z);
     const float ax = idx.x - fx, ay = idx.y - fy, az = idx.z - fz;
@@ -59,7 +49,7 @@ __device__ __forceinline__ float density_at_index(const DevScene& s, V3 idx) {
     const uint32_t Y0 = g.yoff(iclamp(y0, 0, g.ny - 1)), Y1 = g.yoff(iclamp(y0 + 1, 0, g.ny - 1));
     const uint32_t Z0 = g.zoff(iclamp(z0, 0, g.nz - 1)), Z1 = g.zoff(iclamp(z0 + 1, 0, g.nz - 1));
     const uint16_t* d = s.density;
-    auto v = [&](uint32_t i) { return unorm16_value(float(load_at(d, i * 2u))); };
+    auto v = [&](uint32_t i) { return unorm16_value(float(d[i])); };
     const float c000 = v(X0 + Y0 + Z0), c100 = v(X1 + Y0 + Z0), c010 = v(X0 + Y1 + Z0), c110 = v(X1 + Y1 + Z0);
     const float c001 = v(X0 + Y0 + Z1), c101 = v(X1 + Y0 + Z1), c011 = v(X0 + Y1 + Z1), c111 = v(X1 + Y1 + Z1);
     const float c00 = mixf(c000, c100, ax), c10 = mixf(c010, c110, ax);
@@ -80,9 +70,8 @@ __device__ __forceinline__ F2 mask_at(const DevScene& s, V3 idx) {
     const uint32_t Y0 = g.yoff(iclamp(y0, 0, g.ny - 1)), Y1 = g.yoff(iclamp(y0 + 1, 0, g.ny - 1));
     const uint32_t Z0 = g.zoff(iclamp(z0, 0, g.nz - 1)), Z1 = g.zoff(iclamp(z0 + 1, 0, g.nz - 1));
     const uint32_t* m = s.mask;
-    auto ld = [&](uint32_t i) { return load_at(m, i * 4u); };
-    const uint32_t t[8] = {ld(X0 + Y0 + Z0), ld(X1 + Y0 + Z0), ld(X0 + Y1 + Z0), ld(X1 + Y1 + Z0),
-                           ld(X0 + Y0 + Z1), ld(X1 + Y0 + Z1), ld(X0 + Y1 + Z1), ld(X1 + Y1 + Z1)};
+    const uint32_t t[8] = {m[X0 + Y0 + Z0], m[X1 + Y0 + Z0], m[X0 + Y1 + Z0], m[X1 + Y1 + Z0],
+                           m[X0 + Y0 + Z1], m[X1 + Y0 + Z1], m[X0 + Y1 + Z1], m[X1 + Y1 + Z1]};
     F2 out;
 #pragma unroll
     for (int ch = 0; ch < 2; ch++) {
@@ -121,9 +110,8 @@ __device__ __forceinline__ F4 normal_at(const DevScene& s, V3 idx) {
     const uint32_t Y0 = g.yoff(iclamp(y0, 0, g.ny - 1)), Y1 = g.yoff(iclamp(y0 + 1, 0, g.ny - 1));
     const uint32_t Z0 = g.zoff(iclamp(z0, 0, g.nz - 1)), Z1 = g.zoff(iclamp(z0 + 1, 0, g.nz - 1));
     const uint32_t* n = s.normals;
-    auto ld = [&](uint32_t i) { return load_at(n, i * 4u); };
-    const uint32_t t[8] = {ld(X0 + Y0 + Z0), ld(X1 + Y0 + Z0), ld(X0 + Y1 + Z0), ld(X1 + Y1 + Z0),
-                           ld(X0 + Y0 + Z1), ld(X1 + Y0 + Z1), ld(X0 + Y1 + Z1), ld(X1 + Y1 + Z1)};
+    const uint32_t t[8] = {n[X0 + Y0 + Z0], n[X1 + Y0 + Z0], n[X0 + Y1 + Z0], n[X1 + Y1 + Z0],
+                           n[X0 + Y0 + Z1], n[X1 + Y0 + Z1], n[X0 + Y1 + Z1], n[X1 + Y1 + Z1]};
     float r[4];
 #pragma unroll
     for (int ch = 0; ch < 4; ch++) {
@@ -208,19 +196,17 @@ __device__ __forceinline__ float lerp4(float v0, float v1, float v2, float v3, T
     return mixf(lo, hi, t.a);
 }
 
-// Byte offsets (for load_at) of the 4 clamped voxel rows of the window
-// floor(idx) - 1 .. + 2 in a volume of kBytes-byte texels.
+// Addresses of the 4 clamped voxel rows of the window floor(idx) - 1 .. + 2.
 struct WindowAddr {
     uint32_t x[4], y[4], z[4];
 };
-template <int kBytes>
 __device__ __forceinline__ WindowAddr window_addr(const BrickGrid& g, int ix, int iy, int iz) {
     WindowAddr w;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        w.x[k] = g.xoff(iclamp(ix - 1 + k, 0, g.nx - 1)) * uint32_t(kBytes);
-        w.y[k] = g.yoff(iclamp(iy - 1 + k, 0, g.ny - 1)) * uint32_t(kBytes);
-        w.z[k] = g.zoff(iclamp(iz - 1 + k, 0, g.nz - 1)) * uint32_t(kBytes);
+        w.x[k] = g.xoff(iclamp(ix - 1 + k, 0, g.nx - 1));
+        w.y[k] = g.yoff(iclamp(iy - 1 + k, 0, g.ny - 1));
+        w.z[k] = g.zoff(iclamp(iz - 1 + k, 0, g.nz - 1));
     }
     return w;
 }
@@ -228,13 +214,13 @@ __device__ __forceinline__ WindowAddr window_addr(const BrickGrid& g, int ix, in
 // One z slice (4 x 4 texel values) of the window, from the unorm16 density or
 // from its float copy (the same values, without the conversion: the ray marcher
 // reads it; the path tracer's scattered first-bounce fetches run faster on the
-// 2-byte texels). `w` holds byte offsets for the texel type read.
+// 2-byte texels).
 __device__ __forceinline__ void load_slice(const uint16_t* __restrict__ d, const WindowAddr& w, int z, float v[4][4]) {
 #pragma unroll
     for (int y = 0; y < 4; y++) {
         const uint32_t base = w.y[y] + w.z[z];
 #pragma unroll
-        for (int x = 0; x < 4; x++) v[y][x] = unorm16_value(float(load_at(d, w.x[x] + base)));
+        for (int x = 0; x < 4; x++) v[y][x] = unorm16_value(float(d[w.x[x] + base]));
     }
 }
 
@@ -322,7 +308,7 @@ __device__ inline float density_cubic(const DevScene& s, V3 idx) {
     WindowAddr w;
     RowWindow rw;
     if (kFloatTexels) rw = row_window(s, fidx(ix), fidx(iy), fidx(iz));
-    else w = window_addr<2>(s.g, fidx(ix), fidx(iy), fidx(iz));
+    else w = window_addr(s.g, fidx(ix), fidx(iy), fidx(iz));
     float Y[2][2][4];  // [x tap][y tap][z slice]: the fetches lerped along x and y
 #pragma unroll
     for (int z = 0; z < 4; z++) {
@@ -356,7 +342,7 @@ __device__ inline V3 density_gradient_cubic(const DevScene& s, V3 idx) {
     const Tap hx[2] = {tap_of(tx.h0, ix), tap_of(tx.h1, ix)}, dx[2] = {tap_of(tx.dh0, ix), tap_of(tx.dh1, ix)};
     const Tap hy[2] = {tap_of(ty.h0, iy), tap_of(ty.h1, iy)}, dy[2] = {tap_of(ty.dh0, iy), tap_of(ty.dh1, iy)};
     const Tap hz[2] = {tap_of(tz.h0, iz), tap_of(tz.h1, iz)}, dz[2] = {tap_of(tz.dh0, iz), tap_of(tz.dh1, iz)};
-    const WindowAddr w = window_addr<2>(s.g, fidx(ix), fidx(iy), fidx(iz));
+    const WindowAddr w = window_addr(s.g, fidx(ix), fidx(iy), fidx(iz));
     // [x tap][y tap][z tap] of the fetches of d/dx, d/dy and d/dz.
     float GX[2][2][2] = {}, GY[2][2][2] = {}, GZ[2][2][2] = {};
 #pragma unroll
@@ -553,7 +539,7 @@ __device__ __forceinline__ Sample apply_labels(const DevScene& s, Sample smp, V3
     if (mode == 0u) return smp;
     if (s.mask_cells) {  // the branches below, without the mask and label loads
         const uint32_t c = mask_cell(s, idx);
-        if (((load_at(s.mask_cells, (c >> 5) * 4u) >> (c & 31u)) & 1u) == 0u) {
+        if (((s.mask_cells[c >> 5] >> (c & 31u)) & 1u) == 0u) {
             smp.extinction = 0.0f;
             return smp;
         }
@@ -721,7 +707,7 @@ __device__ __forceinline__ int brick_index(const DevScene& s, V3 idx) {
     return int(uint32_t(bx) + uint32_t(s.mx) * (uint32_t(by) + uint32_t(s.my) * uint32_t(bz)));
 }
 __device__ __forceinline__ float brick_majorant_at(const DevScene& s, int b) {
-    return b < 0 ? 1e30f : load_at(s.majorant, uint32_t(b) * 4u);
+    return b < 0 ? 1e30f : s.majorant[b];
 }
 __device__ __forceinline__ float brick_majorant(const DevScene& s, V3 idx) {
     return brick_majorant_at(s, brick_index(s, idx));
