@@ -263,6 +263,35 @@ def test_frame_over_the_item_limit_is_refused(ctx):
     assert np.isfinite(acc).all() and r.accumulated_samples() == 2
 
 
+@pytest.mark.parametrize("knob", [None, ("VVR_DVR_POOL", "1"), ("VVR_DVR_SEGMENTS", "2")])
+def test_dvr_segments_long_rays_and_full_pool(ctx, monkeypatch, knob):
+    """The segmented march's hand-offs to the one loop, bit for bit: a faint
+    transfer function keeps rays marching past their 32 recorded samples; a pool
+    of one record per pixel (VVR_DVR_POOL) leaves segments without room, whose
+    rays go on from the segment's start; a walk of 2 segments (VVR_DVR_SEGMENTS)
+    hands rays on after 16 loop iterations."""
+    dims = (128, 128, 128)
+    hu = phantom.ct(dims, 9)
+    faint = TransferFunction([(-1000.0, (0.0, 0.0, 0.0), 0.0), (-300.0, (0.8, 0.6, 0.5), 0.006),
+                              (1500.0, (1.0, 1.0, 1.0), 0.012)])
+    if knob:
+        monkeypatch.setenv(*knob)
+    r = Renderer(ctx, hu, faint)
+    p = workloads.dvr_params(r, 1)
+    r.resize(96, 80)
+    frames = {}
+    for env in ("1", "0"):
+        monkeypatch.setenv("VVR_DVR_PASSES", env)
+        r.record(1, p)
+        frames[env] = r.read_hdr()
+    a, c = frames["1"], frames["0"]
+    # Rays of more than 32 visible samples (alpha ~0.003 each), none of them opaque.
+    long_rays = (a[..., 3] < 0.9).mean()
+    print(f"{knob}: {long_rays:.3f} of pixels below transmittance 0.9, min {a[..., 3].min():.3f}")
+    assert long_rays > 0.02 and a[..., 3].min() > 0.01
+    assert np.array_equal(a.view(np.uint32), c.view(np.uint32))
+
+
 @pytest.mark.parametrize("mode", ["plain", "lit", "labels", "preint", "clip"])
 def test_dvr_passes_equal_one_loop(ctx, monkeypatch, mode):
     """The ray marcher in passes gives the one-loop frame bit for bit, with the

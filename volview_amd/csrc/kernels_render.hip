@@ -330,7 +330,7 @@ __global__ void __launch_bounds__(64) k_dvr_walk(DevScene sc, DvrPasses P, uint3
         float tt = march_start(r, step_mm, px, py);
         int cached = -2;  // the brick whose majorant `maj` holds (brick_index gives -1 and up)
         float maj = 0.0f;
-        while (tt < r.t1 && nseg < kDvrSegMax) {
+        while (tt < r.t1 && nseg < P.seg_max) {
             P.seg_tt[size_t(nseg) * npx + pixel] = tt;
             nseg++;
             for (uint32_t k = 0; k < kDvrSegIters && tt < r.t1; k++, tt += step_mm) {

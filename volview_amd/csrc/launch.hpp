@@ -78,6 +78,7 @@ struct DvrPasses {
     // and two counters: items listed, items taken.
     uint32_t* seg_items;
     uint32_t* seg_n;
+    uint32_t seg_max;  // segments walked per ray, 1 .. kDvrSegMax
 };
 inline size_t dvr_tiles(uint32_t width, uint32_t height) { return size_t((width + 7) / 8) * ((height + 7) / 8); }
 inline size_t dvr_passes_bytes(uint32_t width, uint32_t height) {

@@ -574,7 +574,12 @@ struct vvr_renderer {
             dvr_passes.walk = reinterpret_cast<float2*>(take(px * 8));
             dvr_passes.pool = reinterpret_cast<float4*>(take(px * kDvrPoolPerPixel * 16));
             dvr_passes.seg_items = reinterpret_cast<uint32_t*>(take(dvr_tiles(width, height) * kDvrSegMax * 4));
-            dvr_passes.pool_cap = uint32_t(std::min<size_t>(px * kDvrPoolPerPixel, size_t(1) << 28));
+            // VVR_DVR_POOL (records per pixel, 1 .. kDvrPoolPerPixel) shrinks the pool, so
+            // that segments find it full and their rays go on in the one loop.
+            const uint32_t per_px = std::max(1u, std::min(env_u32("VVR_DVR_POOL", kDvrPoolPerPixel), kDvrPoolPerPixel));
+            dvr_passes.pool_cap = uint32_t(std::min<size_t>(px * per_px, size_t(1) << 28));
+            // VVR_DVR_SEGMENTS (1 .. kDvrSegMax) shortens the walk: longer rays go on in the one loop.
+            dvr_passes.seg_max = std::max(1u, std::min(env_u32("VVR_DVR_SEGMENTS", kDvrSegMax), kDvrSegMax));
         }
         return dvr_passes;
     }
