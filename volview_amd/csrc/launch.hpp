@@ -59,6 +59,8 @@ constexpr uint32_t kDvrRounds = 1;
 constexpr uint32_t kDvrSegIters = 8;
 constexpr uint32_t kDvrSegMax = 32;
 constexpr uint32_t kDvrPoolPerPixel = 32;
+static_assert(kDvrSegMax <= 64, "a tile's wave lists its segment items one per lane");
+static_assert(kDvrSegIters < 15, "a segment's record count and the full-pool mark share 4 bits");
 struct DvrPasses {
     float4* rec;        // kDvrSamples records of 2 vectors per pixel
     float4* state;      // per pixel: the march state after a round
@@ -81,10 +83,12 @@ struct DvrPasses {
     uint32_t seg_max;  // segments walked per ray, 1 .. kDvrSegMax
 };
 inline size_t dvr_tiles(uint32_t width, uint32_t height) { return size_t((width + 7) / 8) * ((height + 7) / 8); }
-inline size_t dvr_passes_bytes(uint32_t width, uint32_t height) {
-    const size_t pixels = size_t(width) * height;
+inline size_t dvr_passes_bytes_for(size_t pixels, size_t tiles) {
     return pixels * (kDvrSamples * 32 + 16 + 16 + kDvrSamples * 4 + kDvrSegMax * 8 + 8 + kDvrPoolPerPixel * 16) +
-           dvr_tiles(width, height) * kDvrSegMax * 4 + 12 * 256;  // each buffer rounded up to 256 bytes
+           tiles * kDvrSegMax * 4 + 12 * 256;  // each buffer rounded up to 256 bytes
+}
+inline size_t dvr_passes_bytes(uint32_t width, uint32_t height) {
+    return dvr_passes_bytes_for(size_t(width) * height, dvr_tiles(width, height));
 }
 void launch_dvr_passes(const DevScene& s, const DvrPasses& P, uint16_t* hdr, uint32_t width, uint32_t height,
                        int cus, bool segmented, hipStream_t st);

@@ -48,6 +48,49 @@ inline uint32_t env_u32(const char* name, uint32_t fallback) {
     return x ? uint32_t(std::min<unsigned long>(x, 1ul << 30)) : fallback;
 }
 
+// A frame with uniforms `u` can be ray marched in passes unless a label may be
+// drawn as a ghost, whose opacity comes from its shading frame, which the march
+// cannot know.
+inline bool dvr_in_passes(const vvr_uniforms& u) {
+    if (uint32_t(u.labels[0]) != VVR_LABELS_OFF)
+        for (int l = 1; l < 256; l++)
+            if (u.label_colors[l][3] > 1.5f) return false;
+    return true;
+}
+// Whether its march runs as parallel segments: not when pre-integrated, whose
+// samples need the previous step's density.
+inline bool dvr_segmented(const vvr_uniforms& u) { return u.lighting[2] == 0u; }
+
+// The buffers of launch_dvr_passes for frames of up to `px` pixels in `tiles`
+// 8 x 8 tiles, carved from `mem` (dvr_passes_bytes of such a frame).
+inline DvrPasses carve_dvr_passes(uint8_t* mem, size_t px, size_t tiles) {
+    DvrPasses P{};
+    uint8_t* p = mem;
+    auto take = [&](size_t bytes) {
+        uint8_t* q = p;
+        p += (bytes + 255) / 256 * 256;
+        return q;
+    };
+    P.rec = reinterpret_cast<float4*>(take(px * kDvrSamples * 32));
+    P.state = reinterpret_cast<float4*>(take(px * 16));
+    P.radiance = reinterpret_cast<float4*>(take(px * 16));
+    P.list = reinterpret_cast<uint32_t*>(take(px * kDvrSamples * 4));
+    P.count = reinterpret_cast<uint32_t*>(take(16));  // and the segment counters
+    P.seg_n = P.count + 2;
+    P.seg_tt = reinterpret_cast<float*>(take(px * kDvrSegMax * 4));
+    P.seg_out = reinterpret_cast<uint32_t*>(take(px * kDvrSegMax * 4));
+    P.walk = reinterpret_cast<float2*>(take(px * 8));
+    P.pool = reinterpret_cast<float4*>(take(px * kDvrPoolPerPixel * 16));
+    P.seg_items = reinterpret_cast<uint32_t*>(take(tiles * kDvrSegMax * 4));
+    // VVR_DVR_POOL (records per pixel, 1 .. kDvrPoolPerPixel) shrinks the pool, so
+    // that segments find it full and their rays go on in the one loop.
+    const uint32_t per_px = std::max(1u, std::min(env_u32("VVR_DVR_POOL", kDvrPoolPerPixel), kDvrPoolPerPixel));
+    P.pool_cap = uint32_t(std::min<size_t>(px * per_px, size_t(1) << 28));
+    // VVR_DVR_SEGMENTS (1 .. kDvrSegMax) shortens the walk: longer rays go on in the one loop.
+    P.seg_max = std::max(1u, std::min(env_u32("VVR_DVR_SEGMENTS", kDvrSegMax), kDvrSegMax));
+    return P;
+}
+
 // Device state of the wavefront path tracer: the path slots, queues, the chunk
 // sums of the requests in flight and the request table.
 struct WfEngine {
@@ -535,51 +578,23 @@ struct vvr_renderer {
     uint8_t* dvr_mem = nullptr;
     DvrPasses dvr_passes{};
     int cus = 1;  // compute units of the device
-    // Whether a frame with uniforms `u` is ray marched in passes: not when a
-    // label may be drawn as a ghost, whose opacity the march cannot know (nor
-    // with VVR_DVR_PASSES=0, or buffers beyond 4 GiB).
+    // Whether a frame with uniforms `u` is ray marched in passes (dvr_in_passes;
+    // nor with VVR_DVR_PASSES=0, or buffers beyond 4 GiB).
     bool marches_in_passes(const vvr_uniforms& u) const {
         const char* env = std::getenv("VVR_DVR_PASSES");
         if ((env && env[0] == '0') || dvr_passes_bytes(width, height) > (size_t(4) << 30)) return false;
-        if (uint32_t(u.labels[0]) != VVR_LABELS_OFF)
-            for (int l = 1; l < 256; l++)
-                if (u.label_colors[l][3] > 1.5f) return false;
-        return true;
+        return dvr_in_passes(u);
     }
-    // Whether the march of such a frame runs as parallel segments: not for
-    // pre-integrated frames, whose samples need the previous step's density
-    // (nor with VVR_DVR_PASSES=2, the march as one loop per ray).
+    // Whether the march of such a frame runs as parallel segments
+    // (dvr_segmented; VVR_DVR_PASSES=2: the march as one loop per ray).
     bool segments_march(const vvr_uniforms& u) const {
         const char* env = std::getenv("VVR_DVR_PASSES");
-        return u.lighting[2] == 0u && !(env && env[0] == '2');
+        return dvr_segmented(u) && !(env && env[0] == '2');
     }
     const DvrPasses& passes() {
         if (!dvr_mem) {
-            const size_t px = size_t(width) * height;
             dvr_mem = dalloc<uint8_t>(dvr_passes_bytes(width, height), "ray-marching passes");
-            uint8_t* p = dvr_mem;
-            auto take = [&](size_t bytes) {
-                uint8_t* q = p;
-                p += (bytes + 255) / 256 * 256;
-                return q;
-            };
-            dvr_passes.rec = reinterpret_cast<float4*>(take(px * kDvrSamples * 32));
-            dvr_passes.state = reinterpret_cast<float4*>(take(px * 16));
-            dvr_passes.radiance = reinterpret_cast<float4*>(take(px * 16));
-            dvr_passes.list = reinterpret_cast<uint32_t*>(take(px * kDvrSamples * 4));
-            dvr_passes.count = reinterpret_cast<uint32_t*>(take(16));  // and the segment counters
-            dvr_passes.seg_n = dvr_passes.count + 2;
-            dvr_passes.seg_tt = reinterpret_cast<float*>(take(px * kDvrSegMax * 4));
-            dvr_passes.seg_out = reinterpret_cast<uint32_t*>(take(px * kDvrSegMax * 4));
-            dvr_passes.walk = reinterpret_cast<float2*>(take(px * 8));
-            dvr_passes.pool = reinterpret_cast<float4*>(take(px * kDvrPoolPerPixel * 16));
-            dvr_passes.seg_items = reinterpret_cast<uint32_t*>(take(dvr_tiles(width, height) * kDvrSegMax * 4));
-            // VVR_DVR_POOL (records per pixel, 1 .. kDvrPoolPerPixel) shrinks the pool, so
-            // that segments find it full and their rays go on in the one loop.
-            const uint32_t per_px = std::max(1u, std::min(env_u32("VVR_DVR_POOL", kDvrPoolPerPixel), kDvrPoolPerPixel));
-            dvr_passes.pool_cap = uint32_t(std::min<size_t>(px * per_px, size_t(1) << 28));
-            // VVR_DVR_SEGMENTS (1 .. kDvrSegMax) shortens the walk: longer rays go on in the one loop.
-            dvr_passes.seg_max = std::max(1u, std::min(env_u32("VVR_DVR_SEGMENTS", kDvrSegMax), kDvrSegMax));
+            dvr_passes = carve_dvr_passes(dvr_mem, size_t(width) * height, dvr_tiles(width, height));
         }
         return dvr_passes;
     }

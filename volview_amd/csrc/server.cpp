@@ -185,7 +185,12 @@ struct vvr_server {
     uint32_t n_active = 0;
     uint32_t entries_in_use = 0;
 
-    // ray marching
+    // ray marching: the passes' buffers (launch_dvr_passes, one request at a time,
+    // grown to the largest frame seen), and the batched one-loop kernel's job tables
+    // for frames that may draw ghosted labels
+    uint8_t* dvr_mem = nullptr;
+    size_t dvr_px = 0, dvr_tiles_cap = 0;
+    DvrPasses dvr_passes{};
     DvrJob* d_jobs[kDvrRing] = {};
     DvrJob* h_jobs[kDvrRing] = {};
     hipEvent_t jobs_ev[kDvrRing] = {};
@@ -407,7 +412,38 @@ void vvr_server::free_entry(int e) {
     entries_in_use--;
 }
 
-void vvr_server::launch_dvr(const std::vector<Job*>& batch) {
+// Ray-marched requests: in passes (the march as parallel segments, as a
+// renderer's own frames) one after another on the server's stream, through
+// buffers grown to the largest frame; frames that may draw ghosted labels, or
+// whose buffers would pass 4 GiB, in one batched launch of the one-loop kernel.
+// Both give the same frame bit for bit.
+void vvr_server::launch_dvr(const std::vector<Job*>& jobs) {
+    std::vector<Job*> batch;
+    for (Job* j : jobs) {
+        const vvr_uniforms& hu = h_uniforms[j->uslot];
+        const size_t px = size_t(j->width) * j->height, tiles = dvr_tiles(j->width, j->height);
+        if (!dvr_in_passes(hu) || dvr_passes_bytes(j->width, j->height) > (size_t(4) << 30)) {
+            batch.push_back(j);
+            continue;
+        }
+        if (px > dvr_px || tiles > dvr_tiles_cap) {
+            // Kernels queued on the stream may still use the old buffers.
+            // (Like the path slots, the server's own buffers are not charged to the
+            // requests' HBM budget.)
+            check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+            const size_t want_px = std::max(px, dvr_px), want_tiles = std::max(tiles, dvr_tiles_cap);
+            dfree(dvr_mem);
+            dvr_mem = nullptr;
+            dvr_px = dvr_tiles_cap = 0;  // until the new buffers exist
+            dvr_mem = dalloc<uint8_t>(dvr_passes_bytes_for(want_px, want_tiles), "server ray-marching passes");
+            dvr_passes = carve_dvr_passes(dvr_mem, want_px, want_tiles);
+            dvr_px = want_px, dvr_tiles_cap = want_tiles;
+        }
+        launch_dvr_passes(j->vol->scene(d_uniforms + j->uslot), dvr_passes, j->hdr, j->width, j->height, j->vol->cus,
+                          dvr_segmented(hu), stream);
+        check(hipGetLastError(), "ray marching passes");
+        read_back(j, 0);
+    }
     for (size_t b0 = 0; b0 < batch.size(); b0 += 64) {
         const size_t nb = std::min<size_t>(64, batch.size() - b0);
         const uint32_t k = jobs_next;
@@ -1056,7 +1092,7 @@ void vvr_server_destroy(vvr_server* s) {
         s->let_go(j);
         free_host_side(j);
     }
-    dfree(s->d_table), dfree(s->d_ctr), dfree(s->d_uniforms);
+    dfree(s->d_table), dfree(s->d_ctr), dfree(s->d_uniforms), dfree(s->dvr_mem);
     hfree(s->h_table), hfree(s->h_uniforms), hfree(s->h_ctr_buf[0]), hfree(s->h_ctr_buf[1]);
     for (auto& e : s->ctr_ev)
         if (e) (void)hipEventDestroy(e);
