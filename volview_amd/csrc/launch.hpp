@@ -57,7 +57,7 @@ void launch_dvr(const DevScene& s, uint16_t* hdr, uint32_t width, uint32_t heigh
 constexpr uint32_t kDvrSamples = 32;  // samples a pixel records per round
 constexpr uint32_t kDvrRounds = 1;
 constexpr uint32_t kDvrSegIters = 8;
-constexpr uint32_t kDvrSegMax = 32;
+constexpr uint32_t kDvrSegMax = 64;
 constexpr uint32_t kDvrPoolPerPixel = 32;
 static_assert(kDvrSegMax <= 64, "a tile's wave lists its segment items one per lane");
 static_assert(kDvrSegIters < 15, "a segment's record count and the full-pool mark share 4 bits");
