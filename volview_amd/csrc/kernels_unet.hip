@@ -533,8 +533,12 @@ __global__ void k_unet_input(const float4* __restrict__ accum, const float4* __r
         if (len > 1e-3f) f[6] = gx / len, f[7] = gy / len, f[8] = gz / len;
         f[9] = a.w / n;
     }
+    uint32_t w16[8];
 #pragma unroll
-    for (int k = 0; k < 16; k++) x[size_t(i) * 16 + k] = f2bf(f[k]);
+    for (int k = 0; k < 8; k++) w16[k] = uint32_t(f2bf(f[2 * k])) | (uint32_t(f2bf(f[2 * k + 1])) << 16);
+    uint4* xo = reinterpret_cast<uint4*>(x + size_t(i) * 16);  // the pixel's 16 channels: two 16-byte stores
+    xo[0] = make_uint4(w16[0], w16[1], w16[2], w16[3]);
+    xo[1] = make_uint4(w16[4], w16[5], w16[6], w16[7]);
     if (alb_out) {
         alb_out[size_t(i) * 3] = ar, alb_out[size_t(i) * 3 + 1] = ag, alb_out[size_t(i) * 3 + 2] = ab;
     }
