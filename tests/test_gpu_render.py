@@ -104,29 +104,37 @@ def pt_scene(ctx, labels_on, dims=(64, 64, 48), seed=12):
     return hu, tf, r, labels, n
 
 
-def oracle_pt_sum(oracle, r, hu, tf, n, labels, w, h, first, count, precision=32, bounces=8):
+def oracle_pt_sum(oracle, r, hu, tf, n, labels, w, h, first, count, precision=32, bounces=8, ghosts=()):
     """The oracle's sums over sample indices first .. first + count - 1, each traced
-    as a reference frame of one path."""
+    as a reference frame of one path (labels in `ghosts` drawn as ghosts)."""
     want = [np.zeros((w * h, 4), np.float64) for _ in range(3)]
     for k in range(first, first + count):
         q = workloads.pt_params(r, n, 1, bounces=bounces, g=0.8, camera_index=2)
+        for l in ghosts:
+            q.labels.ghost[l] = 1
         u, sc = oracle_for(oracle, r, hu, tf, q, w, h, has_labels=labels is not None, accumulated=k, labels=labels)
         for acc, part in zip(want, sc.pt(u, all_pixels(w, h), precision)):
             acc += part
     return want
 
 
-@pytest.mark.parametrize("labels_on", [True, False])
+@pytest.mark.parametrize("labels_on", [True, False, "ghost"])
 def test_pt_traces_the_oracles_paths(ctx, oracle, labels_on):
     """Same random streams: nearly every pixel's sum agrees to 1e-3. A formula
-    error would move every pixel; only float-rounding flips may move a few."""
-    hu, tf, r, labels, n = pt_scene(ctx, labels_on)
+    error would move every pixel; only float-rounding flips may move a few.
+    With a ghosted label the free flights run the kernel with the ghost branch;
+    without, the one compiled without it."""
+    hu, tf, r, labels, n = pt_scene(ctx, bool(labels_on))
     w, h, spp = 32, 32, 4
     r.resize(w, h)
-    r.record(1, workloads.pt_params(r, n, spp, bounces=8, g=0.8, camera_index=2))
+    p = workloads.pt_params(r, n, spp, bounces=8, g=0.8, camera_index=2)
+    ghosts = (2, 5) if labels_on == "ghost" else ()
+    for l in ghosts:
+        p.labels.ghost[l] = 1
+    r.record(1, p)
     acc, alb, nrm = r.read_accum()
     assert r.accumulated_samples() == spp
-    want = oracle_pt_sum(oracle, r, hu, tf, n, labels, w, h, 0, spp)
+    want = oracle_pt_sum(oracle, r, hu, tf, n, labels, w, h, 0, spp, ghosts=ghosts)
     for what, got, ref in (("radiance", acc, want[0]), ("albedo guide", alb[..., :3], want[1][:, :3]),
                            ("normal guide", nrm, want[2])):
         got = got.reshape(ref.shape).astype(np.float64)

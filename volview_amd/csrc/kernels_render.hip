@@ -606,8 +606,11 @@ void launch_dvr_passes(const DevScene& s, const DvrPasses& P, uint16_t* hdr, uin
         hipLaunchKernelGGL(k_dvr_reset, dim3(1), dim3(64), 0, st, P.count);
         if (round == 0 && segmented) {
             hipLaunchKernelGGL(k_dvr_walk, tiles, dim3(64), 0, st, s, P, width, height);
-            hipLaunchKernelGGL(k_dvr_segments, dim3(uint32_t(std::max(cus, 1)) * 32u), dim3(64), 0, st, s, P, width,
-                               height, tiles.x);
+            // 32 waves per CU, or one per item a frame this small can have: each wave's
+            // part of the pool is the pool over the waves.
+            const uint32_t seg_waves = uint32_t(std::min<size_t>(size_t(std::max(cus, 1)) * 32u,
+                                                                 size_t(tiles.x) * tiles.y * kDvrSegMax));
+            hipLaunchKernelGGL(k_dvr_segments, dim3(seg_waves), dim3(64), 0, st, s, P, width, height, tiles.x);
             hipLaunchKernelGGL(k_dvr_merge, tiles, dim3(64), 0, st, s, P, width, height);
         } else {
             hipLaunchKernelGGL(k_dvr_march, tiles, dim3(64), 0, st, s, P, width, height, round);

@@ -512,7 +512,7 @@ __global__ void __launch_bounds__(64) k_wf_regen_served(const PtRequest* __restr
 // Stage 1: free flight over the `*n_in` entries of `in` (kNone: padding): new
 // flights, and flights a previous launch suspended (p_flight); each runs at
 // most `steps` tracking iterations, and one that is not done goes back on ff.
-template <bool kBatched>
+template <bool kBatched, bool kGhosts = true>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
 k_wf_extend(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q, const uint32_t* __restrict__ in,
             const uint32_t* __restrict__ n_in, uint32_t cur, int steps) {
@@ -535,7 +535,7 @@ k_wf_extend(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q, const uin
             const PtRequest& R = reqs[r];
             Flight f = flight_begin(R.scene.u, st.origin, st.dir);
             if (fl.y != kWfNewFlight) f.tt = bitsf(fl.x), f.i = int(fl.y);
-            flow = stage_extend_steps(R.scene, frame_consts(R.scene.u), st, ev, f, f.i + steps);
+            flow = stage_extend_steps<kGhosts>(R.scene, frame_consts(R.scene.u), st, ev, f, f.i + steps);
             fl = make_uint2(fbits(f.tt), uint32_t(f.i));
         });
         if (active) {
@@ -725,6 +725,7 @@ enum TailPhase : uint32_t { kFlight, kShadow, kScatter, kEnded };
 // the chunk sums get the same additions in the same order as with iterations
 // to the end. The chunk's sums are stored and the slot is left idle. False for
 // an idle slot.
+template <bool kGhosts = true>
 __device__ __forceinline__ bool run_to_end(const PtRequest& R, const WfSlots& S, uint32_t slot, uint32_t phase) {
     const DevScene& sc = R.scene;
     const FrameConsts fc = frame_consts(sc.u);
@@ -755,7 +756,7 @@ __device__ __forceinline__ bool run_to_end(const PtRequest& R, const WfSlots& S,
                 f.tt = bitsf(fl.x), f.i = int(fl.y);
                 resumed = false;
             }
-            if (stage_extend_steps(sc, fc, st, ev, f, 4096) == kFlowHit) {
+            if (stage_extend_steps<kGhosts>(sc, fc, st, ev, f, 4096) == kFlowHit) {
                 stage_shade(sc, fc, st, ev);
                 if (st.bounce == 0u) {
                     c1 = make_float4(c1.x + st.albedo.x, c1.y + st.albedo.y, c1.z + st.albedo.z, c1.w + 0.0f);
@@ -802,6 +803,7 @@ __device__ __forceinline__ bool run_to_end(const PtRequest& R, const WfSlots& S,
 // few paths remain: each path still in flight runs to its end in one launch,
 // a lane per slot. Entries: the ended paths of end[cur] (regen's turn), the
 // suspended shadow rays of sh[cur] and the flights of ff[cur] (new or suspended).
+template <bool kGhosts>
 __global__ void __launch_bounds__(64) k_wf_finish(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
                                                   uint32_t cur) {
     const uint32_t n_end = Q.counts[kWfEND0 + cur], n_sh = Q.counts[kWfSH0 + cur];
@@ -811,7 +813,7 @@ __global__ void __launch_bounds__(64) k_wf_finish(const PtRequest* __restrict__ 
         if (q < n_end) slot = Q.end[cur][q], phase = kEnded;
         else if (q < n_end + n_sh) slot = Q.sh[cur][q - n_end], phase = kShadow;
         else slot = Q.ff[cur][q - n_end - n_sh], phase = kFlight;
-        run_to_end(reqs[0], S, slot, phase);
+        run_to_end<kGhosts>(reqs[0], S, slot, phase);
     }
 }
 
@@ -935,6 +937,7 @@ WfGrid wf_grid(int cus) {
     g.single = WfStageGrid{waves_for(k_wf_regen<false>, cus, 32), waves_for(k_wf_extend<false>, cus, 32),
                            waves_for(k_wf_shade<false>, cus, 32), waves_for(k_wf_shadow<false>, cus, 32),
                            waves_for(k_wf_scatter<false>, cus, 32)};
+    g.extend_no_ghosts = waves_for(k_wf_extend<false, false>, cus, 32);
     // The served regen's occupancy bounds the batched regen grid too (it runs in its place).
     g.batched = WfStageGrid{std::min(waves_for(k_wf_regen<true>, cus, 32), waves_for(k_wf_regen_served, cus, 32)),
                             waves_for(k_wf_extend<true>, cus, 32), waves_for(k_wf_shade<true>, cus, 32),
@@ -954,7 +957,7 @@ namespace {
 // each queue binned by request first.
 template <bool kBatched>
 void wf_stages(const PtRequest* reqs, const WfSlots& S, const WfQueues& Q, uint32_t cur, const WfGrid& grid, bool sort,
-               bool env_map, hipStream_t st) {
+               bool env_map, bool ghosts, hipStream_t st) {
     const WfStageGrid& g = kBatched ? grid.batched : grid.single;
     const uint32_t* ff = Q.ff[cur];
     const uint32_t* n_ff = Q.counts + kWfFF0 + cur;
@@ -968,8 +971,12 @@ void wf_stages(const PtRequest* reqs, const WfSlots& S, const WfQueues& Q, uint3
         ff = Q.sorted;
         n_ff = Q.counts + kWfSORTFF;
     }
-    hipLaunchKernelGGL(k_wf_extend<kBatched>, dim3(g.extend), dim3(64), 0, st, reqs, S, Q, ff, n_ff, cur,
-                       grid.flight_steps);
+    if (!kBatched && !ghosts)
+        hipLaunchKernelGGL((k_wf_extend<false, false>), dim3(grid.extend_no_ghosts), dim3(64), 0, st, reqs, S, Q, ff,
+                           n_ff, cur, grid.flight_steps);
+    else
+        hipLaunchKernelGGL(k_wf_extend<kBatched>, dim3(g.extend), dim3(64), 0, st, reqs, S, Q, ff, n_ff, cur,
+                           grid.flight_steps);
     if (sort) {
         uint32_t* hist = Q.bins + 2 * kWfBins;
         hipLaunchKernelGGL(k_wf_bin_count, dim3(grid.sort), dim3(256), 0, st, ev, n_ev, S.req, hist);
@@ -988,13 +995,13 @@ void wf_stages(const PtRequest* reqs, const WfSlots& S, const WfQueues& Q, uint3
 }  // namespace
 
 void wf_iteration(const PtRequest* reqs, uint32_t n_reqs, uint32_t total_items, const WfSlots& S, const WfQueues& Q,
-                  uint32_t cur, const WfGrid& g, bool env_map, hipStream_t st) {
+                  uint32_t cur, const WfGrid& g, bool env_map, bool ghosts, hipStream_t st) {
     if (n_reqs > 1) {
         hipLaunchKernelGGL(k_wf_regen<true>, dim3(g.batched.regen), dim3(64), 0, st, reqs, n_reqs, total_items, S, Q, cur);
-        wf_stages<true>(reqs, S, Q, cur, g, true, env_map, st);  // n_reqs <= kWfBins (WfEngine::run)
+        wf_stages<true>(reqs, S, Q, cur, g, true, env_map, true, st);  // n_reqs <= kWfBins (WfEngine::run)
     } else {
         hipLaunchKernelGGL(k_wf_regen<false>, dim3(g.single.regen), dim3(64), 0, st, reqs, n_reqs, total_items, S, Q, cur);
-        wf_stages<false>(reqs, S, Q, cur, g, false, env_map, st);
+        wf_stages<false>(reqs, S, Q, cur, g, false, env_map, ghosts, st);
     }
 }
 
@@ -1006,12 +1013,14 @@ void wf_init_served(const WfSlots& S, const WfQueues& Q, uint32_t slots, hipStre
 void wf_iteration_served(const PtRequest* reqs, const WfActive& A, const WfServed& V, const WfSlots& S,
                          const WfQueues& Q, uint32_t cur, const WfGrid& g, bool env_map, hipStream_t st) {
     hipLaunchKernelGGL(k_wf_regen_served, dim3(g.batched.regen), dim3(64), 0, st, reqs, A, V, S, Q, cur);
-    wf_stages<true>(reqs, S, Q, cur, g, true, env_map, st);
+    wf_stages<true>(reqs, S, Q, cur, g, true, env_map, true, st);
 }
 
-void wf_finish(const PtRequest* reqs, const WfSlots& S, const WfQueues& Q, uint32_t cur, uint32_t paths,
+void wf_finish(const PtRequest* reqs, const WfSlots& S, const WfQueues& Q, uint32_t cur, uint32_t paths, bool ghosts,
                hipStream_t st) {
-    hipLaunchKernelGGL(k_wf_finish, dim3(std::max(1u, (paths + 63u) / 64u)), dim3(64), 0, st, reqs, S, Q, cur);
+    const dim3 grid(std::max(1u, (paths + 63u) / 64u));
+    if (ghosts) hipLaunchKernelGGL(k_wf_finish<true>, grid, dim3(64), 0, st, reqs, S, Q, cur);
+    else hipLaunchKernelGGL(k_wf_finish<false>, grid, dim3(64), 0, st, reqs, S, Q, cur);
 }
 
 void wf_tail_gather(const WfSlots& S, const WfQueues& Q, uint32_t cur, uint32_t entry, bool drop, uint32_t* list,

@@ -175,6 +175,7 @@ struct WfStageGrid {
 };
 struct WfGrid {
     WfStageGrid single, batched;  // one request / several (their kernels' occupancies differ)
+    int extend_no_ghosts;         // one request whose labels cannot be drawn as ghosts
     int sort;                     // 256-lane workgroups of the queue sort
     // Tracking-loop iterations a free flight / shadow ray runs per launch before
     // it is suspended and re-queued, so that a wave's lanes do not idle behind
@@ -188,8 +189,10 @@ WfGrid wf_grid(int cus);
 // All `slots` slots empty on the end queue of iteration 0, item queue at 0.
 void wf_init(const WfSlots& S, const WfQueues& Q, uint32_t slots, hipStream_t st);
 // env_map: a request in flight renders with a light map (adds its stage).
+// ghosts: the (single) request's labels may be drawn as ghosts (else its
+// free flights run without the ghost branch).
 void wf_iteration(const PtRequest* reqs, uint32_t n_reqs, uint32_t total_items, const WfSlots& S, const WfQueues& Q,
-                  uint32_t cur, const WfGrid& g, bool env_map, hipStream_t st);
+                  uint32_t cur, const WfGrid& g, bool env_map, bool ghosts, hipStream_t st);
 // Served engine: all slots idle; one iteration drawing work from the active
 // requests of the table `reqs` (see k_wf_regen_served).
 void wf_init_served(const WfSlots& S, const WfQueues& Q, uint32_t slots, hipStream_t st);
@@ -198,8 +201,8 @@ void wf_iteration_served(const PtRequest* reqs, const WfActive& A, const WfServe
 // Chunk sums -> accumulation, guides and HDR images of one request.
 // Single-request engine, every item handed out: runs the paths still in flight
 // (at most `paths`) to their ends in one launch, after the iterations queued so
-// far; `cur` is the next iteration's.
-void wf_finish(const PtRequest* reqs, const WfSlots& S, const WfQueues& Q, uint32_t cur, uint32_t paths,
+// far; `cur` is the next iteration's; `ghosts` as for wf_iteration.
+void wf_finish(const PtRequest* reqs, const WfSlots& S, const WfQueues& Q, uint32_t cur, uint32_t paths, bool ghosts,
                hipStream_t st);
 // Served engine, after the iterations queued so far on `st` (`cur` the next
 // one's): the paths of request-table entry `entry` leave the queues. With

@@ -48,15 +48,17 @@ inline uint32_t env_u32(const char* name, uint32_t fallback) {
     return x ? uint32_t(std::min<unsigned long>(x, 1ul << 30)) : fallback;
 }
 
-// A frame with uniforms `u` can be ray marched in passes unless a label may be
-// drawn as a ghost, whose opacity comes from its shading frame, which the march
-// cannot know.
-inline bool dvr_in_passes(const vvr_uniforms& u) {
+// Whether a label of a frame with uniforms `u` may be drawn as a ghost (a
+// label colour's alpha above 1.5 in a label mode).
+inline bool labels_may_draw_ghosts(const vvr_uniforms& u) {
     if (uint32_t(u.labels[0]) != VVR_LABELS_OFF)
         for (int l = 1; l < 256; l++)
-            if (u.label_colors[l][3] > 1.5f) return false;
-    return true;
+            if (u.label_colors[l][3] > 1.5f) return true;
+    return false;
 }
+// Such a frame can be ray marched in passes unless a label may be drawn as a
+// ghost, whose opacity comes from its shading frame, which the march cannot know.
+inline bool dvr_in_passes(const vvr_uniforms& u) { return !labels_may_draw_ghosts(u); }
 // Whether its march runs as parallel segments: not when pre-integrated, whose
 // samples need the previous step's density.
 inline bool dvr_segmented(const vvr_uniforms& u) { return u.lighting[2] == 0u; }
@@ -222,8 +224,9 @@ struct WfEngine {
 
     // Renders the requests (scene, images and shape set) on `st`. Returns once
     // every path is traced or queued to be; the resolves are queued last.
-    // env_map: some request renders with a light map.
-    void run(const PtRequest* reqs, uint32_t n, hipStream_t st, bool env_map = false) {
+    // env_map: some request renders with a light map; ghosts: a request's labels
+    // may be drawn as ghosts.
+    void run(const PtRequest* reqs, uint32_t n, hipStream_t st, bool env_map = false, bool ghosts = true) {
         if (n == 0) return;
         size_t partial_f4 = 0;
         uint64_t items = 0;
@@ -255,7 +258,8 @@ struct WfEngine {
         constexpr uint32_t kBatch = 4;
         uint32_t iter = 0;
         auto launch_batch = [&](int k) {
-            for (uint32_t j = 0; j < kBatch; j++, iter++) wf_iteration(d_reqs, n, base, S, Q, iter & 1u, grid, env_map, st);
+            for (uint32_t j = 0; j < kBatch; j++, iter++)
+                wf_iteration(d_reqs, n, base, S, Q, iter & 1u, grid, env_map, ghosts, st);
             check(hipGetLastError(), "wavefront iteration");
             check(hipMemcpyAsync(&h_live[k], Q.counts + kWfLIVE, 4, hipMemcpyDeviceToHost, st), "live count");
             check(hipEventRecord(live_ev[k], st), "hipEventRecord");
@@ -268,7 +272,7 @@ struct WfEngine {
             // Fewer live slots than were filled: every item is handed out (regen
             // leaves a slot idle only then). Few paths left: finish them in one launch.
             if (n == 1 && h_live[k] < std::min(want, grid.tail_paths)) {
-                wf_finish(d_reqs, S, Q, iter & 1u, h_live[k], st);
+                wf_finish(d_reqs, S, Q, iter & 1u, h_live[k], ghosts, st);
                 check(hipGetLastError(), "wavefront tail");
                 break;
             }
