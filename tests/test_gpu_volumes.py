@@ -84,6 +84,30 @@ def test_majorant_label_mode_large_tiles(ctx, oracle):
     assert (got == 0).any() and (got > 0).any()
 
 
+@pytest.mark.parametrize("dims", [(64, 48, 40), (66, 45, 38)])  # whole and partial 4^3 micro-bricks
+@pytest.mark.parametrize("mode", [_abi.LABELS_OFF, _abi.LABELS_TINT])
+def test_majorant_after_transfer_function_change(ctx, oracle, mode, dims):
+    """The density's cell records are built once and kept: after a change of
+    transfer function (and, in tint mode, of the mask) the rebuilt majorant still
+    equals the oracle's bit for bit."""
+    hu, labels, sg, tf, r, p = organ_scene(ctx, dims, 5, mode, soft=False)
+    render_once(r, p)
+    first = r.read_majorant()
+    for preset in (1, 2):
+        tf2 = TransferFunction.preset(preset)
+        r.set_transfer_function(tf2)
+        if mode == _abi.LABELS_TINT:
+            p.labels.colors[preset][3] = 0.0  # a hidden label: a different mask
+        render_once(r, p)
+        u = oracle_uniforms(oracle, r, hu, p, tf2, has_labels=True)
+        sc = oracle.Scene(oracle.hu_to_unorm16(hu), tf=tf2, labels=labels)
+        if mode != _abi.LABELS_OFF:
+            sc.set("mask", r.read_volume(_abi.VOL_MASK))
+        got = r.read_majorant()
+        assert np.array_equal(got, sc.majorant(u))
+        assert not np.array_equal(got, first)
+
+
 def organ_scene(ctx, dims, seed, mode, soft=True, **label_kw):
     hu = phantom.ct(dims, seed)
     labels, sg, _, n = phantom.organs(dims, seed, 40)
@@ -132,9 +156,10 @@ def test_mask_soft_grid_other_shape(ctx, oracle, soft_dims, src_axis, flip):
     assert got[..., 0].min() < got[..., 0].max() and got[..., 1].max() > 0  # neither constant nor unghosted
 
 
+@pytest.mark.parametrize("dims", [(70, 64, 58), (64, 48, 32)])  # partial and whole 4^3 micro-bricks
 @pytest.mark.parametrize("mode", [_abi.LABELS_TINT, _abi.LABELS_ISOLATE, _abi.LABELS_HYBRID])
-def test_mask_binary_path_bit_exact(ctx, oracle, mode):
-    hu, labels, sg, tf, r, p = organ_scene(ctx, (70, 64, 58), 4, mode, soft=False)
+def test_mask_binary_path_bit_exact(ctx, oracle, mode, dims):
+    hu, labels, sg, tf, r, p = organ_scene(ctx, dims, 4, mode, soft=False)
     p.labels.colors[3][3] = 0.0  # a hidden label (cut in tint mode)
     p.labels.ghost[5] = 1
     render_once(r, p)

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <stdexcept>
+#include <type_traits>
 
 #include <cmath>
 
@@ -413,10 +414,11 @@ __global__ void __launch_bounds__(64) k_mask_cells(const uint32_t* __restrict__ 
 // eight lanes read it with one 16-byte load each (two of the mask). Each lane
 // folds its voxels into the micro-brick's 2^3 cells, the lanes combine by
 // shuffles, then
-//  - modes off and tint: the cells' density min | max << 16 go to a 32-byte
-//    record per micro-brick, and (tint) a byte with bit c set when a mask value
-//    (r or g) of cell c is non-zero; k_majorant_bricks reduces each brick's
-//    window from a tile of records staged in LDS and writes the bound;
+//  - modes off and tint: the cells' density min | (65535 - max) << 16 go to a
+//    32-byte record per micro-brick (so that one packed 16-bit minimum combines
+//    two cells), and (tint) a byte with bit c set when a mask value (r or g) of
+//    cell c is non-zero; k_majorant_bricks reduces each brick's window from a
+//    tile of records staged in LDS and writes the bound;
 //  - isolate and hybrid: the bound is the label opacity wherever a mask value
 //    of the window is non-zero, else 0, so with the grid cleared to 0 the lanes
 //    write the opacity straight into the (up to 8) bricks whose windows hold a
@@ -431,14 +433,31 @@ __global__ void __launch_bounds__(kBlock) k_fill_f32(float* __restrict__ out, ui
 
 // Micro-bricks per group of eight lanes (their loads are issued before any is
 // used). One measured fastest: 83 us on the PT-1 volume against 89 with four,
-// where fewer lanes are left to cover the latency.
+// where fewer lanes are left to cover the latency, and on 256^3-512^3 volumes
+// in mode off (24.8 against 28.0 us at 256^3).
 constexpr int kMajRun = 1;
 
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+// Per-half minimum and maximum of two pairs of unsigned 16-bit values.
+__device__ __forceinline__ uint32_t pk_min_u16(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t,
+                              __builtin_elementwise_min(__builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b)));
+}
+__device__ __forceinline__ uint32_t pk_max_u16(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t,
+                              __builtin_elementwise_max(__builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b)));
+}
+
+// kWhole: every micro-brick lies wholly inside the volume (each extent a
+// multiple of 4), so no voxel needs the inside test and, outside the label
+// modes, no lane needs its micro-brick's coordinates.
+template <bool kWhole>
 __global__ void __launch_bounds__(kBlock) k_majorant_cells(const uint16_t* __restrict__ d,
                                                            const uint32_t* __restrict__ mask,
                                                            uint32_t* __restrict__ cell_mm, uint8_t* __restrict__ cell_nz,
                                                            float* __restrict__ out, BrickGrid g, int mode, int mx,
-                                                           int my, int mz, const vvr_uniforms* __restrict__ u) {
+                                                           int my, int mz, const vvr_uniforms* __restrict__ u,
+                                                           bool with_density, bool with_mask) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t m0 = (t >> 3) * kMajRun, k = t & 7u;
     const uint32_t nmb = uint32_t(g.bx) * uint32_t(g.by) * uint32_t(g.bz);
@@ -451,8 +470,8 @@ __global__ void __launch_bounds__(kBlock) k_majorant_cells(const uint16_t* __res
         const uint32_t m = m0 + i;
         if (m >= nmb) break;
         const size_t base = size_t(m) * 64u;
-        if (mode <= 1) dw[i] = reinterpret_cast<const uint4*>(d + base)[k];
-        if (mode != 0) {
+        if (with_density) dw[i] = reinterpret_cast<const uint4*>(d + base)[k];
+        if (with_mask) {
             mw[i][0] = reinterpret_cast<const uint4*>(mask + base)[k];
             mw[i][1] = reinterpret_cast<const uint4*>(mask + base)[k + 8];
         }
@@ -461,36 +480,43 @@ __global__ void __launch_bounds__(kBlock) k_majorant_cells(const uint16_t* __res
     for (int i = 0; i < kMajRun; i++) {
         const uint32_t m = m0 + i;
         if (m >= nmb) break;  // uniform over the group
-        const int mbx = int(m % uint32_t(g.bx)), mby = int((m / uint32_t(g.bx)) % uint32_t(g.by));
-        const int mbz = int(m / (uint32_t(g.bx) * uint32_t(g.by)));
-        const int rx = g.nx - 4 * mbx, ry = g.ny - 4 * mby, rz = g.nz - 4 * mbz;  // voxels left per axis, >= 1
+        int mbx = 0, mby = 0, mbz = 0, rx = 4, ry = 4, rz = 4;  // voxels left per axis, >= 1
+        if (!kWhole || mode >= 2) {
+            mbx = int(m % uint32_t(g.bx));
+            mby = int((m / uint32_t(g.bx)) % uint32_t(g.by));
+            mbz = int(m / (uint32_t(g.bx) * uint32_t(g.by)));
+            if (!kWhole) rx = g.nx - 4 * mbx, ry = g.ny - 4 * mby, rz = g.nz - 4 * mbz;
+        }
         // Voxel v = dx + 4 dy + 16 dz of the micro-brick lies inside the volume.
-        auto inside = [&](int v) { return (v & 3) < rx && ((v >> 2) & 3) < ry && (v >> 4) < rz; };
-        if (mode <= 1) {
-            // Chunk k holds voxels 8k .. 8k + 7: dz = k >> 1, dy = 2 (k & 1) + (j >> 2), dx = j & 3, i.e.
-            // cells cx = 0, 1 of cy = k & 1, cz = k >> 2; lane k ^ 2 holds the same cells' other dz.
+        auto inside = [&](int v) { return kWhole || ((v & 3) < rx && ((v >> 2) & 3) < ry && (v >> 4) < rz); };
+        if (with_density) {
+            // Chunk k holds voxels 8k .. 8k + 7 as four words of two: dz = k >> 1, dy = 2 (k & 1) + (j >> 1),
+            // dx = 2 (j & 1) + {0, 1} for word j, i.e. words 0 and 2 hold cell cx = 0 of cy = k & 1,
+            // cz = k >> 2, words 1 and 3 cell cx = 1; lane k ^ 2 holds the same cells' other dz.
+            // Voxels outside the volume count as 65535 for the minimum and 0 for the maximum.
             const uint32_t w[4] = {dw[i].x, dw[i].y, dw[i].z, dw[i].w};
-            uint32_t lo[2] = {0xffffu, 0xffffu}, hi[2] = {0u, 0u};
+            uint32_t wmin[4], wmax[4];
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                if (!inside(int(8 * k) + j)) continue;
-                const uint32_t x = (j & 1) ? (w[j >> 1] >> 16) : (w[j >> 1] & 0xffffu);
-                const int cx = (j >> 1) & 1;
-                lo[cx] = min(lo[cx], x);
-                hi[cx] = max(hi[cx], x);
+            for (int j = 0; j < 4; j++) {
+                const uint32_t keep = (inside(int(8 * k) + 2 * j) ? 0xffffu : 0u) |
+                                      (inside(int(8 * k) + 2 * j + 1) ? 0xffff0000u : 0u);
+                wmin[j] = w[j] | ~keep;
+                wmax[j] = w[j] & keep;
             }
+            uint32_t rec[2];
 #pragma unroll
             for (int c = 0; c < 2; c++) {
-                lo[c] = min(lo[c], uint32_t(__shfl_xor(int(lo[c]), 2)));
-                hi[c] = max(hi[c], uint32_t(__shfl_xor(int(hi[c]), 2)));
+                const uint32_t mn = pk_min_u16(wmin[c], wmin[c + 2]), mxv = pk_max_u16(wmax[c], wmax[c + 2]);
+                const uint32_t lo = min(mn & 0xffffu, mn >> 16), hi = max(mxv & 0xffffu, mxv >> 16);
+                rec[c] = lo | ((0xffffu - hi) << 16);
+                rec[c] = pk_min_u16(rec[c], uint32_t(__shfl_xor(int(rec[c]), 2)));
             }
             if ((k & 2u) == 0u) {  // lanes 0, 1, 4, 5: cells 2 (k & 1) + 4 (k >> 2) + {0, 1}
                 const uint32_t c0 = 2u * (k & 1u) + 4u * (k >> 2);
-                reinterpret_cast<uint2*>(cell_mm + size_t(m) * 8u + c0)[0] =
-                    make_uint2(lo[0] | (hi[0] << 16), lo[1] | (hi[1] << 16));
+                reinterpret_cast<uint2*>(cell_mm + size_t(m) * 8u + c0)[0] = make_uint2(rec[0], rec[1]);
             }
         }
-        if (mode == 0) continue;
+        if (!with_mask) continue;
         // Mask chunks q = k and k + 8 (four texels each): dz = q >> 2, dy = q & 3, i.e. cells
         // cx = dx >> 1 of cy = (q >> 1) & 1, cz = q >> 3.
         uint32_t bits = 0u;
@@ -534,10 +560,43 @@ __global__ void __launch_bounds__(kBlock) k_majorant_cells(const uint16_t* __res
     }
 }
 
+// Maxima of the transfer function's 64 buckets over runs of 2^j buckets:
+// s[j][b] = max of buckets b .. min(b + 2^j, 64) - 1, so that the maximum over any
+// bucket range is that of two runs (exact: max neither rounds nor depends on the
+// order). Built by the workgroup's first wave; read after a barrier.
+__device__ __forceinline__ void build_run_max(float (&s)[7][64], const vvr_uniforms* __restrict__ u) {
+    if (threadIdx.x >= 64) return;
+    const int b = int(threadIdx.x);
+    float v = u->tf_bucket_max[b / 4][b % 4];
+    s[0][b] = v;
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        v = fmaxf(v, __shfl(v, min(b + (1 << j), 63)));
+        s[j + 1][b] = v;
+    }
+}
+
+// The transfer function's buckets of the density codes lo .. hi (majorant.comp:
+// bucket 64 c / 65535, at most 63); first > last when the window has no voxels
+// (lo 65535, hi 0).
+__device__ __forceinline__ uint32_t bucket_range(uint32_t lo, uint32_t hi) {
+    const uint32_t first = min((64u * lo) / 65535u, 63u), last = min((64u * hi) / 65535u, 63u);
+    return first | (last << 8);
+}
+
+// The largest bucket maximum over a range from bucket_range, 0 for none.
+__device__ __forceinline__ float bucket_range_max(const float (&s)[7][64], uint32_t range) {
+    const int first = int(range & 0xffu), last = int(range >> 8);
+    if (first > last) return 0.0f;
+    const int j = 31 - __clz(last - first + 1);
+    return fmaxf(0.0f, fmaxf(s[j][first], s[j][last - (1 << j) + 1]));
+}
+
 // Modes off and tint: bounds from the cell records. A workgroup takes a tile of
-// 4^3 bricks, stages the records of their 10^3-micro-brick window in LDS, and
-// four lanes reduce each brick's 4^3 micro-bricks (16 each: one z layer of the
-// window) before lane 0 of the four writes the bound. (Tiles of 2^3 bricks in
+// 4^3 bricks and stages the records of their 10^3-micro-brick window in LDS.
+// Wave q reduces z layer q of the 4^3 micro-bricks of every brick's window (one
+// brick per lane), so the cells each layer takes are known when compiling, and
+// wave 0 combines the four layers and writes the bounds. (Tiles of 2^3 bricks in
 // one-wave workgroups measured slower: 16 against 13 us on the DVR-1 volume.)
 constexpr int kMajTile = 4;
 constexpr int kMajMb = 2 * kMajTile + 2;  // micro-bricks per axis of a tile's windows
@@ -545,12 +604,13 @@ constexpr int kMajThreads = 256;
 
 __global__ void __launch_bounds__(kMajThreads) k_majorant_bricks(const uint32_t* __restrict__ cell_mm,
                                                                  const uint8_t* __restrict__ cell_nz,
-                                                                 float* __restrict__ out, BrickGrid g, int mode, int mx,
-                                                                 int my, int mz, const vvr_uniforms* __restrict__ u) {
+                                                                 uint16_t* __restrict__ ranges, float* __restrict__ out,
+                                                                 BrickGrid g, int mode, int mx, int my, int mz,
+                                                                 const vvr_uniforms* __restrict__ u) {
     __shared__ uint4 s_rec[kMajMb * kMajMb * kMajMb][2];
     __shared__ uint8_t s_nz[kMajMb * kMajMb * kMajMb];
-    __shared__ float s_bucket[64];  // the transfer function's 64 bucket maxima, read by every brick's range
-    if (threadIdx.x < 64) s_bucket[threadIdx.x] = u->tf_bucket_max[threadIdx.x / 4][threadIdx.x % 4];
+    __shared__ float s_runmax[7][64];
+    build_run_max(s_runmax, u);
     const int tbx = blockIdx.x * kMajTile, tby = blockIdx.y * kMajTile, tbz = blockIdx.z * kMajTile;
     const int m0x = 2 * tbx - 1, m0y = 2 * tby - 1, m0z = 2 * tbz - 1;
     // Every load of the staging first, then the LDS stores.
@@ -560,7 +620,7 @@ __global__ void __launch_bounds__(kMajThreads) k_majorant_bricks(const uint32_t*
 #pragma unroll
     for (int it = 0; it < kIter; it++) {
         const int i = threadIdx.x + it * kMajThreads;
-        v[it] = make_uint4(0xffffu, 0xffffu, 0xffffu, 0xffffu);  // no voxels: min 0xffff, max 0
+        v[it] = make_uint4(~0u, ~0u, ~0u, ~0u);  // no voxels: min 65535, max 0
         nzv[it] = 0u;
         if (i >= kN) continue;
         const int r = i >> 1, h = i & 1;
@@ -579,45 +639,77 @@ __global__ void __launch_bounds__(kMajThreads) k_majorant_bricks(const uint32_t*
         if ((i & 1) == 0) s_nz[i >> 1] = uint8_t(nzv[it]);
     }
     __syncthreads();
-    const int lb = threadIdx.x >> 2, part = threadIdx.x & 3;  // brick of the tile, quarter of its window
+    __shared__ uint32_t s_layer[4][64][2];  // each layer's packed minimum and cell bits, per brick
+    const int layer = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6)), lb = int(threadIdx.x & 63u);
     const int lx = lb % kMajTile, ly = (lb / kMajTile) % kMajTile, lz = lb / (kMajTile * kMajTile);
-    auto window = [](int i) { return i == 0 ? 2u : (i == 3 ? 1u : 3u); };  // cells inside, per axis
-    uint32_t lo = 0xffffu, hi = 0u, nz = 0u;
-    const int wz_i = part;
-    const uint32_t wz = window(wz_i);
+    // Cells of a micro-brick inside the window along an axis, by its place 0..3 in
+    // the window: the upper cell of the first, both of the middle two, the lower of
+    // the last.
+    constexpr uint32_t kWin[4] = {2u, 3u, 3u, 1u};
+    uint32_t acc = ~0u, nz = 0u;
+    auto reduce = [&](auto wz_c) {
+        constexpr int wz_i = decltype(wz_c)::value;
+        constexpr uint32_t wz = kWin[wz_i];
 #pragma unroll
-    for (int wy_i = 0; wy_i < 4; wy_i++)
+        for (int wy_i = 0; wy_i < 4; wy_i++)
 #pragma unroll
-        for (int wx_i = 0; wx_i < 4; wx_i++) {
-            const uint32_t wx = window(wx_i), wy = window(wy_i);
-            uint32_t sel = 0u;
+            for (int wx_i = 0; wx_i < 4; wx_i++) {
+                const uint32_t wx = kWin[wx_i], wy = kWin[wy_i];
+                uint32_t sel = 0u;
 #pragma unroll
-            for (int c = 0; c < 8; c++)
-                if (((wx >> (c & 1)) & 1u) && ((wy >> ((c >> 1) & 1)) & 1u) && ((wz >> (c >> 2)) & 1u)) sel |= 1u << c;
-            const int r = (2 * lx + wx_i) + kMajMb * ((2 * ly + wy_i) + kMajMb * (2 * lz + wz_i));
-            const uint4 a = s_rec[r][0], b = s_rec[r][1];
-            const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+                for (int c = 0; c < 8; c++)
+                    if (((wx >> (c & 1)) & 1u) && ((wy >> ((c >> 1) & 1)) & 1u) && ((wz >> (c >> 2)) & 1u))
+                        sel |= 1u << c;
+                const int r = (2 * lx + wx_i) + kMajMb * ((2 * ly + wy_i) + kMajMb * (2 * lz + wz_i));
+                uint32_t w[8] = {~0u, ~0u, ~0u, ~0u, ~0u, ~0u, ~0u, ~0u};
+                if (sel & 0x0fu) {
+                    const uint4 a = s_rec[r][0];
+                    w[0] = a.x, w[1] = a.y, w[2] = a.z, w[3] = a.w;
+                }
+                if (sel & 0xf0u) {
+                    const uint4 b = s_rec[r][1];
+                    w[4] = b.x, w[5] = b.y, w[6] = b.z, w[7] = b.w;
+                }
 #pragma unroll
-            for (int c = 0; c < 8; c++)
-                if ((sel >> c) & 1u) lo = min(lo, w[c] & 0xffffu), hi = max(hi, w[c] >> 16);
-            nz |= uint32_t(s_nz[r]) & sel;
-        }
-#pragma unroll
-    for (int o = 1; o < 4; o <<= 1) {
-        lo = min(lo, uint32_t(__shfl_xor(int(lo), o)));
-        hi = max(hi, uint32_t(__shfl_xor(int(hi), o)));
-        nz |= uint32_t(__shfl_xor(int(nz), o));
-    }
-    const int bx = tbx + lx, by = tby + ly, bz = tbz + lz;
-    if (part != 0 || bx >= mx || by >= my || bz >= mz) return;
-    auto tf_max = [&](uint32_t clo, uint32_t chi) {
-        const int first = int((64u * clo) / 65535u), last = int((64u * chi) / 65535u);
-        float mv = 0.0f;
-        for (int b = min(first, 63); b <= min(last, 63); b++) mv = fmaxf(mv, s_bucket[b]);
-        return mv;
+                for (int c = 0; c < 8; c++)
+                    if ((sel >> c) & 1u) acc = pk_min_u16(acc, w[c]);
+                if (mode == 1) nz |= uint32_t(s_nz[r]) & sel;
+            }
     };
-    const float bound = (mode == 1 && nz == 0u) ? 0.0f : tf_max(lo, hi);
-    out[size_t(bx) + size_t(mx) * (size_t(by) + size_t(my) * size_t(bz))] = bound;
+    if (layer == 0)
+        reduce(std::integral_constant<int, 0>{});
+    else if (layer == 3)
+        reduce(std::integral_constant<int, 3>{});
+    else if (layer == 1)
+        reduce(std::integral_constant<int, 1>{});
+    else
+        reduce(std::integral_constant<int, 2>{});
+    s_layer[layer][lb][0] = acc;
+    s_layer[layer][lb][1] = nz;
+    __syncthreads();
+    const int bx = tbx + lx, by = tby + ly, bz = tbz + lz;
+    if (layer != 0 || bx >= mx || by >= my || bz >= mz) return;
+#pragma unroll
+    for (int q = 1; q < 4; q++) {
+        acc = pk_min_u16(acc, s_layer[q][lb][0]);
+        nz |= s_layer[q][lb][1];
+    }
+    const uint32_t range = bucket_range(acc & 0xffffu, 0xffffu - (acc >> 16));
+    const size_t brick = size_t(bx) + size_t(mx) * (size_t(by) + size_t(my) * size_t(bz));
+    ranges[brick] = uint16_t(range);
+    out[brick] = (mode == 1 && nz == 0u) ? 0.0f : bucket_range_max(s_runmax, range);
+}
+
+// Mode off after a transfer-function change: each brick's bound from its kept
+// bucket range.
+__global__ void __launch_bounds__(kBlock) k_majorant_ranges(const uint16_t* __restrict__ ranges,
+                                                            float* __restrict__ out, uint32_t bricks,
+                                                            const vvr_uniforms* __restrict__ u) {
+    __shared__ float s_runmax[7][64];
+    build_run_max(s_runmax, u);
+    __syncthreads();
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < bricks) out[i] = bucket_range_max(s_runmax, ranges[i]);
 }
 
 inline unsigned blocks_for(size_t n) { return unsigned((n + kBlock - 1) / kBlock); }
@@ -686,18 +778,28 @@ void launch_mask_cells(const uint32_t* mask, const uint8_t* labels, const BrickG
 }
 
 void launch_majorant(const uint16_t* density, const uint32_t* mask, float* out, const BrickGrid& g, int mode,
-                     const vvr_uniforms* u, uint32_t* cell_mm, uint8_t* cell_nz, hipStream_t st) {
+                     const vvr_uniforms* u, const MajorantCells& c, hipStream_t st) {
     const int mx = (g.nx + 7) / 8, my = (g.ny + 7) / 8, mz = (g.nz + 7) / 8;
     const size_t nmb = size_t(g.bx) * g.by * g.bz, bricks = size_t(mx) * my * mz;
+    if (mode == 0 && !c.density) {
+        hipLaunchKernelGGL(k_majorant_ranges, dim3(blocks_for(bricks)), dim3(kBlock), 0, st, c.ranges, out,
+                           uint32_t(bricks), u);
+        return;
+    }
     if (mode >= 2)
         hipLaunchKernelGGL(k_fill_f32, dim3(blocks_for(bricks)), dim3(kBlock), 0, st, out, uint32_t(bricks), 0.0f);
-    const size_t groups = (nmb + kMajRun - 1) / kMajRun;
-    hipLaunchKernelGGL(k_majorant_cells, dim3(blocks_for(groups * 8)), dim3(kBlock), 0, st, density, mask, cell_mm,
-                       cell_nz, out, g, mode, mx, my, mz, u);
+    const bool with_density = mode <= 1 && c.density, with_mask = mode >= 2 || (mode == 1 && c.mask);
+    if (with_density || with_mask) {
+        const size_t groups = (nmb + kMajRun - 1) / kMajRun;
+        const bool whole = g.nx % 4 == 0 && g.ny % 4 == 0 && g.nz % 4 == 0;
+        hipLaunchKernelGGL(whole ? k_majorant_cells<true> : k_majorant_cells<false>, dim3(blocks_for(groups * 8)),
+                           dim3(kBlock), 0, st, density, mask, c.cells, c.nz, out, g, mode, mx, my, mz, u,
+                           with_density, with_mask);
+    }
     if (mode <= 1) {
         const dim3 grid((mx + kMajTile - 1) / kMajTile, (my + kMajTile - 1) / kMajTile, (mz + kMajTile - 1) / kMajTile);
-        hipLaunchKernelGGL(k_majorant_bricks, grid, dim3(kMajThreads), 0, st, cell_mm, cell_nz, out, g, mode, mx, my,
-                           mz, u);
+        hipLaunchKernelGGL(k_majorant_bricks, grid, dim3(kMajThreads), 0, st, c.cells, c.nz, c.ranges, out, g, mode,
+                           mx, my, mz, u);
     }
 }
 

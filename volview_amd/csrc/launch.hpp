@@ -38,9 +38,19 @@ inline BrickGrid mask_cell_grid(const BrickGrid& g) { return make_brick_grid(g.n
 void launch_mask_cells(const uint32_t* mask, const uint8_t* labels, const BrickGrid& g, int mode,
                        const vvr_uniforms* u, uint32_t* bits, uint8_t* cell_labels, hipStream_t st);
 
-// majorant.comp: `mask` may be null when mode is 0.
+// What the majorant build of modes off and tint keeps per volume.
+struct MajorantCells {
+    uint32_t* cells;   // per micro-brick, its 2^3 cells' density min | (65535 - max) << 16
+    uint8_t* nz;       // tint: per micro-brick, a bit per cell set when some mask value there is non-zero
+    uint16_t* ranges;  // per brick, the transfer-function buckets first | last << 8 of its window's density
+    bool density;      // rebuild cells and ranges (else they hold the density's: it never changes)
+    bool mask;         // tint: rebuild nz (else it holds the current mask's bits)
+};
+
+// majorant.comp: `mask` may be null when mode is 0. Mode off with the density's
+// ranges kept only maps each brick's range through the transfer function.
 void launch_majorant(const uint16_t* density, const uint32_t* mask, float* out, const BrickGrid& g, int mode,
-                     const vvr_uniforms* u, uint32_t* cell_mm, uint8_t* cell_nz, hipStream_t st);
+                     const vvr_uniforms* u, const MajorantCells& c, hipStream_t st);
 
 // dvr.comp: hdr = RGBA16F (width x height x 4 halves).
 void launch_dvr(const DevScene& s, uint16_t* hdr, uint32_t width, uint32_t height, hipStream_t st);

@@ -356,7 +356,14 @@ struct vvr_renderer {
     uint32_t* normals = nullptr;
     float* majorant = nullptr;
     uint32_t* maj_cells = nullptr;  // majorant build (modes off, tint): per micro-brick, its 2^3 cells' density
-    uint8_t* maj_nz = nullptr;      // min | max << 16, and (tint) a bit per cell: some mask value is non-zero
+    uint8_t* maj_nz = nullptr;      // min | (65535 - max) << 16, and (tint) a bit per cell: some mask value is
+                                    // non-zero
+    uint16_t* maj_ranges = nullptr;  // per brick, its window's transfer-function bucket range
+    // The density never changes after upload, so its cell records and brick ranges
+    // are built once; the mask bits of maj_nz follow the mask (mask_builds counts
+    // its rebuilds).
+    bool maj_cells_ready = false;
+    uint64_t mask_builds = 0, maj_nz_build = 0;
     uint8_t* labels = nullptr;
     uint32_t* mask = nullptr;    // image A: what rendering samples
     uint32_t* mask_b = nullptr;  // image B: blur partner
@@ -467,16 +474,24 @@ struct vvr_renderer {
             run(VVR_KERNEL_MASK,
                 [&] { launch_mask_cells(mask, labels, g, int(opacity.mode), du, mask_cells, cell_labels, st); });
             mask_key = opacity;
+            mask_builds++;
         }
         if (!majorant_key || *majorant_key != opacity) {
             // Cell records of the two-pass build, allocated on the first build that needs them.
+            // A transfer-function change only reduces the kept records into bricks.
             const size_t nmb = size_t(g.bx) * g.by * g.bz;
-            if (opacity.mode <= VVR_LABELS_TINT && !maj_cells) maj_cells = dalloc<uint32_t>(8 * nmb, "majorant cells");
+            if (opacity.mode <= VVR_LABELS_TINT && !maj_cells) {
+                maj_cells = dalloc<uint32_t>(8 * nmb, "majorant cells");
+                maj_ranges = dalloc<uint16_t>(size_t(mdims[0]) * mdims[1] * mdims[2], "majorant brick ranges");
+            }
             if (opacity.mode == VVR_LABELS_TINT && !maj_nz) maj_nz = dalloc<uint8_t>(nmb, "majorant mask cells");
+            const MajorantCells cells{maj_cells, maj_nz, maj_ranges, !maj_cells_ready, maj_nz_build != mask_builds};
             run(VVR_KERNEL_MAJORANT, [&] {
                 launch_majorant(density, opacity.mode != VVR_LABELS_OFF ? mask : nullptr, majorant, g, int(opacity.mode),
-                                du, maj_cells, maj_nz, st);
+                                du, cells, st);
             });
+            if (opacity.mode <= VVR_LABELS_TINT) maj_cells_ready = true;
+            if (opacity.mode == VVR_LABELS_TINT) maj_nz_build = mask_builds;
             majorant_key = opacity;
         }
         if (p.path_tracing_enabled) return opacity;  // the path tracer computes its own lighting
@@ -623,7 +638,7 @@ struct vvr_renderer {
         wf.reset();
         dfree(density), dfree(density_f), dfree(normals), dfree(majorant), dfree(labels), dfree(mask), dfree(mask_b), dfree(mask_cells), dfree(cell_labels),
             dfree(soft);
-        dfree(maj_cells), dfree(maj_nz);
+        dfree(maj_cells), dfree(maj_nz), dfree(maj_ranges);
         dfree(shadow), dfree(ao), dfree(env_rad), dfree(env_table), dfree(env_dif);
         dfree(lut), dfree(preint), dfree(d_uniforms[0]), dfree(d_uniforms[1]), dfree(exposure);
         if (h_uniforms) (void)hipHostFree(h_uniforms);
