@@ -512,8 +512,10 @@ __global__ void __launch_bounds__(64) k_wf_regen_served(const PtRequest* __restr
 // Stage 1: free flight over the `*n_in` entries of `in` (kNone: padding): new
 // flights, and flights a previous launch suspended (p_flight); each runs at
 // most `steps` tracking iterations, and one that is not done goes back on ff.
+// Without the ghost branch the kernel is held to 168 registers (three waves per
+// SIMD, 24 spilled): PT-1 218.4 against 221.8 ms per frame at two waves.
 template <bool kBatched, bool kGhosts = true>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kGhosts ? 2 : 3)))
 k_wf_extend(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q, const uint32_t* __restrict__ in,
             const uint32_t* __restrict__ n_in, uint32_t cur, int steps) {
     const uint32_t nxt = cur ^ 1u;
