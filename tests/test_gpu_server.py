@@ -4,7 +4,9 @@
   several volumes equal, bit for bit, the frames of the same request rendered
   alone by its renderer (path-traced ones with samples_per_frame = the
   server's increment, frame after frame): work items carry their request, each
-  path its own seed, and each pixel's paths are summed in path order.
+  path its own seed, and each pixel's paths are summed in path order; also
+  with a ghosted label in the mix and with path-traced requests in isolate mode
+  only (alone, the renderer runs its ghost-free and density-free free flights).
 - Progressive requests deliver frames in order, at most one per increment
   (increments that finish together deliver the newest), the last with every path.
 - A small HBM budget makes requests wait for admission; all still finish and
@@ -39,7 +41,7 @@ def volumes(ctx):
     return a, b, n
 
 
-def mixed_requests(a, b, n):
+def mixed_requests(a, b, n, ghost=False):
     dvr_a = workloads.dvr_params(a, 0)
     pt_a = workloads.pt_params(a, 0, 8, bounces=4, camera_index=1)
     pt_learned = workloads.pt_params(a, 0, 8, bounces=4, camera_index=3)
@@ -48,6 +50,10 @@ def mixed_requests(a, b, n):
     pt_b.path_tracing.denoiser = _abi.DENOISE_ATROUS  # each delivered frame denoised
     dvr_b = workloads.dvr_params(b, 1)
     workloads.organ_label_params(dvr_b, n, 7)  # same labels as pt_b: one opacity key per volume
+    if ghost:  # while pt_b is in flight the free flights keep the ghost branch; alone, pt_a's do not
+        for q in (pt_b, dvr_b):
+            q.labels.ghost[2] = 1
+            q.labels.ghost[5] = 1
     return [(a, dvr_a, 64, 48), (a, pt_a, 32, 24), (b, pt_b, 24, 32), (b, dvr_b, 40, 40),
             (a, workloads.dvr_params(a, 3), 17, 29), (a, pt_learned, 36, 20)]
 
@@ -81,11 +87,14 @@ def drain(srv, ids, timeout_polls=600, cancelled=()):
     raise AssertionError(f"requests did not finish: {set(ids) - finals}")
 
 
-def test_served_frames_equal_alone(ctx):
+@pytest.mark.parametrize("ghost", [False, True, "isolate only"])
+def test_served_frames_equal_alone(ctx, ghost):
     a, b, n = volumes(ctx)
     net = Unet.pretrained(ctx)
     a.set_denoiser(net)
-    reqs = mixed_requests(a, b, n)
+    reqs = mixed_requests(a, b, n, ghost is True)
+    if ghost == "isolate only":  # every path-traced request in isolate mode: no density fetches
+        reqs = [q for q in reqs if q[0] is b]
     inc = 4
     srv = Server(ctx, pt_increment=inc)
     ids = [srv.submit(r, p, w, h, tag=k) for k, (r, p, w, h) in enumerate(reqs)]

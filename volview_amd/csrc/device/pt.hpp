@@ -239,15 +239,18 @@ __device__ __forceinline__ float bsdf_pdf(V3 n, V3 v, V3 l, const Sample& s) {  
 
 // pt.comp:277-292. kGhosts false: for frames whose labels cannot be drawn as
 // ghosts (no label colour's alpha above 1.5), which never take the ghost branch;
-// without its surface frame the free-flight kernel needs 202 instead of 256+
-// registers and no spills.
-template <bool kGhosts = true>
+// without its surface frame the free-flight kernel needs 222 instead of 256+
+// registers and no spills. kDensity false: for frames in isolate mode, which
+// never read the density (density_unused); without its fetches the kernel runs
+// at three waves per SIMD.
+template <bool kGhosts = true, bool kDensity = true>
 __device__ inline float extinction_at(const DevScene& sc, V3 idx, V3 dir, bool smooth_density, Sample& s, int& cap_plane,
                                       float& cut) {
     cut = clip_depth(sc.u, idx, cap_plane);
     if (cut > 0.0f) return 0.0f;
-    const float d =
-        density_unused(sc.u) ? 0.0f : (smooth_density ? density_view(sc, idx) : density_at_index(sc, idx));
+    const float d = (!kDensity || density_unused(sc.u))
+                        ? 0.0f
+                        : (smooth_density ? density_view(sc, idx) : density_at_index(sc, idx));
     s = apply_labels(sc, tf_point(sc, d), idx);
     if (s.extinction <= 0.0f) return 0.0f;
     if (kGhosts && s.ghost) {
@@ -307,7 +310,7 @@ enum Flow : int { kFlowExit = 0, kFlowHit = 1, kFlowSuspended = 2 };
 // Delta tracking against the brick bounds (pt.comp:300-334) from the loop state
 // `f`, stopping before iteration `stop` (kFlowSuspended; `f`, `spectral` and
 // `rng` then resume it exactly where it stopped).
-template <bool kGhosts = true>
+template <bool kGhosts = true, bool kDensity = true>
 __device__ inline Flow free_flight_steps(const DevScene& sc, Flight& f, V3 dir, bool smooth_density, int hero,
                                          V3& spectral, V3& idx, Sample& s, int& cap_plane, float& cut, Rng& rng,
                                          int stop) {
@@ -317,7 +320,7 @@ __device__ inline Flow free_flight_steps(const DevScene& sc, Flight& f, V3 dir, 
         float majorant;
         if (!flight_step(sc, f, rng, majorant)) continue;
         idx = f.origin + f.d_idx * f.tt;
-        const float extinction = extinction_at<kGhosts>(sc, idx, dir, smooth_density, s, cap_plane, cut);
+        const float extinction = extinction_at<kGhosts, kDensity>(sc, idx, dir, smooth_density, s, cap_plane, cut);
         if (extinction <= 0.0f) continue;
         const V3 sigma = path_sigma(u, s, extinction);
         const float sh = component(sigma, hero);
@@ -498,12 +501,12 @@ __device__ __forceinline__ bool stage_extend(const DevScene& sc, const FrameCons
 
 // stage_extend as a bounded run from the flight state `f` (flight_begin of the
 // path's ray, or a suspended run's state): at most up to iteration `stop`.
-template <bool kGhosts = true>
+template <bool kGhosts = true, bool kDensity = true>
 __device__ __forceinline__ Flow stage_extend_steps(const DevScene& sc, const FrameConsts& fc, PathState& st,
                                                    EventState& ev, Flight& f, int stop) {
     ev.cap_plane = -1;
     ev.cut = 0.0f;
-    const Flow flow = free_flight_steps<kGhosts>(sc, f, st.dir, st.bounce == 0u, st.hero, st.spectral, ev.idx, ev.s,
+    const Flow flow = free_flight_steps<kGhosts, kDensity>(sc, f, st.dir, st.bounce == 0u, st.hero, st.spectral, ev.idx, ev.s,
                                                  ev.cap_plane, ev.cut, st.rng, stop);
     if (flow == kFlowExit) path_escape(sc, fc, st);
     return flow;

@@ -512,10 +512,12 @@ __global__ void __launch_bounds__(64) k_wf_regen_served(const PtRequest* __restr
 // Stage 1: free flight over the `*n_in` entries of `in` (kNone: padding): new
 // flights, and flights a previous launch suspended (p_flight); each runs at
 // most `steps` tracking iterations, and one that is not done goes back on ff.
-// Without the ghost branch the kernel is held to 168 registers (three waves per
-// SIMD, 24 spilled): PT-1 218.4 against 221.8 ms per frame at two waves.
-template <bool kBatched, bool kGhosts = true>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kGhosts ? 2 : 3)))
+// Without the ghost branch and the density fetches (isolate mode) the kernel is
+// held to 168 registers (three waves per SIMD, 14 spilled): PT-1 218.4 against
+// 221.8 ms per frame at two waves; with the density read, three waves (24
+// spilled, in the fetches) made Batched-32's path-traced requests 15 % slower.
+template <bool kBatched, bool kGhosts = true, bool kDensity = true>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kGhosts || kDensity ? 2 : 3)))
 k_wf_extend(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q, const uint32_t* __restrict__ in,
             const uint32_t* __restrict__ n_in, uint32_t cur, int steps) {
     const uint32_t nxt = cur ^ 1u;
@@ -537,7 +539,7 @@ k_wf_extend(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q, const uin
             const PtRequest& R = reqs[r];
             Flight f = flight_begin(R.scene.u, st.origin, st.dir);
             if (fl.y != kWfNewFlight) f.tt = bitsf(fl.x), f.i = int(fl.y);
-            flow = stage_extend_steps<kGhosts>(R.scene, frame_consts(R.scene.u), st, ev, f, f.i + steps);
+            flow = stage_extend_steps<kGhosts, kDensity>(R.scene, frame_consts(R.scene.u), st, ev, f, f.i + steps);
             fl = make_uint2(fbits(f.tt), uint32_t(f.i));
         });
         if (active) {
@@ -940,6 +942,7 @@ WfGrid wf_grid(int cus) {
                            waves_for(k_wf_shade<false>, cus, 32), waves_for(k_wf_shadow<false>, cus, 32),
                            waves_for(k_wf_scatter<false>, cus, 32)};
     g.extend_no_ghosts = waves_for(k_wf_extend<false, false>, cus, 32);
+    g.extend_isolate = waves_for(k_wf_extend<false, false, false>, cus, 32);
     // The served regen's occupancy bounds the batched regen grid too (it runs in its place).
     g.batched = WfStageGrid{std::min(waves_for(k_wf_regen<true>, cus, 32), waves_for(k_wf_regen_served, cus, 32)),
                             waves_for(k_wf_extend<true>, cus, 32), waves_for(k_wf_shade<true>, cus, 32),
@@ -959,7 +962,7 @@ namespace {
 // each queue binned by request first.
 template <bool kBatched>
 void wf_stages(const PtRequest* reqs, const WfSlots& S, const WfQueues& Q, uint32_t cur, const WfGrid& grid, bool sort,
-               bool env_map, bool ghosts, hipStream_t st) {
+               bool env_map, bool ghosts, bool density, hipStream_t st) {
     const WfStageGrid& g = kBatched ? grid.batched : grid.single;
     const uint32_t* ff = Q.ff[cur];
     const uint32_t* n_ff = Q.counts + kWfFF0 + cur;
@@ -973,9 +976,14 @@ void wf_stages(const PtRequest* reqs, const WfSlots& S, const WfQueues& Q, uint3
         ff = Q.sorted;
         n_ff = Q.counts + kWfSORTFF;
     }
-    if (!kBatched && !ghosts)
+    // Engines over several requests keep the ghost branch and the density fetches: without
+    // them (k_wf_extend<true, false, ...>) Batched-32 served in 0.91 s against 0.89 s with them.
+    if (!kBatched && !ghosts && density)
         hipLaunchKernelGGL((k_wf_extend<false, false>), dim3(grid.extend_no_ghosts), dim3(64), 0, st, reqs, S, Q, ff,
                            n_ff, cur, grid.flight_steps);
+    else if (!kBatched && !ghosts)
+        hipLaunchKernelGGL((k_wf_extend<false, false, false>), dim3(grid.extend_isolate), dim3(64), 0, st, reqs, S, Q,
+                           ff, n_ff, cur, grid.flight_steps);
     else
         hipLaunchKernelGGL(k_wf_extend<kBatched>, dim3(g.extend), dim3(64), 0, st, reqs, S, Q, ff, n_ff, cur,
                            grid.flight_steps);
@@ -997,13 +1005,13 @@ void wf_stages(const PtRequest* reqs, const WfSlots& S, const WfQueues& Q, uint3
 }  // namespace
 
 void wf_iteration(const PtRequest* reqs, uint32_t n_reqs, uint32_t total_items, const WfSlots& S, const WfQueues& Q,
-                  uint32_t cur, const WfGrid& g, bool env_map, bool ghosts, hipStream_t st) {
+                  uint32_t cur, const WfGrid& g, bool env_map, bool ghosts, bool density, hipStream_t st) {
     if (n_reqs > 1) {
         hipLaunchKernelGGL(k_wf_regen<true>, dim3(g.batched.regen), dim3(64), 0, st, reqs, n_reqs, total_items, S, Q, cur);
-        wf_stages<true>(reqs, S, Q, cur, g, true, env_map, true, st);  // n_reqs <= kWfBins (WfEngine::run)
+        wf_stages<true>(reqs, S, Q, cur, g, true, env_map, ghosts, density, st);  // n_reqs <= kWfBins (WfEngine::run)
     } else {
         hipLaunchKernelGGL(k_wf_regen<false>, dim3(g.single.regen), dim3(64), 0, st, reqs, n_reqs, total_items, S, Q, cur);
-        wf_stages<false>(reqs, S, Q, cur, g, false, env_map, ghosts, st);
+        wf_stages<false>(reqs, S, Q, cur, g, false, env_map, ghosts, density, st);
     }
 }
 
@@ -1015,7 +1023,7 @@ void wf_init_served(const WfSlots& S, const WfQueues& Q, uint32_t slots, hipStre
 void wf_iteration_served(const PtRequest* reqs, const WfActive& A, const WfServed& V, const WfSlots& S,
                          const WfQueues& Q, uint32_t cur, const WfGrid& g, bool env_map, hipStream_t st) {
     hipLaunchKernelGGL(k_wf_regen_served, dim3(g.batched.regen), dim3(64), 0, st, reqs, A, V, S, Q, cur);
-    wf_stages<true>(reqs, S, Q, cur, g, true, env_map, true, st);
+    wf_stages<true>(reqs, S, Q, cur, g, true, env_map, true, true, st);
 }
 
 void wf_finish(const PtRequest* reqs, const WfSlots& S, const WfQueues& Q, uint32_t cur, uint32_t paths, bool ghosts,

@@ -185,7 +185,9 @@ struct WfStageGrid {
 };
 struct WfGrid {
     WfStageGrid single, batched;  // one request / several (their kernels' occupancies differ)
-    int extend_no_ghosts;         // one request whose labels cannot be drawn as ghosts
+    // Free-flight grids of a single request's ghost-free kernels, reading the
+    // density or not (isolate mode).
+    int extend_no_ghosts, extend_isolate;
     int sort;                     // 256-lane workgroups of the queue sort
     // Tracking-loop iterations a free flight / shadow ray runs per launch before
     // it is suspended and re-queued, so that a wave's lanes do not idle behind
@@ -199,10 +201,11 @@ WfGrid wf_grid(int cus);
 // All `slots` slots empty on the end queue of iteration 0, item queue at 0.
 void wf_init(const WfSlots& S, const WfQueues& Q, uint32_t slots, hipStream_t st);
 // env_map: a request in flight renders with a light map (adds its stage).
-// ghosts: the (single) request's labels may be drawn as ghosts (else its
-// free flights run without the ghost branch).
+// ghosts: the (single) request's labels may be drawn as ghosts (else its free
+// flights run without the ghost branch); density: it reads the density (else,
+// in isolate mode, its free flights run without the fetches).
 void wf_iteration(const PtRequest* reqs, uint32_t n_reqs, uint32_t total_items, const WfSlots& S, const WfQueues& Q,
-                  uint32_t cur, const WfGrid& g, bool env_map, bool ghosts, hipStream_t st);
+                  uint32_t cur, const WfGrid& g, bool env_map, bool ghosts, bool density, hipStream_t st);
 // Served engine: all slots idle; one iteration drawing work from the active
 // requests of the table `reqs` (see k_wf_regen_served).
 void wf_init_served(const WfSlots& S, const WfQueues& Q, uint32_t slots, hipStream_t st);

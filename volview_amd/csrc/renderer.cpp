@@ -233,7 +233,9 @@ void record(vvr_renderer* r, uint64_t frame, const vvr_frame_params* p) {
         q.accum = r->accum, q.albedo = r->albedo, q.normal = r->normal, q.hdr = r->hdr;
         WfEngine::shape(q, r->width, r->height, first, spf, r->pt_chunk);  // throws before anything is queued
         wavefront(r);
-        r->timed(VVR_KERNEL_PT, [&] { r->wf->run(&q, 1, r->stream, u.lighting[3] == 2u, labels_may_draw_ghosts(u)); });
+        r->timed(VVR_KERNEL_PT, [&] {
+            r->wf->run(&q, 1, r->stream, u.lighting[3] == 2u, labels_may_draw_ghosts(u), frame_reads_density(u));
+        });
         r->accumulated += spf;
         // renderer.rs:1795-1797: the a-trous denoiser replaces the HDR image with
         // its filtered version of everything accumulated so far.

@@ -56,6 +56,9 @@ inline bool labels_may_draw_ghosts(const vvr_uniforms& u) {
             if (u.label_colors[l][3] > 1.5f) return true;
     return false;
 }
+// Whether a frame with uniforms `u` reads the density (all but isolate mode,
+// whose samples take the label's optics alone: density_unused on the device).
+inline bool frame_reads_density(const vvr_uniforms& u) { return uint32_t(u.labels[0]) != VVR_LABELS_ISOLATE; }
 // Such a frame can be ray marched in passes unless a label may be drawn as a
 // ghost, whose opacity comes from its shading frame, which the march cannot know.
 inline bool dvr_in_passes(const vvr_uniforms& u) { return !labels_may_draw_ghosts(u); }
@@ -225,8 +228,9 @@ struct WfEngine {
     // Renders the requests (scene, images and shape set) on `st`. Returns once
     // every path is traced or queued to be; the resolves are queued last.
     // env_map: some request renders with a light map; ghosts: a request's labels
-    // may be drawn as ghosts.
-    void run(const PtRequest* reqs, uint32_t n, hipStream_t st, bool env_map = false, bool ghosts = true) {
+    // may be drawn as ghosts; density: a request reads the density.
+    void run(const PtRequest* reqs, uint32_t n, hipStream_t st, bool env_map = false, bool ghosts = true,
+             bool density = true) {
         if (n == 0) return;
         size_t partial_f4 = 0;
         uint64_t items = 0;
@@ -259,7 +263,7 @@ struct WfEngine {
         uint32_t iter = 0;
         auto launch_batch = [&](int k) {
             for (uint32_t j = 0; j < kBatch; j++, iter++)
-                wf_iteration(d_reqs, n, base, S, Q, iter & 1u, grid, env_map, ghosts, st);
+                wf_iteration(d_reqs, n, base, S, Q, iter & 1u, grid, env_map, ghosts, density, st);
             check(hipGetLastError(), "wavefront iteration");
             check(hipMemcpyAsync(&h_live[k], Q.counts + kWfLIVE, 4, hipMemcpyDeviceToHost, st), "live count");
             check(hipEventRecord(live_ev[k], st), "hipEventRecord");
