@@ -948,10 +948,12 @@ WfGrid wf_grid(int cus) {
                             waves_for(k_wf_extend<true>, cus, 32), waves_for(k_wf_shade<true>, cus, 32),
                             waves_for(k_wf_shadow<true>, cus, 32), waves_for(k_wf_scatter<true>, cus, 32)};
     g.sort = cus * 2;
-    // Measured at PT-1 (512 x 512 x 400, 1024^2, 64 paths): 16 / 32 gives 220
-    // Msamples/s against 124 unsuspended; 8-24 / 16-48 are within 3 %.
-    g.flight_steps = 16;
-    g.shadow_steps = 32;
+    // Measured at PT-1 (512 x 512 x 400, 1024^2, 64 paths): 16 / 32 gave 220
+    // Msamples/s against 124 unsuspended, 8-24 / 16-48 within 3 %; with the
+    // three-wave free-flight kernel of isolate mode the autotuner picks 24 / 48
+    // (210.7 against 218.8 ms per frame).
+    g.flight_steps = 24;
+    g.shadow_steps = 48;
     g.tail_paths = 1u << 16;
     return g;
 }
