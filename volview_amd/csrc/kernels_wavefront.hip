@@ -943,6 +943,7 @@ WfGrid wf_grid(int cus) {
                            waves_for(k_wf_scatter<false>, cus, 32)};
     g.extend_no_ghosts = waves_for(k_wf_extend<false, false>, cus, 32);
     g.extend_isolate = waves_for(k_wf_extend<false, false, false>, cus, 32);
+    g.batched_extend_isolate = waves_for(k_wf_extend<true, false, false>, cus, 32);
     // The served regen's occupancy bounds the batched regen grid too (it runs in its place).
     g.batched = WfStageGrid{std::min(waves_for(k_wf_regen<true>, cus, 32), waves_for(k_wf_regen_served, cus, 32)),
                             waves_for(k_wf_extend<true>, cus, 32), waves_for(k_wf_shade<true>, cus, 32),
@@ -978,14 +979,16 @@ void wf_stages(const PtRequest* reqs, const WfSlots& S, const WfQueues& Q, uint3
         ff = Q.sorted;
         n_ff = Q.counts + kWfSORTFF;
     }
-    // Engines over several requests keep the ghost branch and the density fetches: without
-    // them (k_wf_extend<true, false, ...>) Batched-32 served in 0.91 s against 0.89 s with them.
-    if (!kBatched && !ghosts && density)
+    // Engines over several requests keep the ghost branch while any request reads the
+    // density: their ghost-free form (k_wf_extend<true, false, true>) served Batched-32 in
+    // 0.91 s against 0.89 s.
+    if (!ghosts && !density)
+        hipLaunchKernelGGL((k_wf_extend<kBatched, false, false>),
+                           dim3(kBatched ? grid.batched_extend_isolate : grid.extend_isolate), dim3(64), 0, st, reqs, S,
+                           Q, ff, n_ff, cur, grid.flight_steps);
+    else if (!kBatched && !ghosts)
         hipLaunchKernelGGL((k_wf_extend<false, false>), dim3(grid.extend_no_ghosts), dim3(64), 0, st, reqs, S, Q, ff,
                            n_ff, cur, grid.flight_steps);
-    else if (!kBatched && !ghosts)
-        hipLaunchKernelGGL((k_wf_extend<false, false, false>), dim3(grid.extend_isolate), dim3(64), 0, st, reqs, S, Q,
-                           ff, n_ff, cur, grid.flight_steps);
     else
         hipLaunchKernelGGL(k_wf_extend<kBatched>, dim3(g.extend), dim3(64), 0, st, reqs, S, Q, ff, n_ff, cur,
                            grid.flight_steps);
@@ -1023,9 +1026,10 @@ void wf_init_served(const WfSlots& S, const WfQueues& Q, uint32_t slots, hipStre
 }
 
 void wf_iteration_served(const PtRequest* reqs, const WfActive& A, const WfServed& V, const WfSlots& S,
-                         const WfQueues& Q, uint32_t cur, const WfGrid& g, bool env_map, hipStream_t st) {
+                         const WfQueues& Q, uint32_t cur, const WfGrid& g, bool env_map, bool isolate,
+                         hipStream_t st) {
     hipLaunchKernelGGL(k_wf_regen_served, dim3(g.batched.regen), dim3(64), 0, st, reqs, A, V, S, Q, cur);
-    wf_stages<true>(reqs, S, Q, cur, g, true, env_map, true, true, st);
+    wf_stages<true>(reqs, S, Q, cur, g, true, env_map, !isolate, !isolate, st);
 }
 
 void wf_finish(const PtRequest* reqs, const WfSlots& S, const WfQueues& Q, uint32_t cur, uint32_t paths, bool ghosts,

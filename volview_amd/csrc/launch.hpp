@@ -185,9 +185,9 @@ struct WfStageGrid {
 };
 struct WfGrid {
     WfStageGrid single, batched;  // one request / several (their kernels' occupancies differ)
-    // Free-flight grids of a single request's ghost-free kernels, reading the
-    // density or not (isolate mode).
-    int extend_no_ghosts, extend_isolate;
+    // Free-flight grids of the ghost-free kernels: a single request's, reading the
+    // density or not (isolate mode), and the several requests' in isolate mode.
+    int extend_no_ghosts, extend_isolate, batched_extend_isolate;
     int sort;                     // 256-lane workgroups of the queue sort
     // Tracking-loop iterations a free flight / shadow ray runs per launch before
     // it is suspended and re-queued, so that a wave's lanes do not idle behind
@@ -209,8 +209,10 @@ void wf_iteration(const PtRequest* reqs, uint32_t n_reqs, uint32_t total_items, 
 // Served engine: all slots idle; one iteration drawing work from the active
 // requests of the table `reqs` (see k_wf_regen_served).
 void wf_init_served(const WfSlots& S, const WfQueues& Q, uint32_t slots, hipStream_t st);
+// isolate: every request with paths in flight is in isolate mode and draws no ghost.
 void wf_iteration_served(const PtRequest* reqs, const WfActive& A, const WfServed& V, const WfSlots& S,
-                         const WfQueues& Q, uint32_t cur, const WfGrid& g, bool env_map, hipStream_t st);
+                         const WfQueues& Q, uint32_t cur, const WfGrid& g, bool env_map, bool isolate,
+                         hipStream_t st);
 // Chunk sums -> accumulation, guides and HDR images of one request.
 // Single-request engine, every item handed out: runs the paths still in flight
 // (at most `paths`) to their ends in one launch, after the iterations queued so

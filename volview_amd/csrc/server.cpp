@@ -510,15 +510,21 @@ void vvr_server::pt_batch() {
     V.head = d_ctr;
     V.done = d_ctr + kTable * kWfCounterStride;
     // The light-map stage runs while any increment with paths in flight uses a map.
-    bool env_map = false;
-    for (uint32_t e = 0; e < kTable && !env_map; e++)
-        env_map = table[e].job && !table[e].complete && h_uniforms[table[e].job->uslot].lighting[3] == 2u;
+    // The free flights drop the density fetches and the ghost branch while every such increment
+    // is in isolate mode and draws no ghost.
+    bool env_map = false, isolate = true;
+    for (uint32_t e = 0; e < kTable; e++) {
+        if (!table[e].job || table[e].complete) continue;
+        const vvr_uniforms& hu = h_uniforms[table[e].job->uslot];
+        env_map = env_map || hu.lighting[3] == 2u;
+        isolate = isolate && !frame_reads_density(hu) && !labels_may_draw_ghosts(hu);
+    }
     for (uint32_t it = 0; it < cfg.iterations; it++) {
         V.idle_in = idle_in;
         V.revive = revive ? 1u : 0u;
         if (revive)
             check(hipMemsetAsync(engine.Q.counts + kWfIDLE0 + (idle_in ^ 1u), 0, 4, stream), "idle queue reset");
-        wf_iteration_served(d_table, A, V, engine.S, engine.Q, cur, engine.grid, env_map, stream);
+        wf_iteration_served(d_table, A, V, engine.S, engine.Q, cur, engine.grid, env_map, isolate, stream);
         check(hipGetLastError(), "wavefront iteration");
         if (revive) idle_in ^= 1u;
         revive = false;
