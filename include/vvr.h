@@ -409,8 +409,8 @@ typedef struct vvr_server_config {
     uint32_t slots;        /* path slots of the wavefront engine (0: 4,194,304) */
     uint32_t pt_increment; /* paths per pixel per delivered frame of a path-traced request (0: all at once) */
     uint32_t max_requests; /* requests admitted at once, at most 256 (0: 256) */
-    uint32_t flight_steps; /* vvr_pt_shape's, for the server's engine (0: 16) */
-    uint32_t shadow_steps; /* (0: 32) */
+    uint32_t flight_steps; /* vvr_pt_shape's, for the server's engine (0: 24) */
+    uint32_t shadow_steps; /* (0: 48) */
     uint32_t iterations;   /* wavefront iterations queued per pass between the host's looks at the
                               counters, 1..64 (0: 2) */
 } vvr_server_config;
