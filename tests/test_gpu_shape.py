@@ -41,7 +41,7 @@ def test_frames_equal_for_every_shape(ctx):
     p = workloads.pt_params(r, n, 8, bounces=8, camera_index=2)
     w, h = 48, 40
     if not any(k in os.environ for k in ("VVR_PT_SLOTS", "VVR_PT_FLIGHT_STEPS", "VVR_PT_SHADOW_STEPS")):
-        assert r.pt_shape() == {"slots": 1 << 22, "flight_steps": 24, "shadow_steps": 48, "tail_paths": 1 << 16}
+        assert r.pt_shape() == {"slots": 1 << 22, "flight_steps": 24, "shadow_steps": 48, "tail_paths": 1 << 17}
     want = frame(r, p, w, h)
     never = Renderer.TAIL_NEVER
     for shape in ({"slots": 192, "flight_steps": 1, "shadow_steps": 1, "tail_paths": never},

@@ -807,8 +807,12 @@ __device__ __forceinline__ bool run_to_end(const PtRequest& R, const WfSlots& S,
 // few paths remain: each path still in flight runs to its end in one launch,
 // a lane per slot. Entries: the ended paths of end[cur] (regen's turn), the
 // suspended shadow rays of sh[cur] and the flights of ff[cur] (new or suspended).
+// Held to two waves per SIMD (256 registers, ~170 spilled, against 393 unbounded):
+// each path's chain is latency-bound either way, and twice the resident lanes let
+// the tail start at 128 Ki live paths instead of 64 Ki (PT-1 205.9 against 210.9
+// ms per frame).
 template <bool kGhosts>
-__global__ void __launch_bounds__(64) k_wf_finish(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_wf_finish(const PtRequest* __restrict__ reqs, WfSlots S, WfQueues Q,
                                                   uint32_t cur) {
     const uint32_t n_end = Q.counts[kWfEND0 + cur], n_sh = Q.counts[kWfSH0 + cur];
     const uint32_t n = n_end + n_sh + Q.counts[kWfFF0 + cur];
@@ -955,7 +959,7 @@ WfGrid wf_grid(int cus) {
     // (210.7 against 218.8 ms per frame).
     g.flight_steps = 24;
     g.shadow_steps = 48;
-    g.tail_paths = 1u << 16;
+    g.tail_paths = 1u << 17;
     return g;
 }
 
