@@ -849,7 +849,10 @@ void create_server(vvr_ctx* ctx, const vvr_server_config* cfg, vvr_server** out)
         s->h_jobs[k] = halloc<DvrJob>(64, "ray-marching job staging");
         check(hipEventCreateWithFlags(&s->jobs_ev[k], hipEventDisableTiming), "hipEventCreate");
     }
-    s->tail_cap = s->engine.grid.tail_paths;
+    // Served tails run k_wf_tail (one wave per SIMD): 64 Ki paths measured best on the
+    // serving trace (256 Ki cost a quarter of the throughput); VVR_PT_TAIL overrides.
+    s->tail_cap = std::getenv("VVR_PT_TAIL") ? s->engine.grid.tail_paths
+                                             : std::min(s->engine.grid.tail_paths, 1u << 16);
     if (s->tail_cap) {
         check(hipStreamCreateWithFlags(&s->tail_stream, hipStreamNonBlocking), "hipStreamCreate");
         for (auto& t : s->tails) {
