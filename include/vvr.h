@@ -380,8 +380,8 @@ int vvr_renderer_get_pt_shape(vvr_renderer* r, vvr_pt_shape* out);
 int vvr_renderer_set_pt_shape(vvr_renderer* r, const vvr_pt_shape* shape);
 /* Times the path-tracing frames of `params` (path_tracing_enabled, the renderer
  * resized) with candidate shapes and keeps the fastest, written to `chosen`: the
- * current shape, step pairs (8,16) (12,24) (16,32) (24,48) (32,64) at the current
- * slots, then 1, 2 and 4 Mi slots with the best pair, then tail thresholds of 32,
+ * current shape, step pairs (8,16) (12,24) (16,32) (24,32) (24,48) (32,64) at the
+ * current slots, then 1, 2, 4 and 8 Mi slots with the best pair, then tail thresholds of 32,
  * 64, 128 and 256 Ki paths and none with the best so far. Each candidate's time is the
  * least of `trials` frames (HIP events); a candidate replaces the best so far only
  * when it is 2 % faster. The trial frames are recorded as frames 1, 2, ... and

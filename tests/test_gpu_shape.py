@@ -41,7 +41,7 @@ def test_frames_equal_for_every_shape(ctx):
     p = workloads.pt_params(r, n, 8, bounces=8, camera_index=2)
     w, h = 48, 40
     if not any(k in os.environ for k in ("VVR_PT_SLOTS", "VVR_PT_FLIGHT_STEPS", "VVR_PT_SHADOW_STEPS")):
-        assert r.pt_shape() == {"slots": 1 << 22, "flight_steps": 24, "shadow_steps": 48, "tail_paths": 1 << 17}
+        assert r.pt_shape() == {"slots": 1 << 23, "flight_steps": 24, "shadow_steps": 32, "tail_paths": 1 << 17}
     want = frame(r, p, w, h)
     never = Renderer.TAIL_NEVER
     for shape in ({"slots": 192, "flight_steps": 1, "shadow_steps": 1, "tail_paths": never},
@@ -62,9 +62,9 @@ def test_autotune_keeps_a_candidate_and_frames(ctx):
     w, h = 64, 48
     r.resize(w, h)
     chosen = r.autotune(p, trials=1)
-    assert (chosen["flight_steps"], chosen["shadow_steps"]) in {(8, 16), (12, 24), (16, 32), (24, 48), (32, 64)}
+    assert (chosen["flight_steps"], chosen["shadow_steps"]) in {(8, 16), (12, 24), (16, 32), (24, 32), (24, 48), (32, 64)}
     # (the default shape is the incumbent, and is one of the candidates)
-    assert chosen["slots"] in {1 << 20, 1 << 21, 1 << 22}
+    assert chosen["slots"] in {1 << 20, 1 << 21, 1 << 22, 1 << 23}
     assert chosen["tail_paths"] in {1 << 15, 1 << 16, 1 << 17, 1 << 18, Renderer.TAIL_NEVER}
     assert r.pt_shape() == chosen
     assert r.accumulated_samples() > 4  # the trial frames accumulated

@@ -180,11 +180,11 @@ void autotune(vvr_renderer* r, const vvr_frame_params* p, uint32_t trials, vvr_p
             const float ms = time_shape(s);
             if (ms < 0.98f * best_ms) best_ms = ms, best = s;
         };
-        const uint32_t steps[][2] = {{8, 16}, {12, 24}, {16, 32}, {24, 48}, {32, 64}};
+        const uint32_t steps[][2] = {{8, 16}, {12, 24}, {16, 32}, {24, 32}, {24, 48}, {32, 64}};
         const vvr_pt_shape s0 = best;
         for (const auto& st : steps) consider(vvr_pt_shape{s0.slots, st[0], st[1], s0.tail_paths});
         const vvr_pt_shape s1 = best;
-        for (uint32_t slots : {1u << 20, 1u << 21, 1u << 22})
+        for (uint32_t slots : {1u << 20, 1u << 21, 1u << 22, 1u << 23})
             consider(vvr_pt_shape{slots, s1.flight_steps, s1.shadow_steps, s1.tail_paths});
         const vvr_pt_shape s2 = best;
         for (uint32_t tail : {1u << 15, 1u << 16, 1u << 17, 1u << 18, UINT32_MAX})

@@ -112,7 +112,7 @@ struct WfEngine {
     uint32_t* h_live = nullptr;   // pinned, two read-back slots
     hipEvent_t live_ev[2] = {nullptr, nullptr};
     WfGrid grid{};
-    uint32_t max_slots = 1u << 22;
+    uint32_t max_slots = 1u << 23;
 
     explicit WfEngine(int dev) : device(dev) {
         int cus = 0;
@@ -121,9 +121,10 @@ struct WfEngine {
         grid.flight_steps = int(std::min(env_u32("VVR_PT_FLIGHT_STEPS", uint32_t(grid.flight_steps)), 4096u));
         grid.shadow_steps = int(std::min(env_u32("VVR_PT_SHADOW_STEPS", uint32_t(grid.shadow_steps)), 4096u));
         if (const char* v = std::getenv("VVR_PT_TAIL")) grid.tail_paths = uint32_t(std::strtoul(v, nullptr, 10));  // 0: never
-        // 4M slots (about 1 GiB): enough paths in flight that the long random
-        // walks of a frame's last iterations still fill the device.
-        max_slots = env_u32("VVR_PT_SLOTS", 1u << 22);
+        // 8M slots (about 3 GiB of the 288): enough paths in flight that the long
+        // random walks of a frame's last iterations still fill the device (PT-1
+        // 201.8 against 207.1 ms per frame at 4M, with 24 / 32 tracking runs).
+        max_slots = env_u32("VVR_PT_SLOTS", 1u << 23);
         check(hipHostMalloc((void**)&h_live, 2 * sizeof(uint32_t), hipHostMallocDefault), "hipHostMalloc");
         for (auto& e : live_ev) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
     }

@@ -830,7 +830,10 @@ void create_server(vvr_ctx* ctx, const vvr_server_config* cfg, vvr_server** out)
     if (!c.iterations) c.iterations = 2;
     if (c.iterations > 64) throw Error{VVR_EINVAL, "iterations per pass must be 1..64"};
     s->cfg = c;
-    s->engine.set_shape(vvr_pt_shape{c.slots, c.flight_steps, c.shadow_steps});
+    // The served engine's defaults, as measured on the serving trace and Batched-32: 4M
+    // slots, tracking runs of 24 / 48.
+    s->engine.set_shape(vvr_pt_shape{c.slots ? c.slots : 1u << 22, c.flight_steps ? c.flight_steps : 24u,
+                                     c.shadow_steps ? c.shadow_steps : 48u});
     check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
     s->engine.reserve_slots(s->engine.max_slots);
     s->engine.reserve(1, 1);
